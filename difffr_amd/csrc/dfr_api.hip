@@ -1,0 +1,931 @@
+// Context, host-side step orchestration and the C ABI of libdfr_hip.so.
+//
+// The step follows SimulatorBase::timeStepNoGUI (Simulator/SimulatorBase.cpp:1142-1200)
+// around TimeStepDiffDFSPH::step (SPlisHSPlasH/DiffDFSPH/TimeStepDiffDFSPH.cpp:526-652):
+// begin step (release window), neighbour search, density + factor, divergence
+// solve, advection, pressure solve, position update, DiffFR's per-step chain
+// rule, rigid velocity/position update and the boundary transform.  The host
+// decides when a Jacobi solve has converged from the FP64 residual the
+// recompute kernel leaves in device memory.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dfr.h"
+#include "dfr_kernels.h"
+
+using namespace dfr;
+
+namespace {
+
+enum TimerClass {
+  TC_SORT = 0, TC_NBR, TC_DENSITY, TC_VUPDATE, TC_RECOMPUTE, TC_PWARM, TC_ELEMWISE, TC_GRADIENT, TC_RIGID, TC_COUNT
+};
+const char* kTimerNames[TC_COUNT] = {"grid_sort", "neighbor_build", "density_factor", "velocity_update",
+                                     "density_recompute", "pressure_warmstart", "advect_position",
+                                     "gradient_pairs", "rigid_step"};
+
+struct BodyInfo {
+  bool dynamic = false;
+  int64_t n = 0;
+  int64_t offset = 0;       // first sample in the concatenated boundary arrays
+  int64_t dyn_first = -1;   // first index in dyn_idx (dynamic bodies)
+  double x_rb[3] = {0, 0, 0}, q[4] = {1, 0, 0, 0}, density = 1000.0;
+  double v0[3] = {0, 0, 0}, w0[3] = {0, 0, 0};
+  std::vector<float> x0;    // local samples
+  bool animated = false;
+};
+
+}  // namespace
+
+struct dfr_ctx {
+  dfr_params P{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool initialized = false;
+
+  // derived constants
+  float h = 0, dtf = 0;
+  StepConst sc{};
+  GridDesc grid{};
+
+  // host staging
+  std::vector<float> fx, fv;     // initial fluid (AoS)
+  std::vector<BodyInfo> bodies;
+  std::vector<int> body_order;   // bodies in boundary-array order (static first)
+
+  FluidBufs f;
+  BoundaryBufs b;
+  GridBufs gb;
+  NbrBufs nb;
+  RigidDev* d_rig = nullptr;
+  std::vector<RigidDev> h_rig_init;  // state restored by dfr_reset
+  double* d_err = nullptr;
+  double* h_err = nullptr;           // pinned
+  uint32_t* h_overflow = nullptr;    // pinned
+  int err_slots = 0;
+
+  float time = 0.f;
+  int64_t step_count = 0;
+  dfr_stats stats{};
+
+  bool profiling = false;
+  struct Rec { int cls; hipEvent_t a, b; };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> event_pool;
+  double t_sum[TC_COUNT] = {0};
+  int64_t t_cnt[TC_COUNT] = {0};
+
+  std::vector<void*> allocations;
+};
+
+// ---------------------------------------------------------------------------
+namespace {
+
+struct HipError {
+  hipError_t e;
+  std::string where;
+};
+
+#define HIPCHK(expr)                                                       \
+  do {                                                                     \
+    hipError_t _e = (expr);                                                \
+    if (_e != hipSuccess) throw HipError{_e, std::string(#expr)};          \
+  } while (0)
+
+template <typename T> T* dalloc(dfr_ctx* c, size_t n) {
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  c->allocations.push_back(p);
+  return static_cast<T*>(p);
+}
+
+int fail(dfr_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+template <typename F> int guarded(dfr_ctx* c, F&& fn) {
+  if (!c) return DFR_ERR_ARG;
+  try {
+    return fn();
+  } catch (const HipError& e) {
+    return fail(c, DFR_ERR_HIP, std::string("HIP error ") + hipGetErrorString(e.e) + " in " + e.where);
+  } catch (const std::exception& e) {
+    return fail(c, DFR_ERR_ARG, e.what());
+  }
+}
+
+hipEvent_t take_event(dfr_ctx* c) {
+  if (!c->event_pool.empty()) { hipEvent_t e = c->event_pool.back(); c->event_pool.pop_back(); return e; }
+  hipEvent_t e;
+  HIPCHK(hipEventCreate(&e));
+  return e;
+}
+
+// Times one launch class with events on the context stream (profiling only).
+struct Timed {
+  dfr_ctx* c; int cls; hipEvent_t a = nullptr;
+  Timed(dfr_ctx* c_, int cls_) : c(c_), cls(cls_) {
+    if (c->profiling) { a = take_event(c); HIPCHK(hipEventRecord(a, c->stream)); }
+  }
+  ~Timed() {
+    if (c->profiling && a) {
+      hipEvent_t e = nullptr;
+      if (!c->event_pool.empty()) { e = c->event_pool.back(); c->event_pool.pop_back(); }
+      else if (hipEventCreate(&e) != hipSuccess) return;
+      if (hipEventRecord(e, c->stream) == hipSuccess) c->pending.push_back({cls, a, e});
+    }
+  }
+};
+
+void drain_timers(dfr_ctx* c) {
+  for (auto& r : c->pending) {
+    float ms = 0.f;
+    HIPCHK(hipEventSynchronize(r.b));
+    HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+    c->t_sum[r.cls] += ms;
+    c->t_cnt[r.cls] += 1;
+    c->event_pool.push_back(r.a);
+    c->event_pool.push_back(r.b);
+  }
+  c->pending.clear();
+}
+
+void sync_check(dfr_ctx* c) {
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+// Grid covering the initial scene with a margin; particles outside are clamped
+// into border cells, which keeps the search exact (the predicate decides) and
+// only costs time.
+GridDesc make_grid(const float lo_in[3], const float hi_in[3], float h) {
+  GridDesc g{};
+  const float cs = h * 1.001f;
+  const float margin = 4.f * cs;
+  for (int k = 0; k < 3; ++k) {
+    g.lo[k] = lo_in[k] - margin;
+    int n = (int)std::ceil((hi_in[k] + margin - g.lo[k]) / cs) + 1;
+    n = (n + 3) & ~3;
+    g.n[k] = std::max(n, 4);
+    g.bn[k] = g.n[k] / 4;
+  }
+  g.inv_cs = 1.f / cs;
+  g.nkeys = (uint32_t)((uint64_t)g.bn[0] * g.bn[1] * g.bn[2] * 64ull);
+  return g;
+}
+
+void fill_rigid_from_body(const dfr_ctx* c, const BodyInfo& bi, RigidDev& r) {
+  std::memset(&r, 0, sizeof(r));
+  for (int k = 0; k < 3; ++k) { r.x[k] = bi.x_rb[k]; r.v0[k] = bi.v0[k]; r.w0[k] = bi.w0[k]; }
+  for (int k = 0; k < 4; ++k) r.q[k] = bi.q[k];
+  r.dynamic = bi.dynamic ? 1 : 0;
+  // Dynamic3dRigidBody::determineMassProperties (Dynamic3dRigidBody.h:184-213)
+  const double rad = c->P.radius;
+  const double dm = 4.0 / 3.0 * M_PI * rad * rad * rad * bi.density;
+  double I0[9] = {0};
+  for (int64_t j = 0; j < bi.n; ++j) {
+    const double p[3] = {bi.x0[3 * j], bi.x0[3 * j + 1], bi.x0[3 * j + 2]};
+    const double pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+    for (int a = 0; a < 3; ++a)
+      for (int bb = 0; bb < 3; ++bb) I0[3 * a + bb] += dm * ((a == bb ? pp : 0.0) - p[a] * p[bb]);
+  }
+  r.mass = dm * (double)bi.n;
+  r.invMass = 1.0 / r.mass;
+  for (int k = 0; k < 9; ++k) r.I0[k] = I0[k];
+  // updateInertia (Dynamic3dRigidBody.h:215-219): I = R I0 R^T, Iinv = I^-1
+  const double w = r.q[0], x = r.q[1], y = r.q[2], z = r.q[3];
+  const double tx = 2 * x, ty = 2 * y, tz = 2 * z, twx = tx * w, twy = ty * w, twz = tz * w;
+  const double txx = tx * x, txy = ty * x, txz = tz * x, tyy = ty * y, tyz = tz * y, tzz = tz * z;
+  const double R[9] = {1 - (tyy + tzz), txy - twz, txz + twy, txy + twz, 1 - (txx + tzz), tyz - twx,
+                       txz - twy, tyz + twx, 1 - (txx + tyy)};
+  double RI[9] = {0}, I[9] = {0};
+  for (int a = 0; a < 3; ++a) for (int bb = 0; bb < 3; ++bb) for (int k = 0; k < 3; ++k) RI[3 * a + bb] += R[3 * a + k] * I0[3 * k + bb];
+  for (int a = 0; a < 3; ++a) for (int bb = 0; bb < 3; ++bb) for (int k = 0; k < 3; ++k) I[3 * a + bb] += RI[3 * a + k] * R[3 * bb + k];
+  for (int k = 0; k < 9; ++k) r.I[k] = I[k];
+  auto cof = [&](int i, int j) {
+    const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+    return I[3 * i1 + j1] * I[3 * i2 + j2] - I[3 * i1 + j2] * I[3 * i2 + j1];
+  };
+  const double det = cof(0, 0) * I[0] + cof(1, 0) * I[3] + cof(2, 0) * I[6];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r.Iinv[3 * i + j] = det != 0.0 ? cof(j, i) / det : 0.0;
+  // Jvv = Jww = I, all else 0 (BoundaryModel_Akinci2012.cpp:94-106)
+  for (int k = 0; k < 3; ++k) { r.J[18 + 4 * k] = 1.0; r.J[45 + 4 * k] = 1.0; }
+}
+
+void upload_fluid_initial(dfr_ctx* c) {
+  const int64_t n = c->f.n;
+  std::vector<float4> p(n), v(n);
+  std::vector<uint32_t> id(n);
+  for (int64_t i = 0; i < n; ++i) {
+    p[i] = make_float4(c->fx[3 * i], c->fx[3 * i + 1], c->fx[3 * i + 2], 0.f);
+    v[i] = make_float4(c->fv[3 * i], c->fv[3 * i + 1], c->fv[3 * i + 2], 0.f);
+    id[i] = (uint32_t)i;
+  }
+  HIPCHK(hipMemcpyAsync(c->f.pos, p.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->f.vel, v.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->f.id, id.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->f.kappa, 0, n * sizeof(float), c->stream));
+  HIPCHK(hipMemsetAsync(c->f.kappav, 0, n * sizeof(float), c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+void place_boundary_and_volumes(dfr_ctx* c) {
+  const int nbodies = (int)c->bodies.size();
+  HIPCHK(hipMemcpyAsync(c->d_rig, c->h_rig_init.data(), sizeof(RigidDev) * nbodies, hipMemcpyHostToDevice, c->stream));
+  launch_boundary_transform(nullptr, c->b.n, c->b, c->d_rig, c->stream);
+  launch_sort_boundary(c->b, c->grid, c->gb, c->stream);
+  std::vector<int> groups(nbodies);
+  for (int k = 0; k < nbodies; ++k) groups[k] = c->bodies[k].dynamic ? k + 1 : 0;
+  float4* tmp = c->f.pos2;  // large enough scratch only if nf >= nb; use a dedicated buffer otherwise
+  float4* own = nullptr;
+  if (c->f.n < c->b.n) {
+    HIPCHK(hipMalloc(&own, sizeof(float4) * std::max<int64_t>(c->b.n, 1)));
+    tmp = own;
+  }
+  launch_boundary_volume_impl(c->b.pos, c->b.x0, c->b.n, groups.data(), nbodies, c->sc.K, c->h * c->h, c->grid,
+                              c->gb, tmp, c->stream);
+  HIPCHK(hipMemcpyAsync(c->b.pos, tmp, sizeof(float4) * c->b.n, hipMemcpyDeviceToDevice, c->stream));
+  sync_check(c);
+  if (own) HIPCHK(hipFree(own));
+}
+
+void begin_step(dfr_ctx* c) {
+  // TimeStepDiffDFSPH::beginStep (:379-429), passive-body branch
+  const int nb = (int)c->bodies.size();
+  std::vector<int> ramp(nb, 0), anim(nb, 0);
+  std::vector<double> fac(nb, 0.0);
+  const float uat = (float)c->P.uniform_acc_time;
+  for (int k = 0; k < nb; ++k) {
+    BodyInfo& bi = c->bodies[k];
+    if (!bi.dynamic) continue;
+    if (c->time <= uat + c->dtf) {
+      float factor = 1.f;
+      if (uat > 1e-3f) {
+        factor = (c->time / uat) > 1.f ? 1.f : (c->time / uat);
+        bi.animated = true;
+      }
+      ramp[k] = 1;
+      fac[k] = (double)factor;
+    } else {
+      bi.animated = false;
+    }
+    anim[k] = bi.animated ? 1 : 0;
+  }
+  launch_begin_step(c->d_rig, nb, ramp.data(), fac.data(), anim.data(), c->stream);
+}
+
+void check_overflow(dfr_ctx* c) {
+  if (c->h_overflow[0] || c->h_overflow[1]) {
+    char buf[256];
+    std::snprintf(buf, sizeof(buf),
+                  "neighbour list capacity exceeded (fluid: %u > max_neighbors=%d, dynamic sample: %u > "
+                  "max_boundary_neighbors=%d)",
+                  c->h_overflow[0], c->P.max_neighbors, c->h_overflow[1], c->P.max_boundary_neighbors);
+    c->h_overflow[0] = c->h_overflow[1] = 0;
+    throw std::runtime_error(buf);
+  }
+}
+
+double read_err(dfr_ctx* c, int slot) {
+  HIPCHK(hipMemcpyAsync(c->h_err + slot, c->d_err + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  check_overflow(c);
+  return c->h_err[slot];
+}
+
+void run_gradient(dfr_ctx* c, int mode) {
+  if (!c->P.compute_gradient) return;
+  GradConst gc{};
+  gc.mode = mode;
+  const float invH = 1.f / c->dtf;
+  const float invH2 = 1.f / c->dtf / c->dtf;
+  // quirk of the reference: the coefficient is an unsigned int (:1266)
+  gc.coeff_trunc = (mode == 0) ? (float)(unsigned int)invH2 : (float)(unsigned int)invH;
+  gc.coeff = (mode == 0) ? invH2 : invH;
+  gc.optimize_rotation = c->P.optimize_rotation;
+  for (size_t k = 0; k < c->bodies.size(); ++k) {
+    const BodyInfo& bi = c->bodies[k];
+    if (!bi.dynamic || bi.animated || bi.n == 0) continue;
+    Timed t(c, TC_GRADIENT);
+    launch_gradient(c->sc, gc, c->f, c->b, c->nb, c->d_rig, (int)bi.dyn_first, (int)bi.n, (int)k, c->stream);
+  }
+}
+
+void do_step(dfr_ctx* c) {
+  const float dt = c->dtf;
+  const int nf = c->f.n;
+  begin_step(c);
+  {
+    Timed t(c, TC_SORT);
+    if (nf > 0) launch_sort_fluid(c->f, c->grid, c->gb, c->stream);
+    launch_sort_boundary(c->b, c->grid, c->gb, c->stream);
+  }
+  {
+    Timed t(c, TC_NBR);
+    HIPCHK(hipMemsetAsync(c->nb.overflow, 0, 2 * sizeof(uint32_t), c->stream));
+    if (nf > 0) launch_build_neighbors(c->f, c->b, c->grid, c->gb, c->nb, c->h * c->h, c->stream);
+    HIPCHK(hipMemcpyAsync(c->h_overflow, c->nb.overflow, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(double) * c->err_slots, c->stream));
+  const bool divOn = c->P.enable_divergence_solver != 0;
+  const bool warmV = c->P.use_divergence_warmstart != 0;
+  const bool warmP = c->P.use_pressure_warmstart != 0;
+  const int scratch_slot = c->err_slots - 1;
+  if (nf > 0) {
+    Timed t(c, TC_DENSITY);
+    launch_density_factor(c->sc, c->f, c->b, c->nb, (divOn && warmV) ? 1 : 0, divOn ? 1 : 0, c->stream);
+  }
+  // ---- divergence solve (TimeStepDiffDFSPH::divergenceSolve, :770-881)
+  int itv = 0;
+  double avgv = 0.0;
+  if (divOn && nf > 0) {
+    const float invH = 1.f / dt;
+    if (warmV) {
+      Timed t(c, TC_VUPDATE);
+      launch_velocity_update(c->sc, c->f, c->b, c->nb, c->d_rig, c->f.kappav, nullptr, c->stream);
+    }
+    {
+      Timed t(c, TC_RECOMPUTE);
+      launch_recompute(c->sc, c->f, c->b, c->nb, RECOMPUTE_DIV, invH, c->d_err + scratch_slot, c->stream);
+    }
+    const double eta = (1.0f / dt) * (float)c->P.max_error_v * 0.01f * c->sc.rho0;
+    bool chk = false;
+    while ((!chk || itv < 1) && itv < c->P.max_iterations_v) {
+      run_gradient(c, 1);
+      {
+        Timed t(c, TC_VUPDATE);
+        launch_velocity_update(c->sc, c->f, c->b, c->nb, c->d_rig, c->f.kf, warmV ? c->f.kappav : nullptr, c->stream);
+      }
+      {
+        Timed t(c, TC_RECOMPUTE);
+        launch_recompute(c->sc, c->f, c->b, c->nb, RECOMPUTE_DIV, invH, c->d_err + itv, c->stream);
+      }
+      avgv = read_err(c, itv) / (double)nf;
+      chk = avgv <= eta;
+      ++itv;
+    }
+  }
+  {
+    Timed t(c, TC_ELEMWISE);
+    if (nf > 0) launch_advect(c->sc, c->f, (divOn && warmV) ? dt : 1.f, c->stream);
+  }
+  // ---- pressure solve (TimeStepDiffDFSPH::pressureSolve, :654-768)
+  int it = 0;
+  double avg = 0.0;
+  if (nf > 0) {
+    const float invH2 = 1.f / (dt * dt);
+    if (warmP) {
+      {
+        Timed t(c, TC_PWARM);
+        launch_pressure_warm(c->sc, c->f, c->b, c->nb, c->stream);
+      }
+      Timed t(c, TC_VUPDATE);
+      launch_velocity_update(c->sc, c->f, c->b, c->nb, c->d_rig, c->f.kappa, nullptr, c->stream);
+    }
+    {
+      Timed t(c, TC_RECOMPUTE);
+      launch_recompute(c->sc, c->f, c->b, c->nb, RECOMPUTE_PRESSURE, invH2, c->d_err + scratch_slot, c->stream);
+    }
+    const double eta = (float)c->P.max_error * 0.01f * c->sc.rho0;
+    const int base = c->P.max_iterations_v;
+    bool chk = false;
+    while ((!chk || it < c->P.min_iterations) && it < c->P.max_iterations) {
+      run_gradient(c, 0);
+      {
+        Timed t(c, TC_VUPDATE);
+        launch_velocity_update(c->sc, c->f, c->b, c->nb, c->d_rig, c->f.kf, warmP ? c->f.kappa : nullptr, c->stream);
+      }
+      {
+        Timed t(c, TC_RECOMPUTE);
+        launch_recompute(c->sc, c->f, c->b, c->nb, RECOMPUTE_PRESSURE, invH2, c->d_err + base + it, c->stream);
+      }
+      avg = read_err(c, base + it) / (double)nf;
+      chk = avg <= eta;
+      ++it;
+    }
+    Timed t(c, TC_ELEMWISE);
+    launch_position(c->sc, c->f, warmP ? dt * dt : 1.f, c->stream);
+  }
+  c->time = c->time + dt;
+  {
+    // backwardPerStep (:493-524) + velocityTimeStep + positionTimeStep
+    Timed t(c, TC_RIGID);
+    RigidStepConst rc{};
+    rc.dt = (double)dt;
+    for (int k = 0; k < 3; ++k) rc.g[k] = (double)(float)c->P.gravity[k];
+    rc.gradient_mode = c->P.gradient_mode;
+    rc.optimize_rotation = c->P.optimize_rotation;
+    rc.gyroscopic = c->P.gyroscopic;
+    rc.compute_gradient = c->P.compute_gradient;
+    rc.nbodies = (int)c->bodies.size();
+    launch_rigid_step(rc, c->d_rig, c->stream);
+    launch_boundary_transform(c->b.dyn_idx, c->b.nd, c->b, c->d_rig, c->stream);
+  }
+  c->step_count++;
+  c->stats.iterations = it;
+  c->stats.iterations_v = itv;
+  c->stats.avg_density_err = avg;
+  c->stats.avg_div_err = avgv;
+  c->stats.total_iterations += it;
+  c->stats.total_iterations_v += itv;
+  c->stats.steps_since_reset += 1;
+  c->stats.time = c->time;
+  c->stats.step_count = c->step_count;
+  c->stats.trajectory_finished =
+      (c->time >= (float)c->P.target_time + (float)c->P.uniform_acc_time) ? 1 : 0;  // endStep (:448)
+}
+
+template <typename T>
+void download_unpermute(const dfr_ctx* c, const T* dsrc, int comps_in, int comps_out, float* out) {
+  const int64_t n = c->f.n;
+  std::vector<T> buf(n);
+  std::vector<uint32_t> id(n);
+  HIPCHK(hipMemcpy(buf.data(), dsrc, n * sizeof(T), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(id.data(), c->f.id, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  for (int64_t k = 0; k < n; ++k) {
+    const float* v = reinterpret_cast<const float*>(&buf[k]);
+    for (int q = 0; q < comps_out; ++q) out[(int64_t)id[k] * comps_out + q] = v[q];
+  }
+  (void)comps_in;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void dfr_default_params(dfr_params* p, double radius, double dt) {
+  std::memset(p, 0, sizeof(*p));
+  p->radius = radius;
+  p->dt = dt;
+  p->rho0 = 1000.0;
+  p->gravity[0] = 0.0; p->gravity[1] = -9.81; p->gravity[2] = 0.0;
+  p->max_error = 0.01;
+  p->max_error_v = 0.1;
+  p->min_iterations = 2;
+  p->max_iterations = 100;
+  p->max_iterations_v = 100;
+  p->use_pressure_warmstart = 1;
+  p->use_divergence_warmstart = 1;
+  p->enable_divergence_solver = 1;
+  p->gradient_mode = 1;
+  p->optimize_rotation = 1;
+  p->gyroscopic = 1;
+  p->compute_gradient = 1;
+  p->uniform_acc_time = 0.0;
+  p->target_time = 1e30;
+  p->max_neighbors = 96;
+  p->max_boundary_neighbors = 128;
+}
+
+dfr_ctx* dfr_create(const dfr_params* p, int device) {
+  if (!p || p->radius <= 0.0 || p->dt <= 0.0) return nullptr;
+  dfr_ctx* c = new dfr_ctx();
+  c->P = *p;
+  if (c->P.max_neighbors <= 0) c->P.max_neighbors = 96;
+  if (c->P.max_boundary_neighbors <= 0) c->P.max_boundary_neighbors = 128;
+  if (c->P.max_neighbors > 65535) c->P.max_neighbors = 65535;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return nullptr;
+  }
+  c->h = 4.f * (float)p->radius;  // Simulation.cpp:195-216
+  c->dtf = (float)p->dt;
+  const float pi = (float)M_PI;
+  const float h3 = c->h * c->h * c->h;
+  c->sc.K.h = c->h;
+  c->sc.K.k = 8.f / (pi * h3);
+  c->sc.K.l = 48.f / (pi * h3);
+  c->sc.K.w0 = c->sc.K.k;  // W(0) = k (6*0 - 6*0 + 1)
+  c->sc.dt = c->dtf;
+  c->sc.rho0 = (float)p->rho0;
+  const float diam = 2.f * (float)p->radius;
+  c->sc.V_f = 0.8f * diam * diam * diam;  // FluidModel.cpp:255-275
+  c->sc.m_f = c->sc.V_f * c->sc.rho0;
+  for (int k = 0; k < 3; ++k) c->sc.g[k] = (float)p->gravity[k];
+  c->sc.eps = 1e-5f;  // TimeStepDiffDFSPH.h:26
+  return c;
+}
+
+int dfr_destroy(dfr_ctx* c) {
+  if (!c) return DFR_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& r : c->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  for (void* p : c->allocations) (void)hipFree(p);
+  if (c->h_err) (void)hipHostFree(c->h_err);
+  if (c->h_overflow) (void)hipHostFree(c->h_overflow);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return DFR_OK;
+}
+
+const char* dfr_last_error(const dfr_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int dfr_set_fluid(dfr_ctx* c, int64_t n, const float* x, const float* v) {
+  return guarded(c, [&]() {
+    if (c->initialized) return fail(c, DFR_ERR_STATE, "dfr_set_fluid after dfr_init");
+    if (n < 0 || (n > 0 && !x)) return fail(c, DFR_ERR_ARG, "dfr_set_fluid: bad arguments");
+    if (n > (int64_t)0x7fffffff) return fail(c, DFR_ERR_ARG, "dfr_set_fluid: too many particles");
+    c->fx.assign(x, x + 3 * n);
+    c->fv.assign(3 * n, 0.f);
+    if (v) c->fv.assign(v, v + 3 * n);
+    c->f.n = (int)n;
+    return DFR_OK;
+  });
+}
+
+int dfr_add_boundary(dfr_ctx* c, int64_t n, const float* x0, int is_dynamic, const double x_rb[3],
+                     const double q[4], double density) {
+  return guarded(c, [&]() {
+    if (c->initialized) return fail(c, DFR_ERR_STATE, "dfr_add_boundary after dfr_init");
+    if (n < 0 || (n > 0 && !x0) || !x_rb || !q) return fail(c, DFR_ERR_ARG, "dfr_add_boundary: bad arguments");
+    if ((int)c->bodies.size() >= DFR_MAX_BODIES) return fail(c, DFR_ERR_ARG, "too many bodies");
+    BodyInfo bi;
+    bi.dynamic = is_dynamic != 0;
+    bi.n = n;
+    bi.x0.assign(x0, x0 + 3 * n);
+    for (int k = 0; k < 3; ++k) bi.x_rb[k] = x_rb[k];
+    for (int k = 0; k < 4; ++k) bi.q[k] = q[k];
+    bi.density = density;
+    c->bodies.push_back(bi);
+    return (int)c->bodies.size() - 1;
+  });
+}
+
+int dfr_set_init_velocity(dfr_ctx* c, int body, const double v0[3], const double w0[3]) {
+  return guarded(c, [&]() {
+    if (body < 0 || body >= (int)c->bodies.size() || !v0 || !w0) return fail(c, DFR_ERR_ARG, "bad body");
+    BodyInfo& bi = c->bodies[body];
+    for (int k = 0; k < 3; ++k) { bi.v0[k] = v0[k]; bi.w0[k] = w0[k]; }
+    if (c->initialized) {
+      HIPCHK(hipMemcpy(&c->d_rig[body].v0[0], v0, 3 * sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(&c->d_rig[body].w0[0], w0, 3 * sizeof(double), hipMemcpyHostToDevice));
+      for (int k = 0; k < 3; ++k) { c->h_rig_init[body].v0[k] = v0[k]; c->h_rig_init[body].w0[k] = w0[k]; }
+    }
+    return DFR_OK;
+  });
+}
+
+int dfr_init(dfr_ctx* c) {
+  return guarded(c, [&]() {
+    if (c->initialized) return fail(c, DFR_ERR_STATE, "dfr_init called twice");
+    HIPCHK(hipSetDevice(c->device));
+    const int nbodies = (int)c->bodies.size();
+    // boundary arrays: static bodies first, then dynamic bodies
+    int64_t off = 0, nd = 0;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int k = 0; k < nbodies; ++k) {
+        BodyInfo& bi = c->bodies[k];
+        if ((pass == 0) == bi.dynamic) continue;
+        bi.offset = off;
+        off += bi.n;
+        if (bi.dynamic) { bi.dyn_first = nd; nd += bi.n; }
+        c->body_order.push_back(k);
+      }
+    c->b.n = (int)off;
+    c->b.nd = (int)nd;
+    const int nf = c->f.n;
+    c->sc.nf = nf;
+    c->sc.nb = c->b.n;
+    c->sc.nbodies = nbodies;
+    // grid bounds from the initial scene
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int64_t i = 0; i < nf; ++i)
+      for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], c->fx[3 * i + k]); hi[k] = std::max(hi[k], c->fx[3 * i + k]); }
+    std::vector<float4> x0(std::max<int64_t>(off, 1));
+    for (int k = 0; k < nbodies; ++k) {
+      const BodyInfo& bi = c->bodies[k];
+      double R[9];
+      {
+        const double w = bi.q[0], x = bi.q[1], y = bi.q[2], z = bi.q[3];
+        const double tx = 2 * x, ty = 2 * y, tz = 2 * z, twx = tx * w, twy = ty * w, twz = tz * w;
+        const double txx = tx * x, txy = ty * x, txz = tz * x, tyy = ty * y, tyz = tz * y, tzz = tz * z;
+        const double Rv[9] = {1 - (tyy + tzz), txy - twz, txz + twy, txy + twz, 1 - (txx + tzz), tyz - twx,
+                              txz - twy, tyz + twx, 1 - (txx + tyy)};
+        std::memcpy(R, Rv, sizeof(R));
+      }
+      for (int64_t j = 0; j < bi.n; ++j) {
+        const float* p = &bi.x0[3 * j];
+        x0[bi.offset + j] = make_float4(p[0], p[1], p[2], (float)k);
+        for (int a = 0; a < 3; ++a) {
+          const double wpos = R[3 * a] * p[0] + R[3 * a + 1] * p[1] + R[3 * a + 2] * p[2] + bi.x_rb[a];
+          lo[a] = std::min(lo[a], (float)wpos);
+          hi[a] = std::max(hi[a], (float)wpos);
+        }
+      }
+    }
+    if (lo[0] > hi[0]) { for (int k = 0; k < 3; ++k) { lo[k] = 0.f; hi[k] = 1.f; } }
+    c->grid = make_grid(lo, hi, c->h);
+    // allocations
+    const size_t NF = (size_t)std::max(nf, 1);
+    c->f.pos = dalloc<float4>(c, NF); c->f.vel = dalloc<float4>(c, NF);
+    c->f.pos2 = dalloc<float4>(c, NF); c->f.vel2 = dalloc<float4>(c, NF);
+    c->f.kappa = dalloc<float>(c, NF); c->f.kappav = dalloc<float>(c, NF);
+    c->f.kappa2 = dalloc<float>(c, NF); c->f.kappav2 = dalloc<float>(c, NF);
+    c->f.id = dalloc<uint32_t>(c, NF); c->f.id2 = dalloc<uint32_t>(c, NF);
+    c->f.rho = dalloc<float>(c, NF); c->f.alpha = dalloc<float>(c, NF);
+    c->f.s = dalloc<float4>(c, NF); c->f.badv = dalloc<float>(c, NF); c->f.kf = dalloc<float>(c, NF);
+    const size_t NB = (size_t)std::max<int64_t>(off, 1);
+    c->b.pos = dalloc<float4>(c, NB); c->b.vel = dalloc<float4>(c, NB); c->b.x0 = dalloc<float4>(c, NB);
+    c->b.dyn_idx = dalloc<uint32_t>(c, std::max<int64_t>(nd, 1));
+    const uint32_t nk = c->grid.nkeys;
+    c->gb.fcount = dalloc<uint32_t>(c, nk); c->gb.fstart = dalloc<uint32_t>(c, nk + 1);
+    c->gb.bcount = dalloc<uint32_t>(c, nk); c->gb.bstart = dalloc<uint32_t>(c, nk + 1);
+    c->gb.key = dalloc<uint32_t>(c, NF); c->gb.slot = dalloc<uint32_t>(c, NF); c->gb.perm = dalloc<uint32_t>(c, NF);
+    c->gb.bkey = dalloc<uint32_t>(c, NB); c->gb.bslot = dalloc<uint32_t>(c, NB); c->gb.bperm = dalloc<uint32_t>(c, NB);
+    c->gb.tile_sums = dalloc<uint32_t>(c, scan_tile_count(nk) + 1);
+    c->nb.maxn = c->P.max_neighbors;
+    c->nb.maxbf = c->P.max_boundary_neighbors;
+    c->nb.nbr = dalloc<uint32_t>(c, (size_t)c->nb.maxn * NF);
+    c->nb.cnt = dalloc<uint32_t>(c, NF);
+    c->nb.bf = dalloc<uint32_t>(c, (size_t)c->nb.maxbf * std::max<int64_t>(nd, 1));
+    c->nb.bfcnt = dalloc<uint32_t>(c, std::max<int64_t>(nd, 1));
+    c->nb.overflow = dalloc<uint32_t>(c, 2);
+    c->sc.maxn = c->nb.maxn;
+    c->sc.maxbf = c->nb.maxbf;
+    c->d_rig = dalloc<RigidDev>(c, std::max(nbodies, 1));
+    c->err_slots = c->P.max_iterations_v + c->P.max_iterations + 2;
+    c->d_err = dalloc<double>(c, c->err_slots);
+    HIPCHK(hipHostMalloc((void**)&c->h_err, sizeof(double) * c->err_slots, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&c->h_overflow, sizeof(uint32_t) * 2, hipHostMallocDefault));
+    c->h_overflow[0] = c->h_overflow[1] = 0;
+    // uploads
+    HIPCHK(hipMemcpy(c->b.x0, x0.data(), sizeof(float4) * off, hipMemcpyHostToDevice));
+    std::vector<uint32_t> dyn(std::max<int64_t>(nd, 1));
+    for (int64_t k = 0; k < nd; ++k) dyn[k] = (uint32_t)(off - nd + k);
+    HIPCHK(hipMemcpy(c->b.dyn_idx, dyn.data(), sizeof(uint32_t) * dyn.size(), hipMemcpyHostToDevice));
+    c->h_rig_init.assign(std::max(nbodies, 1), RigidDev{});
+    for (int k = 0; k < nbodies; ++k) fill_rigid_from_body(c, c->bodies[k], c->h_rig_init[k]);
+    if (nf > 0) upload_fluid_initial(c);
+    place_boundary_and_volumes(c);
+    c->initialized = true;
+    c->time = 0.f;
+    c->step_count = 0;
+    return DFR_OK;
+  });
+}
+
+int dfr_step(dfr_ctx* c, int nsteps) {
+  return guarded(c, [&]() {
+    if (!c->initialized) return fail(c, DFR_ERR_STATE, "dfr_step before dfr_init");
+    HIPCHK(hipSetDevice(c->device));
+    for (int s = 0; s < nsteps; ++s) do_step(c);
+    sync_check(c);
+    check_overflow(c);
+    if (c->profiling) drain_timers(c);
+    return DFR_OK;
+  });
+}
+
+int dfr_reset(dfr_ctx* c) {
+  return guarded(c, [&]() {
+    if (!c->initialized) return fail(c, DFR_ERR_STATE, "dfr_reset before dfr_init");
+    HIPCHK(hipSetDevice(c->device));
+    for (auto& bi : c->bodies) bi.animated = false;
+    if (c->f.n > 0) upload_fluid_initial(c);
+    place_boundary_and_volumes(c);
+    c->time = 0.f;
+    c->step_count = 0;
+    return DFR_OK;
+  });
+}
+
+static int check_body(const dfr_ctx* c, int body) {
+  return (c && c->initialized && body >= 0 && body < (int)c->bodies.size()) ? 0 : 1;
+}
+
+int dfr_get_rigid(const dfr_ctx* c, int body, double out[19]) {
+  if (check_body(c, body) || !out) return DFR_ERR_ARG;
+  RigidDev r;
+  if (hipMemcpy(&r, &c->d_rig[body], sizeof(RigidDev), hipMemcpyDeviceToHost) != hipSuccess) return DFR_ERR_HIP;
+  for (int k = 0; k < 3; ++k) { out[k] = r.x[k]; out[7 + k] = r.v[k]; out[10 + k] = r.w[k]; out[13 + k] = r.Flast[k]; out[16 + k] = r.Tlast[k]; }
+  for (int k = 0; k < 4; ++k) out[3 + k] = r.q[k];
+  return DFR_OK;
+}
+
+int dfr_set_rigid(dfr_ctx* c, int body, const double in[13]) {
+  if (check_body(c, body) || !in) return DFR_ERR_ARG;
+  return guarded(c, [&]() {
+    RigidDev r;
+    HIPCHK(hipMemcpy(&r, &c->d_rig[body], sizeof(RigidDev), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; ++k) { r.x[k] = in[k]; r.v[k] = in[7 + k]; r.w[k] = in[10 + k]; }
+    for (int k = 0; k < 4; ++k) r.q[k] = in[3 + k];
+    // keep the world inertia consistent with the new rotation
+    BodyInfo tmp = c->bodies[body];
+    for (int k = 0; k < 3; ++k) tmp.x_rb[k] = r.x[k];
+    for (int k = 0; k < 4; ++k) tmp.q[k] = r.q[k];
+    RigidDev fresh;
+    fill_rigid_from_body(c, tmp, fresh);
+    std::memcpy(r.I, fresh.I, sizeof(r.I));
+    std::memcpy(r.Iinv, fresh.Iinv, sizeof(r.Iinv));
+    HIPCHK(hipMemcpy(&c->d_rig[body], &r, sizeof(RigidDev), hipMemcpyHostToDevice));
+    launch_boundary_transform(nullptr, c->b.n, c->b, c->d_rig, c->stream);
+    sync_check(c);
+    return DFR_OK;
+  });
+}
+
+int dfr_get_jacobians(const dfr_ctx* c, int body, double out[78]) {
+  if (check_body(c, body) || !out) return DFR_ERR_ARG;
+  return hipMemcpy(out, &c->d_rig[body].J[0], 78 * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess ? DFR_OK
+                                                                                                       : DFR_ERR_HIP;
+}
+
+int dfr_get_net_gradients(const dfr_ctx* c, int body, double out[78]) {
+  if (check_body(c, body) || !out) return DFR_ERR_ARG;
+  return hipMemcpy(out, &c->d_rig[body].netlast[0], 78 * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess
+             ? DFR_OK
+             : DFR_ERR_HIP;
+}
+
+int dfr_reset_gradient(dfr_ctx* c) {
+  return guarded(c, [&]() {
+    if (!c->initialized) return fail(c, DFR_ERR_STATE, "not initialised");
+    for (int k = 0; k < (int)c->bodies.size(); ++k) {
+      double J[78] = {0};
+      for (int d = 0; d < 3; ++d) { J[18 + 4 * d] = 1.0; J[45 + 4 * d] = 1.0; }
+      HIPCHK(hipMemcpy(&c->d_rig[k].J[0], J, sizeof(J), hipMemcpyHostToDevice));
+    }
+    return DFR_OK;
+  });
+}
+
+int dfr_get_fluid(const dfr_ctx* c, float* x, float* v) {
+  if (!c || !c->initialized) return DFR_ERR_STATE;
+  try {
+    if (x) download_unpermute<float4>(c, c->f.pos, 4, 3, x);
+    if (v) download_unpermute<float4>(c, c->f.vel, 4, 3, v);
+  } catch (const HipError&) {
+    return DFR_ERR_HIP;
+  }
+  return DFR_OK;
+}
+
+int dfr_get_field(const dfr_ctx* c, const char* name, float* out) {
+  if (!c || !c->initialized || !name || !out) return DFR_ERR_ARG;
+  const std::string n(name);
+  try {
+    if (n == "density") download_unpermute<float>(c, c->f.rho, 1, 1, out);
+    else if (n == "factor") download_unpermute<float>(c, c->f.alpha, 1, 1, out);
+    else if (n == "density_adv") download_unpermute<float>(c, c->f.badv, 1, 1, out);
+    else if (n == "kappa") download_unpermute<float>(c, c->f.kappa, 1, 1, out);
+    else if (n == "kappa_v") download_unpermute<float>(c, c->f.kappav, 1, 1, out);
+    else if (n == "sum_grad_p_k") download_unpermute<float4>(c, c->f.s, 4, 3, out);
+    else return DFR_ERR_ARG;
+  } catch (const HipError&) {
+    return DFR_ERR_HIP;
+  }
+  return DFR_OK;
+}
+
+int dfr_set_field(dfr_ctx* c, const char* name, const float* in) {
+  if (!c || !c->initialized || !name || !in) return DFR_ERR_ARG;
+  return guarded(c, [&]() {
+    const std::string n(name);
+    float* dst = nullptr;
+    if (n == "kappa") dst = c->f.kappa;
+    else if (n == "kappa_v") dst = c->f.kappav;
+    else return fail(c, DFR_ERR_ARG, "unknown field");
+    const int64_t nf = c->f.n;
+    std::vector<uint32_t> id(nf);
+    HIPCHK(hipMemcpy(id.data(), c->f.id, nf * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<float> buf(nf);
+    for (int64_t k = 0; k < nf; ++k) buf[k] = in[id[k]];
+    HIPCHK(hipMemcpy(dst, buf.data(), nf * sizeof(float), hipMemcpyHostToDevice));
+    return DFR_OK;
+  });
+}
+
+int dfr_get_boundary_field(const dfr_ctx* c, int body, const char* name, float* out) {
+  if (check_body(c, body) || !name || !out) return DFR_ERR_ARG;
+  const BodyInfo& bi = c->bodies[body];
+  std::vector<float4> buf(std::max<int64_t>(bi.n, 1));
+  const std::string n(name);
+  const float4* src = (n == "v") ? c->b.vel : c->b.pos;
+  if (bi.n > 0 &&
+      hipMemcpy(buf.data(), src + bi.offset, bi.n * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess)
+    return DFR_ERR_HIP;
+  if (n == "volume") {
+    for (int64_t j = 0; j < bi.n; ++j) out[j] = buf[j].w;
+  } else if (n == "x" || n == "v") {
+    for (int64_t j = 0; j < bi.n; ++j) { out[3 * j] = buf[j].x; out[3 * j + 1] = buf[j].y; out[3 * j + 2] = buf[j].z; }
+  } else {
+    return DFR_ERR_ARG;
+  }
+  return DFR_OK;
+}
+
+int dfr_get_stats(const dfr_ctx* c, dfr_stats* out) {
+  if (!c || !out) return DFR_ERR_ARG;
+  *out = c->stats;
+  return DFR_OK;
+}
+
+int dfr_reset_stats(dfr_ctx* c) {
+  if (!c) return DFR_ERR_ARG;
+  c->stats.total_iterations = 0;
+  c->stats.total_iterations_v = 0;
+  c->stats.steps_since_reset = 0;
+  for (int k = 0; k < TC_COUNT; ++k) { c->t_sum[k] = 0; c->t_cnt[k] = 0; }
+  return DFR_OK;
+}
+
+int dfr_neighbor_stats(const dfr_ctx* c, double out[3]) {
+  if (!c || !c->initialized || !out) return DFR_ERR_ARG;
+  const int64_t nf = c->f.n, nd = c->b.nd;
+  std::vector<uint32_t> cnt(std::max<int64_t>(nf, 1)), bfc(std::max<int64_t>(nd, 1));
+  if (nf && hipMemcpy(cnt.data(), c->nb.cnt, nf * 4, hipMemcpyDeviceToHost) != hipSuccess) return DFR_ERR_HIP;
+  if (nd && hipMemcpy(bfc.data(), c->nb.bfcnt, nd * 4, hipMemcpyDeviceToHost) != hipSuccess) return DFR_ERR_HIP;
+  double sf = 0, sb = 0, sbf = 0;
+  for (int64_t i = 0; i < nf; ++i) { sf += cnt[i] & 0xffff; sb += cnt[i] >> 16; }
+  for (int64_t k = 0; k < nd; ++k) sbf += bfc[k];
+  out[0] = nf ? sf / nf : 0.0;
+  out[1] = nf ? sb / nf : 0.0;
+  out[2] = nd ? sbf / nd : 0.0;
+  return DFR_OK;
+}
+
+int64_t dfr_get_neighbors(const dfr_ctx* c, int kind, uint32_t* counts, uint32_t* lists, int64_t cap) {
+  if (!c || !c->initialized || !counts || !lists) return DFR_ERR_ARG;
+  const int64_t nf = c->f.n, nd = c->b.nd;
+  std::vector<uint32_t> id(std::max<int64_t>(nf, 1));
+  if (nf && hipMemcpy(id.data(), c->f.id, nf * 4, hipMemcpyDeviceToHost) != hipSuccess) return DFR_ERR_HIP;
+  if (kind == 0 || kind == 1) {
+    std::vector<uint32_t> cnt(std::max<int64_t>(nf, 1));
+    std::vector<uint32_t> nbr((size_t)c->nb.maxn * std::max<int64_t>(nf, 1));
+    if (nf && hipMemcpy(cnt.data(), c->nb.cnt, nf * 4, hipMemcpyDeviceToHost) != hipSuccess) return DFR_ERR_HIP;
+    if (nf && hipMemcpy(nbr.data(), c->nb.nbr, (size_t)c->nb.maxn * nf * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      return DFR_ERR_HIP;
+    // rows in particle-id order
+    std::vector<int64_t> row_of(nf);
+    for (int64_t k = 0; k < nf; ++k) row_of[id[k]] = k;
+    int64_t w = 0;
+    for (int64_t r = 0; r < nf; ++r) {
+      const int64_t k = row_of[r];
+      const uint32_t nF = cnt[k] & 0xffff, nB = cnt[k] >> 16;
+      const uint32_t s0 = (kind == 0) ? 0 : nF, s1 = (kind == 0) ? nF : nF + nB;
+      counts[r] = s1 - s0;
+      for (uint32_t s = s0; s < s1; ++s) {
+        if (w >= cap) return DFR_ERR_CAPACITY;
+        const uint32_t j = nbr[(size_t)s * nf + k];
+        lists[w++] = (kind == 0) ? id[j] : j;
+      }
+    }
+    return nf;
+  }
+  if (kind == 2) {
+    std::vector<uint32_t> cnt(std::max<int64_t>(nd, 1));
+    std::vector<uint32_t> bf((size_t)c->nb.maxbf * std::max<int64_t>(nd, 1));
+    if (nd && hipMemcpy(cnt.data(), c->nb.bfcnt, nd * 4, hipMemcpyDeviceToHost) != hipSuccess) return DFR_ERR_HIP;
+    if (nd && hipMemcpy(bf.data(), c->nb.bf, (size_t)c->nb.maxbf * nd * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      return DFR_ERR_HIP;
+    int64_t w = 0;
+    for (int64_t k = 0; k < nd; ++k) {
+      counts[k] = cnt[k];
+      for (uint32_t s = 0; s < cnt[k]; ++s) {
+        if (w >= cap) return DFR_ERR_CAPACITY;
+        lists[w++] = id[bf[(size_t)s * nd + k]];
+      }
+    }
+    return nd;
+  }
+  return DFR_ERR_ARG;
+}
+
+int dfr_set_profiling(dfr_ctx* c, int on) {
+  if (!c) return DFR_ERR_ARG;
+  c->profiling = on != 0;
+  return DFR_OK;
+}
+
+int dfr_kernel_times(const dfr_ctx* c, int n, char names[][32], double* avg_ms, int64_t* launches) {
+  if (!c) return DFR_ERR_ARG;
+  for (int k = 0; k < TC_COUNT && k < n; ++k) {
+    if (names) { std::strncpy(names[k], kTimerNames[k], 31); names[k][31] = 0; }
+    if (avg_ms) avg_ms[k] = c->t_cnt[k] ? c->t_sum[k] / (double)c->t_cnt[k] : 0.0;
+    if (launches) launches[k] = c->t_cnt[k];
+  }
+  return TC_COUNT;
+}
+
+int64_t dfr_num_fluid(const dfr_ctx* c) { return c ? c->f.n : DFR_ERR_ARG; }
+int64_t dfr_num_boundary(const dfr_ctx* c, int body) {
+  if (!c || body < 0 || body >= (int)c->bodies.size()) return DFR_ERR_ARG;
+  return c->bodies[body].n;
+}
+int dfr_num_bodies(const dfr_ctx* c) { return c ? (int)c->bodies.size() : DFR_ERR_ARG; }
+
+}  // extern "C"

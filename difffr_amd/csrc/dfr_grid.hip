@@ -1,0 +1,316 @@
+// Counting-sort cell grid and compact (transposed ELL) neighbour lists.
+//
+// Replaces CompactNSearch::find_neighbors / z_sort as used by
+// Simulation::performNeighborhoodSearch(Sort) (SPlisHSPlasH/Simulation.cpp:743-777)
+// and TimeStepDiffDFSPH::performNeighborhoodSearch (DiffDFSPH/TimeStepDiffDFSPH.cpp:2044-2056).
+//
+// Every step the fluid is counting-sorted by cell key (brick-Morton order) and
+// physically permuted, so that the gather passes read neighbours from nearby
+// addresses.  Inside a cell particles are ordered by particle id, which makes
+// the order (and hence every FP32 sum) independent of atomic scheduling.
+// Boundary samples are sorted as an index list only (their data never moves).
+#include "dfr_kernels.h"
+
+namespace dfr {
+
+// Logical block index remapped so that each of the 8 XCDs of an MI355X (which
+// receive workgroups round-robin) processes one contiguous range of particles:
+// neighbouring particles then share an L2.
+__device__ inline uint32_t xcd_block() {
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t per = nb / 8u, rem = nb % 8u;
+  const uint32_t x = b % 8u, k = b / 8u;
+  return x * per + min(x, rem) + k;
+}
+__device__ inline uint32_t gid() { return xcd_block() * blockDim.x + threadIdx.x; }
+
+__global__ void k_keys(const float4* __restrict__ pos, int n, GridDesc g, uint32_t* __restrict__ count,
+                       uint32_t* __restrict__ key, uint32_t* __restrict__ slot) {
+  const uint32_t i = gid();
+  if (i >= (uint32_t)n) return;
+  const float4 p = pos[i];
+  int cx, cy, cz;
+  cell_coords(g, p.x, p.y, p.z, cx, cy, cz);
+  const uint32_t k = cell_key(g, cx, cy, cz);
+  key[i] = k;
+  slot[i] = atomicAdd(&count[k], 1u);
+}
+
+__global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_t* __restrict__ slot,
+                          const uint32_t* __restrict__ start, uint32_t* __restrict__ perm) {
+  const uint32_t i = gid();
+  if (i >= (uint32_t)n) return;
+  perm[start[key[i]] + slot[i]] = i;
+}
+
+// Insertion sort of each cell's entries by an order key (particle id for the
+// fluid, sample index for the boundary).  Cells hold a handful of particles.
+__global__ void k_cell_sort(uint32_t nkeys, const uint32_t* __restrict__ start, uint32_t* __restrict__ perm,
+                            const uint32_t* __restrict__ order) {
+  const uint32_t c = gid();
+  if (c >= nkeys) return;
+  const uint32_t s = start[c], e = start[c + 1];
+  if (e - s < 2) return;
+  for (uint32_t a = s + 1; a < e; ++a) {
+    const uint32_t v = perm[a];
+    const uint32_t kv = order ? order[v] : v;
+    uint32_t b = a;
+    while (b > s) {
+      const uint32_t u = perm[b - 1];
+      const uint32_t ku = order ? order[u] : u;
+      if (ku <= kv) break;
+      perm[b] = u;
+      --b;
+    }
+    perm[b] = v;
+  }
+}
+
+// Permute the per-particle fluid state into sorted order.
+__global__ void k_gather_fluid(int n, const uint32_t* __restrict__ perm, const float4* __restrict__ pos,
+                               const float4* __restrict__ vel, const float* __restrict__ kappa,
+                               const float* __restrict__ kappav, const uint32_t* __restrict__ id,
+                               float4* __restrict__ pos2, float4* __restrict__ vel2, float* __restrict__ kappa2,
+                               float* __restrict__ kappav2, uint32_t* __restrict__ id2) {
+  const uint32_t k = gid();
+  if (k >= (uint32_t)n) return;
+  const uint32_t i = perm[k];
+  pos2[k] = pos[i];
+  vel2[k] = vel[i];
+  kappa2[k] = kappa[i];
+  kappav2[k] = kappav[i];
+  id2[k] = id[i];
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of the per-cell counts (three passes, 2048 keys per block).
+// ---------------------------------------------------------------------------
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = DFR_BLOCK * SCAN_ITEMS;
+
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds, uint32_t& total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += t;
+  }
+  if (lane == 63) lds[wid] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int w = 0; w < DFR_BLOCK / 64; ++w) { const uint32_t t = lds[w]; lds[w] = acc; acc += t; }
+    lds[DFR_BLOCK / 64] = acc;
+  }
+  __syncthreads();
+  const uint32_t res = inc - v + lds[wid];
+  total = lds[DFR_BLOCK / 64];
+  __syncthreads();
+  return res;
+}
+
+__global__ void k_scan_tiles(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ out,
+                             uint32_t* __restrict__ tile_sums) {
+  __shared__ uint32_t lds[DFR_BLOCK / 64 + 1];
+  const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0u;
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(sum, lds, total);
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ void k_scan_sums(uint32_t* __restrict__ sums, uint32_t m, uint32_t* __restrict__ total_out) {
+  __shared__ uint32_t lds[DFR_BLOCK / 64 + 1];
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < m; base += DFR_BLOCK) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = (i < m) ? sums[i] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(v, lds, total);
+    if (i < m) sums[i] = ex + carry;
+    carry += total;
+  }
+  if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ void k_scan_add(uint32_t* __restrict__ out, uint32_t n, const uint32_t* __restrict__ tile_sums) {
+  const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  const uint32_t add = tile_sums[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k)
+    if (base + k < n) out[base + k] += add;
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour lists.  Fluid particle i owns slots [0, nF) (fluid indices in the
+// sorted fluid order) and [nF, nF + nB) (boundary sample indices) at
+// nbr[slot * nf + i]; cnt[i] = nF | nB << 16.  Cells are visited in
+// (dz, dy, dx) order and each cell in its sorted order.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(DFR_BLOCK)
+k_build_fluid_nbrs(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t* __restrict__ fstart,
+                   const float4* __restrict__ bpos, const uint32_t* __restrict__ bstart,
+                   const uint32_t* __restrict__ bperm, float h2, int maxn, uint32_t* __restrict__ nbr,
+                   uint32_t* __restrict__ cnt, uint32_t* __restrict__ overflow) {
+  const uint32_t i = gid();
+  if (i >= (uint32_t)nf) return;
+  const float4 p = pos[i];
+  int cx, cy, cz;
+  cell_coords(g, p.x, p.y, p.z, cx, cy, cz);
+  uint32_t nF = 0, nB = 0;
+  const size_t stride = (size_t)nf;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = cz + dz;
+    if (z < 0 || z >= g.n[2]) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = cy + dy;
+      if (y < 0 || y >= g.n[1]) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int x = cx + dx;
+        if (x < 0 || x >= g.n[0]) continue;
+        const uint32_t c = cell_key(g, x, y, z);
+        const uint32_t s = fstart[c], e = fstart[c + 1];
+        for (uint32_t j = s; j < e; ++j) {
+          if (j == i) continue;
+          const float4 q = pos[j];
+          if (dist2_exact(p.x, p.y, p.z, q.x, q.y, q.z) < h2) {
+            if (nF < (uint32_t)maxn) nbr[(size_t)nF * stride + i] = j;
+            ++nF;
+          }
+        }
+      }
+    }
+  }
+  const uint32_t nFk = min(nF, (uint32_t)maxn);
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = cz + dz;
+    if (z < 0 || z >= g.n[2]) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = cy + dy;
+      if (y < 0 || y >= g.n[1]) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int x = cx + dx;
+        if (x < 0 || x >= g.n[0]) continue;
+        const uint32_t c = cell_key(g, x, y, z);
+        const uint32_t s = bstart[c], e = bstart[c + 1];
+        for (uint32_t k = s; k < e; ++k) {
+          const uint32_t j = bperm[k];
+          const float4 q = bpos[j];
+          if (dist2_exact(p.x, p.y, p.z, q.x, q.y, q.z) < h2) {
+            const uint32_t slot = nFk + nB;
+            if (slot < (uint32_t)maxn) nbr[(size_t)slot * stride + i] = j;
+            ++nB;
+          }
+        }
+      }
+    }
+  }
+  if (nF + nB > (uint32_t)maxn) {
+    atomicMax(overflow, nF + nB);
+    nB = (uint32_t)maxn - nFk;
+    nF = nFk;
+  }
+  cnt[i] = nF | (nB << 16);
+}
+
+// Fluid neighbours of the samples of dynamic bodies (the B->F lists the
+// gradient loop of computeRigidBodyGradient walks, TimeStepDiffDFSPH.cpp:1246-1273).
+__global__ void k_build_bf_nbrs(const float4* __restrict__ bpos, const uint32_t* __restrict__ dyn_idx, int nd,
+                                const float4* __restrict__ pos, GridDesc g, const uint32_t* __restrict__ fstart,
+                                float h2, int maxbf, uint32_t* __restrict__ bf, uint32_t* __restrict__ bfcnt,
+                                uint32_t* __restrict__ overflow) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (uint32_t)nd) return;
+  const float4 p = bpos[dyn_idx[k]];
+  int cx, cy, cz;
+  cell_coords(g, p.x, p.y, p.z, cx, cy, cz);
+  uint32_t n = 0;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = cz + dz;
+    if (z < 0 || z >= g.n[2]) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = cy + dy;
+      if (y < 0 || y >= g.n[1]) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int x = cx + dx;
+        if (x < 0 || x >= g.n[0]) continue;
+        const uint32_t c = cell_key(g, x, y, z);
+        for (uint32_t j = fstart[c]; j < fstart[c + 1]; ++j) {
+          const float4 q = pos[j];
+          if (dist2_exact(q.x, q.y, q.z, p.x, p.y, p.z) < h2) {
+            if (n < (uint32_t)maxbf) bf[(size_t)n * nd + k] = j;
+            ++n;
+          }
+        }
+      }
+    }
+  }
+  if (n > (uint32_t)maxbf) { atomicMax(overflow, n); n = maxbf; }
+  bfcnt[k] = n;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(size_t n, unsigned b = DFR_BLOCK) { return (unsigned)((n + b - 1) / b); }
+
+void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* tile_sums, hipStream_t s) {
+  // out has n + 1 entries; out[n] receives the total.
+  const unsigned tiles = (unsigned)((n + SCAN_TILE - 1) / SCAN_TILE);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(DFR_BLOCK), 0, s, in, n, out, tile_sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(DFR_BLOCK), 0, s, tile_sums, tiles, out + n);
+  hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(DFR_BLOCK), 0, s, out, n, tile_sums);
+}
+
+size_t scan_tile_count(uint32_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+void launch_sort_fluid(FluidBufs& f, const GridDesc& g, GridBufs& gb, hipStream_t s) {
+  const int n = f.n;
+  (void)hipMemsetAsync(gb.fcount, 0, sizeof(uint32_t) * g.nkeys, s);
+  hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(DFR_BLOCK), 0, s, f.pos, n, g, gb.fcount, gb.key, gb.slot);
+  launch_exclusive_scan(gb.fcount, g.nkeys, gb.fstart, gb.tile_sums, s);
+  hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(DFR_BLOCK), 0, s, n, gb.key, gb.slot, gb.fstart, gb.perm);
+  hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(g.nkeys)), dim3(DFR_BLOCK), 0, s, g.nkeys, gb.fstart, gb.perm,
+                     (const uint32_t*)f.id);
+  hipLaunchKernelGGL(k_gather_fluid, dim3(blocks_for(n)), dim3(DFR_BLOCK), 0, s, n, gb.perm, f.pos, f.vel, f.kappa,
+                     f.kappav, f.id, f.pos2, f.vel2, f.kappa2, f.kappav2, f.id2);
+  f.swap_sorted();
+}
+
+void launch_sort_boundary(BoundaryBufs& b, const GridDesc& g, GridBufs& gb, hipStream_t s) {
+  const int n = b.n;
+  (void)hipMemsetAsync(gb.bcount, 0, sizeof(uint32_t) * g.nkeys, s);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(DFR_BLOCK), 0, s, b.pos, n, g, gb.bcount, gb.bkey, gb.bslot);
+  }
+  launch_exclusive_scan(gb.bcount, g.nkeys, gb.bstart, gb.tile_sums, s);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(DFR_BLOCK), 0, s, n, gb.bkey, gb.bslot, gb.bstart, gb.bperm);
+    hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(g.nkeys)), dim3(DFR_BLOCK), 0, s, g.nkeys, gb.bstart, gb.bperm,
+                       (const uint32_t*)nullptr);
+  }
+}
+
+void launch_build_neighbors(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_build_fluid_nbrs, dim3(blocks_for(f.n)), dim3(DFR_BLOCK), 0, s, f.pos, f.n, g, gb.fstart,
+                     b.pos, gb.bstart, gb.bperm, h2, nb.maxn, nb.nbr, nb.cnt, nb.overflow);
+  if (b.nd > 0) {
+    hipLaunchKernelGGL(k_build_bf_nbrs, dim3(blocks_for(b.nd)), dim3(DFR_BLOCK), 0, s, b.pos, b.dyn_idx, b.nd, f.pos,
+                       g, gb.fstart, h2, nb.maxbf, nb.bf, nb.bfcnt, nb.overflow + 1);
+  }
+}
+
+}  // namespace dfr

@@ -1,0 +1,100 @@
+// Buffer bundles and kernel launchers of libdfr_hip.so (internal header).
+#pragma once
+#include <utility>
+#include "dfr_device.h"
+
+namespace dfr {
+
+// Fluid state in HBM, structure-of-arrays, permuted into cell order every step.
+struct FluidBufs {
+  int n = 0;
+  float4 *pos = nullptr, *vel = nullptr;       // xyz + unused w
+  float *kappa = nullptr, *kappav = nullptr;   // warm-start stiffness (persist across steps)
+  uint32_t* id = nullptr;                      // original particle id
+  float4 *pos2 = nullptr, *vel2 = nullptr;     // permutation targets
+  float *kappa2 = nullptr, *kappav2 = nullptr;
+  uint32_t* id2 = nullptr;
+  float* rho = nullptr;       // density
+  float* alpha = nullptr;     // DFSPH factor
+  float4* s = nullptr;        // sum_grad_p_k (xyz)
+  float* badv = nullptr;      // rho* (pressure) or d (divergence): "densityAdv"
+  float* kf = nullptr;        // k_i of the current Jacobi iteration
+  void swap_sorted() {
+    std::swap(pos, pos2); std::swap(vel, vel2); std::swap(kappa, kappa2);
+    std::swap(kappav, kappav2); std::swap(id, id2);
+  }
+};
+
+// Boundary samples of all bodies, concatenated body by body (fixed order).
+struct BoundaryBufs {
+  int n = 0;
+  int nd = 0;                 // samples of dynamic/animated bodies
+  float4* pos = nullptr;      // world xyz + volume in w
+  float4* vel = nullptr;      // xyz
+  float4* x0 = nullptr;       // local xyz + body index in w
+  uint32_t* dyn_idx = nullptr;  // indices of dynamic/animated samples
+};
+
+struct GridBufs {
+  uint32_t *fcount = nullptr, *fstart = nullptr, *bcount = nullptr, *bstart = nullptr;
+  uint32_t *key = nullptr, *slot = nullptr, *perm = nullptr;
+  uint32_t *bkey = nullptr, *bslot = nullptr, *bperm = nullptr;
+  uint32_t* tile_sums = nullptr;
+};
+
+struct NbrBufs {
+  int maxn = 0, maxbf = 0;
+  uint32_t* nbr = nullptr;    // [maxn][nf]
+  uint32_t* cnt = nullptr;    // [nf]  nF | nB << 16
+  uint32_t* bf = nullptr;     // [maxbf][nd]
+  uint32_t* bfcnt = nullptr;  // [nd]
+  uint32_t* overflow = nullptr;  // [2] max count seen when a list overflowed
+};
+
+// dfr_grid.hip
+size_t scan_tile_count(uint32_t n);
+void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* tile_sums, hipStream_t s);
+void launch_sort_fluid(FluidBufs& f, const GridDesc& g, GridBufs& gb, hipStream_t s);
+void launch_sort_boundary(BoundaryBufs& b, const GridDesc& g, GridBufs& gb, hipStream_t s);
+void launch_build_neighbors(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
+                            hipStream_t s);
+
+// dfr_solver.hip
+enum RecomputeMode { RECOMPUTE_DIV = 0, RECOMPUTE_PRESSURE = 1 };
+void launch_density_factor(const StepConst& c, FluidBufs& f, BoundaryBufs& b, NbrBufs& nb, int div_warm,
+                           int divergence_on, hipStream_t s);
+void launch_velocity_update(const StepConst& c, FluidBufs& f, BoundaryBufs& b, NbrBufs& nb, RigidDev* rig,
+                            const float* kfield, float* kappa_acc, hipStream_t s);
+void launch_recompute(const StepConst& c, FluidBufs& f, BoundaryBufs& b, NbrBufs& nb, int mode, float kcoef,
+                      double* err_slot, hipStream_t s);
+void launch_pressure_warm(const StepConst& c, FluidBufs& f, BoundaryBufs& b, NbrBufs& nb, hipStream_t s);
+void launch_advect(const StepConst& c, FluidBufs& f, float kappav_scale, hipStream_t s);
+void launch_position(const StepConst& c, FluidBufs& f, float kappa_scale, hipStream_t s);
+void launch_boundary_volume_impl(const float4* bpos_in, const float4* bx0, int nb, const int* groups, int nbodies,
+                                 KernelConst K, float h2, const GridDesc& g, GridBufs& gb, float4* bpos_out,
+                                 hipStream_t s);
+
+// dfr_rigid.hip
+struct GradConst {
+  int mode;              // 0 pressure, 1 divergence
+  float coeff_trunc;     // (unsigned int)(1/dt^2) or (unsigned int)(1/dt)
+  float coeff;           // 1/dt^2 or 1/dt
+  int optimize_rotation;
+};
+void launch_gradient(const StepConst& c, const GradConst& gc, FluidBufs& f, BoundaryBufs& b, NbrBufs& nb,
+                     RigidDev* rig, int k_first, int k_count, int body, hipStream_t s);
+struct RigidStepConst {
+  double dt;
+  double g[3];
+  int gradient_mode;     // 0 Complete, 1 Incomplete, 2 RigidGradOnly
+  int optimize_rotation;
+  int gyroscopic;
+  int compute_gradient;
+  int nbodies;
+};
+void launch_rigid_step(const RigidStepConst& rc, RigidDev* rig, hipStream_t s);
+void launch_boundary_transform(const uint32_t* idx, int n, BoundaryBufs& b, const RigidDev* rig, hipStream_t s);
+void launch_begin_step(RigidDev* rig, int nbodies, const int* ramp_flags, const double* factors,
+                       const int* animated, hipStream_t s);
+
+}  // namespace dfr

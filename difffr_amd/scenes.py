@@ -1,0 +1,221 @@
+"""Procedural fluid-rigid scenes (host side, numpy).
+
+The reference builds its scenes from JSON + OBJ meshes (SimulatorBase::createFluidBlocks,
+Simulator/SimulatorBase.cpp:1638-1735; surface sampling in RigidBody3dBoundarySimulator,
+Simulator/RigidBody3dBoundarySimulator.cpp:68-248).  The workloads measured here use
+procedural shapes instead: a block lattice laid out by the reference's block rule,
+box/tank walls sampled on a regular grid, and spheres/ellipsoids sampled with a
+Fibonacci lattice.  Every sample set is expressed in the body's local frame, like the
+reference's `m_x0` (BoundaryModel_Akinci2012::initModel, :286-374).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class Body:
+    x0: np.ndarray              # (n, 3) local samples
+    dynamic: bool
+    x_rb: np.ndarray            # (3,) position of the local origin
+    q: np.ndarray               # (4,) rotation (w, x, y, z)
+    density: float              # body density for the mass properties
+    v0: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(3))
+    w0: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(3))
+    name: str = ""
+
+
+@dataclasses.dataclass
+class Scene:
+    name: str
+    params: dict
+    fluid_x: np.ndarray         # (N, 3)
+    fluid_v: np.ndarray         # (N, 3)
+    bodies: List[Body]
+    steps: int
+
+    @property
+    def n_fluid(self) -> int:
+        return int(self.fluid_x.shape[0])
+
+    @property
+    def n_boundary(self) -> int:
+        return int(sum(b.x0.shape[0] for b in self.bodies))
+
+
+def default_params(radius: float, dt: float = 1e-3) -> dict:
+    """Solver parameters at the reference defaults (TimeStep.cpp:24-64,
+    TimeStepDiffDFSPH.cpp:90-111, Simulation.cpp:100-125, SceneLoader.cpp:45-87)."""
+    return dict(
+        radius=radius, dt=dt, rho0=1000.0, gravity=(0.0, -9.81, 0.0),
+        max_error=0.01, max_error_v=0.1, min_iterations=2, max_iterations=100,
+        max_iterations_v=100, use_pressure_warmstart=1, use_divergence_warmstart=1,
+        enable_divergence_solver=1, gradient_mode=1, optimize_rotation=1, gyroscopic=1,
+        compute_gradient=1, uniform_acc_time=0.0, target_time=1e30,
+    )
+
+
+def block_lattice(lo, counts, spacing) -> np.ndarray:
+    """Regular lattice with `counts` particles per axis starting at lo + spacing
+    (the reference's block rule, SimulatorBase.cpp:1660-1716, mode 0)."""
+    nx, ny, nz = counts
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    pts = np.stack([ix, iy, iz], axis=-1).reshape(-1, 3).astype(np.float64)
+    return np.asarray(lo, np.float64) + spacing * (pts + 1.0)
+
+
+def box_surface(lo, hi, spacing, open_top=False) -> np.ndarray:
+    """Single-layer samples on the faces of an axis-aligned box, no duplicates on edges."""
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    n = np.maximum(np.round((hi - lo) / spacing).astype(int), 1)
+    axes = [np.linspace(lo[k], hi[k], n[k] + 1) for k in range(3)]
+    pts = []
+    for k in range(3):
+        for side, val in ((0, lo[k]), (1, hi[k])):
+            if open_top and k == 1 and side == 1:
+                continue
+            a, b = [j for j in range(3) if j != k]
+            A, B = np.meshgrid(axes[a], axes[b], indexing="ij")
+            p = np.zeros((A.size, 3))
+            p[:, k] = val
+            p[:, a] = A.ravel()
+            p[:, b] = B.ravel()
+            pts.append(p)
+    pts = np.concatenate(pts)
+    key = np.round(pts / (spacing * 1e-3)).astype(np.int64)
+    _, keep = np.unique(key, axis=0, return_index=True)
+    return pts[np.sort(keep)]
+
+
+def fibonacci_sphere(n: int) -> np.ndarray:
+    i = np.arange(n, dtype=np.float64) + 0.5
+    phi = np.arccos(1.0 - 2.0 * i / n)
+    theta = math.pi * (1.0 + 5.0 ** 0.5) * i
+    return np.stack([np.cos(theta) * np.sin(phi), np.cos(phi), np.sin(theta) * np.sin(phi)], axis=1)
+
+
+def sphere_surface(radius: float, spacing: float) -> np.ndarray:
+    n = max(int(round(4.0 * math.pi * radius * radius / (spacing * spacing))), 8)
+    return radius * fibonacci_sphere(n)
+
+
+def ellipsoid_area(a: float, b: float, c: float) -> float:
+    p = 1.6075  # Knud Thomsen's approximation
+    return 4.0 * math.pi * (((a * b) ** p + (a * c) ** p + (b * c) ** p) / 3.0) ** (1.0 / p)
+
+
+def ellipsoid_surface(axes, spacing: float) -> np.ndarray:
+    a, b, c = axes
+    n = max(int(round(ellipsoid_area(a, b, c) / (spacing * spacing))), 8)
+    return fibonacci_sphere(n) * np.asarray([a, b, c], np.float64)
+
+
+def box_body_surface(extent, spacing: float) -> np.ndarray:
+    e = 0.5 * np.asarray(extent, np.float64)
+    return box_surface(-e, e, spacing)
+
+
+def random_unit_quaternion(rng: np.random.Generator) -> np.ndarray:
+    q = rng.normal(size=4)
+    return q / np.linalg.norm(q)
+
+
+def sphere_in_box(n_per_axis: int = 20, radius: float = 0.025, steps: int = 100,
+                  v0=(1.0, 0.0, 0.0), w0=(0.0, 0.0, 0.0), dt: float = 1e-3) -> Scene:
+    """8,000-particle block (20^3 at spacing 2r, zero velocity) in a closed box of
+    Akinci wall samples, with one dynamic sphere (radius 0.1, density 500,
+    surface samples at spacing 2r) released just above the block with v0, w0.
+
+    The block at spacing 2r spans 20 * 2r = 1.0 m per axis, so the walls sit 2r
+    outside it (1.05 m wide) and the box is 1.5 m tall to hold the sphere."""
+    sp = 2.0 * radius
+    n = n_per_axis
+    fluid = block_lattice((0.0, 0.0, 0.0), (n, n, n), sp)
+    width = sp * (n + 1)
+    walls = box_surface((0.0, 0.0, 0.0), (width, 1.5 * width, width), sp)
+    top = fluid[:, 1].max()
+    sphere_r = 0.1
+    center = np.array([0.5 * width, top + sp + sphere_r, 0.5 * width])
+    bodies = [
+        Body(walls, False, np.zeros(3), np.array([1.0, 0, 0, 0]), 1000.0, name="box"),
+        Body(sphere_surface(sphere_r, sp), True, center, np.array([1.0, 0, 0, 0]), 500.0,
+             np.asarray(v0, np.float64), np.asarray(w0, np.float64), name="sphere"),
+    ]
+    return Scene("sphere_in_box_8k", default_params(radius, dt), fluid, np.zeros_like(fluid), bodies, steps)
+
+
+def tank(counts, spacing: float, body_kind: str = "ellipsoid", seed: int = 0, steps: int = 200,
+         jitter: float = 0.1, dt: float = 1e-3, wall_height: Optional[float] = None,
+         name: str = "tank", body_specs=None) -> Scene:
+    """Jittered lattice (uniform +-jitter*r per axis, seeded) filling an open tank,
+    with one or more floating rigid bodies released just above the surface."""
+    rng = np.random.default_rng(seed)
+    r = 0.5 * spacing
+    fluid = block_lattice((0.0, 0.0, 0.0), counts, spacing)
+    fluid += rng.uniform(-jitter * r, jitter * r, size=fluid.shape)
+    inner = spacing * (np.asarray(counts) + 1)
+    height = wall_height if wall_height is not None else max(1.0, 2.0 * inner[1])
+    walls = box_surface((0.0, 0.0, 0.0), (inner[0], height, inner[2]), spacing, open_top=True)
+    bodies = [Body(walls, False, np.zeros(3), np.array([1.0, 0, 0, 0]), 1000.0, name="tank")]
+    top = float(fluid[:, 1].max())
+    if body_specs is None:
+        if body_kind == "ellipsoid":
+            body_specs = [("ellipsoid", (0.2, 0.1, 0.15), 600.0)]
+        elif body_kind == "sphere":
+            body_specs = [("sphere", 0.15, 500.0)]
+        else:
+            body_specs = []
+    nb = len(body_specs)
+    for k, (kind, size, dens) in enumerate(body_specs):
+        if kind == "ellipsoid":
+            x0 = ellipsoid_surface(size, spacing)
+            half_h = max(size)
+        elif kind == "sphere":
+            x0 = sphere_surface(size, spacing)
+            half_h = size
+        elif kind == "box":
+            x0 = box_body_surface(size, spacing)
+            half_h = 0.5 * float(np.linalg.norm(size))
+        else:
+            raise ValueError(kind)
+        fx = (k + 1) / (nb + 1)
+        center = np.array([fx * inner[0], top + r + half_h + spacing, 0.5 * inner[2]])
+        v0 = rng.uniform(-2.0, 2.0, size=3)
+        w0 = rng.uniform(-3.0, 3.0, size=3)
+        bodies.append(Body(x0, True, center, np.array([1.0, 0, 0, 0]), dens, v0, w0, name=kind))
+    return Scene(name, default_params(r, dt), fluid, np.zeros_like(fluid), bodies, steps)
+
+
+def tank_250k(seed: int = 0, steps: int = 200) -> Scene:
+    """250,000 particles (100 x 25 x 100 at spacing 0.02, r = 0.01) in a ~2 x 1 x 2 m
+    tank to 0.5 m depth, one ellipsoid (0.2/0.1/0.15, density 600) with seeded
+    v0 in [-2,2]^3 and omega0 in [-3,3]^3."""
+    return tank((100, 25, 100), 0.02, "ellipsoid", seed=seed, steps=steps, wall_height=1.0,
+                name="tank_250k_ellipsoid")
+
+
+def tank_1m(seed: int = 0, steps: int = 200) -> Scene:
+    """1,000,188 particles (189 x 28 x 189 at spacing 0.02125, r = 0.010625) in a
+    ~4 x 1 x 4 m tank to 0.6 m depth with one dropped sphere (radius 0.15)."""
+    return tank((189, 28, 189), 0.02125, "sphere", seed=seed, steps=steps, wall_height=1.0,
+                name="tank_1m_sphere")
+
+
+def tank_4m(seed: int = 0, steps: int = 200) -> Scene:
+    """3,958,528 particles (376 x 28 x 376 at spacing 0.02125) in a ~8 x 1 x 8 m tank
+    with three bodies: sphere r=0.15, box 0.3x0.1x0.2, ellipsoid 0.2/0.1/0.15."""
+    specs = [("sphere", 0.15, 500.0), ("box", (0.3, 0.1, 0.2), 500.0),
+             ("ellipsoid", (0.2, 0.1, 0.15), 600.0)]
+    return tank((376, 28, 376), 0.02125, seed=seed, steps=steps, wall_height=1.0,
+                name="tank_4m_3bodies", body_specs=specs)
+
+
+def small_tank(counts=(24, 10, 24), seed: int = 0, steps: int = 20, body: str = "ellipsoid") -> Scene:
+    """A few-thousand-particle tank for fast parity tests (spacing 0.02)."""
+    specs = [("ellipsoid", (0.1, 0.05, 0.075), 600.0)] if body == "ellipsoid" else [("sphere", 0.06, 500.0)]
+    return tank(counts, 0.02, seed=seed, steps=steps, wall_height=0.5, name="small_tank", body_specs=specs)

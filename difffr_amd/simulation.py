@@ -1,0 +1,274 @@
+"""Host-side wrapper of one libdfr_hip.so context.
+
+`Simulation` owns a dfr_ctx built from a `difffr_amd.scenes.Scene`; its
+`boundary_model(i)` objects expose the getters DiffFR's drivers read from
+BoundaryModel_Akinci2012 (pySPlisHSPlasH/BoundaryModelModule.cpp:43-91), and the
+simulation exposes the TimeStepDiffDFSPH-level ones
+(pySPlisHSPlasH/DiffDFSPHModule.cpp:44-102).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class DfrError(RuntimeError):
+    pass
+
+
+def make_params(p: dict) -> native.Params:
+    L = native.load()
+    q = native.Params()
+    L.dfr_default_params(C.byref(q), float(p["radius"]), float(p["dt"]))
+    for k in ("rho0", "max_error", "max_error_v", "uniform_acc_time", "target_time"):
+        if k in p:
+            setattr(q, k, float(p[k]))
+    for k in ("min_iterations", "max_iterations", "max_iterations_v", "use_pressure_warmstart",
+              "use_divergence_warmstart", "enable_divergence_solver", "gradient_mode", "optimize_rotation",
+              "gyroscopic", "compute_gradient", "max_neighbors", "max_boundary_neighbors"):
+        if k in p:
+            setattr(q, k, int(p[k]))
+    if "gravity" in p:
+        for k in range(3):
+            q.gravity[k] = float(p["gravity"][k])
+    return q
+
+
+class BoundaryModel:
+    """Getters of one body (BoundaryModel_Akinci2012.h:169-323)."""
+
+    def __init__(self, sim: "Simulation", index: int):
+        self.sim = sim
+        self.index = index
+
+    def _rigid(self):
+        return self.sim.rigid(self.index)
+
+    def get_position_rb(self):
+        return self._rigid()["x"]
+
+    def get_quaternion_rb_vec4(self):
+        return self._rigid()["q"]
+
+    def get_velocity_rb(self):
+        return self._rigid()["v"]
+
+    def get_angular_velocity_rb(self):
+        return self._rigid()["omega"]
+
+    def getForce(self):
+        return self._rigid()["force"]
+
+    def getTorque(self):
+        return self._rigid()["torque"]
+
+    def _J(self):
+        return self.sim.jacobians(self.index)
+
+    def get_grad_x_to_v0(self):
+        return self._J()["Jxv"]
+
+    def get_grad_x_to_omega0(self):
+        return self._J()["Jxw"]
+
+    def get_grad_v_to_v0(self):
+        return self._J()["Jvv"]
+
+    def get_grad_v_to_omega0(self):
+        return self._J()["Jvw"]
+
+    def get_grad_omega_to_v0(self):
+        return self._J()["Jwv"]
+
+    def get_grad_omega_to_omega0(self):
+        return self._J()["Jww"]
+
+    def get_grad_quaternion_to_v0(self):
+        return self._J()["Jqv"]
+
+    def get_grad_quaternion_to_omega0(self):
+        return self._J()["Jqw"]
+
+
+def unpack_jacobians(o: np.ndarray) -> dict:
+    names = ("Jxv", "Jxw", "Jvv", "Jvw", "Jwv", "Jww")
+    d = {n: o[9 * k:9 * k + 9].reshape(3, 3).copy() for k, n in enumerate(names)}
+    d["Jqv"] = o[54:66].reshape(4, 3).copy()
+    d["Jqw"] = o[66:78].reshape(4, 3).copy()
+    return d
+
+
+def unpack_net(o: np.ndarray) -> dict:
+    names = ("Fv", "Fx", "Fw", "Tw", "Tx", "Tv")
+    d = {n: o[9 * k:9 * k + 9].reshape(3, 3).copy() for k, n in enumerate(names)}
+    d["Fq"] = o[54:66].reshape(3, 4).copy()
+    d["Tq"] = o[66:78].reshape(3, 4).copy()
+    return d
+
+
+class Simulation:
+    """One MI355X simulation context built from a scene."""
+
+    def __init__(self, scene, device: int = 0, params_override: dict | None = None):
+        L = native.load()
+        p = dict(scene.params)
+        if params_override:
+            p.update(params_override)
+        self.params = p
+        self._p = make_params(p)
+        self.ctx = L.dfr_create(C.byref(self._p), int(device))
+        if not self.ctx:
+            raise DfrError("dfr_create failed (no HIP device?)")
+        self.n = scene.n_fluid
+        x = np.ascontiguousarray(scene.fluid_x, np.float32)
+        v = np.ascontiguousarray(scene.fluid_v, np.float32)
+        self._check(L.dfr_set_fluid(self.ctx, self.n, _fp(x), _fp(v)))
+        self.n_bodies = 0
+        self.body_sizes = []
+        self._dyn = [bool(b.dynamic) for b in scene.bodies]
+        for b in scene.bodies:
+            x0 = np.ascontiguousarray(b.x0, np.float32)
+            xr = np.ascontiguousarray(b.x_rb, np.float64)
+            q = np.ascontiguousarray(b.q, np.float64)
+            bid = L.dfr_add_boundary(self.ctx, x0.shape[0], _fp(x0), int(b.dynamic), _dp(xr), _dp(q),
+                                     float(b.density))
+            self._check(bid)
+            v0 = np.ascontiguousarray(b.v0, np.float64)
+            w0 = np.ascontiguousarray(b.w0, np.float64)
+            self._check(L.dfr_set_init_velocity(self.ctx, bid, _dp(v0), _dp(w0)))
+            self.n_bodies += 1
+            self.body_sizes.append(x0.shape[0])
+        self._check(L.dfr_init(self.ctx))
+
+    def _check(self, r):
+        if r < 0:
+            msg = native.load().dfr_last_error(self.ctx)
+            raise DfrError(f"{native.ERRORS.get(r, r)}: {msg.decode() if msg else ''}")
+        return r
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            native.load().dfr_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- stepping -----------------------------------------------------------
+    def step(self, n: int = 1):
+        self._check(native.load().dfr_step(self.ctx, int(n)))
+
+    def reset(self):
+        self._check(native.load().dfr_reset(self.ctx))
+
+    def reset_gradient(self):
+        self._check(native.load().dfr_reset_gradient(self.ctx))
+
+    # -- state --------------------------------------------------------------
+    def fluid(self):
+        x = np.empty((self.n, 3), np.float32)
+        v = np.empty((self.n, 3), np.float32)
+        self._check(native.load().dfr_get_fluid(self.ctx, _fp(x), _fp(v)))
+        return x, v
+
+    def field(self, name: str) -> np.ndarray:
+        k = 3 if name == "sum_grad_p_k" else 1
+        out = np.empty(self.n * k, np.float32)
+        self._check(native.load().dfr_get_field(self.ctx, name.encode(), _fp(out)))
+        return out.reshape(self.n, 3) if k == 3 else out
+
+    def set_field(self, name: str, values: np.ndarray):
+        a = np.ascontiguousarray(values, np.float32)
+        self._check(native.load().dfr_set_field(self.ctx, name.encode(), _fp(a)))
+
+    def body_field(self, body: int, name: str) -> np.ndarray:
+        n = self.body_sizes[body]
+        k = 1 if name == "volume" else 3
+        out = np.empty(n * k, np.float32)
+        self._check(native.load().dfr_get_boundary_field(self.ctx, body, name.encode(), _fp(out)))
+        return out if k == 1 else out.reshape(n, 3)
+
+    def rigid(self, body: int) -> dict:
+        o = np.empty(19)
+        self._check(native.load().dfr_get_rigid(self.ctx, body, _dp(o)))
+        return dict(x=o[0:3], q=o[3:7], v=o[7:10], omega=o[10:13], force=o[13:16], torque=o[16:19])
+
+    def set_rigid(self, body: int, x, q, v, omega):
+        a = np.ascontiguousarray(np.concatenate([x, q, v, omega]), np.float64)
+        self._check(native.load().dfr_set_rigid(self.ctx, body, _dp(a)))
+
+    def jacobians_raw(self, body: int) -> np.ndarray:
+        o = np.empty(78)
+        self._check(native.load().dfr_get_jacobians(self.ctx, body, _dp(o)))
+        return o
+
+    def jacobians(self, body: int) -> dict:
+        return unpack_jacobians(self.jacobians_raw(body))
+
+    def net_gradients_raw(self, body: int) -> np.ndarray:
+        o = np.empty(78)
+        self._check(native.load().dfr_get_net_gradients(self.ctx, body, _dp(o)))
+        return o
+
+    def stats(self) -> dict:
+        s = native.Stats()
+        self._check(native.load().dfr_get_stats(self.ctx, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in native.Stats._fields_}
+
+    def reset_stats(self):
+        self._check(native.load().dfr_reset_stats(self.ctx))
+
+    def neighbor_stats(self) -> dict:
+        o = np.empty(3)
+        self._check(native.load().dfr_neighbor_stats(self.ctx, _dp(o)))
+        return dict(mean_fluid=o[0], mean_boundary=o[1], mean_fluid_per_dynamic_sample=o[2])
+
+    def neighbor_lists(self, kind: int):
+        """kind 0: fluid->fluid (ids), 1: fluid->boundary (global sample index), 2: dynamic sample->fluid."""
+        L = native.load()
+        rows = self.n if kind in (0, 1) else sum(
+            s for s, dyn in zip(self.body_sizes, self._dyn) if dyn)
+        cap = max(rows, 1) * int(self._p.max_neighbors if kind != 2 else self._p.max_boundary_neighbors)
+        counts = np.empty(max(rows, 1), np.uint32)
+        lists = np.empty(max(cap, 1), np.uint32)
+        r = L.dfr_get_neighbors(self.ctx, kind, counts.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                lists.ctypes.data_as(C.POINTER(C.c_uint32)), cap)
+        self._check(int(r))
+        off = np.concatenate([[0], np.cumsum(counts[:r].astype(np.int64))])
+        return [lists[off[k]:off[k + 1]].copy() for k in range(int(r))]
+
+    def set_profiling(self, on: bool):
+        self._check(native.load().dfr_set_profiling(self.ctx, int(on)))
+
+    def kernel_times(self) -> dict:
+        n = 16
+        names = (C.c_char * 32 * n)()
+        avg = np.zeros(n)
+        cnt = np.zeros(n, np.int64)
+        k = native.load().dfr_kernel_times(self.ctx, n, C.cast(names, C.c_void_p), _dp(avg),
+                                           cnt.ctypes.data_as(C.POINTER(C.c_int64)))
+        return {names[i].value.decode(): (float(avg[i]), int(cnt[i])) for i in range(k)}
+
+    # -- TimeStepDiffDFSPH-level getters -------------------------------------
+    def get_boundary_model(self, i: int) -> BoundaryModel:
+        return BoundaryModel(self, i)
+
+    def is_trajectory_finish_callback(self) -> bool:
+        return bool(self.stats()["trajectory_finished"])
+
+    def get_step_count(self) -> int:
+        return int(self.stats()["step_count"])

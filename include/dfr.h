@@ -1,0 +1,178 @@
+/*
+ * libdfr_hip.so -- C ABI of the MI355X (gfx950) differentiable DFSPH
+ * fluid-rigid time step.
+ *
+ * The library owns all device memory of one simulation context.  Host
+ * pointers passed in are borrowed for the duration of the call only.  A
+ * context is bound to one device and one HIP stream; it is not thread-safe.
+ * Every function returns 0 on success and a negative DFR_ERR_* code on
+ * failure; dfr_last_error() then holds a message.
+ *
+ * Reference interfaces replaced (paths relative to the DiffFR repository):
+ *   SPlisHSPlasH/DiffDFSPH/TimeStepDiffDFSPH.{h,cpp}      TimeStepDiffDFSPH::step and getters
+ *   SPlisHSPlasH/BoundaryModel_Akinci2012.{h,cpp}          boundary samples, coupling, gradient storage
+ *   SPlisHSPlasH/Dynamic3dRigidBody.h                      rigid state and integration
+ *   Simulator/SimulatorBase.cpp:1142-1200                  timeStepNoGUI step order
+ */
+#ifndef DFR_H
+#define DFR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DFR_OK 0
+#define DFR_ERR_ARG -1        /* invalid argument or call order */
+#define DFR_ERR_HIP -2        /* HIP runtime error */
+#define DFR_ERR_CAPACITY -3   /* a neighbour list exceeded its capacity */
+#define DFR_ERR_STATE -4      /* context not initialised / already initialised */
+
+typedef struct dfr_ctx dfr_ctx;
+
+/* Solver parameters.  Defaults (dfr_default_params) are the reference's:
+ * TimeStep.cpp:24-64 (minIterations 2, maxIterations 100, maxError 0.01 %),
+ * TimeStepDiffDFSPH.cpp:90-111 (maxIterationsV 100, maxErrorV 0.1 %, warm
+ * starts and divergence solver on, optimize rotation on), Simulation.cpp:100-125
+ * (gravity, rigid-body mode WithGyroscopic), SceneLoader.cpp:45-87
+ * (gradientMode 1). */
+typedef struct dfr_params {
+  double radius;                 /* particle radius r; support radius h = 4r */
+  double dt;                     /* fixed time step */
+  double rho0;                   /* rest density */
+  double gravity[3];
+  double max_error;              /* pressure solve tolerance, percent of rho0 */
+  double max_error_v;            /* divergence solve tolerance, percent of rho0 per second */
+  int32_t min_iterations;
+  int32_t max_iterations;
+  int32_t max_iterations_v;
+  int32_t use_pressure_warmstart;
+  int32_t use_divergence_warmstart;
+  int32_t enable_divergence_solver;
+  int32_t gradient_mode;         /* 0 Complete, 1 Incomplete, 2 RigidGradOnly */
+  int32_t optimize_rotation;
+  int32_t gyroscopic;            /* 1 = RigidBodyMode::WithGyroscopic */
+  int32_t compute_gradient;      /* accumulate DiffFR's forward-mode rigid gradient */
+  double uniform_acc_time;       /* uniform_acc_rb_time */
+  double target_time;
+  int32_t max_neighbors;         /* per fluid particle (fluid + boundary), default 96 */
+  int32_t max_boundary_neighbors;/* fluid neighbours per dynamic sample, default 128 */
+} dfr_params;
+
+typedef struct dfr_stats {
+  int32_t iterations;            /* pressure iterations of the last step (m_iterations) */
+  int32_t iterations_v;          /* divergence iterations of the last step (m_iterationsV) */
+  double avg_density_err;        /* kg/m^3, last pressure iteration */
+  double avg_div_err;            /* kg/m^3/s, last divergence iteration */
+  double time;                   /* simulated time (FP32 accumulation, as TimeManager) */
+  int64_t step_count;
+  int64_t total_iterations;      /* sums since the last dfr_reset_stats */
+  int64_t total_iterations_v;
+  int64_t steps_since_reset;
+  int32_t trajectory_finished;   /* TimeStepDiffDFSPH::is_trajectory_finish_callback */
+} dfr_stats;
+
+/* Fills p with the reference defaults for a given particle radius and dt. */
+void dfr_default_params(dfr_params* p, double radius, double dt);
+
+/* Creates a context on HIP device `device`.  Replaces the construction of
+ * Simulation + TimeStepDiffDFSPH (Simulation.cpp:195-216, 714-720). */
+dfr_ctx* dfr_create(const dfr_params* p, int device);
+int dfr_destroy(dfr_ctx* ctx);
+const char* dfr_last_error(const dfr_ctx* ctx);
+
+/* Fluid particles, AoS float[n*3] like FluidModel::m_x / m_v (FluidModel.h:110-151).
+ * v may be NULL (zero velocity).  Must precede dfr_init. */
+int dfr_set_fluid(dfr_ctx* ctx, int64_t n, const float* x, const float* v);
+
+/* One boundary body of Akinci samples (BoundaryModel_Akinci2012::initModel,
+ * :286-374) given in the body's local frame, placed at x_rb with rotation q
+ * (w,x,y,z); mass properties follow Dynamic3dRigidBody::determineMassProperties
+ * (Dynamic3dRigidBody.h:184-213).  Returns the body index (>= 0). */
+int dfr_add_boundary(dfr_ctx* ctx, int64_t n, const float* x0, int is_dynamic, const double x_rb[3],
+                     const double q_wxyz[4], double density);
+
+/* init_v_rb / init_omega_rb of a dynamic body (TimeStepDiffDFSPH.h set_init_v_rb). */
+int dfr_set_init_velocity(dfr_ctx* ctx, int body, const double v0[3], const double omega0[3]);
+
+/* Uploads, places the boundary samples and computes the Akinci volumes
+ * (Simulation::updateBoundaryVolume, Simulation.cpp:831-902). */
+int dfr_init(dfr_ctx* ctx);
+
+/* Runs nsteps full steps in SimulatorBase::timeStepNoGUI order: DFSPH step with
+ * DiffFR's gradient accumulation (TimeStepDiffDFSPH::step, :526-652), rigid
+ * velocity and position update and the boundary transform. */
+int dfr_step(dfr_ctx* ctx, int nsteps);
+
+/* Restores the fluid and rigid bodies to their state at dfr_init, resets the
+ * gradient chain and time (SimulatorBase::reset, SimulatorBase.cpp:887-934). */
+int dfr_reset(dfr_ctx* ctx);
+
+/* Rigid state out[19]: x(3), q(w,x,y,z)(4), v(3), omega(3), force(3), torque(3)
+ * of the last step (BoundaryModel_Akinci2012::get_position_rb, ...
+ * BoundaryModel_Akinci2012.h:169-190, BoundaryModel::getForce/getTorque). */
+int dfr_get_rigid(const dfr_ctx* ctx, int body, double out[19]);
+
+/* Overwrites the rigid state x, q, v, omega (in[13]). */
+int dfr_set_rigid(dfr_ctx* ctx, int body, const double in[13]);
+
+/* Forward-mode rigid Jacobians out[78]: Jxv, Jxw, Jvv, Jvw, Jwv, Jww (3x3
+ * row-major), then Jqv, Jqw (4x3, rows w,x,y,z) -- get_grad_x_to_v0 etc.,
+ * BoundaryModel_Akinci2012.h:209-323. */
+int dfr_get_jacobians(const dfr_ctx* ctx, int body, double out[78]);
+
+/* Pair-Jacobian sums of the last step out[78]: dF/dv, dF/dx, dF/domega,
+ * dtau/domega, dtau/dx, dtau/dv (3x3), dF/dq, dtau/dq (3x4)
+ * (grad_net_force_to_vn etc., BoundaryModel_Akinci2012.cpp:453-497). */
+int dfr_get_net_gradients(const dfr_ctx* ctx, int body, double out[78]);
+
+/* Clears the forward-mode Jacobians to their initial values
+ * (TimeStepDiffDFSPH reset_gradient / BoundaryModel_Akinci2012::reset_gradient). */
+int dfr_reset_gradient(dfr_ctx* ctx);
+
+/* Fluid state in particle-id order (the order given to dfr_set_fluid).  Any
+ * pointer may be NULL. */
+int dfr_get_fluid(const dfr_ctx* ctx, float* x, float* v);
+
+/* A per-particle field in particle-id order: "density", "factor",
+ * "density_adv", "kappa", "kappa_v" (n floats) or "sum_grad_p_k" (3n). */
+int dfr_get_field(const dfr_ctx* ctx, const char* name, float* out);
+
+/* Overwrites "kappa" or "kappa_v" (n floats, particle-id order). */
+int dfr_set_field(dfr_ctx* ctx, const char* name, const float* in);
+
+/* Boundary sample data of one body: "x", "v" (3n floats) or "volume" (n). */
+int dfr_get_boundary_field(const dfr_ctx* ctx, int body, const char* name, float* out);
+
+int dfr_get_stats(const dfr_ctx* ctx, dfr_stats* out);
+int dfr_reset_stats(dfr_ctx* ctx);
+
+/* Neighbour-list diagnostics of the last step: mean fluid and boundary
+ * neighbours per fluid particle and mean fluid neighbours per dynamic sample. */
+int dfr_neighbor_stats(const dfr_ctx* ctx, double out[3]);
+
+/* Neighbour lists of the last step in particle-id / sample-index terms:
+ * kind 0 fluid->fluid (ids), kind 1 fluid->boundary (global sample index of
+ * `counts`-ordered bodies), kind 2 dynamic sample->fluid (ids).  counts gets
+ * one entry per row; lists gets the concatenated rows (capacity `cap`
+ * entries).  Returns the number of rows or a negative error. */
+int64_t dfr_get_neighbors(const dfr_ctx* ctx, int kind, uint32_t* counts, uint32_t* lists, int64_t cap);
+
+/* Per-kernel-class timing with HIP events on the context's stream
+ * (0 = off).  dfr_kernel_times fills names/avg-ms/launch-count for up to n
+ * classes and returns the number of classes. */
+int dfr_set_profiling(dfr_ctx* ctx, int on);
+int dfr_kernel_times(const dfr_ctx* ctx, int n, char names[][32], double* avg_ms, int64_t* launches);
+
+/* Number of fluid particles / boundary samples / bodies. */
+int64_t dfr_num_fluid(const dfr_ctx* ctx);
+int64_t dfr_num_boundary(const dfr_ctx* ctx, int body);
+int dfr_num_bodies(const dfr_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DFR_H */

@@ -1,0 +1,907 @@
+// ORACLE -- CPU restatement of DiffFR's differentiable DFSPH fluid-rigid step.
+//
+// This is test infrastructure: only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg use it, and only as the checker (or as the timed
+// CPU baseline).  The product (difffr_amd/, libdfr_hip.so) never links it.
+//
+// It follows the reference loop by loop.  Every function cites the reference
+// file:line it restates (paths relative to the DiffFR repository root).
+// Templated on RF (bulk particle arithmetic) and RR (rigid state, force/torque
+// partials, gradient accumulators, chain rule):
+//   <float, float>   = the reference's own precision (Real = float,
+//                      SPlisHSPlasH/Common.h:16);
+//   <double, double> = the FP64 oracle run;
+//   <float, double>  = the GPU design point (FP32 bulk, FP64 rigid).
+//
+// Neighbour predicate (CompactNSearch's own tie rule is not available offline,
+// so this is a documented choice): j is a neighbour of i iff the squared
+// distance, evaluated in RF as dx*dx + dy*dy + dz*dz without fused multiply-add,
+// is strictly below h*h (also evaluated in RF).  Self pairs are excluded.
+// W, gradW and gradGradW vanish at q = 1, so the rule only decides neighbour
+// counts (the 20-neighbour rule of computeDensityChange).
+#pragma once
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "oracle_math.hpp"
+
+namespace orc {
+
+struct Params {
+  double radius = 0.025;         // particle radius r; support radius h = 4r
+  double dt = 1e-3;              // fixed time step (cflMethod 0)
+  double rho0 = 1000.0;          // FluidModel.cpp:45
+  double gravity[3] = {0.0, -9.81, 0.0};
+  double maxError = 0.01;        // percent, TimeStep.cpp:24-64
+  double maxErrorV = 0.1;        // percent, TimeStepDiffDFSPH.cpp:110
+  int minIterations = 2;
+  int maxIterations = 100;
+  int maxIterationsV = 100;
+  int usePressureWarmstart = 1;  // TimeStepDiffDFSPH.cpp:102
+  int useDivergenceWarmstart = 1;// TimeStepDiffDFSPH.cpp:103
+  int enableDivergenceSolver = 1;// TimeStepDiffDFSPH.cpp:97
+  int gradientMode = 1;          // 0 Complete, 1 Incomplete, 2 RigidGradOnly (SceneLoader.cpp:86)
+  int optimizeRotation = 1;      // TimeStepDiffDFSPH.cpp:108
+  int gyroscopic = 1;            // RigidBodyMode::WithGyroscopic, Simulation.cpp:114
+  int computeGradient = 1;       // the BACKWARD build of the reference
+  double uniformAccTime = 0.0;   // uniform_acc_rb_time
+  double targetTime = 1e30;
+};
+
+// Eval of the analytic cubic spline kernel: SPlisHSPlasH/SPHKernels.h:16-176.
+template <typename T> struct Cubic {
+  T h, k, l, w0;
+  void set(T radius) {
+    h = radius;
+    const T pi = T(M_PI);
+    const T h3 = h * h * h;
+    k = T(8.0) / (pi * h3);
+    l = T(48.0) / (pi * h3);
+    w0 = W(T(0));
+  }
+  T W(T r) const {  // SPHKernels.h:37-54
+    T res = 0;
+    const T q = r / h;
+    if (q <= T(1)) {
+      if (q <= T(0.5)) {
+        const T q2 = q * q, q3 = q2 * q;
+        res = k * (T(6) * q3 - T(6) * q2 + T(1));
+      } else {
+        res = k * (T(2) * std::pow(T(1) - q, T(3)));
+      }
+    }
+    return res;
+  }
+  T W(const V3<T>& r) const { return W(r.norm()); }
+  V3<T> gradW(const V3<T>& r) const {  // SPHKernels.h:81-101
+    const T rl = r.norm();
+    const T q = rl / h;
+    if ((rl > T(1.0e-5)) && (q <= T(1))) {
+      const V3<T> gradq = r * (T(1) / (rl * h));
+      if (q <= T(0.5)) return gradq * (l * q * (T(3) * q - T(2)));
+      const T f = T(1) - q;
+      return gradq * (l * (-f * f));
+    }
+    return V3<T>::Zero();
+  }
+  M3<T> gradGradW(const V3<T>& r) const {  // SPHKernels.h:122-149
+    const T rl = r.norm();
+    const T q = rl / h;
+    if ((rl > T(1.0e-5)) && (q <= T(1))) {
+      const V3<T> gradq = r * (T(1) / (rl * h));
+      const M3<T> ggq = (M3<T>::Identity() - outer(r, r) / (rl * rl)) * (T(1) / (rl * h));
+      // (s * gradq) gradq^T, the evaluation order of the Eigen expression
+      if (q <= T(0.5))
+        return ggq * (l * q * (T(3) * q - T(2))) + outer(gradq * (l * (T(6) * q - T(2))), gradq);
+      const T f = T(1) - q;
+      return ggq * (l * (-f * f)) + outer(gradq * (l * T(2) * f), gradq);
+    }
+    return M3<T>::Zero();
+  }
+};
+
+// Squared distance without contraction, shared with the GPU predicate.
+template <typename T> inline T dist2_nofma(const V3<T>& a, const V3<T>& b) {
+  volatile T dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  volatile T xx = dx * dx, yy = dy * dy, zz = dz * dz;
+  volatile T s = xx + yy;
+  return s + zz;
+}
+
+// Uniform grid used only to enumerate candidates; the predicate above decides.
+template <typename RF> struct Grid {
+  RF cs;
+  V3<RF> lo;
+  int nx = 1, ny = 1, nz = 1;
+  std::vector<uint32_t> start, idx;
+  void build(const std::vector<V3<RF>>& pts, RF cell) {
+    cs = cell;
+    V3<RF> mn(1e30f, 1e30f, 1e30f), mx(-1e30f, -1e30f, -1e30f);
+    for (auto& p : pts) {
+      mn.x = std::min(mn.x, p.x); mn.y = std::min(mn.y, p.y); mn.z = std::min(mn.z, p.z);
+      mx.x = std::max(mx.x, p.x); mx.y = std::max(mx.y, p.y); mx.z = std::max(mx.z, p.z);
+    }
+    if (pts.empty()) { mn = V3<RF>(0, 0, 0); mx = mn; }
+    lo = mn - V3<RF>(cs, cs, cs);
+    nx = int((mx.x - lo.x) / cs) + 3; ny = int((mx.y - lo.y) / cs) + 3; nz = int((mx.z - lo.z) / cs) + 3;
+    const size_t nc = size_t(nx) * ny * nz;
+    start.assign(nc + 1, 0);
+    idx.resize(pts.size());
+    std::vector<uint32_t> key(pts.size());
+    for (size_t i = 0; i < pts.size(); ++i) { key[i] = cell_of(pts[i]); start[key[i] + 1]++; }
+    for (size_t c = 0; c < nc; ++c) start[c + 1] += start[c];
+    std::vector<uint32_t> cur(start.begin(), start.end() - 1);
+    for (size_t i = 0; i < pts.size(); ++i) idx[cur[key[i]]++] = uint32_t(i);  // stable by index
+  }
+  void coords(const V3<RF>& p, int& cx, int& cy, int& cz) const {
+    cx = int(std::floor((p.x - lo.x) / cs)); cy = int(std::floor((p.y - lo.y) / cs)); cz = int(std::floor((p.z - lo.z) / cs));
+    cx = std::min(std::max(cx, 0), nx - 1); cy = std::min(std::max(cy, 0), ny - 1); cz = std::min(std::max(cz, 0), nz - 1);
+  }
+  uint32_t cell_of(const V3<RF>& p) const { int cx, cy, cz; coords(p, cx, cy, cz); return uint32_t((size_t(cz) * ny + cy) * nx + cx); }
+  // Neighbours of q among pts (indices into pts), ascending index order within
+  // each cell, cells in (dz, dy, dx) order; finally sorted ascending.
+  void query(const V3<RF>& q, const std::vector<V3<RF>>& pts, RF h2, int64_t self, std::vector<uint32_t>& out) const {
+    out.clear();
+    int cx, cy, cz;
+    coords(q, cx, cy, cz);
+    for (int dz = -1; dz <= 1; ++dz) for (int dy = -1; dy <= 1; ++dy) for (int dx = -1; dx <= 1; ++dx) {
+      const int x = cx + dx, y = cy + dy, z = cz + dz;
+      if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) continue;
+      const size_t c = (size_t(z) * ny + y) * nx + x;
+      for (uint32_t k = start[c]; k < start[c + 1]; ++k) {
+        const uint32_t j = idx[k];
+        if (int64_t(j) == self) continue;
+        if (dist2_nofma(q, pts[j]) < h2) out.push_back(j);
+      }
+    }
+    std::sort(out.begin(), out.end());
+  }
+};
+
+template <typename RF, typename RR> struct Body {
+  bool dynamic = false, animated = false;
+  std::vector<V3<RF>> x0, x, v;  // local samples, world positions, velocities
+  std::vector<RF> V;             // Akinci volumes
+  RR mass = 0, invMass = 0;
+  M3<RR> I0, I, Iinv;
+  V3<RR> xrb, vrb, wrb, x_init, v_init, w_init;  // x_init etc. = state restored by reset
+  Quat<RR> q, q_init;
+  V3<RR> v0, w0;                 // init_v_rb / init_omega_rb of the driver
+  std::vector<V3<RR>> Fth, Tth;  // per-thread partials, BoundaryModel.h:46-58
+  V3<RR> F_last, T_last;         // force/torque of the last step (getForce/getTorque)
+  // per-boundary-particle Jacobians, BoundaryModel_Akinci2012.h:42-50
+  std::vector<M3<RR>> gFv, gFx, gFw, gTw, gTx, gTv;
+  std::vector<M34<RR>> gFq, gTq;
+  // per-body sums, BoundaryModel_Akinci2012.cpp:453-497
+  M3<RR> nFv, nFx, nFw, nTw, nTx, nTv;
+  M34<RR> nFq, nTq;
+  // forward-mode rigid Jacobians, BoundaryModel_Akinci2012.cpp:94-106
+  M3<RR> Jvv, Jvw, Jwv, Jww, Jxv, Jxw;
+  M43<RR> Jqv, Jqw;
+  void reset_jacobians() {
+    Jvv.setIdentity(); Jww.setIdentity();
+    Jvw.setZero(); Jwv.setZero(); Jxv.setZero(); Jxw.setZero();
+    Jqv.setZero(); Jqw.setZero();
+  }
+};
+
+struct Stats {
+  int iterations = 0, iterationsV = 0;
+  double avgDensityErr = 0, avgDivErr = 0;
+};
+
+template <typename RF, typename RR> class Sim {
+ public:
+  Params P;
+  Cubic<RF> K;
+  RF h, dt, rho0, V_f, m_f;
+  V3<RF> g;
+  int nthreads = 1;
+  RF time = 0;
+  int stepCount = 0;
+  // fluid (reference AoS layout, FluidModel.h:110-151)
+  std::vector<V3<RF>> x, v, s;
+  std::vector<RF> rho, alpha, bAdv, kappa, kappaV;
+  std::vector<Body<RF, RR>> bodies;
+  // neighbour lists per point-set pair
+  std::vector<std::vector<uint32_t>> nFF;                // F -> F
+  std::vector<std::vector<std::vector<uint32_t>>> nFB;   // [body][i] F -> B
+  std::vector<std::vector<std::vector<uint32_t>>> nBF;   // [body][j] B -> F (dynamic/animated bodies)
+  Stats stats;
+
+  explicit Sim(const Params& p) : P(p) {
+    h = RF(4.0) * RF(P.radius);  // Simulation.cpp:195-216, support radius = 4r
+    K.set(h);
+    dt = RF(P.dt);
+    rho0 = RF(P.rho0);
+    const RF diam = RF(2.0) * RF(P.radius);
+    V_f = RF(0.8) * diam * diam * diam;  // FluidModel.cpp:255-275
+    m_f = V_f * rho0;
+    g = V3<RF>(RF(P.gravity[0]), RF(P.gravity[1]), RF(P.gravity[2]));
+#ifdef _OPENMP
+    nthreads = omp_get_max_threads();
+#endif
+  }
+
+  void set_fluid(size_t n, const double* xs, const double* vs) {
+    x.resize(n); v.resize(n); s.assign(n, V3<RF>());
+    rho.assign(n, 0); alpha.assign(n, 0); bAdv.assign(n, 0); kappa.assign(n, 0); kappaV.assign(n, 0);
+    for (size_t i = 0; i < n; ++i) {
+      x[i] = V3<RF>(RF(xs[3 * i]), RF(xs[3 * i + 1]), RF(xs[3 * i + 2]));
+      v[i] = vs ? V3<RF>(RF(vs[3 * i]), RF(vs[3 * i + 1]), RF(vs[3 * i + 2])) : V3<RF>();
+    }
+  }
+
+  // Dynamic3dRigidBody::determineMassProperties, Dynamic3dRigidBody.h:184-213:
+  // each sample carries (4/3) pi r^3 rho_body; inertia about the mesh origin.
+  int add_body(size_t n, const double* x0s, int dynamic, const double* xrb, const double* qwxyz, double density) {
+    Body<RF, RR> b;
+    b.dynamic = dynamic != 0;
+    b.x0.resize(n);
+    for (size_t j = 0; j < n; ++j) b.x0[j] = V3<RF>(RF(x0s[3 * j]), RF(x0s[3 * j + 1]), RF(x0s[3 * j + 2]));
+    b.x = b.x0; b.v.assign(n, V3<RF>()); b.V.assign(n, 0);
+    b.xrb = V3<RR>(RR(xrb[0]), RR(xrb[1]), RR(xrb[2]));
+    b.q = Quat<RR>(RR(qwxyz[0]), RR(qwxyz[1]), RR(qwxyz[2]), RR(qwxyz[3]));
+    b.x_init = b.xrb; b.q_init = b.q;
+    const RR r = RR(P.radius);
+    const RR volume = RR(4.0 / 3.0 * M_PI) * r * r * r;
+    const RR dm = volume * RR(density);
+    b.mass = 0; b.I0.setZero();
+    for (size_t j = 0; j < n; ++j) {
+      b.mass += dm;
+      const V3<RR> p(b.x0[j]);
+      b.I0 += (M3<RR>::Identity() * p.dot(p) - outer(p, p)) * dm;
+    }
+    b.invMass = RR(1) / b.mass;
+    update_inertia(b);
+    b.Fth.assign(nthreads, V3<RR>()); b.Tth.assign(nthreads, V3<RR>());
+    b.gFv.assign(n, M3<RR>()); b.gFx.assign(n, M3<RR>()); b.gFw.assign(n, M3<RR>());
+    b.gTw.assign(n, M3<RR>()); b.gTx.assign(n, M3<RR>()); b.gTv.assign(n, M3<RR>());
+    b.gFq.assign(n, M34<RR>()); b.gTq.assign(n, M34<RR>());
+    b.reset_jacobians();
+    bodies.push_back(b);
+    return int(bodies.size()) - 1;
+  }
+
+  // Sets the boundary transform, then the Akinci volumes
+  // (RigidBody3dBoundarySimulator::deferredInit, RigidBody3dBoundarySimulator.cpp:249-265).
+  void init() {
+    update_boundary_particles(true);
+    update_boundary_volume();
+    nFB.assign(bodies.size(), {});
+    nBF.assign(bodies.size(), {});
+  }
+
+  static void update_inertia(Body<RF, RR>& b) {  // Dynamic3dRigidBody.h:215-219
+    const M3<RR> R = b.q.toRotationMatrix();
+    b.I = R * b.I0 * R.transpose();
+    b.Iinv = inverse(b.I);
+  }
+
+  // SimulatorBase::updateBoundaryParticles, SimulatorBase.cpp:1827-1858.
+  void update_boundary_particles(bool force) {
+    for (auto& b : bodies) {
+      const bool moving = b.dynamic || b.animated;
+      if (!(moving || force)) continue;
+#pragma omp parallel for schedule(static)
+      for (long j = 0; j < long(b.x0.size()); ++j) {
+        const V3<RR> xr = b.q.rotate(V3<RR>(b.x0[j])) + b.xrb;
+        b.x[j] = V3<RF>(xr);
+        if (moving) b.v[j] = V3<RF>(b.wrb.cross(V3<RR>(b.x[j]) - b.xrb) + b.vrb);
+        else b.v[j] = V3<RF>();
+      }
+    }
+  }
+
+  // Simulation::updateBoundaryVolume, Simulation.cpp:831-902, with
+  // BoundaryModel_Akinci2012::computeBoundaryVolume, :257-284:
+  // V_b = 1 / (W0 + sum W) over all static samples for a static body, over the
+  // body's own samples for a dynamic one.
+  void update_boundary_volume() {
+    const RF h2 = h * h;
+    std::vector<V3<RF>> statics;
+    std::vector<int> sbody; std::vector<uint32_t> sidx;
+    for (size_t bi = 0; bi < bodies.size(); ++bi)
+      if (!bodies[bi].dynamic && !bodies[bi].animated)
+        for (size_t j = 0; j < bodies[bi].x.size(); ++j) { statics.push_back(bodies[bi].x[j]); sbody.push_back(int(bi)); sidx.push_back(uint32_t(j)); }
+    auto volumes = [&](const std::vector<V3<RF>>& pts, std::vector<RF>& out) {
+      Grid<RF> gr; gr.build(pts, h * RF(1.001));
+      out.assign(pts.size(), 0);
+#pragma omp parallel
+      {
+        std::vector<uint32_t> nb;
+#pragma omp for schedule(static)
+        for (long i = 0; i < long(pts.size()); ++i) {
+          gr.query(pts[i], pts, h2, i, nb);
+          RF delta = K.w0;
+          for (uint32_t k : nb) delta += K.W(pts[i] - pts[k]);
+          out[i] = RF(1.0) / delta;
+        }
+      }
+    };
+    std::vector<RF> vs;
+    volumes(statics, vs);
+    for (size_t k = 0; k < statics.size(); ++k) bodies[sbody[k]].V[sidx[k]] = vs[k];
+    for (auto& b : bodies)
+      if (b.dynamic || b.animated) { std::vector<RF> vb; volumes(b.x, vb); b.V = vb; }
+  }
+
+  // CompactNSearch::find_neighbors as wrapped by Simulation::performNeighborhoodSearch
+  // (Simulation.cpp:743-748); point-set pairs F->F, F->B and B->F as activated at
+  // Simulation.cpp:883-899.  B->F is kept for dynamic/animated bodies only (the
+  // gradient loop visits no other).
+  void neighborhood_search() {
+    const RF h2 = h * h;
+    const RF cell = h * RF(1.001);
+    const long N = long(x.size());
+    Grid<RF> gf; gf.build(x, cell);
+    nFF.assign(N, {});
+#pragma omp parallel
+    {
+      std::vector<uint32_t> nb;
+#pragma omp for schedule(static)
+      for (long i = 0; i < N; ++i) { gf.query(x[i], x, h2, i, nb); nFF[i] = nb; }
+    }
+    nFB.assign(bodies.size(), {});
+    nBF.assign(bodies.size(), {});
+    for (size_t bi = 0; bi < bodies.size(); ++bi) {
+      auto& b = bodies[bi];
+      Grid<RF> gb; gb.build(b.x, cell);
+      nFB[bi].assign(N, {});
+#pragma omp parallel
+      {
+        std::vector<uint32_t> nb;
+#pragma omp for schedule(static)
+        for (long i = 0; i < N; ++i) { gb.query(x[i], b.x, h2, -1, nb); nFB[bi][i] = nb; }
+      }
+      if (b.dynamic || b.animated) {
+        nBF[bi].assign(b.x.size(), {});
+#pragma omp parallel
+        {
+          std::vector<uint32_t> nb;
+#pragma omp for schedule(static)
+          for (long j = 0; j < long(b.x.size()); ++j) { gf.query(b.x[j], x, h2, -1, nb); nBF[bi][j] = nb; }
+        }
+      }
+    }
+  }
+
+  // TimeStepDiffDFSPH::beginStep, TimeStepDiffDFSPH.cpp:353-430 (passive-body branch).
+  void begin_step() {
+    stepCount++;
+    const RF uat = RF(P.uniformAccTime);
+    for (auto& b : bodies) {
+      if (!b.dynamic) continue;
+      if (time <= uat + dt) {
+        RF factor = 1;
+        if (uat > RF(1e-3)) {
+          factor = (time / uat) > RF(1) ? RF(1) : (time / uat);
+          b.animated = true;
+        }
+        b.vrb = b.v0 * RR(factor);
+        b.wrb = b.w0 * RR(factor);
+      } else {
+        b.animated = false;
+      }
+    }
+  }
+
+  // TimeStep::computeDensities, TimeStep.cpp:147-202 (scalar form).
+  void compute_densities() {
+    const long N = long(x.size());
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      RF d = V_f * K.w0;
+      for (uint32_t j : nFF[i]) d += V_f * K.W(x[i] - x[j]);
+      for (size_t bi = 0; bi < bodies.size(); ++bi)
+        for (uint32_t j : nFB[bi][i]) d += bodies[bi].V[j] * K.W(x[i] - bodies[bi].x[j]);
+      rho[i] = d * rho0;
+    }
+  }
+
+  // TimeStepDiffDFSPH::computeDFSPHFactor, TimeStepDiffDFSPH.cpp:883-962.
+  void compute_dfsph_factor() {
+    const long N = long(x.size());
+    const RF eps = RF(1e-5);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      RF sum_grad_p_k = 0;
+      V3<RF> grad_p_i;
+      for (uint32_t j : nFF[i]) {
+        const V3<RF> grad_p_j = K.gradW(x[i] - x[j]) * (-V_f);
+        sum_grad_p_k += grad_p_j.squaredNorm();
+        grad_p_i -= grad_p_j;
+      }
+      for (size_t bi = 0; bi < bodies.size(); ++bi)
+        for (uint32_t j : nFB[bi][i]) {
+          const V3<RF> grad_p_j = K.gradW(x[i] - bodies[bi].x[j]) * (-bodies[bi].V[j]);
+          grad_p_i -= grad_p_j;
+        }
+      s[i] = -grad_p_i;
+      sum_grad_p_k += grad_p_i.squaredNorm();
+      alpha[i] = (sum_grad_p_k > eps) ? -RF(1.0) / sum_grad_p_k : RF(0);
+    }
+  }
+
+  // TimeStepDiffDFSPH::computeDensityAdv, TimeStepDiffDFSPH.cpp:1926-1975.
+  void compute_density_adv(long i) {
+    RF delta = 0;
+    for (uint32_t j : nFF[i]) delta += V_f * (v[i] - v[j]).dot(K.gradW(x[i] - x[j]));
+    for (size_t bi = 0; bi < bodies.size(); ++bi)
+      for (uint32_t j : nFB[bi][i]) delta += bodies[bi].V[j] * (v[i] - bodies[bi].v[j]).dot(K.gradW(x[i] - bodies[bi].x[j]));
+    RF da = rho[i] / rho0 + dt * delta;
+    bAdv[i] = std::max(da, RF(1.0));
+  }
+
+  // TimeStepDiffDFSPH::computeDensityChange, TimeStepDiffDFSPH.cpp:1977-2041.
+  void compute_density_change(long i) {
+    RF da = 0;
+    unsigned nn = unsigned(nFF[i].size());
+    for (uint32_t j : nFF[i]) da += V_f * (v[i] - v[j]).dot(K.gradW(x[i] - x[j]));
+    for (size_t bi = 0; bi < bodies.size(); ++bi) {
+      nn += unsigned(nFB[bi][i].size());
+      for (uint32_t j : nFB[bi][i]) da += bodies[bi].V[j] * (v[i] - bodies[bi].v[j]).dot(K.gradW(x[i] - bodies[bi].x[j]));
+    }
+    da = std::max(da, RF(0.0));
+    if (nn < 20) da = 0;
+    bAdv[i] = da;
+  }
+
+  // BoundaryModel::addForce, BoundaryModel.h:46-58 (per-thread partials).
+  void add_force(Body<RF, RR>& b, const V3<RF>& pos, const V3<RF>& f) {
+    if (!b.dynamic) return;
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    b.Fth[tid] += V3<RR>(f);
+    b.Tth[tid] += (V3<RR>(pos) - b.xrb).cross(V3<RR>(f));
+  }
+
+  // The velocity update shared by the warm starts and the Jacobi iterations:
+  // fluid pairs with |kSum| > eps, boundary pairs when |k_i| > eps, with the
+  // Akinci reaction force on the boundary (TimeStepDiffDFSPH.cpp:1024-1055,
+  // :1123-1196, :1753-1784, :1847-1901).
+  void velocity_update(long i, RF ki, const std::vector<RF>& kfield) {
+    const RF eps = RF(1e-5);
+    V3<RF>& vi = v[i];
+    for (uint32_t j : nFF[i]) {
+      const RF kj = kfield[j];
+      const RF kSum = ki + kj;  // density0 ratio is 1 for a single fluid
+      if (std::fabs(kSum) > eps) {
+        const V3<RF> grad_p_j = K.gradW(x[i] - x[j]) * (-V_f);
+        vi -= grad_p_j * (dt * kSum);
+      }
+    }
+    if (std::fabs(ki) > eps) {
+      const RF invH = RF(1.0) / dt;
+      for (size_t bi = 0; bi < bodies.size(); ++bi) {
+        auto& b = bodies[bi];
+        for (uint32_t j : nFB[bi][i]) {
+          const V3<RF> grad_p_j = K.gradW(x[i] - b.x[j]) * (-b.V[j]);
+          const V3<RF> velChange = grad_p_j * (-dt * RF(1.0) * ki);
+          vi += velChange;
+          const V3<RF> force = velChange * (-m_f) * invH;
+          add_force(b, b.x[j], force);
+        }
+      }
+    }
+  }
+
+  // TimeStepDiffDFSPH::warmstartDivergenceSolve, TimeStepDiffDFSPH.cpp:1698-1788.
+  void warmstart_divergence_solve() {
+    const long N = long(x.size());
+    const RF invH = RF(1.0) / dt;
+#pragma omp parallel
+    {
+#pragma omp for schedule(static)
+      for (long i = 0; i < N; ++i) {
+        compute_density_change(i);
+        if (bAdv[i] > RF(0.0)) kappaV[i] = RF(0.5) * std::max(kappaV[i], RF(-0.5)) * invH;
+        else kappaV[i] = 0;
+      }
+#pragma omp for schedule(static)
+      for (long i = 0; i < N; ++i) velocity_update(i, kappaV[i], kappaV);
+    }
+  }
+
+  // TimeStepDiffDFSPH::warmstartPressureSolve, TimeStepDiffDFSPH.cpp:964-1059.
+  void warmstart_pressure_solve() {
+    const long N = long(x.size());
+    const RF invH2 = RF(1.0) / (dt * dt);
+#pragma omp parallel
+    {
+#pragma omp for schedule(static)
+      for (long i = 0; i < N; ++i) {
+        compute_density_adv(i);
+        if (bAdv[i] > RF(1.0)) kappa[i] = RF(0.5) * std::max(kappa[i], RF(-0.00025)) * invH2;
+        else kappa[i] = 0;
+      }
+#pragma omp for schedule(static)
+      for (long i = 0; i < N; ++i) velocity_update(i, kappa[i], kappa);
+    }
+  }
+
+  // TimeStepDiffDFSPH::pressureSolveIteration, TimeStepDiffDFSPH.cpp:1061-1220.
+  RF pressure_solve_iteration() {
+    const long N = long(x.size());
+    const RF h2 = dt * dt;
+    const RF invH2 = RF(1.0) / h2;
+    compute_rigid_body_gradient(0);
+    std::vector<RF> kfield(N);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) kfield[i] = (bAdv[i] - RF(1.0)) * alpha[i] * invH2;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      const RF ki = kfield[i];
+      if (P.usePressureWarmstart) kappa[i] += ki;
+      velocity_update(i, ki, kfield);
+    }
+    RF err_f = 0;
+#pragma omp parallel for reduction(+ : err_f) schedule(static)
+    for (long i = 0; i < N; ++i) {
+      compute_density_adv(i);
+      err_f += rho0 * bAdv[i] - rho0;
+    }
+    return err_f / RF(N);
+  }
+
+  // TimeStepDiffDFSPH::divergenceSolveIteration, TimeStepDiffDFSPH.cpp:1790-1922.
+  RF divergence_solve_iteration() {
+    const long N = long(x.size());
+    const RF invH = RF(1.0) / dt;
+    compute_rigid_body_gradient(1);
+    std::vector<RF> kfield(N);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) kfield[i] = bAdv[i] * alpha[i] * invH;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      const RF ki = kfield[i];
+      if (P.useDivergenceWarmstart) kappaV[i] += ki;
+      velocity_update(i, ki, kfield);
+    }
+    RF err_f = 0;
+#pragma omp parallel for reduction(+ : err_f) schedule(static)
+    for (long i = 0; i < N; ++i) {
+      compute_density_change(i);
+      err_f += rho0 * bAdv[i];
+    }
+    return err_f / RF(N);
+  }
+
+  // TimeStepDiffDFSPH::divergenceSolve, TimeStepDiffDFSPH.cpp:770-881.
+  void divergence_solve() {
+    const long N = long(x.size());
+    if (P.useDivergenceWarmstart) warmstart_divergence_solve();
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) compute_density_change(i);
+    int it = 0;
+    bool chk = false;
+    RF avg = 0;
+    const RF eta = (RF(1.0) / dt) * RF(P.maxErrorV) * RF(0.01) * rho0;
+    while ((!chk || it < 1) && it < P.maxIterationsV) {
+      avg = divergence_solve_iteration();
+      chk = avg <= eta;
+      it++;
+    }
+    stats.iterationsV = it;
+    stats.avgDivErr = avg;
+    if (P.useDivergenceWarmstart)
+      for (long i = 0; i < N; ++i) kappaV[i] *= dt;
+  }
+
+  // TimeStepDiffDFSPH::pressureSolve, TimeStepDiffDFSPH.cpp:654-768.
+  void pressure_solve() {
+    const long N = long(x.size());
+    if (P.usePressureWarmstart) warmstart_pressure_solve();
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) compute_density_adv(i);
+    int it = 0;
+    bool chk = false;
+    RF avg = 0;
+    const RF eta = RF(P.maxError) * RF(0.01) * rho0;
+    while ((!chk || it < P.minIterations) && it < P.maxIterations) {
+      avg = pressure_solve_iteration();
+      chk = avg <= eta;
+      it++;
+    }
+    stats.iterations = it;
+    stats.avgDensityErr = avg;
+    if (P.usePressureWarmstart) {
+      const RF h2 = dt * dt;
+      for (long i = 0; i < N; ++i) kappa[i] *= h2;
+    }
+  }
+
+  // --------------------------------------------------------------------------
+  // Reference-mode gradient: computeRigidBodyGradient + computeGradient
+  // (TimeStepDiffDFSPH.cpp:1226-1482) and helpers (:1485-1694).
+  // --------------------------------------------------------------------------
+  V3<RF> grad_b_to_fluid_v(long i, int mode) const {  // :1632-1694
+    const RF bi = bAdv[i];
+    if (mode == 0 ? bi <= RF(1.0) : bi <= RF(0.0)) return V3<RF>();
+    V3<RF> gsum;
+    for (uint32_t j : nFF[i]) gsum += K.gradW(x[i] - x[j]) * V_f;
+    for (size_t bi2 = 0; bi2 < bodies.size(); ++bi2)
+      for (uint32_t j : nFB[bi2][i]) gsum += K.gradW(x[i] - bodies[bi2].x[j]) * bodies[bi2].V[j];
+    if (mode == 0) gsum *= dt;
+    return gsum;
+  }
+
+  void compute_gradient(Body<RF, RR>& b, long i, uint32_t j, RF b_i, RF ki, const V3<RF>& force, int mode) {
+    const RF invH = RF(1.0) / dt;
+    const RF invH2 = RF(1.0) / dt / dt;
+    const V3<RF>& xi = x[i];
+    const V3<RF>& xj = b.x[j];
+    const RF Vj = b.V[j];
+    const V3<RF> grad_p_j = K.gradW(xi - xj) * (-Vj);
+    const M3<RF> H = K.gradGradW(xi - xj) * Vj;  // grad2_p_j_to_xj
+    const RF factor = alpha[i];
+    const V3<RF> vij = v[i] - b.v[j];
+    // :1518-1577
+    V3<RF> grad_b_xj;
+    if (mode == 0) {
+      if (bAdv[i] > RF(1.0)) grad_b_xj = K.gradW(xi - xj) * (-Vj) / rho0 - (K.gradGradW(xi - xj) * vij) * (dt * Vj);
+    } else {
+      if (bAdv[i] > RF(0.0)) grad_b_xj = -((K.gradGradW(xi - xj) * vij) * Vj);
+    }
+    // :1485-1516
+    V3<RF> grad_factor_xj;
+    if (factor < RF(0.)) grad_factor_xj = (H * s[i]) * (RF(2.) * factor * factor);
+    const RF coeff = (mode == 0) ? invH2 : invH;
+    const V3<RF> grad_ki_xj = (grad_b_xj * factor + grad_factor_xj * b_i) * coeff;
+    const M3<RF> dvc_dxj = -(outer(grad_p_j, grad_ki_xj) + H * ki);
+    // :1579-1628
+    V3<RF> grad_b_vj;
+    if (mode == 0) {
+      if (bAdv[i] > RF(1.0)) grad_b_vj = K.gradW(xi - xj) * (-dt * Vj);
+    } else {
+      if (bAdv[i] > RF(0.0)) grad_b_vj = K.gradW(xi - xj) * (-Vj);
+    }
+    const V3<RF> grad_ki_vj = grad_b_vj * factor * coeff;
+    const M3<RF> dvc_dvj = -outer(grad_p_j, grad_ki_vj);
+    const V3<RF> grad_b_vi = grad_b_to_fluid_v(i, mode);
+    const V3<RF> grad_ki_vi = grad_b_vi * factor * coeff;
+    const M3<RF> dvc_dvi = -outer(grad_p_j, grad_ki_vi);
+    const M3<RF> gFx = dvc_dxj * (-m_f);
+    const M3<RF> gFv = inverse(M3<RF>::Identity() - dvc_dvi * dt) * (dvc_dvj * (-m_f));
+    b.gFx[j] += M3<RR>(gFx);
+    b.gFv[j] += M3<RR>(gFv);
+    if (P.optimizeRotation) {  // :1425-1480
+      const V3<RF> rj = V3<RF>(V3<RR>(xj) - b.xrb);
+      const Quat<RF> q(b.q);
+      const M34<RF> Jq = grad_Rqp_to_q(q, b.x0[j]);
+      const V3<RF> om(b.wrb);
+      const M34<RF> gFq = gFx * Jq + gFv * skew(om) * Jq;
+      const M3<RF> gFw = gFv * skew(rj).transpose();
+      const M34<RF> gTq = skew(rj) * gFq + skew(force).transpose() * Jq;
+      const M3<RF> gTw = skew(rj) * gFw;
+      b.gFw[j] += M3<RR>(gFw);
+      b.gFq[j] += M34<RR>(gFq);
+      b.gTw[j] += M3<RR>(gTw);
+      b.gTq[j] += M34<RR>(gTq);
+      b.gTv[j] += M3<RR>(skew(rj) * gFv);
+      b.gTx[j] += M3<RR>(skew(rj) * gFx);
+    }
+  }
+
+  void compute_rigid_body_gradient(int mode) {  // :1226-1279
+    if (!P.computeGradient) return;
+    const RF invH = RF(1.0) / dt;
+    const RF invH2 = RF(1.0) / dt / dt;
+    for (size_t bi = 0; bi < bodies.size(); ++bi) {
+      auto& b = bodies[bi];
+      if (!(b.dynamic && !b.animated)) continue;
+#pragma omp parallel for schedule(static)
+      for (long j = 0; j < long(b.x.size()); ++j) {
+        for (uint32_t i : nBF[bi][j]) {
+          const RF b_i = (mode == 0) ? bAdv[i] - RF(1.0) : bAdv[i];
+          // quirk: the coefficient is truncated to an unsigned int (:1266)
+          const unsigned int coeff = (mode == 0) ? (unsigned int)invH2 : (unsigned int)invH;
+          const RF ki = b_i * alpha[i] * RF(coeff);
+          const V3<RF> grad_p_j = K.gradW(x[i] - b.x[j]) * (-b.V[j]);
+          const V3<RF> velChange = grad_p_j * (-dt * RF(1.0) * ki);
+          const V3<RF> force = velChange * (-m_f) * invH;
+          compute_gradient(b, i, uint32_t(j), b_i, ki, force, mode);
+        }
+      }
+    }
+  }
+
+  // BoundaryModel_Akinci2012::accumulate_and_reset_gradient, :453-497.
+  void accumulate_and_reset_gradient(Body<RF, RR>& b) {
+    b.nFv.setZero(); b.nFx.setZero(); b.nFw.setZero(); b.nFq.setZero();
+    b.nTw.setZero(); b.nTq.setZero(); b.nTv.setZero(); b.nTx.setZero();
+    for (size_t j = 0; j < b.x.size(); ++j) {
+      b.nFv += b.gFv[j]; b.nFx += b.gFx[j]; b.nFw += b.gFw[j]; b.nFq += b.gFq[j];
+      b.nTw += b.gTw[j]; b.nTq += b.gTq[j]; b.nTv += b.gTv[j]; b.nTx += b.gTx[j];
+      b.gFv[j].setZero(); b.gFx[j].setZero(); b.gFw[j].setZero(); b.gFq[j].setZero();
+      b.gTw[j].setZero(); b.gTv[j].setZero(); b.gTx[j].setZero(); b.gTq[j].setZero();
+    }
+  }
+
+  void force_and_torque(const Body<RF, RR>& b, V3<RR>& F, V3<RR>& T) const {  // BoundaryModel.cpp:38-53
+    F = V3<RR>(); T = V3<RR>();
+    for (int t = 0; t < nthreads; ++t) { F += b.Fth[t]; T += b.Tth[t]; }
+  }
+
+  // compute_grad_inertia_v_to_{omega0,v0}, BoundaryModel_Akinci2012.cpp:524-541.
+  static M3<RR> grad_inertia_v(const Body<RF, RR>& b, const V3<RR>& vv, const M43<RR>& Jq) {
+    const M3<RR> R = b.q.toRotationMatrix();
+    const M3<RR> a = grad_Rqp_to_q(b.q, b.I0 * (R.transpose() * vv)) * Jq;
+    const M3<RR> c = R * b.I0 * (grad_RqTp_to_q(b.q, vv) * Jq);
+    return a + c;
+  }
+
+  // BoundaryModel_Akinci2012::perform_chain_rule, :543-889.
+  void perform_chain_rule(Body<RF, RR>& b, RR dtR) {
+    const int mode = P.gradientMode;
+    const RR invMass = RR(1.0) / (b.mass + RR(1e-10));
+    const M3<RR> I = b.I, Iinv = b.Iinv;
+    const V3<RR> om = b.wrb;
+    V3<RR> temp_v;
+    if (mode == 0 || mode == 1) {
+      M3<RR> dF_v0, dF_w0;
+      if (mode == 0) {
+        dF_v0 = b.nFx * b.Jxv + b.nFv * b.Jvv + b.nFq * b.Jqv + b.nFw * b.Jwv;
+        dF_w0 = b.nFx * b.Jxw + b.nFv * b.Jvw + b.nFq * b.Jqw + b.nFw * b.Jww;
+      } else {
+        dF_v0 = b.nFv * b.Jvv + b.nFw * b.Jwv;
+        dF_w0 = b.nFv * b.Jvw + b.nFw * b.Jww;
+      }
+      b.Jvv = b.Jvv + dF_v0 * (dtR * invMass);
+      b.Jvw = b.Jvw + dF_w0 * (dtR * invMass);
+      b.Jxv = b.Jxv + b.Jvv * dtR;
+      b.Jxw = b.Jxw + b.Jvw * dtR;
+      V3<RR> F, T;
+      force_and_torque(b, F, T);
+      M3<RR> dT_w0, dT_v0;
+      if (mode == 0) {
+        dT_w0 = b.nTx * b.Jxw + b.nTv * b.Jvw + b.nTq * b.Jqw + b.nTw * b.Jww;
+        dT_v0 = b.nTx * b.Jxv + b.nTv * b.Jvv + b.nTq * b.Jqv + b.nTw * b.Jwv;
+      } else {
+        dT_w0 = b.nTv * b.Jvw + b.nTw * b.Jww;
+        dT_v0 = b.nTv * b.Jvv + b.nTw * b.Jwv;
+      }
+      M3<RR> dTau_w0, dTau_v0;
+      if (P.gyroscopic) {
+        temp_v = (I * om).cross(om) + T;
+        const M3<RR> dLw_w0 = skew(I * om) * b.Jww + skew(om).transpose() * (I * b.Jww + grad_inertia_v(b, om, b.Jqw));
+        const M3<RR> dLw_v0 = skew(I * om) * b.Jwv + skew(om).transpose() * (I * b.Jwv + grad_inertia_v(b, om, b.Jqv));
+        dTau_w0 = dLw_w0 + dT_w0;
+        dTau_v0 = dLw_v0 + dT_v0;
+      } else {
+        temp_v = T;
+        dTau_w0 = dT_w0;
+        dTau_v0 = dT_v0;
+      }
+      // quirk: "+" on the d(I^-1) term (:641-647)
+      const M3<RR> gIi_w0 = Iinv * grad_inertia_v(b, Iinv * temp_v, b.Jqw);
+      const M3<RR> gIi_v0 = Iinv * grad_inertia_v(b, Iinv * temp_v, b.Jqv);
+      b.Jww = b.Jww + (gIi_w0 + Iinv * dTau_w0) * dtR;
+      b.Jwv = b.Jwv + (gIi_v0 + Iinv * dTau_v0) * dtR;
+    } else {  // RigidGradOnly, :674-709
+      b.Jxv = b.Jxv + b.Jvv * dtR;
+      b.Jxw = b.Jxw + b.Jvw * dtR;
+      if (P.gyroscopic) {
+        temp_v = (I * om).cross(om);
+        const M3<RR> gIi_w0 = Iinv * grad_inertia_v(b, Iinv * temp_v, b.Jqw);
+        const M3<RR> gIi_v0 = Iinv * grad_inertia_v(b, Iinv * temp_v, b.Jqv);
+        const M3<RR> dLw_w0 = skew(I * om) * b.Jww + skew(om).transpose() * (I * b.Jww + grad_inertia_v(b, om, b.Jqw));
+        const M3<RR> dLw_v0 = skew(I * om) * b.Jwv + skew(om).transpose() * (I * b.Jwv + grad_inertia_v(b, om, b.Jqv));
+        b.Jww = b.Jww + (gIi_w0 + Iinv * dLw_w0) * dtR;
+        b.Jwv = b.Jwv + (gIi_v0 + Iinv * dLw_v0) * dtR;
+      }
+      // temp_v stays zero without the gyroscopic term
+    }
+    if (P.optimizeRotation) {  // :827-871
+      const Quat<RR> q = b.q;
+      const V3<RR> nw = om + (Iinv * temp_v) * dtR;
+      const Quat<RR> p(0, nw.x, nw.y, nw.z);
+      Quat<RR> nq = q;
+      const Quat<RR> pq = p * nq;
+      const RR hh = dtR * RR(0.5);
+      nq.x += hh * pq.x; nq.y += hh * pq.y; nq.z += hh * pq.z; nq.w += hh * pq.w;
+      const Quat<RR> nqn = nq.normalized();
+      const RR qv[4] = {nqn.w, nqn.x, nqn.y, nqn.z};
+      M4<RR> Nq;
+      const RR nn = nq.norm();
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) Nq(r, c) = ((r == c ? RR(1) : RR(0)) - qv[r] * qv[c]) / nn;
+      const M4<RR> Pq = grad_p_q_product_to_q(p);
+      const M43<RR> Pw = grad_omega_q_product_to_omega(q);
+      b.Jqw = Nq * (b.Jqw + (Pw * b.Jww + Pq * b.Jqw) * (dtR / RR(2.)));
+      b.Jqv = Nq * (b.Jqv + (Pw * b.Jwv + Pq * b.Jqv) * (dtR / RR(2.)));
+    }
+  }
+
+  // TimeStepDiffDFSPH::backwardPerStep, :493-524.
+  void backward_per_step() {
+    if (!P.computeGradient) return;
+    for (auto& b : bodies)
+      if (b.dynamic && !b.animated) {
+        accumulate_and_reset_gradient(b);
+        perform_chain_rule(b, RR(dt));
+      }
+  }
+
+  // RigidBody3dBoundarySimulator::velocityTimeStep (:278-320) with
+  // BoundarySimulator::updateBoundaryForces (BoundarySimulator.cpp:10-36) and
+  // Dynamic3dRigidBody::addForce/addTorque/addGravity (Dynamic3dRigidBody.h:113-142, 228-233).
+  void rigid_velocity_step() {
+    const RR dtR = RR(dt);
+    const V3<RR> gR(RR(P.gravity[0]), RR(P.gravity[1]), RR(P.gravity[2]));
+    for (auto& b : bodies) {
+      if (!b.dynamic) continue;
+      V3<RR> F, T;
+      force_and_torque(b, F, T);
+      if (!b.animated) {
+        b.vrb += F * (b.invMass * dtR);
+        if (P.gyroscopic) {
+          const V3<RR> L = b.I * b.wrb;
+          b.wrb += (b.Iinv * (L.cross(b.wrb) + T)) * dtR;
+        } else {
+          b.wrb += (b.Iinv * T) * dtR;
+        }
+      }
+      b.F_last = F; b.T_last = T;
+      for (int t = 0; t < nthreads; ++t) { b.Fth[t] = V3<RR>(); b.Tth[t] = V3<RR>(); }
+      if (!b.animated) b.vrb += (gR * b.mass) * (b.invMass * dtR);
+    }
+  }
+
+  // RigidBody3dBoundarySimulator::positionTimeStep (:322-344) with
+  // Dynamic3dRigidBody::animate (Dynamic3dRigidBody.h:143-154).
+  void rigid_position_step() {
+    const RR dtR = RR(dt);
+    for (auto& b : bodies) {
+      if (!b.dynamic) continue;
+      b.xrb += b.vrb * dtR;
+      const Quat<RR> wq(0, b.wrb.x, b.wrb.y, b.wrb.z);
+      const Quat<RR> d = wq * b.q;
+      const RR hh = dtR * RR(0.5);
+      b.q.x += hh * d.x; b.q.y += hh * d.y; b.q.z += hh * d.z; b.q.w += hh * d.w;
+      b.q.normalize();
+      update_inertia(b);
+    }
+    update_boundary_particles(false);
+  }
+
+  // One full step in SimulatorBase::timeStepNoGUI order (SimulatorBase.cpp:1142-1200)
+  // around TimeStepDiffDFSPH::step (TimeStepDiffDFSPH.cpp:526-652).
+  void step() {
+    begin_step();
+    neighborhood_search();
+    compute_densities();
+    compute_dfsph_factor();
+    if (P.enableDivergenceSolver) divergence_solve();
+    else stats.iterationsV = 0;
+    const long N = long(x.size());
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) v[i] += g * dt;  // clearAccelerations + advect (:588-605)
+    pressure_solve();
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) x[i] += v[i] * dt;  // :613-634
+    time = time + dt;
+    backward_per_step();
+    rigid_velocity_step();
+    rigid_position_step();
+  }
+
+  // Reset of the rigid bodies and gradients between trajectories
+  // (Dynamic3dRigidBody::reset, BoundaryModel_Akinci2012::reset).
+  void reset_bodies() {
+    for (auto& b : bodies) {
+      b.xrb = b.x_init; b.q = b.q_init; b.vrb = V3<RR>(); b.wrb = V3<RR>();
+      update_inertia(b);
+      b.reset_jacobians();
+    }
+    update_boundary_particles(true);
+    time = 0; stepCount = 0;
+  }
+};
+
+}  // namespace orc

@@ -1,0 +1,236 @@
+// ORACLE C entry points (test infrastructure: loaded by tests/, smoke() and
+// bench.py's cpu_baseline leg through ctypes, never by the product).
+//
+// Precision variants: 0 = <float,float> (reference precision),
+//                     1 = <double,double> (FP64 oracle),
+//                     2 = <float,double> (FP32 bulk, FP64 rigid).
+#include <cstring>
+#include <memory>
+#include <string>
+#include "dfsph_oracle.hpp"
+
+using namespace orc;
+
+struct OracleBase {
+  virtual ~OracleBase() = default;
+  virtual int add_body(size_t n, const double* x0, int dyn, const double* xrb, const double* q, double dens) = 0;
+  virtual void set_fluid(size_t n, const double* x, const double* v) = 0;
+  virtual void set_init_velocity(int body, const double* v0, const double* w0) = 0;
+  virtual void init() = 0;
+  virtual void step() = 0;
+  virtual void reset_bodies() = 0;
+  virtual size_t n_fluid() const = 0;
+  virtual size_t n_body_particles(int body) const = 0;
+  virtual int get_fluid(double* x, double* v) const = 0;
+  virtual int get_field(const char* name, double* out) const = 0;
+  virtual int get_body_field(int body, const char* name, double* out) const = 0;
+  virtual int get_rigid(int body, double* out) const = 0;
+  virtual int get_jacobians(int body, double* out) const = 0;
+  virtual int get_net_gradients(int body, double* out) const = 0;
+  virtual int get_mass_props(int body, double* out) const = 0;
+  virtual void get_stats(int* it, int* itv, double* err, double* errv) const = 0;
+  virtual int neighbor_counts(int kind, int body, unsigned* out) const = 0;
+  virtual int neighbor_list(int kind, int body, size_t i, unsigned* out, int cap) const = 0;
+  virtual int set_state(const char* name, const double* in) = 0;
+  virtual void stage(int which) = 0;
+};
+
+template <typename RF, typename RR> struct Oracle : OracleBase {
+  Sim<RF, RR> sim;
+  explicit Oracle(const Params& p) : sim(p) {}
+  int add_body(size_t n, const double* x0, int dyn, const double* xrb, const double* q, double dens) override {
+    return sim.add_body(n, x0, dyn, xrb, q, dens);
+  }
+  void set_fluid(size_t n, const double* x, const double* v) override { sim.set_fluid(n, x, v); }
+  void set_init_velocity(int body, const double* v0, const double* w0) override {
+    auto& b = sim.bodies[body];
+    b.v0 = V3<RR>(RR(v0[0]), RR(v0[1]), RR(v0[2]));
+    b.w0 = V3<RR>(RR(w0[0]), RR(w0[1]), RR(w0[2]));
+  }
+  void init() override { sim.init(); }
+  void step() override { sim.step(); }
+  void reset_bodies() override { sim.reset_bodies(); }
+  size_t n_fluid() const override { return sim.x.size(); }
+  size_t n_body_particles(int body) const override { return sim.bodies[body].x.size(); }
+  int get_fluid(double* x, double* v) const override {
+    for (size_t i = 0; i < sim.x.size(); ++i)
+      for (int k = 0; k < 3; ++k) {
+        if (x) x[3 * i + k] = double(sim.x[i][k]);
+        if (v) v[3 * i + k] = double(sim.v[i][k]);
+      }
+    return 0;
+  }
+  int get_field(const char* name, double* out) const override {
+    const std::string n(name);
+    const size_t N = sim.x.size();
+    const std::vector<RF>* f = nullptr;
+    if (n == "density") f = &sim.rho;
+    else if (n == "factor") f = &sim.alpha;
+    else if (n == "density_adv") f = &sim.bAdv;
+    else if (n == "kappa") f = &sim.kappa;
+    else if (n == "kappa_v") f = &sim.kappaV;
+    if (f) { for (size_t i = 0; i < N; ++i) out[i] = double((*f)[i]); return 0; }
+    if (n == "sum_grad_p_k") {
+      for (size_t i = 0; i < N; ++i) for (int k = 0; k < 3; ++k) out[3 * i + k] = double(sim.s[i][k]);
+      return 0;
+    }
+    return -1;
+  }
+  int get_body_field(int body, const char* name, double* out) const override {
+    const std::string n(name);
+    const auto& b = sim.bodies[body];
+    if (n == "volume") { for (size_t j = 0; j < b.V.size(); ++j) out[j] = double(b.V[j]); return 0; }
+    if (n == "x") { for (size_t j = 0; j < b.x.size(); ++j) for (int k = 0; k < 3; ++k) out[3 * j + k] = double(b.x[j][k]); return 0; }
+    if (n == "v") { for (size_t j = 0; j < b.v.size(); ++j) for (int k = 0; k < 3; ++k) out[3 * j + k] = double(b.v[j][k]); return 0; }
+    return -1;
+  }
+  // x(3) q(w,x,y,z)(4) v(3) omega(3) F_last(3) T_last(3) = 19
+  int get_rigid(int body, double* out) const override {
+    const auto& b = sim.bodies[body];
+    for (int k = 0; k < 3; ++k) out[k] = double(b.xrb[k]);
+    out[3] = double(b.q.w); out[4] = double(b.q.x); out[5] = double(b.q.y); out[6] = double(b.q.z);
+    for (int k = 0; k < 3; ++k) { out[7 + k] = double(b.vrb[k]); out[10 + k] = double(b.wrb[k]); }
+    for (int k = 0; k < 3; ++k) { out[13 + k] = double(b.F_last[k]); out[16 + k] = double(b.T_last[k]); }
+    return 0;
+  }
+  // Jxv Jxw Jvv Jvw Jwv Jww (row-major 3x3 each) then Jqv Jqw (4x3 each) = 78
+  int get_jacobians(int body, double* out) const override {
+    const auto& b = sim.bodies[body];
+    const M3<RR>* m3[6] = {&b.Jxv, &b.Jxw, &b.Jvv, &b.Jvw, &b.Jwv, &b.Jww};
+    int o = 0;
+    for (auto m : m3) for (int i = 0; i < 9; ++i) out[o++] = double(m->a[i]);
+    for (int i = 0; i < 12; ++i) out[o++] = double(b.Jqv.a[i]);
+    for (int i = 0; i < 12; ++i) out[o++] = double(b.Jqw.a[i]);
+    return 0;
+  }
+  // Fv Fx Fw Tw Tx Tv (3x3) then Fq Tq (3x4) = 78 (last step's per-body sums)
+  int get_net_gradients(int body, double* out) const override {
+    const auto& b = sim.bodies[body];
+    const M3<RR>* m3[6] = {&b.nFv, &b.nFx, &b.nFw, &b.nTw, &b.nTx, &b.nTv};
+    int o = 0;
+    for (auto m : m3) for (int i = 0; i < 9; ++i) out[o++] = double(m->a[i]);
+    for (int i = 0; i < 12; ++i) out[o++] = double(b.nFq.a[i]);
+    for (int i = 0; i < 12; ++i) out[o++] = double(b.nTq.a[i]);
+    return 0;
+  }
+  // mass, I0 (9), I (9), Iinv (9) = 28
+  int get_mass_props(int body, double* out) const override {
+    const auto& b = sim.bodies[body];
+    out[0] = double(b.mass);
+    for (int i = 0; i < 9; ++i) { out[1 + i] = double(b.I0.a[i]); out[10 + i] = double(b.I.a[i]); out[19 + i] = double(b.Iinv.a[i]); }
+    return 0;
+  }
+  void get_stats(int* it, int* itv, double* err, double* errv) const override {
+    *it = sim.stats.iterations; *itv = sim.stats.iterationsV;
+    *err = sim.stats.avgDensityErr; *errv = sim.stats.avgDivErr;
+  }
+  const std::vector<std::vector<uint32_t>>* lists(int kind, int body) const {
+    if (kind == 0) return &sim.nFF;
+    if (kind == 1) return body < int(sim.nFB.size()) ? &sim.nFB[body] : nullptr;
+    if (kind == 2) return body < int(sim.nBF.size()) ? &sim.nBF[body] : nullptr;
+    return nullptr;
+  }
+  int neighbor_counts(int kind, int body, unsigned* out) const override {
+    auto L = lists(kind, body);
+    if (!L) return -1;
+    for (size_t i = 0; i < L->size(); ++i) out[i] = unsigned((*L)[i].size());
+    return int(L->size());
+  }
+  int neighbor_list(int kind, int body, size_t i, unsigned* out, int cap) const override {
+    auto L = lists(kind, body);
+    if (!L || i >= L->size()) return -1;
+    const auto& l = (*L)[i];
+    const int n = int(l.size());
+    for (int k = 0; k < n && k < cap; ++k) out[k] = l[k];
+    return n;
+  }
+  int set_state(const char* name, const double* in) override {
+    const std::string n(name);
+    const size_t N = sim.x.size();
+    if (n == "kappa") { for (size_t i = 0; i < N; ++i) sim.kappa[i] = RF(in[i]); return 0; }
+    if (n == "kappa_v") { for (size_t i = 0; i < N; ++i) sim.kappaV[i] = RF(in[i]); return 0; }
+    if (n == "time") { sim.time = RF(in[0]); return 0; }
+    return -1;
+  }
+  // Partial steps for per-kernel parity: 0 = neighbour search + density + factor.
+  void stage(int which) override {
+    if (which == 0) {
+      sim.neighborhood_search();
+      sim.compute_densities();
+      sim.compute_dfsph_factor();
+    }
+  }
+};
+
+static thread_local std::string g_err;
+
+extern "C" {
+
+void* orc_create(int variant, const Params* p) {
+  try {
+    switch (variant) {
+      case 0: return static_cast<OracleBase*>(new Oracle<float, float>(*p));
+      case 1: return static_cast<OracleBase*>(new Oracle<double, double>(*p));
+      case 2: return static_cast<OracleBase*>(new Oracle<float, double>(*p));
+    }
+  } catch (const std::exception& e) { g_err = e.what(); }
+  return nullptr;
+}
+void orc_destroy(void* h) { delete static_cast<OracleBase*>(h); }
+int orc_params_size() { return int(sizeof(Params)); }
+int orc_set_fluid(void* h, size_t n, const double* x, const double* v) { static_cast<OracleBase*>(h)->set_fluid(n, x, v); return 0; }
+int orc_add_body(void* h, size_t n, const double* x0, int dyn, const double* xrb, const double* q, double dens) {
+  return static_cast<OracleBase*>(h)->add_body(n, x0, dyn, xrb, q, dens);
+}
+int orc_set_init_velocity(void* h, int body, const double* v0, const double* w0) { static_cast<OracleBase*>(h)->set_init_velocity(body, v0, w0); return 0; }
+int orc_init(void* h) { static_cast<OracleBase*>(h)->init(); return 0; }
+int orc_step(void* h, int nsteps) { for (int s = 0; s < nsteps; ++s) static_cast<OracleBase*>(h)->step(); return 0; }
+int orc_reset_bodies(void* h) { static_cast<OracleBase*>(h)->reset_bodies(); return 0; }
+int orc_stage(void* h, int which) { static_cast<OracleBase*>(h)->stage(which); return 0; }
+size_t orc_n_fluid(void* h) { return static_cast<OracleBase*>(h)->n_fluid(); }
+size_t orc_n_body_particles(void* h, int body) { return static_cast<OracleBase*>(h)->n_body_particles(body); }
+int orc_get_fluid(void* h, double* x, double* v) { return static_cast<OracleBase*>(h)->get_fluid(x, v); }
+int orc_get_field(void* h, const char* name, double* out) { return static_cast<OracleBase*>(h)->get_field(name, out); }
+int orc_get_body_field(void* h, int body, const char* name, double* out) { return static_cast<OracleBase*>(h)->get_body_field(body, name, out); }
+int orc_get_rigid(void* h, int body, double* out) { return static_cast<OracleBase*>(h)->get_rigid(body, out); }
+int orc_get_jacobians(void* h, int body, double* out) { return static_cast<OracleBase*>(h)->get_jacobians(body, out); }
+int orc_get_net_gradients(void* h, int body, double* out) { return static_cast<OracleBase*>(h)->get_net_gradients(body, out); }
+int orc_get_mass_props(void* h, int body, double* out) { return static_cast<OracleBase*>(h)->get_mass_props(body, out); }
+int orc_get_stats(void* h, int* it, int* itv, double* err, double* errv) { static_cast<OracleBase*>(h)->get_stats(it, itv, err, errv); return 0; }
+int orc_neighbor_counts(void* h, int kind, int body, unsigned* out) { return static_cast<OracleBase*>(h)->neighbor_counts(kind, body, out); }
+int orc_neighbor_list(void* h, int kind, int body, size_t i, unsigned* out, int cap) { return static_cast<OracleBase*>(h)->neighbor_list(kind, body, i, out, cap); }
+int orc_set_state(void* h, const char* name, const double* in) { return static_cast<OracleBase*>(h)->set_state(name, in); }
+
+// Closed-form probes of the cubic kernel for the kernel tests.
+void orc_kernel_eval(int variant_double, double radius, const double* r, int n, double* W, double* gW, double* ggW) {
+  if (variant_double) {
+    Cubic<double> K; K.set(radius);
+    for (int i = 0; i < n; ++i) {
+      V3<double> v(r[3 * i], r[3 * i + 1], r[3 * i + 2]);
+      W[i] = K.W(v);
+      auto g = K.gradW(v); for (int k = 0; k < 3; ++k) gW[3 * i + k] = g[k];
+      auto H = K.gradGradW(v); for (int k = 0; k < 9; ++k) ggW[9 * i + k] = H.a[k];
+    }
+  } else {
+    Cubic<float> K; K.set(float(radius));
+    for (int i = 0; i < n; ++i) {
+      V3<float> v(float(r[3 * i]), float(r[3 * i + 1]), float(r[3 * i + 2]));
+      W[i] = K.W(v);
+      auto g = K.gradW(v); for (int k = 0; k < 3; ++k) gW[3 * i + k] = g[k];
+      auto H = K.gradGradW(v); for (int k = 0; k < 9; ++k) ggW[9 * i + k] = H.a[k];
+    }
+  }
+}
+
+// GradientUtils probes (double) for the finite-difference tests.
+void orc_grad_utils(const double* q, const double* p, double* Rqp, double* RqTp, double* PQ, double* WQ) {
+  Quat<double> qq(q[0], q[1], q[2], q[3]);
+  V3<double> pp(p[0], p[1], p[2]);
+  auto a = grad_Rqp_to_q(qq, pp); for (int i = 0; i < 12; ++i) Rqp[i] = a.a[i];
+  auto b = grad_RqTp_to_q(qq, pp); for (int i = 0; i < 12; ++i) RqTp[i] = b.a[i];
+  auto c = grad_p_q_product_to_q(qq); for (int i = 0; i < 16; ++i) PQ[i] = c.a[i];
+  auto d = grad_omega_q_product_to_omega(qq); for (int i = 0; i < 12; ++i) WQ[i] = d.a[i];
+}
+
+const char* orc_last_error() { return g_err.c_str(); }
+}

@@ -1,0 +1,219 @@
+"""ctypes binding of the CPU oracle (test infrastructure).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+module; it is the checker, never the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+VARIANTS = {"f32": 0, "f64": 1, "f32_rigid64": 2}
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("radius", C.c_double), ("dt", C.c_double), ("rho0", C.c_double),
+        ("gravity", C.c_double * 3), ("maxError", C.c_double), ("maxErrorV", C.c_double),
+        ("minIterations", C.c_int), ("maxIterations", C.c_int), ("maxIterationsV", C.c_int),
+        ("usePressureWarmstart", C.c_int), ("useDivergenceWarmstart", C.c_int),
+        ("enableDivergenceSolver", C.c_int), ("gradientMode", C.c_int),
+        ("optimizeRotation", C.c_int), ("gyroscopic", C.c_int), ("computeGradient", C.c_int),
+        ("uniformAccTime", C.c_double), ("targetTime", C.c_double),
+    ]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        L.orc_create.restype = P
+        L.orc_create.argtypes = [C.c_int, C.POINTER(Params)]
+        L.orc_destroy.argtypes = [P]
+        L.orc_params_size.restype = C.c_int
+        L.orc_set_fluid.argtypes = [P, C.c_size_t, dp, dp]
+        L.orc_add_body.argtypes = [P, C.c_size_t, dp, C.c_int, dp, dp, C.c_double]
+        L.orc_set_init_velocity.argtypes = [P, C.c_int, dp, dp]
+        L.orc_init.argtypes = [P]
+        L.orc_step.argtypes = [P, C.c_int]
+        L.orc_reset_bodies.argtypes = [P]
+        L.orc_stage.argtypes = [P, C.c_int]
+        L.orc_n_fluid.restype = C.c_size_t
+        L.orc_n_fluid.argtypes = [P]
+        L.orc_n_body_particles.restype = C.c_size_t
+        L.orc_n_body_particles.argtypes = [P, C.c_int]
+        L.orc_get_fluid.argtypes = [P, dp, dp]
+        L.orc_get_field.argtypes = [P, C.c_char_p, dp]
+        L.orc_get_body_field.argtypes = [P, C.c_int, C.c_char_p, dp]
+        for f in ("orc_get_rigid", "orc_get_jacobians", "orc_get_net_gradients", "orc_get_mass_props"):
+            getattr(L, f).argtypes = [P, C.c_int, dp]
+        L.orc_get_stats.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int), dp, dp]
+        L.orc_neighbor_counts.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_uint)]
+        L.orc_neighbor_list.argtypes = [P, C.c_int, C.c_int, C.c_size_t, C.POINTER(C.c_uint), C.c_int]
+        L.orc_set_state.argtypes = [P, C.c_char_p, dp]
+        L.orc_kernel_eval.argtypes = [C.c_int, C.c_double, dp, C.c_int, dp, dp, dp]
+        L.orc_grad_utils.argtypes = [dp, dp, dp, dp, dp, dp]
+        assert L.orc_params_size() == C.sizeof(Params), "oracle Params layout mismatch"
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_params(p: dict) -> Params:
+    q = Params()
+    q.radius = p["radius"]; q.dt = p["dt"]; q.rho0 = p["rho0"]
+    for k in range(3):
+        q.gravity[k] = p["gravity"][k]
+    q.maxError = p["max_error"]; q.maxErrorV = p["max_error_v"]
+    q.minIterations = p["min_iterations"]; q.maxIterations = p["max_iterations"]
+    q.maxIterationsV = p["max_iterations_v"]
+    q.usePressureWarmstart = p["use_pressure_warmstart"]
+    q.useDivergenceWarmstart = p["use_divergence_warmstart"]
+    q.enableDivergenceSolver = p["enable_divergence_solver"]
+    q.gradientMode = p["gradient_mode"]; q.optimizeRotation = p["optimize_rotation"]
+    q.gyroscopic = p["gyroscopic"]; q.computeGradient = p["compute_gradient"]
+    q.uniformAccTime = p["uniform_acc_time"]; q.targetTime = p["target_time"]
+    return q
+
+
+class OracleSim:
+    """One oracle simulation built from a difffr_amd.scenes.Scene."""
+
+    def __init__(self, scene, variant: str = "f32_rigid64", params_override: dict | None = None):
+        L = lib()
+        p = dict(scene.params)
+        if params_override:
+            p.update(params_override)
+        self.params = p
+        self._p = make_params(p)
+        self.h = L.orc_create(VARIANTS[variant], C.byref(self._p))
+        if not self.h:
+            raise RuntimeError("orc_create failed")
+        x = np.ascontiguousarray(scene.fluid_x, np.float64)
+        v = np.ascontiguousarray(scene.fluid_v, np.float64)
+        L.orc_set_fluid(self.h, x.shape[0], _dp(x), _dp(v))
+        self.n_bodies = 0
+        for b in scene.bodies:
+            x0 = np.ascontiguousarray(b.x0, np.float64)
+            xr = np.ascontiguousarray(b.x_rb, np.float64)
+            q = np.ascontiguousarray(b.q, np.float64)
+            bid = L.orc_add_body(self.h, x0.shape[0], _dp(x0), int(b.dynamic), _dp(xr), _dp(q), float(b.density))
+            v0 = np.ascontiguousarray(b.v0, np.float64)
+            w0 = np.ascontiguousarray(b.w0, np.float64)
+            L.orc_set_init_velocity(self.h, bid, _dp(v0), _dp(w0))
+            self.n_bodies += 1
+        L.orc_init(self.h)
+        self.n = x.shape[0]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def step(self, n: int = 1):
+        lib().orc_step(self.h, n)
+
+    def stage(self, which: int = 0):
+        lib().orc_stage(self.h, which)
+
+    def fluid(self):
+        x = np.empty((self.n, 3)); v = np.empty((self.n, 3))
+        lib().orc_get_fluid(self.h, _dp(x), _dp(v))
+        return x, v
+
+    def field(self, name: str) -> np.ndarray:
+        k = 3 if name == "sum_grad_p_k" else 1
+        out = np.empty(self.n * k)
+        if lib().orc_get_field(self.h, name.encode(), _dp(out)) != 0:
+            raise KeyError(name)
+        return out.reshape(self.n, 3) if k == 3 else out
+
+    def body_field(self, body: int, name: str) -> np.ndarray:
+        n = lib().orc_n_body_particles(self.h, body)
+        k = 1 if name == "volume" else 3
+        out = np.empty(n * k)
+        if lib().orc_get_body_field(self.h, body, name.encode(), _dp(out)) != 0:
+            raise KeyError(name)
+        return out if k == 1 else out.reshape(n, 3)
+
+    def rigid(self, body: int) -> dict:
+        o = np.empty(19)
+        lib().orc_get_rigid(self.h, body, _dp(o))
+        return dict(x=o[0:3], q=o[3:7], v=o[7:10], omega=o[10:13], force=o[13:16], torque=o[16:19])
+
+    def jacobians(self, body: int) -> np.ndarray:
+        o = np.empty(78)
+        lib().orc_get_jacobians(self.h, body, _dp(o))
+        return o
+
+    def net_gradients(self, body: int) -> np.ndarray:
+        o = np.empty(78)
+        lib().orc_get_net_gradients(self.h, body, _dp(o))
+        return o
+
+    def mass_props(self, body: int) -> dict:
+        o = np.empty(28)
+        lib().orc_get_mass_props(self.h, body, _dp(o))
+        return dict(mass=o[0], I0=o[1:10].reshape(3, 3), I=o[10:19].reshape(3, 3), Iinv=o[19:28].reshape(3, 3))
+
+    def stats(self) -> dict:
+        it = C.c_int(); itv = C.c_int(); e = C.c_double(); ev = C.c_double()
+        lib().orc_get_stats(self.h, C.byref(it), C.byref(itv), C.byref(e), C.byref(ev))
+        return dict(iterations=it.value, iterations_v=itv.value, avg_density_err=e.value, avg_div_err=ev.value)
+
+    def neighbor_lists(self, kind: int, body: int = 0):
+        """kind 0: fluid->fluid, 1: fluid->body `body`, 2: body `body`->fluid."""
+        L = lib()
+        n = self.n if kind in (0, 1) else L.orc_n_body_particles(self.h, body)
+        counts = np.empty(n, np.uint32)
+        r = L.orc_neighbor_counts(self.h, kind, body, counts.ctypes.data_as(C.POINTER(C.c_uint)))
+        if r < 0:
+            raise ValueError("no such neighbour list")
+        out = []
+        buf = np.empty(max(int(counts.max(initial=0)), 1), np.uint32)
+        for i in range(n):
+            c = L.orc_neighbor_list(self.h, kind, body, i, buf.ctypes.data_as(C.POINTER(C.c_uint)), buf.size)
+            out.append(buf[:c].copy())
+        return out
+
+    def set_state(self, name: str, values: np.ndarray):
+        a = np.ascontiguousarray(values, np.float64)
+        if lib().orc_set_state(self.h, name.encode(), _dp(a)) != 0:
+            raise KeyError(name)
+
+
+def kernel_eval(radius: float, r: np.ndarray, double: bool = True):
+    r = np.ascontiguousarray(r, np.float64).reshape(-1, 3)
+    n = r.shape[0]
+    W = np.empty(n); gW = np.empty((n, 3)); ggW = np.empty((n, 3, 3))
+    lib().orc_kernel_eval(int(double), radius, _dp(r), n, _dp(W), _dp(gW), _dp(ggW))
+    return W, gW, ggW
+
+
+def grad_utils(q: np.ndarray, p: np.ndarray):
+    q = np.ascontiguousarray(q, np.float64); p = np.ascontiguousarray(p, np.float64)
+    a = np.empty((3, 4)); b = np.empty((3, 4)); c = np.empty((4, 4)); d = np.empty((4, 3))
+    lib().orc_grad_utils(_dp(q), _dp(p), _dp(a), _dp(b), _dp(c), _dp(d))
+    return a, b, c, d

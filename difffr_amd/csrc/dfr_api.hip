@@ -283,13 +283,14 @@ void begin_step(dfr_ctx* c) {
 }
 
 void check_overflow(dfr_ctx* c) {
-  if (c->h_overflow[0] || c->h_overflow[1]) {
-    char buf[256];
+  if (c->h_overflow[0] || c->h_overflow[1] || c->h_overflow[2]) {
+    char buf[320];
     std::snprintf(buf, sizeof(buf),
-                  "neighbour list capacity exceeded (fluid: %u > max_neighbors=%d, dynamic sample: %u > "
-                  "max_boundary_neighbors=%d)",
-                  c->h_overflow[0], c->P.max_neighbors, c->h_overflow[1], c->P.max_boundary_neighbors);
-    c->h_overflow[0] = c->h_overflow[1] = 0;
+                  "neighbour list capacity exceeded (fluid neighbours: %u > %d, boundary neighbours: %u > "
+                  "max_neighbors=%d, fluid neighbours of a dynamic sample: %u > max_boundary_neighbors=%d)",
+                  c->h_overflow[0], MAXF, c->h_overflow[2], c->P.max_neighbors, c->h_overflow[1],
+                  c->P.max_boundary_neighbors);
+    c->h_overflow[0] = c->h_overflow[1] = c->h_overflow[2] = 0;
     throw std::runtime_error(buf);
   }
 }
@@ -330,9 +331,12 @@ void do_step(dfr_ctx* c) {
   }
   {
     Timed t(c, TC_NBR);
-    HIPCHK(hipMemsetAsync(c->nb.overflow, 0, 2 * sizeof(uint32_t), c->stream));
-    if (nf > 0) launch_build_neighbors(c->f, c->b, c->grid, c->gb, c->nb, c->h * c->h, c->stream);
-    HIPCHK(hipMemcpyAsync(c->h_overflow, c->nb.overflow, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemsetAsync(c->nb.overflow, 0, 3 * sizeof(uint32_t), c->stream));
+    if (nf > 0) {
+      launch_build_tiled(c->f, c->b, c->grid, c->gb, c->nb, c->h * c->h, c->stream);
+      launch_build_bf(c->f, c->b, c->grid, c->gb, c->nb, c->h * c->h, c->stream);
+    }
+    HIPCHK(hipMemcpyAsync(c->h_overflow, c->nb.overflow, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   }
   HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(double) * c->err_slots, c->stream));
   const bool divOn = c->P.enable_divergence_solver != 0;
@@ -341,7 +345,8 @@ void do_step(dfr_ctx* c) {
   const int scratch_slot = c->err_slots - 1;
   if (nf > 0) {
     Timed t(c, TC_DENSITY);
-    launch_density_factor(c->sc, c->f, c->b, c->nb, (divOn && warmV) ? 1 : 0, divOn ? 1 : 0, c->stream);
+    launch_density_factor_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, (divOn && warmV) ? 1 : 0, divOn ? 1 : 0,
+                            c->stream);
   }
   // ---- divergence solve (TimeStepDiffDFSPH::divergenceSolve, :770-881)
   int itv = 0;
@@ -350,11 +355,12 @@ void do_step(dfr_ctx* c) {
     const float invH = 1.f / dt;
     if (warmV) {
       Timed t(c, TC_VUPDATE);
-      launch_velocity_update(c->sc, c->f, c->b, c->nb, c->d_rig, c->f.kappav, nullptr, c->stream);
+      launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kappav, nullptr, c->stream);
     }
     {
       Timed t(c, TC_RECOMPUTE);
-      launch_recompute(c->sc, c->f, c->b, c->nb, RECOMPUTE_DIV, invH, c->d_err + scratch_slot, c->stream);
+      launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_DIV, invH, c->d_err + scratch_slot,
+                         c->stream);
     }
     const double eta = (1.0f / dt) * (float)c->P.max_error_v * 0.01f * c->sc.rho0;
     bool chk = false;
@@ -362,11 +368,12 @@ void do_step(dfr_ctx* c) {
       run_gradient(c, 1);
       {
         Timed t(c, TC_VUPDATE);
-        launch_velocity_update(c->sc, c->f, c->b, c->nb, c->d_rig, c->f.kf, warmV ? c->f.kappav : nullptr, c->stream);
+        launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kf,
+                                 warmV ? c->f.kappav : nullptr, c->stream);
       }
       {
         Timed t(c, TC_RECOMPUTE);
-        launch_recompute(c->sc, c->f, c->b, c->nb, RECOMPUTE_DIV, invH, c->d_err + itv, c->stream);
+        launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_DIV, invH, c->d_err + itv, c->stream);
       }
       avgv = read_err(c, itv) / (double)nf;
       chk = avgv <= eta;
@@ -385,14 +392,15 @@ void do_step(dfr_ctx* c) {
     if (warmP) {
       {
         Timed t(c, TC_PWARM);
-        launch_pressure_warm(c->sc, c->f, c->b, c->nb, c->stream);
+        launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, PRESSURE_WARMSTART, 0.f, nullptr, c->stream);
       }
       Timed t(c, TC_VUPDATE);
-      launch_velocity_update(c->sc, c->f, c->b, c->nb, c->d_rig, c->f.kappa, nullptr, c->stream);
+      launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kappa, nullptr, c->stream);
     }
     {
       Timed t(c, TC_RECOMPUTE);
-      launch_recompute(c->sc, c->f, c->b, c->nb, RECOMPUTE_PRESSURE, invH2, c->d_err + scratch_slot, c->stream);
+      launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_PRESSURE, invH2,
+                         c->d_err + scratch_slot, c->stream);
     }
     const double eta = (float)c->P.max_error * 0.01f * c->sc.rho0;
     const int base = c->P.max_iterations_v;
@@ -401,11 +409,13 @@ void do_step(dfr_ctx* c) {
       run_gradient(c, 0);
       {
         Timed t(c, TC_VUPDATE);
-        launch_velocity_update(c->sc, c->f, c->b, c->nb, c->d_rig, c->f.kf, warmP ? c->f.kappa : nullptr, c->stream);
+        launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kf,
+                                 warmP ? c->f.kappa : nullptr, c->stream);
       }
       {
         Timed t(c, TC_RECOMPUTE);
-        launch_recompute(c->sc, c->f, c->b, c->nb, RECOMPUTE_PRESSURE, invH2, c->d_err + base + it, c->stream);
+        launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_PRESSURE, invH2, c->d_err + base + it,
+                           c->stream);
       }
       avg = read_err(c, base + it) / (double)nf;
       chk = avg <= eta;
@@ -484,7 +494,7 @@ void dfr_default_params(dfr_params* p, double radius, double dt) {
   p->compute_gradient = 1;
   p->uniform_acc_time = 0.0;
   p->target_time = 1e30;
-  p->max_neighbors = 96;
+  p->max_neighbors = 64;
   p->max_boundary_neighbors = 128;
 }
 
@@ -492,7 +502,7 @@ dfr_ctx* dfr_create(const dfr_params* p, int device) {
   if (!p || p->radius <= 0.0 || p->dt <= 0.0) return nullptr;
   dfr_ctx* c = new dfr_ctx();
   c->P = *p;
-  if (c->P.max_neighbors <= 0) c->P.max_neighbors = 96;
+  if (c->P.max_neighbors <= 0) c->P.max_neighbors = 64;
   if (c->P.max_boundary_neighbors <= 0) c->P.max_boundary_neighbors = 128;
   if (c->P.max_neighbors > 65535) c->P.max_neighbors = 65535;
   c->device = device;
@@ -508,6 +518,9 @@ dfr_ctx* dfr_create(const dfr_params* p, int device) {
   c->sc.K.k = 8.f / (pi * h3);
   c->sc.K.l = 48.f / (pi * h3);
   c->sc.K.w0 = c->sc.K.k;  // W(0) = k (6*0 - 6*0 + 1)
+  c->sc.K.inv_h = 1.f / c->h;
+  c->sc.K.l_ih = c->sc.K.l / c->h;
+  c->sc.K.l_ih2 = c->sc.K.l / (c->h * c->h);
   c->sc.dt = c->dtf;
   c->sc.rho0 = (float)p->rho0;
   const float diam = 2.f * (float)p->radius;
@@ -647,21 +660,23 @@ int dfr_init(dfr_ctx* c) {
     c->gb.key = dalloc<uint32_t>(c, NF); c->gb.slot = dalloc<uint32_t>(c, NF); c->gb.perm = dalloc<uint32_t>(c, NF);
     c->gb.bkey = dalloc<uint32_t>(c, NB); c->gb.bslot = dalloc<uint32_t>(c, NB); c->gb.bperm = dalloc<uint32_t>(c, NB);
     c->gb.tile_sums = dalloc<uint32_t>(c, scan_tile_count(nk) + 1);
-    c->nb.maxn = c->P.max_neighbors;
+    c->nb.maxb = c->P.max_neighbors;
     c->nb.maxbf = c->P.max_boundary_neighbors;
-    c->nb.nbr = dalloc<uint32_t>(c, (size_t)c->nb.maxn * NF);
+    c->nb.nbr8 = dalloc<uint4>(c, (size_t)NPACK * NF);
+    c->nb.nbrB = dalloc<uint32_t>(c, (size_t)c->nb.maxb * NF);
     c->nb.cnt = dalloc<uint32_t>(c, NF);
     c->nb.bf = dalloc<uint32_t>(c, (size_t)c->nb.maxbf * std::max<int64_t>(nd, 1));
     c->nb.bfcnt = dalloc<uint32_t>(c, std::max<int64_t>(nd, 1));
-    c->nb.overflow = dalloc<uint32_t>(c, 2);
-    c->sc.maxn = c->nb.maxn;
+    c->nb.overflow = dalloc<uint32_t>(c, 3);
+    c->sc.maxn = MAXF;
+    set_tiled_kernel_attributes();
     c->sc.maxbf = c->nb.maxbf;
     c->d_rig = dalloc<RigidDev>(c, std::max(nbodies, 1));
     c->err_slots = c->P.max_iterations_v + c->P.max_iterations + 2;
     c->d_err = dalloc<double>(c, c->err_slots);
     HIPCHK(hipHostMalloc((void**)&c->h_err, sizeof(double) * c->err_slots, hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void**)&c->h_overflow, sizeof(uint32_t) * 2, hipHostMallocDefault));
-    c->h_overflow[0] = c->h_overflow[1] = 0;
+    HIPCHK(hipHostMalloc((void**)&c->h_overflow, sizeof(uint32_t) * 3, hipHostMallocDefault));
+    c->h_overflow[0] = c->h_overflow[1] = c->h_overflow[2] = 0;
     // uploads
     HIPCHK(hipMemcpy(c->b.x0, x0.data(), sizeof(float4) * off, hipMemcpyHostToDevice));
     std::vector<uint32_t> dyn(std::max<int64_t>(nd, 1));
@@ -865,20 +880,31 @@ int64_t dfr_get_neighbors(const dfr_ctx* c, int kind, uint32_t* counts, uint32_t
   if (nf && hipMemcpy(id.data(), c->f.id, nf * 4, hipMemcpyDeviceToHost) != hipSuccess) return DFR_ERR_HIP;
   if (kind == 0 || kind == 1) {
     std::vector<uint32_t> cnt(std::max<int64_t>(nf, 1));
-    std::vector<uint32_t> nbr((size_t)c->nb.maxn * std::max<int64_t>(nf, 1));
+    const int slots = (kind == 0) ? MAXF : c->nb.maxb;
+    std::vector<uint32_t> nbr((size_t)slots * std::max<int64_t>(nf, 1));
     if (nf && hipMemcpy(cnt.data(), c->nb.cnt, nf * 4, hipMemcpyDeviceToHost) != hipSuccess) return DFR_ERR_HIP;
-    if (nf && hipMemcpy(nbr.data(), c->nb.nbr, (size_t)c->nb.maxn * nf * 4, hipMemcpyDeviceToHost) != hipSuccess)
-      return DFR_ERR_HIP;
+    if (nf && kind == 0) {
+      uint32_t* tmp = nullptr;
+      if (hipMalloc(&tmp, (size_t)slots * nf * 4) != hipSuccess) return DFR_ERR_HIP;
+      dfr_ctx* cc = const_cast<dfr_ctx*>(c);
+      launch_expand_lists(cc->grid, cc->gb, cc->f, cc->nb, tmp, cc->stream);
+      const bool ok = hipStreamSynchronize(c->stream) == hipSuccess &&
+                      hipMemcpy(nbr.data(), tmp, (size_t)slots * nf * 4, hipMemcpyDeviceToHost) == hipSuccess;
+      (void)hipFree(tmp);
+      if (!ok) return DFR_ERR_HIP;
+    } else if (nf) {
+      if (hipMemcpy(nbr.data(), c->nb.nbrB, (size_t)slots * nf * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return DFR_ERR_HIP;
+    }
     // rows in particle-id order
     std::vector<int64_t> row_of(nf);
     for (int64_t k = 0; k < nf; ++k) row_of[id[k]] = k;
     int64_t w = 0;
     for (int64_t r = 0; r < nf; ++r) {
       const int64_t k = row_of[r];
-      const uint32_t nF = cnt[k] & 0xffff, nB = cnt[k] >> 16;
-      const uint32_t s0 = (kind == 0) ? 0 : nF, s1 = (kind == 0) ? nF : nF + nB;
-      counts[r] = s1 - s0;
-      for (uint32_t s = s0; s < s1; ++s) {
+      const uint32_t n = (kind == 0) ? (cnt[k] & 0xffff) : (cnt[k] >> 16);
+      counts[r] = n;
+      for (uint32_t s = 0; s < n; ++s) {
         if (w >= cap) return DFR_ERR_CAPACITY;
         const uint32_t j = nbr[(size_t)s * nf + k];
         lists[w++] = (kind == 0) ? id[j] : j;

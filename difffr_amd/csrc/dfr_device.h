@@ -50,6 +50,9 @@ __device__ inline float dist2_exact(float ax, float ay, float az, float bx, floa
 // ---------------------------------------------------------------------------
 struct KernelConst {
   float h, k, l, w0;
+  float inv_h;   // 1/h
+  float l_ih;    // l/h
+  float l_ih2;   // l/h^2
 };
 
 __device__ inline float cubic_W(const KernelConst& K, float rl) {
@@ -77,6 +80,44 @@ __device__ inline float cubic_gradW_scale(const KernelConst& K, float rl) {
     const float f = 1.f - q;
     return K.l * (-f * f) * inv;
   }
+  return 0.f;
+}
+
+// Division-free forms used by the gather passes.  With q = |r|/h:
+//   gradW(r) = c r,  c = l (3q - 2) / h^2          for q <= 1/2
+//                    c = -l (1 - q)^2 / (h |r|)     for 1/2 < q <= 1
+// (the q / (|r| h) factor of SPHKernels.h:81-101 is 1/h^2), and 1/|r| comes from
+// one hardware reciprocal square root.  Results differ from the reference's
+// sqrt/divide form by a few ulp.
+struct PairKernel {
+  float W;    // W(|r|)
+  float c;    // gradW(r) = c * r
+};
+__device__ inline PairKernel cubic_pair(const KernelConst& K, float r2) {
+  PairKernel o{0.f, 0.f};
+  if (!(r2 > 1.0e-10f)) {             // |r| <= 1e-5: gradient is zero, W = W(0)
+    o.W = K.k;
+    return o;
+  }
+  const float irl = __builtin_amdgcn_rsqf(r2);
+  const float rl = r2 * irl;
+  const float q = rl * K.inv_h;
+  if (q <= 0.5f) {
+    o.W = K.k * (6.f * q * q * (q - 1.f) + 1.f);
+    o.c = K.l_ih2 * (3.f * q - 2.f);
+  } else if (q <= 1.f) {
+    const float f = 1.f - q;
+    o.W = 2.f * K.k * (f * f * f);
+    o.c = -K.l_ih * (f * f) * irl;
+  }
+  return o;
+}
+__device__ inline float cubic_grad_c(const KernelConst& K, float r2) {
+  if (!(r2 > 1.0e-10f)) return 0.f;
+  const float irl = __builtin_amdgcn_rsqf(r2);
+  const float q = r2 * irl * K.inv_h;
+  if (q <= 0.5f) return K.l_ih2 * (3.f * q - 2.f);
+  if (q <= 1.f) { const float f = 1.f - q; return -K.l_ih * (f * f) * irl; }
   return 0.f;
 }
 
@@ -136,4 +177,55 @@ __device__ inline float wave_sum_f(float v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// Logical block index remapped so that each of the 8 XCDs of an MI355X (which
+// receive workgroups round-robin) processes one contiguous range of the grid:
+// neighbouring particles/bricks then share an L2.
+__device__ inline uint32_t xcd_block() {
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t per = nb / 8u, rem = nb % 8u;
+  const uint32_t x = b % 8u, k = b / 8u;
+  return x * per + min(x, rem) + k;
+}
+__device__ inline uint32_t gid() { return xcd_block() * blockDim.x + threadIdx.x; }
+
+// Exclusive scan of one value per thread over an NT-thread block.
+// lds needs NT/64 + 1 entries.
+template <int NT = DFR_BLOCK>
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds, uint32_t& total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += t;
+  }
+  if (lane == 63) lds[wid] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int w = 0; w < NT / 64; ++w) { const uint32_t t = lds[w]; lds[w] = acc; acc += t; }
+    lds[NT / 64] = acc;
+  }
+  __syncthreads();
+  const uint32_t res = inc - v + lds[wid];
+  total = lds[NT / 64];
+  __syncthreads();
+  return res;
+}
+
+// Block-level FP64 sum of one value per thread, atomically added to *out.
+template <int NT = DFR_BLOCK>
+__device__ inline void block_add_double(double v, double* out) {
+  __shared__ double part[NT / 64];
+  v = wave_sum_d(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) part[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < NT / 64; ++w) t += part[w];
+    if (t != 0.0) atomicAdd(out, t);
+  }
 }

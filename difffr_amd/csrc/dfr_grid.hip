@@ -13,17 +13,6 @@
 
 namespace dfr {
 
-// Logical block index remapped so that each of the 8 XCDs of an MI355X (which
-// receive workgroups round-robin) processes one contiguous range of particles:
-// neighbouring particles then share an L2.
-__device__ inline uint32_t xcd_block() {
-  const uint32_t nb = gridDim.x, b = blockIdx.x;
-  const uint32_t per = nb / 8u, rem = nb % 8u;
-  const uint32_t x = b % 8u, k = b / 8u;
-  return x * per + min(x, rem) + k;
-}
-__device__ inline uint32_t gid() { return xcd_block() * blockDim.x + threadIdx.x; }
-
 __global__ void k_keys(const float4* __restrict__ pos, int n, GridDesc g, uint32_t* __restrict__ count,
                        uint32_t* __restrict__ key, uint32_t* __restrict__ slot) {
   const uint32_t i = gid();
@@ -88,28 +77,6 @@ __global__ void k_gather_fluid(int n, const uint32_t* __restrict__ perm, const f
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = DFR_BLOCK * SCAN_ITEMS;
 
-__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds, uint32_t& total) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  uint32_t inc = v;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t t = __shfl_up(inc, off, 64);
-    if (lane >= off) inc += t;
-  }
-  if (lane == 63) lds[wid] = inc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t acc = 0;
-    for (int w = 0; w < DFR_BLOCK / 64; ++w) { const uint32_t t = lds[w]; lds[w] = acc; acc += t; }
-    lds[DFR_BLOCK / 64] = acc;
-  }
-  __syncthreads();
-  const uint32_t res = inc - v + lds[wid];
-  total = lds[DFR_BLOCK / 64];
-  __syncthreads();
-  return res;
-}
-
 __global__ void k_scan_tiles(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ out,
                              uint32_t* __restrict__ tile_sums) {
   __shared__ uint32_t lds[DFR_BLOCK / 64 + 1];
@@ -151,78 +118,6 @@ __global__ void k_scan_add(uint32_t* __restrict__ out, uint32_t n, const uint32_
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k)
     if (base + k < n) out[base + k] += add;
-}
-
-// ---------------------------------------------------------------------------
-// Neighbour lists.  Fluid particle i owns slots [0, nF) (fluid indices in the
-// sorted fluid order) and [nF, nF + nB) (boundary sample indices) at
-// nbr[slot * nf + i]; cnt[i] = nF | nB << 16.  Cells are visited in
-// (dz, dy, dx) order and each cell in its sorted order.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(DFR_BLOCK)
-k_build_fluid_nbrs(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t* __restrict__ fstart,
-                   const float4* __restrict__ bpos, const uint32_t* __restrict__ bstart,
-                   const uint32_t* __restrict__ bperm, float h2, int maxn, uint32_t* __restrict__ nbr,
-                   uint32_t* __restrict__ cnt, uint32_t* __restrict__ overflow) {
-  const uint32_t i = gid();
-  if (i >= (uint32_t)nf) return;
-  const float4 p = pos[i];
-  int cx, cy, cz;
-  cell_coords(g, p.x, p.y, p.z, cx, cy, cz);
-  uint32_t nF = 0, nB = 0;
-  const size_t stride = (size_t)nf;
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int z = cz + dz;
-    if (z < 0 || z >= g.n[2]) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int y = cy + dy;
-      if (y < 0 || y >= g.n[1]) continue;
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int x = cx + dx;
-        if (x < 0 || x >= g.n[0]) continue;
-        const uint32_t c = cell_key(g, x, y, z);
-        const uint32_t s = fstart[c], e = fstart[c + 1];
-        for (uint32_t j = s; j < e; ++j) {
-          if (j == i) continue;
-          const float4 q = pos[j];
-          if (dist2_exact(p.x, p.y, p.z, q.x, q.y, q.z) < h2) {
-            if (nF < (uint32_t)maxn) nbr[(size_t)nF * stride + i] = j;
-            ++nF;
-          }
-        }
-      }
-    }
-  }
-  const uint32_t nFk = min(nF, (uint32_t)maxn);
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int z = cz + dz;
-    if (z < 0 || z >= g.n[2]) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int y = cy + dy;
-      if (y < 0 || y >= g.n[1]) continue;
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int x = cx + dx;
-        if (x < 0 || x >= g.n[0]) continue;
-        const uint32_t c = cell_key(g, x, y, z);
-        const uint32_t s = bstart[c], e = bstart[c + 1];
-        for (uint32_t k = s; k < e; ++k) {
-          const uint32_t j = bperm[k];
-          const float4 q = bpos[j];
-          if (dist2_exact(p.x, p.y, p.z, q.x, q.y, q.z) < h2) {
-            const uint32_t slot = nFk + nB;
-            if (slot < (uint32_t)maxn) nbr[(size_t)slot * stride + i] = j;
-            ++nB;
-          }
-        }
-      }
-    }
-  }
-  if (nF + nB > (uint32_t)maxn) {
-    atomicMax(overflow, nF + nB);
-    nB = (uint32_t)maxn - nFk;
-    nF = nFk;
-  }
-  cnt[i] = nF | (nB << 16);
 }
 
 // Fluid neighbours of the samples of dynamic bodies (the B->F lists the
@@ -303,10 +198,8 @@ void launch_sort_boundary(BoundaryBufs& b, const GridDesc& g, GridBufs& gb, hipS
   }
 }
 
-void launch_build_neighbors(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
-                            hipStream_t s) {
-  hipLaunchKernelGGL(k_build_fluid_nbrs, dim3(blocks_for(f.n)), dim3(DFR_BLOCK), 0, s, f.pos, f.n, g, gb.fstart,
-                     b.pos, gb.bstart, gb.bperm, h2, nb.maxn, nb.nbr, nb.cnt, nb.overflow);
+void launch_build_bf(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
+                     hipStream_t s) {
   if (b.nd > 0) {
     hipLaunchKernelGGL(k_build_bf_nbrs, dim3(blocks_for(b.nd)), dim3(DFR_BLOCK), 0, s, b.pos, b.dyn_idx, b.nd, f.pos,
                        g, gb.fstart, h2, nb.maxbf, nb.bf, nb.bfcnt, nb.overflow + 1);

@@ -5,6 +5,14 @@
 
 namespace dfr {
 
+// Brick tiles: LDS slots per halo (a 6x6x6-cell halo holds ~1700-2200 particles
+// at rest density) and fluid-neighbour slots per particle (16-bit tile slots).
+constexpr int TILE_CAP = 3072;
+constexpr int MAXF = 64;
+constexpr int PACK = 8;               // 16-bit slots per 16-byte pack
+constexpr int NPACK = MAXF / PACK;    // packs per particle
+constexpr int PASS_BLOCK = 512;       // threads of a gather-pass workgroup (one brick)
+
 // Fluid state in HBM, structure-of-arrays, permuted into cell order every step.
 struct FluidBufs {
   int n = 0;
@@ -43,12 +51,13 @@ struct GridBufs {
 };
 
 struct NbrBufs {
-  int maxn = 0, maxbf = 0;
-  uint32_t* nbr = nullptr;    // [maxn][nf]
+  int maxb = 0, maxbf = 0;
+  uint4* nbr8 = nullptr;      // [NPACK][nf] fluid neighbours as tile slots, 8 per pack
+  uint32_t* nbrB = nullptr;   // [maxb][nf] boundary neighbours (sample index)
   uint32_t* cnt = nullptr;    // [nf]  nF | nB << 16
-  uint32_t* bf = nullptr;     // [maxbf][nd]
+  uint32_t* bf = nullptr;     // [maxbf][nd] fluid neighbours of dynamic samples (sorted index)
   uint32_t* bfcnt = nullptr;  // [nd]
-  uint32_t* overflow = nullptr;  // [2] max count seen when a list overflowed
+  uint32_t* overflow = nullptr;  // [3] max count seen when a list overflowed (fluid, B->F, boundary)
 };
 
 // dfr_grid.hip
@@ -56,18 +65,24 @@ size_t scan_tile_count(uint32_t n);
 void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* tile_sums, hipStream_t s);
 void launch_sort_fluid(FluidBufs& f, const GridDesc& g, GridBufs& gb, hipStream_t s);
 void launch_sort_boundary(BoundaryBufs& b, const GridDesc& g, GridBufs& gb, hipStream_t s);
-void launch_build_neighbors(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
-                            hipStream_t s);
+void launch_build_bf(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
+                     hipStream_t s);
+
+// dfr_tiled.hip
+void launch_build_tiled(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
+                        hipStream_t s);
+void launch_density_factor_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
+                             NbrBufs& nb, int div_warm, int divergence_on, hipStream_t s);
+void launch_velocity_update_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
+                              NbrBufs& nb, RigidDev* rig, const float* kfield, float* kappa_acc, hipStream_t s);
+// mode 0: divergence recompute, 1: pressure recompute, 2: pressure warm start
+void launch_recompute_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
+                        NbrBufs& nb, int mode, float kcoef, double* err_slot, hipStream_t s);
+void launch_expand_lists(const GridDesc& g, GridBufs& gb, FluidBufs& f, NbrBufs& nb, uint32_t* out, hipStream_t s);
+void set_tiled_kernel_attributes();
 
 // dfr_solver.hip
-enum RecomputeMode { RECOMPUTE_DIV = 0, RECOMPUTE_PRESSURE = 1 };
-void launch_density_factor(const StepConst& c, FluidBufs& f, BoundaryBufs& b, NbrBufs& nb, int div_warm,
-                           int divergence_on, hipStream_t s);
-void launch_velocity_update(const StepConst& c, FluidBufs& f, BoundaryBufs& b, NbrBufs& nb, RigidDev* rig,
-                            const float* kfield, float* kappa_acc, hipStream_t s);
-void launch_recompute(const StepConst& c, FluidBufs& f, BoundaryBufs& b, NbrBufs& nb, int mode, float kcoef,
-                      double* err_slot, hipStream_t s);
-void launch_pressure_warm(const StepConst& c, FluidBufs& f, BoundaryBufs& b, NbrBufs& nb, hipStream_t s);
+enum RecomputeMode { RECOMPUTE_DIV = 0, RECOMPUTE_PRESSURE = 1, PRESSURE_WARMSTART = 2 };
 void launch_advect(const StepConst& c, FluidBufs& f, float kappav_scale, hipStream_t s);
 void launch_position(const StepConst& c, FluidBufs& f, float kappa_scale, hipStream_t s);
 void launch_boundary_volume_impl(const float4* bpos_in, const float4* bx0, int nb, const int* groups, int nbodies,

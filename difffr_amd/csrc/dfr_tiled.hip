@@ -1,0 +1,648 @@
+// Brick-tiled neighbour build and gather passes.
+//
+// A workgroup owns one brick of 4x4x4 cells.  After the counting sort the
+// brick's particles are contiguous, and every neighbour of them lies in the
+// brick's 6x6x6-cell halo.  The workgroup stages the halo's particle data in
+// LDS with coalesced loads (cell ranges are contiguous), so the per-neighbour
+// gathers of all passes read LDS instead of L2.  Fluid neighbour lists store
+// 16-bit slots into the halo ("tile slots"; slot order = halo cells in
+// (z, y, x) order, particles of a cell in sorted order), eight slots per
+// 16-byte pack, packs stored pack-major (nbr8[p * N + i]) so that a wavefront
+// reads and writes 1 KB per instruction.  Boundary neighbours keep 32-bit
+// sample indices.
+//
+// A halo larger than the LDS capacity is served from global memory through the
+// same slot -> particle mapping (binary search over the halo cell offsets).
+#include "dfr_kernels.h"
+
+namespace dfr {
+
+constexpr int HALO = 216;
+
+template <int NT> struct TileHdr {
+  uint32_t cstart[HALO];      // first particle (global sorted index) of each halo cell
+  uint32_t coff[HALO + 1];    // tile slot of each halo cell's first particle
+  uint32_t scan[NT / 64 + 1];
+  uint32_t own_begin, own_end, total;
+  int bx, by, bz;
+};
+
+template <int NT>
+__device__ inline void tile_setup(const GridDesc& g, const uint32_t* __restrict__ fstart, uint32_t brick,
+                                  TileHdr<NT>& T) {
+  const int bx = (int)(brick % (uint32_t)g.bn[0]);
+  const int by = (int)((brick / (uint32_t)g.bn[0]) % (uint32_t)g.bn[1]);
+  const int bz = (int)(brick / ((uint32_t)g.bn[0] * (uint32_t)g.bn[1]));
+  const int t = threadIdx.x;
+  uint32_t cnt = 0;
+  if (t < HALO) {
+    const int cx = 4 * bx - 1 + t % 6, cy = 4 * by - 1 + (t / 6) % 6, cz = 4 * bz - 1 + t / 36;
+    uint32_t st = 0;
+    if (cx >= 0 && cy >= 0 && cz >= 0 && cx < g.n[0] && cy < g.n[1] && cz < g.n[2]) {
+      const uint32_t key = cell_key(g, cx, cy, cz);
+      st = fstart[key];
+      cnt = fstart[key + 1] - st;
+    }
+    T.cstart[t] = st;
+  }
+  uint32_t total;
+  const uint32_t ex = block_exclusive_scan<NT>(cnt, T.scan, total);
+  if (t < HALO) T.coff[t] = ex;
+  if (t == 0) {
+    T.coff[HALO] = total;
+    T.total = total;
+    T.own_begin = fstart[brick * 64u];
+    T.own_end = fstart[brick * 64u + 64u];
+    T.bx = bx; T.by = by; T.bz = bz;
+  }
+  __syncthreads();
+}
+
+// Global sorted index of a tile slot (slot < total).
+template <int NT> __device__ inline uint32_t tile_global(const TileHdr<NT>& T, uint32_t slot) {
+  int lo = 0, hi = HALO - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (T.coff[mid] <= slot) lo = mid; else hi = mid - 1;
+  }
+  return T.cstart[lo] + (slot - T.coff[lo]);
+}
+
+// Halo cell index of the cell (cx, cy, cz) + (dx, dy, dz) for a particle of the brick.
+template <int NT>
+__device__ inline int halo_cell(const TileHdr<NT>& T, int cx, int cy, int cz, int dx, int dy, int dz) {
+  return (cz + dz - 4 * T.bz + 1) * 36 + (cy + dy - 4 * T.by + 1) * 6 + (cx + dx - 4 * T.bx + 1);
+}
+
+// Two-phase halo staging: every thread first resolves its slots to particle
+// indices (LDS-only binary searches, independent), then issues all its global
+// loads before writing LDS, so the loads are in flight together.
+template <int NT> constexpr int per_thread() { return TILE_CAP / NT; }
+
+template <int NT>
+__device__ inline void tile_indices(const TileHdr<NT>& T, uint32_t total, uint32_t* js) {
+#pragma unroll
+  for (int r = 0; r < per_thread<NT>(); ++r) {
+    const uint32_t s = threadIdx.x + r * NT;
+    js[r] = (s < total) ? tile_global<NT>(T, s) : 0u;
+  }
+}
+
+__device__ inline uint32_t pack_get(const uint4& v, int u) {
+  const uint32_t w = (u < 2) ? v.x : (u < 4) ? v.y : (u < 6) ? v.z : v.w;
+  return (u & 1) ? (w >> 16) : (w & 0xffffu);
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour build (CompactNSearch::find_neighbors replacement).
+// ---------------------------------------------------------------------------
+constexpr int BB = DFR_BLOCK;  // build workgroup
+
+__global__ void __launch_bounds__(BB)
+k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t* __restrict__ fstart,
+              const float4* __restrict__ bpos, const uint32_t* __restrict__ bstart,
+              const uint32_t* __restrict__ bperm, float h2, int maxb, uint4* __restrict__ nbr8,
+              uint32_t* __restrict__ nbrB, uint32_t* __restrict__ cnt, uint32_t* __restrict__ overflow) {
+  __shared__ TileHdr<BB> T;
+  __shared__ uint32_t bcs[HALO], bcn[HALO];  // boundary sample range of each halo cell
+  __shared__ float lx[TILE_CAP], ly[TILE_CAP], lz[TILE_CAP];
+  __shared__ uint16_t stage[MAXF * BB];
+  const uint32_t brick = xcd_block();
+  tile_setup<BB>(g, fstart, brick, T);
+  if (T.own_begin == T.own_end) return;
+  const uint32_t total = T.total;
+  const int t = threadIdx.x;
+  if (total > 65535u) {
+    // tile slots are 16-bit: leave empty lists (safe for every pass) and report
+    for (uint32_t i = T.own_begin + t; i < T.own_end; i += BB) cnt[i] = 0u;
+    if (t == 0) atomicMax(&overflow[0], total);
+    return;
+  }
+  if (t < HALO) {
+    const int cx = 4 * T.bx - 1 + t % 6, cy = 4 * T.by - 1 + (t / 6) % 6, cz = 4 * T.bz - 1 + t / 36;
+    uint32_t st = 0, n = 0;
+    if (cx >= 0 && cy >= 0 && cz >= 0 && cx < g.n[0] && cy < g.n[1] && cz < g.n[2]) {
+      const uint32_t key = cell_key(g, cx, cy, cz);
+      st = bstart[key];
+      n = bstart[key + 1] - st;
+    }
+    bcs[t] = st;
+    bcn[t] = n;
+  }
+  const bool in_lds = total <= (uint32_t)TILE_CAP;
+  if (in_lds) {
+    constexpr int PTB = per_thread<BB>();
+    uint32_t js[PTB];
+    tile_indices<BB>(T, total, js);
+    float4 q[PTB];
+#pragma unroll
+    for (int r = 0; r < PTB; ++r)
+      if (t + r * BB < (int)total) q[r] = pos[js[r]];
+#pragma unroll
+    for (int r = 0; r < PTB; ++r) {
+      const uint32_t s = t + r * BB;
+      if (s < total) { lx[s] = q[r].x; ly[s] = q[r].y; lz[s] = q[r].z; }
+    }
+  }
+  __syncthreads();
+  const size_t stride = (size_t)nf;
+  for (uint32_t base = T.own_begin; base < T.own_end; base += BB) {
+    const uint32_t i = base + t;
+    const bool active = i < T.own_end;
+    uint32_t nF = 0, nB = 0;
+    if (active) {
+      const float4 p = pos[i];
+      int cx, cy, cz;
+      cell_coords(g, p.x, p.y, p.z, cx, cy, cz);
+      const int hc0 = halo_cell<BB>(T, cx, cy, cz, 0, 0, 0);
+      const uint32_t self = T.coff[hc0] + (i - T.cstart[hc0]);
+      // the three x-neighbour cells of a (dz, dy) row are consecutive halo
+      // cells, so their candidates form one contiguous slot range
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy) {
+          const int hm = halo_cell<BB>(T, cx, cy, cz, -1, dy, dz);
+          const uint32_t r0 = T.coff[hm], r1 = T.coff[hm + 3];
+          if (in_lds) {
+            for (uint32_t s0 = r0; s0 < r1; s0 += 4) {
+              float qx[4], qy[4], qz[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const uint32_t s = min(s0 + u, r1 - 1);
+                qx[u] = lx[s]; qy[u] = ly[s]; qz[u] = lz[s];
+              }
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const uint32_t s = s0 + u;
+                if (s < r1 && s != self && dist2_exact(p.x, p.y, p.z, qx[u], qy[u], qz[u]) < h2) {
+                  if (nF < (uint32_t)MAXF) stage[nF * BB + t] = (uint16_t)s;
+                  ++nF;
+                }
+              }
+            }
+          } else {
+            for (uint32_t s = r0; s < r1; ++s) {
+              if (s == self) continue;
+              const float4 q = pos[tile_global<BB>(T, s)];
+              if (dist2_exact(p.x, p.y, p.z, q.x, q.y, q.z) < h2) {
+                if (nF < (uint32_t)MAXF) stage[nF * BB + t] = (uint16_t)s;
+                ++nF;
+              }
+            }
+          }
+        }
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            const int hc = halo_cell<BB>(T, cx, cy, cz, dx, dy, dz);
+            const uint32_t n = bcn[hc];
+            if (n == 0) continue;
+            const uint32_t st = bcs[hc];
+            for (uint32_t k = 0; k < n; ++k) {
+              const uint32_t j = bperm[st + k];
+              const float4 q = bpos[j];
+              if (dist2_exact(p.x, p.y, p.z, q.x, q.y, q.z) < h2) {
+                if (nB < (uint32_t)maxb) nbrB[(size_t)nB * stride + i] = j;
+                ++nB;
+              }
+            }
+          }
+      if (nF > (uint32_t)MAXF) { atomicMax(&overflow[0], nF); nF = MAXF; }
+      if (nB > (uint32_t)maxb) { atomicMax(&overflow[2], nB); nB = maxb; }
+      cnt[i] = nF | (nB << 16);
+    }
+    __syncthreads();
+    // pack-major write-out: for each pack the lanes of a wave store consecutive i
+    for (int pk = 0; pk < NPACK; ++pk) {
+      const bool w = active && (uint32_t)(pk * PACK) < nF;
+      if (__ballot(w) == 0ull) break;
+      if (w) {
+        uint32_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t k0 = pk * PACK + 2 * u, k1 = k0 + 1;
+          const uint32_t lo = (k0 < nF) ? stage[k0 * BB + t] : 0u;
+          const uint32_t hi = (k1 < nF) ? stage[k1 * BB + t] : 0u;
+          v[u] = lo | (hi << 16);
+        }
+        nbr8[(size_t)pk * stride + i] = make_uint4(v[0], v[1], v[2], v[3]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gather passes: PASS_BLOCK threads per brick.  Dynamic LDS per slot:
+// x, y, z (+ per-pass fields).
+// ---------------------------------------------------------------------------
+constexpr int PB = PASS_BLOCK;
+constexpr int PT = per_thread<PB>();
+
+struct PassArgs {
+  StepConst c;
+  GridDesc g;
+  const uint32_t* fstart;
+  const float4* pos;
+  float4* vel;
+  const float4* bpos;
+  const float4* bvel;
+  const float4* bx0;
+  const uint4* nbr8;
+  const uint32_t* nbrB;
+  const uint32_t* cnt;
+  int nb_static;
+};
+
+// Visits the nF fluid neighbour slots of particle i in list order.  All packs
+// are loaded up front (independent 16-byte loads).
+template <class Body>
+__device__ inline void for_fluid_slots(const uint4* __restrict__ nbr8, size_t stride, uint32_t i, uint32_t nF,
+                                       Body&& body) {
+  uint4 pk[NPACK];
+#pragma unroll
+  for (int p = 0; p < NPACK; ++p)
+    pk[p] = ((uint32_t)(p * PACK) < nF) ? nbr8[(size_t)p * stride + i] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+  for (int p = 0; p < NPACK; ++p) {
+#pragma unroll
+    for (int u = 0; u < PACK; ++u)
+      if ((uint32_t)(p * PACK + u) < nF) body(pack_get(pk[p], u));
+  }
+}
+
+// TimeStep::computeDensities (TimeStep.cpp:147-202) + computeDFSPHFactor
+// (TimeStepDiffDFSPH.cpp:883-962) + the first loop of warmstartDivergenceSolve
+// (:1720-1732), one pass.
+__global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alpha, float4* __restrict__ s_out,
+                   float* __restrict__ badv, float* __restrict__ kappav, int div_warm, int divergence_on) {
+  __shared__ TileHdr<PB> T;
+  extern __shared__ float lds[];
+  const StepConst& c = a.c;
+  const uint32_t brick = xcd_block();
+  tile_setup<PB>(a.g, a.fstart, brick, T);
+  if (T.own_begin == T.own_end) return;
+  const uint32_t total = T.total;
+  const bool in_lds = total <= (uint32_t)TILE_CAP;
+  float* lx = lds; float* ly = lds + TILE_CAP; float* lz = lds + 2 * TILE_CAP;
+  float* lvx = lds + 3 * TILE_CAP; float* lvy = lds + 4 * TILE_CAP; float* lvz = lds + 5 * TILE_CAP;
+  const int t = threadIdx.x;
+  if (in_lds) {
+    uint32_t js[PT];
+    tile_indices<PB>(T, total, js);
+    float4 q[PT], v[PT];
+#pragma unroll
+    for (int r = 0; r < PT; ++r)
+      if (t + r * PB < (int)total) {
+        q[r] = a.pos[js[r]];
+        if (divergence_on) v[r] = a.vel[js[r]];
+      }
+#pragma unroll
+    for (int r = 0; r < PT; ++r) {
+      const uint32_t s = t + r * PB;
+      if (s < total) {
+        lx[s] = q[r].x; ly[s] = q[r].y; lz[s] = q[r].z;
+        if (divergence_on) { lvx[s] = v[r].x; lvy[s] = v[r].y; lvz[s] = v[r].z; }
+      }
+    }
+  }
+  __syncthreads();
+  const size_t stride = (size_t)c.nf;
+  for (uint32_t i = T.own_begin + t; i < T.own_end; i += PB) {
+    const float4 p = a.pos[i];
+    float4 vi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (divergence_on) vi = a.vel[i];
+    const uint32_t cc = a.cnt[i];
+    const uint32_t nF = cc & 0xffffu, nB = cc >> 16;
+    float dens = c.V_f * c.K.w0;
+    float sumsq = 0.f, gx = 0.f, gy = 0.f, gz = 0.f, delta = 0.f;
+    // sums over fluid neighbours without the common factor V_f (applied once)
+    float wsum = 0.f, sq = 0.f, fgx = 0.f, fgy = 0.f, fgz = 0.f, fdelta = 0.f;
+    auto pair = [&](float qx, float qy, float qz, float vx, float vy, float vz) {
+      const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
+      const PairKernel kw = cubic_pair(c.K, rx * rx + ry * ry + rz * rz);
+      wsum += kw.W;
+      const float wx = kw.c * rx, wy = kw.c * ry, wz = kw.c * rz;
+      sq += wx * wx + wy * wy + wz * wz;
+      fgx += wx; fgy += wy; fgz += wz;
+      if (divergence_on) fdelta += (vi.x - vx) * wx + (vi.y - vy) * wy + (vi.z - vz) * wz;
+    };
+    if (in_lds) {
+      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) {
+        if (divergence_on) pair(lx[s], ly[s], lz[s], lvx[s], lvy[s], lvz[s]);
+        else pair(lx[s], ly[s], lz[s], 0.f, 0.f, 0.f);
+      });
+    } else {
+      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) {
+        const uint32_t j = tile_global<PB>(T, s);
+        const float4 q = a.pos[j];
+        const float4 v = divergence_on ? a.vel[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        pair(q.x, q.y, q.z, v.x, v.y, v.z);
+      });
+    }
+    dens += c.V_f * wsum;
+    sumsq = c.V_f * c.V_f * sq;
+    gx = c.V_f * fgx; gy = c.V_f * fgy; gz = c.V_f * fgz;
+    delta = c.V_f * fdelta;
+    for (uint32_t k = 0; k < nB; ++k) {
+      const uint32_t j = a.nbrB[(size_t)k * stride + i];
+      const float4 q = a.bpos[j];
+      const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+      const PairKernel kw = cubic_pair(c.K, rx * rx + ry * ry + rz * rz);
+      dens += q.w * kw.W;
+      const float wx = kw.c * rx, wy = kw.c * ry, wz = kw.c * rz;
+      gx += q.w * wx; gy += q.w * wy; gz += q.w * wz;
+      if (divergence_on) {
+        const float4 vb = a.bvel[j];
+        delta += q.w * ((vi.x - vb.x) * wx + (vi.y - vb.y) * wy + (vi.z - vb.z) * wz);
+      }
+    }
+    rho[i] = dens * c.rho0;
+    s_out[i] = make_float4(-gx, -gy, -gz, 0.f);
+    sumsq += gx * gx + gy * gy + gz * gz;
+    alpha[i] = (sumsq > c.eps) ? -1.f / sumsq : 0.f;
+    if (divergence_on) {
+      float d = fmaxf(delta, 0.f);
+      if (nF + nB < 20u) d = 0.f;
+      badv[i] = d;
+      if (div_warm) {
+        const float invH = 1.f / c.dt;
+        kappav[i] = (d > 0.f) ? 0.5f * fmaxf(kappav[i], -0.5f) * invH : 0.f;
+      }
+    }
+  }
+}
+
+// Jacobi velocity update shared by the two warm starts and every iteration of
+// both solves (TimeStepDiffDFSPH.cpp:1024-1055, :1101-1197, :1753-1784,
+// :1826-1902): v_i -= dt sum_j (k_i + k_j) grad_p_ij over fluid pairs with
+// |k_i + k_j| > eps; when |k_i| > eps each boundary neighbour b changes v_i by
+// -dt k_i grad_p_ib and receives f = -m_i dv / dt (BoundaryModel::addForce,
+// BoundaryModel.h:46-58), summed here in FP64 per dynamic body.
+// LDS per slot: x, y, z, k.
+__global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restrict__ kappa_acc,
+                    RigidDev* __restrict__ rig) {
+  __shared__ TileHdr<PB> T;
+  extern __shared__ float lds[];
+  const StepConst& c = a.c;
+  const uint32_t brick = xcd_block();
+  tile_setup<PB>(a.g, a.fstart, brick, T);
+  if (T.own_begin == T.own_end) return;
+  const uint32_t total = T.total;
+  const bool in_lds = total <= (uint32_t)TILE_CAP;
+  float* lx = lds; float* ly = lds + TILE_CAP; float* lz = lds + 2 * TILE_CAP; float* lk = lds + 3 * TILE_CAP;
+  const int t = threadIdx.x;
+  if (in_lds) {
+    uint32_t js[PT];
+    tile_indices<PB>(T, total, js);
+    float4 q[PT];
+    float kk[PT];
+#pragma unroll
+    for (int r = 0; r < PT; ++r)
+      if (t + r * PB < (int)total) { q[r] = a.pos[js[r]]; kk[r] = kfield[js[r]]; }
+#pragma unroll
+    for (int r = 0; r < PT; ++r) {
+      const uint32_t s = t + r * PB;
+      if (s < total) { lx[s] = q[r].x; ly[s] = q[r].y; lz[s] = q[r].z; lk[s] = kk[r]; }
+    }
+  }
+  __syncthreads();
+  const size_t stride = (size_t)c.nf;
+  int cb = -1;
+  double F0 = 0, F1 = 0, F2 = 0, T0 = 0, T1 = 0, T2 = 0;
+  const float invH = 1.f / c.dt;
+  for (uint32_t i = T.own_begin + t; i < T.own_end; i += PB) {
+    const float ki = kfield[i];
+    if (kappa_acc) kappa_acc[i] += ki;
+    const float4 p = a.pos[i];
+    float4 vi = a.vel[i];
+    const uint32_t cc = a.cnt[i];
+    const uint32_t nF = cc & 0xffffu, nB = cc >> 16;
+    // v_i -= dt (k_i + k_j) grad_p_ij = dt V_f (k_i + k_j) c_ij r_ij; dt V_f applied once
+    float ux = 0.f, uy = 0.f, uz = 0.f;
+    auto pair = [&](float qx, float qy, float qz, float kj) {
+      const float kSum = ki + kj;
+      if (fabsf(kSum) > c.eps) {
+        const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
+        const float s = kSum * cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
+        ux += s * rx; uy += s * ry; uz += s * rz;
+      }
+    };
+    if (in_lds) {
+      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) { pair(lx[s], ly[s], lz[s], lk[s]); });
+    } else {
+      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) {
+        const uint32_t j = tile_global<PB>(T, s);
+        const float4 q = a.pos[j];
+        pair(q.x, q.y, q.z, kfield[j]);
+      });
+    }
+    {
+      const float dv = c.dt * c.V_f;
+      vi.x += dv * ux; vi.y += dv * uy; vi.z += dv * uz;
+    }
+    if (fabsf(ki) > c.eps) {
+      for (uint32_t k = 0; k < nB; ++k) {
+        const uint32_t j = a.nbrB[(size_t)k * stride + i];
+        const float4 q = a.bpos[j];
+        const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+        const float sc = cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
+        const float gpx = -q.w * (sc * rx), gpy = -q.w * (sc * ry), gpz = -q.w * (sc * rz);
+        const float am = -c.dt * 1.f * ki;
+        const float dvx = am * gpx, dvy = am * gpy, dvz = am * gpz;
+        vi.x += dvx; vi.y += dvy; vi.z += dvz;
+        if ((int)j >= a.nb_static) {
+          const int body = (int)a.bx0[j].w;
+          if (rig[body].dynamic) {
+            if (body != cb && cb >= 0) {
+              atomicAdd(&rig[cb].F[0], F0); atomicAdd(&rig[cb].F[1], F1); atomicAdd(&rig[cb].F[2], F2);
+              atomicAdd(&rig[cb].T[0], T0); atomicAdd(&rig[cb].T[1], T1); atomicAdd(&rig[cb].T[2], T2);
+              F0 = F1 = F2 = T0 = T1 = T2 = 0.0;
+            }
+            cb = body;
+            const float fx = -c.m_f * dvx * invH, fy = -c.m_f * dvy * invH, fz = -c.m_f * dvz * invH;
+            const double rxd = (double)q.x - rig[body].x[0], ryd = (double)q.y - rig[body].x[1],
+                         rzd = (double)q.z - rig[body].x[2];
+            F0 += fx; F1 += fy; F2 += fz;
+            T0 += ryd * fz - rzd * fy; T1 += rzd * fx - rxd * fz; T2 += rxd * fy - ryd * fx;
+          }
+        }
+      }
+    }
+    a.vel[i] = vi;
+  }
+  // Wave-level FP64 reduction of the coupling partials, one body at a time.
+  while (true) {
+    const unsigned long long m = __ballot(cb >= 0);
+    if (m == 0ull) break;
+    const int src = __ffsll((long long)m) - 1;
+    const int b0 = __shfl(cb, src, 64);
+    const bool mine = (cb == b0);
+    const double a0 = wave_sum_d(mine ? F0 : 0.0), a1 = wave_sum_d(mine ? F1 : 0.0), a2 = wave_sum_d(mine ? F2 : 0.0);
+    const double a3 = wave_sum_d(mine ? T0 : 0.0), a4 = wave_sum_d(mine ? T1 : 0.0), a5 = wave_sum_d(mine ? T2 : 0.0);
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&rig[b0].F[0], a0); atomicAdd(&rig[b0].F[1], a1); atomicAdd(&rig[b0].F[2], a2);
+      atomicAdd(&rig[b0].T[0], a3); atomicAdd(&rig[b0].T[1], a4); atomicAdd(&rig[b0].T[2], a5);
+    }
+    if (mine) cb = -1;
+  }
+}
+
+// TimeStepDiffDFSPH::computeDensityChange (:1977-2041) or computeDensityAdv
+// (:1926-1975) for every particle, followed by the residual sum of
+// divergenceSolveIteration / pressureSolveIteration (:1906-1916, :1203-1213)
+// and k_i = b_i alpha_i kcoef for the next Jacobi update (mode 0 divergence,
+// 1 pressure); mode 2 is the first loop of warmstartPressureSolve (:987-999).
+// LDS per slot: x, y, z, vx, vy, vz.
+__global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, const float* __restrict__ alpha,
+              float* __restrict__ badv, float* __restrict__ kf, float* __restrict__ kappa, double* __restrict__ err) {
+  __shared__ TileHdr<PB> T;
+  extern __shared__ float lds[];
+  const StepConst& c = a.c;
+  const uint32_t brick = xcd_block();
+  tile_setup<PB>(a.g, a.fstart, brick, T);
+  if (T.own_begin == T.own_end) return;
+  const uint32_t total = T.total;
+  const bool in_lds = total <= (uint32_t)TILE_CAP;
+  float* lx = lds; float* ly = lds + TILE_CAP; float* lz = lds + 2 * TILE_CAP;
+  float* lvx = lds + 3 * TILE_CAP; float* lvy = lds + 4 * TILE_CAP; float* lvz = lds + 5 * TILE_CAP;
+  const int t = threadIdx.x;
+  if (in_lds) {
+    uint32_t js[PT];
+    tile_indices<PB>(T, total, js);
+    float4 q[PT], v[PT];
+#pragma unroll
+    for (int r = 0; r < PT; ++r)
+      if (t + r * PB < (int)total) { q[r] = a.pos[js[r]]; v[r] = a.vel[js[r]]; }
+#pragma unroll
+    for (int r = 0; r < PT; ++r) {
+      const uint32_t s = t + r * PB;
+      if (s < total) {
+        lx[s] = q[r].x; ly[s] = q[r].y; lz[s] = q[r].z;
+        lvx[s] = v[r].x; lvy[s] = v[r].y; lvz[s] = v[r].z;
+      }
+    }
+  }
+  __syncthreads();
+  const size_t stride = (size_t)c.nf;
+  double e = 0.0;
+  for (uint32_t i = T.own_begin + t; i < T.own_end; i += PB) {
+    const float4 p = a.pos[i];
+    const float4 vi = a.vel[i];
+    const uint32_t cc = a.cnt[i];
+    const uint32_t nF = cc & 0xffffu, nB = cc >> 16;
+    float delta = 0.f;
+    float fdelta = 0.f;  // fluid part without the common factor V_f
+    auto pair = [&](float qx, float qy, float qz, float vx, float vy, float vz) {
+      const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
+      const float sc = cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
+      fdelta += sc * ((vi.x - vx) * rx + (vi.y - vy) * ry + (vi.z - vz) * rz);
+    };
+    if (in_lds) {
+      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) { pair(lx[s], ly[s], lz[s], lvx[s], lvy[s], lvz[s]); });
+    } else {
+      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) {
+        const uint32_t j = tile_global<PB>(T, s);
+        const float4 q = a.pos[j];
+        const float4 v = a.vel[j];
+        pair(q.x, q.y, q.z, v.x, v.y, v.z);
+      });
+    }
+    delta = c.V_f * fdelta;
+    for (uint32_t k = 0; k < nB; ++k) {
+      const uint32_t j = a.nbrB[(size_t)k * stride + i];
+      const float4 q = a.bpos[j];
+      const float4 vb = a.bvel[j];
+      const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+      const float sc = cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
+      delta += q.w * sc * ((vi.x - vb.x) * rx + (vi.y - vb.y) * ry + (vi.z - vb.z) * rz);
+    }
+    float b;
+    if (mode == 0) {
+      b = fmaxf(delta, 0.f);
+      if (nF + nB < 20u) b = 0.f;
+      kf[i] = b * alpha[i] * kcoef;
+      e += (double)(c.rho0 * b);
+    } else {
+      b = fmaxf(rho[i] / c.rho0 + c.dt * delta, 1.f);
+      if (mode == 1) {
+        kf[i] = (b - 1.f) * alpha[i] * kcoef;
+        e += (double)(c.rho0 * b - c.rho0);
+      } else {
+        const float invH2 = 1.f / (c.dt * c.dt);
+        kappa[i] = (b > 1.f) ? 0.5f * fmaxf(kappa[i], -0.00025f) * invH2 : 0.f;
+      }
+    }
+    badv[i] = b;
+  }
+  if (mode != 2) block_add_double<PB>(e, err);
+}
+
+// Expands tile slots back to sorted fluid indices (diagnostics).
+__global__ void __launch_bounds__(BB)
+k_expand_lists(GridDesc g, const uint32_t* __restrict__ fstart, int nf, const uint4* __restrict__ nbr8,
+               const uint32_t* __restrict__ cnt, uint32_t* __restrict__ out) {
+  __shared__ TileHdr<BB> T;
+  const uint32_t brick = blockIdx.x;
+  tile_setup<BB>(g, fstart, brick, T);
+  if (T.own_begin == T.own_end) return;
+  const size_t stride = (size_t)nf;
+  for (uint32_t i = T.own_begin + threadIdx.x; i < T.own_end; i += BB) {
+    const uint32_t nF = cnt[i] & 0xffffu;
+    uint32_t k = 0;
+    for_fluid_slots(nbr8, stride, i, nF, [&](uint32_t s) { out[(size_t)(k++) * stride + i] = tile_global<BB>(T, s); });
+  }
+}
+
+// ---------------------------------------------------------------------------
+static inline unsigned nbricks(const GridDesc& g) { return (unsigned)(g.bn[0] * g.bn[1] * g.bn[2]); }
+
+static PassArgs make_args(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
+                          NbrBufs& nb) {
+  PassArgs a;
+  a.c = c; a.g = g; a.fstart = gb.fstart; a.pos = f.pos; a.vel = f.vel; a.bpos = b.pos; a.bvel = b.vel;
+  a.bx0 = b.x0; a.nbr8 = nb.nbr8; a.nbrB = nb.nbrB; a.cnt = nb.cnt; a.nb_static = b.n - b.nd;
+  return a;
+}
+
+void launch_build_tiled(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_build_tiled, dim3(nbricks(g)), dim3(BB), 0, s, f.pos, f.n, g, gb.fstart, b.pos, gb.bstart,
+                     gb.bperm, h2, nb.maxb, nb.nbr8, nb.nbrB, nb.cnt, nb.overflow);
+}
+
+void launch_density_factor_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
+                             NbrBufs& nb, int div_warm, int divergence_on, hipStream_t s) {
+  const size_t shm = sizeof(float) * (size_t)TILE_CAP * (divergence_on ? 6 : 3);
+  hipLaunchKernelGGL(k_density_factor_t, dim3(nbricks(g)), dim3(PB), shm, s, make_args(c, g, gb, f, b, nb), f.rho,
+                     f.alpha, f.s, f.badv, f.kappav, div_warm, divergence_on);
+}
+
+void launch_velocity_update_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
+                              NbrBufs& nb, RigidDev* rig, const float* kfield, float* kappa_acc, hipStream_t s) {
+  const size_t shm = sizeof(float) * (size_t)TILE_CAP * 4;
+  hipLaunchKernelGGL(k_velocity_update_t, dim3(nbricks(g)), dim3(PB), shm, s, make_args(c, g, gb, f, b, nb), kfield,
+                     kappa_acc, rig);
+}
+
+void launch_recompute_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
+                        NbrBufs& nb, int mode, float kcoef, double* err_slot, hipStream_t s) {
+  const size_t shm = sizeof(float) * (size_t)TILE_CAP * 6;
+  hipLaunchKernelGGL(k_recompute_t, dim3(nbricks(g)), dim3(PB), shm, s, make_args(c, g, gb, f, b, nb), mode, kcoef,
+                     f.rho, f.alpha, f.badv, f.kf, f.kappa, err_slot);
+}
+
+void launch_expand_lists(const GridDesc& g, GridBufs& gb, FluidBufs& f, NbrBufs& nb, uint32_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_expand_lists, dim3(nbricks(g)), dim3(BB), 0, s, g, gb.fstart, f.n, nb.nbr8, nb.cnt, out);
+}
+
+void set_tiled_kernel_attributes() {
+  const int shm6 = (int)(sizeof(float) * TILE_CAP * 6), shm4 = (int)(sizeof(float) * TILE_CAP * 4);
+  (void)hipFuncSetAttribute((const void*)k_density_factor_t, hipFuncAttributeMaxDynamicSharedMemorySize, shm6);
+  (void)hipFuncSetAttribute((const void*)k_recompute_t, hipFuncAttributeMaxDynamicSharedMemorySize, shm6);
+  (void)hipFuncSetAttribute((const void*)k_velocity_update_t, hipFuncAttributeMaxDynamicSharedMemorySize, shm4);
+}
+
+}  // namespace dfr

@@ -93,32 +93,33 @@ struct PairKernel {
   float W;    // W(|r|)
   float c;    // gradW(r) = c * r
 };
+// Both evaluations are branch-free (selects only): a wavefront evaluates the
+// inner and outer polynomial for every pair instead of diverging on q <= 1/2.
 __device__ inline PairKernel cubic_pair(const KernelConst& K, float r2) {
-  PairKernel o{0.f, 0.f};
-  if (!(r2 > 1.0e-10f)) {             // |r| <= 1e-5: gradient is zero, W = W(0)
-    o.W = K.k;
-    return o;
-  }
   const float irl = __builtin_amdgcn_rsqf(r2);
   const float rl = r2 * irl;
   const float q = rl * K.inv_h;
-  if (q <= 0.5f) {
-    o.W = K.k * (6.f * q * q * (q - 1.f) + 1.f);
-    o.c = K.l_ih2 * (3.f * q - 2.f);
-  } else if (q <= 1.f) {
-    const float f = 1.f - q;
-    o.W = 2.f * K.k * (f * f * f);
-    o.c = -K.l_ih * (f * f) * irl;
-  }
+  const float f = 1.f - q;
+  const bool inner = q <= 0.5f;
+  const float Win = K.k * (6.f * q * q * (q - 1.f) + 1.f);
+  const float Wout = 2.f * K.k * (f * f * f);
+  const float cin = K.l_ih2 * (3.f * q - 2.f);
+  const float cout = -K.l_ih * (f * f) * irl;
+  const bool far = !(q <= 1.f);
+  PairKernel o;
+  // |r| <= 1e-5: gradient is zero, W = W(0)
+  o.W = (r2 > 1.0e-10f) ? (far ? 0.f : (inner ? Win : Wout)) : K.k;
+  o.c = (r2 > 1.0e-10f && !far) ? (inner ? cin : cout) : 0.f;
   return o;
 }
 __device__ inline float cubic_grad_c(const KernelConst& K, float r2) {
-  if (!(r2 > 1.0e-10f)) return 0.f;
   const float irl = __builtin_amdgcn_rsqf(r2);
   const float q = r2 * irl * K.inv_h;
-  if (q <= 0.5f) return K.l_ih2 * (3.f * q - 2.f);
-  if (q <= 1.f) { const float f = 1.f - q; return -K.l_ih * (f * f) * irl; }
-  return 0.f;
+  const float f = 1.f - q;
+  const float cin = K.l_ih2 * (3.f * q - 2.f);
+  const float cout = -K.l_ih * (f * f) * irl;
+  const float cc = (q <= 0.5f) ? cin : cout;
+  return (r2 > 1.0e-10f && q <= 1.f) ? cc : 0.f;
 }
 
 // gradGradW(r) = a * I + b * r r^T (SPHKernels.h:122-149 rewritten with

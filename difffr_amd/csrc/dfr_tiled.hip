@@ -88,6 +88,12 @@ __device__ inline void tile_indices(const TileHdr<NT>& T, uint32_t total, uint32
   }
 }
 
+// Uniform early exit for bricks that own no particle (about half of a tank's
+// bricks lie above the free surface): two scalar loads, no halo setup.
+__device__ inline bool brick_empty(const uint32_t* __restrict__ fstart, uint32_t brick) {
+  return fstart[brick * 64u] == fstart[brick * 64u + 64u];
+}
+
 __device__ inline uint32_t pack_get(const uint4& v, int u) {
   const uint32_t w = (u < 2) ? v.x : (u < 4) ? v.y : (u < 6) ? v.z : v.w;
   return (u & 1) ? (w >> 16) : (w & 0xffffu);
@@ -108,6 +114,7 @@ k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t
   __shared__ float lx[TILE_CAP], ly[TILE_CAP], lz[TILE_CAP];
   __shared__ uint16_t stage[MAXF * BB];
   const uint32_t brick = xcd_block();
+  if (brick_empty(fstart, brick)) return;
   tile_setup<BB>(g, fstart, brick, T);
   if (T.own_begin == T.own_end) return;
   const uint32_t total = T.total;
@@ -270,6 +277,18 @@ __device__ inline void for_fluid_slots(const uint4* __restrict__ nbr8, size_t st
   }
 }
 
+// Same visit order without unrolling, for the global-memory fallback (a halo
+// beyond the LDS capacity): keeps the rare path out of the unrolled hot loop.
+template <class Body>
+__device__ inline void for_fluid_slots_rolled(const uint4* __restrict__ nbr8, size_t stride, uint32_t i,
+                                              uint32_t nF, Body&& body) {
+#pragma unroll 1
+  for (uint32_t k = 0; k < nF; ++k) {
+    const uint4 v = nbr8[(size_t)(k >> 3) * stride + i];
+    body(pack_get(v, (int)(k & 7u)));
+  }
+}
+
 // TimeStep::computeDensities (TimeStep.cpp:147-202) + computeDFSPHFactor
 // (TimeStepDiffDFSPH.cpp:883-962) + the first loop of warmstartDivergenceSolve
 // (:1720-1732), one pass.
@@ -280,6 +299,7 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
   extern __shared__ float lds[];
   const StepConst& c = a.c;
   const uint32_t brick = xcd_block();
+  if (brick_empty(a.fstart, brick)) return;
   tile_setup<PB>(a.g, a.fstart, brick, T);
   if (T.own_begin == T.own_end) return;
   const uint32_t total = T.total;
@@ -333,7 +353,7 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
         else pair(lx[s], ly[s], lz[s], 0.f, 0.f, 0.f);
       });
     } else {
-      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) {
+      for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {
         const uint32_t j = tile_global<PB>(T, s);
         const float4 q = a.pos[j];
         const float4 v = divergence_on ? a.vel[j] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -387,6 +407,7 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
   extern __shared__ float lds[];
   const StepConst& c = a.c;
   const uint32_t brick = xcd_block();
+  if (brick_empty(a.fstart, brick)) return;
   tile_setup<PB>(a.g, a.fstart, brick, T);
   if (T.own_begin == T.own_end) return;
   const uint32_t total = T.total;
@@ -432,7 +453,7 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
     if (in_lds) {
       for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) { pair(lx[s], ly[s], lz[s], lk[s]); });
     } else {
-      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) {
+      for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {
         const uint32_t j = tile_global<PB>(T, s);
         const float4 q = a.pos[j];
         pair(q.x, q.y, q.z, kfield[j]);
@@ -502,6 +523,7 @@ k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, 
   extern __shared__ float lds[];
   const StepConst& c = a.c;
   const uint32_t brick = xcd_block();
+  if (brick_empty(a.fstart, brick)) return;
   tile_setup<PB>(a.g, a.fstart, brick, T);
   if (T.own_begin == T.own_end) return;
   const uint32_t total = T.total;
@@ -543,7 +565,7 @@ k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, 
     if (in_lds) {
       for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) { pair(lx[s], ly[s], lz[s], lvx[s], lvy[s], lvz[s]); });
     } else {
-      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) {
+      for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {
         const uint32_t j = tile_global<PB>(T, s);
         const float4 q = a.pos[j];
         const float4 v = a.vel[j];

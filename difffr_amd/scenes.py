@@ -219,3 +219,33 @@ def small_tank(counts=(24, 10, 24), seed: int = 0, steps: int = 20, body: str = 
     """A few-thousand-particle tank for fast parity tests (spacing 0.02)."""
     specs = [("ellipsoid", (0.1, 0.05, 0.075), 600.0)] if body == "ellipsoid" else [("sphere", 0.06, 500.0)]
     return tank(counts, 0.02, seed=seed, steps=steps, wall_height=0.5, name="small_tank", body_specs=specs)
+
+
+def submerged_body(counts=(16, 10, 16), spacing: float = 0.02, body: str = "sphere", seed: int = 0,
+                   steps: int = 10, v0=(0.5, -0.3, 0.2), w0=(1.0, -2.0, 0.5), dt: float = 1e-3) -> Scene:
+    """A small tank with the rigid body already half submerged (fluid particles
+    inside it removed), so fluid-rigid coupling acts from the first step.  Used
+    by the adjoint and finite-difference tests."""
+    rng = np.random.default_rng(seed)
+    r = 0.5 * spacing
+    fluid = block_lattice((0.0, 0.0, 0.0), counts, spacing)
+    fluid += rng.uniform(-0.1 * r, 0.1 * r, size=fluid.shape)
+    inner = spacing * (np.asarray(counts) + 1)
+    walls = box_surface((0.0, 0.0, 0.0), (inner[0], 2.0 * inner[1], inner[2]), spacing, open_top=True)
+    top = float(fluid[:, 1].max())
+    if body == "sphere":
+        size = 0.06
+        x0 = sphere_surface(size, spacing)
+        center = np.array([0.5 * inner[0], top, 0.5 * inner[2]])
+        inside = np.linalg.norm(fluid - center, axis=1) < size + r
+    else:
+        size = (0.08, 0.04, 0.06)
+        x0 = ellipsoid_surface(size, spacing)
+        center = np.array([0.5 * inner[0], top, 0.5 * inner[2]])
+        d = (fluid - center) / (np.asarray(size) + r)
+        inside = np.sum(d * d, axis=1) < 1.0
+    fluid = fluid[~inside]
+    bodies = [Body(walls, False, np.zeros(3), np.array([1.0, 0, 0, 0]), 1000.0, name="tank"),
+              Body(x0, True, center, np.array([1.0, 0, 0, 0]), 500.0, np.asarray(v0, np.float64),
+                   np.asarray(w0, np.float64), name=body)]
+    return Scene("submerged_" + body, default_params(r, dt), fluid, np.zeros_like(fluid), bodies, steps)

@@ -193,6 +193,30 @@ struct Stats {
   double avgDensityErr = 0, avgDivErr = 0;
 };
 
+// What the exact adjoint reads back for one step: the values every operation
+// of the step consumed, recorded as the step runs (dfsph_adjoint.hpp).
+template <typename RF, typename RR> struct BodyTape {
+  std::vector<V3<RF>> xb, vb;        // boundary samples at the start of the step
+  V3<RR> xrb, vpre, wpre, v, w, F, T;  // v, w after the release-window override
+  Quat<RR> q;
+  M3<RR> I, Iinv;
+  int ramp = 0;
+  RR factor = 0;
+  bool animated = false;
+};
+template <typename RF, typename RR> struct StepTape {
+  std::vector<V3<RF>> x, G;          // positions during the step, sum_grad_p_k
+  std::vector<std::vector<uint32_t>> nFF;
+  std::vector<std::vector<std::vector<uint32_t>>> nFB;
+  std::vector<RF> rho, alpha;
+  std::vector<RF> kvPrev, d0, kvW;   // divergence warm start: kappa_v in, d, kappa_v used
+  std::vector<RF> kPrev, rsW, kW;    // pressure warm start: kappa in, rho*, kappa used
+  std::vector<std::vector<V3<RF>>> vD, vP;   // velocity at the start of each iteration
+  std::vector<std::vector<RF>> kD, bD, kP, bP;  // k_i and d / rho* of each iteration
+  std::vector<BodyTape<RF, RR>> bodies;
+  int Iv = 0, Ip = 0;
+};
+
 template <typename RF, typename RR> class Sim {
  public:
   Params P;
@@ -211,6 +235,13 @@ template <typename RF, typename RR> class Sim {
   std::vector<std::vector<std::vector<uint32_t>>> nFB;   // [body][i] F -> B
   std::vector<std::vector<std::vector<uint32_t>>> nBF;   // [body][j] B -> F (dynamic/animated bodies)
   Stats stats;
+  // Exact-adjoint support: per-step tapes, and optional fixed Jacobi iteration
+  // counts per step (so that finite differences see the same iteration counts).
+  bool record = false;
+  std::vector<StepTape<RF, RR>> tapes;
+  StepTape<RF, RR>* tp = nullptr;
+  std::vector<int> fixIv, fixIp;
+  int fixPos = 0;
 
   explicit Sim(const Params& p) : P(p) {
     h = RF(4.0) * RF(P.radius);  // Simulation.cpp:195-216, support radius = 4r
@@ -373,7 +404,8 @@ template <typename RF, typename RR> class Sim {
   void begin_step() {
     stepCount++;
     const RF uat = RF(P.uniformAccTime);
-    for (auto& b : bodies) {
+    for (size_t k = 0; k < bodies.size(); ++k) {
+      auto& b = bodies[k];
       if (!b.dynamic) continue;
       if (time <= uat + dt) {
         RF factor = 1;
@@ -383,6 +415,7 @@ template <typename RF, typename RR> class Sim {
         }
         b.vrb = b.v0 * RR(factor);
         b.wrb = b.w0 * RR(factor);
+        if (tp) { tp->bodies[k].ramp = 1; tp->bodies[k].factor = RR(factor); }
       } else {
         b.animated = false;
       }
@@ -495,34 +528,32 @@ template <typename RF, typename RR> class Sim {
   void warmstart_divergence_solve() {
     const long N = long(x.size());
     const RF invH = RF(1.0) / dt;
-#pragma omp parallel
-    {
-#pragma omp for schedule(static)
-      for (long i = 0; i < N; ++i) {
-        compute_density_change(i);
-        if (bAdv[i] > RF(0.0)) kappaV[i] = RF(0.5) * std::max(kappaV[i], RF(-0.5)) * invH;
-        else kappaV[i] = 0;
-      }
-#pragma omp for schedule(static)
-      for (long i = 0; i < N; ++i) velocity_update(i, kappaV[i], kappaV);
+    if (tp) tp->kvPrev = kappaV;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      compute_density_change(i);
+      if (bAdv[i] > RF(0.0)) kappaV[i] = RF(0.5) * std::max(kappaV[i], RF(-0.5)) * invH;
+      else kappaV[i] = 0;
     }
+    if (tp) { tp->d0 = bAdv; tp->kvW = kappaV; }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) velocity_update(i, kappaV[i], kappaV);
   }
 
   // TimeStepDiffDFSPH::warmstartPressureSolve, TimeStepDiffDFSPH.cpp:964-1059.
   void warmstart_pressure_solve() {
     const long N = long(x.size());
     const RF invH2 = RF(1.0) / (dt * dt);
-#pragma omp parallel
-    {
-#pragma omp for schedule(static)
-      for (long i = 0; i < N; ++i) {
-        compute_density_adv(i);
-        if (bAdv[i] > RF(1.0)) kappa[i] = RF(0.5) * std::max(kappa[i], RF(-0.00025)) * invH2;
-        else kappa[i] = 0;
-      }
-#pragma omp for schedule(static)
-      for (long i = 0; i < N; ++i) velocity_update(i, kappa[i], kappa);
+    if (tp) tp->kPrev = kappa;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      compute_density_adv(i);
+      if (bAdv[i] > RF(1.0)) kappa[i] = RF(0.5) * std::max(kappa[i], RF(-0.00025)) * invH2;
+      else kappa[i] = 0;
     }
+    if (tp) { tp->rsW = bAdv; tp->kW = kappa; }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) velocity_update(i, kappa[i], kappa);
   }
 
   // TimeStepDiffDFSPH::pressureSolveIteration, TimeStepDiffDFSPH.cpp:1061-1220.
@@ -534,6 +565,7 @@ template <typename RF, typename RR> class Sim {
     std::vector<RF> kfield(N);
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < N; ++i) kfield[i] = (bAdv[i] - RF(1.0)) * alpha[i] * invH2;
+    if (tp) { tp->vP.push_back(v); tp->kP.push_back(kfield); tp->bP.push_back(bAdv); }
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < N; ++i) {
       const RF ki = kfield[i];
@@ -557,6 +589,7 @@ template <typename RF, typename RR> class Sim {
     std::vector<RF> kfield(N);
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < N; ++i) kfield[i] = bAdv[i] * alpha[i] * invH;
+    if (tp) { tp->vD.push_back(v); tp->kD.push_back(kfield); tp->bD.push_back(bAdv); }
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < N; ++i) {
       const RF ki = kfield[i];
@@ -582,7 +615,8 @@ template <typename RF, typename RR> class Sim {
     bool chk = false;
     RF avg = 0;
     const RF eta = (RF(1.0) / dt) * RF(P.maxErrorV) * RF(0.01) * rho0;
-    while ((!chk || it < 1) && it < P.maxIterationsV) {
+    const int fixed = fixPos < int(fixIv.size()) ? fixIv[fixPos] : -1;
+    while (fixed >= 0 ? it < fixed : ((!chk || it < 1) && it < P.maxIterationsV)) {
       avg = divergence_solve_iteration();
       chk = avg <= eta;
       it++;
@@ -603,7 +637,8 @@ template <typename RF, typename RR> class Sim {
     bool chk = false;
     RF avg = 0;
     const RF eta = RF(P.maxError) * RF(0.01) * rho0;
-    while ((!chk || it < P.minIterations) && it < P.maxIterations) {
+    const int fixed = fixPos < int(fixIp.size()) ? fixIp[fixPos] : -1;
+    while (fixed >= 0 ? it < fixed : ((!chk || it < P.minIterations) && it < P.maxIterations)) {
       avg = pressure_solve_iteration();
       chk = avg <= eta;
       it++;
@@ -873,10 +908,28 @@ template <typename RF, typename RR> class Sim {
   // One full step in SimulatorBase::timeStepNoGUI order (SimulatorBase.cpp:1142-1200)
   // around TimeStepDiffDFSPH::step (TimeStepDiffDFSPH.cpp:526-652).
   void step() {
+    tp = nullptr;
+    if (record) {
+      tapes.emplace_back();
+      tp = &tapes.back();
+      tp->bodies.resize(bodies.size());
+      for (size_t k = 0; k < bodies.size(); ++k) {
+        auto& t = tp->bodies[k];
+        const auto& b = bodies[k];
+        t.xb = b.x; t.vb = b.v; t.vpre = b.vrb; t.wpre = b.wrb;
+      }
+    }
     begin_step();
+    if (tp)
+      for (size_t k = 0; k < bodies.size(); ++k) {
+        auto& t = tp->bodies[k];
+        const auto& b = bodies[k];
+        t.xrb = b.xrb; t.q = b.q; t.v = b.vrb; t.w = b.wrb; t.I = b.I; t.Iinv = b.Iinv; t.animated = b.animated;
+      }
     neighborhood_search();
     compute_densities();
     compute_dfsph_factor();
+    if (tp) { tp->x = x; tp->nFF = nFF; tp->nFB = nFB; tp->rho = rho; tp->alpha = alpha; tp->G = s; }
     if (P.enableDivergenceSolver) divergence_solve();
     else stats.iterationsV = 0;
     const long N = long(x.size());
@@ -888,7 +941,13 @@ template <typename RF, typename RR> class Sim {
     time = time + dt;
     backward_per_step();
     rigid_velocity_step();
+    if (tp) {
+      for (size_t k = 0; k < bodies.size(); ++k) { tp->bodies[k].F = bodies[k].F_last; tp->bodies[k].T = bodies[k].T_last; }
+      tp->Iv = stats.iterationsV; tp->Ip = stats.iterations;
+    }
     rigid_position_step();
+    tp = nullptr;
+    ++fixPos;
   }
 
   // Reset of the rigid bodies and gradients between trajectories

@@ -7,7 +7,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
-#include "dfsph_oracle.hpp"
+#include "dfsph_adjoint.hpp"
 
 using namespace orc;
 
@@ -33,6 +33,11 @@ struct OracleBase {
   virtual int neighbor_list(int kind, int body, size_t i, unsigned* out, int cap) const = 0;
   virtual int set_state(const char* name, const double* in) = 0;
   virtual void stage(int which) = 0;
+  virtual void set_record(int on) = 0;
+  virtual void set_fixed_iterations(int n, const int* iv, const int* ip) = 0;
+  virtual int recorded_iterations(int* iv, int* ip, int cap) const = 0;
+  virtual void adjoint(const double* seed, double* out) = 0;
+  virtual int tape_field(int step, const char* name, int iter, double* out) const = 0;
 };
 
 template <typename RF, typename RR> struct Oracle : OracleBase {
@@ -152,6 +157,48 @@ template <typename RF, typename RR> struct Oracle : OracleBase {
     if (n == "time") { sim.time = RF(in[0]); return 0; }
     return -1;
   }
+  void set_record(int on) override {
+    sim.record = on != 0;
+    sim.tapes.clear();
+  }
+  void set_fixed_iterations(int n, const int* iv, const int* ip) override {
+    sim.fixIv.assign(iv, iv + n);
+    sim.fixIp.assign(ip, ip + n);
+    sim.fixPos = 0;
+  }
+  int recorded_iterations(int* iv, int* ip, int cap) const override {
+    const int n = int(sim.tapes.size());
+    for (int s = 0; s < n && s < cap; ++s) { iv[s] = sim.tapes[s].Iv; ip[s] = sim.tapes[s].Ip; }
+    return n;
+  }
+  // Fields of a recorded step (for tests): d0, kvW, rsW, kW (warm starts),
+  // bD/kD/bP/kP (iteration iter), nn (fluid + boundary neighbour count).
+  int tape_field(int step, const char* name, int iter, double* out) const override {
+    if (step < 0 || step >= int(sim.tapes.size())) return -1;
+    const auto& T = sim.tapes[step];
+    const std::string n(name);
+    const std::vector<RF>* f = nullptr;
+    if (n == "d0") f = &T.d0; else if (n == "kvW") f = &T.kvW; else if (n == "rsW") f = &T.rsW;
+    else if (n == "kW") f = &T.kW; else if (n == "alpha") f = &T.alpha; else if (n == "rho") f = &T.rho;
+    else if (n == "bD" && iter < int(T.bD.size())) f = &T.bD[iter];
+    else if (n == "kD" && iter < int(T.kD.size())) f = &T.kD[iter];
+    else if (n == "bP" && iter < int(T.bP.size())) f = &T.bP[iter];
+    else if (n == "kP" && iter < int(T.kP.size())) f = &T.kP[iter];
+    if (f) { for (size_t i = 0; i < f->size(); ++i) out[i] = double((*f)[i]); return int(f->size()); }
+    if (n == "nn") {
+      for (size_t i = 0; i < T.nFF.size(); ++i) {
+        size_t c = T.nFF[i].size();
+        for (const auto& l : T.nFB) c += l[i].size();
+        out[i] = double(c);
+      }
+      return int(T.nFF.size());
+    }
+    return -1;
+  }
+  void adjoint(const double* seed, double* out) override {
+    Adjoint<RF, RR> adj(sim);
+    adj.run(seed, out);
+  }
   // Partial steps for per-kernel parity: 0 = neighbour search + density + factor.
   void stage(int which) override {
     if (which == 0) {
@@ -200,6 +247,26 @@ int orc_get_stats(void* h, int* it, int* itv, double* err, double* errv) { stati
 int orc_neighbor_counts(void* h, int kind, int body, unsigned* out) { return static_cast<OracleBase*>(h)->neighbor_counts(kind, body, out); }
 int orc_neighbor_list(void* h, int kind, int body, size_t i, unsigned* out, int cap) { return static_cast<OracleBase*>(h)->neighbor_list(kind, body, i, out, cap); }
 int orc_set_state(void* h, const char* name, const double* in) { return static_cast<OracleBase*>(h)->set_state(name, in); }
+
+// Exact adjoint: record tapes while stepping, optionally with fixed iteration
+// counts, then run the reverse pass (dfsph_adjoint.hpp).
+int orc_set_record(void* h, int on) { static_cast<OracleBase*>(h)->set_record(on); return 0; }
+int orc_set_fixed_iterations(void* h, int n, const int* iv, const int* ip) {
+  static_cast<OracleBase*>(h)->set_fixed_iterations(n, iv, ip);
+  return 0;
+}
+int orc_recorded_iterations(void* h, int* iv, int* ip, int cap) {
+  return static_cast<OracleBase*>(h)->recorded_iterations(iv, ip, cap);
+}
+int orc_tape_field(void* h, int step, const char* name, int iter, double* out) {
+  return static_cast<OracleBase*>(h)->tape_field(step, name, iter, out);
+}
+int orc_adjoint(void* h, const double* seed, double* out) {
+  try {
+    static_cast<OracleBase*>(h)->adjoint(seed, out);
+  } catch (const std::exception& e) { g_err = e.what(); return -1; }
+  return 0;
+}
 
 // Closed-form probes of the cubic kernel for the kernel tests.
 void orc_kernel_eval(int variant_double, double radius, const double* r, int n, double* W, double* gW, double* ggW) {

@@ -72,6 +72,12 @@ def lib():
         L.orc_set_state.argtypes = [P, C.c_char_p, dp]
         L.orc_kernel_eval.argtypes = [C.c_int, C.c_double, dp, C.c_int, dp, dp, dp]
         L.orc_grad_utils.argtypes = [dp, dp, dp, dp, dp, dp]
+        ip = C.POINTER(C.c_int)
+        L.orc_set_record.argtypes = [P, C.c_int]
+        L.orc_set_fixed_iterations.argtypes = [P, C.c_int, ip, ip]
+        L.orc_recorded_iterations.argtypes = [P, ip, ip, C.c_int]
+        L.orc_adjoint.argtypes = [P, dp, dp]
+        L.orc_tape_field.argtypes = [P, C.c_int, C.c_char_p, C.c_int, dp]
         assert L.orc_params_size() == C.sizeof(Params), "oracle Params layout mismatch"
         _lib = L
     return _lib
@@ -131,6 +137,48 @@ class OracleSim:
         if getattr(self, "h", None):
             lib().orc_destroy(self.h)
             self.h = None
+
+    def set_init_velocity(self, body: int, v0, w0):
+        v = np.ascontiguousarray(v0, np.float64)
+        w = np.ascontiguousarray(w0, np.float64)
+        lib().orc_set_init_velocity(self.h, body, _dp(v), _dp(w))
+
+    def set_record(self, on: bool = True):
+        """Record a tape of every following step for adjoint()."""
+        lib().orc_set_record(self.h, int(on))
+
+    def set_fixed_iterations(self, iv, ip):
+        """Run step s with exactly iv[s] divergence and ip[s] pressure iterations."""
+        a = np.ascontiguousarray(iv, np.int32)
+        b = np.ascontiguousarray(ip, np.int32)
+        assert a.shape == b.shape
+        lib().orc_set_fixed_iterations(self.h, a.shape[0], a.ctypes.data_as(C.POINTER(C.c_int)),
+                                       b.ctypes.data_as(C.POINTER(C.c_int)))
+
+    def recorded_iterations(self):
+        cap = 100000
+        a = np.zeros(cap, np.int32)
+        b = np.zeros(cap, np.int32)
+        n = lib().orc_recorded_iterations(self.h, a.ctypes.data_as(C.POINTER(C.c_int)),
+                                          b.ctypes.data_as(C.POINTER(C.c_int)), cap)
+        return a[:n].copy(), b[:n].copy()
+
+    def tape_field(self, step: int, name: str, it: int = 0) -> np.ndarray:
+        out = np.zeros(self.n, np.float64)
+        n = lib().orc_tape_field(self.h, step, name.encode(), it, _dp(out))
+        if n < 0:
+            raise KeyError(name)
+        return out[:n]
+
+    def adjoint(self, seed: np.ndarray) -> np.ndarray:
+        """Reverse pass over the recorded steps.  seed: (n_bodies, 7) adjoint of
+        (x_rb, q(w,x,y,z)) after the last step; returns (n_bodies, 6) adjoint of
+        (v0, omega0)."""
+        sd = np.ascontiguousarray(seed, np.float64).reshape(self.n_bodies, 7)
+        out = np.zeros((self.n_bodies, 6), np.float64)
+        if lib().orc_adjoint(self.h, _dp(sd), _dp(out)) != 0:
+            raise RuntimeError("orc_adjoint failed")
+        return out
 
     def step(self, n: int = 1):
         lib().orc_step(self.h, n)

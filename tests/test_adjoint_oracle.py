@@ -1,0 +1,81 @@
+"""Pins the oracle's exact adjoint (oracle/dfsph_adjoint.hpp) on the CPU.
+
+* central finite differences of the FP64 oracle with the Jacobi iteration
+  counts held at the recorded values (the map is then smooth wherever no clamp
+  or mask flips inside the difference interval; over these horizons none does);
+* the fluid-free closed form dx_N/dv0 = N dt I (two release-window steps
+  included: both override v with v0, so every step moves the body by dt v0);
+* FP32 tapes against FP64 tapes (the GPU computes in FP32).
+"""
+import numpy as np
+import pytest
+
+from difffr_amd import scenes
+from oracle import pyoracle as po
+
+
+def _run(scene, variant, v0, w0, steps, fix=None, record=False):
+    s = po.OracleSim(scene, variant)
+    s.set_init_velocity(1, v0, w0)
+    if fix is not None:
+        s.set_fixed_iterations(*fix)
+    if record:
+        s.set_record(True)
+    s.step(steps)
+    r = s.rigid(1)
+    return s, np.concatenate([r["x"], r["q"]])
+
+
+def _adjoint_and_fd(scene, steps, seed, eps=1e-6):
+    b = scene.bodies[1]
+    s, _ = _run(scene, "f64", b.v0, b.w0, steps, record=True)
+    fix = s.recorded_iterations()
+    sd = np.zeros((len(scene.bodies), 7))
+    sd[1] = seed
+    g = s.adjoint(sd)[1]
+    p0 = np.concatenate([b.v0, b.w0])
+    fd = np.zeros(6)
+    for k in range(6):
+        p = p0.copy(); p[k] += eps
+        m = p0.copy(); m[k] -= eps
+        _, a = _run(scene, "f64", p[:3], p[3:], steps, fix)
+        _, c = _run(scene, "f64", m[:3], m[3:], steps, fix)
+        fd[k] = seed @ (a - c) / (2 * eps)
+    return g, fd
+
+
+@pytest.mark.parametrize("body", ["sphere", "ellipsoid"])
+def test_adjoint_matches_central_differences(body):
+    sc = scenes.submerged_body(body=body, steps=15)
+    seed = np.random.default_rng(1).normal(size=7)
+    g, fd = _adjoint_and_fd(sc, 15, seed)
+    scale = np.abs(fd).max()
+    assert scale > 1e-4  # the coupling is active: the gradient is far from the fluid-free one
+    np.testing.assert_allclose(g, fd, rtol=0, atol=1e-6 * scale)
+
+
+def test_adjoint_free_flight_closed_form():
+    sc = scenes.submerged_body(steps=12)
+    sc.fluid_x = sc.fluid_x[:0]
+    sc.fluid_v = sc.fluid_v[:0]
+    sc.bodies = sc.bodies[1:]  # the body alone
+    sc.bodies.insert(0, scenes.Body(np.zeros((0, 3)), False, np.zeros(3), np.array([1.0, 0, 0, 0]), 1.0))
+    n, dt = 12, sc.params["dt"]
+    b = sc.bodies[1]
+    s, _ = _run(sc, "f64", b.v0, b.w0, n, record=True)
+    for k in range(3):
+        sd = np.zeros((2, 7)); sd[1, k] = 1.0
+        g = s.adjoint(sd)[1]
+        expect = np.zeros(6); expect[k] = n * dt
+        np.testing.assert_allclose(g, expect, atol=1e-12)
+
+
+def test_adjoint_fp32_tapes_track_fp64():
+    sc = scenes.submerged_body(body="ellipsoid", steps=20)
+    b = sc.bodies[1]
+    seed = np.zeros((2, 7)); seed[1] = np.random.default_rng(2).normal(size=7)
+    s64, _ = _run(sc, "f64", b.v0, b.w0, 20, record=True)
+    s32, _ = _run(sc, "f32", b.v0, b.w0, 20, fix=s64.recorded_iterations(), record=True)
+    g64 = s64.adjoint(seed)[1]
+    g32 = s32.adjoint(seed)[1]
+    np.testing.assert_allclose(g32, g64, rtol=0, atol=1e-3 * np.abs(g64).max())

@@ -23,11 +23,14 @@ using namespace dfr;
 namespace {
 
 enum TimerClass {
-  TC_SORT = 0, TC_NBR, TC_DENSITY, TC_VUPDATE, TC_RECOMPUTE, TC_PWARM, TC_ELEMWISE, TC_GRADIENT, TC_RIGID, TC_COUNT
+  TC_SORT = 0, TC_NBR, TC_DENSITY, TC_VUPDATE, TC_RECOMPUTE, TC_PWARM, TC_ELEMWISE, TC_GRADIENT, TC_RIGID,
+  TC_ADJ_UPDATE, TC_ADJ_DELTA, TC_ADJ_FACTOR, TC_ADJ_MISC, TC_CHECKPOINT, TC_COUNT
 };
 const char* kTimerNames[TC_COUNT] = {"grid_sort", "neighbor_build", "density_factor", "velocity_update",
                                      "density_recompute", "pressure_warmstart", "advect_position",
-                                     "gradient_pairs", "rigid_step"};
+                                     "gradient_pairs", "rigid_step", "adj_velocity_update",
+                                     "adj_density_change", "adj_density_factor", "adj_rigid_misc",
+                                     "checkpoint_copy"};
 
 struct BodyInfo {
   bool dynamic = false;
@@ -38,6 +41,53 @@ struct BodyInfo {
   double v0[3] = {0, 0, 0}, w0[3] = {0, 0, 0};
   std::vector<float> x0;    // local samples
   bool animated = false;
+};
+
+// State of the simulation at the start of a step (what dfr_step reads), in one
+// device block: fluid in its current order, boundary samples, rigid bodies.
+struct Snapshot {
+  void* block = nullptr;
+  float4 *pos = nullptr, *vel = nullptr, *bpos = nullptr, *bvel = nullptr;
+  float *kappa = nullptr, *kappav = nullptr;
+  uint32_t* id = nullptr;
+  RigidDev* rig = nullptr;
+  float time = 0.f;
+  int64_t step_count = 0;
+  std::vector<uint8_t> animated;
+};
+
+// What the reverse pass of one step reads besides the step's live buffers
+// (neighbour lists, grid, permutation, density, factor, sum_grad_p_k): the
+// values that the step overwrote while it ran.
+struct StepTapeDev {
+  float4* x = nullptr;                    // fluid positions during the step (sorted order)
+  float *kvPrev = nullptr, *d0 = nullptr, *kvW = nullptr;  // divergence warm start
+  float *kPrev = nullptr, *rsW = nullptr, *kW = nullptr;   // pressure warm start
+  float4 *bpos = nullptr, *bvel = nullptr;  // boundary samples at the start of the step
+  RigidDev *rig_pre = nullptr, *rig_forced = nullptr;  // before / after the release-window override
+  std::vector<float4*> vD, vP;            // velocity at the start of each Jacobi iteration
+  std::vector<float*> kD, bD, kP, bP;     // k_i and d / rho* of each iteration
+  int Iv = 0, Ip = 0;
+  AdjStepTable tbl{};
+};
+
+// Exact adjoint of a rollout: checkpoints every k steps of the primal run and
+// the reverse pass's working buffers.
+struct AdjState {
+  bool active = false;
+  int k = 1;
+  int64_t steps = 0;                       // primal steps since dfr_adjoint_begin
+  std::vector<Snapshot> ckpt;              // state before steps 0, k, 2k, ...
+  std::vector<Snapshot> seg;               // states inside the segment being reversed
+  Snapshot final_state;
+  std::vector<int> iv, ip;                 // iteration counts of the primal steps
+  bool bufs = false;
+  float4 *xb = nullptr, *vb = nullptr, *xb2 = nullptr, *vb2 = nullptr;
+  float *kb = nullptr, *kvb = nullptr, *kb2 = nullptr, *kvb2 = nullptr;
+  float *dbar = nullptr, *alb = nullptr, *rhob = nullptr;
+  double *sxb = nullptr, *svb = nullptr;
+  AdjBodyDev* body = nullptr;
+  StepTapeDev tape;
 };
 
 }  // namespace
@@ -82,6 +132,14 @@ struct dfr_ctx {
   int64_t t_cnt[TC_COUNT] = {0};
 
   std::vector<void*> allocations;
+
+  // exact adjoint
+  AdjState adj;
+  StepTapeDev* rec = nullptr;     // step being recorded for the reverse pass
+  bool replay = false;            // re-running steps for the reverse pass
+  int force_iv = -1, force_ip = -1;  // iteration counts imposed on the next step
+  std::vector<int> fixed_iv, fixed_ip;  // dfr_set_fixed_iterations
+  size_t fixed_pos = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -280,6 +338,14 @@ void begin_step(dfr_ctx* c) {
     anim[k] = bi.animated ? 1 : 0;
   }
   launch_begin_step(c->d_rig, nb, ramp.data(), fac.data(), anim.data(), c->stream);
+  if (c->rec) {
+    AdjStepTable& t = c->rec->tbl;
+    for (int k = 0; k < DFR_MAX_BODIES; ++k) {
+      t.ramp[k] = k < nb ? ramp[k] : 0;
+      t.animated[k] = k < nb ? anim[k] : 0;
+      t.factor[k] = k < nb ? fac[k] : 0.0;
+    }
+  }
 }
 
 void check_overflow(dfr_ctx* c) {
@@ -303,7 +369,7 @@ double read_err(dfr_ctx* c, int slot) {
 }
 
 void run_gradient(dfr_ctx* c, int mode) {
-  if (!c->P.compute_gradient) return;
+  if (!c->P.compute_gradient || c->replay) return;
   GradConst gc{};
   gc.mode = mode;
   const float invH = 1.f / c->dtf;
@@ -320,14 +386,51 @@ void run_gradient(dfr_ctx* c, int mode) {
   }
 }
 
+template <typename T> void d2d(dfr_ctx* c, T* dst, const T* src, size_t n) {
+  if (n > 0) HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+}
+
+// Per-iteration tape buffers, grown on first use.
+void tape_iteration(dfr_ctx* c, std::vector<float4*>& v, std::vector<float*>& k, std::vector<float*>& b, int m) {
+  const size_t n = (size_t)std::max(c->f.n, 1);
+  while ((int)v.size() <= m) {
+    v.push_back(dalloc<float4>(c, n));
+    k.push_back(dalloc<float>(c, n));
+    b.push_back(dalloc<float>(c, n));
+  }
+  d2d(c, v[m], c->f.vel, c->f.n);
+  d2d(c, k[m], c->f.kf, c->f.n);
+  d2d(c, b[m], c->f.badv, c->f.n);
+}
+
+// Iteration count imposed on the current step (-1: run to convergence).
+int imposed(dfr_ctx* c, bool pressure) {
+  const int forced = pressure ? c->force_ip : c->force_iv;
+  if (forced >= 0) return forced;
+  const std::vector<int>& v = pressure ? c->fixed_ip : c->fixed_iv;
+  return c->fixed_pos < v.size() ? v[c->fixed_pos] : -1;
+}
+
 void do_step(dfr_ctx* c) {
   const float dt = c->dtf;
   const int nf = c->f.n;
+  StepTapeDev* rec = c->rec;
+  if (rec) {
+    d2d(c, rec->rig_pre, c->d_rig, c->bodies.size());
+    d2d(c, rec->bpos, c->b.pos, c->b.n);
+    d2d(c, rec->bvel, c->b.vel, c->b.n);
+  }
   begin_step(c);
+  if (rec) d2d(c, rec->rig_forced, c->d_rig, c->bodies.size());
   {
     Timed t(c, TC_SORT);
     if (nf > 0) launch_sort_fluid(c->f, c->grid, c->gb, c->stream);
     launch_sort_boundary(c->b, c->grid, c->gb, c->stream);
+  }
+  if (rec) {
+    d2d(c, rec->x, c->f.pos, nf);
+    d2d(c, rec->kvPrev, c->f.kappav, nf);
+    d2d(c, rec->kPrev, c->f.kappa, nf);
   }
   {
     Timed t(c, TC_NBR);
@@ -348,6 +451,11 @@ void do_step(dfr_ctx* c) {
     launch_density_factor_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, (divOn && warmV) ? 1 : 0, divOn ? 1 : 0,
                             c->stream);
   }
+  if (rec && nf > 0) {
+    d2d(c, rec->d0, c->f.badv, nf);
+    d2d(c, rec->kvW, c->f.kappav, nf);
+  }
+  const int fixV = imposed(c, false), fixP = imposed(c, true);
   // ---- divergence solve (TimeStepDiffDFSPH::divergenceSolve, :770-881)
   int itv = 0;
   double avgv = 0.0;
@@ -364,8 +472,9 @@ void do_step(dfr_ctx* c) {
     }
     const double eta = (1.0f / dt) * (float)c->P.max_error_v * 0.01f * c->sc.rho0;
     bool chk = false;
-    while ((!chk || itv < 1) && itv < c->P.max_iterations_v) {
+    while (fixV >= 0 ? itv < fixV : ((!chk || itv < 1) && itv < c->P.max_iterations_v)) {
       run_gradient(c, 1);
+      if (rec) tape_iteration(c, rec->vD, rec->kD, rec->bD, itv);
       {
         Timed t(c, TC_VUPDATE);
         launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kf,
@@ -373,10 +482,13 @@ void do_step(dfr_ctx* c) {
       }
       {
         Timed t(c, TC_RECOMPUTE);
-        launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_DIV, invH, c->d_err + itv, c->stream);
+        launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_DIV, invH,
+                           c->d_err + std::min(itv, c->P.max_iterations_v - 1), c->stream);
       }
-      avgv = read_err(c, itv) / (double)nf;
-      chk = avgv <= eta;
+      if (fixV < 0) {
+        avgv = read_err(c, itv) / (double)nf;
+        chk = avgv <= eta;
+      }
       ++itv;
     }
   }
@@ -394,6 +506,10 @@ void do_step(dfr_ctx* c) {
         Timed t(c, TC_PWARM);
         launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, PRESSURE_WARMSTART, 0.f, nullptr, c->stream);
       }
+      if (rec) {
+        d2d(c, rec->rsW, c->f.badv, nf);
+        d2d(c, rec->kW, c->f.kappa, nf);
+      }
       Timed t(c, TC_VUPDATE);
       launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kappa, nullptr, c->stream);
     }
@@ -405,8 +521,9 @@ void do_step(dfr_ctx* c) {
     const double eta = (float)c->P.max_error * 0.01f * c->sc.rho0;
     const int base = c->P.max_iterations_v;
     bool chk = false;
-    while ((!chk || it < c->P.min_iterations) && it < c->P.max_iterations) {
+    while (fixP >= 0 ? it < fixP : ((!chk || it < c->P.min_iterations) && it < c->P.max_iterations)) {
       run_gradient(c, 0);
+      if (rec) tape_iteration(c, rec->vP, rec->kP, rec->bP, it);
       {
         Timed t(c, TC_VUPDATE);
         launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kf,
@@ -414,11 +531,13 @@ void do_step(dfr_ctx* c) {
       }
       {
         Timed t(c, TC_RECOMPUTE);
-        launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_PRESSURE, invH2, c->d_err + base + it,
-                           c->stream);
+        launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_PRESSURE, invH2,
+                           c->d_err + base + std::min(it, c->P.max_iterations - 1), c->stream);
       }
-      avg = read_err(c, base + it) / (double)nf;
-      chk = avg <= eta;
+      if (fixP < 0) {
+        avg = read_err(c, base + it) / (double)nf;
+        chk = avg <= eta;
+      }
       ++it;
     }
     Timed t(c, TC_ELEMWISE);
@@ -434,12 +553,14 @@ void do_step(dfr_ctx* c) {
     rc.gradient_mode = c->P.gradient_mode;
     rc.optimize_rotation = c->P.optimize_rotation;
     rc.gyroscopic = c->P.gyroscopic;
-    rc.compute_gradient = c->P.compute_gradient;
+    rc.compute_gradient = c->replay ? 0 : c->P.compute_gradient;
     rc.nbodies = (int)c->bodies.size();
     launch_rigid_step(rc, c->d_rig, c->stream);
     launch_boundary_transform(c->b.dyn_idx, c->b.nd, c->b, c->d_rig, c->stream);
   }
   c->step_count++;
+  if (rec) { rec->Iv = itv; rec->Ip = it; }
+  if (c->fixed_pos < c->fixed_iv.size()) c->fixed_pos++;
   c->stats.iterations = it;
   c->stats.iterations_v = itv;
   c->stats.avg_density_err = avg;
@@ -465,6 +586,188 @@ void download_unpermute(const dfr_ctx* c, const T* dsrc, int comps_in, int comps
     for (int q = 0; q < comps_out; ++q) out[(int64_t)id[k] * comps_out + q] = v[q];
   }
   (void)comps_in;
+}
+
+// ---------------------------------------------------------------------------
+// Exact adjoint: checkpoints, record-replay and the reverse pass of a step.
+// ---------------------------------------------------------------------------
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+void snapshot_alloc(dfr_ctx* c, Snapshot& S) {
+  if (S.block) return;
+  const size_t nf = (size_t)std::max(c->f.n, 1), nb = (size_t)std::max(c->b.n, 1),
+               nr = (size_t)std::max<size_t>(c->bodies.size(), 1);
+  const size_t sz[7] = {align256(nf * 16), align256(nf * 16), align256(nf * 4), align256(nf * 4),
+                        align256(nf * 4), align256(nb * 16) * 2, align256(nr * sizeof(RigidDev))};
+  size_t total = 0;
+  for (size_t v : sz) total += v;
+  HIPCHK(hipMalloc(&S.block, total));
+  char* p = static_cast<char*>(S.block);
+  S.pos = reinterpret_cast<float4*>(p); p += sz[0];
+  S.vel = reinterpret_cast<float4*>(p); p += sz[1];
+  S.kappa = reinterpret_cast<float*>(p); p += sz[2];
+  S.kappav = reinterpret_cast<float*>(p); p += sz[3];
+  S.id = reinterpret_cast<uint32_t*>(p); p += sz[4];
+  S.bpos = reinterpret_cast<float4*>(p); p += sz[5] / 2;
+  S.bvel = reinterpret_cast<float4*>(p); p += sz[5] / 2;
+  S.rig = reinterpret_cast<RigidDev*>(p);
+}
+
+void snapshot_free(Snapshot& S) {
+  if (S.block) (void)hipFree(S.block);
+  S = Snapshot{};
+}
+
+void snapshot_save(dfr_ctx* c, Snapshot& S) {
+  Timed t(c, TC_CHECKPOINT);
+  snapshot_alloc(c, S);
+  const int nf = c->f.n;
+  d2d(c, S.pos, c->f.pos, nf); d2d(c, S.vel, c->f.vel, nf);
+  d2d(c, S.kappa, c->f.kappa, nf); d2d(c, S.kappav, c->f.kappav, nf); d2d(c, S.id, c->f.id, nf);
+  d2d(c, S.bpos, c->b.pos, c->b.n); d2d(c, S.bvel, c->b.vel, c->b.n);
+  d2d(c, S.rig, c->d_rig, c->bodies.size());
+  S.time = c->time;
+  S.step_count = c->step_count;
+  S.animated.resize(c->bodies.size());
+  for (size_t k = 0; k < c->bodies.size(); ++k) S.animated[k] = c->bodies[k].animated ? 1 : 0;
+}
+
+void snapshot_restore(dfr_ctx* c, const Snapshot& S) {
+  Timed t(c, TC_CHECKPOINT);
+  const int nf = c->f.n;
+  d2d(c, c->f.pos, S.pos, nf); d2d(c, c->f.vel, S.vel, nf);
+  d2d(c, c->f.kappa, S.kappa, nf); d2d(c, c->f.kappav, S.kappav, nf); d2d(c, c->f.id, S.id, nf);
+  d2d(c, c->b.pos, S.bpos, c->b.n); d2d(c, c->b.vel, S.bvel, c->b.n);
+  d2d(c, c->d_rig, S.rig, c->bodies.size());
+  c->time = S.time;
+  c->step_count = S.step_count;
+  for (size_t k = 0; k < c->bodies.size(); ++k) c->bodies[k].animated = S.animated[k] != 0;
+}
+
+void adjoint_buffers(dfr_ctx* c) {
+  AdjState& A = c->adj;
+  if (A.bufs) return;
+  const size_t nf = (size_t)std::max(c->f.n, 1), nb = (size_t)std::max(c->b.n, 1),
+               nd = (size_t)std::max(c->b.nd, 1), nr = (size_t)std::max<size_t>(c->bodies.size(), 1);
+  A.xb = dalloc<float4>(c, nf); A.vb = dalloc<float4>(c, nf); A.xb2 = dalloc<float4>(c, nf); A.vb2 = dalloc<float4>(c, nf);
+  A.kb = dalloc<float>(c, nf); A.kvb = dalloc<float>(c, nf); A.kb2 = dalloc<float>(c, nf); A.kvb2 = dalloc<float>(c, nf);
+  A.dbar = dalloc<float>(c, nf); A.alb = dalloc<float>(c, nf); A.rhob = dalloc<float>(c, nf);
+  A.sxb = dalloc<double>(c, 3 * nd); A.svb = dalloc<double>(c, 3 * nd);
+  A.body = dalloc<AdjBodyDev>(c, nr);
+  StepTapeDev& T = A.tape;
+  T.x = dalloc<float4>(c, nf);
+  T.kvPrev = dalloc<float>(c, nf); T.d0 = dalloc<float>(c, nf); T.kvW = dalloc<float>(c, nf);
+  T.kPrev = dalloc<float>(c, nf); T.rsW = dalloc<float>(c, nf); T.kW = dalloc<float>(c, nf);
+  T.bpos = dalloc<float4>(c, nb); T.bvel = dalloc<float4>(c, nb);
+  T.rig_pre = dalloc<RigidDev>(c, nr); T.rig_forced = dalloc<RigidDev>(c, nr);
+  set_adjoint_kernel_attributes();
+  A.bufs = true;
+}
+
+void adjoint_release(dfr_ctx* c) {
+  AdjState& A = c->adj;
+  for (auto& S : A.ckpt) snapshot_free(S);
+  for (auto& S : A.seg) snapshot_free(S);
+  snapshot_free(A.final_state);
+  A.ckpt.clear();
+  A.seg.clear();
+  A.iv.clear();
+  A.ip.clear();
+  A.steps = 0;
+  A.active = false;
+}
+
+// Reverse pass of the step just replayed with c->rec = &tape: the adjoint
+// fields (sorted order of that step) become the adjoint of the state before
+// the step (order of the previous step), and the rigid adjoints move back one
+// step.
+void reverse_step(dfr_ctx* c) {
+  AdjState& A = c->adj;
+  const StepTapeDev& T = A.tape;
+  const int nf = c->f.n;
+  const float dt = c->dtf;
+  const int nbodies = (int)c->bodies.size();
+  AdjPass a{};
+  a.c = c->sc; a.g = c->grid; a.fstart = c->gb.fstart; a.x = T.x;
+  a.nbr8 = c->nb.nbr8; a.nbrB = c->nb.nbrB; a.cnt = c->nb.cnt;
+  a.bx = T.bpos; a.bv = T.bvel; a.bx0 = c->b.x0; a.nb_static = c->b.n - c->b.nd;
+  a.sxb = A.sxb; a.svb = A.svb; a.body = A.body;
+  {
+    Timed t(c, TC_ADJ_MISC);
+    RigidStepConst rc{};
+    rc.dt = (double)dt;
+    rc.gyroscopic = c->P.gyroscopic;
+    rc.nbodies = nbodies;
+    launch_adj_rigid(rc, T.tbl, T.rig_forced, T.rig_pre, c->d_rig, A.body, nbodies, c->stream);
+    if (c->b.nd > 0) {
+      HIPCHK(hipMemsetAsync(A.sxb, 0, sizeof(double) * 3 * c->b.nd, c->stream));
+      HIPCHK(hipMemsetAsync(A.svb, 0, sizeof(double) * 3 * c->b.nd, c->stream));
+    }
+  }
+  if (nf > 0) {
+    {
+      Timed t(c, TC_ADJ_MISC);
+      launch_adj_position(nf, dt, A.xb, A.vb, c->stream);
+      HIPCHK(hipMemsetAsync(A.alb, 0, sizeof(float) * nf, c->stream));
+      HIPCHK(hipMemsetAsync(A.rhob, 0, sizeof(float) * nf, c->stream));
+    }
+    const bool warmP = c->P.use_pressure_warmstart != 0;
+    const float h2 = dt * dt;
+    for (int m = T.Ip - 1; m >= 0; --m) {
+      {
+        Timed t(c, TC_ADJ_UPDATE);
+        launch_adj_update(a, ADJ_PRESSURE_ITER, T.kP[m], A.vb, warmP ? A.kb : nullptr, h2, T.bP[m], nullptr,
+                          c->f.alpha, A.dbar, A.alb, A.rhob, A.xb, c->stream);
+      }
+      Timed t(c, TC_ADJ_DELTA);
+      launch_adj_delta(a, T.vP[m], A.dbar, A.vb, A.xb, c->stream);
+    }
+    if (warmP) {
+      Timed t(c, TC_ADJ_UPDATE);
+      launch_adj_update(a, ADJ_PRESSURE_WARM, T.kW, A.vb, A.kb, h2, T.rsW, T.kPrev, c->f.alpha, A.kb, A.alb,
+                        A.rhob, A.xb, c->stream);
+    }
+    if (c->P.enable_divergence_solver) {
+      const bool warmV = c->P.use_divergence_warmstart != 0;
+      for (int m = T.Iv - 1; m >= 0; --m) {
+        {
+          Timed t(c, TC_ADJ_UPDATE);
+          launch_adj_update(a, ADJ_DIV_ITER, T.kD[m], A.vb, warmV ? A.kvb : nullptr, dt, T.bD[m], nullptr,
+                            c->f.alpha, A.dbar, A.alb, A.rhob, A.xb, c->stream);
+        }
+        Timed t(c, TC_ADJ_DELTA);
+        launch_adj_delta(a, T.vD[m], A.dbar, A.vb, A.xb, c->stream);
+      }
+      if (warmV) {
+        Timed t(c, TC_ADJ_UPDATE);
+        launch_adj_update(a, ADJ_DIV_WARM, T.kvW, A.vb, A.kvb, dt, T.d0, T.kvPrev, c->f.alpha, A.kvb, A.alb,
+                          A.rhob, A.xb, c->stream);
+      }
+    }
+    {
+      Timed t(c, TC_ADJ_FACTOR);
+      launch_adj_factor(a, c->f.alpha, A.alb, A.rhob, c->f.s, A.xb, c->stream);
+    }
+  }
+  Timed t(c, TC_ADJ_MISC);
+  for (int k = 0; k < nbodies; ++k) {
+    const BodyInfo& bi = c->bodies[k];
+    if (bi.dynamic && bi.n > 0) launch_adj_boundary_reduce(a, k, (int)bi.dyn_first, (int)bi.n, A.body, c->stream);
+  }
+  if (nf > 0) {
+    launch_adj_unpermute(nf, c->gb.perm, A.xb, A.vb, A.kb, A.kvb, A.xb2, A.vb2, A.kb2, A.kvb2, c->stream);
+    std::swap(A.xb, A.xb2); std::swap(A.vb, A.vb2); std::swap(A.kb, A.kb2); std::swap(A.kvb, A.kvb2);
+  }
+}
+
+// One recorded step s of the rollout: impose the primal's iteration counts.
+void replay_step(dfr_ctx* c, int64_t s, bool record) {
+  c->force_iv = c->adj.iv[s];
+  c->force_ip = c->adj.ip[s];
+  c->rec = record ? &c->adj.tape : nullptr;
+  do_step(c);
+  c->rec = nullptr;
+  c->force_iv = c->force_ip = -1;
 }
 
 }  // namespace
@@ -535,6 +838,7 @@ int dfr_destroy(dfr_ctx* c) {
   if (!c) return DFR_ERR_ARG;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  adjoint_release(c);
   for (auto& r : c->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   for (void* p : c->allocations) (void)hipFree(p);
@@ -584,6 +888,7 @@ int dfr_set_init_velocity(dfr_ctx* c, int body, const double v0[3], const double
     BodyInfo& bi = c->bodies[body];
     for (int k = 0; k < 3; ++k) { bi.v0[k] = v0[k]; bi.w0[k] = w0[k]; }
     if (c->initialized) {
+      adjoint_release(c);
       HIPCHK(hipMemcpy(&c->d_rig[body].v0[0], v0, 3 * sizeof(double), hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(&c->d_rig[body].w0[0], w0, 3 * sizeof(double), hipMemcpyHostToDevice));
       for (int k = 0; k < 3; ++k) { c->h_rig_init[body].v0[k] = v0[k]; c->h_rig_init[body].w0[k] = w0[k]; }
@@ -697,7 +1002,19 @@ int dfr_step(dfr_ctx* c, int nsteps) {
   return guarded(c, [&]() {
     if (!c->initialized) return fail(c, DFR_ERR_STATE, "dfr_step before dfr_init");
     HIPCHK(hipSetDevice(c->device));
-    for (int s = 0; s < nsteps; ++s) do_step(c);
+    AdjState& A = c->adj;
+    for (int s = 0; s < nsteps; ++s) {
+      if (A.active && A.steps % A.k == 0) {
+        A.ckpt.emplace_back();
+        snapshot_save(c, A.ckpt.back());
+      }
+      do_step(c);
+      if (A.active) {
+        A.iv.push_back(c->stats.iterations_v);
+        A.ip.push_back(c->stats.iterations);
+        A.steps++;
+      }
+    }
     sync_check(c);
     check_overflow(c);
     if (c->profiling) drain_timers(c);
@@ -708,6 +1025,7 @@ int dfr_step(dfr_ctx* c, int nsteps) {
 int dfr_reset(dfr_ctx* c) {
   return guarded(c, [&]() {
     if (!c->initialized) return fail(c, DFR_ERR_STATE, "dfr_reset before dfr_init");
+    adjoint_release(c);
     HIPCHK(hipSetDevice(c->device));
     for (auto& bi : c->bodies) bi.animated = false;
     if (c->f.n > 0) upload_fluid_initial(c);
@@ -734,6 +1052,7 @@ int dfr_get_rigid(const dfr_ctx* c, int body, double out[19]) {
 int dfr_set_rigid(dfr_ctx* c, int body, const double in[13]) {
   if (check_body(c, body) || !in) return DFR_ERR_ARG;
   return guarded(c, [&]() {
+    adjoint_release(c);
     RigidDev r;
     HIPCHK(hipMemcpy(&r, &c->d_rig[body], sizeof(RigidDev), hipMemcpyDeviceToHost));
     for (int k = 0; k < 3; ++k) { r.x[k] = in[k]; r.v[k] = in[7 + k]; r.w[k] = in[10 + k]; }
@@ -809,6 +1128,7 @@ int dfr_get_field(const dfr_ctx* c, const char* name, float* out) {
 int dfr_set_field(dfr_ctx* c, const char* name, const float* in) {
   if (!c || !c->initialized || !name || !in) return DFR_ERR_ARG;
   return guarded(c, [&]() {
+    adjoint_release(c);
     const std::string n(name);
     float* dst = nullptr;
     if (n == "kappa") dst = c->f.kappa;
@@ -945,6 +1265,101 @@ int dfr_kernel_times(const dfr_ctx* c, int n, char names[][32], double* avg_ms, 
     if (launches) launches[k] = c->t_cnt[k];
   }
   return TC_COUNT;
+}
+
+int dfr_adjoint_begin(dfr_ctx* c, int checkpoint_every) {
+  return guarded(c, [&]() {
+    if (!c->initialized) return fail(c, DFR_ERR_STATE, "dfr_adjoint_begin before dfr_init");
+    HIPCHK(hipSetDevice(c->device));
+    adjoint_release(c);
+    adjoint_buffers(c);
+    c->adj.k = checkpoint_every > 0 ? checkpoint_every : 1;
+    c->adj.active = true;
+    return DFR_OK;
+  });
+}
+
+int dfr_adjoint_backward(dfr_ctx* c, const double* seed, double* grad) {
+  return guarded(c, [&]() {
+    AdjState& A = c->adj;
+    if (!A.active || A.steps == 0) return fail(c, DFR_ERR_STATE, "dfr_adjoint_backward without a recorded rollout");
+    if (!seed || !grad) return fail(c, DFR_ERR_ARG, "dfr_adjoint_backward: null seed or grad");
+    HIPCHK(hipSetDevice(c->device));
+    const int nbodies = (int)c->bodies.size();
+    const dfr_stats saved_stats = c->stats;
+    snapshot_save(c, A.final_state);
+    std::vector<AdjBodyDev> hb(std::max(nbodies, 1));
+    std::memset(hb.data(), 0, sizeof(AdjBodyDev) * hb.size());
+    for (int b = 0; b < nbodies; ++b) {
+      for (int k = 0; k < 3; ++k) hb[b].x[k] = seed[7 * b + k];
+      for (int k = 0; k < 4; ++k) hb[b].q[k] = seed[7 * b + 3 + k];
+    }
+    HIPCHK(hipMemcpyAsync(A.body, hb.data(), sizeof(AdjBodyDev) * nbodies, hipMemcpyHostToDevice, c->stream));
+    const size_t nf = (size_t)c->f.n;
+    HIPCHK(hipMemsetAsync(A.xb, 0, sizeof(float4) * nf, c->stream));
+    HIPCHK(hipMemsetAsync(A.vb, 0, sizeof(float4) * nf, c->stream));
+    HIPCHK(hipMemsetAsync(A.kb, 0, sizeof(float) * nf, c->stream));
+    HIPCHK(hipMemsetAsync(A.kvb, 0, sizeof(float) * nf, c->stream));
+    struct ReplayFlag {
+      dfr_ctx* c;
+      explicit ReplayFlag(dfr_ctx* x) : c(x) { c->replay = true; }
+      ~ReplayFlag() { c->replay = false; c->rec = nullptr; c->force_iv = c->force_ip = -1; }
+    } flag(c);
+    const int64_t S = A.steps, k = A.k;
+    if (k > 1 && (int64_t)A.seg.size() < k) A.seg.resize(k);
+    for (int64_t seg = (S - 1) / k; seg >= 0; --seg) {
+      const int64_t s0 = seg * k, s1 = std::min(s0 + k, S);
+      if (s1 - s0 > 1) {  // states at the start of steps s0+1 .. s1-1
+        snapshot_restore(c, A.ckpt[seg]);
+        for (int64_t s = s0; s < s1 - 1; ++s) {
+          replay_step(c, s, false);
+          snapshot_save(c, A.seg[s - s0 + 1]);
+        }
+      }
+      for (int64_t s = s1 - 1; s >= s0; --s) {
+        snapshot_restore(c, s == s0 ? A.ckpt[seg] : A.seg[s - s0]);
+        replay_step(c, s, true);
+        reverse_step(c);
+      }
+      sync_check(c);
+      check_overflow(c);
+    }
+    snapshot_restore(c, A.final_state);
+    HIPCHK(hipMemcpyAsync(hb.data(), A.body, sizeof(AdjBodyDev) * nbodies, hipMemcpyDeviceToHost, c->stream));
+    sync_check(c);
+    if (c->profiling) drain_timers(c);
+    for (int b = 0; b < nbodies; ++b)
+      for (int q = 0; q < 3; ++q) { grad[6 * b + q] = hb[b].v0[q]; grad[6 * b + 3 + q] = hb[b].w0[q]; }
+    c->stats = saved_stats;
+    return DFR_OK;
+  });
+}
+
+int dfr_adjoint_end(dfr_ctx* c) {
+  return guarded(c, [&]() {
+    HIPCHK(hipSetDevice(c->device));
+    if (c->stream) HIPCHK(hipStreamSynchronize(c->stream));
+    adjoint_release(c);
+    return DFR_OK;
+  });
+}
+
+int64_t dfr_adjoint_iterations(const dfr_ctx* c, int* iv, int* ip, int64_t cap) {
+  if (!c) return DFR_ERR_ARG;
+  const int64_t n = c->adj.steps;
+  for (int64_t s = 0; s < n && s < cap; ++s) {
+    if (iv) iv[s] = c->adj.iv[s];
+    if (ip) ip[s] = c->adj.ip[s];
+  }
+  return n;
+}
+
+int dfr_set_fixed_iterations(dfr_ctx* c, int n, const int* iv, const int* ip) {
+  if (!c || n < 0 || (n > 0 && (!iv || !ip))) return DFR_ERR_ARG;
+  c->fixed_iv.assign(iv, iv + n);
+  c->fixed_ip.assign(ip, ip + n);
+  c->fixed_pos = 0;
+  return DFR_OK;
 }
 
 int64_t dfr_num_fluid(const dfr_ctx* c) { return c ? c->f.n : DFR_ERR_ARG; }

@@ -112,4 +112,63 @@ void launch_boundary_transform(const uint32_t* idx, int n, BoundaryBufs& b, cons
 void launch_begin_step(RigidDev* rig, int nbodies, const int* ramp_flags, const double* factors,
                        const int* animated, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// dfr_adjoint.hip: reverse pass of one recorded step.
+// ---------------------------------------------------------------------------
+constexpr int TILE_CAP_R = 2560;  // halo slots staged in LDS by the reverse passes
+
+// Adjoint of one rigid body during the reverse of a step (FP64).
+struct AdjBodyDev {
+  double x[3], q[4], v[3], w[3];  // adjoint of the rigid state (x_rb, q (w,x,y,z), v, omega)
+  double v0[3], w0[3];            // accumulated dL/d init_v_rb, dL/d init_omega_rb
+  double Fb[3], Tb[3];            // adjoint of the step's force / torque sums
+  double xr[3];                   // x_rb during the step (torque lever arm)
+  double wpre[3];                 // omega that produced the boundary velocities of the step
+  double q0[4];                   // q during the step
+  int dynamic, pad;
+};
+
+// Per-body flags of the step being reversed (release window of beginStep).
+struct AdjStepTable {
+  int ramp[DFR_MAX_BODIES];
+  int animated[DFR_MAX_BODIES];
+  double factor[DFR_MAX_BODIES];
+};
+
+// What every reverse gather pass reads: the step's geometry (tape positions,
+// neighbour lists, boundary state at the start of the step) and where the
+// boundary-sample adjoints go.
+struct AdjPass {
+  StepConst c;
+  GridDesc g;
+  const uint32_t* fstart;
+  const float4* x;     // fluid positions during the step (sorted order)
+  const uint4* nbr8;
+  const uint32_t* nbrB;
+  const uint32_t* cnt;
+  const float4* bx;    // boundary positions at the start of the step, w = volume
+  const float4* bv;    // boundary velocities at the start of the step
+  const float4* bx0;   // local sample, w = body index
+  int nb_static;
+  double* sxb;         // [3 * nd] adjoint of dynamic sample positions
+  double* svb;         // [3 * nd] adjoint of dynamic sample velocities
+  const AdjBodyDev* body;
+};
+
+enum AdjUpdateMode { ADJ_DIV_ITER = 0, ADJ_PRESSURE_ITER = 1, ADJ_DIV_WARM = 2, ADJ_PRESSURE_WARM = 3 };
+void launch_adj_update(const AdjPass& a, int mode, const float* k, const float4* vbar, const float* kacc,
+                       float kacc_scale, const float* bval, const float* kprev, const float* alpha, float* out,
+                       float* alpha_bar, float* rho_bar, float4* xbar, hipStream_t s);
+void launch_adj_delta(const AdjPass& a, const float4* v, const float* dbar, float4* vbar, float4* xbar,
+                      hipStream_t s);
+void launch_adj_factor(const AdjPass& a, const float* alpha, const float* alpha_bar, const float* rho_bar,
+                       const float4* G, float4* xbar, hipStream_t s);
+void launch_adj_position(int n, float dt, const float4* xbar, float4* vbar, hipStream_t s);
+void launch_adj_unpermute(int n, const uint32_t* perm, const float4* xb, const float4* vb, const float* kb,
+                          const float* kvb, float4* xb2, float4* vb2, float* kb2, float* kvb2, hipStream_t s);
+void launch_adj_boundary_reduce(const AdjPass& a, int body, int first, int n, AdjBodyDev* ab, hipStream_t s);
+void launch_adj_rigid(const RigidStepConst& rc, const AdjStepTable& t, const RigidDev* forced, const RigidDev* pre,
+                      const RigidDev* post, AdjBodyDev* ab, int nbodies, hipStream_t s);
+void set_adjoint_kernel_attributes();
+
 }  // namespace dfr

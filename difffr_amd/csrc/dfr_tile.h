@@ -1,0 +1,140 @@
+// Brick tiles shared by the forward and reverse gather passes.
+//
+// A workgroup owns one brick of 4x4x4 cells.  After the counting sort the
+// brick's particles are contiguous and all their neighbours lie in the brick's
+// 6x6x6-cell halo; tile slots number the halo's particles (halo cells in
+// (z, y, x) order, particles of a cell in sorted order).  Fluid neighbour
+// lists hold 16-bit tile slots, eight per 16-byte pack, packs stored
+// pack-major (nbr8[p * N + i]).
+#pragma once
+#include "dfr_kernels.h"
+
+namespace dfr {
+
+constexpr int HALO = 216;
+
+template <int NT> struct TileHdr {
+  uint32_t cstart[HALO];      // first particle (global sorted index) of each halo cell
+  uint32_t coff[HALO + 1];    // tile slot of each halo cell's first particle
+  uint32_t scan[NT / 64 + 1];
+  uint32_t own_begin, own_end, total;
+  int bx, by, bz;
+};
+
+template <int NT>
+__device__ inline void tile_setup(const GridDesc& g, const uint32_t* __restrict__ fstart, uint32_t brick,
+                                  TileHdr<NT>& T) {
+  const int bx = (int)(brick % (uint32_t)g.bn[0]);
+  const int by = (int)((brick / (uint32_t)g.bn[0]) % (uint32_t)g.bn[1]);
+  const int bz = (int)(brick / ((uint32_t)g.bn[0] * (uint32_t)g.bn[1]));
+  const int t = threadIdx.x;
+  uint32_t cnt = 0;
+  if (t < HALO) {
+    const int cx = 4 * bx - 1 + t % 6, cy = 4 * by - 1 + (t / 6) % 6, cz = 4 * bz - 1 + t / 36;
+    uint32_t st = 0;
+    if (cx >= 0 && cy >= 0 && cz >= 0 && cx < g.n[0] && cy < g.n[1] && cz < g.n[2]) {
+      const uint32_t key = cell_key(g, cx, cy, cz);
+      st = fstart[key];
+      cnt = fstart[key + 1] - st;
+    }
+    T.cstart[t] = st;
+  }
+  uint32_t total;
+  const uint32_t ex = block_exclusive_scan<NT>(cnt, T.scan, total);
+  if (t < HALO) T.coff[t] = ex;
+  if (t == 0) {
+    T.coff[HALO] = total;
+    T.total = total;
+    T.own_begin = fstart[brick * 64u];
+    T.own_end = fstart[brick * 64u + 64u];
+    T.bx = bx; T.by = by; T.bz = bz;
+  }
+  __syncthreads();
+}
+
+// Global sorted index of a tile slot (slot < total).
+template <int NT> __device__ inline uint32_t tile_global(const TileHdr<NT>& T, uint32_t slot) {
+  int lo = 0, hi = HALO - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (T.coff[mid] <= slot) lo = mid; else hi = mid - 1;
+  }
+  return T.cstart[lo] + (slot - T.coff[lo]);
+}
+
+// Halo cell index of the cell (cx, cy, cz) + (dx, dy, dz) for a particle of the brick.
+template <int NT>
+__device__ inline int halo_cell(const TileHdr<NT>& T, int cx, int cy, int cz, int dx, int dy, int dz) {
+  return (cz + dz - 4 * T.bz + 1) * 36 + (cy + dy - 4 * T.by + 1) * 6 + (cx + dx - 4 * T.bx + 1);
+}
+
+// Two-phase halo staging: every thread first resolves its slots to particle
+// indices (LDS-only binary searches, independent), then issues all its global
+// loads before writing LDS, so the loads are in flight together.
+template <int NT, int CAP = TILE_CAP> constexpr int per_thread() { return CAP / NT; }
+
+template <int NT, int CAP = TILE_CAP>
+__device__ inline void tile_indices(const TileHdr<NT>& T, uint32_t total, uint32_t* js) {
+#pragma unroll
+  for (int r = 0; r < per_thread<NT, CAP>(); ++r) {
+    const uint32_t s = threadIdx.x + r * NT;
+    js[r] = (s < total) ? tile_global<NT>(T, s) : 0u;
+  }
+}
+
+// Uniform early exit for bricks that own no particle (about half of a tank's
+// bricks lie above the free surface): two scalar loads, no halo setup.
+__device__ inline bool brick_empty(const uint32_t* __restrict__ fstart, uint32_t brick) {
+  return fstart[brick * 64u] == fstart[brick * 64u + 64u];
+}
+
+__device__ inline uint32_t pack_get(const uint4& v, int u) {
+  const uint32_t w = (u < 2) ? v.x : (u < 4) ? v.y : (u < 6) ? v.z : v.w;
+  return (u & 1) ? (w >> 16) : (w & 0xffffu);
+}
+
+struct PassArgs {
+  StepConst c;
+  GridDesc g;
+  const uint32_t* fstart;
+  const float4* pos;
+  float4* vel;
+  const float4* bpos;
+  const float4* bvel;
+  const float4* bx0;
+  const uint4* nbr8;
+  const uint32_t* nbrB;
+  const uint32_t* cnt;
+  int nb_static;
+};
+
+// Visits the nF fluid neighbour slots of particle i in list order.  All packs
+// are loaded up front (independent 16-byte loads).
+template <class Body>
+__device__ inline void for_fluid_slots(const uint4* __restrict__ nbr8, size_t stride, uint32_t i, uint32_t nF,
+                                       Body&& body) {
+  uint4 pk[NPACK];
+#pragma unroll
+  for (int p = 0; p < NPACK; ++p)
+    pk[p] = ((uint32_t)(p * PACK) < nF) ? nbr8[(size_t)p * stride + i] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+  for (int p = 0; p < NPACK; ++p) {
+#pragma unroll
+    for (int u = 0; u < PACK; ++u)
+      if ((uint32_t)(p * PACK + u) < nF) body(pack_get(pk[p], u));
+  }
+}
+
+// Same visit order without unrolling, for the global-memory fallback (a halo
+// beyond the LDS capacity): keeps the rare path out of the unrolled hot loop.
+template <class Body>
+__device__ inline void for_fluid_slots_rolled(const uint4* __restrict__ nbr8, size_t stride, uint32_t i,
+                                              uint32_t nF, Body&& body) {
+#pragma unroll 1
+  for (uint32_t k = 0; k < nF; ++k) {
+    const uint4 v = nbr8[(size_t)(k >> 3) * stride + i];
+    body(pack_get(v, (int)(k & 7u)));
+  }
+}
+
+}  // namespace dfr

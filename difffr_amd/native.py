@@ -21,7 +21,8 @@ EXPORTS = [
     "dfr_get_jacobians", "dfr_get_net_gradients", "dfr_reset_gradient", "dfr_get_fluid", "dfr_get_field",
     "dfr_set_field", "dfr_get_boundary_field", "dfr_get_stats", "dfr_reset_stats", "dfr_neighbor_stats",
     "dfr_get_neighbors", "dfr_set_profiling", "dfr_kernel_times", "dfr_num_fluid", "dfr_num_boundary",
-    "dfr_num_bodies",
+    "dfr_num_bodies", "dfr_adjoint_begin", "dfr_adjoint_backward", "dfr_adjoint_end", "dfr_adjoint_iterations",
+    "dfr_set_fixed_iterations",
 ]
 
 
@@ -96,6 +97,13 @@ def load(path: str | None = None):
     L.dfr_num_boundary.argtypes = [P, C.c_int]
     L.dfr_num_boundary.restype = C.c_int64
     L.dfr_num_bodies.argtypes = [P]
+    ip = C.POINTER(C.c_int32)
+    L.dfr_adjoint_begin.argtypes = [P, C.c_int]
+    L.dfr_adjoint_backward.argtypes = [P, dp, dp]
+    L.dfr_adjoint_end.argtypes = [P]
+    L.dfr_adjoint_iterations.argtypes = [P, ip, ip, C.c_int64]
+    L.dfr_adjoint_iterations.restype = C.c_int64
+    L.dfr_set_fixed_iterations.argtypes = [P, C.c_int, ip, ip]
     _lib = L
     return L
 

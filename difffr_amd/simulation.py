@@ -255,13 +255,51 @@ class Simulation:
         self._check(native.load().dfr_set_profiling(self.ctx, int(on)))
 
     def kernel_times(self) -> dict:
-        n = 16
+        n = 32
         names = (C.c_char * 32 * n)()
         avg = np.zeros(n)
         cnt = np.zeros(n, np.int64)
         k = native.load().dfr_kernel_times(self.ctx, n, C.cast(names, C.c_void_p), _dp(avg),
                                            cnt.ctypes.data_as(C.POINTER(C.c_int64)))
         return {names[i].value.decode(): (float(avg[i]), int(cnt[i])) for i in range(k)}
+
+    # -- exact adjoint --------------------------------------------------------
+    def set_init_velocity(self, body: int, v0, w0):
+        v = np.ascontiguousarray(v0, np.float64)
+        w = np.ascontiguousarray(w0, np.float64)
+        self._check(native.load().dfr_set_init_velocity(self.ctx, int(body), _dp(v), _dp(w)))
+
+    def adjoint_begin(self, checkpoint_every: int = 1):
+        """Start recording a rollout for adjoint_backward (checkpoint every k steps)."""
+        self._check(native.load().dfr_adjoint_begin(self.ctx, int(checkpoint_every)))
+
+    def adjoint_backward(self, seed: np.ndarray) -> np.ndarray:
+        """seed (n_bodies, 7): dL/d(x_rb, q (w,x,y,z)) after the last recorded step.
+        Returns (n_bodies, 6): dL/d(init_v_rb, init_omega_rb)."""
+        sd = np.ascontiguousarray(seed, np.float64).reshape(self.n_bodies, 7)
+        out = np.zeros((self.n_bodies, 6), np.float64)
+        self._check(native.load().dfr_adjoint_backward(self.ctx, _dp(sd), _dp(out)))
+        return out
+
+    def adjoint_end(self):
+        self._check(native.load().dfr_adjoint_end(self.ctx))
+
+    def adjoint_iterations(self):
+        L = native.load()
+        n = int(L.dfr_adjoint_iterations(self.ctx, None, None, 0))
+        iv = np.zeros(max(n, 1), np.int32)
+        ip = np.zeros(max(n, 1), np.int32)
+        L.dfr_adjoint_iterations(self.ctx, iv.ctypes.data_as(C.POINTER(C.c_int32)),
+                                 ip.ctypes.data_as(C.POINTER(C.c_int32)), n)
+        return iv[:n].copy(), ip[:n].copy()
+
+    def set_fixed_iterations(self, iv, ip):
+        """Run the next len(iv) steps with exactly iv[s] / ip[s] iterations."""
+        a = np.ascontiguousarray(iv, np.int32)
+        b = np.ascontiguousarray(ip, np.int32)
+        self._check(native.load().dfr_set_fixed_iterations(self.ctx, int(a.shape[0]),
+                                                            a.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                            b.ctypes.data_as(C.POINTER(C.c_int32))))
 
     # -- TimeStepDiffDFSPH-level getters -------------------------------------
     def get_boundary_model(self, i: int) -> BoundaryModel:

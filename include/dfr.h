@@ -166,6 +166,36 @@ int64_t dfr_get_neighbors(const dfr_ctx* ctx, int kind, uint32_t* counts, uint32
 int dfr_set_profiling(dfr_ctx* ctx, int on);
 int dfr_kernel_times(const dfr_ctx* ctx, int n, char names[][32], double* avg_ms, int64_t* launches);
 
+/* Exact adjoint of a rollout (no counterpart in the reference, whose only
+ * gradient is the forward-mode approximation above; this is the adjoint path a
+ * differentiable step needs, cf. TimeStepDiffDFSPH::backwardPerStep,
+ * TimeStepDiffDFSPH.cpp:493-524, which it replaces for exact gradients).
+ *
+ * dfr_adjoint_begin marks the current state as the start of a rollout; the
+ * following dfr_step calls save a checkpoint (44 bytes per fluid particle, 32
+ * per boundary sample) every `checkpoint_every` steps (<= 0: every step) and
+ * log their iteration counts.  dfr_adjoint_backward then returns
+ * grad[6*b..6*b+5] = dL/d(init_v_rb, init_omega_rb) of body b for the seed
+ * seed[7*b..7*b+6] = dL/d(x_rb, q (w,x,y,z)) of every body after the last
+ * step: each segment of checkpoint_every steps is re-run from its checkpoint,
+ * every step is re-run once more recording its Jacobi fields, and the reverse
+ * kernels run through it.  Clamps, the eps guards, the 20-neighbour rule and
+ * iteration counts are held at their values in the run (the derivative of the
+ * piecewise-smooth step).  The state after the last step is restored on
+ * return, so backward can be called again with another seed.  dfr_reset,
+ * dfr_set_rigid, dfr_set_field and dfr_set_init_velocity end the recording;
+ * dfr_adjoint_end frees the checkpoints. */
+int dfr_adjoint_begin(dfr_ctx* ctx, int checkpoint_every);
+int dfr_adjoint_backward(dfr_ctx* ctx, const double* seed, double* grad);
+int dfr_adjoint_end(dfr_ctx* ctx);
+/* Iteration counts (divergence, pressure) of the steps recorded since
+ * dfr_adjoint_begin; returns the number of steps. */
+int64_t dfr_adjoint_iterations(const dfr_ctx* ctx, int* iv, int* ip, int64_t cap);
+
+/* Imposes exactly iv[s] divergence and ip[s] pressure iterations on the next
+ * n steps (n = 0 clears), e.g. to replay another run's iteration counts. */
+int dfr_set_fixed_iterations(dfr_ctx* ctx, int n, const int* iv, const int* ip);
+
 /* Number of fluid particles / boundary samples / bodies. */
 int64_t dfr_num_fluid(const dfr_ctx* ctx);
 int64_t dfr_num_boundary(const dfr_ctx* ctx, int body);

@@ -14,8 +14,9 @@
 // update (RigidBody3dBoundarySimulator.cpp:278-344) and the boundary transform
 // (SimulatorBase.cpp:1827-1858).
 //
-// The loops scatter (pair i -> j) and run serially in double precision; the
-// GPU reverse gathers instead, so the two share no arithmetic order.
+// The loops scatter (pair i -> j) in double precision into per-thread
+// accumulators (OpenMP), merged after every pass; the GPU reverse gathers
+// instead, so the two share no arithmetic order.
 #pragma once
 #include "dfsph_oracle.hpp"
 
@@ -36,8 +37,55 @@ template <typename RF, typename RR> struct Adjoint {
   // per-step scratch
   std::vector<std::vector<V>> sx, sv;  // adjoints of boundary sample positions / velocities
   std::vector<V> Fb, Tb;               // adjoints of the step's force / torque per body
+  // per-thread accumulators of a scatter pass (kept zero between passes)
+  struct Acc {
+    std::vector<V> x, v;
+    std::vector<A> k;
+    std::vector<std::vector<V>> sx, sv;
+  };
+  std::vector<Acc> acc;
 
   explicit Adjoint(const Sim<RF, RR>& s) : S(s) {}
+
+  // Runs f(i, acc) for every particle in parallel, then adds the per-thread
+  // accumulators into xb, vb, kbuf (if given), sx, sv and clears them.
+  template <class F> void scatter(long N, std::vector<A>* kbuf, F&& f) {
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    if (int(acc.size()) != nt) acc.assign(nt, Acc());
+    for (auto& o : acc)
+      if (long(o.x.size()) != N || o.sx.size() != sx.size()) {
+        o.x.assign(N, V()); o.v.assign(N, V()); o.k.assign(N, A(0));
+        o.sx.assign(sx.size(), {}); o.sv.assign(sv.size(), {});
+        for (size_t b = 0; b < sx.size(); ++b) { o.sx[b].assign(sx[b].size(), V()); o.sv[b].assign(sv[b].size(), V()); }
+      }
+#pragma omp parallel
+    {
+      int tid = 0;
+#ifdef _OPENMP
+      tid = omp_get_thread_num();
+#endif
+      Acc& o = acc[tid];
+#pragma omp for schedule(dynamic, 512)
+      for (long i = 0; i < N; ++i) f(i, o);
+    }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i)
+      for (auto& o : acc) {
+        xb[i] += o.x[i]; vb[i] += o.v[i];
+        o.x[i] = V(); o.v[i] = V();
+        if (kbuf) (*kbuf)[i] += o.k[i];
+        o.k[i] = A(0);
+      }
+    for (auto& o : acc)
+      for (size_t b = 0; b < sx.size(); ++b)
+        for (size_t j = 0; j < sx[b].size(); ++j) {
+          sx[b][j] += o.sx[b][j]; sv[b][j] += o.sv[b][j];
+          o.sx[b][j] = V(); o.sv[b][j] = V();
+        }
+  }
 
   static V cv(const V3<RF>& a) { return V(A(a.x), A(a.y), A(a.z)); }
   V gw(const V3<RF>& r) const { return cv(S.K.gradW(r)); }
@@ -77,7 +125,10 @@ template <typename RF, typename RR> struct Adjoint {
     const RF eps = RF(1e-5);
     const A Vf = A(S.V_f), mf = A(S.m_f), dt = A(S.dt);
     const long N = long(T.x.size());
-    for (long i = 0; i < N; ++i) {
+    scatter(N, &kbuf, [&](long i, Acc& o) {
+      auto& xb = o.x;
+      auto& kbuf = o.k;
+      auto& sx = o.sx;
       const V vbi = vb[i];
       for (uint32_t j : T.nFF[i]) {
         const RF kSum = k[i] + k[j];
@@ -91,7 +142,7 @@ template <typename RF, typename RR> struct Adjoint {
         xb[i] += t;
         xb[j] -= t;
       }
-      if (!(std::fabs(k[i]) > eps)) continue;
+      if (!(std::fabs(k[i]) > eps)) return;
       for (size_t bi = 0; bi < S.bodies.size(); ++bi) {
         const auto& B = S.bodies[bi];
         const auto& bt = T.bodies[bi];
@@ -115,7 +166,7 @@ template <typename RF, typename RR> struct Adjoint {
           sx[bi][jb] += f.cross(Tb[bi]);
         }
       }
-    }
+    });
   }
 
   // ---- reverse of delta_i = sum_j V_f (v_i - v_j).gradW_ij + sum_b V_b (v_i - v_b).gradW_ib
@@ -123,9 +174,13 @@ template <typename RF, typename RR> struct Adjoint {
   void reverse_delta(const StepTape<RF, RR>& T, const std::vector<V3<RF>>& v, const std::vector<A>& db) {
     const A Vf = A(S.V_f);
     const long N = long(T.x.size());
-    for (long i = 0; i < N; ++i) {
+    scatter(N, nullptr, [&](long i, Acc& o) {
+      auto& xb = o.x;
+      auto& vb = o.v;
+      auto& sx = o.sx;
+      auto& sv = o.sv;
       const A d = db[i];
-      if (d == A(0)) continue;
+      if (d == A(0)) return;
       for (uint32_t j : T.nFF[i]) {
         const V3<RF> r = T.x[i] - T.x[j];
         const V g = gw(r) * (d * Vf);
@@ -148,16 +203,18 @@ template <typename RF, typename RR> struct Adjoint {
           if (B.dynamic) { sv[bi][jb] -= g; sx[bi][jb] -= t; }
         }
       }
-    }
+    });
   }
 
   // ---- reverse of alpha_i = -1 / S_i, S_i = sum_j |g_ij|^2 + |G_i|^2 (computeDFSPHFactor)
   void reverse_factor(const StepTape<RF, RR>& T, const std::vector<A>& ab) {
     const A Vf = A(S.V_f);
     const long N = long(T.x.size());
-    for (long i = 0; i < N; ++i) {
+    scatter(N, nullptr, [&](long i, Acc& o) {
+      auto& xb = o.x;
+      auto& sx = o.sx;
       const A al = A(T.alpha[i]);
-      if (al == A(0) || ab[i] == A(0)) continue;
+      if (al == A(0) || ab[i] == A(0)) return;
       const A Sb = ab[i] * al * al;  // d(-1/S)/dS = 1/S^2 = alpha^2
       const V G = cv(T.G[i]);
       for (uint32_t j : T.nFF[i]) {
@@ -177,15 +234,17 @@ template <typename RF, typename RR> struct Adjoint {
           if (B.dynamic) sx[bi][jb] += t;
         }
       }
-    }
+    });
   }
 
   // ---- reverse of rho_i / rho0 = V_f W0 + sum_j V_f W_ij + sum_b V_b W_ib
   void reverse_density(const StepTape<RF, RR>& T, const std::vector<A>& rb) {
     const A Vf = A(S.V_f);
     const long N = long(T.x.size());
-    for (long i = 0; i < N; ++i) {
-      if (rb[i] == A(0)) continue;
+    scatter(N, nullptr, [&](long i, Acc& o) {
+      auto& xb = o.x;
+      auto& sx = o.sx;
+      if (rb[i] == A(0)) return;
       for (uint32_t j : T.nFF[i]) {
         const V t = gw(T.x[i] - T.x[j]) * (rb[i] * Vf);
         xb[i] += t;
@@ -200,7 +259,7 @@ template <typename RF, typename RR> struct Adjoint {
           if (B.dynamic) sx[bi][jb] -= t;
         }
       }
-    }
+    });
   }
 
   // ---- fluid part of one step, given Fb/Tb

@@ -6,7 +6,11 @@ with Akinci coupling, rigid update, a checkpoint every `--checkpoint-every`
 steps) followed by its exact adjoint (every step re-run once recording its
 Jacobi fields, then the reverse kernels), i.e. forward + adjoint particle-steps
 per second.  Inputs are resident in HBM; the rollout starts from a state
-settled by W untimed steps.
+settled by W untimed steps.  It is timed twice on the same seed: with the
+reference's warm start (halved previous kappa) and with the learned warm start,
+whose network is first trained (untimed) on pairs captured from a rollout of
+another seed; `value` is the learned run, both runs iterate to the same
+tolerance.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--checkpoint-every k] [--no-cpu-baseline]
 """
@@ -34,6 +38,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--workload", default="tank_1m", choices=["tank_1m", "tank_250k", "tank_4m"])
     ap.add_argument("--checkpoint-every", type=int, default=1)
+    ap.add_argument("--no-learned", action="store_true", help="measure the plain warm start only")
+    ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--train-iters", type=int, default=2000)
+    ap.add_argument("--capture-fraction", type=float, default=0.02)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--seed", type=int, default=0)
@@ -94,30 +102,45 @@ def cpu_baseline(scene, steps: int):
                        f"forward + exact adjoint of the CPU oracle (FP32 forward, FP64 adjoint, OpenMP)")
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
+def train_network(args, build):
+    """Untimed: capture (features, converged kappa) pairs from a plain rollout of a
+    training seed and fit the warm-start network on the GPU."""
     import torch
-    from difffr_amd import scenes
     from difffr_amd.simulation import Simulation
+    sim = Simulation(build(seed=args.seed + 1000, steps=args.train_steps), params_override={"compute_gradient": 0})
+    sim.mlp_capture(capacity=4_000_000, fraction=args.capture_fraction, seed=args.seed)
+    t0 = time.perf_counter()
+    sim.step(args.train_steps)
+    sim.mlp_capture(0)
+    n_pairs = sim.mlp_dataset()[0].shape[0]
+    t1 = time.perf_counter()
+    losses = sim.mlp_train(args.train_iters, batch=16384, lr=2e-3, seed=args.seed)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    P, norm = sim.mlp_get_params()
+    sim.close()
+    info = {"training_seed": args.seed + 1000, "pairs": int(n_pairs), "capture_steps": args.train_steps,
+            "adam_iterations": args.train_iters, "batch": 16384,
+            "loss_first": float(losses[:50].mean()), "loss_last": float(losses[-50:].mean()),
+            "capture_s": t1 - t0, "train_s": t2 - t1}
+    return P, norm, info
 
-    assert torch.cuda.is_available(), "bench.py needs an MI355X"
-    torch.cuda.init()
-    build = {"tank_1m": scenes.tank_1m, "tank_250k": scenes.tank_250k, "tank_4m": scenes.tank_4m}[args.workload]
-    sc = build(seed=args.seed, steps=args.warmup + args.steps)
+
+def timed_run(args, sc, learned, P=None, norm=None):
+    """Settle W steps (plain), then time a K-step rollout with checkpoints + its
+    exact adjoint, bracketed by device synchronisation; kernel-class durations
+    recorded with HIP events on the library's stream."""
+    import torch
+    from difffr_amd.simulation import Simulation
     # the exact adjoint replaces DiffFR's forward-mode gradient on this path
     sim = Simulation(sc, params_override={"compute_gradient": 0})
     sim.step(args.warmup)
+    if learned:
+        sim.mlp_set_params(P, norm)
+        sim.mlp_enable(True)
     sim.reset_stats()
     seed = np.zeros((sim.n_bodies, 7))
     seed[-1] = np.random.default_rng(args.seed).normal(size=7)
-
-    # Timed region: K-step rollout with checkpoints + its exact adjoint, bracketed
-    # by device synchronisation; kernel-class durations recorded with HIP events
-    # on the library's stream.
     sim.set_profiling(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -129,23 +152,49 @@ def main():
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     sim.set_profiling(False)
-    elapsed = t2 - t0
     st = sim.stats()
     times = sim.kernel_times()
     nst = sim.neighbor_stats()
     sim.adjoint_end()
+    sim.close()
     assert np.all(np.isfinite(grad)), "non-finite adjoint"
-
+    steps_done = max(st["steps_since_reset"], 1)
     n = sc.n_fluid
-    value = n * args.steps / elapsed
+    return {"elapsed": t2 - t0, "fwd": t1 - t0, "adj": t2 - t1, "value": n * args.steps / (t2 - t0),
+            "forward_value": n * args.steps / (t1 - t0), "times": times, "nst": nst,
+            "I_p": st["total_iterations"] / steps_done, "I_v": st["total_iterations_v"] / steps_done,
+            "err": st["avg_density_err"], "grad": grad[-1].tolist()}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    from difffr_amd import scenes
+
+    assert torch.cuda.is_available(), "bench.py needs an MI355X"
+    torch.cuda.init()
+    build = {"tank_1m": scenes.tank_1m, "tank_250k": scenes.tank_250k, "tank_4m": scenes.tank_4m}[args.workload]
+    sc = build(seed=args.seed, steps=args.warmup + args.steps)
+    n = sc.n_fluid
+    plain = timed_run(args, sc, False)
+    learned_on = not args.no_learned
+    if learned_on:
+        P, norm, tinfo = train_network(args, build)
+        learned = timed_run(args, sc, True, P, norm)
+        main_run = learned
+    else:
+        main_run = plain
+
+    times, nst = main_run["times"], main_run["nst"]
     per_kernel_total = {k: v[0] * v[1] for k, v in times.items()}
     dominant = max(per_kernel_total, key=per_kernel_total.get)
     nbar = nst["mean_fluid"] + nst["mean_boundary"]
     own = own_bytes_per_particle()
-    if dominant in own:
-        bytes_per_launch = n * (4 + 4 * nbar + own[dominant])
-    else:
-        bytes_per_launch = None
+    bytes_per_launch = n * (4 + 4 * nbar + own[dominant]) if dominant in own else None
     avg_ms = times[dominant][0]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if bytes_per_launch else None
     traffic = load_traffic(dominant, args.workload)
@@ -153,28 +202,35 @@ def main():
                 "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
                 "kernel": dominant, "avg_launch_ms": avg_ms, "launches": times[dominant][1],
                 "algorithmic_bytes_per_launch": bytes_per_launch}
-    steps_done = max(st["steps_since_reset"], 1)
+    summary = lambda r: {"fwd_adj_value": r["value"], "forward_value": r["forward_value"],
+                         "ms_per_step": 1e3 * r["elapsed"] / args.steps,
+                         "forward_ms_per_step": 1e3 * r["fwd"] / args.steps,
+                         "adjoint_ms_per_step": 1e3 * r["adj"] / args.steps,
+                         "mean_pressure_iterations": r["I_p"], "mean_divergence_iterations": r["I_v"],
+                         "final_avg_density_err": r["err"], "dL_dv0w0": r["grad"]}
     line = {
-        "metric": METRIC, "value": value, "unit": "fluid particle-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
-        "forward_ms_per_step": 1e3 * (t1 - t0) / args.steps, "adjoint_ms_per_step": 1e3 * (t2 - t1) / args.steps,
-        "forward_value": n * args.steps / (t1 - t0),
+        "metric": METRIC, "value": main_run["value"], "unit": "fluid particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * main_run["elapsed"] / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded jittered lattice, procedural rigid body)",
+        "data": "synthetic (seeded jittered lattice, procedural rigid body; warm-start network trained on "
+                "another seed's rollout)",
         "config": {
             "workload": f"{sc.name}: {n} fluid particles + {sc.n_boundary} boundary samples, "
                         f"dt=1e-3, 0.01% density / 0.1% divergence tolerance",
-            "path": f"plain solver (learned warm start not in this measurement), forward rollout with a "
-                    f"checkpoint every {args.checkpoint_every} step(s) + exact adjoint (record-replay of every step "
-                    f"+ reverse kernels) of dL/d(v0, omega0)",
-            "mean_pressure_iterations": st["total_iterations"] / steps_done,
-            "mean_divergence_iterations": st["total_iterations_v"] / steps_done,
-            "final_avg_density_err": st["avg_density_err"],
+            "path": f"{'learned' if learned_on else 'plain'} kappa warm start; forward rollout with a checkpoint "
+                    f"every {args.checkpoint_every} step(s) + exact adjoint (record-replay of every step + "
+                    f"reverse kernels) of dL/d(v0, omega0)",
             "mean_fluid_neighbors": nst["mean_fluid"], "mean_boundary_neighbors": nst["mean_boundary"],
+            "plain": summary(plain),
         },
         "roofline": roofline,
         "kernel_ms": {k: {"avg_ms": v[0], "launches": v[1]} for k, v in times.items()},
     }
+    if learned_on:
+        line["config"]["learned"] = summary(learned)
+        line["config"]["learned_over_plain"] = learned["value"] / plain["value"]
+        line["config"]["pressure_iterations_saved"] = plain["I_p"] - learned["I_p"]
+        line["config"]["training"] = tinfo
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
     print(json.dumps(line))

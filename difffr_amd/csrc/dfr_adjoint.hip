@@ -62,6 +62,8 @@ __device__ inline void atomic_add3(double* p, float x, float y, float z) {
 //           (pressure iteration; out = adjoint of delta, rho_bar += adjoint of rho/rho0)
 //   MODE 2: k = kappa_v warm start    (out = adjoint of the incoming kappa_v)
 //   MODE 3: k = kappa warm start      (out = adjoint of the incoming kappa)
+//   MODE 4/5: learned kappa_v / kappa warm start k = [d > 0] y_v / dt,
+//           [rho* > 1] y_p / dt^2 (out[2i] / out[2i+1] = adjoint of the prediction)
 // vbar is the adjoint of the updated velocity (unchanged: the update adds to v).
 // LDS per slot: x, y, z, vbar (3), k.
 // ---------------------------------------------------------------------------
@@ -193,6 +195,11 @@ k_adj_update_t(AdjPass a, const float* __restrict__ k, const float4* __restrict_
     } else if (MODE == ADJ_PRESSURE_WARM) {
       const float invH2 = 1.f / (c.dt * c.dt);
       out[i] = (b > 1.f && kprev[i] > -0.00025f) ? kb * 0.5f * invH2 : 0.f;
+    } else if (MODE == ADJ_PRESSURE_WARM_LEARNED) {
+      const float invH2 = 1.f / (c.dt * c.dt);
+      out[2 * (size_t)i + 1] = (b > 1.f) ? kb * invH2 : 0.f;
+    } else if (MODE == ADJ_DIV_WARM_LEARNED) {
+      out[2 * (size_t)i] = (b > 0.f) ? kb / c.dt : 0.f;
     } else {
       const float invH = 1.f / c.dt;
       out[i] = (b > 0.f && kprev[i] > -0.5f) ? kb * 0.5f * invH : 0.f;
@@ -313,7 +320,8 @@ k_adj_delta_t(AdjPass a, const float4* __restrict__ v, const float* __restrict__
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(2, 8)))
 k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restrict__ alpha_bar,
-               const float* __restrict__ rho_bar, const float4* __restrict__ G, float4* __restrict__ xbar) {
+               const float* __restrict__ rho_bar, const float4* __restrict__ G, float4* __restrict__ xbar,
+               const float* __restrict__ rbb) {
   __shared__ TileHdr<RB> T;
   extern __shared__ float lds[];
   const StepConst& c = a.c;
@@ -380,13 +388,15 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
     }
     const float tv = 2.f * c.V_f;
     float xbx = c.V_f * dx + tv * fx, xby = c.V_f * dy + tv * fy, xbz = c.V_f * dz + tv * fz;
-    if (Si != 0.f || ri != 0.f) {
+    // rbb: adjoint of the boundary part of the density alone (a warm-start feature)
+    const float rib = ri + (rbb ? rbb[i] : 0.f);
+    if (Si != 0.f || rib != 0.f) {
       for (uint32_t kq = 0; kq < nB; ++kq) {
         const uint32_t j = a.nbrB[(size_t)kq * stride + i];
         const float4 q = a.bx[j];
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
         const PairCE ce = cubic_ce(c.K, rx * rx + ry * ry + rz * rz);
-        const float sr = ri * q.w * ce.c;
+        const float sr = rib * q.w * ce.c;
         const float rG = rx * Gi.x + ry * Gi.y + rz * Gi.z;
         const float s2 = 2.f * Si * q.w;
         const float hx = s2 * (ce.c * Gi.x + ce.e * rG * rx), hy = s2 * (ce.c * Gi.y + ce.e * rG * ry),
@@ -400,6 +410,25 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
     xo.x += xbx; xo.y += xby; xo.z += xbz;
     xbar[i] = xo;
   }
+}
+
+// Adjoint of the learned warm start's features (f = rho/rho0 - 1, dt delta(v0),
+// neighbour count, boundary density, kappa_prev, kappa_v_prev) given
+// fbar = dL/df from the network's input gradient: kappa / kappa_v of the
+// previous step get fbar[4] / fbar[5] (plus the pass-through when the solve did
+// not overwrite them), rho/rho0 gets fbar[0], the divergence pass at v0 gets
+// dt fbar[1], the boundary density gets fbar[3].
+__global__ void k_adj_learned(int n, const float* __restrict__ fbar, int keep_kappa, int keep_kappav, float dt,
+                              float* __restrict__ kb, float* __restrict__ kvb, float* __restrict__ rhob,
+                              float* __restrict__ dbar, float* __restrict__ rbb) {
+  const uint32_t i = gid();
+  if (i >= (uint32_t)n) return;
+  const float* f = fbar + 6 * (size_t)i;
+  kb[i] = f[4] + (keep_kappa ? kb[i] : 0.f);
+  kvb[i] = f[5] + (keep_kappav ? kvb[i] : 0.f);
+  rhob[i] += f[0];
+  dbar[i] = dt * f[1];
+  rbb[i] = f[3];
 }
 
 // x' = x + dt v: the adjoint of the final velocity receives dt xbar.
@@ -604,9 +633,17 @@ void launch_adj_update(const AdjPass& a, int mode, const float* k, const float4*
       hipLaunchKernelGGL(k_adj_update_t<ADJ_DIV_WARM>, grid, blk, SHM7, s, a, k, vbar, kacc, kacc_scale, bval, kprev,
                          alpha, out, alpha_bar, rho_bar, xbar);
       break;
-    default:
+    case ADJ_PRESSURE_WARM:
       hipLaunchKernelGGL(k_adj_update_t<ADJ_PRESSURE_WARM>, grid, blk, SHM7, s, a, k, vbar, kacc, kacc_scale, bval,
                          kprev, alpha, out, alpha_bar, rho_bar, xbar);
+      break;
+    case ADJ_DIV_WARM_LEARNED:
+      hipLaunchKernelGGL(k_adj_update_t<ADJ_DIV_WARM_LEARNED>, grid, blk, SHM7, s, a, k, vbar, kacc, kacc_scale, bval,
+                         kprev, alpha, out, alpha_bar, rho_bar, xbar);
+      break;
+    default:
+      hipLaunchKernelGGL(k_adj_update_t<ADJ_PRESSURE_WARM_LEARNED>, grid, blk, SHM7, s, a, k, vbar, kacc, kacc_scale,
+                         bval, kprev, alpha, out, alpha_bar, rho_bar, xbar);
   }
 }
 
@@ -616,8 +653,16 @@ void launch_adj_delta(const AdjPass& a, const float4* v, const float* dbar, floa
 }
 
 void launch_adj_factor(const AdjPass& a, const float* alpha, const float* alpha_bar, const float* rho_bar,
-                       const float4* G, float4* xbar, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_factor_t, dim3(nbricks(a.g)), dim3(RB), SHM8, s, a, alpha, alpha_bar, rho_bar, G, xbar);
+                       const float4* G, float4* xbar, const float* rbb, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_factor_t, dim3(nbricks(a.g)), dim3(RB), SHM8, s, a, alpha, alpha_bar, rho_bar, G, xbar,
+                     rbb);
+}
+
+void launch_adj_learned(int n, const float* fbar, int keep_kappa, int keep_kappav, float dt, float* kb, float* kvb,
+                        float* rhob, float* dbar, float* rbb, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_adj_learned, dim3(nblk(n)), dim3(DFR_BLOCK), 0, s, n, fbar, keep_kappa, keep_kappav, dt, kb,
+                     kvb, rhob, dbar, rbb);
 }
 
 void launch_adj_position(int n, float dt, const float4* xbar, float4* vbar, hipStream_t s) {
@@ -651,6 +696,10 @@ void set_adjoint_kernel_attributes() {
   (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_DIV_WARM>, hipFuncAttributeMaxDynamicSharedMemorySize, s7);
   (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_PRESSURE_WARM>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             s7);
+  (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_DIV_WARM_LEARNED>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, s7);
+  (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_PRESSURE_WARM_LEARNED>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, s7);
   (void)hipFuncSetAttribute((const void*)k_adj_delta_t, hipFuncAttributeMaxDynamicSharedMemorySize, s7);
   (void)hipFuncSetAttribute((const void*)k_adj_factor_t, hipFuncAttributeMaxDynamicSharedMemorySize, s8);
 }

@@ -24,13 +24,13 @@ namespace {
 
 enum TimerClass {
   TC_SORT = 0, TC_NBR, TC_DENSITY, TC_VUPDATE, TC_RECOMPUTE, TC_PWARM, TC_ELEMWISE, TC_GRADIENT, TC_RIGID,
-  TC_ADJ_UPDATE, TC_ADJ_DELTA, TC_ADJ_FACTOR, TC_ADJ_MISC, TC_CHECKPOINT, TC_COUNT
+  TC_ADJ_UPDATE, TC_ADJ_DELTA, TC_ADJ_FACTOR, TC_ADJ_MISC, TC_CHECKPOINT, TC_MLP, TC_COUNT
 };
 const char* kTimerNames[TC_COUNT] = {"grid_sort", "neighbor_build", "density_factor", "velocity_update",
                                      "density_recompute", "pressure_warmstart", "advect_position",
                                      "gradient_pairs", "rigid_step", "adj_velocity_update",
                                      "adj_density_change", "adj_density_factor", "adj_rigid_misc",
-                                     "checkpoint_copy"};
+                                     "checkpoint_copy", "mlp_warmstart"};
 
 struct BodyInfo {
   bool dynamic = false;
@@ -61,6 +61,7 @@ struct Snapshot {
 // values that the step overwrote while it ran.
 struct StepTapeDev {
   float4* x = nullptr;                    // fluid positions during the step (sorted order)
+  float4* v0 = nullptr;                   // fluid velocities after the sort (learned warm-start feature)
   float *kvPrev = nullptr, *d0 = nullptr, *kvW = nullptr;  // divergence warm start
   float *kPrev = nullptr, *rsW = nullptr, *kW = nullptr;   // pressure warm start
   float4 *bpos = nullptr, *bvel = nullptr;  // boundary samples at the start of the step
@@ -69,6 +70,29 @@ struct StepTapeDev {
   std::vector<float*> kD, bD, kP, bP;     // k_i and d / rho* of each iteration
   int Iv = 0, Ip = 0;
   AdjStepTable tbl{};
+};
+
+// Learned kappa warm start: parameters, Adam state, captured training pairs.
+struct MlpState {
+  float *P = nullptr, *G = nullptr, *m = nullptr, *v = nullptr;  // FP32 parameters, gradient, Adam moments
+  void* frag = nullptr;                    // bf16 MFMA fragments of the parameters
+  MlpNorm norm{};
+  bool has_params = false, norm_set = false, enabled = false;
+  int adam_t = 0;
+  float* pred = nullptr;                   // [n][2] predictions of the current step
+  float* kpred = nullptr;                  // [n] predicted kappa (stored units)
+  float2* aux = nullptr;                   // [n] (velocity divergence, boundary density)
+  // training pairs
+  bool capturing = false;
+  int64_t cap = 0;
+  uint32_t threshold = 0;
+  uint64_t salt = 0;
+  float *feat = nullptr, *targ = nullptr;
+  uint32_t *rows = nullptr, *count = nullptr;  // count[0] total, count[1] at the start of the step
+  uint32_t* batch = nullptr;
+  int batch_cap = 0;
+  double* losses = nullptr;
+  int losses_cap = 0;
 };
 
 // Exact adjoint of a rollout: checkpoints every k steps of the primal run and
@@ -85,6 +109,7 @@ struct AdjState {
   float4 *xb = nullptr, *vb = nullptr, *xb2 = nullptr, *vb2 = nullptr;
   float *kb = nullptr, *kvb = nullptr, *kb2 = nullptr, *kvb2 = nullptr;
   float *dbar = nullptr, *alb = nullptr, *rhob = nullptr;
+  float *ybar = nullptr, *fbar = nullptr, *rbb = nullptr;  // learned warm start: prediction / feature adjoints
   double *sxb = nullptr, *svb = nullptr;
   AdjBodyDev* body = nullptr;
   StepTapeDev tape;
@@ -140,6 +165,7 @@ struct dfr_ctx {
   int force_iv = -1, force_ip = -1;  // iteration counts imposed on the next step
   std::vector<int> fixed_iv, fixed_ip;  // dfr_set_fixed_iterations
   size_t fixed_pos = 0;
+  MlpState mlp;
 };
 
 // ---------------------------------------------------------------------------
@@ -429,6 +455,7 @@ void do_step(dfr_ctx* c) {
   }
   if (rec) {
     d2d(c, rec->x, c->f.pos, nf);
+    if (c->mlp.enabled) d2d(c, rec->v0, c->f.vel, nf);
     d2d(c, rec->kvPrev, c->f.kappav, nf);
     d2d(c, rec->kPrev, c->f.kappa, nf);
   }
@@ -446,10 +473,25 @@ void do_step(dfr_ctx* c) {
   const bool warmV = c->P.use_divergence_warmstart != 0;
   const bool warmP = c->P.use_pressure_warmstart != 0;
   const int scratch_slot = c->err_slots - 1;
+  MlpState& ml = c->mlp;
+  const bool learned = ml.enabled;
   if (nf > 0) {
     Timed t(c, TC_DENSITY);
-    launch_density_factor_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, (divOn && warmV) ? 1 : 0, divOn ? 1 : 0,
-                            c->stream);
+    launch_density_factor_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, (divOn && warmV && !learned) ? 1 : 0,
+                            divOn ? 1 : 0, (learned || ml.capturing) ? ml.aux : nullptr, c->stream);
+  }
+  const MlpFields fields{c->f.rho, ml.aux, c->nb.cnt, c->f.kappa, c->f.kappav, c->sc.rho0, dt};
+  if (ml.capturing && nf > 0) {
+    d2d(c, ml.count + 1, ml.count, 1);
+    launch_mlp_capture_features(fields, c->f.id, nf, ml.salt ^ (uint64_t)c->step_count * 0xD1B54A32D192ED03ull,
+                                ml.threshold, ml.feat, ml.rows, ml.count, (uint32_t)ml.cap, c->stream);
+  }
+  if (learned && nf > 0) {
+    // learned warm starts replace warmstartDivergenceSolve's and warmstartPressureSolve's halved kappa
+    Timed t(c, TC_MLP);
+    launch_mlp_infer_fields(ml.frag, ml.P, ml.norm, fields, nf, ml.pred, c->stream);
+    launch_mlp_apply_div(nf, ml.pred, (divOn && warmV) ? c->f.badv : nullptr, 1.f / dt, c->f.kappav, ml.kpred,
+                         c->stream);
   }
   if (rec && nf > 0) {
     d2d(c, rec->d0, c->f.badv, nf);
@@ -504,7 +546,9 @@ void do_step(dfr_ctx* c) {
     if (warmP) {
       {
         Timed t(c, TC_PWARM);
-        launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, PRESSURE_WARMSTART, 0.f, nullptr, c->stream);
+        launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb,
+                           learned ? PRESSURE_WARMSTART_LEARNED : PRESSURE_WARMSTART, 0.f, nullptr, c->stream,
+                           ml.kpred);
       }
       if (rec) {
         d2d(c, rec->rsW, c->f.badv, nf);
@@ -542,6 +586,9 @@ void do_step(dfr_ctx* c) {
     }
     Timed t(c, TC_ELEMWISE);
     launch_position(c->sc, c->f, warmP ? dt * dt : 1.f, c->stream);
+    if (ml.capturing)  // the converged stiffness of this step is the training target
+      launch_mlp_capture_targets(c->f.kappa, c->f.kappav, ml.rows, ml.count + 1, (uint32_t)nf, ml.count,
+                                 (uint32_t)ml.cap, ml.targ, c->stream);
   }
   c->time = c->time + dt;
   {
@@ -652,10 +699,12 @@ void adjoint_buffers(dfr_ctx* c) {
   A.xb = dalloc<float4>(c, nf); A.vb = dalloc<float4>(c, nf); A.xb2 = dalloc<float4>(c, nf); A.vb2 = dalloc<float4>(c, nf);
   A.kb = dalloc<float>(c, nf); A.kvb = dalloc<float>(c, nf); A.kb2 = dalloc<float>(c, nf); A.kvb2 = dalloc<float>(c, nf);
   A.dbar = dalloc<float>(c, nf); A.alb = dalloc<float>(c, nf); A.rhob = dalloc<float>(c, nf);
+  A.ybar = dalloc<float>(c, 2 * nf); A.fbar = dalloc<float>(c, 6 * nf); A.rbb = dalloc<float>(c, nf);
   A.sxb = dalloc<double>(c, 3 * nd); A.svb = dalloc<double>(c, 3 * nd);
   A.body = dalloc<AdjBodyDev>(c, nr);
   StepTapeDev& T = A.tape;
   T.x = dalloc<float4>(c, nf);
+  T.v0 = dalloc<float4>(c, nf);
   T.kvPrev = dalloc<float>(c, nf); T.d0 = dalloc<float>(c, nf); T.kvW = dalloc<float>(c, nf);
   T.kPrev = dalloc<float>(c, nf); T.rsW = dalloc<float>(c, nf); T.kW = dalloc<float>(c, nf);
   T.bpos = dalloc<float4>(c, nb); T.bvel = dalloc<float4>(c, nb);
@@ -712,7 +761,9 @@ void reverse_step(dfr_ctx* c) {
       HIPCHK(hipMemsetAsync(A.rhob, 0, sizeof(float) * nf, c->stream));
     }
     const bool warmP = c->P.use_pressure_warmstart != 0;
+    const bool learned = c->mlp.enabled;
     const float h2 = dt * dt;
+    if (learned) HIPCHK(hipMemsetAsync(A.ybar, 0, sizeof(float) * 2 * nf, c->stream));
     for (int m = T.Ip - 1; m >= 0; --m) {
       {
         Timed t(c, TC_ADJ_UPDATE);
@@ -724,8 +775,12 @@ void reverse_step(dfr_ctx* c) {
     }
     if (warmP) {
       Timed t(c, TC_ADJ_UPDATE);
-      launch_adj_update(a, ADJ_PRESSURE_WARM, T.kW, A.vb, A.kb, h2, T.rsW, T.kPrev, c->f.alpha, A.kb, A.alb,
-                        A.rhob, A.xb, c->stream);
+      if (learned)
+        launch_adj_update(a, ADJ_PRESSURE_WARM_LEARNED, T.kW, A.vb, A.kb, h2, T.rsW, nullptr, c->f.alpha, A.ybar,
+                          A.alb, A.rhob, A.xb, c->stream);
+      else
+        launch_adj_update(a, ADJ_PRESSURE_WARM, T.kW, A.vb, A.kb, h2, T.rsW, T.kPrev, c->f.alpha, A.kb, A.alb,
+                          A.rhob, A.xb, c->stream);
     }
     if (c->P.enable_divergence_solver) {
       const bool warmV = c->P.use_divergence_warmstart != 0;
@@ -740,13 +795,27 @@ void reverse_step(dfr_ctx* c) {
       }
       if (warmV) {
         Timed t(c, TC_ADJ_UPDATE);
-        launch_adj_update(a, ADJ_DIV_WARM, T.kvW, A.vb, A.kvb, dt, T.d0, T.kvPrev, c->f.alpha, A.kvb, A.alb,
-                          A.rhob, A.xb, c->stream);
+        if (learned)
+          launch_adj_update(a, ADJ_DIV_WARM_LEARNED, T.kvW, A.vb, A.kvb, dt, T.d0, nullptr, c->f.alpha, A.ybar,
+                            A.alb, A.rhob, A.xb, c->stream);
+        else
+          launch_adj_update(a, ADJ_DIV_WARM, T.kvW, A.vb, A.kvb, dt, T.d0, T.kvPrev, c->f.alpha, A.kvb, A.alb,
+                            A.rhob, A.xb, c->stream);
       }
+    }
+    if (learned) {
+      // through the network to its features, then to the state they were computed from
+      Timed t(c, TC_MLP);
+      const MlpFields fields{c->f.rho, c->mlp.aux, c->nb.cnt, T.kPrev, T.kvPrev, c->sc.rho0, dt};
+      launch_mlp_input_grad(c->mlp.frag, c->mlp.P, c->mlp.norm, nullptr, &fields, nf, A.ybar, A.fbar, c->stream);
+      const bool divWarm = c->P.enable_divergence_solver && c->P.use_divergence_warmstart;
+      launch_adj_learned(nf, A.fbar, warmP ? 0 : 1, divWarm ? 0 : 1, dt, A.kb, A.kvb, A.rhob, A.dbar, A.rbb,
+                         c->stream);
+      if (c->P.enable_divergence_solver) launch_adj_delta(a, T.v0, A.dbar, A.vb, A.xb, c->stream);
     }
     {
       Timed t(c, TC_ADJ_FACTOR);
-      launch_adj_factor(a, c->f.alpha, A.alb, A.rhob, c->f.s, A.xb, c->stream);
+      launch_adj_factor(a, c->f.alpha, A.alb, A.rhob, c->f.s, A.xb, learned ? A.rbb : nullptr, c->stream);
     }
   }
   Timed t(c, TC_ADJ_MISC);
@@ -768,6 +837,92 @@ void replay_step(dfr_ctx* c, int64_t s, bool record) {
   do_step(c);
   c->rec = nullptr;
   c->force_iv = c->force_ip = -1;
+}
+
+// Device scratch for the host-array entry points below.
+template <typename T> struct DevTmp {
+  T* p = nullptr;
+  explicit DevTmp(size_t n) { if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) throw HipError{hipErrorOutOfMemory, "DevTmp"}; }
+  ~DevTmp() { if (p) (void)hipFree(p); }
+};
+
+// ---------------------------------------------------------------------------
+// Learned warm start: allocation, initialisation, training loop.
+// ---------------------------------------------------------------------------
+void mlp_alloc(dfr_ctx* c) {
+  MlpState& M = c->mlp;
+  if (M.P) return;
+  M.P = dalloc<float>(c, MLP_NP);
+  M.G = dalloc<float>(c, MLP_NP);
+  M.m = dalloc<float>(c, MLP_NP);
+  M.v = dalloc<float>(c, MLP_NP);
+  HIPCHK(hipMemsetAsync(M.G, 0, sizeof(float) * MLP_NP, c->stream));
+  HIPCHK(hipMemsetAsync(M.m, 0, sizeof(float) * MLP_NP, c->stream));
+  HIPCHK(hipMemsetAsync(M.v, 0, sizeof(float) * MLP_NP, c->stream));
+  M.frag = dalloc<char>(c, mlp_frag_bytes());
+  const size_t nf = (size_t)std::max(c->f.n, 1);
+  M.pred = dalloc<float>(c, 2 * nf);
+  M.kpred = dalloc<float>(c, nf);
+  M.aux = dalloc<float2>(c, nf);
+  M.count = dalloc<uint32_t>(c, 2);
+  HIPCHK(hipMemsetAsync(M.count, 0, sizeof(uint32_t) * 2, c->stream));
+  set_mlp_kernel_attributes();
+  for (int k = 0; k < 6; ++k) { M.norm.mu_in[k] = 0.f; M.norm.is_in[k] = 1.f; }
+  for (int k = 0; k < 2; ++k) { M.norm.mu_out[k] = 0.f; M.norm.sd_out[k] = 1.f; }
+}
+
+void mlp_upload_params(dfr_ctx* c, const std::vector<float>& P) {
+  MlpState& M = c->mlp;
+  HIPCHK(hipMemcpyAsync(M.P, P.data(), sizeof(float) * MLP_NP, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(M.m, 0, sizeof(float) * MLP_NP, c->stream));
+  HIPCHK(hipMemsetAsync(M.v, 0, sizeof(float) * MLP_NP, c->stream));
+  launch_mlp_pack(M.P, M.frag, c->stream);
+  M.adam_t = 0;
+  M.has_params = true;
+}
+
+// Uniform init with bound sqrt(6 / fan_in) (He et al. 2015) for the SiLU
+// layers, a tenth of it for the linear output layer; zero biases.
+void mlp_init_params(dfr_ctx* c, uint64_t seed) {
+  static const int O[4] = {64, 64, 64, 2}, I[4] = {6, 64, 64, 64}, OW[4] = {0, 448, 4608, 8768};
+  std::vector<float> P(MLP_NP, 0.f);
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  auto uni = [&]() {
+    st ^= st >> 12; st ^= st << 25; st ^= st >> 27;
+    return (double)((st * 0x2545F4914F6CDD1Dull) >> 11) * (1.0 / 9007199254740992.0);
+  };
+  for (int L = 0; L < 4; ++L) {
+    const double b = std::sqrt(6.0 / I[L]) * (L == 3 ? 0.1 : 1.0);
+    for (int k = 0; k < O[L] * I[L]; ++k) P[OW[L] + k] = (float)((2.0 * uni() - 1.0) * b);
+  }
+  mlp_upload_params(c, P);
+}
+
+uint32_t mlp_dataset_size(dfr_ctx* c) {
+  MlpState& M = c->mlp;
+  if (!M.count) return 0;
+  uint32_t n = 0;
+  HIPCHK(hipMemcpyAsync(&n, M.count, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return std::min<uint32_t>(n, (uint32_t)M.cap);
+}
+
+// Feature and target mean / standard deviation over the captured pairs.
+void mlp_fit_norm(dfr_ctx* c, uint32_t n) {
+  MlpState& M = c->mlp;
+  std::vector<float> f((size_t)n * 6), t((size_t)n * 2);
+  HIPCHK(hipMemcpy(f.data(), M.feat, sizeof(float) * f.size(), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(t.data(), M.targ, sizeof(float) * t.size(), hipMemcpyDeviceToHost));
+  auto stats = [&](const std::vector<float>& a, int w, int k, float& mu, float& sd) {
+    double s1 = 0, s2 = 0;
+    for (uint32_t r = 0; r < n; ++r) { const double x = a[(size_t)r * w + k]; s1 += x; s2 += x * x; }
+    const double m = s1 / n, var = std::max(s2 / n - m * m, 0.0);
+    mu = (float)m;
+    sd = var > 1e-30 ? (float)std::sqrt(var) : 1.f;
+  };
+  for (int k = 0; k < 6; ++k) { float sd; stats(f, 6, k, M.norm.mu_in[k], sd); M.norm.is_in[k] = 1.f / sd; }
+  for (int k = 0; k < 2; ++k) stats(t, 2, k, M.norm.mu_out[k], M.norm.sd_out[k]);
+  M.norm_set = true;
 }
 
 }  // namespace
@@ -1360,6 +1515,175 @@ int dfr_set_fixed_iterations(dfr_ctx* c, int n, const int* iv, const int* ip) {
   c->fixed_ip.assign(ip, ip + n);
   c->fixed_pos = 0;
   return DFR_OK;
+}
+
+int dfr_mlp_capture(dfr_ctx* c, int64_t capacity, double fraction, uint64_t seed) {
+  return guarded(c, [&]() {
+    if (!c->initialized) return fail(c, DFR_ERR_STATE, "dfr_mlp_capture before dfr_init");
+    if (capacity < 0 || fraction < 0.0 || fraction > 1.0) return fail(c, DFR_ERR_ARG, "dfr_mlp_capture: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    mlp_alloc(c);
+    MlpState& M = c->mlp;
+    if (capacity == 0) { M.capturing = false; return DFR_OK; }
+    if (capacity > (int64_t)0xffffffffu) return fail(c, DFR_ERR_ARG, "dfr_mlp_capture: capacity too large");
+    if (capacity != M.cap) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      for (void* p : {(void*)M.feat, (void*)M.targ, (void*)M.rows})
+        if (p) { (void)hipFree(p); c->allocations.erase(std::find(c->allocations.begin(), c->allocations.end(), p)); }
+      M.feat = dalloc<float>(c, (size_t)capacity * 6);
+      M.targ = dalloc<float>(c, (size_t)capacity * 2);
+      M.rows = dalloc<uint32_t>(c, (size_t)capacity);
+      M.cap = capacity;
+    }
+    HIPCHK(hipMemsetAsync(M.count, 0, sizeof(uint32_t) * 2, c->stream));
+    M.threshold = fraction >= 1.0 ? 0xffffffffu : (uint32_t)(fraction * 4294967296.0);
+    M.salt = seed * 0xBF58476D1CE4E5B9ull + 1;
+    M.capturing = true;
+    return DFR_OK;
+  });
+}
+
+int64_t dfr_mlp_dataset_size(dfr_ctx* c) {
+  if (!c) return DFR_ERR_ARG;
+  return guarded(c, [&]() { return (int)mlp_dataset_size(c); });
+}
+
+int64_t dfr_mlp_dataset(dfr_ctx* c, float* feat, float* targ, int64_t cap) {
+  if (!c) return DFR_ERR_ARG;
+  return guarded(c, [&]() {
+    const uint32_t n = mlp_dataset_size(c);
+    const size_t k = (size_t)std::min<int64_t>(n, cap);
+    if (feat && k) HIPCHK(hipMemcpy(feat, c->mlp.feat, sizeof(float) * 6 * k, hipMemcpyDeviceToHost));
+    if (targ && k) HIPCHK(hipMemcpy(targ, c->mlp.targ, sizeof(float) * 2 * k, hipMemcpyDeviceToHost));
+    return (int)n;
+  });
+}
+
+int dfr_mlp_init_params(dfr_ctx* c, uint64_t seed) {
+  return guarded(c, [&]() {
+    if (!c->initialized) return fail(c, DFR_ERR_STATE, "dfr_mlp_init_params before dfr_init");
+    HIPCHK(hipSetDevice(c->device));
+    mlp_alloc(c);
+    mlp_init_params(c, seed);
+    sync_check(c);
+    return DFR_OK;
+  });
+}
+
+int dfr_mlp_set_params(dfr_ctx* c, const float* params, const float* norm) {
+  return guarded(c, [&]() {
+    if (!c->initialized || !params || !norm) return fail(c, DFR_ERR_ARG, "dfr_mlp_set_params: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    mlp_alloc(c);
+    MlpNorm& N = c->mlp.norm;
+    for (int k = 0; k < 6; ++k) { N.mu_in[k] = norm[k]; N.is_in[k] = 1.f / norm[6 + k]; }
+    for (int k = 0; k < 2; ++k) { N.mu_out[k] = norm[12 + k]; N.sd_out[k] = norm[14 + k]; }
+    c->mlp.norm_set = true;
+    mlp_upload_params(c, std::vector<float>(params, params + MLP_NP));
+    sync_check(c);
+    return DFR_OK;
+  });
+}
+
+int dfr_mlp_get_params(dfr_ctx* c, float* params, float* norm) {
+  return guarded(c, [&]() {
+    if (!c->mlp.has_params) return fail(c, DFR_ERR_STATE, "no network parameters");
+    if (params) HIPCHK(hipMemcpy(params, c->mlp.P, sizeof(float) * MLP_NP, hipMemcpyDeviceToHost));
+    if (norm) {
+      const MlpNorm& N = c->mlp.norm;
+      for (int k = 0; k < 6; ++k) { norm[k] = N.mu_in[k]; norm[6 + k] = 1.f / N.is_in[k]; }
+      for (int k = 0; k < 2; ++k) { norm[12 + k] = N.mu_out[k]; norm[14 + k] = N.sd_out[k]; }
+    }
+    return DFR_OK;
+  });
+}
+
+int dfr_mlp_train(dfr_ctx* c, int iters, int batch, double lr, uint64_t seed, double* losses) {
+  return guarded(c, [&]() {
+    if (!c->initialized || iters <= 0 || batch <= 0 || lr <= 0.0) return fail(c, DFR_ERR_ARG, "dfr_mlp_train: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    mlp_alloc(c);
+    MlpState& M = c->mlp;
+    const uint32_t n = mlp_dataset_size(c);
+    if (n == 0) return fail(c, DFR_ERR_STATE, "dfr_mlp_train: no captured training pairs");
+    if (!M.norm_set) mlp_fit_norm(c, n);
+    if (!M.has_params) mlp_init_params(c, seed);
+    if (M.batch_cap < batch) { M.batch = dalloc<uint32_t>(c, batch); M.batch_cap = batch; }
+    if (M.losses_cap < iters) { M.losses = dalloc<double>(c, iters); M.losses_cap = iters; }
+    HIPCHK(hipMemsetAsync(M.losses, 0, sizeof(double) * iters, c->stream));
+    for (int it = 0; it < iters; ++it) {
+      Timed t(c, TC_MLP);
+      launch_mlp_batch(M.batch, batch, seed * 0x9E3779B97F4A7C15ull + (uint64_t)(M.adam_t + 1) * 0x632BE59BD9B4E019ull, n,
+                       c->stream);
+      launch_mlp_train_batch(M.frag, M.P, M.norm, M.feat, M.targ, M.batch, batch, M.G, M.losses + it, c->stream);
+      const double cosw = 0.5 * (1.0 + std::cos(M_PI * (double)it / (double)iters));
+      launch_mlp_adam(M.P, M.G, M.m, M.v, (float)(lr * (0.1 + 0.9 * cosw)), ++M.adam_t, c->stream);
+      launch_mlp_pack(M.P, M.frag, c->stream);
+    }
+    std::vector<double> h(iters);
+    HIPCHK(hipMemcpyAsync(h.data(), M.losses, sizeof(double) * iters, hipMemcpyDeviceToHost, c->stream));
+    sync_check(c);
+    if (c->profiling) drain_timers(c);
+    if (losses)
+      for (int it = 0; it < iters; ++it) losses[it] = h[it] / (2.0 * batch);
+    return DFR_OK;
+  });
+}
+
+int dfr_mlp_enable(dfr_ctx* c, int on) {
+  return guarded(c, [&]() {
+    if (on && !c->mlp.has_params) return fail(c, DFR_ERR_STATE, "dfr_mlp_enable: no trained network");
+    adjoint_release(c);
+    c->mlp.enabled = on != 0;
+    return DFR_OK;
+  });
+}
+
+int dfr_mlp_predict(dfr_ctx* c, int64_t n, const float* feat, float* out) {
+  return guarded(c, [&]() {
+    if (!c->mlp.has_params || n < 0 || (n > 0 && (!feat || !out))) return fail(c, DFR_ERR_ARG, "dfr_mlp_predict: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    DevTmp<float> f(6 * n), o(2 * n);
+    HIPCHK(hipMemcpy(f.p, feat, sizeof(float) * 6 * n, hipMemcpyHostToDevice));
+    launch_mlp_infer_array(c->mlp.frag, c->mlp.P, c->mlp.norm, f.p, (int)n, o.p, c->stream);
+    sync_check(c);
+    HIPCHK(hipMemcpy(out, o.p, sizeof(float) * 2 * n, hipMemcpyDeviceToHost));
+    return DFR_OK;
+  });
+}
+
+int dfr_mlp_loss_grad(dfr_ctx* c, int64_t n, const float* feat, const float* targ, float* grad, double* loss) {
+  return guarded(c, [&]() {
+    if (!c->mlp.has_params || n <= 0 || !feat || !targ || !grad) return fail(c, DFR_ERR_ARG, "dfr_mlp_loss_grad: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    MlpState& M = c->mlp;
+    DevTmp<float> f(6 * n), t(2 * n), g(MLP_NP);
+    DevTmp<double> l(1);
+    HIPCHK(hipMemcpy(f.p, feat, sizeof(float) * 6 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t.p, targ, sizeof(float) * 2 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(g.p, 0, sizeof(float) * MLP_NP, c->stream));
+    HIPCHK(hipMemsetAsync(l.p, 0, sizeof(double), c->stream));
+    launch_mlp_train_batch(M.frag, M.P, M.norm, f.p, t.p, nullptr, (int)n, g.p, l.p, c->stream);
+    sync_check(c);
+    HIPCHK(hipMemcpy(grad, g.p, sizeof(float) * MLP_NP, hipMemcpyDeviceToHost));
+    if (loss) HIPCHK(hipMemcpy(loss, l.p, sizeof(double), hipMemcpyDeviceToHost));
+    return DFR_OK;
+  });
+}
+
+int dfr_mlp_input_grad(dfr_ctx* c, int64_t n, const float* feat, const float* ybar, float* fbar) {
+  return guarded(c, [&]() {
+    if (!c->mlp.has_params || n <= 0 || !feat || !ybar || !fbar) return fail(c, DFR_ERR_ARG, "dfr_mlp_input_grad: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    MlpState& M = c->mlp;
+    DevTmp<float> f(6 * n), y(2 * n), o(6 * n);
+    HIPCHK(hipMemcpy(f.p, feat, sizeof(float) * 6 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(y.p, ybar, sizeof(float) * 2 * n, hipMemcpyHostToDevice));
+    launch_mlp_input_grad(M.frag, M.P, M.norm, f.p, nullptr, (int)n, y.p, o.p, c->stream);
+    sync_check(c);
+    HIPCHK(hipMemcpy(fbar, o.p, sizeof(float) * 6 * n, hipMemcpyDeviceToHost));
+    return DFR_OK;
+  });
 }
 
 int64_t dfr_num_fluid(const dfr_ctx* c) { return c ? c->f.n : DFR_ERR_ARG; }

@@ -71,18 +71,21 @@ void launch_build_bf(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs&
 // dfr_tiled.hip
 void launch_build_tiled(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
                         hipStream_t s);
+// aux (optional): per particle (velocity divergence, boundary density sum_b V_b W_ib)
 void launch_density_factor_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
-                             NbrBufs& nb, int div_warm, int divergence_on, hipStream_t s);
+                             NbrBufs& nb, int div_warm, int divergence_on, float2* aux, hipStream_t s);
 void launch_velocity_update_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
                               NbrBufs& nb, RigidDev* rig, const float* kfield, float* kappa_acc, hipStream_t s);
-// mode 0: divergence recompute, 1: pressure recompute, 2: pressure warm start
+// mode 0: divergence recompute, 1: pressure recompute, 2: pressure warm start,
+// 3: pressure warm start from predicted stiffness kpred (learned)
 void launch_recompute_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
-                        NbrBufs& nb, int mode, float kcoef, double* err_slot, hipStream_t s);
+                        NbrBufs& nb, int mode, float kcoef, double* err_slot, hipStream_t s,
+                        const float* kpred = nullptr);
 void launch_expand_lists(const GridDesc& g, GridBufs& gb, FluidBufs& f, NbrBufs& nb, uint32_t* out, hipStream_t s);
 void set_tiled_kernel_attributes();
 
 // dfr_solver.hip
-enum RecomputeMode { RECOMPUTE_DIV = 0, RECOMPUTE_PRESSURE = 1, PRESSURE_WARMSTART = 2 };
+enum RecomputeMode { RECOMPUTE_DIV = 0, RECOMPUTE_PRESSURE = 1, PRESSURE_WARMSTART = 2, PRESSURE_WARMSTART_LEARNED = 3 };
 void launch_advect(const StepConst& c, FluidBufs& f, float kappav_scale, hipStream_t s);
 void launch_position(const StepConst& c, FluidBufs& f, float kappa_scale, hipStream_t s);
 void launch_boundary_volume_impl(const float4* bpos_in, const float4* bx0, int nb, const int* groups, int nbodies,
@@ -155,14 +158,19 @@ struct AdjPass {
   const AdjBodyDev* body;
 };
 
-enum AdjUpdateMode { ADJ_DIV_ITER = 0, ADJ_PRESSURE_ITER = 1, ADJ_DIV_WARM = 2, ADJ_PRESSURE_WARM = 3 };
+enum AdjUpdateMode {
+  ADJ_DIV_ITER = 0, ADJ_PRESSURE_ITER = 1, ADJ_DIV_WARM = 2, ADJ_PRESSURE_WARM = 3,
+  ADJ_DIV_WARM_LEARNED = 4, ADJ_PRESSURE_WARM_LEARNED = 5
+};
 void launch_adj_update(const AdjPass& a, int mode, const float* k, const float4* vbar, const float* kacc,
                        float kacc_scale, const float* bval, const float* kprev, const float* alpha, float* out,
                        float* alpha_bar, float* rho_bar, float4* xbar, hipStream_t s);
 void launch_adj_delta(const AdjPass& a, const float4* v, const float* dbar, float4* vbar, float4* xbar,
                       hipStream_t s);
 void launch_adj_factor(const AdjPass& a, const float* alpha, const float* alpha_bar, const float* rho_bar,
-                       const float4* G, float4* xbar, hipStream_t s);
+                       const float4* G, float4* xbar, const float* rbb, hipStream_t s);
+void launch_adj_learned(int n, const float* fbar, int keep_kappa, int keep_kappav, float dt, float* kb, float* kvb,
+                        float* rhob, float* dbar, float* rbb, hipStream_t s);
 void launch_adj_position(int n, float dt, const float4* xbar, float4* vbar, hipStream_t s);
 void launch_adj_unpermute(int n, const uint32_t* perm, const float4* xb, const float4* vb, const float* kb,
                           const float* kvb, float4* xb2, float4* vb2, float* kb2, float* kvb2, hipStream_t s);
@@ -170,5 +178,47 @@ void launch_adj_boundary_reduce(const AdjPass& a, int body, int first, int n, Ad
 void launch_adj_rigid(const RigidStepConst& rc, const AdjStepTable& t, const RigidDev* forced, const RigidDev* pre,
                       const RigidDev* post, AdjBodyDev* ab, int nbodies, hipStream_t s);
 void set_adjoint_kernel_attributes();
+
+// ---------------------------------------------------------------------------
+// dfr_mlp.hip: learned kappa warm start (6 -> 64 -> 64 -> 64 -> 2, SiLU).
+// Flat FP32 parameters: W1[64][6] b1[64] W2[64][64] b2[64] W3[64][64] b3[64]
+// W4[2][64] b4[2].
+// ---------------------------------------------------------------------------
+constexpr int MLP_NP = 8898;
+
+struct MlpNorm {
+  float mu_in[6], is_in[6];    // feature mean and 1/std
+  float mu_out[2], sd_out[2];  // target (kappa_v, kappa) mean and std
+};
+
+struct MlpFields {            // the step's fields (sorted order)
+  const float* rho;
+  const float2* aux;          // (velocity divergence, boundary density) from the density pass
+  const uint32_t* cnt;
+  const float* kappa;
+  const float* kappav;
+  float rho0, dt;
+};
+
+size_t mlp_frag_bytes();
+void launch_mlp_pack(const float* P, void* frag, hipStream_t s);
+void launch_mlp_infer_fields(const void* frag, const float* P, const MlpNorm& norm, const MlpFields& f, int n,
+                             float* out, hipStream_t s);
+void launch_mlp_infer_array(const void* frag, const float* P, const MlpNorm& norm, const float* feat, int n,
+                            float* out, hipStream_t s);
+void launch_mlp_train_batch(const void* frag, const float* P, const MlpNorm& norm, const float* feat,
+                            const float* targ, const uint32_t* index, int batch, float* grad, double* loss,
+                            hipStream_t s);
+void launch_mlp_input_grad(const void* frag, const float* P, const MlpNorm& norm, const float* feat,
+                           const MlpFields* fields, int n, const float* ybar, float* fbar, hipStream_t s);
+void launch_mlp_adam(float* P, float* G, float* m, float* v, float lr, int t, hipStream_t s);
+void launch_mlp_batch(uint32_t* idx, int batch, uint64_t seed, uint32_t size, hipStream_t s);
+void launch_mlp_capture_features(const MlpFields& f, const uint32_t* id, int n, uint64_t salt, uint32_t threshold,
+                                 float* feat, uint32_t* rows, uint32_t* count, uint32_t cap, hipStream_t s);
+void launch_mlp_capture_targets(const float* kappa, const float* kappav, const uint32_t* rows, const uint32_t* first,
+                                uint32_t max_new, const uint32_t* count, uint32_t cap, float* targ, hipStream_t s);
+void launch_mlp_apply_div(int n, const float* pred, const float* d, float inv_dt, float* kappav, float* kpred,
+                          hipStream_t s);
+void set_mlp_kernel_attributes();
 
 }  // namespace dfr

@@ -168,7 +168,8 @@ constexpr int PT = per_thread<PB>();
 // (:1720-1732), one pass.
 __global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alpha, float4* __restrict__ s_out,
-                   float* __restrict__ badv, float* __restrict__ kappav, int div_warm, int divergence_on) {
+                   float* __restrict__ badv, float* __restrict__ kappav, int div_warm, int divergence_on,
+                   float2* __restrict__ aux) {
   __shared__ TileHdr<PB> T;
   extern __shared__ float lds[];
   const StepConst& c = a.c;
@@ -238,12 +239,14 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
     sumsq = c.V_f * c.V_f * sq;
     gx = c.V_f * fgx; gy = c.V_f * fgy; gz = c.V_f * fgz;
     delta = c.V_f * fdelta;
+    float bdens = 0.f;  // boundary part of the density sum (a feature of the learned warm start)
     for (uint32_t k = 0; k < nB; ++k) {
       const uint32_t j = a.nbrB[(size_t)k * stride + i];
       const float4 q = a.bpos[j];
       const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
       const PairKernel kw = cubic_pair(c.K, rx * rx + ry * ry + rz * rz);
       dens += q.w * kw.W;
+      bdens += q.w * kw.W;
       const float wx = kw.c * rx, wy = kw.c * ry, wz = kw.c * rz;
       gx += q.w * wx; gy += q.w * wy; gz += q.w * wz;
       if (divergence_on) {
@@ -252,6 +255,7 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
       }
     }
     rho[i] = dens * c.rho0;
+    if (aux) aux[i] = make_float2(delta, bdens);
     s_out[i] = make_float4(-gx, -gy, -gz, 0.f);
     sumsq += gx * gx + gy * gy + gz * gz;
     alpha[i] = (sumsq > c.eps) ? -1.f / sumsq : 0.f;
@@ -392,7 +396,8 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
 // LDS per slot: x, y, z, vx, vy, vz.
 __global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, const float* __restrict__ alpha,
-              float* __restrict__ badv, float* __restrict__ kf, float* __restrict__ kappa, double* __restrict__ err) {
+              float* __restrict__ badv, float* __restrict__ kf, float* __restrict__ kappa, double* __restrict__ err,
+              const float* __restrict__ kpred) {
   __shared__ TileHdr<PB> T;
   extern __shared__ float lds[];
   const StepConst& c = a.c;
@@ -466,14 +471,17 @@ k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, 
       if (mode == 1) {
         kf[i] = (b - 1.f) * alpha[i] * kcoef;
         e += (double)(c.rho0 * b - c.rho0);
-      } else {
+      } else if (mode == PRESSURE_WARMSTART) {
         const float invH2 = 1.f / (c.dt * c.dt);
         kappa[i] = (b > 1.f) ? 0.5f * fmaxf(kappa[i], -0.00025f) * invH2 : 0.f;
+      } else {  // learned warm start: the predicted stiffness (stored units)
+        const float invH2 = 1.f / (c.dt * c.dt);
+        kappa[i] = (b > 1.f) ? kpred[i] * invH2 : 0.f;
       }
     }
     badv[i] = b;
   }
-  if (mode != 2) block_add_double<PB>(e, err);
+  if (mode == RECOMPUTE_DIV || mode == RECOMPUTE_PRESSURE) block_add_double<PB>(e, err);
 }
 
 // Expands tile slots back to sorted fluid indices (diagnostics).
@@ -510,10 +518,10 @@ void launch_build_tiled(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBu
 }
 
 void launch_density_factor_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
-                             NbrBufs& nb, int div_warm, int divergence_on, hipStream_t s) {
+                             NbrBufs& nb, int div_warm, int divergence_on, float2* aux, hipStream_t s) {
   const size_t shm = sizeof(float) * (size_t)TILE_CAP * (divergence_on ? 6 : 3);
   hipLaunchKernelGGL(k_density_factor_t, dim3(nbricks(g)), dim3(PB), shm, s, make_args(c, g, gb, f, b, nb), f.rho,
-                     f.alpha, f.s, f.badv, f.kappav, div_warm, divergence_on);
+                     f.alpha, f.s, f.badv, f.kappav, div_warm, divergence_on, aux);
 }
 
 void launch_velocity_update_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
@@ -524,10 +532,10 @@ void launch_velocity_update_t(const StepConst& c, const GridDesc& g, GridBufs& g
 }
 
 void launch_recompute_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
-                        NbrBufs& nb, int mode, float kcoef, double* err_slot, hipStream_t s) {
+                        NbrBufs& nb, int mode, float kcoef, double* err_slot, hipStream_t s, const float* kpred) {
   const size_t shm = sizeof(float) * (size_t)TILE_CAP * 6;
   hipLaunchKernelGGL(k_recompute_t, dim3(nbricks(g)), dim3(PB), shm, s, make_args(c, g, gb, f, b, nb), mode, kcoef,
-                     f.rho, f.alpha, f.badv, f.kf, f.kappa, err_slot);
+                     f.rho, f.alpha, f.badv, f.kf, f.kappa, err_slot, kpred);
 }
 
 void launch_expand_lists(const GridDesc& g, GridBufs& gb, FluidBufs& f, NbrBufs& nb, uint32_t* out, hipStream_t s) {

@@ -22,8 +22,12 @@ EXPORTS = [
     "dfr_set_field", "dfr_get_boundary_field", "dfr_get_stats", "dfr_reset_stats", "dfr_neighbor_stats",
     "dfr_get_neighbors", "dfr_set_profiling", "dfr_kernel_times", "dfr_num_fluid", "dfr_num_boundary",
     "dfr_num_bodies", "dfr_adjoint_begin", "dfr_adjoint_backward", "dfr_adjoint_end", "dfr_adjoint_iterations",
-    "dfr_set_fixed_iterations",
+    "dfr_set_fixed_iterations", "dfr_mlp_capture", "dfr_mlp_dataset_size", "dfr_mlp_dataset", "dfr_mlp_init_params",
+    "dfr_mlp_set_params", "dfr_mlp_get_params", "dfr_mlp_train", "dfr_mlp_enable", "dfr_mlp_predict",
+    "dfr_mlp_loss_grad", "dfr_mlp_input_grad",
 ]
+
+MLP_NP = 8898
 
 
 class Params(C.Structure):
@@ -104,6 +108,19 @@ def load(path: str | None = None):
     L.dfr_adjoint_iterations.argtypes = [P, ip, ip, C.c_int64]
     L.dfr_adjoint_iterations.restype = C.c_int64
     L.dfr_set_fixed_iterations.argtypes = [P, C.c_int, ip, ip]
+    L.dfr_mlp_capture.argtypes = [P, C.c_int64, C.c_double, C.c_uint64]
+    L.dfr_mlp_dataset_size.argtypes = [P]
+    L.dfr_mlp_dataset_size.restype = C.c_int64
+    L.dfr_mlp_dataset.argtypes = [P, fp, fp, C.c_int64]
+    L.dfr_mlp_dataset.restype = C.c_int64
+    L.dfr_mlp_init_params.argtypes = [P, C.c_uint64]
+    L.dfr_mlp_set_params.argtypes = [P, fp, fp]
+    L.dfr_mlp_get_params.argtypes = [P, fp, fp]
+    L.dfr_mlp_train.argtypes = [P, C.c_int, C.c_int, C.c_double, C.c_uint64, dp]
+    L.dfr_mlp_enable.argtypes = [P, C.c_int]
+    L.dfr_mlp_predict.argtypes = [P, C.c_int64, fp, fp]
+    L.dfr_mlp_loss_grad.argtypes = [P, C.c_int64, fp, fp, fp, dp]
+    L.dfr_mlp_input_grad.argtypes = [P, C.c_int64, fp, fp, fp]
     _lib = L
     return L
 

@@ -301,6 +301,63 @@ class Simulation:
                                                             a.ctypes.data_as(C.POINTER(C.c_int32)),
                                                             b.ctypes.data_as(C.POINTER(C.c_int32))))
 
+    # -- learned warm start ----------------------------------------------------
+    def mlp_capture(self, capacity: int, fraction: float = 1.0, seed: int = 0):
+        """Record (features, converged (kappa_v, kappa)) pairs during the next steps."""
+        self._check(native.load().dfr_mlp_capture(self.ctx, int(capacity), float(fraction), int(seed)))
+
+    def mlp_dataset(self):
+        L = native.load()
+        n = int(self._check(L.dfr_mlp_dataset_size(self.ctx)))
+        f = np.zeros((max(n, 1), 6), np.float32)
+        t = np.zeros((max(n, 1), 2), np.float32)
+        self._check(L.dfr_mlp_dataset(self.ctx, _fp(f), _fp(t), n))
+        return f[:n], t[:n]
+
+    def mlp_init_params(self, seed: int = 0):
+        self._check(native.load().dfr_mlp_init_params(self.ctx, int(seed)))
+
+    def mlp_set_params(self, params: np.ndarray, norm: np.ndarray):
+        p = np.ascontiguousarray(params, np.float32)
+        q = np.ascontiguousarray(norm, np.float32)
+        assert p.shape == (native.MLP_NP,) and q.shape == (16,)
+        self._check(native.load().dfr_mlp_set_params(self.ctx, _fp(p), _fp(q)))
+
+    def mlp_get_params(self):
+        p = np.zeros(native.MLP_NP, np.float32)
+        q = np.zeros(16, np.float32)
+        self._check(native.load().dfr_mlp_get_params(self.ctx, _fp(p), _fp(q)))
+        return p, q
+
+    def mlp_train(self, iters: int, batch: int = 16384, lr: float = 2e-3, seed: int = 0) -> np.ndarray:
+        losses = np.zeros(int(iters))
+        self._check(native.load().dfr_mlp_train(self.ctx, int(iters), int(batch), float(lr), int(seed), _dp(losses)))
+        return losses
+
+    def mlp_enable(self, on: bool = True):
+        self._check(native.load().dfr_mlp_enable(self.ctx, int(on)))
+
+    def mlp_predict(self, feat: np.ndarray) -> np.ndarray:
+        f = np.ascontiguousarray(feat, np.float32).reshape(-1, 6)
+        out = np.zeros((f.shape[0], 2), np.float32)
+        self._check(native.load().dfr_mlp_predict(self.ctx, f.shape[0], _fp(f), _fp(out)))
+        return out
+
+    def mlp_loss_grad(self, feat: np.ndarray, targ: np.ndarray):
+        f = np.ascontiguousarray(feat, np.float32).reshape(-1, 6)
+        t = np.ascontiguousarray(targ, np.float32).reshape(-1, 2)
+        g = np.zeros(native.MLP_NP, np.float32)
+        loss = np.zeros(1)
+        self._check(native.load().dfr_mlp_loss_grad(self.ctx, f.shape[0], _fp(f), _fp(t), _fp(g), _dp(loss)))
+        return float(loss[0]), g
+
+    def mlp_input_grad(self, feat: np.ndarray, ybar: np.ndarray) -> np.ndarray:
+        f = np.ascontiguousarray(feat, np.float32).reshape(-1, 6)
+        y = np.ascontiguousarray(ybar, np.float32).reshape(-1, 2)
+        out = np.zeros((f.shape[0], 6), np.float32)
+        self._check(native.load().dfr_mlp_input_grad(self.ctx, f.shape[0], _fp(f), _fp(y), _fp(out)))
+        return out
+
     # -- TimeStepDiffDFSPH-level getters -------------------------------------
     def get_boundary_model(self, i: int) -> BoundaryModel:
         return BoundaryModel(self, i)

@@ -196,6 +196,39 @@ int64_t dfr_adjoint_iterations(const dfr_ctx* ctx, int* iv, int* ip, int64_t cap
  * n steps (n = 0 clears), e.g. to replay another run's iteration counts. */
 int dfr_set_fixed_iterations(dfr_ctx* ctx, int n, const int* iv, const int* ip);
 
+/* Learned kappa warm start (the learned part of this library; no counterpart
+ * in the reference, whose warm starts halve the previous step's stiffness,
+ * TimeStepDiffDFSPH.cpp:964-1059 and :1698-1788).  A 6 -> 64 -> 64 -> 64 -> 2
+ * MLP (SiLU, bf16 weights on MFMA, FP32 accumulation) maps per-particle
+ * features (rho/rho0 - 1, dt * velocity divergence, neighbour count, boundary
+ * density, previous kappa, previous kappa_v) to the stiffness the two solves
+ * converge to; the Jacobi solves still iterate to the same tolerance.
+ *
+ * dfr_mlp_capture(capacity, fraction, seed) records, during the following
+ * steps, (features, converged (kappa_v, kappa)) pairs of a hashed `fraction` of
+ * the particles into a device buffer of `capacity` rows (capacity 0 stops).
+ * dfr_mlp_train runs `iters` Adam steps (batch rows, cosine-decayed lr) on the
+ * captured pairs with the library's MFMA forward/backward kernels; losses[it]
+ * (optional) gets the mean squared normalised error of each batch.
+ * Parameters are a flat float[8898] (W1[64][6] b1[64] W2[64][64] b2[64]
+ * W3[64][64] b3[64] W4[2][64] b4[2]); norm is float[16] (feature mean[6],
+ * feature std[6], target mean[2], target std[2]).  dfr_mlp_enable switches the
+ * learned warm start on or off for the following steps. */
+int dfr_mlp_capture(dfr_ctx* ctx, int64_t capacity, double fraction, uint64_t seed);
+int64_t dfr_mlp_dataset_size(dfr_ctx* ctx);
+int64_t dfr_mlp_dataset(dfr_ctx* ctx, float* feat, float* targ, int64_t cap);
+int dfr_mlp_init_params(dfr_ctx* ctx, uint64_t seed);
+int dfr_mlp_set_params(dfr_ctx* ctx, const float* params, const float* norm);
+int dfr_mlp_get_params(dfr_ctx* ctx, float* params, float* norm);
+int dfr_mlp_train(dfr_ctx* ctx, int iters, int batch, double lr, uint64_t seed, double* losses);
+int dfr_mlp_enable(dfr_ctx* ctx, int on);
+/* The network on host arrays: predictions out[n][2] (kappa_v, kappa; stored
+ * units) for features feat[n][6]; the summed squared normalised error and the
+ * gradient of its mean over the n rows; dL/dfeatures for dL/dpredictions. */
+int dfr_mlp_predict(dfr_ctx* ctx, int64_t n, const float* feat, float* out);
+int dfr_mlp_loss_grad(dfr_ctx* ctx, int64_t n, const float* feat, const float* targ, float* grad, double* loss);
+int dfr_mlp_input_grad(dfr_ctx* ctx, int64_t n, const float* feat, const float* ybar, float* fbar);
+
 /* Number of fluid particles / boundary samples / bodies. */
 int64_t dfr_num_fluid(const dfr_ctx* ctx);
 int64_t dfr_num_boundary(const dfr_ctx* ctx, int body);

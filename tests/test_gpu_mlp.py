@@ -1,0 +1,146 @@
+"""Learned warm start on the GPU: the bf16 MFMA network kernels against the
+numpy restatement (oracle/mlp_oracle.py, pinned by test_mlp_oracle.py), and
+the warm start's effect on the Jacobi solves.
+
+Tolerances: bf16 operands make a rounding decision flip when the GPU's FP32
+sums (in MFMA order) and numpy's differ in the last bits; such flips move an
+output by ~2^-8 of a layer's scale, so outputs and gradients are compared at
+2e-2 of their largest magnitude (relative L2 norm of the gradient below 1e-2).
+"""
+import numpy as np
+import pytest
+
+from difffr_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _sim(sc=None, **params):
+    from difffr_amd.simulation import Simulation
+    sc = sc or scenes.submerged_body(steps=5)
+    return Simulation(sc, params_override=params or None)
+
+
+def _random_net(seed):
+    from oracle import mlp_oracle as mo
+    rng = np.random.default_rng(seed)
+    P = np.zeros(mo.NP, np.float32)
+    o = 0
+    for m, n in mo.SIZES:
+        P[o:o + m * n] = rng.uniform(-1, 1, m * n) * np.sqrt(6.0 / n)
+        o += m * n
+        P[o:o + m] = rng.normal(scale=0.1, size=m)
+        o += m
+    norm = np.concatenate([rng.normal(size=6), rng.uniform(0.5, 2.0, 6), rng.normal(size=2),
+                           rng.uniform(0.5, 2.0, 2)]).astype(np.float32)
+    return P, norm, rng
+
+
+def test_mlp_forward_matches_oracle(gpu_ready):
+    from oracle import mlp_oracle as mo
+    sim = _sim()
+    P, norm, rng = _random_net(0)
+    sim.mlp_set_params(P, norm)
+    feat = (norm[:6] + rng.normal(size=(1000, 6)) * norm[6:12]).astype(np.float32)  # 1000 = 15 waves + 40
+    out = sim.mlp_predict(feat)
+    ref = mo.predict(P, norm, feat)
+    print("max |gpu - oracle|", np.abs(out - ref).max(), "scale", np.abs(ref).max(), flush=True)
+    assert np.all(np.isfinite(out))
+    np.testing.assert_allclose(out, ref, rtol=0, atol=2e-2 * np.abs(ref).max())
+    # the exact (unrounded) network is close too: bf16 is a small perturbation
+    ref64 = mo.predict(P, norm, feat, exact=True)
+    assert np.abs(out - ref64).max() <= 5e-2 * np.abs(ref64).max()
+
+
+def test_mlp_loss_gradient_matches_oracle(gpu_ready):
+    from oracle import mlp_oracle as mo
+    sim = _sim()
+    P, norm, rng = _random_net(1)
+    sim.mlp_set_params(P, norm)
+    n = 777
+    feat = (norm[:6] + rng.normal(size=(n, 6)) * norm[6:12]).astype(np.float32)
+    targ = (norm[12:14] + rng.normal(size=(n, 2)) * norm[14:16]).astype(np.float32)
+    loss, g = sim.mlp_loss_grad(feat, targ)
+    loss_ref, g_ref = mo.loss_grad(P, norm, feat, targ)
+    rel = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    print("loss", loss, loss_ref, "grad rel L2", rel, flush=True)
+    assert abs(loss - loss_ref) <= 1e-2 * loss_ref
+    assert rel < 1e-2
+    # every layer's block individually
+    o = 0
+    for m, k in mo.SIZES:
+        blk = slice(o, o + m * k + m)
+        assert np.linalg.norm(g[blk] - g_ref[blk]) <= 2e-2 * np.linalg.norm(g_ref[blk])
+        o += m * k + m
+
+
+def test_mlp_input_gradient_matches_oracle(gpu_ready):
+    from oracle import mlp_oracle as mo
+    sim = _sim()
+    P, norm, rng = _random_net(2)
+    sim.mlp_set_params(P, norm)
+    n = 300
+    feat = (norm[:6] + rng.normal(size=(n, 6)) * norm[6:12]).astype(np.float32)
+    ybar = rng.normal(size=(n, 2)).astype(np.float32)
+    fb = sim.mlp_input_grad(feat, ybar)
+    ref = mo.input_grad(P, norm, feat, ybar)
+    print("max |gpu - oracle|", np.abs(fb - ref).max(), "scale", np.abs(ref).max(), flush=True)
+    np.testing.assert_allclose(fb, ref, rtol=0, atol=2e-2 * np.abs(ref).max())
+
+
+def _capture_and_train(sim, steps, iters, fraction=0.5, seed=0):
+    sim.mlp_capture(capacity=4_000_000, fraction=fraction, seed=seed)
+    sim.step(steps)
+    sim.mlp_capture(0)
+    feat, targ = sim.mlp_dataset()
+    losses = sim.mlp_train(iters, batch=8192, lr=2e-3, seed=seed)
+    return feat, targ, losses
+
+
+def test_mlp_training_fits_captured_pairs(gpu_ready):
+    sc = scenes.small_tank(counts=(30, 14, 30), seed=0, steps=60, body="sphere")
+    sim = _sim(sc, compute_gradient=0)
+    feat, targ, losses = _capture_and_train(sim, 60, 600)
+    print("pairs", feat.shape[0], "loss first/last", losses[:20].mean(), losses[-20:].mean(), flush=True)
+    assert feat.shape[0] > 10000
+    assert np.all(np.isfinite(losses))
+    assert losses[-20:].mean() < 0.5 * losses[:20].mean()
+    # the trained network reproduces its training targets better than predicting the mean
+    rows = np.random.default_rng(0).choice(feat.shape[0], 20000, replace=False)
+    pred = sim.mlp_predict(feat[rows])
+    err = np.mean((pred - targ[rows]) ** 2, axis=0)
+    var = np.var(targ[rows], axis=0)
+    assert np.all(err < 0.5 * var)
+
+
+def _deep_tank(seed, steps):
+    # 28 particle layers, the depth of the 1M-particle tank, so the plain solver needs more than its minimum
+    return scenes.tank((24, 28, 24), 0.02, "sphere", seed=seed, steps=steps, wall_height=1.0)
+
+
+def test_learned_warm_start_saves_iterations(gpu_ready):
+    """Train on one seed's rollout, run another seed with and without the
+    learned warm start: same tolerance, fewer pressure iterations."""
+    sim = _sim(_deep_tank(0, 200), compute_gradient=0)
+    _capture_and_train(sim, 200, 2000, fraction=0.5)
+    P, norm = sim.mlp_get_params()
+    sim.close()
+    res = {}
+    for learned in (False, True):
+        s = _sim(_deep_tank(7, 160), compute_gradient=0)
+        s.step(100)
+        if learned:
+            s.mlp_set_params(P, norm)
+            s.mlp_enable(True)
+        s.reset_stats()
+        errs = []
+        for _ in range(60):
+            s.step(1)
+            errs.append(s.stats()["avg_density_err"])
+        st = s.stats()
+        res[learned] = (st["total_iterations"] / 60, st["total_iterations_v"] / 60, max(errs))
+        s.close()
+    print("plain (I_p, I_v, max err)", res[False], "learned", res[True], flush=True)
+    eta = 0.01 * 0.01 * 1000.0
+    assert res[True][2] <= eta + 1e-9 and res[False][2] <= eta + 1e-9
+    assert res[True][0] < 0.8 * res[False][0]

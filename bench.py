@@ -2,10 +2,10 @@
 
 Prints ONE JSON line.  The timed region is a K-step rollout of the 1M-particle
 tank (neighbour search, density/factor, divergence and pressure Jacobi solves
-with Akinci coupling, rigid update, a checkpoint every `--checkpoint-every`
-steps) followed by its exact adjoint (every step re-run once recording its
-Jacobi fields, then the reverse kernels), i.e. forward + adjoint particle-steps
-per second.  Inputs are resident in HBM; the rollout starts from a state
+with Akinci coupling, rigid update) that keeps every step's tape in HBM
+(`--checkpoint-every 0`; with k >= 1 it checkpoints every k steps and re-runs
+the steps during the backward pass instead), followed by its exact adjoint
+(the reverse kernels), i.e. forward + adjoint particle-steps per second.  Inputs are resident in HBM; the rollout starts from a state
 settled by W untimed steps.  It is timed twice on the same seed: with the
 reference's warm start (halved previous kappa) and with the learned warm start,
 whose network is first trained (untimed) on pairs captured from a rollout of
@@ -37,7 +37,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--workload", default="tank_1m", choices=["tank_1m", "tank_250k", "tank_4m"])
-    ap.add_argument("--checkpoint-every", type=int, default=1)
+    ap.add_argument("--checkpoint-every", type=int, default=0,
+                    help="0: keep every step's tape in HBM (no recomputation); k >= 1: checkpoint every k steps "
+                         "and re-run the steps during the backward pass")
     ap.add_argument("--no-learned", action="store_true", help="measure the plain warm start only")
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--train-iters", type=int, default=2000)
@@ -134,10 +136,15 @@ def timed_run(args, sc, learned, P=None, norm=None):
     from difffr_amd.simulation import Simulation
     # the exact adjoint replaces DiffFR's forward-mode gradient on this path
     sim = Simulation(sc, params_override={"compute_gradient": 0})
-    sim.step(args.warmup)
+    grow = min(args.steps, args.warmup)
+    sim.step(args.warmup - grow)
     if learned:
         sim.mlp_set_params(P, norm)
         sim.mlp_enable(True)
+    # the last `grow` warm-up steps record an (untimed) rollout, so the tape arena
+    # and checkpoint blocks are allocated before the timed one reuses them
+    sim.adjoint_begin(args.checkpoint_every)
+    sim.step(grow)
     sim.reset_stats()
     seed = np.zeros((sim.n_bodies, 7))
     seed[-1] = np.random.default_rng(args.seed).normal(size=7)
@@ -217,9 +224,11 @@ def main():
         "config": {
             "workload": f"{sc.name}: {n} fluid particles + {sc.n_boundary} boundary samples, "
                         f"dt=1e-3, 0.01% density / 0.1% divergence tolerance",
-            "path": f"{'learned' if learned_on else 'plain'} kappa warm start; forward rollout with a checkpoint "
-                    f"every {args.checkpoint_every} step(s) + exact adjoint (record-replay of every step + "
-                    f"reverse kernels) of dL/d(v0, omega0)",
+            "path": f"{'learned' if learned_on else 'plain'} kappa warm start; forward rollout + exact adjoint of "
+                    f"dL/d(v0, omega0); "
+                    + ("every step's tape kept in HBM (no recomputation)" if args.checkpoint_every == 0 else
+                       f"checkpoint every {args.checkpoint_every} step(s), steps re-run with recording in the "
+                       f"backward pass"),
             "mean_fluid_neighbors": nst["mean_fluid"], "mean_boundary_neighbors": nst["mean_boundary"],
             "plain": summary(plain),
         },

@@ -24,13 +24,13 @@ namespace {
 
 enum TimerClass {
   TC_SORT = 0, TC_NBR, TC_DENSITY, TC_VUPDATE, TC_RECOMPUTE, TC_PWARM, TC_ELEMWISE, TC_GRADIENT, TC_RIGID,
-  TC_ADJ_UPDATE, TC_ADJ_DELTA, TC_ADJ_FACTOR, TC_ADJ_MISC, TC_CHECKPOINT, TC_MLP, TC_COUNT
+  TC_ADJ_UPDATE, TC_ADJ_DELTA, TC_ADJ_FACTOR, TC_ADJ_MISC, TC_CHECKPOINT, TC_MLP, TC_ADJ_MLP, TC_COUNT
 };
 const char* kTimerNames[TC_COUNT] = {"grid_sort", "neighbor_build", "density_factor", "velocity_update",
                                      "density_recompute", "pressure_warmstart", "advect_position",
                                      "gradient_pairs", "rigid_step", "adj_velocity_update",
                                      "adj_density_change", "adj_density_factor", "adj_rigid_misc",
-                                     "checkpoint_copy", "mlp_warmstart"};
+                                     "checkpoint_copy", "mlp_warmstart", "adj_mlp_warmstart"};
 
 struct BodyInfo {
   bool dynamic = false;
@@ -56,9 +56,45 @@ struct Snapshot {
   std::vector<uint8_t> animated;
 };
 
-// What the reverse pass of one step reads besides the step's live buffers
-// (neighbour lists, grid, permutation, density, factor, sum_grad_p_k): the
-// values that the step overwrote while it ran.
+// Bump allocator over large device chunks for per-step tapes (kept across
+// rollouts, freed by dfr_adjoint_end / dfr_destroy).
+struct Arena {
+  std::vector<std::pair<char*, size_t>> chunks;
+  size_t cur = 0, off = 0;
+  static constexpr size_t CHUNK = (size_t)2 << 30;
+  void* alloc(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    while (cur < chunks.size() && chunks[cur].second - off < bytes) { ++cur; off = 0; }
+    if (cur == chunks.size()) {
+      const size_t sz = std::max(bytes, CHUNK);
+      void* p = nullptr;
+      const hipError_t e = hipMalloc(&p, sz);
+      if (e != hipSuccess) throw std::runtime_error("out of device memory for the adjoint tape "
+                                                    "(use checkpointing: dfr_adjoint_begin with checkpoint_every >= 1)");
+      chunks.push_back({static_cast<char*>(p), sz});
+      off = 0;
+    }
+    void* p = chunks[cur].first + off;
+    off += bytes;
+    return p;
+  }
+  size_t used() const {
+    size_t u = 0;
+    for (size_t k = 0; k < cur && k < chunks.size(); ++k) u += chunks[k].second;
+    return u + off;
+  }
+  void reset() { cur = 0; off = 0; }
+  void release() {
+    for (auto& ch : chunks) (void)hipFree(ch.first);
+    chunks.clear();
+    reset();
+  }
+};
+
+// What the reverse pass of one step reads: the values that the step
+// overwrote while it ran, and the step's structures (neighbour lists, grid,
+// permutation, density, factor, sum_grad_p_k), which are the live buffers
+// right after a replayed step or copies kept in the arena.
 struct StepTapeDev {
   float4* x = nullptr;                    // fluid positions during the step (sorted order)
   float4* v0 = nullptr;                   // fluid velocities after the sort (learned warm-start feature)
@@ -70,6 +106,14 @@ struct StepTapeDev {
   std::vector<float*> kD, bD, kP, bP;     // k_i and d / rho* of each iteration
   int Iv = 0, Ip = 0;
   AdjStepTable tbl{};
+  Arena* arena = nullptr;                 // set: per-step buffers come from the arena
+  // structures of the step
+  const uint32_t *fstart = nullptr, *perm = nullptr, *nbrB = nullptr, *cnt = nullptr;
+  const uint4* nbr8 = nullptr;
+  const float *rho = nullptr, *alpha = nullptr;
+  const float4* s = nullptr;
+  const float2* aux = nullptr;
+  const RigidDev* rig_post = nullptr;
 };
 
 // Learned kappa warm start: parameters, Adam state, captured training pairs.
@@ -101,7 +145,20 @@ struct AdjState {
   bool active = false;
   int k = 1;
   int64_t steps = 0;                       // primal steps since dfr_adjoint_begin
-  std::vector<Snapshot> ckpt;              // state before steps 0, k, 2k, ...
+  std::vector<Snapshot> ckpt;              // state before steps 0, k, 2k, ... (k >= 1); blocks reused
+  size_t nckpt = 0;                        // checkpoints of the current rollout
+  std::vector<StepTapeDev> tapes;          // k == 0: every step's tape, kept in HBM
+  Arena arena;
+  // kept tapes: the steps write their structures straight into arena buffers;
+  // the context's own buffers are swapped back in when the rollout ends
+  struct Live {
+    uint4* nbr8; uint32_t *nbrB, *cnt, *fstart, *perm;
+    float *rho, *alpha;
+    float4* s;
+    float2* aux;
+  };
+  bool swapped = false;
+  Live own{};
   std::vector<Snapshot> seg;               // states inside the segment being reversed
   Snapshot final_state;
   std::vector<int> iv, ip;                 // iteration counts of the primal steps
@@ -417,12 +474,16 @@ template <typename T> void d2d(dfr_ctx* c, T* dst, const T* src, size_t n) {
 }
 
 // Per-iteration tape buffers, grown on first use.
+template <typename T> T* talloc(dfr_ctx* c, StepTapeDev* rec, size_t n) {
+  return rec && rec->arena ? static_cast<T*>(rec->arena->alloc(sizeof(T) * std::max<size_t>(n, 1))) : dalloc<T>(c, n);
+}
+
 void tape_iteration(dfr_ctx* c, std::vector<float4*>& v, std::vector<float*>& k, std::vector<float*>& b, int m) {
   const size_t n = (size_t)std::max(c->f.n, 1);
   while ((int)v.size() <= m) {
-    v.push_back(dalloc<float4>(c, n));
-    k.push_back(dalloc<float>(c, n));
-    b.push_back(dalloc<float>(c, n));
+    v.push_back(talloc<float4>(c, c->rec, n));
+    k.push_back(talloc<float>(c, c->rec, n));
+    b.push_back(talloc<float>(c, c->rec, n));
   }
   d2d(c, v[m], c->f.vel, c->f.n);
   d2d(c, k[m], c->f.kf, c->f.n);
@@ -713,32 +774,117 @@ void adjoint_buffers(dfr_ctx* c) {
   A.bufs = true;
 }
 
+// A stored step's own buffers, from the arena.
+void tape_alloc_fixed(dfr_ctx* c, StepTapeDev& T) {
+  Arena& R = c->adj.arena;
+  T.arena = &R;
+  const size_t nf = (size_t)std::max(c->f.n, 1), nb = (size_t)std::max(c->b.n, 1),
+               nr = (size_t)std::max<size_t>(c->bodies.size(), 1);
+  auto fa = [&](size_t n) { return static_cast<float*>(R.alloc(sizeof(float) * n)); };
+  T.x = static_cast<float4*>(R.alloc(sizeof(float4) * nf));
+  T.v0 = c->mlp.enabled ? static_cast<float4*>(R.alloc(sizeof(float4) * nf)) : nullptr;
+  T.kvPrev = fa(nf); T.d0 = fa(nf); T.kvW = fa(nf); T.kPrev = fa(nf); T.rsW = fa(nf); T.kW = fa(nf);
+  T.bpos = static_cast<float4*>(R.alloc(sizeof(float4) * nb));
+  T.bvel = static_cast<float4*>(R.alloc(sizeof(float4) * nb));
+  T.rig_pre = static_cast<RigidDev*>(R.alloc(sizeof(RigidDev) * nr));
+  T.rig_forced = static_cast<RigidDev*>(R.alloc(sizeof(RigidDev) * nr));
+}
+
+// The reverse pass right after a replayed step reads the live structures.
+void live_view(dfr_ctx* c, StepTapeDev& T) {
+  T.fstart = c->gb.fstart; T.perm = c->gb.perm;
+  T.nbr8 = c->nb.nbr8; T.nbrB = c->nb.nbrB; T.cnt = c->nb.cnt;
+  T.rho = c->f.rho; T.alpha = c->f.alpha; T.s = c->f.s; T.aux = c->mlp.aux;
+  T.rig_post = c->d_rig;
+}
+
+template <typename T0> const T0* arena_copy(dfr_ctx* c, const T0* src, size_t n) {
+  T0* dst = static_cast<T0*>(c->adj.arena.alloc(sizeof(T0) * std::max<size_t>(n, 1)));
+  d2d(c, dst, src, n);
+  return dst;
+}
+
+// Points the context's per-step structures (neighbour lists, grid offsets,
+// permutation, density, factor, sum_grad_p_k, network features) at fresh arena
+// buffers, so the step about to run leaves them in place as part of its tape.
+void tape_swap_in(dfr_ctx* c, StepTapeDev& T) {
+  AdjState& A = c->adj;
+  Arena& R = A.arena;
+  const size_t NF = (size_t)std::max(c->f.n, 1);
+  if (!A.swapped) {
+    A.own = {c->nb.nbr8, c->nb.nbrB, c->nb.cnt, c->gb.fstart, c->gb.perm, c->f.rho, c->f.alpha, c->f.s, c->mlp.aux};
+    A.swapped = true;
+  }
+  c->nb.nbr8 = static_cast<uint4*>(R.alloc(sizeof(uint4) * NPACK * NF));
+  c->nb.nbrB = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * (size_t)c->nb.maxb * NF));
+  c->nb.cnt = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * NF));
+  c->gb.fstart = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * ((size_t)c->grid.nkeys + 1)));
+  c->gb.perm = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * NF));
+  c->f.rho = static_cast<float*>(R.alloc(sizeof(float) * NF));
+  c->f.alpha = static_cast<float*>(R.alloc(sizeof(float) * NF));
+  c->f.s = static_cast<float4*>(R.alloc(sizeof(float4) * NF));
+  if (c->mlp.aux) c->mlp.aux = static_cast<float2*>(R.alloc(sizeof(float2) * NF));
+  T.nbr8 = c->nb.nbr8; T.nbrB = c->nb.nbrB; T.cnt = c->nb.cnt; T.fstart = c->gb.fstart; T.perm = c->gb.perm;
+  T.rho = c->f.rho; T.alpha = c->f.alpha; T.s = c->f.s; T.aux = c->mlp.aux;
+}
+
+// Ends the swap: the context's own buffers receive the last step's structures.
+void tape_swap_out(dfr_ctx* c) {
+  AdjState& A = c->adj;
+  if (!A.swapped) return;
+  const int nf = c->f.n;
+  const auto& o = A.own;
+  d2d(c, o.nbr8, (const uint4*)c->nb.nbr8, (size_t)NPACK * nf);
+  d2d(c, o.nbrB, (const uint32_t*)c->nb.nbrB, (size_t)c->nb.maxb * nf);
+  d2d(c, o.cnt, (const uint32_t*)c->nb.cnt, (size_t)nf);
+  d2d(c, o.fstart, (const uint32_t*)c->gb.fstart, (size_t)c->grid.nkeys + 1);
+  d2d(c, o.perm, (const uint32_t*)c->gb.perm, (size_t)nf);
+  d2d(c, o.rho, (const float*)c->f.rho, (size_t)nf);
+  d2d(c, o.alpha, (const float*)c->f.alpha, (size_t)nf);
+  d2d(c, o.s, (const float4*)c->f.s, (size_t)nf);
+  if (o.aux) d2d(c, o.aux, (const float2*)c->mlp.aux, (size_t)nf);
+  c->nb.nbr8 = o.nbr8; c->nb.nbrB = o.nbrB; c->nb.cnt = o.cnt; c->gb.fstart = o.fstart; c->gb.perm = o.perm;
+  c->f.rho = o.rho; c->f.alpha = o.alpha; c->f.s = o.s; c->mlp.aux = o.aux;
+  A.swapped = false;
+}
+
+// Ends the current rollout; device blocks (checkpoints, tape arena) stay
+// allocated for the next one until adjoint_free.
 void adjoint_release(dfr_ctx* c) {
   AdjState& A = c->adj;
+  tape_swap_out(c);
+  A.nckpt = 0;
+  A.iv.clear();
+  A.ip.clear();
+  A.tapes.clear();
+  A.arena.reset();
+  A.steps = 0;
+  A.active = false;
+}
+
+void adjoint_free(dfr_ctx* c) {
+  AdjState& A = c->adj;
+  adjoint_release(c);
   for (auto& S : A.ckpt) snapshot_free(S);
   for (auto& S : A.seg) snapshot_free(S);
   snapshot_free(A.final_state);
   A.ckpt.clear();
   A.seg.clear();
-  A.iv.clear();
-  A.ip.clear();
-  A.steps = 0;
-  A.active = false;
+  A.arena.release();
 }
 
 // Reverse pass of the step just replayed with c->rec = &tape: the adjoint
 // fields (sorted order of that step) become the adjoint of the state before
 // the step (order of the previous step), and the rigid adjoints move back one
 // step.
-void reverse_step(dfr_ctx* c) {
+void reverse_step(dfr_ctx* c, const StepTapeDev& T) {
   AdjState& A = c->adj;
-  const StepTapeDev& T = A.tape;
   const int nf = c->f.n;
   const float dt = c->dtf;
   const int nbodies = (int)c->bodies.size();
   AdjPass a{};
-  a.c = c->sc; a.g = c->grid; a.fstart = c->gb.fstart; a.x = T.x;
-  a.nbr8 = c->nb.nbr8; a.nbrB = c->nb.nbrB; a.cnt = c->nb.cnt;
+  a.c = c->sc; a.g = c->grid; a.fstart = T.fstart; a.x = T.x;
+  a.nbr8 = T.nbr8; a.nbrB = T.nbrB; a.cnt = T.cnt;
   a.bx = T.bpos; a.bv = T.bvel; a.bx0 = c->b.x0; a.nb_static = c->b.n - c->b.nd;
   a.sxb = A.sxb; a.svb = A.svb; a.body = A.body;
   {
@@ -747,7 +893,7 @@ void reverse_step(dfr_ctx* c) {
     rc.dt = (double)dt;
     rc.gyroscopic = c->P.gyroscopic;
     rc.nbodies = nbodies;
-    launch_adj_rigid(rc, T.tbl, T.rig_forced, T.rig_pre, c->d_rig, A.body, nbodies, c->stream);
+    launch_adj_rigid(rc, T.tbl, T.rig_forced, T.rig_pre, T.rig_post, A.body, nbodies, c->stream);
     if (c->b.nd > 0) {
       HIPCHK(hipMemsetAsync(A.sxb, 0, sizeof(double) * 3 * c->b.nd, c->stream));
       HIPCHK(hipMemsetAsync(A.svb, 0, sizeof(double) * 3 * c->b.nd, c->stream));
@@ -768,7 +914,7 @@ void reverse_step(dfr_ctx* c) {
       {
         Timed t(c, TC_ADJ_UPDATE);
         launch_adj_update(a, ADJ_PRESSURE_ITER, T.kP[m], A.vb, warmP ? A.kb : nullptr, h2, T.bP[m], nullptr,
-                          c->f.alpha, A.dbar, A.alb, A.rhob, A.xb, c->stream);
+                          T.alpha, A.dbar, A.alb, A.rhob, A.xb, c->stream);
       }
       Timed t(c, TC_ADJ_DELTA);
       launch_adj_delta(a, T.vP[m], A.dbar, A.vb, A.xb, c->stream);
@@ -776,10 +922,10 @@ void reverse_step(dfr_ctx* c) {
     if (warmP) {
       Timed t(c, TC_ADJ_UPDATE);
       if (learned)
-        launch_adj_update(a, ADJ_PRESSURE_WARM_LEARNED, T.kW, A.vb, A.kb, h2, T.rsW, nullptr, c->f.alpha, A.ybar,
+        launch_adj_update(a, ADJ_PRESSURE_WARM_LEARNED, T.kW, A.vb, A.kb, h2, T.rsW, nullptr, T.alpha, A.ybar,
                           A.alb, A.rhob, A.xb, c->stream);
       else
-        launch_adj_update(a, ADJ_PRESSURE_WARM, T.kW, A.vb, A.kb, h2, T.rsW, T.kPrev, c->f.alpha, A.kb, A.alb,
+        launch_adj_update(a, ADJ_PRESSURE_WARM, T.kW, A.vb, A.kb, h2, T.rsW, T.kPrev, T.alpha, A.kb, A.alb,
                           A.rhob, A.xb, c->stream);
     }
     if (c->P.enable_divergence_solver) {
@@ -788,7 +934,7 @@ void reverse_step(dfr_ctx* c) {
         {
           Timed t(c, TC_ADJ_UPDATE);
           launch_adj_update(a, ADJ_DIV_ITER, T.kD[m], A.vb, warmV ? A.kvb : nullptr, dt, T.bD[m], nullptr,
-                            c->f.alpha, A.dbar, A.alb, A.rhob, A.xb, c->stream);
+                            T.alpha, A.dbar, A.alb, A.rhob, A.xb, c->stream);
         }
         Timed t(c, TC_ADJ_DELTA);
         launch_adj_delta(a, T.vD[m], A.dbar, A.vb, A.xb, c->stream);
@@ -796,17 +942,17 @@ void reverse_step(dfr_ctx* c) {
       if (warmV) {
         Timed t(c, TC_ADJ_UPDATE);
         if (learned)
-          launch_adj_update(a, ADJ_DIV_WARM_LEARNED, T.kvW, A.vb, A.kvb, dt, T.d0, nullptr, c->f.alpha, A.ybar,
+          launch_adj_update(a, ADJ_DIV_WARM_LEARNED, T.kvW, A.vb, A.kvb, dt, T.d0, nullptr, T.alpha, A.ybar,
                             A.alb, A.rhob, A.xb, c->stream);
         else
-          launch_adj_update(a, ADJ_DIV_WARM, T.kvW, A.vb, A.kvb, dt, T.d0, T.kvPrev, c->f.alpha, A.kvb, A.alb,
+          launch_adj_update(a, ADJ_DIV_WARM, T.kvW, A.vb, A.kvb, dt, T.d0, T.kvPrev, T.alpha, A.kvb, A.alb,
                             A.rhob, A.xb, c->stream);
       }
     }
     if (learned) {
       // through the network to its features, then to the state they were computed from
-      Timed t(c, TC_MLP);
-      const MlpFields fields{c->f.rho, c->mlp.aux, c->nb.cnt, T.kPrev, T.kvPrev, c->sc.rho0, dt};
+      Timed t(c, TC_ADJ_MLP);
+      const MlpFields fields{T.rho, T.aux, T.cnt, T.kPrev, T.kvPrev, c->sc.rho0, dt};
       launch_mlp_input_grad(c->mlp.frag, c->mlp.P, c->mlp.norm, nullptr, &fields, nf, A.ybar, A.fbar, c->stream);
       const bool divWarm = c->P.enable_divergence_solver && c->P.use_divergence_warmstart;
       launch_adj_learned(nf, A.fbar, warmP ? 0 : 1, divWarm ? 0 : 1, dt, A.kb, A.kvb, A.rhob, A.dbar, A.rbb,
@@ -815,7 +961,7 @@ void reverse_step(dfr_ctx* c) {
     }
     {
       Timed t(c, TC_ADJ_FACTOR);
-      launch_adj_factor(a, c->f.alpha, A.alb, A.rhob, c->f.s, A.xb, learned ? A.rbb : nullptr, c->stream);
+      launch_adj_factor(a, T.alpha, A.alb, A.rhob, T.s, A.xb, learned ? A.rbb : nullptr, c->stream);
     }
   }
   Timed t(c, TC_ADJ_MISC);
@@ -824,7 +970,7 @@ void reverse_step(dfr_ctx* c) {
     if (bi.dynamic && bi.n > 0) launch_adj_boundary_reduce(a, k, (int)bi.dyn_first, (int)bi.n, A.body, c->stream);
   }
   if (nf > 0) {
-    launch_adj_unpermute(nf, c->gb.perm, A.xb, A.vb, A.kb, A.kvb, A.xb2, A.vb2, A.kb2, A.kvb2, c->stream);
+    launch_adj_unpermute(nf, T.perm, A.xb, A.vb, A.kb, A.kvb, A.xb2, A.vb2, A.kb2, A.kvb2, c->stream);
     std::swap(A.xb, A.xb2); std::swap(A.vb, A.vb2); std::swap(A.kb, A.kb2); std::swap(A.kvb, A.kvb2);
   }
 }
@@ -993,7 +1139,10 @@ int dfr_destroy(dfr_ctx* c) {
   if (!c) return DFR_ERR_ARG;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  adjoint_release(c);
+  try {
+    adjoint_free(c);
+  } catch (...) {
+  }
   for (auto& r : c->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   for (void* p : c->allocations) (void)hipFree(p);
@@ -1159,11 +1308,22 @@ int dfr_step(dfr_ctx* c, int nsteps) {
     HIPCHK(hipSetDevice(c->device));
     AdjState& A = c->adj;
     for (int s = 0; s < nsteps; ++s) {
-      if (A.active && A.steps % A.k == 0) {
-        A.ckpt.emplace_back();
-        snapshot_save(c, A.ckpt.back());
+      if (A.active && A.k == 0) {  // keep every step's tape in HBM
+        A.tapes.emplace_back();
+        StepTapeDev& T = A.tapes.back();
+        tape_alloc_fixed(c, T);
+        tape_swap_in(c, T);
+        c->rec = &T;
+        do_step(c);
+        c->rec = nullptr;
+        T.rig_post = arena_copy(c, (const RigidDev*)c->d_rig, c->bodies.size());
+      } else {
+        if (A.active && A.steps % A.k == 0) {
+          if (A.nckpt == A.ckpt.size()) A.ckpt.emplace_back();
+          snapshot_save(c, A.ckpt[A.nckpt++]);
+        }
+        do_step(c);
       }
-      do_step(c);
       if (A.active) {
         A.iv.push_back(c->stats.iterations_v);
         A.ip.push_back(c->stats.iterations);
@@ -1428,7 +1588,7 @@ int dfr_adjoint_begin(dfr_ctx* c, int checkpoint_every) {
     HIPCHK(hipSetDevice(c->device));
     adjoint_release(c);
     adjoint_buffers(c);
-    c->adj.k = checkpoint_every > 0 ? checkpoint_every : 1;
+    c->adj.k = checkpoint_every > 0 ? checkpoint_every : 0;
     c->adj.active = true;
     return DFR_OK;
   });
@@ -1442,7 +1602,8 @@ int dfr_adjoint_backward(dfr_ctx* c, const double* seed, double* grad) {
     HIPCHK(hipSetDevice(c->device));
     const int nbodies = (int)c->bodies.size();
     const dfr_stats saved_stats = c->stats;
-    snapshot_save(c, A.final_state);
+    const bool stored = A.k == 0;
+    if (!stored) snapshot_save(c, A.final_state);
     std::vector<AdjBodyDev> hb(std::max(nbodies, 1));
     std::memset(hb.data(), 0, sizeof(AdjBodyDev) * hb.size());
     for (int b = 0; b < nbodies; ++b) {
@@ -1460,9 +1621,13 @@ int dfr_adjoint_backward(dfr_ctx* c, const double* seed, double* grad) {
       explicit ReplayFlag(dfr_ctx* x) : c(x) { c->replay = true; }
       ~ReplayFlag() { c->replay = false; c->rec = nullptr; c->force_iv = c->force_ip = -1; }
     } flag(c);
-    const int64_t S = A.steps, k = A.k;
-    if (k > 1 && (int64_t)A.seg.size() < k) A.seg.resize(k);
-    for (int64_t seg = (S - 1) / k; seg >= 0; --seg) {
+    const int64_t S = A.steps, k = std::max(A.k, 1);
+    if (stored) {
+      for (int64_t s = S - 1; s >= 0; --s) reverse_step(c, A.tapes[s]);
+      sync_check(c);
+    }
+    if (!stored && k > 1 && (int64_t)A.seg.size() < k) A.seg.resize(k);
+    for (int64_t seg = stored ? -1 : (S - 1) / k; seg >= 0; --seg) {
       const int64_t s0 = seg * k, s1 = std::min(s0 + k, S);
       if (s1 - s0 > 1) {  // states at the start of steps s0+1 .. s1-1
         snapshot_restore(c, A.ckpt[seg]);
@@ -1474,12 +1639,13 @@ int dfr_adjoint_backward(dfr_ctx* c, const double* seed, double* grad) {
       for (int64_t s = s1 - 1; s >= s0; --s) {
         snapshot_restore(c, s == s0 ? A.ckpt[seg] : A.seg[s - s0]);
         replay_step(c, s, true);
-        reverse_step(c);
+        live_view(c, A.tape);
+        reverse_step(c, A.tape);
       }
       sync_check(c);
       check_overflow(c);
     }
-    snapshot_restore(c, A.final_state);
+    if (!stored) snapshot_restore(c, A.final_state);
     HIPCHK(hipMemcpyAsync(hb.data(), A.body, sizeof(AdjBodyDev) * nbodies, hipMemcpyDeviceToHost, c->stream));
     sync_check(c);
     if (c->profiling) drain_timers(c);
@@ -1494,7 +1660,7 @@ int dfr_adjoint_end(dfr_ctx* c) {
   return guarded(c, [&]() {
     HIPCHK(hipSetDevice(c->device));
     if (c->stream) HIPCHK(hipStreamSynchronize(c->stream));
-    adjoint_release(c);
+    adjoint_free(c);
     return DFR_OK;
   });
 }

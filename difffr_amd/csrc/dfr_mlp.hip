@@ -135,9 +135,14 @@ struct MlpLds {
   static constexpr int TRAIN_BYTES = DB + 64 * 4;
 };
 
-__device__ inline float silu(float z) { return z / (1.f + __expf(-z)); }
+// SiLU with the hardware exp2 and reciprocal (v_exp_f32, v_rcp_f32, ~1 ulp):
+// sigma(z) = 1 / (1 + 2^(-z log2 e)).
+__device__ inline float sigmoid_fast(float z) {
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.44269504088896341f * z));
+}
+__device__ inline float silu(float z) { return z * sigmoid_fast(z); }
 __device__ inline float silu_d(float z) {
-  const float s = 1.f / (1.f + __expf(-z));
+  const float s = sigmoid_fast(z);
   return s * (1.f + z * (1.f - s));
 }
 
@@ -198,23 +203,28 @@ __global__ void __launch_bounds__(64) k_mlp(MlpArgs a) {
     for (int mt = 0; mt < mtiles; ++mt)
 #pragma unroll
       for (int kc = 0; kc < kt; ++kc) Aw[mt][kc] = Fw[(mt * kt + kc) * 64 + l];
+    float bv[4][4];  // the lane's biases: unit 16 mt + 4 lg + j
+#pragma unroll
+    for (int mt = 0; mt < mtiles; ++mt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[mt][j] = (16 * mt + 4 * lg + j < MLP_O[L]) ? bias[16 * mt + 4 * lg + j] : 0.f;
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
+      bf16x8 Bf[2];  // this particle tile's B fragments, shared by every M tile
+#pragma unroll
+      for (int kc = 0; kc < kt; ++kc)
+        Bf[kc] = *reinterpret_cast<const bf16x8*>(in + (16 * nt + lr) * ldi + 32 * kc + 8 * lg);
 #pragma unroll
       for (int mt = 0; mt < mtiles; ++mt) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kc = 0; kc < kt; ++kc) {
-          const bf16x8 A = Aw[mt][kc];
-          const bf16x8 B = *reinterpret_cast<const bf16x8*>(in + (16 * nt + lr) * ldi + 32 * kc + 8 * lg);
-          acc = mfma(A, B, acc);
-        }
+        for (int kc = 0; kc < kt; ++kc) acc = mfma(Aw[mt][kc], Bf[kc], acc);
         const int pp = 16 * nt + lr;
         if (L < 3) {
           bf16x4 hv;
           for (int j = 0; j < 4; ++j) {
             const int m = 16 * mt + 4 * lg + j;
-            const float z = acc[j] + bias[m];
+            const float z = acc[j] + bv[mt][j];
             if (MODE != MLP_INFER) zb[((L * 16 + nt * 4 + mt) * 64 + l) * 4 + j] = z;
             const __bf16 h = (__bf16)silu(z);
             hv[j] = h;
@@ -222,8 +232,8 @@ __global__ void __launch_bounds__(64) k_mlp(MlpArgs a) {
           }
           *reinterpret_cast<bf16x4*>(outp + pp * LDH + 16 * mt + 4 * lg) = hv;
         } else if (lg == 0) {
-          yb[pp * 2 + 0] = acc[0] + bias[0];
-          yb[pp * 2 + 1] = acc[1] + bias[1];
+          yb[pp * 2 + 0] = acc[0] + bv[0][0];
+          yb[pp * 2 + 1] = acc[1] + bv[0][1];
         }
       }
     }
@@ -346,6 +356,158 @@ __global__ void __launch_bounds__(64) k_mlp(MlpArgs a) {
       }
     __syncthreads();
     if (MODE == MLP_TRAIN) atomicAdd(&a.grad[MLP_OB[L - 1] + l], dbs[l]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// dL/d(features) for dL/d(predictions): the network's input gradient, used by
+// the exact adjoint through the learned warm start.  Keeps only the bf16
+// activations h0, h1, h2 in LDS and recomputes each pre-activation
+// Z_l = W_l H_{l-1} + b_l in the backward sweep (same MFMA, same inputs, so the
+// same values as the forward), which lands in the same D-fragment layout as
+// dH_l; 32 KB of LDS per wavefront.
+// ---------------------------------------------------------------------------
+struct IgLds {
+  static constexpr int H0 = 0;                    // bf16 [64][LD0]
+  static constexpr int H1 = H0 + 64 * LD0 * 2;    // bf16 [64][LDH]
+  static constexpr int H2 = H1 + 64 * LDH * 2;    // bf16 [64][LDH]
+  static constexpr int DZ = H2 + 64 * LDH * 2;    // bf16 [64][LDH]
+  static constexpr int BYTES = DZ + 64 * LDH * 2;
+};
+
+// The B fragments of particle tile nt of a particle-major activation (kt K-chunks).
+__device__ inline void load_b(const __bf16* in, int ldi, int nt, int kt, int lr, int lg, bf16x8 (&B)[2]) {
+#pragma unroll
+  for (int kc = 0; kc < 2; ++kc)
+    if (kc < kt) B[kc] = *reinterpret_cast<const bf16x8*>(in + (16 * nt + lr) * ldi + 32 * kc + 8 * lg);
+}
+// One 16x16 tile of Z = W H from preloaded fragments.
+__device__ inline f32x4 tile_mma(const bf16x8 (&A)[4][2], const bf16x8 (&B)[2], int mt, int kt) {
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kc = 0; kc < 2; ++kc)
+    if (kc < kt) acc = mfma(A[mt][kc], B[kc], acc);
+  return acc;
+}
+// The lane's biases of layer L: unit 16 mt + 4 lg + j.
+__device__ inline void load_bias(const float* P, int L, int lg, float (&b)[4][4]) {
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[mt][j] = P[MLP_OB[L] + 16 * mt + 4 * lg + j];
+}
+
+template <int L> __device__ inline void load_frags(const bf16x8* frag, int base, bf16x8 (&A)[4][2], int l, int mt_n,
+                                                   int kt) {
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc)
+      if (mt < mt_n && kc < kt) A[mt][kc] = frag[(base + mt * kt + kc) * 64 + l];
+}
+
+__global__ void __launch_bounds__(64) k_mlp_input_grad(MlpArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __bf16* h0 = reinterpret_cast<__bf16*>(smem + IgLds::H0);
+  __bf16* h1 = reinterpret_cast<__bf16*>(smem + IgLds::H1);
+  __bf16* h2 = reinterpret_cast<__bf16*>(smem + IgLds::H2);
+  __bf16* dz = reinterpret_cast<__bf16*>(smem + IgLds::DZ);
+  const int l = threadIdx.x;
+  const int lr = l & 15, lg = l >> 4;
+  const int p0 = blockIdx.x * 64;
+  const int p = p0 + l;
+  const bool valid = p < a.n;
+  const MlpNorm& N = a.norm;
+  float fr[6];
+  if (valid) {
+    if (a.feat) for (int k = 0; k < 6; ++k) fr[k] = a.feat[(size_t)p * 6 + k];
+    else features_from_fields(a.fields, (uint32_t)p, fr);
+  } else {
+    for (int k = 0; k < 6; ++k) fr[k] = N.mu_in[k];
+  }
+  for (int k = 0; k < 32; ++k) h0[l * LD0 + k] = (__bf16)((k < 6) ? (fr[k] - N.mu_in[k]) * N.is_in[k] : 0.f);
+  // output adjoint in normalised units, K padded to 32
+  const float g0 = valid ? a.ybar[(size_t)p * 2 + 0] * N.sd_out[0] : 0.f;
+  const float g1 = valid ? a.ybar[(size_t)p * 2 + 1] * N.sd_out[1] : 0.f;
+  for (int k = 0; k < 32; ++k) dz[l * LDH + k] = (__bf16)((k == 0) ? g0 : (k == 1) ? g1 : 0.f);
+  __syncthreads();
+  bf16x8 Aw[4][2];
+  float bv[4][4];
+  // forward: h1 = silu(W1 h0 + b1), h2 = silu(W2 h1 + b2)
+#pragma unroll
+  for (int L = 0; L < 2; ++L) {
+    const int kt = (L == 0) ? 1 : 2;
+    load_frags<0>(a.frag, MLP_FW[L], Aw, l, 4, kt);
+    load_bias(a.P, L, lg, bv);
+    const __bf16* hin = (L == 0) ? h0 : h1;
+    __bf16* hout = (L == 0) ? h1 : h2;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      bf16x8 Bf[2];
+      load_b(hin, (L == 0) ? LD0 : LDH, nt, kt, lr, lg, Bf);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        const f32x4 z = tile_mma(Aw, Bf, mt, kt);
+        bf16x4 hv;
+        for (int j = 0; j < 4; ++j) hv[j] = (__bf16)silu(z[j] + bv[mt][j]);
+        *reinterpret_cast<bf16x4*>(hout + (16 * nt + lr) * LDH + 16 * mt + 4 * lg) = hv;
+      }
+    }
+    __syncthreads();
+  }
+  // backward: for L = 3, 2, 1: dH = W_L^T dZ_L; dZ_{L-1} = dH * silu'(Z_{L-1}), Z recomputed
+#pragma unroll
+  for (int L = 3; L >= 1; --L) {
+    const int kt = (MLP_O[L] + 31) / 32;
+    bf16x8 At[4][2];
+    load_frags<0>(a.frag, MLP_FT[L], At, l, 4, kt);
+    f32x4 dh[4][4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      bf16x8 Bf[2];
+      load_b(dz, LDH, nt, kt, lr, lg, Bf);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) dh[nt][mt] = tile_mma(At, Bf, mt, kt);
+    }
+    __syncthreads();  // every lane has read dz
+    // Z_{L-1} = W_{L-1} H_{L-2} + b_{L-1}
+    const int kz = (L - 1 == 0) ? 1 : 2;
+    load_frags<0>(a.frag, MLP_FW[L - 1], Aw, l, 4, kz);
+    load_bias(a.P, L - 1, lg, bv);
+    const __bf16* hin = (L - 1 == 0) ? h0 : (L - 1 == 1) ? h1 : h2;
+    const int ldi = (L - 1 == 0) ? LD0 : LDH;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      bf16x8 Bf[2];
+      load_b(hin, ldi, nt, kz, lr, lg, Bf);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        const f32x4 z = tile_mma(Aw, Bf, mt, kz);
+        bf16x4 dv;
+        const int pp = 16 * nt + lr;
+        for (int j = 0; j < 4; ++j)
+          dv[j] = (__bf16)((p0 + pp < a.n) ? dh[nt][mt][j] * silu_d(z[j] + bv[mt][j]) : 0.f);
+        *reinterpret_cast<bf16x4*>(dz + pp * LDH + 16 * mt + 4 * lg) = dv;
+      }
+    }
+    __syncthreads();
+  }
+  // dH_0 = W_1^T dZ_1 (inputs padded to 16): dL/df = dL/dx * is
+  bf16x8 At0[4][2];
+  load_frags<0>(a.frag, MLP_FT[0], At0, l, 1, 2);
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc) {
+      const bf16x8 B = *reinterpret_cast<const bf16x8*>(dz + (16 * nt + lr) * LDH + 32 * kc + 8 * lg);
+      acc = mfma(At0[0][kc], B, acc);
+    }
+    const int pp = 16 * nt + lr;
+    for (int j = 0; j < 4; ++j) {
+      const int k = 4 * lg + j;
+      if (k < 6 && p0 + pp < a.n) a.fbar[(size_t)(p0 + pp) * 6 + k] = acc[j] * N.is_in[k];
+    }
   }
 }
 
@@ -482,7 +644,7 @@ void launch_mlp_input_grad(const void* frag, const float* P, const MlpNorm& norm
   if (fields) a.fields = *fields;
   a.ybar = ybar;
   a.fbar = fbar;
-  hipLaunchKernelGGL(k_mlp<MLP_INPUT_GRAD>, dim3(nb64(n)), dim3(64), MlpLds::IG_BYTES, s, a);
+  hipLaunchKernelGGL(k_mlp_input_grad, dim3(nb64(n)), dim3(64), IgLds::BYTES, s, a);
 }
 
 void launch_mlp_adam(float* P, float* G, float* m, float* v, float lr, int t, hipStream_t s) {
@@ -521,8 +683,8 @@ void set_mlp_kernel_attributes() {
                             MlpLds::INFER_BYTES);
   (void)hipFuncSetAttribute((const void*)k_mlp<MLP_TRAIN>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             MlpLds::TRAIN_BYTES);
-  (void)hipFuncSetAttribute((const void*)k_mlp<MLP_INPUT_GRAD>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            MlpLds::IG_BYTES);
+  (void)hipFuncSetAttribute((const void*)k_mlp_input_grad, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            IgLds::BYTES);
 }
 
 }  // namespace dfr

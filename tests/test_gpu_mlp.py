@@ -144,3 +144,27 @@ def test_learned_warm_start_saves_iterations(gpu_ready):
     eta = 0.01 * 0.01 * 1000.0
     assert res[True][2] <= eta + 1e-9 and res[False][2] <= eta + 1e-9
     assert res[True][0] < 0.8 * res[False][0]
+
+
+def test_learned_adjoint_stored_and_replayed_agree(gpu_ready):
+    """The exact adjoint through the learned warm start (network input gradient,
+    feature adjoints) is the same whether the step tapes are kept or re-recorded."""
+    sim = _sim(_deep_tank(0, 60), compute_gradient=0)
+    _capture_and_train(sim, 60, 300, fraction=0.5)
+    P, norm = sim.mlp_get_params()
+    sim.close()
+    seed = np.zeros((2, 7))
+    seed[1] = np.random.default_rng(5).normal(size=7)
+    grads = []
+    for k in (0, 1, 4):
+        s = _sim(_deep_tank(3, 30), compute_gradient=0)
+        s.mlp_set_params(P, norm)
+        s.mlp_enable(True)
+        s.adjoint_begin(k)
+        s.step(12)
+        grads.append(s.adjoint_backward(seed)[1])
+        s.close()
+    print("learned-mode adjoint", grads[0], flush=True)
+    assert np.all(np.isfinite(grads[0])) and np.abs(grads[0]).max() > 0
+    for g in grads[1:]:
+        np.testing.assert_allclose(g, grads[0], rtol=0, atol=1e-5 * np.abs(grads[0]).max())

@@ -237,14 +237,16 @@ template <typename RF, typename RR> struct Adjoint {
     });
   }
 
-  // ---- reverse of rho_i / rho0 = V_f W0 + sum_j V_f W_ij + sum_b V_b W_ib
-  void reverse_density(const StepTape<RF, RR>& T, const std::vector<A>& rb) {
+  // ---- reverse of rho_i / rho0 = V_f W0 + sum_j V_f W_ij + sum_b V_b W_ib, and
+  // of its boundary part sum_b V_b W_ib (adjoint rbb, learned warm start only)
+  void reverse_density(const StepTape<RF, RR>& T, const std::vector<A>& rb, const std::vector<A>* rbb = nullptr) {
     const A Vf = A(S.V_f);
     const long N = long(T.x.size());
     scatter(N, nullptr, [&](long i, Acc& o) {
       auto& xb = o.x;
       auto& sx = o.sx;
-      if (rb[i] == A(0)) return;
+      const A rbi = rb[i] + (rbb ? (*rbb)[i] : A(0));
+      if (rb[i] == A(0) && rbi == A(0)) return;
       for (uint32_t j : T.nFF[i]) {
         const V t = gw(T.x[i] - T.x[j]) * (rb[i] * Vf);
         xb[i] += t;
@@ -254,7 +256,7 @@ template <typename RF, typename RR> struct Adjoint {
         const auto& B = S.bodies[bi];
         const auto& bt = T.bodies[bi];
         for (uint32_t jb : T.nFB[bi][i]) {
-          const V t = gw(T.x[i] - bt.xb[jb]) * (rb[i] * A(B.V[jb]));
+          const V t = gw(T.x[i] - bt.xb[jb]) * (rbi * A(B.V[jb]));
           xb[i] += t;
           if (B.dynamic) sx[bi][jb] -= t;
         }
@@ -268,6 +270,8 @@ template <typename RF, typename RR> struct Adjoint {
     const A dt = A(S.dt);
     const A invH = A(RF(1.0) / S.dt), invH2 = A(RF(1.0) / (S.dt * S.dt)), h2 = A(S.dt * S.dt);
     std::vector<A> rhob(N, 0), alb(N, 0), kbuf(N), db(N), kacc(N, 0);
+    const bool learned = !T.feat.empty();
+    std::vector<A> yb(learned ? 2 * N : 0, A(0));  // adjoint of the network's predictions
     for (long i = 0; i < N; ++i) vb[i] += xb[i] * dt;  // x' = x + dt v
     // pressure solve
     const bool warmP = S.P.usePressureWarmstart != 0;
@@ -287,8 +291,10 @@ template <typename RF, typename RR> struct Adjoint {
     if (warmP) {
       kbuf = kacc;
       reverse_update(T, T.kW, kbuf);
-      for (long i = 0; i < N; ++i)
-        kb[i] = (T.rsW[i] > RF(1) && T.kPrev[i] > RF(-0.00025)) ? kbuf[i] * A(0.5) * invH2 : A(0);
+      for (long i = 0; i < N; ++i) {
+        if (learned) yb[2 * i + 1] = (T.rsW[i] > RF(1)) ? kbuf[i] * invH2 : A(0);  // kappa_w = y1 / dt^2
+        else kb[i] = (T.rsW[i] > RF(1) && T.kPrev[i] > RF(-0.00025)) ? kbuf[i] * A(0.5) * invH2 : A(0);
+      }
     }
     // divergence solve
     if (S.P.enableDivergenceSolver) {
@@ -306,12 +312,32 @@ template <typename RF, typename RR> struct Adjoint {
       if (warmV) {
         kbuf = kacc;
         reverse_update(T, T.kvW, kbuf);
-        for (long i = 0; i < N; ++i)
-          kvb[i] = (T.d0[i] > RF(0) && T.kvPrev[i] > RF(-0.5)) ? kbuf[i] * A(0.5) * invH : A(0);
+        for (long i = 0; i < N; ++i) {
+          if (learned) yb[2 * i] = (T.d0[i] > RF(0)) ? kbuf[i] * invH : A(0);  // kappa_v,w = y0 / dt
+          else kvb[i] = (T.d0[i] > RF(0) && T.kvPrev[i] > RF(-0.5)) ? kbuf[i] * A(0.5) * invH : A(0);
+        }
       }
     }
+    std::vector<A> rbb;
+    if (learned) {
+      // through the network to its features: rho/rho0 - 1, dt delta(v0),
+      // neighbour count (constant), boundary density, kappa_prev, kappa_v_prev
+      const bool divWarm = S.P.enableDivergenceSolver && S.P.useDivergenceWarmstart;
+      rbb.assign(N, A(0));
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < N; ++i) {
+        double fb[6];
+        S.net.input_grad(&T.feat[6 * i], &yb[2 * i], fb);
+        kb[i] = fb[4] + (warmP ? A(0) : kb[i]);
+        kvb[i] = fb[5] + (divWarm ? A(0) : kvb[i]);
+        rhob[i] += fb[0];
+        db[i] = dt * fb[1];
+        rbb[i] = fb[3];
+      }
+      if (S.P.enableDivergenceSolver) reverse_delta(T, T.v0, db);
+    }
     reverse_factor(T, alb);
-    reverse_density(T, rhob);
+    reverse_density(T, rhob, learned ? &rbb : nullptr);
   }
 
   // ---- rigid update of body k (velocityTimeStep + positionTimeStep), in place:

@@ -27,6 +27,7 @@
 #include <omp.h>
 #endif
 #include "oracle_math.hpp"
+#include "warmnet.hpp"
 
 namespace orc {
 
@@ -215,6 +216,9 @@ template <typename RF, typename RR> struct StepTape {
   std::vector<std::vector<RF>> kD, bD, kP, bP;  // k_i and d / rho* of each iteration
   std::vector<BodyTape<RF, RR>> bodies;
   int Iv = 0, Ip = 0;
+  // learned warm start: network features [N][6] and the velocity delta was taken at
+  std::vector<double> feat;
+  std::vector<V3<RF>> v0;
 };
 
 template <typename RF, typename RR> class Sim {
@@ -242,6 +246,11 @@ template <typename RF, typename RR> class Sim {
   StepTape<RF, RR>* tp = nullptr;
   std::vector<int> fixIv, fixIp;
   int fixPos = 0;
+  // Learned warm start (when net.enabled): the network's predictions from the
+  // features of the step's start replace the halved previous stiffness of
+  // both warm starts.
+  WarmNet net;
+  std::vector<double> pred;  // [N][2]: kappa_v dt, kappa dt^2
 
   explicit Sim(const Params& p) : P(p) {
     h = RF(4.0) * RF(P.radius);  // Simulation.cpp:195-216, support radius = 4r
@@ -469,18 +478,53 @@ template <typename RF, typename RR> class Sim {
     bAdv[i] = std::max(da, RF(1.0));
   }
 
-  // TimeStepDiffDFSPH::computeDensityChange, TimeStepDiffDFSPH.cpp:1977-2041.
-  void compute_density_change(long i) {
+  // Density change rate before the clamp (TimeStepDiffDFSPH.cpp:1977-2041),
+  // and the neighbour count.
+  RF density_change_raw(long i, unsigned& nn) const {
     RF da = 0;
-    unsigned nn = unsigned(nFF[i].size());
+    nn = unsigned(nFF[i].size());
     for (uint32_t j : nFF[i]) da += V_f * (v[i] - v[j]).dot(K.gradW(x[i] - x[j]));
     for (size_t bi = 0; bi < bodies.size(); ++bi) {
       nn += unsigned(nFB[bi][i].size());
       for (uint32_t j : nFB[bi][i]) da += bodies[bi].V[j] * (v[i] - bodies[bi].v[j]).dot(K.gradW(x[i] - bodies[bi].x[j]));
     }
-    da = std::max(da, RF(0.0));
+    return da;
+  }
+
+  // TimeStepDiffDFSPH::computeDensityChange, TimeStepDiffDFSPH.cpp:1977-2041.
+  void compute_density_change(long i) {
+    unsigned nn = 0;
+    RF da = std::max(density_change_raw(i, nn), RF(0.0));
     if (nn < 20) da = 0;
     bAdv[i] = da;
+  }
+
+  // Features of the learned warm start at the step's start (after densities
+  // and factors): rho/rho0 - 1, dt * (density change rate before the clamp;
+  // 0 without the divergence solver), neighbour count, boundary part of
+  // rho/rho0, previous kappa and kappa_v (stored units), then the network's
+  // predictions of (kappa_v dt, kappa dt^2).
+  void learned_predict() {
+    const long N = long(x.size());
+    pred.assign(2 * N, 0.0);
+    std::vector<double> feat(6 * N);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      unsigned nn = 0;
+      const RF da = density_change_raw(i, nn);
+      RF bd = 0;
+      for (size_t bi = 0; bi < bodies.size(); ++bi)
+        for (uint32_t j : nFB[bi][i]) bd += bodies[bi].V[j] * K.W(x[i] - bodies[bi].x[j]);
+      double* f = &feat[6 * i];
+      f[0] = double(rho[i] / rho0 - RF(1));
+      f[1] = P.enableDivergenceSolver ? double(dt * da) : 0.0;
+      f[2] = double(nn);
+      f[3] = double(bd);
+      f[4] = double(kappa[i]);
+      f[5] = double(kappaV[i]);
+      net.forward(f, &pred[2 * i]);
+    }
+    if (tp) { tp->feat = std::move(feat); tp->v0 = v; }
   }
 
   // BoundaryModel::addForce, BoundaryModel.h:46-58 (per-thread partials).
@@ -532,7 +576,8 @@ template <typename RF, typename RR> class Sim {
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < N; ++i) {
       compute_density_change(i);
-      if (bAdv[i] > RF(0.0)) kappaV[i] = RF(0.5) * std::max(kappaV[i], RF(-0.5)) * invH;
+      if (bAdv[i] > RF(0.0))
+        kappaV[i] = net.enabled ? RF(pred[2 * i]) * invH : RF(0.5) * std::max(kappaV[i], RF(-0.5)) * invH;
       else kappaV[i] = 0;
     }
     if (tp) { tp->d0 = bAdv; tp->kvW = kappaV; }
@@ -548,7 +593,8 @@ template <typename RF, typename RR> class Sim {
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < N; ++i) {
       compute_density_adv(i);
-      if (bAdv[i] > RF(1.0)) kappa[i] = RF(0.5) * std::max(kappa[i], RF(-0.00025)) * invH2;
+      if (bAdv[i] > RF(1.0))
+        kappa[i] = net.enabled ? RF(pred[2 * i + 1]) * invH2 : RF(0.5) * std::max(kappa[i], RF(-0.00025)) * invH2;
       else kappa[i] = 0;
     }
     if (tp) { tp->rsW = bAdv; tp->kW = kappa; }
@@ -930,6 +976,10 @@ template <typename RF, typename RR> class Sim {
     compute_densities();
     compute_dfsph_factor();
     if (tp) { tp->x = x; tp->nFF = nFF; tp->nFB = nFB; tp->rho = rho; tp->alpha = alpha; tp->G = s; }
+    if (net.enabled) {
+      if (tp) { tp->kPrev = kappa; tp->kvPrev = kappaV; }
+      learned_predict();
+    }
     if (P.enableDivergenceSolver) divergence_solve();
     else stats.iterationsV = 0;
     const long N = long(x.size());

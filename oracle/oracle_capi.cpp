@@ -38,6 +38,7 @@ struct OracleBase {
   virtual int recorded_iterations(int* iv, int* ip, int cap) const = 0;
   virtual void adjoint(const double* seed, double* out) = 0;
   virtual int tape_field(int step, const char* name, int iter, double* out) const = 0;
+  virtual void set_warm_net(const float* params, const float* norm, int bf16) = 0;
 };
 
 template <typename RF, typename RR> struct Oracle : OracleBase {
@@ -185,6 +186,10 @@ template <typename RF, typename RR> struct Oracle : OracleBase {
     else if (n == "bP" && iter < int(T.bP.size())) f = &T.bP[iter];
     else if (n == "kP" && iter < int(T.kP.size())) f = &T.kP[iter];
     if (f) { for (size_t i = 0; i < f->size(); ++i) out[i] = double((*f)[i]); return int(f->size()); }
+    if (n == "feat") {
+      for (size_t i = 0; i < T.feat.size(); ++i) out[i] = T.feat[i];
+      return int(T.feat.size());
+    }
     if (n == "nn") {
       for (size_t i = 0; i < T.nFF.size(); ++i) {
         size_t c = T.nFF[i].size();
@@ -194,6 +199,11 @@ template <typename RF, typename RR> struct Oracle : OracleBase {
       return int(T.nFF.size());
     }
     return -1;
+  }
+  // params == nullptr: plain warm starts again
+  void set_warm_net(const float* params, const float* norm, int bf16) override {
+    if (params) sim.net.set(params, norm, bf16);
+    else sim.net.enabled = false;
   }
   void adjoint(const double* seed, double* out) override {
     Adjoint<RF, RR> adj(sim);
@@ -297,6 +307,28 @@ void orc_grad_utils(const double* q, const double* p, double* Rqp, double* RqTp,
   auto b = grad_RqTp_to_q(qq, pp); for (int i = 0; i < 12; ++i) RqTp[i] = b.a[i];
   auto c = grad_p_q_product_to_q(qq); for (int i = 0; i < 16; ++i) PQ[i] = c.a[i];
   auto d = grad_omega_q_product_to_omega(qq); for (int i = 0; i < 12; ++i) WQ[i] = d.a[i];
+}
+
+// Learned warm start: network for the simulation (params == nullptr: off), and
+// the network alone on n feature rows.
+int orc_set_warm_net(void* h, const float* params, const float* norm, int bf16) {
+  static_cast<OracleBase*>(h)->set_warm_net(params, norm, bf16);
+  return 0;
+}
+int orc_net_predict(const float* params, const float* norm, int bf16, long n, const double* feat, double* out) {
+  WarmNet net;
+  net.set(params, norm, bf16);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) net.forward(feat + 6 * i, out + 2 * i);
+  return 0;
+}
+int orc_net_input_grad(const float* params, const float* norm, int bf16, long n, const double* feat,
+                       const double* ybar, double* out) {
+  WarmNet net;
+  net.set(params, norm, bf16);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) net.input_grad(feat + 6 * i, ybar + 2 * i, out + 6 * i);
+  return 0;
 }
 
 const char* orc_last_error() { return g_err.c_str(); }

@@ -78,6 +78,10 @@ def lib():
         L.orc_recorded_iterations.argtypes = [P, ip, ip, C.c_int]
         L.orc_adjoint.argtypes = [P, dp, dp]
         L.orc_tape_field.argtypes = [P, C.c_int, C.c_char_p, C.c_int, dp]
+        fp = C.POINTER(C.c_float)
+        L.orc_set_warm_net.argtypes = [P, fp, fp, C.c_int]
+        L.orc_net_predict.argtypes = [fp, fp, C.c_int, C.c_long, dp, dp]
+        L.orc_net_input_grad.argtypes = [fp, fp, C.c_int, C.c_long, dp, dp, dp]
         assert L.orc_params_size() == C.sizeof(Params), "oracle Params layout mismatch"
         _lib = L
     return _lib
@@ -164,7 +168,7 @@ class OracleSim:
         return a[:n].copy(), b[:n].copy()
 
     def tape_field(self, step: int, name: str, it: int = 0) -> np.ndarray:
-        out = np.zeros(self.n, np.float64)
+        out = np.zeros(6 * self.n, np.float64)  # "feat" is [n][6]
         n = lib().orc_tape_field(self.h, step, name.encode(), it, _dp(out))
         if n < 0:
             raise KeyError(name)
@@ -182,6 +186,16 @@ class OracleSim:
 
     def step(self, n: int = 1):
         lib().orc_step(self.h, n)
+
+    def set_warm_net(self, params, norm, bf16: bool = True):
+        """Learned warm start with the 8898 network parameters and the 16 values
+        of the input/output normalisation (params None: plain warm starts).
+        bf16=True rounds as the GPU's matrix cores do; False is exact float64."""
+        if params is None:
+            lib().orc_set_warm_net(self.h, None, None, 0)
+            return
+        self._net = (np.ascontiguousarray(params, np.float32), np.ascontiguousarray(norm, np.float32))
+        lib().orc_set_warm_net(self.h, _fp(self._net[0]), _fp(self._net[1]), int(bf16))
 
     def stage(self, which: int = 0):
         lib().orc_stage(self.h, which)
@@ -250,6 +264,30 @@ class OracleSim:
         a = np.ascontiguousarray(values, np.float64)
         if lib().orc_set_state(self.h, name.encode(), _dp(a)) != 0:
             raise KeyError(name)
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def net_predict(params, norm, feat, bf16: bool = True) -> np.ndarray:
+    """The oracle's warm-start network on feature rows [n][6] -> [n][2]."""
+    P = np.ascontiguousarray(params, np.float32)
+    nm = np.ascontiguousarray(norm, np.float32)
+    f = np.ascontiguousarray(feat, np.float64).reshape(-1, 6)
+    out = np.zeros((f.shape[0], 2))
+    lib().orc_net_predict(_fp(P), _fp(nm), int(bf16), f.shape[0], _dp(f), _dp(out))
+    return out
+
+
+def net_input_grad(params, norm, feat, ybar, bf16: bool = True) -> np.ndarray:
+    P = np.ascontiguousarray(params, np.float32)
+    nm = np.ascontiguousarray(norm, np.float32)
+    f = np.ascontiguousarray(feat, np.float64).reshape(-1, 6)
+    yb = np.ascontiguousarray(ybar, np.float64).reshape(-1, 2)
+    out = np.zeros((f.shape[0], 6))
+    lib().orc_net_input_grad(_fp(P), _fp(nm), int(bf16), f.shape[0], _dp(f), _dp(yb), _dp(out))
+    return out
 
 
 def kernel_eval(radius: float, r: np.ndarray, double: bool = True):

@@ -168,3 +168,50 @@ def test_learned_adjoint_stored_and_replayed_agree(gpu_ready):
     assert np.all(np.isfinite(grads[0])) and np.abs(grads[0]).max() > 0
     for g in grads[1:]:
         np.testing.assert_allclose(g, grads[0], rtol=0, atol=1e-5 * np.abs(grads[0]).max())
+
+
+def test_learned_adjoint_matches_oracle(gpu_ready):
+    """Network trained on the GPU, then a learned-mode rollout with its exact
+    adjoint against the oracle's learned step and adjoint (oracle/warmnet.hpp
+    with the GPU's bf16 rounding, FP64 elsewhere, replaying the GPU run's
+    iteration counts).  Tolerance 1e-3 of the largest component, as for the
+    plain adjoint: bf16 decision flips between the two summation orders move
+    single warm-start values by ~2^-8, far below that."""
+    from oracle import pyoracle as po
+    steps = 12
+    sc = scenes.submerged_body(body="ellipsoid", steps=steps)
+    trainer = _sim(sc, compute_gradient=0)
+    _capture_and_train(trainer, steps, 400, fraction=1.0)
+    P, norm = trainer.mlp_get_params()
+    trainer.close()
+    seed = np.zeros((2, 7))
+    seed[1] = np.random.default_rng(8).normal(size=7)
+    print(f"[learned adjoint vs oracle] steps={steps} n={sc.n_fluid}", flush=True)
+    s = _sim(sc, compute_gradient=0)
+    s.mlp_set_params(P, norm)
+    s.mlp_enable(True)
+    s.adjoint_begin(0)
+    s.step(steps)
+    g = s.adjoint_backward(seed)[1]
+    iters = s.adjoint_iterations()
+    rig = s.rigid(1)
+    s.close()
+    o = po.OracleSim(sc, "f64")
+    o.set_warm_net(P, norm, bf16=True)
+    o.set_fixed_iterations(*iters)
+    o.set_record(True)
+    o.step(steps)
+    ref = o.adjoint(seed)[1]
+    ro = o.rigid(1)
+    print("gpu   ", g, "\noracle", ref, "\niterations", iters, flush=True)
+    np.testing.assert_allclose(rig["x"], ro["x"], rtol=0, atol=1e-5)  # FP32 fluid against FP64, 12 steps
+    np.testing.assert_allclose(rig["q"], ro["q"], rtol=0, atol=1e-5)
+    scale = np.abs(ref).max()
+    assert np.all(np.isfinite(g))
+    np.testing.assert_allclose(g, ref, rtol=0, atol=1e-3 * scale)
+    # the learned warm start shapes the gradient: the plain oracle's is far off
+    o2 = po.OracleSim(sc, "f64")
+    o2.set_fixed_iterations(*iters)
+    o2.set_record(True)
+    o2.step(steps)
+    assert np.abs(o2.adjoint(seed)[1] - ref).max() > 2e-2 * scale

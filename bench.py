@@ -162,6 +162,7 @@ def timed_run(args, sc, learned, P=None, norm=None):
     st = sim.stats()
     times = sim.kernel_times()
     nst = sim.neighbor_stats()
+    nst.update(sim.halo_stats())
     sim.adjoint_end()
     sim.close()
     assert np.all(np.isfinite(grad)), "non-finite adjoint"
@@ -230,6 +231,7 @@ def main():
                        f"checkpoint every {args.checkpoint_every} step(s), steps re-run with recording in the "
                        f"backward pass"),
             "mean_fluid_neighbors": nst["mean_fluid"], "mean_boundary_neighbors": nst["mean_boundary"],
+            "largest_halo": nst["largest_halo"], "halo_lds_capacity": nst["capacity"],
             "plain": summary(plain),
         },
         "roofline": roofline,

@@ -23,8 +23,7 @@
 namespace dfr {
 
 constexpr int RB = PASS_BLOCK;
-constexpr int CAPR = TILE_CAP_R;
-constexpr int PTR = per_thread<RB, CAPR>();
+constexpr int PTR = per_thread<RB, TILE_CAP>();
 
 struct PairCE {
   float c, e;
@@ -68,7 +67,7 @@ __device__ inline void atomic_add3(double* p, float x, float y, float z) {
 // LDS per slot: x, y, z, vbar (3), k.
 // ---------------------------------------------------------------------------
 template <int MODE>
-__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(6, 8)))
 k_adj_update_t(AdjPass a, const float* __restrict__ k, const float4* __restrict__ vbar,
                const float* kacc, float kacc_scale, const float* __restrict__ bval,
                const float* __restrict__ kprev, const float* __restrict__ alpha, float* out,
@@ -80,13 +79,14 @@ k_adj_update_t(AdjPass a, const float* __restrict__ k, const float4* __restrict_
   if (brick_empty(a.fstart, brick)) return;
   tile_setup<RB>(a.g, a.fstart, brick, T);
   const uint32_t total = T.total;
-  const bool in_lds = total <= (uint32_t)CAPR;
-  float* lx = lds; float* ly = lds + CAPR; float* lz = lds + 2 * CAPR;
-  float* lbx = lds + 3 * CAPR; float* lby = lds + 4 * CAPR; float* lbz = lds + 5 * CAPR; float* lk = lds + 6 * CAPR;
+  const int cap = a.cap;
+  const bool in_lds = total <= (uint32_t)cap;
+  float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
+  float* lbx = lds + 3 * cap; float* lby = lds + 4 * cap; float* lbz = lds + 5 * cap; float* lk = lds + 6 * cap;
   const int t = threadIdx.x;
   if (in_lds) {
     uint32_t js[PTR];
-    tile_indices<RB, CAPR>(T, total, js);
+    tile_indices<RB>(T, total, js);
     float4 q[PTR], w[PTR];
     float kk[PTR];
 #pragma unroll
@@ -214,7 +214,7 @@ k_adj_update_t(AdjPass a, const float* __restrict__ k, const float4* __restrict_
 //   xbar_i += V_f sum_j (d_i + d_j) H_ij (v_i - v_j) + d_i sum_b V_b H_ib (v_i - v_b)
 // LDS per slot: x, y, z, v (3), d.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(6, 8)))
 k_adj_delta_t(AdjPass a, const float4* __restrict__ v, const float* __restrict__ dbar, float4* __restrict__ vbar,
               float4* __restrict__ xbar) {
   __shared__ TileHdr<RB> T;
@@ -224,13 +224,14 @@ k_adj_delta_t(AdjPass a, const float4* __restrict__ v, const float* __restrict__
   if (brick_empty(a.fstart, brick)) return;
   tile_setup<RB>(a.g, a.fstart, brick, T);
   const uint32_t total = T.total;
-  const bool in_lds = total <= (uint32_t)CAPR;
-  float* lx = lds; float* ly = lds + CAPR; float* lz = lds + 2 * CAPR;
-  float* lvx = lds + 3 * CAPR; float* lvy = lds + 4 * CAPR; float* lvz = lds + 5 * CAPR; float* ld = lds + 6 * CAPR;
+  const int cap = a.cap;
+  const bool in_lds = total <= (uint32_t)cap;
+  float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
+  float* lvx = lds + 3 * cap; float* lvy = lds + 4 * cap; float* lvz = lds + 5 * cap; float* ld = lds + 6 * cap;
   const int t = threadIdx.x;
   if (in_lds) {
     uint32_t js[PTR];
-    tile_indices<RB, CAPR>(T, total, js);
+    tile_indices<RB>(T, total, js);
     float4 q[PTR], w[PTR];
     float dd[PTR];
 #pragma unroll
@@ -329,15 +330,16 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
   if (brick_empty(a.fstart, brick)) return;
   tile_setup<RB>(a.g, a.fstart, brick, T);
   const uint32_t total = T.total;
-  const bool in_lds = total <= (uint32_t)CAPR;
-  float* lx = lds; float* ly = lds + CAPR; float* lz = lds + 2 * CAPR;
-  float* lS = lds + 3 * CAPR; float* lr = lds + 4 * CAPR;
-  float* lgx = lds + 5 * CAPR; float* lgy = lds + 6 * CAPR; float* lgz = lds + 7 * CAPR;
+  const int cap = a.cap;
+  const bool in_lds = total <= (uint32_t)cap;
+  float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
+  float* lS = lds + 3 * cap; float* lr = lds + 4 * cap;
+  float* lgx = lds + 5 * cap; float* lgy = lds + 6 * cap; float* lgz = lds + 7 * cap;
   const int t = threadIdx.x;
   auto sbar = [&](uint32_t j) { const float al = alpha[j]; return alpha_bar[j] * al * al; };
   if (in_lds) {
     uint32_t js[PTR];
-    tile_indices<RB, CAPR>(T, total, js);
+    tile_indices<RB>(T, total, js);
     float4 q[PTR], g[PTR];
     float sb[PTR], rb[PTR];
 #pragma unroll
@@ -613,13 +615,14 @@ __global__ void k_adj_rigid(RigidStepConst rc, AdjStepTable tb, const RigidDev* 
 // ---------------------------------------------------------------------------
 static inline unsigned nbricks(const GridDesc& g) { return (unsigned)(g.bn[0] * g.bn[1] * g.bn[2]); }
 static inline unsigned nblk(size_t n) { return (unsigned)((n + DFR_BLOCK - 1) / DFR_BLOCK); }
-constexpr size_t SHM7 = sizeof(float) * (size_t)CAPR * 7;
-constexpr size_t SHM8 = sizeof(float) * (size_t)CAPR * 8;
+// dynamic LDS of the reverse passes: 7 or 8 floats per staged halo slot
+static inline size_t shm_slots(const AdjPass& a, int floats) { return sizeof(float) * (size_t)a.cap * floats; }
 
 void launch_adj_update(const AdjPass& a, int mode, const float* k, const float4* vbar, const float* kacc,
                        float kacc_scale, const float* bval, const float* kprev, const float* alpha, float* out,
                        float* alpha_bar, float* rho_bar, float4* xbar, hipStream_t s) {
   const dim3 grid(nbricks(a.g)), blk(RB);
+  const size_t SHM7 = shm_slots(a, 7);
   switch (mode) {
     case ADJ_DIV_ITER:
       hipLaunchKernelGGL(k_adj_update_t<ADJ_DIV_ITER>, grid, blk, SHM7, s, a, k, vbar, kacc, kacc_scale, bval, kprev,
@@ -649,13 +652,13 @@ void launch_adj_update(const AdjPass& a, int mode, const float* k, const float4*
 
 void launch_adj_delta(const AdjPass& a, const float4* v, const float* dbar, float4* vbar, float4* xbar,
                       hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_delta_t, dim3(nbricks(a.g)), dim3(RB), SHM7, s, a, v, dbar, vbar, xbar);
+  hipLaunchKernelGGL(k_adj_delta_t, dim3(nbricks(a.g)), dim3(RB), shm_slots(a, 7), s, a, v, dbar, vbar, xbar);
 }
 
 void launch_adj_factor(const AdjPass& a, const float* alpha, const float* alpha_bar, const float* rho_bar,
                        const float4* G, float4* xbar, const float* rbb, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_factor_t, dim3(nbricks(a.g)), dim3(RB), SHM8, s, a, alpha, alpha_bar, rho_bar, G, xbar,
-                     rbb);
+  hipLaunchKernelGGL(k_adj_factor_t, dim3(nbricks(a.g)), dim3(RB), shm_slots(a, 8), s, a, alpha, alpha_bar, rho_bar, G,
+                     xbar, rbb);
 }
 
 void launch_adj_learned(int n, const float* fbar, int keep_kappa, int keep_kappav, float dt, float* kb, float* kvb,
@@ -689,7 +692,7 @@ void launch_adj_rigid(const RigidStepConst& rc, const AdjStepTable& t, const Rig
 }
 
 void set_adjoint_kernel_attributes() {
-  const int s7 = (int)SHM7, s8 = (int)SHM8;
+  const int s7 = (int)(sizeof(float) * TILE_CAP * 7), s8 = (int)(sizeof(float) * TILE_CAP * 8);
   (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_DIV_ITER>, hipFuncAttributeMaxDynamicSharedMemorySize, s7);
   (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_PRESSURE_ITER>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             s7);

@@ -199,7 +199,9 @@ struct dfr_ctx {
   std::vector<RigidDev> h_rig_init;  // state restored by dfr_reset
   double* d_err = nullptr;
   double* h_err = nullptr;           // pinned
-  uint32_t* h_overflow = nullptr;    // pinned
+  uint32_t* h_overflow = nullptr;    // pinned [4], as NbrBufs::overflow
+  int halo_cap_forced = -1;          // >= 0: LDS halo capacity fixed by dfr_set_halo_capacity
+  uint32_t halo_max = 0;             // largest halo of the last build
   int err_slots = 0;
 
   float time = 0.f;
@@ -431,7 +433,22 @@ void begin_step(dfr_ctx* c) {
   }
 }
 
+// The gather passes stage halos of up to nb.cap slots in LDS; the capacity
+// follows the largest halo of the last build (64-slot margin, rounded to 64),
+// so a launch reserves no more LDS than the halos need and more workgroups fit
+// on a CU.  A halo that outgrows it before the next update is served from
+// global memory with the same arithmetic.
+void update_halo_cap(dfr_ctx* c) {
+  if (c->h_overflow[3]) c->halo_max = c->h_overflow[3];
+  if (c->halo_cap_forced >= 0) {
+    c->nb.cap = std::min(c->halo_cap_forced, TILE_CAP);
+  } else if (c->halo_max) {
+    c->nb.cap = (int)std::min<uint32_t>((uint32_t)TILE_CAP, (c->halo_max + 64u + 63u) & ~63u);
+  }
+}
+
 void check_overflow(dfr_ctx* c) {
+  update_halo_cap(c);
   if (c->h_overflow[0] || c->h_overflow[1] || c->h_overflow[2]) {
     char buf[320];
     std::snprintf(buf, sizeof(buf),
@@ -522,12 +539,12 @@ void do_step(dfr_ctx* c) {
   }
   {
     Timed t(c, TC_NBR);
-    HIPCHK(hipMemsetAsync(c->nb.overflow, 0, 3 * sizeof(uint32_t), c->stream));
+    HIPCHK(hipMemsetAsync(c->nb.overflow, 0, 4 * sizeof(uint32_t), c->stream));
     if (nf > 0) {
       launch_build_tiled(c->f, c->b, c->grid, c->gb, c->nb, c->h * c->h, c->stream);
       launch_build_bf(c->f, c->b, c->grid, c->gb, c->nb, c->h * c->h, c->stream);
     }
-    HIPCHK(hipMemcpyAsync(c->h_overflow, c->nb.overflow, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_overflow, c->nb.overflow, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   }
   HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(double) * c->err_slots, c->stream));
   const bool divOn = c->P.enable_divergence_solver != 0;
@@ -886,6 +903,7 @@ void reverse_step(dfr_ctx* c, const StepTapeDev& T) {
   a.c = c->sc; a.g = c->grid; a.fstart = T.fstart; a.x = T.x;
   a.nbr8 = T.nbr8; a.nbrB = T.nbrB; a.cnt = T.cnt;
   a.bx = T.bpos; a.bv = T.bvel; a.bx0 = c->b.x0; a.nb_static = c->b.n - c->b.nd;
+  a.cap = c->nb.cap;
   a.sxb = A.sxb; a.svb = A.svb; a.body = A.body;
   {
     Timed t(c, TC_ADJ_MISC);
@@ -1276,7 +1294,7 @@ int dfr_init(dfr_ctx* c) {
     c->nb.cnt = dalloc<uint32_t>(c, NF);
     c->nb.bf = dalloc<uint32_t>(c, (size_t)c->nb.maxbf * std::max<int64_t>(nd, 1));
     c->nb.bfcnt = dalloc<uint32_t>(c, std::max<int64_t>(nd, 1));
-    c->nb.overflow = dalloc<uint32_t>(c, 3);
+    c->nb.overflow = dalloc<uint32_t>(c, 4);
     c->sc.maxn = MAXF;
     set_tiled_kernel_attributes();
     c->sc.maxbf = c->nb.maxbf;
@@ -1284,8 +1302,8 @@ int dfr_init(dfr_ctx* c) {
     c->err_slots = c->P.max_iterations_v + c->P.max_iterations + 2;
     c->d_err = dalloc<double>(c, c->err_slots);
     HIPCHK(hipHostMalloc((void**)&c->h_err, sizeof(double) * c->err_slots, hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void**)&c->h_overflow, sizeof(uint32_t) * 3, hipHostMallocDefault));
-    c->h_overflow[0] = c->h_overflow[1] = c->h_overflow[2] = 0;
+    HIPCHK(hipHostMalloc((void**)&c->h_overflow, sizeof(uint32_t) * 4, hipHostMallocDefault));
+    c->h_overflow[0] = c->h_overflow[1] = c->h_overflow[2] = c->h_overflow[3] = 0;
     // uploads
     HIPCHK(hipMemcpy(c->b.x0, x0.data(), sizeof(float4) * off, hipMemcpyHostToDevice));
     std::vector<uint32_t> dyn(std::max<int64_t>(nd, 1));
@@ -1490,6 +1508,22 @@ int dfr_reset_stats(dfr_ctx* c) {
   c->stats.total_iterations_v = 0;
   c->stats.steps_since_reset = 0;
   for (int k = 0; k < TC_COUNT; ++k) { c->t_sum[k] = 0; c->t_cnt[k] = 0; }
+  return DFR_OK;
+}
+
+int dfr_set_halo_capacity(dfr_ctx* c, int slots) {
+  if (!c) return DFR_ERR_ARG;
+  c->halo_cap_forced = slots < 0 ? -1 : std::min(slots, TILE_CAP);
+  c->nb.cap = slots < 0 ? TILE_CAP : c->halo_cap_forced;
+  if (slots < 0 && c->h_overflow) update_halo_cap(c);
+  return DFR_OK;
+}
+
+int dfr_halo_stats(const dfr_ctx* c, int64_t out[3]) {
+  if (!c || !out) return DFR_ERR_ARG;
+  out[0] = c->halo_max;
+  out[1] = c->nb.cap;
+  out[2] = TILE_CAP;
   return DFR_OK;
 }
 

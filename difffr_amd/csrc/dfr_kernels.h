@@ -5,9 +5,12 @@
 
 namespace dfr {
 
-// Brick tiles: LDS slots per halo (a 6x6x6-cell halo holds ~1700-2200 particles
-// at rest density) and fluid-neighbour slots per particle (16-bit tile slots).
-constexpr int TILE_CAP = 3072;
+// Brick tiles: largest halo staged in LDS (a 6x6x6-cell halo holds ~2160
+// particles at rest density: V_f = 0.8 (2r)^3 packs 10 particles into a cell of
+// edge h = 4r; the capacity in use, NbrBufs::cap, follows the largest halo of
+// recent builds and never exceeds this) and fluid-neighbour slots per particle
+// (16-bit tile slots).
+constexpr int TILE_CAP = 2560;
 constexpr int MAXF = 64;
 constexpr int PACK = 8;               // 16-bit slots per 16-byte pack
 constexpr int NPACK = MAXF / PACK;    // packs per particle
@@ -57,7 +60,9 @@ struct NbrBufs {
   uint32_t* cnt = nullptr;    // [nf]  nF | nB << 16
   uint32_t* bf = nullptr;     // [maxbf][nd] fluid neighbours of dynamic samples (sorted index)
   uint32_t* bfcnt = nullptr;  // [nd]
-  uint32_t* overflow = nullptr;  // [3] max count seen when a list overflowed (fluid, B->F, boundary)
+  uint32_t* overflow = nullptr;  // [4] max count seen when a list overflowed (fluid, B->F, boundary),
+                                 //     largest halo of the last build
+  int cap = TILE_CAP;            // halo slots the passes stage in LDS; larger halos read global memory
 };
 
 // dfr_grid.hip
@@ -118,8 +123,6 @@ void launch_begin_step(RigidDev* rig, int nbodies, const int* ramp_flags, const 
 // ---------------------------------------------------------------------------
 // dfr_adjoint.hip: reverse pass of one recorded step.
 // ---------------------------------------------------------------------------
-constexpr int TILE_CAP_R = 2560;  // halo slots staged in LDS by the reverse passes
-
 // Adjoint of one rigid body during the reverse of a step (FP64).
 struct AdjBodyDev {
   double x[3], q[4], v[3], w[3];  // adjoint of the rigid state (x_rb, q (w,x,y,z), v, omega)
@@ -153,6 +156,7 @@ struct AdjPass {
   const float4* bv;    // boundary velocities at the start of the step
   const float4* bx0;   // local sample, w = body index
   int nb_static;
+  int cap;             // halo slots staged in LDS (larger halos read global memory)
   double* sxb;         // [3 * nd] adjoint of dynamic sample positions
   double* svb;         // [3 * nd] adjoint of dynamic sample velocities
   const AdjBodyDev* body;

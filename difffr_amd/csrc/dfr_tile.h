@@ -70,7 +70,8 @@ __device__ inline int halo_cell(const TileHdr<NT>& T, int cx, int cy, int cz, in
 
 // Two-phase halo staging: every thread first resolves its slots to particle
 // indices (LDS-only binary searches, independent), then issues all its global
-// loads before writing LDS, so the loads are in flight together.
+// loads before writing LDS, so the loads are in flight together.  CAP is the
+// largest capacity (registers per thread); slots >= total are skipped.
 template <int NT, int CAP = TILE_CAP> constexpr int per_thread() { return CAP / NT; }
 
 template <int NT, int CAP = TILE_CAP>
@@ -106,6 +107,7 @@ struct PassArgs {
   const uint32_t* nbrB;
   const uint32_t* cnt;
   int nb_static;
+  int cap;  // halo slots staged in LDS (larger halos read global memory)
 };
 
 // Visits the nF fluid neighbour slots of particle i in list order.  All packs

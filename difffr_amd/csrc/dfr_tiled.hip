@@ -22,21 +22,49 @@ namespace dfr {
 // ---------------------------------------------------------------------------
 constexpr int BB = DFR_BLOCK;  // build workgroup
 
+// Fluid neighbour slots of one particle as they are found: a two-pack buffer in
+// registers (16 slots; a chunk of candidates adds at most 4 to fewer than 8),
+// flushed to the pack-major list whenever a pack is full.
+struct PackWriter {
+  uint32_t w[6] = {0u, 0u, 0u, 0u, 0u, 0u};  // slots 0..11 of the buffer, two per word
+  uint32_t nb = 0;                           // slots in the buffer
+  uint32_t np = 0;                           // packs written
+  __device__ inline void push(uint32_t s) {
+    const uint32_t word = nb >> 1, v = s << ((nb & 1u) << 4);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] |= (word == (uint32_t)k) ? v : 0u;
+    ++nb;
+  }
+  __device__ inline void flush_full(uint4* __restrict__ nbr8, size_t stride, uint32_t i) {
+    if (nb >= (uint32_t)PACK) {
+      if (np < (uint32_t)NPACK) nbr8[(size_t)np * stride + i] = make_uint4(w[0], w[1], w[2], w[3]);
+      ++np;
+      nb -= PACK;
+      w[0] = w[4]; w[1] = w[5]; w[2] = 0u; w[3] = 0u; w[4] = 0u; w[5] = 0u;
+    }
+  }
+  __device__ inline uint32_t finish(uint4* __restrict__ nbr8, size_t stride, uint32_t i) {
+    if (nb > 0u && np < (uint32_t)NPACK) nbr8[(size_t)np * stride + i] = make_uint4(w[0], w[1], w[2], w[3]);
+    return np * PACK + nb;
+  }
+};
+
+// Dynamic LDS: x, y, z of the halo (cap slots each).
 __global__ void __launch_bounds__(BB)
 k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t* __restrict__ fstart,
               const float4* __restrict__ bpos, const uint32_t* __restrict__ bstart,
-              const uint32_t* __restrict__ bperm, float h2, int maxb, uint4* __restrict__ nbr8,
+              const uint32_t* __restrict__ bperm, float h2, int maxb, int cap, uint4* __restrict__ nbr8,
               uint32_t* __restrict__ nbrB, uint32_t* __restrict__ cnt, uint32_t* __restrict__ overflow) {
   __shared__ TileHdr<BB> T;
   __shared__ uint32_t bcs[HALO], bcn[HALO];  // boundary sample range of each halo cell
-  __shared__ float lx[TILE_CAP], ly[TILE_CAP], lz[TILE_CAP];
-  __shared__ uint16_t stage[MAXF * BB];
+  extern __shared__ float lds[];
   const uint32_t brick = xcd_block();
   if (brick_empty(fstart, brick)) return;
   tile_setup<BB>(g, fstart, brick, T);
   if (T.own_begin == T.own_end) return;
   const uint32_t total = T.total;
   const int t = threadIdx.x;
+  if (t == 0) atomicMax(&overflow[3], total);
   if (total > 65535u) {
     // tile slots are 16-bit: leave empty lists (safe for every pass) and report
     for (uint32_t i = T.own_begin + t; i < T.own_end; i += BB) cnt[i] = 0u;
@@ -54,7 +82,8 @@ k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t
     bcs[t] = st;
     bcn[t] = n;
   }
-  const bool in_lds = total <= (uint32_t)TILE_CAP;
+  const bool in_lds = total <= (uint32_t)cap;
+  float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
   if (in_lds) {
     constexpr int PTB = per_thread<BB>();
     uint32_t js[PTB];
@@ -71,88 +100,64 @@ k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t
   }
   __syncthreads();
   const size_t stride = (size_t)nf;
-  for (uint32_t base = T.own_begin; base < T.own_end; base += BB) {
-    const uint32_t i = base + t;
-    const bool active = i < T.own_end;
-    uint32_t nF = 0, nB = 0;
-    if (active) {
-      const float4 p = pos[i];
-      int cx, cy, cz;
-      cell_coords(g, p.x, p.y, p.z, cx, cy, cz);
-      const int hc0 = halo_cell<BB>(T, cx, cy, cz, 0, 0, 0);
-      const uint32_t self = T.coff[hc0] + (i - T.cstart[hc0]);
-      // the three x-neighbour cells of a (dz, dy) row are consecutive halo
-      // cells, so their candidates form one contiguous slot range
-      for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy) {
-          const int hm = halo_cell<BB>(T, cx, cy, cz, -1, dy, dz);
-          const uint32_t r0 = T.coff[hm], r1 = T.coff[hm + 3];
-          if (in_lds) {
-            for (uint32_t s0 = r0; s0 < r1; s0 += 4) {
-              float qx[4], qy[4], qz[4];
+  for (uint32_t i = T.own_begin + t; i < T.own_end; i += BB) {
+    const float4 p = pos[i];
+    int cx, cy, cz;
+    cell_coords(g, p.x, p.y, p.z, cx, cy, cz);
+    const int hc0 = halo_cell<BB>(T, cx, cy, cz, 0, 0, 0);
+    const uint32_t self = T.coff[hc0] + (i - T.cstart[hc0]);
+    PackWriter pw;
+    // the three x-neighbour cells of a (dz, dy) row are consecutive halo
+    // cells, so their candidates form one contiguous slot range
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int hm = halo_cell<BB>(T, cx, cy, cz, -1, dy, dz);
+        const uint32_t r0 = T.coff[hm], r1 = T.coff[hm + 3];
+        if (in_lds) {
+          for (uint32_t s0 = r0; s0 < r1; s0 += 4) {
+            float qx[4], qy[4], qz[4];
 #pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                const uint32_t s = min(s0 + u, r1 - 1);
-                qx[u] = lx[s]; qy[u] = ly[s]; qz[u] = lz[s];
-              }
+            for (int u = 0; u < 4; ++u) {
+              const uint32_t s = min(s0 + u, r1 - 1);
+              qx[u] = lx[s]; qy[u] = ly[s]; qz[u] = lz[s];
+            }
 #pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                const uint32_t s = s0 + u;
-                if (s < r1 && s != self && dist2_exact(p.x, p.y, p.z, qx[u], qy[u], qz[u]) < h2) {
-                  if (nF < (uint32_t)MAXF) stage[nF * BB + t] = (uint16_t)s;
-                  ++nF;
-                }
-              }
+            for (int u = 0; u < 4; ++u) {
+              const uint32_t s = s0 + u;
+              if (s < r1 && s != self && dist2_exact(p.x, p.y, p.z, qx[u], qy[u], qz[u]) < h2) pw.push(s);
             }
-          } else {
-            for (uint32_t s = r0; s < r1; ++s) {
-              if (s == self) continue;
-              const float4 q = pos[tile_global<BB>(T, s)];
-              if (dist2_exact(p.x, p.y, p.z, q.x, q.y, q.z) < h2) {
-                if (nF < (uint32_t)MAXF) stage[nF * BB + t] = (uint16_t)s;
-                ++nF;
-              }
-            }
+            pw.flush_full(nbr8, stride, i);
+          }
+        } else {
+          for (uint32_t s = r0; s < r1; ++s) {
+            if (s == self) continue;
+            const float4 q = pos[tile_global<BB>(T, s)];
+            if (dist2_exact(p.x, p.y, p.z, q.x, q.y, q.z) < h2) pw.push(s);
+            pw.flush_full(nbr8, stride, i);
           }
         }
-      for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-          for (int dx = -1; dx <= 1; ++dx) {
-            const int hc = halo_cell<BB>(T, cx, cy, cz, dx, dy, dz);
-            const uint32_t n = bcn[hc];
-            if (n == 0) continue;
-            const uint32_t st = bcs[hc];
-            for (uint32_t k = 0; k < n; ++k) {
-              const uint32_t j = bperm[st + k];
-              const float4 q = bpos[j];
-              if (dist2_exact(p.x, p.y, p.z, q.x, q.y, q.z) < h2) {
-                if (nB < (uint32_t)maxb) nbrB[(size_t)nB * stride + i] = j;
-                ++nB;
-              }
-            }
-          }
-      if (nF > (uint32_t)MAXF) { atomicMax(&overflow[0], nF); nF = MAXF; }
-      if (nB > (uint32_t)maxb) { atomicMax(&overflow[2], nB); nB = maxb; }
-      cnt[i] = nF | (nB << 16);
-    }
-    __syncthreads();
-    // pack-major write-out: for each pack the lanes of a wave store consecutive i
-    for (int pk = 0; pk < NPACK; ++pk) {
-      const bool w = active && (uint32_t)(pk * PACK) < nF;
-      if (__ballot(w) == 0ull) break;
-      if (w) {
-        uint32_t v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const uint32_t k0 = pk * PACK + 2 * u, k1 = k0 + 1;
-          const uint32_t lo = (k0 < nF) ? stage[k0 * BB + t] : 0u;
-          const uint32_t hi = (k1 < nF) ? stage[k1 * BB + t] : 0u;
-          v[u] = lo | (hi << 16);
-        }
-        nbr8[(size_t)pk * stride + i] = make_uint4(v[0], v[1], v[2], v[3]);
       }
-    }
-    __syncthreads();
+    uint32_t nF = pw.finish(nbr8, stride, i);
+    uint32_t nB = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int hc = halo_cell<BB>(T, cx, cy, cz, dx, dy, dz);
+          const uint32_t n = bcn[hc];
+          if (n == 0) continue;
+          const uint32_t st = bcs[hc];
+          for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t j = bperm[st + k];
+            const float4 q = bpos[j];
+            if (dist2_exact(p.x, p.y, p.z, q.x, q.y, q.z) < h2) {
+              if (nB < (uint32_t)maxb) nbrB[(size_t)nB * stride + i] = j;
+              ++nB;
+            }
+          }
+        }
+    if (nF > (uint32_t)MAXF) { atomicMax(&overflow[0], nF); nF = MAXF; }
+    if (nB > (uint32_t)maxb) { atomicMax(&overflow[2], nB); nB = maxb; }
+    cnt[i] = nF | (nB << 16);
   }
 }
 
@@ -166,7 +171,7 @@ constexpr int PT = per_thread<PB>();
 // TimeStep::computeDensities (TimeStep.cpp:147-202) + computeDFSPHFactor
 // (TimeStepDiffDFSPH.cpp:883-962) + the first loop of warmstartDivergenceSolve
 // (:1720-1732), one pass.
-__global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(6, 8)))
 k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alpha, float4* __restrict__ s_out,
                    float* __restrict__ badv, float* __restrict__ kappav, int div_warm, int divergence_on,
                    float2* __restrict__ aux) {
@@ -178,9 +183,10 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
   tile_setup<PB>(a.g, a.fstart, brick, T);
   if (T.own_begin == T.own_end) return;
   const uint32_t total = T.total;
-  const bool in_lds = total <= (uint32_t)TILE_CAP;
-  float* lx = lds; float* ly = lds + TILE_CAP; float* lz = lds + 2 * TILE_CAP;
-  float* lvx = lds + 3 * TILE_CAP; float* lvy = lds + 4 * TILE_CAP; float* lvz = lds + 5 * TILE_CAP;
+  const int cap = a.cap;
+  const bool in_lds = total <= (uint32_t)cap;
+  float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
+  float* lvx = lds + 3 * cap; float* lvy = lds + 4 * cap; float* lvz = lds + 5 * cap;
   const int t = threadIdx.x;
   if (in_lds) {
     uint32_t js[PT];
@@ -278,7 +284,7 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
 // -dt k_i grad_p_ib and receives f = -m_i dv / dt (BoundaryModel::addForce,
 // BoundaryModel.h:46-58), summed here in FP64 per dynamic body.
 // LDS per slot: x, y, z, k.
-__global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(6, 8)))
 k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restrict__ kappa_acc,
                     RigidDev* __restrict__ rig) {
   __shared__ TileHdr<PB> T;
@@ -289,8 +295,9 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
   tile_setup<PB>(a.g, a.fstart, brick, T);
   if (T.own_begin == T.own_end) return;
   const uint32_t total = T.total;
-  const bool in_lds = total <= (uint32_t)TILE_CAP;
-  float* lx = lds; float* ly = lds + TILE_CAP; float* lz = lds + 2 * TILE_CAP; float* lk = lds + 3 * TILE_CAP;
+  const int cap = a.cap;
+  const bool in_lds = total <= (uint32_t)cap;
+  float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap; float* lk = lds + 3 * cap;
   const int t = threadIdx.x;
   if (in_lds) {
     uint32_t js[PT];
@@ -394,7 +401,7 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
 // and k_i = b_i alpha_i kcoef for the next Jacobi update (mode 0 divergence,
 // 1 pressure); mode 2 is the first loop of warmstartPressureSolve (:987-999).
 // LDS per slot: x, y, z, vx, vy, vz.
-__global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(6, 8)))
 k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, const float* __restrict__ alpha,
               float* __restrict__ badv, float* __restrict__ kf, float* __restrict__ kappa, double* __restrict__ err,
               const float* __restrict__ kpred) {
@@ -406,9 +413,10 @@ k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, 
   tile_setup<PB>(a.g, a.fstart, brick, T);
   if (T.own_begin == T.own_end) return;
   const uint32_t total = T.total;
-  const bool in_lds = total <= (uint32_t)TILE_CAP;
-  float* lx = lds; float* ly = lds + TILE_CAP; float* lz = lds + 2 * TILE_CAP;
-  float* lvx = lds + 3 * TILE_CAP; float* lvy = lds + 4 * TILE_CAP; float* lvz = lds + 5 * TILE_CAP;
+  const int cap = a.cap;
+  const bool in_lds = total <= (uint32_t)cap;
+  float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
+  float* lvx = lds + 3 * cap; float* lvy = lds + 4 * cap; float* lvz = lds + 5 * cap;
   const int t = threadIdx.x;
   if (in_lds) {
     uint32_t js[PT];
@@ -508,32 +516,34 @@ static PassArgs make_args(const StepConst& c, const GridDesc& g, GridBufs& gb, F
   PassArgs a;
   a.c = c; a.g = g; a.fstart = gb.fstart; a.pos = f.pos; a.vel = f.vel; a.bpos = b.pos; a.bvel = b.vel;
   a.bx0 = b.x0; a.nbr8 = nb.nbr8; a.nbrB = nb.nbrB; a.cnt = nb.cnt; a.nb_static = b.n - b.nd;
+  a.cap = nb.cap;
   return a;
 }
 
 void launch_build_tiled(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
                         hipStream_t s) {
-  hipLaunchKernelGGL(k_build_tiled, dim3(nbricks(g)), dim3(BB), 0, s, f.pos, f.n, g, gb.fstart, b.pos, gb.bstart,
-                     gb.bperm, h2, nb.maxb, nb.nbr8, nb.nbrB, nb.cnt, nb.overflow);
+  const size_t shm = sizeof(float) * (size_t)nb.cap * 3;
+  hipLaunchKernelGGL(k_build_tiled, dim3(nbricks(g)), dim3(BB), shm, s, f.pos, f.n, g, gb.fstart, b.pos, gb.bstart,
+                     gb.bperm, h2, nb.maxb, nb.cap, nb.nbr8, nb.nbrB, nb.cnt, nb.overflow);
 }
 
 void launch_density_factor_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
                              NbrBufs& nb, int div_warm, int divergence_on, float2* aux, hipStream_t s) {
-  const size_t shm = sizeof(float) * (size_t)TILE_CAP * (divergence_on ? 6 : 3);
+  const size_t shm = sizeof(float) * (size_t)nb.cap * (divergence_on ? 6 : 3);
   hipLaunchKernelGGL(k_density_factor_t, dim3(nbricks(g)), dim3(PB), shm, s, make_args(c, g, gb, f, b, nb), f.rho,
                      f.alpha, f.s, f.badv, f.kappav, div_warm, divergence_on, aux);
 }
 
 void launch_velocity_update_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
                               NbrBufs& nb, RigidDev* rig, const float* kfield, float* kappa_acc, hipStream_t s) {
-  const size_t shm = sizeof(float) * (size_t)TILE_CAP * 4;
+  const size_t shm = sizeof(float) * (size_t)nb.cap * 4;
   hipLaunchKernelGGL(k_velocity_update_t, dim3(nbricks(g)), dim3(PB), shm, s, make_args(c, g, gb, f, b, nb), kfield,
                      kappa_acc, rig);
 }
 
 void launch_recompute_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
                         NbrBufs& nb, int mode, float kcoef, double* err_slot, hipStream_t s, const float* kpred) {
-  const size_t shm = sizeof(float) * (size_t)TILE_CAP * 6;
+  const size_t shm = sizeof(float) * (size_t)nb.cap * 6;
   hipLaunchKernelGGL(k_recompute_t, dim3(nbricks(g)), dim3(PB), shm, s, make_args(c, g, gb, f, b, nb), mode, kcoef,
                      f.rho, f.alpha, f.badv, f.kf, f.kappa, err_slot, kpred);
 }

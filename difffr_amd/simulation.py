@@ -237,6 +237,16 @@ class Simulation:
         self._check(native.load().dfr_neighbor_stats(self.ctx, _dp(o)))
         return dict(mean_fluid=o[0], mean_boundary=o[1], mean_fluid_per_dynamic_sample=o[2])
 
+    def set_halo_capacity(self, slots: int = -1):
+        """LDS halo capacity of the gather passes (-1: follow the largest halo;
+        0: every halo read from global memory)."""
+        self._check(native.load().dfr_set_halo_capacity(self.ctx, int(slots)))
+
+    def halo_stats(self) -> dict:
+        o = np.zeros(3, np.int64)
+        self._check(native.load().dfr_halo_stats(self.ctx, o.ctypes.data_as(C.POINTER(C.c_int64))))
+        return dict(largest_halo=int(o[0]), capacity=int(o[1]), max_capacity=int(o[2]))
+
     def neighbor_lists(self, kind: int):
         """kind 0: fluid->fluid (ids), 1: fluid->boundary (global sample index), 2: dynamic sample->fluid."""
         L = native.load()

@@ -153,6 +153,16 @@ int dfr_reset_stats(dfr_ctx* ctx);
  * neighbours per fluid particle and mean fluid neighbours per dynamic sample. */
 int dfr_neighbor_stats(const dfr_ctx* ctx, double out[3]);
 
+/* Halo tiles of the gather passes (no counterpart in the reference: a tuning
+ * and test hook of this implementation).  Each pass stages a brick's halo of
+ * up to `slots` particles in LDS; by default (slots < 0) the capacity follows
+ * the largest halo of the last neighbour build.  A larger halo, or every halo
+ * with slots = 0, is read from global memory with identical arithmetic.
+ * dfr_halo_stats: out = {largest halo of the last build, capacity in use,
+ * maximum capacity}. */
+int dfr_set_halo_capacity(dfr_ctx* ctx, int slots);
+int dfr_halo_stats(const dfr_ctx* ctx, int64_t out[3]);
+
 /* Neighbour lists of the last step in particle-id / sample-index terms:
  * kind 0 fluid->fluid (ids), kind 1 fluid->boundary (global sample index of
  * `counts`-ordered bodies), kind 2 dynamic sample->fluid (ids).  counts gets

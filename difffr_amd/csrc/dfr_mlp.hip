@@ -37,6 +37,16 @@ __device__ constexpr int MLP_OB[4] = {384, 4544, 8704, 8896}; // bias offsets
 __device__ constexpr int MLP_FW[4] = {0, 4, 12, 20};
 __device__ constexpr int MLP_FT[4] = {42, 34, 26, 22};
 constexpr int MLP_NFRAG = 44;
+// A second set of the same 44 fragments, K-permuted for register chaining
+// (fragment f of the chain set is at MLP_NFRAG + f).  The D tile of a layer
+// gives lane group g the rows 4g..4g+3; the two D tiles 2c, 2c+1 therefore
+// form lane group g's eight B elements of K chunk c if element j of that group
+// stands for unit chain_unit(c, g, j) -- and the A fragments use the same
+// unit for that K slot, so the product is unchanged.
+constexpr int MLP_NFRAG_ALL = 2 * MLP_NFRAG;
+__host__ __device__ constexpr int chain_unit(int kc, int g, int j) {
+  return 32 * kc + (j < 4 ? 4 * g + j : 12 + 4 * g + j);
+}
 
 constexpr int LD0 = 40;   // particle-major stride of the padded input (32 + 8)
 constexpr int LDH = 72;   // particle-major / unit-major stride of 64-wide layers (64 + 8)
@@ -53,9 +63,10 @@ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // ---------------------------------------------------------------------------
 __global__ void k_mlp_pack(const float* __restrict__ P, bf16x8* __restrict__ F) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= MLP_NFRAG * 64) return;
+  if (g >= MLP_NFRAG_ALL * 64) return;
   const int l = g & 63;
-  const int f = g >> 6;  // fragment index
+  const bool chain = (g >> 6) >= MLP_NFRAG;
+  const int f = (g >> 6) - (chain ? MLP_NFRAG : 0);  // fragment index within its set
   int layer = -1, tr = 0, local = 0;
   for (int L = 0; L < 4; ++L) {
     const int nf = ceil_div(MLP_O[L], 16) * ceil_div(MLP_I[L], 32);
@@ -71,7 +82,7 @@ __global__ void k_mlp_pack(const float* __restrict__ P, bf16x8* __restrict__ F) 
     const int mt = local / kt, kc = local % kt;
     const int m = 16 * mt + (l & 15);
     for (int j = 0; j < 8; ++j) {
-      const int k = 32 * kc + 8 * (l >> 4) + j;
+      const int k = chain ? chain_unit(kc, l >> 4, j) : 32 * kc + 8 * (l >> 4) + j;
       const int o = tr ? k : m, i = tr ? m : k;
       const float w = (o < O && i < I) ? P[MLP_OW[layer] + o * I + i] : 0.f;
       v[j] = (__bf16)w;
@@ -360,153 +371,153 @@ __global__ void __launch_bounds__(64) k_mlp(MlpArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// dL/d(features) for dL/d(predictions): the network's input gradient, used by
-// the exact adjoint through the learned warm start.  Keeps only the bf16
-// activations h0, h1, h2 in LDS and recomputes each pre-activation
-// Z_l = W_l H_{l-1} + b_l in the backward sweep (same MFMA, same inputs, so the
-// same values as the forward), which lands in the same D-fragment layout as
-// dH_l; 32 KB of LDS per wavefront.
+// Register-chained network: inference and the input gradient dL/d(features)
+// for dL/d(predictions) (used by the exact adjoint through the learned warm
+// start).  A workgroup stages the K-permuted fragments (22 forward, + 22
+// transposed for the gradient) and the biases in LDS once; each wavefront then
+// carries 16-particle tiles through every layer in registers: the D tiles of a
+// layer, rounded to bf16, are the next layer's B operand as they stand.  The
+// gradient keeps silu'(z) of the three hidden layers (D layout, FP32) from the
+// tile's forward sweep, so nothing is recomputed.
 // ---------------------------------------------------------------------------
-struct IgLds {
-  static constexpr int H0 = 0;                    // bf16 [64][LD0]
-  static constexpr int H1 = H0 + 64 * LD0 * 2;    // bf16 [64][LDH]
-  static constexpr int H2 = H1 + 64 * LDH * 2;    // bf16 [64][LDH]
-  static constexpr int DZ = H2 + 64 * LDH * 2;    // bf16 [64][LDH]
-  static constexpr int BYTES = DZ + 64 * LDH * 2;
+constexpr int CB = 256;                        // chain workgroup: 4 wavefronts x 64 particles
+constexpr int CHAIN_BIAS = 3 * 64 + 16;        // biases of layers 0..3 (output layer zero-padded to a tile)
+template <bool IG> struct ChainLds {
+  static constexpr int NF = IG ? MLP_NFRAG : MLP_FT[3];  // chain fragments staged (forward ones first)
+  static constexpr int FRAG = 0;
+  static constexpr int BIAS = NF * 64 * 16;
+  static constexpr int BYTES = BIAS + CHAIN_BIAS * 4;
 };
+__device__ constexpr int CHAIN_OB[4] = {0, 64, 128, 192};  // bias offsets in the staged copy
 
-// The B fragments of particle tile nt of a particle-major activation (kt K-chunks).
-__device__ inline void load_b(const __bf16* in, int ldi, int nt, int kt, int lr, int lg, bf16x8 (&B)[2]) {
+__device__ inline bf16x8 pack2(f32x4 lo, f32x4 hi) {
+  bf16x8 v;
 #pragma unroll
-  for (int kc = 0; kc < 2; ++kc)
-    if (kc < kt) B[kc] = *reinterpret_cast<const bf16x8*>(in + (16 * nt + lr) * ldi + 32 * kc + 8 * lg);
-}
-// One 16x16 tile of Z = W H from preloaded fragments.
-__device__ inline f32x4 tile_mma(const bf16x8 (&A)[4][2], const bf16x8 (&B)[2], int mt, int kt) {
-  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int kc = 0; kc < 2; ++kc)
-    if (kc < kt) acc = mfma(A[mt][kc], B[kc], acc);
-  return acc;
-}
-// The lane's biases of layer L: unit 16 mt + 4 lg + j.
-__device__ inline void load_bias(const float* P, int L, int lg, float (&b)[4][4]) {
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) b[mt][j] = P[MLP_OB[L] + 16 * mt + 4 * lg + j];
+  for (int j = 0; j < 4; ++j) { v[j] = (__bf16)lo[j]; v[4 + j] = (__bf16)hi[j]; }
+  return v;
 }
 
-template <int L> __device__ inline void load_frags(const bf16x8* frag, int base, bf16x8 (&A)[4][2], int l, int mt_n,
-                                                   int kt) {
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-    for (int kc = 0; kc < 2; ++kc)
-      if (mt < mt_n && kc < kt) A[mt][kc] = frag[(base + mt * kt + kc) * 64 + l];
-}
-
-__global__ void __launch_bounds__(64) k_mlp_input_grad(MlpArgs a) {
+template <bool IG>
+__global__ void __launch_bounds__(CB) k_mlp_chain(MlpArgs a) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  __bf16* h0 = reinterpret_cast<__bf16*>(smem + IgLds::H0);
-  __bf16* h1 = reinterpret_cast<__bf16*>(smem + IgLds::H1);
-  __bf16* h2 = reinterpret_cast<__bf16*>(smem + IgLds::H2);
-  __bf16* dz = reinterpret_cast<__bf16*>(smem + IgLds::DZ);
-  const int l = threadIdx.x;
-  const int lr = l & 15, lg = l >> 4;
-  const int p0 = blockIdx.x * 64;
-  const int p = p0 + l;
-  const bool valid = p < a.n;
-  const MlpNorm& N = a.norm;
-  float fr[6];
-  if (valid) {
-    if (a.feat) for (int k = 0; k < 6; ++k) fr[k] = a.feat[(size_t)p * 6 + k];
-    else features_from_fields(a.fields, (uint32_t)p, fr);
-  } else {
-    for (int k = 0; k < 6; ++k) fr[k] = N.mu_in[k];
+  using L = ChainLds<IG>;
+  bf16x8* F = reinterpret_cast<bf16x8*>(smem + L::FRAG);
+  float* bias = reinterpret_cast<float*>(smem + L::BIAS);
+  for (int k = threadIdx.x; k < L::NF * 64; k += CB) F[k] = a.frag[MLP_NFRAG * 64 + k];
+  for (int k = threadIdx.x; k < CHAIN_BIAS; k += CB) {
+    const int layer = k < 192 ? k / 64 : 3;
+    const int u = k - CHAIN_OB[layer];
+    bias[k] = u < MLP_O[layer] ? a.P[MLP_OB[layer] + u] : 0.f;
   }
-  for (int k = 0; k < 32; ++k) h0[l * LD0 + k] = (__bf16)((k < 6) ? (fr[k] - N.mu_in[k]) * N.is_in[k] : 0.f);
-  // output adjoint in normalised units, K padded to 32
-  const float g0 = valid ? a.ybar[(size_t)p * 2 + 0] * N.sd_out[0] : 0.f;
-  const float g1 = valid ? a.ybar[(size_t)p * 2 + 1] * N.sd_out[1] : 0.f;
-  for (int k = 0; k < 32; ++k) dz[l * LDH + k] = (__bf16)((k == 0) ? g0 : (k == 1) ? g1 : 0.f);
   __syncthreads();
-  bf16x8 Aw[4][2];
-  float bv[4][4];
-  // forward: h1 = silu(W1 h0 + b1), h2 = silu(W2 h1 + b2)
-#pragma unroll
-  for (int L = 0; L < 2; ++L) {
-    const int kt = (L == 0) ? 1 : 2;
-    load_frags<0>(a.frag, MLP_FW[L], Aw, l, 4, kt);
-    load_bias(a.P, L, lg, bv);
-    const __bf16* hin = (L == 0) ? h0 : h1;
-    __bf16* hout = (L == 0) ? h1 : h2;
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      bf16x8 Bf[2];
-      load_b(hin, (L == 0) ? LD0 : LDH, nt, kt, lr, lg, Bf);
-#pragma unroll
-      for (int mt = 0; mt < 4; ++mt) {
-        const f32x4 z = tile_mma(Aw, Bf, mt, kt);
-        bf16x4 hv;
-        for (int j = 0; j < 4; ++j) hv[j] = (__bf16)silu(z[j] + bv[mt][j]);
-        *reinterpret_cast<bf16x4*>(hout + (16 * nt + lr) * LDH + 16 * mt + 4 * lg) = hv;
-      }
-    }
-    __syncthreads();
-  }
-  // backward: for L = 3, 2, 1: dH = W_L^T dZ_L; dZ_{L-1} = dH * silu'(Z_{L-1}), Z recomputed
-#pragma unroll
-  for (int L = 3; L >= 1; --L) {
-    const int kt = (MLP_O[L] + 31) / 32;
-    bf16x8 At[4][2];
-    load_frags<0>(a.frag, MLP_FT[L], At, l, 4, kt);
-    f32x4 dh[4][4];
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      bf16x8 Bf[2];
-      load_b(dz, LDH, nt, kt, lr, lg, Bf);
-#pragma unroll
-      for (int mt = 0; mt < 4; ++mt) dh[nt][mt] = tile_mma(At, Bf, mt, kt);
-    }
-    __syncthreads();  // every lane has read dz
-    // Z_{L-1} = W_{L-1} H_{L-2} + b_{L-1}
-    const int kz = (L - 1 == 0) ? 1 : 2;
-    load_frags<0>(a.frag, MLP_FW[L - 1], Aw, l, 4, kz);
-    load_bias(a.P, L - 1, lg, bv);
-    const __bf16* hin = (L - 1 == 0) ? h0 : (L - 1 == 1) ? h1 : h2;
-    const int ldi = (L - 1 == 0) ? LD0 : LDH;
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      bf16x8 Bf[2];
-      load_b(hin, ldi, nt, kz, lr, lg, Bf);
-#pragma unroll
-      for (int mt = 0; mt < 4; ++mt) {
-        const f32x4 z = tile_mma(Aw, Bf, mt, kz);
-        bf16x4 dv;
-        const int pp = 16 * nt + lr;
-        for (int j = 0; j < 4; ++j)
-          dv[j] = (__bf16)((p0 + pp < a.n) ? dh[nt][mt][j] * silu_d(z[j] + bv[mt][j]) : 0.f);
-        *reinterpret_cast<bf16x4*>(dz + pp * LDH + 16 * mt + 4 * lg) = dv;
-      }
-    }
-    __syncthreads();
-  }
-  // dH_0 = W_1^T dZ_1 (inputs padded to 16): dL/df = dL/dx * is
-  bf16x8 At0[4][2];
-  load_frags<0>(a.frag, MLP_FT[0], At0, l, 1, 2);
-#pragma unroll
+  const int l = threadIdx.x & 63;
+  const int lr = l & 15, lg = l >> 4;
+  const int p0 = (blockIdx.x * (CB / 64) + (threadIdx.x >> 6)) * 64;
+  if (p0 >= a.n) return;  // no barrier follows
+  const MlpNorm& N = a.norm;
+#pragma unroll 1
   for (int nt = 0; nt < 4; ++nt) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kc = 0; kc < 2; ++kc) {
-      const bf16x8 B = *reinterpret_cast<const bf16x8*>(dz + (16 * nt + lr) * LDH + 32 * kc + 8 * lg);
-      acc = mfma(At0[0][kc], B, acc);
+    // Fragments and biases are re-read from LDS for every tile: the empty asm
+    // hides the loop invariance, so they are not hoisted into registers.
+    const bf16x8* Ft = F;
+    const float* bt = bias;
+    asm volatile("" : "+v"(Ft), "+v"(bt));
+    auto frag = [&](int f) { return Ft[f * 64 + l]; };
+    auto lbias = [&](int layer, int mt) {
+      return *reinterpret_cast<const f32x4*>(bt + CHAIN_OB[layer] + 16 * mt + 4 * lg);
+    };
+    const int p = p0 + 16 * nt + lr;
+    const bool valid = p < a.n;
+    // ---- input: lane group 0 holds features 0..3, group 1 features 4, 5 (chain_unit(0, g, j))
+    float x[6];
+    if (valid) {
+      if (a.feat) {
+        for (int k = 0; k < 6; ++k) x[k] = a.feat[(size_t)p * 6 + k];
+      } else {
+        features_from_fields(a.fields, (uint32_t)p, x);
+      }
+      for (int k = 0; k < 6; ++k) x[k] = (x[k] - N.mu_in[k]) * N.is_in[k];
+    } else {
+      for (int k = 0; k < 6; ++k) x[k] = 0.f;
     }
-    const int pp = 16 * nt + lr;
-    for (int j = 0; j < 4; ++j) {
-      const int k = 4 * lg + j;
-      if (k < 6 && p0 + pp < a.n) a.fbar[(size_t)(p0 + pp) * 6 + k] = acc[j] * N.is_in[k];
+    bf16x8 B[2];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float v = (j < 4 && lg == 0) ? x[j] : (j < 2 && lg == 1) ? x[4 + j] : 0.f;
+      B[0][j] = (__bf16)v;
+      B[1][j] = (__bf16)0.f;
+    }
+    // ---- hidden layers 0..2: z = W h + b, h' = silu(z)
+    f32x4 sp[3][4];  // silu'(z) per hidden layer (IG only)
+#pragma unroll
+    for (int layer = 0; layer < 3; ++layer) {
+      const int kt = layer == 0 ? 1 : 2;
+      f32x4 h[4];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        f32x4 acc = lbias(layer, mt);
+#pragma unroll
+        for (int kc = 0; kc < 2; ++kc)
+          if (kc < kt) acc = mfma(frag(MLP_FW[layer] + mt * kt + kc), B[kc], acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float z = acc[j];
+          const float sg = sigmoid_fast(z);
+          h[mt][j] = z * sg;
+          if (IG) sp[layer][mt][j] = sg * (1.f + z * (1.f - sg));
+        }
+      }
+      B[0] = pack2(h[0], h[1]);
+      B[1] = pack2(h[2], h[3]);
+    }
+    // ---- output layer (2 units, rows 0 and 1 of lane group 0)
+    f32x4 y = lbias(3, 0);
+    y = mfma(frag(MLP_FW[3] + 0), B[0], y);
+    y = mfma(frag(MLP_FW[3] + 1), B[1], y);
+    if (!IG) {
+      if (valid && lg == 0) {
+        a.out[(size_t)p * 2 + 0] = y[0] * N.sd_out[0] + N.mu_out[0];
+        a.out[(size_t)p * 2 + 1] = y[1] * N.sd_out[1] + N.mu_out[1];
+      }
+      continue;
+    }
+    // ---- backward: dZ_3 = ybar * sd_out (normalised units), then dH = W^T dZ, dZ = dH * silu'(z)
+    float g0 = 0.f, g1 = 0.f;
+    if (valid && lg == 0) {
+      g0 = a.ybar[(size_t)p * 2 + 0] * N.sd_out[0];
+      g1 = a.ybar[(size_t)p * 2 + 1] * N.sd_out[1];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      B[0][j] = (__bf16)(j == 0 ? g0 : j == 1 ? g1 : 0.f);
+      B[1][j] = (__bf16)0.f;
+    }
+#pragma unroll
+    for (int layer = 3; layer >= 1; --layer) {
+      const int kt = layer == 3 ? 1 : 2;
+      f32x4 dz[4];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < 2; ++kc)
+          if (kc < kt) acc = mfma(frag(MLP_FT[layer] + mt * kt + kc), B[kc], acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) dz[mt][j] = acc[j] * sp[layer - 1][mt][j];
+      }
+      B[0] = pack2(dz[0], dz[1]);
+      B[1] = pack2(dz[2], dz[3]);
+    }
+    // dH_0 = W_1^T dZ_1 (features 0..3 in lane group 0, 4..5 in group 1): dL/df = dL/dx * is
+    f32x4 din = {0.f, 0.f, 0.f, 0.f};
+    din = mfma(frag(MLP_FT[0] + 0), B[0], din);
+    din = mfma(frag(MLP_FT[0] + 1), B[1], din);
+    if (valid && lg < 2) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = 4 * lg + j;
+        if (k < 6) a.fbar[(size_t)p * 6 + k] = din[j] * N.is_in[k];
+      }
     }
   }
 }
@@ -589,11 +600,12 @@ __global__ void k_mlp_apply_div(int n, const float* __restrict__ pred, const flo
 static inline unsigned nb64(int n) { return (unsigned)((n + 63) / 64); }
 
 void launch_mlp_pack(const float* P, void* frag, hipStream_t s) {
-  hipLaunchKernelGGL(k_mlp_pack, dim3((MLP_NFRAG * 64 + 255) / 256), dim3(256), 0, s, P,
+  hipLaunchKernelGGL(k_mlp_pack, dim3((MLP_NFRAG_ALL * 64 + 255) / 256), dim3(256), 0, s, P,
                      static_cast<bf16x8*>(frag));
 }
 
-size_t mlp_frag_bytes() { return sizeof(bf16x8) * (size_t)MLP_NFRAG * 64; }
+size_t mlp_frag_bytes() { return sizeof(bf16x8) * (size_t)MLP_NFRAG_ALL * 64; }
+static inline unsigned nb_chain(int n) { return (unsigned)((n + CB - 1) / CB); }
 
 static MlpArgs base_args(const void* frag, const float* P, const MlpNorm& norm, int n) {
   MlpArgs a{};
@@ -610,7 +622,7 @@ void launch_mlp_infer_fields(const void* frag, const float* P, const MlpNorm& no
   MlpArgs a = base_args(frag, P, norm, n);
   a.fields = f;
   a.out = out;
-  hipLaunchKernelGGL(k_mlp<MLP_INFER>, dim3(nb64(n)), dim3(64), MlpLds::INFER_BYTES, s, a);
+  hipLaunchKernelGGL(k_mlp_chain<false>, dim3(nb_chain(n)), dim3(CB), ChainLds<false>::BYTES, s, a);
 }
 
 void launch_mlp_infer_array(const void* frag, const float* P, const MlpNorm& norm, const float* feat, int n,
@@ -619,7 +631,7 @@ void launch_mlp_infer_array(const void* frag, const float* P, const MlpNorm& nor
   MlpArgs a = base_args(frag, P, norm, n);
   a.feat = feat;
   a.out = out;
-  hipLaunchKernelGGL(k_mlp<MLP_INFER>, dim3(nb64(n)), dim3(64), MlpLds::INFER_BYTES, s, a);
+  hipLaunchKernelGGL(k_mlp_chain<false>, dim3(nb_chain(n)), dim3(CB), ChainLds<false>::BYTES, s, a);
 }
 
 void launch_mlp_train_batch(const void* frag, const float* P, const MlpNorm& norm, const float* feat,
@@ -644,7 +656,7 @@ void launch_mlp_input_grad(const void* frag, const float* P, const MlpNorm& norm
   if (fields) a.fields = *fields;
   a.ybar = ybar;
   a.fbar = fbar;
-  hipLaunchKernelGGL(k_mlp_input_grad, dim3(nb64(n)), dim3(64), IgLds::BYTES, s, a);
+  hipLaunchKernelGGL(k_mlp_chain<true>, dim3(nb_chain(n)), dim3(CB), ChainLds<true>::BYTES, s, a);
 }
 
 void launch_mlp_adam(float* P, float* G, float* m, float* v, float lr, int t, hipStream_t s) {
@@ -679,12 +691,8 @@ void launch_mlp_apply_div(int n, const float* pred, const float* d, float inv_dt
 }
 
 void set_mlp_kernel_attributes() {
-  (void)hipFuncSetAttribute((const void*)k_mlp<MLP_INFER>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            MlpLds::INFER_BYTES);
   (void)hipFuncSetAttribute((const void*)k_mlp<MLP_TRAIN>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             MlpLds::TRAIN_BYTES);
-  (void)hipFuncSetAttribute((const void*)k_mlp_input_grad, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            IgLds::BYTES);
 }
 
 }  // namespace dfr

@@ -84,6 +84,8 @@ struct Arena {
     return u + off;
   }
   void reset() { cur = 0; off = 0; }
+  std::pair<size_t, size_t> mark() const { return {cur, off}; }
+  void rewind(std::pair<size_t, size_t> m) { cur = m.first; off = m.second; }
   void release() {
     for (auto& ch : chunks) (void)hipFree(ch.first);
     chunks.clear();
@@ -203,6 +205,14 @@ struct dfr_ctx {
   int halo_cap_forced = -1;          // >= 0: LDS halo capacity fixed by dfr_set_halo_capacity
   uint32_t halo_max = 0;             // largest halo of the last build
   int err_slots = 0;
+  // Jacobi solves without a host round trip per iteration: stop flags
+  // (divergence, pressure), per-iteration probes in mapped pinned memory,
+  // and two events the host waits on one iteration late
+  int* d_stop = nullptr;
+  SolveProbe* h_probe = nullptr;
+  SolveProbe* d_probe = nullptr;
+  int probe_cap = 0;
+  hipEvent_t pipe_ev[2] = {nullptr, nullptr};
 
   float time = 0.f;
   int64_t step_count = 0;
@@ -292,8 +302,10 @@ void drain_timers(dfr_ctx* c) {
     float ms = 0.f;
     HIPCHK(hipEventSynchronize(r.b));
     HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
-    c->t_sum[r.cls] += ms;
-    c->t_cnt[r.cls] += 1;
+    if (r.cls >= 0) {  // launches of a Jacobi iteration that did not run are not counted
+      c->t_sum[r.cls] += ms;
+      c->t_cnt[r.cls] += 1;
+    }
     c->event_pool.push_back(r.a);
     c->event_pool.push_back(r.b);
   }
@@ -490,6 +502,31 @@ template <typename T> void d2d(dfr_ctx* c, T* dst, const T* src, size_t n) {
   if (n > 0) HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
 }
 
+// Device copies and zero fills issued at one point of the step, sent as one
+// k_multi_copy launch (pieces that are not 16-byte aligned go through the
+// runtime's own copy / fill).
+struct Batch {
+  dfr_ctx* c;
+  CopyList L{};
+  explicit Batch(dfr_ctx* c_) : c(c_) { L.n = 0; }
+  template <typename T> void copy(T* dst, const T* src, size_t n) { add(dst, src, n * sizeof(T)); }
+  template <typename T> void zero(T* dst, size_t n) { add(dst, nullptr, n * sizeof(T)); }
+  void add(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return;
+    if (((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15u) != 0 || (bytes & 3u) != 0) {
+      if (src) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+      else HIPCHK(hipMemsetAsync(dst, 0, bytes, c->stream));
+      return;
+    }
+    if (L.n == COPY_SEGS) flush();
+    L.seg[L.n++] = CopySeg{src, dst, (uint64_t)bytes};
+  }
+  void flush() {
+    launch_multi_copy(L, c->stream);
+    L.n = 0;
+  }
+};
+
 // Per-iteration tape buffers, grown on first use.
 template <typename T> T* talloc(dfr_ctx* c, StepTapeDev* rec, size_t n) {
   return rec && rec->arena ? static_cast<T*>(rec->arena->alloc(sizeof(T) * std::max<size_t>(n, 1))) : dalloc<T>(c, n);
@@ -502,9 +539,83 @@ void tape_iteration(dfr_ctx* c, std::vector<float4*>& v, std::vector<float*>& k,
     k.push_back(talloc<float>(c, c->rec, n));
     b.push_back(talloc<float>(c, c->rec, n));
   }
-  d2d(c, v[m], c->f.vel, c->f.n);
-  d2d(c, k[m], c->f.kf, c->f.n);
-  d2d(c, b[m], c->f.badv, c->f.n);
+  Batch bt(c);
+  bt.copy(v[m], c->f.vel, c->f.n);
+  bt.copy(k[m], c->f.kf, c->f.n);
+  bt.copy(b[m], c->f.badv, c->f.n);
+  bt.flush();
+}
+
+// The Jacobi iterations of one solve.  fixed >= 0: exactly that many (replay,
+// imposed counts).  Otherwise the reference's loop condition decides
+// (divergenceSolve :816-828, pressureSolve :700-712), evaluated on the device
+// by k_solve_check: every launch of an iteration is guarded by the solve's
+// stop flag, and the host keeps one iteration enqueued beyond the one whose
+// probe it waits for, so the stream never drains between iterations.  The
+// iteration enqueued after the last one is a no-op: its tape slot is given
+// back and its timers are not counted.  With the reference-mode gradient on,
+// each iteration waits for its residual instead (those launches are not
+// guarded).
+struct IterTape {
+  std::vector<float4*>* v;
+  std::vector<float*>* k;
+  std::vector<float*>* b;
+};
+
+template <class Iter>
+int run_solve(dfr_ctx* c, int solve, int fixed, int min_it, int max_it, double eta, double* err0, int n, Iter&& iter,
+              IterTape* tape, double& avg) {
+  if (fixed >= 0) {
+    for (int it = 0; it < fixed; ++it) iter(it, nullptr);
+    return fixed;
+  }
+  if (c->P.compute_gradient && !c->replay) {
+    bool chk = false;
+    int it = 0;
+    while ((!chk || it < min_it) && it < max_it) {
+      iter(it, nullptr);
+      avg = read_err(c, (int)(err0 - c->d_err) + std::min(it, max_it - 1)) / (double)n;
+      chk = avg <= eta;
+      ++it;
+    }
+    return it;
+  }
+  int* stop = c->d_stop + solve;
+  SolveProbe* hp = c->h_probe + (size_t)solve * c->probe_cap;
+  SolveProbe* dp = c->d_probe + (size_t)solve * c->probe_cap;
+  Arena* arena = (c->rec && c->rec->arena) ? c->rec->arena : nullptr;
+  for (int it = 0;; ++it) {
+    hp[it].done = -1;
+    const std::pair<size_t, size_t> mark = arena ? arena->mark() : std::pair<size_t, size_t>{0, 0};
+    const size_t tsz = tape ? tape->v->size() : 0;
+    const size_t timers = c->pending.size();
+    iter(it, stop);
+    launch_solve_check(err0 + std::min(it, max_it - 1), n, eta, it, min_it, max_it, stop, dp + it, c->stream);
+    HIPCHK(hipEventRecord(c->pipe_ev[it & 1], c->stream));
+    if (it >= 1) {
+      HIPCHK(hipEventSynchronize(c->pipe_ev[(it - 1) & 1]));
+      check_overflow(c);
+      const SolveProbe pr = hp[it - 1];
+      if (pr.done != 0 && pr.done != 1) throw std::runtime_error("Jacobi solve: iteration probe not written");
+      if (pr.done) {
+        for (size_t r = timers; r < c->pending.size(); ++r) c->pending[r].cls = -1;
+        if (tape && arena && tape->v->size() > tsz) {
+          tape->v->resize(tsz);
+          tape->k->resize(tsz);
+          tape->b->resize(tsz);
+          arena->rewind(mark);
+        }
+        avg = pr.avg;
+        return it;
+      }
+    }
+    if (it + 1 >= max_it) {  // the last iteration the loop allows, and the previous one did not end the solve
+      HIPCHK(hipEventSynchronize(c->pipe_ev[it & 1]));
+      check_overflow(c);
+      avg = hp[it].avg;
+      return it + 1;
+    }
+  }
 }
 
 // Iteration count imposed on the current step (-1: run to convergence).
@@ -520,33 +631,41 @@ void do_step(dfr_ctx* c) {
   const int nf = c->f.n;
   StepTapeDev* rec = c->rec;
   if (rec) {
-    d2d(c, rec->rig_pre, c->d_rig, c->bodies.size());
-    d2d(c, rec->bpos, c->b.pos, c->b.n);
-    d2d(c, rec->bvel, c->b.vel, c->b.n);
+    Batch bt(c);
+    bt.copy(rec->rig_pre, c->d_rig, c->bodies.size());
+    bt.copy(rec->bpos, c->b.pos, c->b.n);
+    bt.copy(rec->bvel, c->b.vel, c->b.n);
+    bt.flush();
   }
   begin_step(c);
-  if (rec) d2d(c, rec->rig_forced, c->d_rig, c->bodies.size());
   {
     Timed t(c, TC_SORT);
     if (nf > 0) launch_sort_fluid(c->f, c->grid, c->gb, c->stream);
     launch_sort_boundary(c->b, c->grid, c->gb, c->stream);
   }
-  if (rec) {
-    d2d(c, rec->x, c->f.pos, nf);
-    if (c->mlp.enabled) d2d(c, rec->v0, c->f.vel, nf);
-    d2d(c, rec->kvPrev, c->f.kappav, nf);
-    d2d(c, rec->kPrev, c->f.kappa, nf);
+  {
+    // tape of the step's start (the sort leaves the rigid state alone) and the step's counters
+    Batch bt(c);
+    if (rec) {
+      bt.copy(rec->rig_forced, c->d_rig, c->bodies.size());
+      bt.copy(rec->x, c->f.pos, nf);
+      if (c->mlp.enabled) bt.copy(rec->v0, c->f.vel, nf);
+      bt.copy(rec->kvPrev, c->f.kappav, nf);
+      bt.copy(rec->kPrev, c->f.kappa, nf);
+    }
+    bt.zero(c->nb.overflow, 4);
+    bt.zero(c->d_err, c->err_slots);
+    bt.zero(c->d_stop, 2);
+    bt.flush();
   }
   {
     Timed t(c, TC_NBR);
-    HIPCHK(hipMemsetAsync(c->nb.overflow, 0, 4 * sizeof(uint32_t), c->stream));
     if (nf > 0) {
       launch_build_tiled(c->f, c->b, c->grid, c->gb, c->nb, c->h * c->h, c->stream);
       launch_build_bf(c->f, c->b, c->grid, c->gb, c->nb, c->h * c->h, c->stream);
     }
     HIPCHK(hipMemcpyAsync(c->h_overflow, c->nb.overflow, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   }
-  HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(double) * c->err_slots, c->stream));
   const bool divOn = c->P.enable_divergence_solver != 0;
   const bool warmV = c->P.use_divergence_warmstart != 0;
   const bool warmP = c->P.use_pressure_warmstart != 0;
@@ -572,8 +691,10 @@ void do_step(dfr_ctx* c) {
                          c->stream);
   }
   if (rec && nf > 0) {
-    d2d(c, rec->d0, c->f.badv, nf);
-    d2d(c, rec->kvW, c->f.kappav, nf);
+    Batch bt(c);
+    bt.copy(rec->d0, c->f.badv, nf);
+    bt.copy(rec->kvW, c->f.kappav, nf);
+    bt.flush();
   }
   const int fixV = imposed(c, false), fixP = imposed(c, true);
   // ---- divergence solve (TimeStepDiffDFSPH::divergenceSolve, :770-881)
@@ -591,26 +712,23 @@ void do_step(dfr_ctx* c) {
                          c->stream);
     }
     const double eta = (1.0f / dt) * (float)c->P.max_error_v * 0.01f * c->sc.rho0;
-    bool chk = false;
-    while (fixV >= 0 ? itv < fixV : ((!chk || itv < 1) && itv < c->P.max_iterations_v)) {
-      run_gradient(c, 1);
-      if (rec) tape_iteration(c, rec->vD, rec->kD, rec->bD, itv);
-      {
-        Timed t(c, TC_VUPDATE);
-        launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kf,
-                                 warmV ? c->f.kappav : nullptr, c->stream);
-      }
-      {
-        Timed t(c, TC_RECOMPUTE);
-        launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_DIV, invH,
-                           c->d_err + std::min(itv, c->P.max_iterations_v - 1), c->stream);
-      }
-      if (fixV < 0) {
-        avgv = read_err(c, itv) / (double)nf;
-        chk = avgv <= eta;
-      }
-      ++itv;
-    }
+    const int maxv = c->P.max_iterations_v;
+    IterTape tv{rec ? &rec->vD : nullptr, rec ? &rec->kD : nullptr, rec ? &rec->bD : nullptr};
+    itv = run_solve(
+        c, 0, fixV, 1, maxv, eta, c->d_err, nf,
+        [&](int m, const int* stop) {
+          run_gradient(c, 1);
+          if (rec) tape_iteration(c, rec->vD, rec->kD, rec->bD, m);
+          {
+            Timed t(c, TC_VUPDATE);
+            launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kf,
+                                     warmV ? c->f.kappav : nullptr, c->stream, stop);
+          }
+          Timed t(c, TC_RECOMPUTE);
+          launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_DIV, invH,
+                             c->d_err + std::min(m, maxv - 1), c->stream, nullptr, stop);
+        },
+        rec ? &tv : nullptr, avgv);
   }
   {
     Timed t(c, TC_ELEMWISE);
@@ -629,8 +747,10 @@ void do_step(dfr_ctx* c) {
                            ml.kpred);
       }
       if (rec) {
-        d2d(c, rec->rsW, c->f.badv, nf);
-        d2d(c, rec->kW, c->f.kappa, nf);
+        Batch bt(c);
+        bt.copy(rec->rsW, c->f.badv, nf);
+        bt.copy(rec->kW, c->f.kappa, nf);
+        bt.flush();
       }
       Timed t(c, TC_VUPDATE);
       launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kappa, nullptr, c->stream);
@@ -641,27 +761,23 @@ void do_step(dfr_ctx* c) {
                          c->d_err + scratch_slot, c->stream);
     }
     const double eta = (float)c->P.max_error * 0.01f * c->sc.rho0;
-    const int base = c->P.max_iterations_v;
-    bool chk = false;
-    while (fixP >= 0 ? it < fixP : ((!chk || it < c->P.min_iterations) && it < c->P.max_iterations)) {
-      run_gradient(c, 0);
-      if (rec) tape_iteration(c, rec->vP, rec->kP, rec->bP, it);
-      {
-        Timed t(c, TC_VUPDATE);
-        launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kf,
-                                 warmP ? c->f.kappa : nullptr, c->stream);
-      }
-      {
-        Timed t(c, TC_RECOMPUTE);
-        launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_PRESSURE, invH2,
-                           c->d_err + base + std::min(it, c->P.max_iterations - 1), c->stream);
-      }
-      if (fixP < 0) {
-        avg = read_err(c, base + it) / (double)nf;
-        chk = avg <= eta;
-      }
-      ++it;
-    }
+    const int base = c->P.max_iterations_v, maxp = c->P.max_iterations;
+    IterTape tp{rec ? &rec->vP : nullptr, rec ? &rec->kP : nullptr, rec ? &rec->bP : nullptr};
+    it = run_solve(
+        c, 1, fixP, c->P.min_iterations, maxp, eta, c->d_err + base, nf,
+        [&](int m, const int* stop) {
+          run_gradient(c, 0);
+          if (rec) tape_iteration(c, rec->vP, rec->kP, rec->bP, m);
+          {
+            Timed t(c, TC_VUPDATE);
+            launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kf,
+                                     warmP ? c->f.kappa : nullptr, c->stream, stop);
+          }
+          Timed t(c, TC_RECOMPUTE);
+          launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_PRESSURE, invH2,
+                             c->d_err + base + std::min(m, maxp - 1), c->stream, nullptr, stop);
+        },
+        rec ? &tp : nullptr, avg);
     Timed t(c, TC_ELEMWISE);
     launch_position(c->sc, c->f, warmP ? dt * dt : 1.f, c->stream);
     if (ml.capturing)  // the converged stiffness of this step is the training target
@@ -747,10 +863,12 @@ void snapshot_save(dfr_ctx* c, Snapshot& S) {
   Timed t(c, TC_CHECKPOINT);
   snapshot_alloc(c, S);
   const int nf = c->f.n;
-  d2d(c, S.pos, c->f.pos, nf); d2d(c, S.vel, c->f.vel, nf);
-  d2d(c, S.kappa, c->f.kappa, nf); d2d(c, S.kappav, c->f.kappav, nf); d2d(c, S.id, c->f.id, nf);
-  d2d(c, S.bpos, c->b.pos, c->b.n); d2d(c, S.bvel, c->b.vel, c->b.n);
-  d2d(c, S.rig, c->d_rig, c->bodies.size());
+  Batch bt(c);
+  bt.copy(S.pos, c->f.pos, nf); bt.copy(S.vel, c->f.vel, nf);
+  bt.copy(S.kappa, c->f.kappa, nf); bt.copy(S.kappav, c->f.kappav, nf); bt.copy(S.id, c->f.id, nf);
+  bt.copy(S.bpos, c->b.pos, c->b.n); bt.copy(S.bvel, c->b.vel, c->b.n);
+  bt.copy(S.rig, c->d_rig, c->bodies.size());
+  bt.flush();
   S.time = c->time;
   S.step_count = c->step_count;
   S.animated.resize(c->bodies.size());
@@ -760,10 +878,13 @@ void snapshot_save(dfr_ctx* c, Snapshot& S) {
 void snapshot_restore(dfr_ctx* c, const Snapshot& S) {
   Timed t(c, TC_CHECKPOINT);
   const int nf = c->f.n;
-  d2d(c, c->f.pos, S.pos, nf); d2d(c, c->f.vel, S.vel, nf);
-  d2d(c, c->f.kappa, S.kappa, nf); d2d(c, c->f.kappav, S.kappav, nf); d2d(c, c->f.id, S.id, nf);
-  d2d(c, c->b.pos, S.bpos, c->b.n); d2d(c, c->b.vel, S.bvel, c->b.n);
-  d2d(c, c->d_rig, S.rig, c->bodies.size());
+  Batch bt(c);
+  bt.copy(c->f.pos, (const float4*)S.pos, nf); bt.copy(c->f.vel, (const float4*)S.vel, nf);
+  bt.copy(c->f.kappa, (const float*)S.kappa, nf); bt.copy(c->f.kappav, (const float*)S.kappav, nf);
+  bt.copy(c->f.id, (const uint32_t*)S.id, nf);
+  bt.copy(c->b.pos, (const float4*)S.bpos, c->b.n); bt.copy(c->b.vel, (const float4*)S.bvel, c->b.n);
+  bt.copy(c->d_rig, (const RigidDev*)S.rig, c->bodies.size());
+  bt.flush();
   c->time = S.time;
   c->step_count = S.step_count;
   for (size_t k = 0; k < c->bodies.size(); ++k) c->bodies[k].animated = S.animated[k] != 0;
@@ -912,22 +1033,26 @@ void reverse_step(dfr_ctx* c, const StepTapeDev& T) {
     rc.gyroscopic = c->P.gyroscopic;
     rc.nbodies = nbodies;
     launch_adj_rigid(rc, T.tbl, T.rig_forced, T.rig_pre, T.rig_post, A.body, nbodies, c->stream);
+    Batch bt(c);
     if (c->b.nd > 0) {
-      HIPCHK(hipMemsetAsync(A.sxb, 0, sizeof(double) * 3 * c->b.nd, c->stream));
-      HIPCHK(hipMemsetAsync(A.svb, 0, sizeof(double) * 3 * c->b.nd, c->stream));
+      bt.zero(A.sxb, 3 * (size_t)c->b.nd);
+      bt.zero(A.svb, 3 * (size_t)c->b.nd);
     }
+    if (nf > 0) {
+      bt.zero(A.alb, (size_t)nf);
+      bt.zero(A.rhob, (size_t)nf);
+      if (c->mlp.enabled) bt.zero(A.ybar, 2 * (size_t)nf);
+    }
+    bt.flush();
   }
   if (nf > 0) {
     {
       Timed t(c, TC_ADJ_MISC);
       launch_adj_position(nf, dt, A.xb, A.vb, c->stream);
-      HIPCHK(hipMemsetAsync(A.alb, 0, sizeof(float) * nf, c->stream));
-      HIPCHK(hipMemsetAsync(A.rhob, 0, sizeof(float) * nf, c->stream));
     }
     const bool warmP = c->P.use_pressure_warmstart != 0;
     const bool learned = c->mlp.enabled;
     const float h2 = dt * dt;
-    if (learned) HIPCHK(hipMemsetAsync(A.ybar, 0, sizeof(float) * 2 * nf, c->stream));
     for (int m = T.Ip - 1; m >= 0; --m) {
       {
         Timed t(c, TC_ADJ_UPDATE);
@@ -1166,6 +1291,9 @@ int dfr_destroy(dfr_ctx* c) {
   for (void* p : c->allocations) (void)hipFree(p);
   if (c->h_err) (void)hipHostFree(c->h_err);
   if (c->h_overflow) (void)hipHostFree(c->h_overflow);
+  if (c->h_probe) (void)hipHostFree(c->h_probe);
+  for (auto e : c->pipe_ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return DFR_OK;
@@ -1304,6 +1432,12 @@ int dfr_init(dfr_ctx* c) {
     HIPCHK(hipHostMalloc((void**)&c->h_err, sizeof(double) * c->err_slots, hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&c->h_overflow, sizeof(uint32_t) * 4, hipHostMallocDefault));
     c->h_overflow[0] = c->h_overflow[1] = c->h_overflow[2] = c->h_overflow[3] = 0;
+    c->d_stop = dalloc<int>(c, 2);
+    c->probe_cap = std::max(c->P.max_iterations, c->P.max_iterations_v) + 1;
+    HIPCHK(hipHostMalloc((void**)&c->h_probe, sizeof(SolveProbe) * 2 * c->probe_cap,
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer((void**)&c->d_probe, c->h_probe, 0));
+    for (auto& e : c->pipe_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // uploads
     HIPCHK(hipMemcpy(c->b.x0, x0.data(), sizeof(float4) * off, hipMemcpyHostToDevice));
     std::vector<uint32_t> dyn(std::max<int64_t>(nd, 1));

@@ -13,16 +13,33 @@
 
 namespace dfr {
 
+// Cell key and a slot within the cell.  The fluid arrives in the previous
+// step's cell order, so a wavefront's keys form runs of equal values; each run
+// takes its slots with one atomic (the leader's), the other lanes add their
+// rank in the run.  Slot order within a cell is then fixed by k_cell_sort.
 __global__ void k_keys(const float4* __restrict__ pos, int n, GridDesc g, uint32_t* __restrict__ count,
                        uint32_t* __restrict__ key, uint32_t* __restrict__ slot) {
   const uint32_t i = gid();
-  if (i >= (uint32_t)n) return;
-  const float4 p = pos[i];
-  int cx, cy, cz;
-  cell_coords(g, p.x, p.y, p.z, cx, cy, cz);
-  const uint32_t k = cell_key(g, cx, cy, cz);
-  key[i] = k;
-  slot[i] = atomicAdd(&count[k], 1u);
+  const bool valid = i < (uint32_t)n;
+  uint32_t k = 0xffffffffu;  // lanes past the end form their own run and take no slot
+  if (valid) {
+    const float4 p = pos[i];
+    int cx, cy, cz;
+    cell_coords(g, p.x, p.y, p.z, cx, cy, cz);
+    k = cell_key(g, cx, cy, cz);
+    key[i] = k;
+  }
+  const int lane = (int)(threadIdx.x & 63u);
+  const uint32_t kprev = __shfl_up(k, 1, 64);
+  const unsigned long long heads = __ballot(lane == 0 || kprev != k);
+  const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane
+  const int first = 63 - __clzll(heads & upto);
+  const unsigned long long after = heads & ~upto;
+  const int end = after ? __ffsll((long long)after) - 1 : 64;
+  uint32_t base = 0;
+  if (valid && lane == first) base = atomicAdd(&count[k], (uint32_t)(end - first));
+  base = __shfl(base, first, 64);
+  if (valid) slot[i] = base + (uint32_t)(lane - first);
 }
 
 __global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_t* __restrict__ slot,
@@ -32,14 +49,45 @@ __global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_
   perm[start[key[i]] + slot[i]] = i;
 }
 
-// Insertion sort of each cell's entries by an order key (particle id for the
-// fluid, sample index for the boundary).  Cells hold a handful of particles.
+// Sort of each cell's entries by an order key (particle id for the fluid,
+// sample index for the boundary; keys are unique, so the result does not
+// depend on the slot order k_keys produced).  Cells hold about ten particles:
+// up to 16 entries are loaded together and sorted in registers by a bitonic
+// network; larger cells use an insertion sort.
+constexpr int CELL_NET = 16;
 __global__ void k_cell_sort(uint32_t nkeys, const uint32_t* __restrict__ start, uint32_t* __restrict__ perm,
                             const uint32_t* __restrict__ order) {
   const uint32_t c = gid();
   if (c >= nkeys) return;
   const uint32_t s = start[c], e = start[c + 1];
-  if (e - s < 2) return;
+  const uint32_t m = e - s;
+  if (m < 2) return;
+  if (m <= (uint32_t)CELL_NET) {
+    uint32_t v[CELL_NET], kv[CELL_NET];
+#pragma unroll
+    for (int a = 0; a < CELL_NET; ++a) v[a] = ((uint32_t)a < m) ? perm[s + a] : 0u;
+#pragma unroll
+    for (int a = 0; a < CELL_NET; ++a) kv[a] = ((uint32_t)a < m) ? (order ? order[v[a]] : v[a]) : 0xffffffffu;
+#pragma unroll
+    for (int kk = 2; kk <= CELL_NET; kk <<= 1)
+#pragma unroll
+      for (int j = kk >> 1; j > 0; j >>= 1)
+#pragma unroll
+        for (int a = 0; a < CELL_NET; ++a) {
+          const int b = a ^ j;
+          if (b > a) {
+            const bool asc = (a & kk) == 0;
+            const bool sw = asc ? (kv[a] > kv[b]) : (kv[a] < kv[b]);
+            const uint32_t ka = kv[a], kb = kv[b], va = v[a], vb = v[b];
+            kv[a] = sw ? kb : ka; kv[b] = sw ? ka : kb;
+            v[a] = sw ? vb : va; v[b] = sw ? va : vb;
+          }
+        }
+#pragma unroll
+    for (int a = 0; a < CELL_NET; ++a)
+      if ((uint32_t)a < m) perm[s + a] = v[a];
+    return;
+  }
   for (uint32_t a = s + 1; a < e; ++a) {
     const uint32_t v = perm[a];
     const uint32_t kv = order ? order[v] : v;

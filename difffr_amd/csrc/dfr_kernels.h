@@ -79,13 +79,15 @@ void launch_build_tiled(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBu
 // aux (optional): per particle (velocity divergence, boundary density sum_b V_b W_ib)
 void launch_density_factor_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
                              NbrBufs& nb, int div_warm, int divergence_on, float2* aux, hipStream_t s);
+// stop (optional): device flag of a Jacobi solve; a set flag makes the launch a no-op
 void launch_velocity_update_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
-                              NbrBufs& nb, RigidDev* rig, const float* kfield, float* kappa_acc, hipStream_t s);
+                              NbrBufs& nb, RigidDev* rig, const float* kfield, float* kappa_acc, hipStream_t s,
+                              const int* stop = nullptr);
 // mode 0: divergence recompute, 1: pressure recompute, 2: pressure warm start,
 // 3: pressure warm start from predicted stiffness kpred (learned)
 void launch_recompute_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
                         NbrBufs& nb, int mode, float kcoef, double* err_slot, hipStream_t s,
-                        const float* kpred = nullptr);
+                        const float* kpred = nullptr, const int* stop = nullptr);
 void launch_expand_lists(const GridDesc& g, GridBufs& gb, FluidBufs& f, NbrBufs& nb, uint32_t* out, hipStream_t s);
 void set_tiled_kernel_attributes();
 
@@ -93,6 +95,30 @@ void set_tiled_kernel_attributes();
 enum RecomputeMode { RECOMPUTE_DIV = 0, RECOMPUTE_PRESSURE = 1, PRESSURE_WARMSTART = 2, PRESSURE_WARMSTART_LEARNED = 3 };
 void launch_advect(const StepConst& c, FluidBufs& f, float kappav_scale, hipStream_t s);
 void launch_position(const StepConst& c, FluidBufs& f, float kappa_scale, hipStream_t s);
+// Several device copies / zero fills in one launch (16-byte aligned pointers,
+// sizes a multiple of 4 bytes); src == nullptr fills dst with zeros.
+struct CopySeg {
+  const void* src;
+  void* dst;
+  uint64_t bytes;
+};
+constexpr int COPY_SEGS = 8;
+struct CopyList {
+  CopySeg seg[COPY_SEGS];
+  int n;
+};
+void launch_multi_copy(const CopyList& list, hipStream_t s);
+// Convergence test of one Jacobi iteration, on the device: probe gets the
+// average residual err / n and whether the solve ends after iteration it
+// (the loop condition of the reference); stop is set when it does.  A launch
+// after the solve ended changes nothing.
+struct SolveProbe {
+  double avg;
+  int done;  // 1: the solve ends after this iteration, 0: it continues
+  int pad;
+};
+void launch_solve_check(const double* err, int n, double eta, int it, int min_it, int max_it, int* stop,
+                        SolveProbe* probe, hipStream_t s);
 void launch_boundary_volume_impl(const float4* bpos_in, const float4* bx0, int nb, const int* groups, int nbodies,
                                  KernelConst K, float h2, const GridDesc& g, GridBufs& gb, float4* bpos_out,
                                  hipStream_t s);

@@ -1,5 +1,6 @@
 // Elementwise passes of the DFSPH step (advection, position update) and the
 // Akinci boundary volume.  The neighbour-gather passes live in dfr_tiled.hip.
+#include <algorithm>
 #include "dfr_kernels.h"
 
 namespace dfr {
@@ -73,8 +74,51 @@ __global__ void k_boundary_volume(const float4* __restrict__ bpos_in, const floa
   bpos_out[j] = o;
 }
 
+// One Jacobi iteration's convergence test (divergenceSolve :816-828,
+// pressureSolve :700-712): continue while (avg > eta or it + 1 < min) and
+// it + 1 < max.  probe is host-visible memory.
+__global__ void k_solve_check(const double* __restrict__ err, int n, double eta, int it, int min_it, int max_it,
+                              int* __restrict__ stop, SolveProbe* probe) {
+  if (threadIdx.x != 0 || *stop) return;
+  const double avg = *err / (double)n;
+  const bool chk = avg <= eta;
+  const int done = ((chk && it + 1 >= min_it) || it + 1 >= max_it) ? 1 : 0;
+  *stop = done;
+  probe->avg = avg;
+  probe->done = done;
+  __threadfence_system();
+}
+
+// One segment per blockIdx.y: 16-byte words, then the 4-byte tail.
+__global__ void k_multi_copy(CopyList L) {
+  const CopySeg sg = L.seg[blockIdx.y];
+  const uint64_t n16 = sg.bytes >> 4;
+  uint4* d = static_cast<uint4*>(sg.dst);
+  const uint4* s = static_cast<const uint4*>(sg.src);
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += (uint64_t)gridDim.x * blockDim.x)
+    d[k] = s ? s[k] : make_uint4(0u, 0u, 0u, 0u);
+  const uint32_t tail = (uint32_t)((sg.bytes & 15u) >> 2);
+  if (blockIdx.x == 0 && threadIdx.x < tail) {
+    uint32_t* dw = static_cast<uint32_t*>(sg.dst) + n16 * 4;
+    dw[threadIdx.x] = s ? static_cast<const uint32_t*>(sg.src)[n16 * 4 + threadIdx.x] : 0u;
+  }
+}
+
 // ---------------------------------------------------------------------------
 static inline unsigned nblk(size_t n) { return (unsigned)((n + DFR_BLOCK - 1) / DFR_BLOCK); }
+
+void launch_multi_copy(const CopyList& list, hipStream_t s) {
+  if (list.n <= 0) return;
+  uint64_t most = 0;
+  for (int k = 0; k < list.n; ++k) most = std::max<uint64_t>(most, list.seg[k].bytes >> 4);
+  const unsigned bx = (unsigned)std::min<uint64_t>(std::max<uint64_t>((most + DFR_BLOCK - 1) / DFR_BLOCK, 1), 2048);
+  hipLaunchKernelGGL(k_multi_copy, dim3(bx, (unsigned)list.n), dim3(DFR_BLOCK), 0, s, list);
+}
+
+void launch_solve_check(const double* err, int n, double eta, int it, int min_it, int max_it, int* stop,
+                        SolveProbe* probe, hipStream_t s) {
+  hipLaunchKernelGGL(k_solve_check, dim3(1), dim3(64), 0, s, err, n, eta, it, min_it, max_it, stop, probe);
+}
 
 void launch_advect(const StepConst& c, FluidBufs& f, float kappav_scale, hipStream_t s) {
   hipLaunchKernelGGL(k_advect, dim3(nblk(c.nf)), dim3(DFR_BLOCK), 0, s, c, f.vel, f.kappav, kappav_scale);

@@ -286,10 +286,11 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
 // LDS per slot: x, y, z, k.
 __global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(6, 8)))
 k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restrict__ kappa_acc,
-                    RigidDev* __restrict__ rig) {
+                    RigidDev* __restrict__ rig, const int* __restrict__ stop) {
   __shared__ TileHdr<PB> T;
   extern __shared__ float lds[];
   const StepConst& c = a.c;
+  if (stop && *stop) return;
   const uint32_t brick = xcd_block();
   if (brick_empty(a.fstart, brick)) return;
   tile_setup<PB>(a.g, a.fstart, brick, T);
@@ -404,10 +405,11 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
 __global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(6, 8)))
 k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, const float* __restrict__ alpha,
               float* __restrict__ badv, float* __restrict__ kf, float* __restrict__ kappa, double* __restrict__ err,
-              const float* __restrict__ kpred) {
+              const float* __restrict__ kpred, const int* __restrict__ stop) {
   __shared__ TileHdr<PB> T;
   extern __shared__ float lds[];
   const StepConst& c = a.c;
+  if (stop && *stop) return;
   const uint32_t brick = xcd_block();
   if (brick_empty(a.fstart, brick)) return;
   tile_setup<PB>(a.g, a.fstart, brick, T);
@@ -535,17 +537,19 @@ void launch_density_factor_t(const StepConst& c, const GridDesc& g, GridBufs& gb
 }
 
 void launch_velocity_update_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
-                              NbrBufs& nb, RigidDev* rig, const float* kfield, float* kappa_acc, hipStream_t s) {
+                              NbrBufs& nb, RigidDev* rig, const float* kfield, float* kappa_acc, hipStream_t s,
+                              const int* stop) {
   const size_t shm = sizeof(float) * (size_t)nb.cap * 4;
   hipLaunchKernelGGL(k_velocity_update_t, dim3(nbricks(g)), dim3(PB), shm, s, make_args(c, g, gb, f, b, nb), kfield,
-                     kappa_acc, rig);
+                     kappa_acc, rig, stop);
 }
 
 void launch_recompute_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
-                        NbrBufs& nb, int mode, float kcoef, double* err_slot, hipStream_t s, const float* kpred) {
+                        NbrBufs& nb, int mode, float kcoef, double* err_slot, hipStream_t s, const float* kpred,
+                        const int* stop) {
   const size_t shm = sizeof(float) * (size_t)nb.cap * 6;
   hipLaunchKernelGGL(k_recompute_t, dim3(nbricks(g)), dim3(PB), shm, s, make_args(c, g, gb, f, b, nb), mode, kcoef,
-                     f.rho, f.alpha, f.badv, f.kf, f.kappa, err_slot, kpred);
+                     f.rho, f.alpha, f.badv, f.kf, f.kappa, err_slot, kpred, stop);
 }
 
 void launch_expand_lists(const GridDesc& g, GridBufs& gb, FluidBufs& f, NbrBufs& nb, uint32_t* out, hipStream_t s) {

@@ -219,6 +219,7 @@ struct dfr_ctx {
   dfr_stats stats{};
 
   bool profiling = false;
+  uint32_t prof_mask = 0;  // timer classes recorded while profiling
   struct Rec { int cls; hipEvent_t a, b; };
   std::vector<Rec> pending;
   std::vector<hipEvent_t> event_pool;
@@ -285,7 +286,7 @@ hipEvent_t take_event(dfr_ctx* c) {
 struct Timed {
   dfr_ctx* c; int cls; hipEvent_t a = nullptr;
   Timed(dfr_ctx* c_, int cls_) : c(c_), cls(cls_) {
-    if (c->profiling) { a = take_event(c); HIPCHK(hipEventRecord(a, c->stream)); }
+    if (c->profiling && ((c->prof_mask >> cls) & 1u)) { a = take_event(c); HIPCHK(hipEventRecord(a, c->stream)); }
   }
   ~Timed() {
     if (c->profiling && a) {
@@ -1737,6 +1738,14 @@ int64_t dfr_get_neighbors(const dfr_ctx* c, int kind, uint32_t* counts, uint32_t
 int dfr_set_profiling(dfr_ctx* c, int on) {
   if (!c) return DFR_ERR_ARG;
   c->profiling = on != 0;
+  c->prof_mask = on ? 0xffffffffu : 0u;
+  return DFR_OK;
+}
+
+int dfr_set_profiling_classes(dfr_ctx* c, uint32_t mask) {
+  if (!c) return DFR_ERR_ARG;
+  c->prof_mask = mask;
+  c->profiling = mask != 0;
   return DFR_OK;
 }
 

@@ -25,6 +25,7 @@ EXPORTS = [
     "dfr_set_fixed_iterations", "dfr_mlp_capture", "dfr_mlp_dataset_size", "dfr_mlp_dataset", "dfr_mlp_init_params",
     "dfr_mlp_set_params", "dfr_mlp_get_params", "dfr_mlp_train", "dfr_mlp_enable", "dfr_mlp_predict",
     "dfr_mlp_loss_grad", "dfr_mlp_input_grad", "dfr_set_halo_capacity", "dfr_halo_stats",
+    "dfr_set_profiling_classes",
 ]
 
 MLP_NP = 8898
@@ -93,6 +94,7 @@ def load(path: str | None = None):
     L.dfr_reset_stats.argtypes = [P]
     L.dfr_neighbor_stats.argtypes = [P, dp]
     L.dfr_set_halo_capacity.argtypes = [P, C.c_int]
+    L.dfr_set_profiling_classes.argtypes = [P, C.c_uint32]
     L.dfr_halo_stats.argtypes = [P, C.POINTER(C.c_int64)]
     L.dfr_get_neighbors.argtypes = [P, C.c_int, up, up, C.c_int64]
     L.dfr_get_neighbors.restype = C.c_int64

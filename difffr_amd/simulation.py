@@ -261,8 +261,16 @@ class Simulation:
         off = np.concatenate([[0], np.cumsum(counts[:r].astype(np.int64))])
         return [lists[off[k]:off[k + 1]].copy() for k in range(int(r))]
 
-    def set_profiling(self, on: bool):
-        self._check(native.load().dfr_set_profiling(self.ctx, int(on)))
+    def set_profiling(self, on: bool, classes=None):
+        """Kernel-class timing with HIP events; classes: names to time (None: all)."""
+        if classes is None:
+            self._check(native.load().dfr_set_profiling(self.ctx, int(on)))
+            return
+        names = list(self.kernel_times())
+        mask = 0
+        for cl in classes:
+            mask |= 1 << names.index(cl)
+        self._check(native.load().dfr_set_profiling_classes(self.ctx, mask if on else 0))
 
     def kernel_times(self) -> dict:
         n = 32

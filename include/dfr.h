@@ -174,6 +174,10 @@ int64_t dfr_get_neighbors(const dfr_ctx* ctx, int kind, uint32_t* counts, uint32
  * (0 = off).  dfr_kernel_times fills names/avg-ms/launch-count for up to n
  * classes and returns the number of classes. */
 int dfr_set_profiling(dfr_ctx* ctx, int on);
+/* Only the classes whose bit is set in mask (bit k = k-th class of
+ * dfr_kernel_times), so that timing one kernel class adds no events around
+ * the others; mask 0 = off. */
+int dfr_set_profiling_classes(dfr_ctx* ctx, uint32_t mask);
 int dfr_kernel_times(const dfr_ctx* ctx, int n, char names[][32], double* avg_ms, int64_t* launches);
 
 /* Exact adjoint of a rollout (no counterpart in the reference, whose only

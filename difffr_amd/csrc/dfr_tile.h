@@ -110,20 +110,27 @@ struct PassArgs {
   int cap;  // halo slots staged in LDS (larger halos read global memory)
 };
 
-// Visits the nF fluid neighbour slots of particle i in list order.  All packs
-// are loaded up front (independent 16-byte loads).
+// Visits the nF fluid neighbour slots of particle i in list order: a loop over
+// the packs with the eight slots of a pack unrolled, the pack two ahead in
+// flight (a compact loop body keeps the instruction stream small).
 template <class Body>
 __device__ inline void for_fluid_slots(const uint4* __restrict__ nbr8, size_t stride, uint32_t i, uint32_t nF,
                                        Body&& body) {
-  uint4 pk[NPACK];
-#pragma unroll
-  for (int p = 0; p < NPACK; ++p)
-    pk[p] = ((uint32_t)(p * PACK) < nF) ? nbr8[(size_t)p * stride + i] : make_uint4(0u, 0u, 0u, 0u);
-#pragma unroll
-  for (int p = 0; p < NPACK; ++p) {
+  // rows past the last used one are read as that row again (in bounds, in
+  // cache) and never used
+  const uint32_t np = (nF + PACK - 1) / PACK;
+  const uint32_t last = np > 0 ? np - 1 : 0;
+  uint4 cur = nbr8[i];
+  uint4 nxt = nbr8[(size_t)min(1u, last) * stride + i];
+#pragma unroll 1
+  for (uint32_t p = 0; p < np; ++p) {
+    const uint4 pk = cur;
+    cur = nxt;
+    nxt = nbr8[(size_t)min(p + 2, last) * stride + i];
+    const uint32_t base = p * PACK;
 #pragma unroll
     for (int u = 0; u < PACK; ++u)
-      if ((uint32_t)(p * PACK + u) < nF) body(pack_get(pk[p], u));
+      if (base + (uint32_t)u < nF) body(pack_get(pk, u));
   }
 }
 

@@ -22,30 +22,40 @@ namespace dfr {
 // ---------------------------------------------------------------------------
 constexpr int BB = DFR_BLOCK;  // build workgroup
 
-// Fluid neighbour slots of one particle as they are found: a two-pack buffer in
-// registers (16 slots; a chunk of candidates adds at most 4 to fewer than 8),
-// flushed to the pack-major list whenever a pack is full.
+// Fluid neighbour slots of one particle as they are found: a 16-entry ring per
+// thread in LDS (entry e of thread t at ring[e * BB + t], so a wavefront's
+// writes are consecutive), one 16-bit write per accepted slot; whenever eight
+// are pending (a chunk of candidates adds at most 4, so at most 11 are) they
+// leave as one pack of the pack-major list.
+constexpr int RING = 16;
 struct PackWriter {
-  uint32_t w[6] = {0u, 0u, 0u, 0u, 0u, 0u};  // slots 0..11 of the buffer, two per word
-  uint32_t nb = 0;                           // slots in the buffer
-  uint32_t np = 0;                           // packs written
+  uint16_t* ring;  // this block's ring
+  int t;
+  uint32_t n = 0;   // slots accepted
+  uint32_t np = 0;  // packs written
   __device__ inline void push(uint32_t s) {
-    const uint32_t word = nb >> 1, v = s << ((nb & 1u) << 4);
+    ring[(n & (RING - 1)) * BB + t] = (uint16_t)s;
+    ++n;
+  }
+  __device__ inline void store(uint4* __restrict__ nbr8, size_t stride, uint32_t i, uint32_t count) {
+    uint32_t w[4];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) w[k] |= (word == (uint32_t)k) ? v : 0u;
-    ++nb;
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t e0 = np * PACK + 2 * k, e1 = e0 + 1;
+      const uint32_t lo = (2 * k < (int)count) ? ring[(e0 & (RING - 1)) * BB + t] : 0u;
+      const uint32_t hi = (2 * k + 1 < (int)count) ? ring[(e1 & (RING - 1)) * BB + t] : 0u;
+      w[k] = lo | (hi << 16);
+    }
+    if (np < (uint32_t)NPACK) nbr8[(size_t)np * stride + i] = make_uint4(w[0], w[1], w[2], w[3]);
+    ++np;
   }
   __device__ inline void flush_full(uint4* __restrict__ nbr8, size_t stride, uint32_t i) {
-    if (nb >= (uint32_t)PACK) {
-      if (np < (uint32_t)NPACK) nbr8[(size_t)np * stride + i] = make_uint4(w[0], w[1], w[2], w[3]);
-      ++np;
-      nb -= PACK;
-      w[0] = w[4]; w[1] = w[5]; w[2] = 0u; w[3] = 0u; w[4] = 0u; w[5] = 0u;
-    }
+    if (n - np * PACK >= (uint32_t)PACK) store(nbr8, stride, i, PACK);
   }
   __device__ inline uint32_t finish(uint4* __restrict__ nbr8, size_t stride, uint32_t i) {
-    if (nb > 0u && np < (uint32_t)NPACK) nbr8[(size_t)np * stride + i] = make_uint4(w[0], w[1], w[2], w[3]);
-    return np * PACK + nb;
+    const uint32_t rest = n - np * PACK;
+    if (rest > 0u) store(nbr8, stride, i, rest);
+    return n;
   }
 };
 
@@ -57,6 +67,7 @@ k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t
               uint32_t* __restrict__ nbrB, uint32_t* __restrict__ cnt, uint32_t* __restrict__ overflow) {
   __shared__ TileHdr<BB> T;
   __shared__ uint32_t bcs[HALO], bcn[HALO];  // boundary sample range of each halo cell
+  __shared__ uint16_t ring[RING * BB];
   extern __shared__ float lds[];
   const uint32_t brick = xcd_block();
   if (brick_empty(fstart, brick)) return;
@@ -106,7 +117,7 @@ k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t
     cell_coords(g, p.x, p.y, p.z, cx, cy, cz);
     const int hc0 = halo_cell<BB>(T, cx, cy, cz, 0, 0, 0);
     const uint32_t self = T.coff[hc0] + (i - T.cstart[hc0]);
-    PackWriter pw;
+    PackWriter pw{ring, t};
     // the three x-neighbour cells of a (dz, dy) row are consecutive halo
     // cells, so their candidates form one contiguous slot range
     for (int dz = -1; dz <= 1; ++dz)

@@ -40,9 +40,15 @@ __device__ inline void cell_coords(const GridDesc& g, float x, float y, float z,
 
 // Squared distance with each product and sum rounded separately (no fused
 // multiply-add), so the neighbour predicate matches the CPU oracle bit for bit.
+__device__ inline float mul_unfused(float a, float b) {
+  // an opaque product: the compiler cannot contract it with the following add
+  float r;
+  asm("v_mul_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
 __device__ inline float dist2_exact(float ax, float ay, float az, float bx, float by, float bz) {
-  const float dx = __fsub_rn(ax, bx), dy = __fsub_rn(ay, by), dz = __fsub_rn(az, bz);
-  return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+  const float dx = ax - bx, dy = ay - by, dz = az - bz;
+  return (mul_unfused(dx, dx) + mul_unfused(dy, dy)) + mul_unfused(dz, dz);
 }
 
 // ---------------------------------------------------------------------------

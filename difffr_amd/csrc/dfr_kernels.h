@@ -14,7 +14,7 @@ constexpr int TILE_CAP = 2560;
 constexpr int MAXF = 64;
 constexpr int PACK = 8;               // 16-bit slots per 16-byte pack
 constexpr int NPACK = MAXF / PACK;    // packs per particle
-constexpr int PASS_BLOCK = 512;       // threads of a gather-pass workgroup (one brick)
+constexpr int PASS_BLOCK = 640;       // threads of a gather-pass workgroup (one brick: ~640 particles at rest density)
 
 // Fluid state in HBM, structure-of-arrays, permuted into cell order every step.
 struct FluidBufs {

@@ -20,7 +20,7 @@ namespace dfr {
 // ---------------------------------------------------------------------------
 // Neighbour build (CompactNSearch::find_neighbors replacement).
 // ---------------------------------------------------------------------------
-constexpr int BB = DFR_BLOCK;  // build workgroup
+constexpr int BB = PASS_BLOCK;  // build workgroup: one particle of the brick per thread
 
 // Fluid neighbour slots of one particle as they are found: a 16-entry ring per
 // thread in LDS (entry e of thread t at ring[e * BB + t], so a wavefront's
@@ -178,6 +178,10 @@ k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t
 // ---------------------------------------------------------------------------
 constexpr int PB = PASS_BLOCK;
 constexpr int PT = per_thread<PB>();
+// The velocity update stages 4 floats per slot, so LDS would fit four 640-thread
+// workgroups but the CU holds at most 32 wavefronts: 512 threads keep four.
+constexpr int VB = 512;
+constexpr int VT = per_thread<VB>();
 
 // TimeStep::computeDensities (TimeStep.cpp:147-202) + computeDFSPHFactor
 // (TimeStepDiffDFSPH.cpp:883-962) + the first loop of warmstartDivergenceSolve
@@ -295,16 +299,16 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
 // -dt k_i grad_p_ib and receives f = -m_i dv / dt (BoundaryModel::addForce,
 // BoundaryModel.h:46-58), summed here in FP64 per dynamic body.
 // LDS per slot: x, y, z, k.
-__global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(6, 8)))
+__global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(6, 8)))
 k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restrict__ kappa_acc,
                     RigidDev* __restrict__ rig, const int* __restrict__ stop) {
-  __shared__ TileHdr<PB> T;
+  __shared__ TileHdr<VB> T;
   extern __shared__ float lds[];
   const StepConst& c = a.c;
   if (stop && *stop) return;
   const uint32_t brick = xcd_block();
   if (brick_empty(a.fstart, brick)) return;
-  tile_setup<PB>(a.g, a.fstart, brick, T);
+  tile_setup<VB>(a.g, a.fstart, brick, T);
   if (T.own_begin == T.own_end) return;
   const uint32_t total = T.total;
   const int cap = a.cap;
@@ -312,16 +316,16 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
   float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap; float* lk = lds + 3 * cap;
   const int t = threadIdx.x;
   if (in_lds) {
-    uint32_t js[PT];
-    tile_indices<PB>(T, total, js);
-    float4 q[PT];
-    float kk[PT];
+    uint32_t js[VT];
+    tile_indices<VB>(T, total, js);
+    float4 q[VT];
+    float kk[VT];
 #pragma unroll
-    for (int r = 0; r < PT; ++r)
-      if (t + r * PB < (int)total) { q[r] = a.pos[js[r]]; kk[r] = kfield[js[r]]; }
+    for (int r = 0; r < VT; ++r)
+      if (t + r * VB < (int)total) { q[r] = a.pos[js[r]]; kk[r] = kfield[js[r]]; }
 #pragma unroll
-    for (int r = 0; r < PT; ++r) {
-      const uint32_t s = t + r * PB;
+    for (int r = 0; r < VT; ++r) {
+      const uint32_t s = t + r * VB;
       if (s < total) { lx[s] = q[r].x; ly[s] = q[r].y; lz[s] = q[r].z; lk[s] = kk[r]; }
     }
   }
@@ -330,7 +334,7 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
   int cb = -1;
   double F0 = 0, F1 = 0, F2 = 0, T0 = 0, T1 = 0, T2 = 0;
   const float invH = 1.f / c.dt;
-  for (uint32_t i = T.own_begin + t; i < T.own_end; i += PB) {
+  for (uint32_t i = T.own_begin + t; i < T.own_end; i += VB) {
     const float ki = kfield[i];
     if (kappa_acc) kappa_acc[i] += ki;
     const float4 p = a.pos[i];
@@ -351,7 +355,7 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
       for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) { pair(lx[s], ly[s], lz[s], lk[s]); });
     } else {
       for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {
-        const uint32_t j = tile_global<PB>(T, s);
+        const uint32_t j = tile_global<VB>(T, s);
         const float4 q = a.pos[j];
         pair(q.x, q.y, q.z, kfield[j]);
       });
@@ -551,7 +555,7 @@ void launch_velocity_update_t(const StepConst& c, const GridDesc& g, GridBufs& g
                               NbrBufs& nb, RigidDev* rig, const float* kfield, float* kappa_acc, hipStream_t s,
                               const int* stop) {
   const size_t shm = sizeof(float) * (size_t)nb.cap * 4;
-  hipLaunchKernelGGL(k_velocity_update_t, dim3(nbricks(g)), dim3(PB), shm, s, make_args(c, g, gb, f, b, nb), kfield,
+  hipLaunchKernelGGL(k_velocity_update_t, dim3(nbricks(g)), dim3(VB), shm, s, make_args(c, g, gb, f, b, nb), kfield,
                      kappa_acc, rig, stop);
 }
 

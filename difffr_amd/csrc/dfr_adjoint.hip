@@ -76,17 +76,17 @@ k_adj_update_t(AdjPass a, const float* __restrict__ k, const float4* __restrict_
   extern __shared__ float lds[];
   const StepConst& c = a.c;
   const uint32_t brick = xcd_block();
-  if (brick_empty(a.fstart, brick)) return;
-  tile_setup<RB>(a.g, a.fstart, brick, T);
-  const uint32_t total = T.total;
   const int cap = a.cap;
-  const bool in_lds = total <= (uint32_t)cap;
+  BrickView B;
+  if (!brick_open<RB>(a.g, a.fstart, a.thdr, brick, cap, T, B)) return;
+  const uint32_t total = B.total;
+  const bool in_lds = B.in_lds;
   float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
   float* lbx = lds + 3 * cap; float* lby = lds + 4 * cap; float* lbz = lds + 5 * cap; float* lk = lds + 6 * cap;
   const int t = threadIdx.x;
   if (in_lds) {
     uint32_t js[PTR];
-    tile_indices<RB>(T, total, js);
+    map_indices<RB>(a.tmap, brick, total, js);
     float4 q[PTR], w[PTR];
     float kk[PTR];
 #pragma unroll
@@ -103,7 +103,7 @@ k_adj_update_t(AdjPass a, const float* __restrict__ k, const float4* __restrict_
   }
   __syncthreads();
   const size_t stride = (size_t)c.nf;
-  for (uint32_t i = T.own_begin + t; i < T.own_end; i += RB) {
+  for (uint32_t i = B.own_begin + t; i < B.own_end; i += RB) {
     const float ki = k[i];
     const float4 p = a.x[i];
     const float4 vb = vbar[i];
@@ -221,17 +221,17 @@ k_adj_delta_t(AdjPass a, const float4* __restrict__ v, const float* __restrict__
   extern __shared__ float lds[];
   const StepConst& c = a.c;
   const uint32_t brick = xcd_block();
-  if (brick_empty(a.fstart, brick)) return;
-  tile_setup<RB>(a.g, a.fstart, brick, T);
-  const uint32_t total = T.total;
   const int cap = a.cap;
-  const bool in_lds = total <= (uint32_t)cap;
+  BrickView B;
+  if (!brick_open<RB>(a.g, a.fstart, a.thdr, brick, cap, T, B)) return;
+  const uint32_t total = B.total;
+  const bool in_lds = B.in_lds;
   float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
   float* lvx = lds + 3 * cap; float* lvy = lds + 4 * cap; float* lvz = lds + 5 * cap; float* ld = lds + 6 * cap;
   const int t = threadIdx.x;
   if (in_lds) {
     uint32_t js[PTR];
-    tile_indices<RB>(T, total, js);
+    map_indices<RB>(a.tmap, brick, total, js);
     float4 q[PTR], w[PTR];
     float dd[PTR];
 #pragma unroll
@@ -248,7 +248,7 @@ k_adj_delta_t(AdjPass a, const float4* __restrict__ v, const float* __restrict__
   }
   __syncthreads();
   const size_t stride = (size_t)c.nf;
-  for (uint32_t i = T.own_begin + t; i < T.own_end; i += RB) {
+  for (uint32_t i = B.own_begin + t; i < B.own_end; i += RB) {
     const float di = dbar[i];
     const float4 p = a.x[i];
     const float4 vi = v[i];
@@ -327,11 +327,11 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
   extern __shared__ float lds[];
   const StepConst& c = a.c;
   const uint32_t brick = xcd_block();
-  if (brick_empty(a.fstart, brick)) return;
-  tile_setup<RB>(a.g, a.fstart, brick, T);
-  const uint32_t total = T.total;
   const int cap = a.cap;
-  const bool in_lds = total <= (uint32_t)cap;
+  BrickView B;
+  if (!brick_open<RB>(a.g, a.fstart, a.thdr, brick, cap, T, B)) return;
+  const uint32_t total = B.total;
+  const bool in_lds = B.in_lds;
   float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
   float* lS = lds + 3 * cap; float* lr = lds + 4 * cap;
   float* lgx = lds + 5 * cap; float* lgy = lds + 6 * cap; float* lgz = lds + 7 * cap;
@@ -339,7 +339,7 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
   auto sbar = [&](uint32_t j) { const float al = alpha[j]; return alpha_bar[j] * al * al; };
   if (in_lds) {
     uint32_t js[PTR];
-    tile_indices<RB>(T, total, js);
+    map_indices<RB>(a.tmap, brick, total, js);
     float4 q[PTR], g[PTR];
     float sb[PTR], rb[PTR];
 #pragma unroll
@@ -356,7 +356,7 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
   }
   __syncthreads();
   const size_t stride = (size_t)c.nf;
-  for (uint32_t i = T.own_begin + t; i < T.own_end; i += RB) {
+  for (uint32_t i = B.own_begin + t; i < B.own_end; i += RB) {
     const float Si = sbar(i), ri = rho_bar[i];
     const float4 p = a.x[i];
     const float4 Gi = G[i];

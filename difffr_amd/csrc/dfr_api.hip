@@ -112,6 +112,8 @@ struct StepTapeDev {
   // structures of the step
   const uint32_t *fstart = nullptr, *perm = nullptr, *nbrB = nullptr, *cnt = nullptr;
   const uint4* nbr8 = nullptr;
+  const uint4* thdr = nullptr;            // brick headers and halo slot maps
+  const uint32_t* tmap = nullptr;
   const float *rho = nullptr, *alpha = nullptr;
   const float4* s = nullptr;
   const float2* aux = nullptr;
@@ -154,7 +156,7 @@ struct AdjState {
   // kept tapes: the steps write their structures straight into arena buffers;
   // the context's own buffers are swapped back in when the rollout ends
   struct Live {
-    uint4* nbr8; uint32_t *nbrB, *cnt, *fstart, *perm;
+    uint4* nbr8; uint4* thdr; uint32_t *tmap, *nbrB, *cnt, *fstart, *perm;
     float *rho, *alpha;
     float4* s;
     float2* aux;
@@ -932,7 +934,7 @@ void tape_alloc_fixed(dfr_ctx* c, StepTapeDev& T) {
 // The reverse pass right after a replayed step reads the live structures.
 void live_view(dfr_ctx* c, StepTapeDev& T) {
   T.fstart = c->gb.fstart; T.perm = c->gb.perm;
-  T.nbr8 = c->nb.nbr8; T.nbrB = c->nb.nbrB; T.cnt = c->nb.cnt;
+  T.nbr8 = c->nb.nbr8; T.thdr = c->nb.thdr; T.tmap = c->nb.tmap; T.nbrB = c->nb.nbrB; T.cnt = c->nb.cnt;
   T.rho = c->f.rho; T.alpha = c->f.alpha; T.s = c->f.s; T.aux = c->mlp.aux;
   T.rig_post = c->d_rig;
 }
@@ -951,10 +953,14 @@ void tape_swap_in(dfr_ctx* c, StepTapeDev& T) {
   Arena& R = A.arena;
   const size_t NF = (size_t)std::max(c->f.n, 1);
   if (!A.swapped) {
-    A.own = {c->nb.nbr8, c->nb.nbrB, c->nb.cnt, c->gb.fstart, c->gb.perm, c->f.rho, c->f.alpha, c->f.s, c->mlp.aux};
+    A.own = {c->nb.nbr8, c->nb.thdr, c->nb.tmap, c->nb.nbrB, c->nb.cnt, c->gb.fstart, c->gb.perm, c->f.rho,
+             c->f.alpha, c->f.s, c->mlp.aux};
     A.swapped = true;
   }
   c->nb.nbr8 = static_cast<uint4*>(R.alloc(sizeof(uint4) * NPACK * NF));
+  const size_t NB = (size_t)c->grid.bn[0] * c->grid.bn[1] * c->grid.bn[2];
+  c->nb.thdr = static_cast<uint4*>(R.alloc(sizeof(uint4) * NB));
+  c->nb.tmap = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * NB * TILE_CAP));
   c->nb.nbrB = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * (size_t)c->nb.maxb * NF));
   c->nb.cnt = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * NF));
   c->gb.fstart = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * ((size_t)c->grid.nkeys + 1)));
@@ -963,7 +969,8 @@ void tape_swap_in(dfr_ctx* c, StepTapeDev& T) {
   c->f.alpha = static_cast<float*>(R.alloc(sizeof(float) * NF));
   c->f.s = static_cast<float4*>(R.alloc(sizeof(float4) * NF));
   if (c->mlp.aux) c->mlp.aux = static_cast<float2*>(R.alloc(sizeof(float2) * NF));
-  T.nbr8 = c->nb.nbr8; T.nbrB = c->nb.nbrB; T.cnt = c->nb.cnt; T.fstart = c->gb.fstart; T.perm = c->gb.perm;
+  T.nbr8 = c->nb.nbr8; T.thdr = c->nb.thdr; T.tmap = c->nb.tmap; T.nbrB = c->nb.nbrB; T.cnt = c->nb.cnt;
+  T.fstart = c->gb.fstart; T.perm = c->gb.perm;
   T.rho = c->f.rho; T.alpha = c->f.alpha; T.s = c->f.s; T.aux = c->mlp.aux;
 }
 
@@ -974,6 +981,11 @@ void tape_swap_out(dfr_ctx* c) {
   const int nf = c->f.n;
   const auto& o = A.own;
   d2d(c, o.nbr8, (const uint4*)c->nb.nbr8, (size_t)NPACK * nf);
+  {
+    const size_t NB = (size_t)c->grid.bn[0] * c->grid.bn[1] * c->grid.bn[2];
+    d2d(c, o.thdr, (const uint4*)c->nb.thdr, NB);
+    d2d(c, o.tmap, (const uint32_t*)c->nb.tmap, NB * TILE_CAP);
+  }
   d2d(c, o.nbrB, (const uint32_t*)c->nb.nbrB, (size_t)c->nb.maxb * nf);
   d2d(c, o.cnt, (const uint32_t*)c->nb.cnt, (size_t)nf);
   d2d(c, o.fstart, (const uint32_t*)c->gb.fstart, (size_t)c->grid.nkeys + 1);
@@ -982,7 +994,8 @@ void tape_swap_out(dfr_ctx* c) {
   d2d(c, o.alpha, (const float*)c->f.alpha, (size_t)nf);
   d2d(c, o.s, (const float4*)c->f.s, (size_t)nf);
   if (o.aux) d2d(c, o.aux, (const float2*)c->mlp.aux, (size_t)nf);
-  c->nb.nbr8 = o.nbr8; c->nb.nbrB = o.nbrB; c->nb.cnt = o.cnt; c->gb.fstart = o.fstart; c->gb.perm = o.perm;
+  c->nb.nbr8 = o.nbr8; c->nb.thdr = o.thdr; c->nb.tmap = o.tmap; c->nb.nbrB = o.nbrB; c->nb.cnt = o.cnt;
+  c->gb.fstart = o.fstart; c->gb.perm = o.perm;
   c->f.rho = o.rho; c->f.alpha = o.alpha; c->f.s = o.s; c->mlp.aux = o.aux;
   A.swapped = false;
 }
@@ -1023,7 +1036,7 @@ void reverse_step(dfr_ctx* c, const StepTapeDev& T) {
   const int nbodies = (int)c->bodies.size();
   AdjPass a{};
   a.c = c->sc; a.g = c->grid; a.fstart = T.fstart; a.x = T.x;
-  a.nbr8 = T.nbr8; a.nbrB = T.nbrB; a.cnt = T.cnt;
+  a.nbr8 = T.nbr8; a.thdr = T.thdr; a.tmap = T.tmap; a.nbrB = T.nbrB; a.cnt = T.cnt;
   a.bx = T.bpos; a.bv = T.bvel; a.bx0 = c->b.x0; a.nb_static = c->b.n - c->b.nd;
   a.cap = c->nb.cap;
   a.sxb = A.sxb; a.svb = A.svb; a.body = A.body;
@@ -1419,6 +1432,11 @@ int dfr_init(dfr_ctx* c) {
     c->nb.maxb = c->P.max_neighbors;
     c->nb.maxbf = c->P.max_boundary_neighbors;
     c->nb.nbr8 = dalloc<uint4>(c, (size_t)NPACK * NF);
+    {
+      const size_t NB = (size_t)c->grid.bn[0] * c->grid.bn[1] * c->grid.bn[2];
+      c->nb.thdr = dalloc<uint4>(c, NB);
+      c->nb.tmap = dalloc<uint32_t>(c, NB * TILE_CAP);
+    }
     c->nb.nbrB = dalloc<uint32_t>(c, (size_t)c->nb.maxb * NF);
     c->nb.cnt = dalloc<uint32_t>(c, NF);
     c->nb.bf = dalloc<uint32_t>(c, (size_t)c->nb.maxbf * std::max<int64_t>(nd, 1));

@@ -56,6 +56,8 @@ struct GridBufs {
 struct NbrBufs {
   int maxb = 0, maxbf = 0;
   uint4* nbr8 = nullptr;      // [NPACK][nf] fluid neighbours as tile slots, 8 per pack
+  uint4* thdr = nullptr;      // [nbricks] own_begin, own_end, halo total of each brick (written by the build)
+  uint32_t* tmap = nullptr;   // [nbricks][TILE_CAP] sorted particle index of each halo slot
   uint32_t* nbrB = nullptr;   // [maxb][nf] boundary neighbours (sample index)
   uint32_t* cnt = nullptr;    // [nf]  nF | nB << 16
   uint32_t* bf = nullptr;     // [maxbf][nd] fluid neighbours of dynamic samples (sorted index)
@@ -176,6 +178,8 @@ struct AdjPass {
   const uint32_t* fstart;
   const float4* x;     // fluid positions during the step (sorted order)
   const uint4* nbr8;
+  const uint4* thdr;   // brick headers and halo slot maps of the step
+  const uint32_t* tmap;
   const uint32_t* nbrB;
   const uint32_t* cnt;
   const float4* bx;    // boundary positions at the start of the step, w = volume

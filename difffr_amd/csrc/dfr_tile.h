@@ -104,11 +104,42 @@ struct PassArgs {
   const float4* bvel;
   const float4* bx0;
   const uint4* nbr8;
+  const uint4* thdr;      // brick headers and halo slot maps written by the build
+  const uint32_t* tmap;
   const uint32_t* nbrB;
   const uint32_t* cnt;
   int nb_static;
   int cap;  // halo slots staged in LDS (larger halos read global memory)
 };
+
+// A pass's view of its brick.  The build leaves, per brick, a header (own
+// particle range, halo total) and the halo's slot -> particle map, so a pass
+// starts with one header load and coalesced map loads; only a halo beyond the
+// LDS capacity sets up the cell table for the binary-search fallback.
+struct BrickView {
+  uint32_t own_begin, own_end, total;
+  bool in_lds;
+};
+template <int NT>
+__device__ inline bool brick_open(const GridDesc& g, const uint32_t* __restrict__ fstart, const uint4* __restrict__ thdr,
+                                  uint32_t brick, int cap, TileHdr<NT>& T, BrickView& B) {
+  const uint4 h = thdr[brick];
+  if (h.x == h.y) return false;  // no own particle (uniform exit)
+  B.own_begin = h.x;
+  B.own_end = h.y;
+  B.total = h.z;
+  B.in_lds = h.z <= (uint32_t)cap;
+  if (!B.in_lds) tile_setup<NT>(g, fstart, brick, T);
+  return true;
+}
+template <int NT, int CAP = TILE_CAP>
+__device__ inline void map_indices(const uint32_t* __restrict__ tmap, uint32_t brick, uint32_t total, uint32_t* js) {
+#pragma unroll
+  for (int r = 0; r < per_thread<NT, CAP>(); ++r) {
+    const uint32_t s = threadIdx.x + r * NT;
+    js[r] = (s < total) ? tmap[(size_t)brick * TILE_CAP + s] : 0u;
+  }
+}
 
 // Visits the nF fluid neighbour slots of particle i in list order: a loop over
 // the packs with the eight slots of a pack unrolled, the pack two ahead in

@@ -64,14 +64,19 @@ __global__ void __launch_bounds__(BB)
 k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t* __restrict__ fstart,
               const float4* __restrict__ bpos, const uint32_t* __restrict__ bstart,
               const uint32_t* __restrict__ bperm, float h2, int maxb, int cap, uint4* __restrict__ nbr8,
-              uint32_t* __restrict__ nbrB, uint32_t* __restrict__ cnt, uint32_t* __restrict__ overflow) {
+              uint32_t* __restrict__ nbrB, uint32_t* __restrict__ cnt, uint32_t* __restrict__ overflow,
+              uint4* __restrict__ thdr, uint32_t* __restrict__ tmap) {
   __shared__ TileHdr<BB> T;
   __shared__ uint32_t bcs[HALO], bcn[HALO];  // boundary sample range of each halo cell
   __shared__ uint16_t ring[RING * BB];
   extern __shared__ float lds[];
   const uint32_t brick = xcd_block();
-  if (brick_empty(fstart, brick)) return;
+  if (brick_empty(fstart, brick)) {
+    if (threadIdx.x == 0) thdr[brick] = make_uint4(0u, 0u, 0u, 0u);
+    return;
+  }
   tile_setup<BB>(g, fstart, brick, T);
+  if (threadIdx.x == 0) thdr[brick] = make_uint4(T.own_begin, T.own_end, T.total, 0u);
   if (T.own_begin == T.own_end) return;
   const uint32_t total = T.total;
   const int t = threadIdx.x;
@@ -95,10 +100,19 @@ k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t
   }
   const bool in_lds = total <= (uint32_t)cap;
   float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
+  // the halo's slot -> particle map, for every slot a pass could stage
+  constexpr int PTB = per_thread<BB>();
+  uint32_t js[PTB];
+  {
+    const uint32_t mapped = min(total, (uint32_t)TILE_CAP);
+    tile_indices<BB>(T, mapped, js);
+#pragma unroll
+    for (int r = 0; r < PTB; ++r) {
+      const uint32_t s = t + r * BB;
+      if (s < mapped) tmap[(size_t)brick * TILE_CAP + s] = js[r];
+    }
+  }
   if (in_lds) {
-    constexpr int PTB = per_thread<BB>();
-    uint32_t js[PTB];
-    tile_indices<BB>(T, total, js);
     float4 q[PTB];
 #pragma unroll
     for (int r = 0; r < PTB; ++r)
@@ -194,18 +208,17 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
   extern __shared__ float lds[];
   const StepConst& c = a.c;
   const uint32_t brick = xcd_block();
-  if (brick_empty(a.fstart, brick)) return;
-  tile_setup<PB>(a.g, a.fstart, brick, T);
-  if (T.own_begin == T.own_end) return;
-  const uint32_t total = T.total;
   const int cap = a.cap;
-  const bool in_lds = total <= (uint32_t)cap;
+  BrickView B;
+  if (!brick_open<PB>(a.g, a.fstart, a.thdr, brick, cap, T, B)) return;
+  const uint32_t total = B.total;
+  const bool in_lds = B.in_lds;
   float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
   float* lvx = lds + 3 * cap; float* lvy = lds + 4 * cap; float* lvz = lds + 5 * cap;
   const int t = threadIdx.x;
   if (in_lds) {
     uint32_t js[PT];
-    tile_indices<PB>(T, total, js);
+    map_indices<PB>(a.tmap, brick, total, js);
     float4 q[PT], v[PT];
 #pragma unroll
     for (int r = 0; r < PT; ++r)
@@ -224,7 +237,7 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
   }
   __syncthreads();
   const size_t stride = (size_t)c.nf;
-  for (uint32_t i = T.own_begin + t; i < T.own_end; i += PB) {
+  for (uint32_t i = B.own_begin + t; i < B.own_end; i += PB) {
     const float4 p = a.pos[i];
     float4 vi = make_float4(0.f, 0.f, 0.f, 0.f);
     if (divergence_on) vi = a.vel[i];
@@ -307,17 +320,16 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
   const StepConst& c = a.c;
   if (stop && *stop) return;
   const uint32_t brick = xcd_block();
-  if (brick_empty(a.fstart, brick)) return;
-  tile_setup<VB>(a.g, a.fstart, brick, T);
-  if (T.own_begin == T.own_end) return;
-  const uint32_t total = T.total;
   const int cap = a.cap;
-  const bool in_lds = total <= (uint32_t)cap;
+  BrickView B;
+  if (!brick_open<VB>(a.g, a.fstart, a.thdr, brick, cap, T, B)) return;
+  const uint32_t total = B.total;
+  const bool in_lds = B.in_lds;
   float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap; float* lk = lds + 3 * cap;
   const int t = threadIdx.x;
   if (in_lds) {
     uint32_t js[VT];
-    tile_indices<VB>(T, total, js);
+    map_indices<VB>(a.tmap, brick, total, js);
     float4 q[VT];
     float kk[VT];
 #pragma unroll
@@ -334,7 +346,7 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
   int cb = -1;
   double F0 = 0, F1 = 0, F2 = 0, T0 = 0, T1 = 0, T2 = 0;
   const float invH = 1.f / c.dt;
-  for (uint32_t i = T.own_begin + t; i < T.own_end; i += VB) {
+  for (uint32_t i = B.own_begin + t; i < B.own_end; i += VB) {
     const float ki = kfield[i];
     if (kappa_acc) kappa_acc[i] += ki;
     const float4 p = a.pos[i];
@@ -426,18 +438,17 @@ k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, 
   const StepConst& c = a.c;
   if (stop && *stop) return;
   const uint32_t brick = xcd_block();
-  if (brick_empty(a.fstart, brick)) return;
-  tile_setup<PB>(a.g, a.fstart, brick, T);
-  if (T.own_begin == T.own_end) return;
-  const uint32_t total = T.total;
   const int cap = a.cap;
-  const bool in_lds = total <= (uint32_t)cap;
+  BrickView B;
+  if (!brick_open<PB>(a.g, a.fstart, a.thdr, brick, cap, T, B)) return;
+  const uint32_t total = B.total;
+  const bool in_lds = B.in_lds;
   float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
   float* lvx = lds + 3 * cap; float* lvy = lds + 4 * cap; float* lvz = lds + 5 * cap;
   const int t = threadIdx.x;
   if (in_lds) {
     uint32_t js[PT];
-    tile_indices<PB>(T, total, js);
+    map_indices<PB>(a.tmap, brick, total, js);
     float4 q[PT], v[PT];
 #pragma unroll
     for (int r = 0; r < PT; ++r)
@@ -454,7 +465,7 @@ k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, 
   __syncthreads();
   const size_t stride = (size_t)c.nf;
   double e = 0.0;
-  for (uint32_t i = T.own_begin + t; i < T.own_end; i += PB) {
+  for (uint32_t i = B.own_begin + t; i < B.own_end; i += PB) {
     const float4 p = a.pos[i];
     const float4 vi = a.vel[i];
     const uint32_t cc = a.cnt[i];
@@ -533,6 +544,7 @@ static PassArgs make_args(const StepConst& c, const GridDesc& g, GridBufs& gb, F
   PassArgs a;
   a.c = c; a.g = g; a.fstart = gb.fstart; a.pos = f.pos; a.vel = f.vel; a.bpos = b.pos; a.bvel = b.vel;
   a.bx0 = b.x0; a.nbr8 = nb.nbr8; a.nbrB = nb.nbrB; a.cnt = nb.cnt; a.nb_static = b.n - b.nd;
+  a.thdr = nb.thdr; a.tmap = nb.tmap;
   a.cap = nb.cap;
   return a;
 }
@@ -541,7 +553,7 @@ void launch_build_tiled(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBu
                         hipStream_t s) {
   const size_t shm = sizeof(float) * (size_t)nb.cap * 3;
   hipLaunchKernelGGL(k_build_tiled, dim3(nbricks(g)), dim3(BB), shm, s, f.pos, f.n, g, gb.fstart, b.pos, gb.bstart,
-                     gb.bperm, h2, nb.maxb, nb.cap, nb.nbr8, nb.nbrB, nb.cnt, nb.overflow);
+                     gb.bperm, h2, nb.maxb, nb.cap, nb.nbr8, nb.nbrB, nb.cnt, nb.overflow, nb.thdr, nb.tmap);
 }
 
 void launch_density_factor_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,

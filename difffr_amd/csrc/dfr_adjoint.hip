@@ -73,7 +73,7 @@ k_adj_update_t(AdjPass a, const float* __restrict__ k, const float4* __restrict_
                const float* __restrict__ kprev, const float* __restrict__ alpha, float* out,
                float* __restrict__ alpha_bar, float* __restrict__ rho_bar, float4* __restrict__ xbar) {
   __shared__ TileHdr<RB> T;
-  extern __shared__ float lds[];
+  extern __shared__ float4 lds4[];
   const StepConst& c = a.c;
   const uint32_t brick = xcd_block();
   const int cap = a.cap;
@@ -81,8 +81,8 @@ k_adj_update_t(AdjPass a, const float* __restrict__ k, const float4* __restrict_
   if (!brick_open<RB>(a.g, a.fstart, a.thdr, brick, cap, T, B)) return;
   const uint32_t total = B.total;
   const bool in_lds = B.in_lds;
-  float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
-  float* lbx = lds + 3 * cap; float* lby = lds + 4 * cap; float* lbz = lds + 5 * cap; float* lk = lds + 6 * cap;
+  float4* LA = lds4;        // x, y, z, k
+  float4* LB = lds4 + cap;  // vbar
   const int t = threadIdx.x;
   if (in_lds) {
     uint32_t js[PTR];
@@ -96,8 +96,8 @@ k_adj_update_t(AdjPass a, const float* __restrict__ k, const float4* __restrict_
     for (int r = 0; r < PTR; ++r) {
       const uint32_t s = t + r * RB;
       if (s < total) {
-        lx[s] = q[r].x; ly[s] = q[r].y; lz[s] = q[r].z;
-        lbx[s] = w[r].x; lby[s] = w[r].y; lbz[s] = w[r].z; lk[s] = kk[r];
+        LA[s] = make_float4(q[r].x, q[r].y, q[r].z, kk[r]);
+        LB[s] = make_float4(w[r].x, w[r].y, w[r].z, w[r].w);
       }
     }
   }
@@ -111,9 +111,10 @@ k_adj_update_t(AdjPass a, const float* __restrict__ k, const float4* __restrict_
     const uint32_t nF = cc & 0xffffu, nB = cc >> 16;
     // fluid pairs, without the common factor dt V_f
     float kbf = 0.f, xfx = 0.f, xfy = 0.f, xfz = 0.f;
-    auto pair = [&](float qx, float qy, float qz, float wx, float wy, float wz, float kj) {
+    // m = 0 for pairs with |k_i + k_j| <= eps and for slots that are not live
+    auto pair = [&](float qx, float qy, float qz, float wx, float wy, float wz, float kj, bool live) {
       const float kSum = ki + kj;
-      const float m = (fabsf(kSum) > c.eps) ? 1.f : 0.f;
+      const float m = (live && fabsf(kSum) > c.eps) ? 1.f : 0.f;
       const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
       const PairCE ce = cubic_ce(c.K, rx * rx + ry * ry + rz * rz);
       const float ux = vb.x - wx, uy = vb.y - wy, uz = vb.z - wz;
@@ -126,15 +127,16 @@ k_adj_update_t(AdjPass a, const float* __restrict__ k, const float4* __restrict_
       xfz += sk * (ce.c * uz + er * rz);
     };
     if (in_lds) {
-      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) {
-        pair(lx[s], ly[s], lz[s], lbx[s], lby[s], lbz[s], lk[s]);
-      });
+      for_fluid_groups<4>(a.nbr8, stride, i, nF, [&](uint32_t s) { return Slot8{LA[s], LB[s]}; },
+                          [&](const Slot8& q, bool live) {
+                            pair(q.a.x, q.a.y, q.a.z, q.b.x, q.b.y, q.b.z, q.a.w, live);
+                          });
     } else {
       for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {
         const uint32_t j = tile_global<RB>(T, s);
         const float4 q = a.x[j];
         const float4 w = vbar[j];
-        pair(q.x, q.y, q.z, w.x, w.y, w.z, k[j]);
+        pair(q.x, q.y, q.z, w.x, w.y, w.z, k[j], true);
       });
     }
     const float dtVf = c.dt * c.V_f;
@@ -218,7 +220,7 @@ __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(6, 8)))
 k_adj_delta_t(AdjPass a, const float4* __restrict__ v, const float* __restrict__ dbar, float4* __restrict__ vbar,
               float4* __restrict__ xbar) {
   __shared__ TileHdr<RB> T;
-  extern __shared__ float lds[];
+  extern __shared__ float4 lds4[];
   const StepConst& c = a.c;
   const uint32_t brick = xcd_block();
   const int cap = a.cap;
@@ -226,8 +228,8 @@ k_adj_delta_t(AdjPass a, const float4* __restrict__ v, const float* __restrict__
   if (!brick_open<RB>(a.g, a.fstart, a.thdr, brick, cap, T, B)) return;
   const uint32_t total = B.total;
   const bool in_lds = B.in_lds;
-  float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
-  float* lvx = lds + 3 * cap; float* lvy = lds + 4 * cap; float* lvz = lds + 5 * cap; float* ld = lds + 6 * cap;
+  float4* LA = lds4;        // x, y, z, dbar
+  float4* LB = lds4 + cap;  // v
   const int t = threadIdx.x;
   if (in_lds) {
     uint32_t js[PTR];
@@ -241,8 +243,8 @@ k_adj_delta_t(AdjPass a, const float4* __restrict__ v, const float* __restrict__
     for (int r = 0; r < PTR; ++r) {
       const uint32_t s = t + r * RB;
       if (s < total) {
-        lx[s] = q[r].x; ly[s] = q[r].y; lz[s] = q[r].z;
-        lvx[s] = w[r].x; lvy[s] = w[r].y; lvz[s] = w[r].z; ld[s] = dd[r];
+        LA[s] = make_float4(q[r].x, q[r].y, q[r].z, dd[r]);
+        LB[s] = make_float4(w[r].x, w[r].y, w[r].z, w[r].w);
       }
     }
   }
@@ -255,8 +257,9 @@ k_adj_delta_t(AdjPass a, const float4* __restrict__ v, const float* __restrict__
     const uint32_t cc = a.cnt[i];
     const uint32_t nF = cc & 0xffffu, nB = cc >> 16;
     float gx = 0.f, gy = 0.f, gz = 0.f, hx = 0.f, hy = 0.f, hz = 0.f;  // without the factor V_f
-    auto pair = [&](float qx, float qy, float qz, float wx, float wy, float wz, float dj) {
-      const float s = di + dj;
+    // slots that are not live get the weight s = 0
+    auto pair = [&](float qx, float qy, float qz, float wx, float wy, float wz, float dj, bool live) {
+      const float s = live ? di + dj : 0.f;
       const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
       const PairCE ce = cubic_ce(c.K, rx * rx + ry * ry + rz * rz);
       const float sc = s * ce.c;
@@ -266,15 +269,16 @@ k_adj_delta_t(AdjPass a, const float4* __restrict__ v, const float* __restrict__
       hx += sc * ux + se * rx; hy += sc * uy + se * ry; hz += sc * uz + se * rz;
     };
     if (in_lds) {
-      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) {
-        pair(lx[s], ly[s], lz[s], lvx[s], lvy[s], lvz[s], ld[s]);
-      });
+      for_fluid_groups<4>(a.nbr8, stride, i, nF, [&](uint32_t s) { return Slot8{LA[s], LB[s]}; },
+                          [&](const Slot8& q, bool live) {
+                            pair(q.a.x, q.a.y, q.a.z, q.b.x, q.b.y, q.b.z, q.a.w, live);
+                          });
     } else {
       for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {
         const uint32_t j = tile_global<RB>(T, s);
         const float4 q = a.x[j];
         const float4 w = v[j];
-        pair(q.x, q.y, q.z, w.x, w.y, w.z, dbar[j]);
+        pair(q.x, q.y, q.z, w.x, w.y, w.z, dbar[j], true);
       });
     }
     float vbx = c.V_f * gx, vby = c.V_f * gy, vbz = c.V_f * gz;
@@ -324,7 +328,7 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
                const float* __restrict__ rho_bar, const float4* __restrict__ G, float4* __restrict__ xbar,
                const float* __restrict__ rbb) {
   __shared__ TileHdr<RB> T;
-  extern __shared__ float lds[];
+  extern __shared__ float4 lds4[];
   const StepConst& c = a.c;
   const uint32_t brick = xcd_block();
   const int cap = a.cap;
@@ -332,9 +336,8 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
   if (!brick_open<RB>(a.g, a.fstart, a.thdr, brick, cap, T, B)) return;
   const uint32_t total = B.total;
   const bool in_lds = B.in_lds;
-  float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
-  float* lS = lds + 3 * cap; float* lr = lds + 4 * cap;
-  float* lgx = lds + 5 * cap; float* lgy = lds + 6 * cap; float* lgz = lds + 7 * cap;
+  float4* LA = lds4;        // x, y, z, Sb
+  float4* LB = lds4 + cap;  // rb, G
   const int t = threadIdx.x;
   auto sbar = [&](uint32_t j) { const float al = alpha[j]; return alpha_bar[j] * al * al; };
   if (in_lds) {
@@ -349,8 +352,8 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
     for (int r = 0; r < PTR; ++r) {
       const uint32_t s = t + r * RB;
       if (s < total) {
-        lx[s] = q[r].x; ly[s] = q[r].y; lz[s] = q[r].z;
-        lS[s] = sb[r]; lr[s] = rb[r]; lgx[s] = g[r].x; lgy[s] = g[r].y; lgz[s] = g[r].z;
+        LA[s] = make_float4(q[r].x, q[r].y, q[r].z, sb[r]);
+        LB[s] = make_float4(rb[r], g[r].x, g[r].y, g[r].z);
       }
     }
   }
@@ -364,9 +367,12 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
     const uint32_t nF = cc & 0xffffu, nB = cc >> 16;
     float dx = 0.f, dy = 0.f, dz = 0.f;  // density part, without V_f
     float fx = 0.f, fy = 0.f, fz = 0.f;  // factor part, without 2 V_f
-    auto pair = [&](float qx, float qy, float qz, float Sj, float rj, float Gx, float Gy, float Gz) {
+    // slots that are not live get c = e = 0 (every term carries one of them)
+    auto pair = [&](float qx, float qy, float qz, float Sj, float rj, float Gx, float Gy, float Gz, bool live) {
       const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
-      const PairCE ce = cubic_ce(c.K, rx * rx + ry * ry + rz * rz);
+      PairCE ce = cubic_ce(c.K, rx * rx + ry * ry + rz * rz);
+      ce.c = live ? ce.c : 0.f;
+      ce.e = live ? ce.e : 0.f;
       const float sr = (ri + rj) * ce.c;
       dx += sr * rx; dy += sr * ry; dz += sr * rz;
       // u = (Sb_i + Sb_j) V_f gradW - Sb_i G_i + Sb_j G_j
@@ -377,15 +383,16 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
       fx += ce.c * ux + er * rx; fy += ce.c * uy + er * ry; fz += ce.c * uz + er * rz;
     };
     if (in_lds) {
-      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) {
-        pair(lx[s], ly[s], lz[s], lS[s], lr[s], lgx[s], lgy[s], lgz[s]);
-      });
+      for_fluid_groups<4>(a.nbr8, stride, i, nF, [&](uint32_t s) { return Slot8{LA[s], LB[s]}; },
+                          [&](const Slot8& q, bool live) {
+                            pair(q.a.x, q.a.y, q.a.z, q.a.w, q.b.x, q.b.y, q.b.z, q.b.w, live);
+                          });
     } else {
       for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {
         const uint32_t j = tile_global<RB>(T, s);
         const float4 q = a.x[j];
         const float4 g = G[j];
-        pair(q.x, q.y, q.z, sbar(j), rho_bar[j], g.x, g.y, g.z);
+        pair(q.x, q.y, q.z, sbar(j), rho_bar[j], g.x, g.y, g.z, true);
       });
     }
     const float tv = 2.f * c.V_f;
@@ -615,44 +622,44 @@ __global__ void k_adj_rigid(RigidStepConst rc, AdjStepTable tb, const RigidDev* 
 // ---------------------------------------------------------------------------
 static inline unsigned nbricks(const GridDesc& g) { return (unsigned)(g.bn[0] * g.bn[1] * g.bn[2]); }
 static inline unsigned nblk(size_t n) { return (unsigned)((n + DFR_BLOCK - 1) / DFR_BLOCK); }
-// dynamic LDS of the reverse passes: 7 or 8 floats per staged halo slot
+// dynamic LDS of the reverse passes: two 16-byte records per staged halo slot
 static inline size_t shm_slots(const AdjPass& a, int floats) { return sizeof(float) * (size_t)a.cap * floats; }
 
 void launch_adj_update(const AdjPass& a, int mode, const float* k, const float4* vbar, const float* kacc,
                        float kacc_scale, const float* bval, const float* kprev, const float* alpha, float* out,
                        float* alpha_bar, float* rho_bar, float4* xbar, hipStream_t s) {
   const dim3 grid(nbricks(a.g)), blk(RB);
-  const size_t SHM7 = shm_slots(a, 7);
+  const size_t SHM8 = shm_slots(a, 8);
   switch (mode) {
     case ADJ_DIV_ITER:
-      hipLaunchKernelGGL(k_adj_update_t<ADJ_DIV_ITER>, grid, blk, SHM7, s, a, k, vbar, kacc, kacc_scale, bval, kprev,
+      hipLaunchKernelGGL(k_adj_update_t<ADJ_DIV_ITER>, grid, blk, SHM8, s, a, k, vbar, kacc, kacc_scale, bval, kprev,
                          alpha, out, alpha_bar, rho_bar, xbar);
       break;
     case ADJ_PRESSURE_ITER:
-      hipLaunchKernelGGL(k_adj_update_t<ADJ_PRESSURE_ITER>, grid, blk, SHM7, s, a, k, vbar, kacc, kacc_scale, bval,
+      hipLaunchKernelGGL(k_adj_update_t<ADJ_PRESSURE_ITER>, grid, blk, SHM8, s, a, k, vbar, kacc, kacc_scale, bval,
                          kprev, alpha, out, alpha_bar, rho_bar, xbar);
       break;
     case ADJ_DIV_WARM:
-      hipLaunchKernelGGL(k_adj_update_t<ADJ_DIV_WARM>, grid, blk, SHM7, s, a, k, vbar, kacc, kacc_scale, bval, kprev,
+      hipLaunchKernelGGL(k_adj_update_t<ADJ_DIV_WARM>, grid, blk, SHM8, s, a, k, vbar, kacc, kacc_scale, bval, kprev,
                          alpha, out, alpha_bar, rho_bar, xbar);
       break;
     case ADJ_PRESSURE_WARM:
-      hipLaunchKernelGGL(k_adj_update_t<ADJ_PRESSURE_WARM>, grid, blk, SHM7, s, a, k, vbar, kacc, kacc_scale, bval,
+      hipLaunchKernelGGL(k_adj_update_t<ADJ_PRESSURE_WARM>, grid, blk, SHM8, s, a, k, vbar, kacc, kacc_scale, bval,
                          kprev, alpha, out, alpha_bar, rho_bar, xbar);
       break;
     case ADJ_DIV_WARM_LEARNED:
-      hipLaunchKernelGGL(k_adj_update_t<ADJ_DIV_WARM_LEARNED>, grid, blk, SHM7, s, a, k, vbar, kacc, kacc_scale, bval,
+      hipLaunchKernelGGL(k_adj_update_t<ADJ_DIV_WARM_LEARNED>, grid, blk, SHM8, s, a, k, vbar, kacc, kacc_scale, bval,
                          kprev, alpha, out, alpha_bar, rho_bar, xbar);
       break;
     default:
-      hipLaunchKernelGGL(k_adj_update_t<ADJ_PRESSURE_WARM_LEARNED>, grid, blk, SHM7, s, a, k, vbar, kacc, kacc_scale,
+      hipLaunchKernelGGL(k_adj_update_t<ADJ_PRESSURE_WARM_LEARNED>, grid, blk, SHM8, s, a, k, vbar, kacc, kacc_scale,
                          bval, kprev, alpha, out, alpha_bar, rho_bar, xbar);
   }
 }
 
 void launch_adj_delta(const AdjPass& a, const float4* v, const float* dbar, float4* vbar, float4* xbar,
                       hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_delta_t, dim3(nbricks(a.g)), dim3(RB), shm_slots(a, 7), s, a, v, dbar, vbar, xbar);
+  hipLaunchKernelGGL(k_adj_delta_t, dim3(nbricks(a.g)), dim3(RB), shm_slots(a, 8), s, a, v, dbar, vbar, xbar);
 }
 
 void launch_adj_factor(const AdjPass& a, const float* alpha, const float* alpha_bar, const float* rho_bar,
@@ -692,18 +699,18 @@ void launch_adj_rigid(const RigidStepConst& rc, const AdjStepTable& t, const Rig
 }
 
 void set_adjoint_kernel_attributes() {
-  const int s7 = (int)(sizeof(float) * TILE_CAP * 7), s8 = (int)(sizeof(float) * TILE_CAP * 8);
-  (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_DIV_ITER>, hipFuncAttributeMaxDynamicSharedMemorySize, s7);
+  const int s8 = (int)(sizeof(float) * TILE_CAP * 8);
+  (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_DIV_ITER>, hipFuncAttributeMaxDynamicSharedMemorySize, s8);
   (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_PRESSURE_ITER>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            s7);
-  (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_DIV_WARM>, hipFuncAttributeMaxDynamicSharedMemorySize, s7);
+                            s8);
+  (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_DIV_WARM>, hipFuncAttributeMaxDynamicSharedMemorySize, s8);
   (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_PRESSURE_WARM>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            s7);
+                            s8);
   (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_DIV_WARM_LEARNED>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, s7);
+                            hipFuncAttributeMaxDynamicSharedMemorySize, s8);
   (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_PRESSURE_WARM_LEARNED>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, s7);
-  (void)hipFuncSetAttribute((const void*)k_adj_delta_t, hipFuncAttributeMaxDynamicSharedMemorySize, s7);
+                            hipFuncAttributeMaxDynamicSharedMemorySize, s8);
+  (void)hipFuncSetAttribute((const void*)k_adj_delta_t, hipFuncAttributeMaxDynamicSharedMemorySize, s8);
   (void)hipFuncSetAttribute((const void*)k_adj_factor_t, hipFuncAttributeMaxDynamicSharedMemorySize, s8);
 }
 

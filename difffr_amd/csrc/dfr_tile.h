@@ -165,6 +165,49 @@ __device__ inline void for_fluid_slots(const uint4* __restrict__ nbr8, size_t st
   }
 }
 
+// Branch-free visit of the same slots for LDS-staged halos.  Within a pack the
+// slots go in groups of G: `load(s)` of every slot of a group is issued before
+// the first `body`, so the group's LDS reads are in flight together, then
+// `body(v, live)` runs in slot order.  `live` is false for the slots past nF
+// in the last pack; those hold slot 0 (a staged slot, so the read stays in
+// bounds), and the body must select their contribution to zero.  A group
+// wholly past nF is skipped.
+template <int G, class Load, class Body>
+__device__ inline void for_fluid_groups(const uint4* __restrict__ nbr8, size_t stride, uint32_t i, uint32_t nF,
+                                        Load&& load, Body&& body) {
+  static_assert(PACK % G == 0, "the group size divides the pack");
+  const uint32_t np = (nF + PACK - 1) / PACK;
+  const uint32_t last = np > 0 ? np - 1 : 0;
+  uint4 cur = nbr8[i];
+  uint4 nxt = nbr8[(size_t)min(1u, last) * stride + i];
+#pragma unroll 1
+  for (uint32_t p = 0; p < np; ++p) {
+    const uint4 pk = cur;
+    cur = nxt;
+    nxt = nbr8[(size_t)min(p + 2, last) * stride + i];
+    const uint32_t rem = nF - p * PACK;  // >= 1
+#pragma unroll
+    for (int g0 = 0; g0 < PACK; g0 += G) {
+      if (g0 > 0 && rem <= (uint32_t)g0) break;
+      decltype(load(0u)) v[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) v[u] = load(pack_get(pk, g0 + u));
+#pragma unroll
+      for (int u = 0; u < G; ++u) body(v[u], (uint32_t)(g0 + u) < rem);
+    }
+  }
+}
+
+// Halo slot records in LDS: the fields a pass gathers per neighbour,
+// interleaved so one slot is one or two 16-byte reads.
+struct Slot6 {
+  float4 a;
+  float2 b;
+};
+struct Slot8 {
+  float4 a, b;
+};
+
 // Same visit order without unrolling, for the global-memory fallback (a halo
 // beyond the LDS capacity): keeps the rare path out of the unrolled hot loop.
 template <class Body>

@@ -200,12 +200,12 @@ constexpr int VT = per_thread<VB>();
 // TimeStep::computeDensities (TimeStep.cpp:147-202) + computeDFSPHFactor
 // (TimeStepDiffDFSPH.cpp:883-962) + the first loop of warmstartDivergenceSolve
 // (:1720-1732), one pass.
-__global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(6, 8)))
+__global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alpha, float4* __restrict__ s_out,
                    float* __restrict__ badv, float* __restrict__ kappav, int div_warm, int divergence_on,
                    float2* __restrict__ aux) {
   __shared__ TileHdr<PB> T;
-  extern __shared__ float lds[];
+  extern __shared__ float4 lds4[];
   const StepConst& c = a.c;
   const uint32_t brick = xcd_block();
   const int cap = a.cap;
@@ -213,8 +213,8 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
   if (!brick_open<PB>(a.g, a.fstart, a.thdr, brick, cap, T, B)) return;
   const uint32_t total = B.total;
   const bool in_lds = B.in_lds;
-  float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
-  float* lvx = lds + 3 * cap; float* lvy = lds + 4 * cap; float* lvz = lds + 5 * cap;
+  float4* LA = lds4;                                          // x, y, z, vx
+  float2* LB = reinterpret_cast<float2*>(lds4 + cap);         // vy, vz
   const int t = threadIdx.x;
   if (in_lds) {
     uint32_t js[PT];
@@ -224,14 +224,14 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
     for (int r = 0; r < PT; ++r)
       if (t + r * PB < (int)total) {
         q[r] = a.pos[js[r]];
-        if (divergence_on) v[r] = a.vel[js[r]];
+        v[r] = divergence_on ? a.vel[js[r]] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
 #pragma unroll
     for (int r = 0; r < PT; ++r) {
       const uint32_t s = t + r * PB;
       if (s < total) {
-        lx[s] = q[r].x; ly[s] = q[r].y; lz[s] = q[r].z;
-        if (divergence_on) { lvx[s] = v[r].x; lvy[s] = v[r].y; lvz[s] = v[r].z; }
+        LA[s] = make_float4(q[r].x, q[r].y, q[r].z, v[r].x);
+        LB[s] = make_float2(v[r].y, v[r].z);
       }
     }
   }
@@ -247,26 +247,28 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
     float sumsq = 0.f, gx = 0.f, gy = 0.f, gz = 0.f, delta = 0.f;
     // sums over fluid neighbours without the common factor V_f (applied once)
     float wsum = 0.f, sq = 0.f, fgx = 0.f, fgy = 0.f, fgz = 0.f, fdelta = 0.f;
-    auto pair = [&](float qx, float qy, float qz, float vx, float vy, float vz) {
+    // a slot that is not live contributes W = 0 and gradW = 0 (exact no-ops)
+    auto pair = [&](float qx, float qy, float qz, float vx, float vy, float vz, bool live) {
       const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
       const PairKernel kw = cubic_pair(c.K, rx * rx + ry * ry + rz * rz);
-      wsum += kw.W;
-      const float wx = kw.c * rx, wy = kw.c * ry, wz = kw.c * rz;
+      const float W = live ? kw.W : 0.f, kc = live ? kw.c : 0.f;
+      wsum += W;
+      const float wx = kc * rx, wy = kc * ry, wz = kc * rz;
       sq += wx * wx + wy * wy + wz * wz;
       fgx += wx; fgy += wy; fgz += wz;
       if (divergence_on) fdelta += (vi.x - vx) * wx + (vi.y - vy) * wy + (vi.z - vz) * wz;
     };
     if (in_lds) {
-      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) {
-        if (divergence_on) pair(lx[s], ly[s], lz[s], lvx[s], lvy[s], lvz[s]);
-        else pair(lx[s], ly[s], lz[s], 0.f, 0.f, 0.f);
-      });
+      for_fluid_groups<4>(a.nbr8, stride, i, nF, [&](uint32_t s) { return Slot6{LA[s], LB[s]}; },
+                          [&](const Slot6& q, bool live) {
+                            pair(q.a.x, q.a.y, q.a.z, q.a.w, q.b.x, q.b.y, live);
+                          });
     } else {
       for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {
         const uint32_t j = tile_global<PB>(T, s);
         const float4 q = a.pos[j];
         const float4 v = divergence_on ? a.vel[j] : make_float4(0.f, 0.f, 0.f, 0.f);
-        pair(q.x, q.y, q.z, v.x, v.y, v.z);
+        pair(q.x, q.y, q.z, v.x, v.y, v.z, true);
       });
     }
     dens += c.V_f * wsum;
@@ -312,11 +314,11 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
 // -dt k_i grad_p_ib and receives f = -m_i dv / dt (BoundaryModel::addForce,
 // BoundaryModel.h:46-58), summed here in FP64 per dynamic body.
 // LDS per slot: x, y, z, k.
-__global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(6, 8)))
+__global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restrict__ kappa_acc,
                     RigidDev* __restrict__ rig, const int* __restrict__ stop) {
   __shared__ TileHdr<VB> T;
-  extern __shared__ float lds[];
+  extern __shared__ float4 lds4[];
   const StepConst& c = a.c;
   if (stop && *stop) return;
   const uint32_t brick = xcd_block();
@@ -325,7 +327,7 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
   if (!brick_open<VB>(a.g, a.fstart, a.thdr, brick, cap, T, B)) return;
   const uint32_t total = B.total;
   const bool in_lds = B.in_lds;
-  float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap; float* lk = lds + 3 * cap;
+  float4* L = lds4;  // x, y, z, k
   const int t = threadIdx.x;
   if (in_lds) {
     uint32_t js[VT];
@@ -338,7 +340,7 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
 #pragma unroll
     for (int r = 0; r < VT; ++r) {
       const uint32_t s = t + r * VB;
-      if (s < total) { lx[s] = q[r].x; ly[s] = q[r].y; lz[s] = q[r].z; lk[s] = kk[r]; }
+      if (s < total) L[s] = make_float4(q[r].x, q[r].y, q[r].z, kk[r]);
     }
   }
   __syncthreads();
@@ -355,21 +357,22 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
     const uint32_t nF = cc & 0xffffu, nB = cc >> 16;
     // v_i -= dt (k_i + k_j) grad_p_ij = dt V_f (k_i + k_j) c_ij r_ij; dt V_f applied once
     float ux = 0.f, uy = 0.f, uz = 0.f;
-    auto pair = [&](float qx, float qy, float qz, float kj) {
+    // pairs with |k_i + k_j| <= eps (and slots that are not live) add s = 0
+    auto pair = [&](float qx, float qy, float qz, float kj, bool live) {
       const float kSum = ki + kj;
-      if (fabsf(kSum) > c.eps) {
-        const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
-        const float s = kSum * cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
-        ux += s * rx; uy += s * ry; uz += s * rz;
-      }
+      const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
+      const float gc = cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
+      const float s = (live && fabsf(kSum) > c.eps) ? kSum * gc : 0.f;
+      ux += s * rx; uy += s * ry; uz += s * rz;
     };
     if (in_lds) {
-      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) { pair(lx[s], ly[s], lz[s], lk[s]); });
+      for_fluid_groups<4>(a.nbr8, stride, i, nF, [&](uint32_t s) { return L[s]; },
+                          [&](const float4& q, bool live) { pair(q.x, q.y, q.z, q.w, live); });
     } else {
       for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {
         const uint32_t j = tile_global<VB>(T, s);
         const float4 q = a.pos[j];
-        pair(q.x, q.y, q.z, kfield[j]);
+        pair(q.x, q.y, q.z, kfield[j], true);
       });
     }
     {
@@ -434,7 +437,7 @@ k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, 
               float* __restrict__ badv, float* __restrict__ kf, float* __restrict__ kappa, double* __restrict__ err,
               const float* __restrict__ kpred, const int* __restrict__ stop) {
   __shared__ TileHdr<PB> T;
-  extern __shared__ float lds[];
+  extern __shared__ float4 lds4[];
   const StepConst& c = a.c;
   if (stop && *stop) return;
   const uint32_t brick = xcd_block();
@@ -443,8 +446,8 @@ k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, 
   if (!brick_open<PB>(a.g, a.fstart, a.thdr, brick, cap, T, B)) return;
   const uint32_t total = B.total;
   const bool in_lds = B.in_lds;
-  float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
-  float* lvx = lds + 3 * cap; float* lvy = lds + 4 * cap; float* lvz = lds + 5 * cap;
+  float4* LA = lds4;                                          // x, y, z, vx
+  float2* LB = reinterpret_cast<float2*>(lds4 + cap);         // vy, vz
   const int t = threadIdx.x;
   if (in_lds) {
     uint32_t js[PT];
@@ -457,8 +460,8 @@ k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, 
     for (int r = 0; r < PT; ++r) {
       const uint32_t s = t + r * PB;
       if (s < total) {
-        lx[s] = q[r].x; ly[s] = q[r].y; lz[s] = q[r].z;
-        lvx[s] = v[r].x; lvy[s] = v[r].y; lvz[s] = v[r].z;
+        LA[s] = make_float4(q[r].x, q[r].y, q[r].z, v[r].x);
+        LB[s] = make_float2(v[r].y, v[r].z);
       }
     }
   }
@@ -472,19 +475,23 @@ k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, 
     const uint32_t nF = cc & 0xffffu, nB = cc >> 16;
     float delta = 0.f;
     float fdelta = 0.f;  // fluid part without the common factor V_f
-    auto pair = [&](float qx, float qy, float qz, float vx, float vy, float vz) {
+    auto pair = [&](float qx, float qy, float qz, float vx, float vy, float vz, bool live) {
       const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
-      const float sc = cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
+      const float gc = cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
+      const float sc = live ? gc : 0.f;
       fdelta += sc * ((vi.x - vx) * rx + (vi.y - vy) * ry + (vi.z - vz) * rz);
     };
     if (in_lds) {
-      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) { pair(lx[s], ly[s], lz[s], lvx[s], lvy[s], lvz[s]); });
+      for_fluid_groups<4>(a.nbr8, stride, i, nF, [&](uint32_t s) { return Slot6{LA[s], LB[s]}; },
+                          [&](const Slot6& q, bool live) {
+                            pair(q.a.x, q.a.y, q.a.z, q.a.w, q.b.x, q.b.y, live);
+                          });
     } else {
       for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {
         const uint32_t j = tile_global<PB>(T, s);
         const float4 q = a.pos[j];
         const float4 v = a.vel[j];
-        pair(q.x, q.y, q.z, v.x, v.y, v.z);
+        pair(q.x, q.y, q.z, v.x, v.y, v.z, true);
       });
     }
     delta = c.V_f * fdelta;
@@ -558,7 +565,7 @@ void launch_build_tiled(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBu
 
 void launch_density_factor_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
                              NbrBufs& nb, int div_warm, int divergence_on, float2* aux, hipStream_t s) {
-  const size_t shm = sizeof(float) * (size_t)nb.cap * (divergence_on ? 6 : 3);
+  const size_t shm = sizeof(float) * (size_t)nb.cap * 6;
   hipLaunchKernelGGL(k_density_factor_t, dim3(nbricks(g)), dim3(PB), shm, s, make_args(c, g, gb, f, b, nb), f.rho,
                      f.alpha, f.s, f.badv, f.kappav, div_warm, divergence_on, aux);
 }

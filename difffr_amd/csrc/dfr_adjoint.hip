@@ -127,7 +127,7 @@ k_adj_update_t(AdjPass a, const float* __restrict__ k, const float4* __restrict_
       xfz += sk * (ce.c * uz + er * rz);
     };
     if (in_lds) {
-      for_fluid_groups<4>(a.nbr8, stride, i, nF, [&](uint32_t s) { return Slot8{LA[s], LB[s]}; },
+      for_fluid_groups<2>(a.nbr8, stride, i, nF, [&](uint32_t s) { return Slot8{LA[s], LB[s]}; },
                           [&](const Slot8& q, bool live) {
                             pair(q.a.x, q.a.y, q.a.z, q.b.x, q.b.y, q.b.z, q.a.w, live);
                           });
@@ -269,7 +269,7 @@ k_adj_delta_t(AdjPass a, const float4* __restrict__ v, const float* __restrict__
       hx += sc * ux + se * rx; hy += sc * uy + se * ry; hz += sc * uz + se * rz;
     };
     if (in_lds) {
-      for_fluid_groups<4>(a.nbr8, stride, i, nF, [&](uint32_t s) { return Slot8{LA[s], LB[s]}; },
+      for_fluid_groups<2>(a.nbr8, stride, i, nF, [&](uint32_t s) { return Slot8{LA[s], LB[s]}; },
                           [&](const Slot8& q, bool live) {
                             pair(q.a.x, q.a.y, q.a.z, q.b.x, q.b.y, q.b.z, q.a.w, live);
                           });
@@ -383,7 +383,7 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
       fx += ce.c * ux + er * rx; fy += ce.c * uy + er * ry; fz += ce.c * uz + er * rz;
     };
     if (in_lds) {
-      for_fluid_groups<4>(a.nbr8, stride, i, nF, [&](uint32_t s) { return Slot8{LA[s], LB[s]}; },
+      for_fluid_groups<2>(a.nbr8, stride, i, nF, [&](uint32_t s) { return Slot8{LA[s], LB[s]}; },
                           [&](const Slot8& q, bool live) {
                             pair(q.a.x, q.a.y, q.a.z, q.a.w, q.b.x, q.b.y, q.b.z, q.b.w, live);
                           });

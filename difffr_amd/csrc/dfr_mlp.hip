@@ -380,7 +380,9 @@ __global__ void __launch_bounds__(64) k_mlp(MlpArgs a) {
 // gradient keeps silu'(z) of the three hidden layers (D layout, FP32) from the
 // tile's forward sweep, so nothing is recomputed.
 // ---------------------------------------------------------------------------
-constexpr int CB = 256;                        // chain workgroup: 4 wavefronts x 64 particles
+// chain workgroup: 64 particles per wavefront; the gradient's workgroup is
+// twice as large, so eight wavefronts share one staged copy of its 44 fragments
+template <bool IG> constexpr int chain_block() { return IG ? 512 : 256; }
 constexpr int CHAIN_BIAS = 3 * 64 + 16;        // biases of layers 0..3 (output layer zero-padded to a tile)
 template <bool IG> struct ChainLds {
   static constexpr int NF = IG ? MLP_NFRAG : MLP_FT[3];  // chain fragments staged (forward ones first)
@@ -398,7 +400,9 @@ __device__ inline bf16x8 pack2(f32x4 lo, f32x4 hi) {
 }
 
 template <bool IG>
-__global__ void __launch_bounds__(CB) k_mlp_chain(MlpArgs a) {
+__global__ void __launch_bounds__(chain_block<IG>()) __attribute__((amdgpu_waves_per_eu(IG ? 4 : 1, 8)))
+k_mlp_chain(MlpArgs a) {
+  constexpr int CB = chain_block<IG>();
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   using L = ChainLds<IG>;
   bf16x8* F = reinterpret_cast<bf16x8*>(smem + L::FRAG);
@@ -605,7 +609,9 @@ void launch_mlp_pack(const float* P, void* frag, hipStream_t s) {
 }
 
 size_t mlp_frag_bytes() { return sizeof(bf16x8) * (size_t)MLP_NFRAG_ALL * 64; }
-static inline unsigned nb_chain(int n) { return (unsigned)((n + CB - 1) / CB); }
+template <bool IG> static inline unsigned nb_chain(int n) {
+  return (unsigned)((n + chain_block<IG>() - 1) / chain_block<IG>());
+}
 
 static MlpArgs base_args(const void* frag, const float* P, const MlpNorm& norm, int n) {
   MlpArgs a{};
@@ -622,7 +628,8 @@ void launch_mlp_infer_fields(const void* frag, const float* P, const MlpNorm& no
   MlpArgs a = base_args(frag, P, norm, n);
   a.fields = f;
   a.out = out;
-  hipLaunchKernelGGL(k_mlp_chain<false>, dim3(nb_chain(n)), dim3(CB), ChainLds<false>::BYTES, s, a);
+  hipLaunchKernelGGL(k_mlp_chain<false>, dim3(nb_chain<false>(n)), dim3(chain_block<false>()),
+                     ChainLds<false>::BYTES, s, a);
 }
 
 void launch_mlp_infer_array(const void* frag, const float* P, const MlpNorm& norm, const float* feat, int n,
@@ -631,7 +638,8 @@ void launch_mlp_infer_array(const void* frag, const float* P, const MlpNorm& nor
   MlpArgs a = base_args(frag, P, norm, n);
   a.feat = feat;
   a.out = out;
-  hipLaunchKernelGGL(k_mlp_chain<false>, dim3(nb_chain(n)), dim3(CB), ChainLds<false>::BYTES, s, a);
+  hipLaunchKernelGGL(k_mlp_chain<false>, dim3(nb_chain<false>(n)), dim3(chain_block<false>()),
+                     ChainLds<false>::BYTES, s, a);
 }
 
 void launch_mlp_train_batch(const void* frag, const float* P, const MlpNorm& norm, const float* feat,
@@ -656,7 +664,8 @@ void launch_mlp_input_grad(const void* frag, const float* P, const MlpNorm& norm
   if (fields) a.fields = *fields;
   a.ybar = ybar;
   a.fbar = fbar;
-  hipLaunchKernelGGL(k_mlp_chain<true>, dim3(nb_chain(n)), dim3(CB), ChainLds<true>::BYTES, s, a);
+  hipLaunchKernelGGL(k_mlp_chain<true>, dim3(nb_chain<true>(n)), dim3(chain_block<true>()),
+                     ChainLds<true>::BYTES, s, a);
 }
 
 void launch_mlp_adam(float* P, float* G, float* m, float* v, float lr, int t, hipStream_t s) {

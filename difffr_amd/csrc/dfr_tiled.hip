@@ -259,7 +259,7 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
       if (divergence_on) fdelta += (vi.x - vx) * wx + (vi.y - vy) * wy + (vi.z - vz) * wz;
     };
     if (in_lds) {
-      for_fluid_groups<4>(a.nbr8, stride, i, nF, [&](uint32_t s) { return Slot6{LA[s], LB[s]}; },
+      for_fluid_groups<2>(a.nbr8, stride, i, nF, [&](uint32_t s) { return Slot6{LA[s], LB[s]}; },
                           [&](const Slot6& q, bool live) {
                             pair(q.a.x, q.a.y, q.a.z, q.a.w, q.b.x, q.b.y, live);
                           });
@@ -366,7 +366,7 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
       ux += s * rx; uy += s * ry; uz += s * rz;
     };
     if (in_lds) {
-      for_fluid_groups<4>(a.nbr8, stride, i, nF, [&](uint32_t s) { return L[s]; },
+      for_fluid_groups<2>(a.nbr8, stride, i, nF, [&](uint32_t s) { return L[s]; },
                           [&](const float4& q, bool live) { pair(q.x, q.y, q.z, q.w, live); });
     } else {
       for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {

@@ -72,7 +72,7 @@ __device__ inline int halo_cell(const TileHdr<NT>& T, int cx, int cy, int cz, in
 // indices (LDS-only binary searches, independent), then issues all its global
 // loads before writing LDS, so the loads are in flight together.  CAP is the
 // largest capacity (registers per thread); slots >= total are skipped.
-template <int NT, int CAP = TILE_CAP> constexpr int per_thread() { return CAP / NT; }
+template <int NT, int CAP = TILE_CAP> constexpr int per_thread() { return (CAP + NT - 1) / NT; }
 
 template <int NT, int CAP = TILE_CAP>
 __device__ inline void tile_indices(const TileHdr<NT>& T, uint32_t total, uint32_t* js) {

@@ -59,7 +59,12 @@ struct PackWriter {
   }
 };
 
-// Dynamic LDS: x, y, z of the halo (cap slots each).
+// Dynamic LDS: x, y, z of the halo in three arrays of a fixed stride, so one
+// address serves a chunk of four candidates (immediate offsets); a chunk reads
+// up to three slots past its row's end, which stay inside the arrays because
+// a halo is staged only if it has at most BSTRIDE - 4 slots (three workgroups
+// of 640 threads still fit a CU's LDS).
+constexpr int BSTRIDE = 2304;
 __global__ void __launch_bounds__(BB)
 k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t* __restrict__ fstart,
               const float4* __restrict__ bpos, const uint32_t* __restrict__ bstart,
@@ -98,8 +103,8 @@ k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t
     bcs[t] = st;
     bcn[t] = n;
   }
-  const bool in_lds = total <= (uint32_t)cap;
-  float* lx = lds; float* ly = lds + cap; float* lz = lds + 2 * cap;
+  const bool in_lds = total <= (uint32_t)min(cap, BSTRIDE - 4);
+  float* lx = lds; float* ly = lds + BSTRIDE; float* lz = lds + 2 * BSTRIDE;
   // the halo's slot -> particle map, for every slot a pass could stage
   constexpr int PTB = per_thread<BB>();
   uint32_t js[PTB];
@@ -143,7 +148,7 @@ k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t
             float qx[4], qy[4], qz[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-              const uint32_t s = min(s0 + u, r1 - 1);
+              const uint32_t s = s0 + u;  // <= r1 + 2 <= total + 2 < BSTRIDE
               qx[u] = lx[s]; qy[u] = ly[s]; qz[u] = lz[s];
             }
 #pragma unroll
@@ -558,7 +563,7 @@ static PassArgs make_args(const StepConst& c, const GridDesc& g, GridBufs& gb, F
 
 void launch_build_tiled(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
                         hipStream_t s) {
-  const size_t shm = sizeof(float) * (size_t)nb.cap * 3;
+  const size_t shm = sizeof(float) * 3 * (size_t)BSTRIDE;
   hipLaunchKernelGGL(k_build_tiled, dim3(nbricks(g)), dim3(BB), shm, s, f.pos, f.n, g, gb.fstart, b.pos, gb.bstart,
                      gb.bperm, h2, nb.maxb, nb.cap, nb.nbr8, nb.nbrB, nb.cnt, nb.overflow, nb.thdr, nb.tmap);
 }
@@ -595,6 +600,8 @@ void set_tiled_kernel_attributes() {
   (void)hipFuncSetAttribute((const void*)k_density_factor_t, hipFuncAttributeMaxDynamicSharedMemorySize, shm6);
   (void)hipFuncSetAttribute((const void*)k_recompute_t, hipFuncAttributeMaxDynamicSharedMemorySize, shm6);
   (void)hipFuncSetAttribute((const void*)k_velocity_update_t, hipFuncAttributeMaxDynamicSharedMemorySize, shm4);
+  (void)hipFuncSetAttribute((const void*)k_build_tiled, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(float) * 3 * BSTRIDE));
 }
 
 }  // namespace dfr

@@ -138,35 +138,40 @@ k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t
     const uint32_t self = T.coff[hc0] + (i - T.cstart[hc0]);
     PackWriter pw{ring, t};
     // the three x-neighbour cells of a (dz, dy) row are consecutive halo
-    // cells, so their candidates form one contiguous slot range
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int hm = halo_cell<BB>(T, cx, cy, cz, -1, dy, dz);
-        const uint32_t r0 = T.coff[hm], r1 = T.coff[hm + 3];
-        if (in_lds) {
-          for (uint32_t s0 = r0; s0 < r1; s0 += 4) {
-            float qx[4], qy[4], qz[4];
+    // cells, so their candidates form one contiguous slot range; the centre
+    // row goes as the two ranges either side of the particle's own slot
+    // (ranges 4 and 5), which keeps the candidate order and needs no
+    // per-candidate self test
+    for (int row = 0; row < 10; ++row) {
+      const int rr = row < 5 ? row : row - 1;
+      const int dz = rr / 3 - 1, dy = rr % 3 - 1;
+      const int hm = halo_cell<BB>(T, cx, cy, cz, -1, dy, dz);
+      uint32_t r0 = T.coff[hm], r1 = T.coff[hm + 3];
+      if (row == 4) r1 = self;
+      if (row == 5) r0 = self + 1;
+      if (in_lds) {
+        for (uint32_t s0 = r0; s0 < r1; s0 += 4) {
+          float qx[4], qy[4], qz[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const uint32_t s = s0 + u;  // <= r1 + 2 <= total + 2 < BSTRIDE
-              qx[u] = lx[s]; qy[u] = ly[s]; qz[u] = lz[s];
-            }
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t s = s0 + u;  // <= r1 + 2 <= total + 2 < BSTRIDE
+            qx[u] = lx[s]; qy[u] = ly[s]; qz[u] = lz[s];
+          }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const uint32_t s = s0 + u;
-              if (s < r1 && s != self && dist2_exact(p.x, p.y, p.z, qx[u], qy[u], qz[u]) < h2) pw.push(s);
-            }
-            pw.flush_full(nbr8, stride, i);
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t s = s0 + u;
+            if (s < r1 && dist2_exact(p.x, p.y, p.z, qx[u], qy[u], qz[u]) < h2) pw.push(s);
           }
-        } else {
-          for (uint32_t s = r0; s < r1; ++s) {
-            if (s == self) continue;
-            const float4 q = pos[tile_global<BB>(T, s)];
-            if (dist2_exact(p.x, p.y, p.z, q.x, q.y, q.z) < h2) pw.push(s);
-            pw.flush_full(nbr8, stride, i);
-          }
+          pw.flush_full(nbr8, stride, i);
+        }
+      } else {
+        for (uint32_t s = r0; s < r1; ++s) {
+          const float4 q = pos[tile_global<BB>(T, s)];
+          if (dist2_exact(p.x, p.y, p.z, q.x, q.y, q.z) < h2) pw.push(s);
+          pw.flush_full(nbr8, stride, i);
         }
       }
+    }
     uint32_t nF = pw.finish(nbr8, stride, i);
     uint32_t nB = 0;
     for (int dz = -1; dz <= 1; ++dz)

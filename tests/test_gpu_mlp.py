@@ -170,20 +170,51 @@ def test_learned_adjoint_stored_and_replayed_agree(gpu_ready):
         np.testing.assert_allclose(g, grads[0], rtol=0, atol=1e-5 * np.abs(grads[0]).max())
 
 
+def _seeded_network(sc, steps, seed=5):
+    """A fixed network: seeded He-uniform weights, feature normalisation from the
+    features an oracle run of the scene records, output scale of the stored
+    stiffness.  (A network trained on the GPU differs from run to run -- the
+    capture appends rows and the gradient sums with float atomics -- so a
+    comparison against it would not be reproducible.)"""
+    from oracle import mlp_oracle as mo
+    from oracle import pyoracle as po
+    rng = np.random.default_rng(seed)
+    P = np.zeros(mo.NP, np.float32)
+    o = 0
+    for m, n in mo.SIZES:
+        P[o:o + m * n] = rng.uniform(-1, 1, m * n) * np.sqrt(6.0 / n)
+        o += m * n
+        P[o:o + m] = rng.normal(scale=0.1, size=m)
+        o += m
+    out = np.asarray([-1.4e-3, -1e-8, 1.3e-3, 1.4e-7])
+    probe = np.concatenate([np.zeros(6), np.ones(6), out]).astype(np.float32)
+    s = po.OracleSim(sc, "f64")
+    s.set_warm_net(P, probe, bf16=False)
+    s.set_record(True)
+    s.step(steps)
+    f = np.concatenate([s.tape_field(k, "feat").reshape(-1, 6) for k in range(steps)])
+    sd = f.std(axis=0)
+    sd[sd < 1e-30] = 1.0
+    return P, np.concatenate([f.mean(axis=0), sd, out]).astype(np.float32)
+
+
 def test_learned_adjoint_matches_oracle(gpu_ready):
-    """Network trained on the GPU, then a learned-mode rollout with its exact
-    adjoint against the oracle's learned step and adjoint (oracle/warmnet.hpp
-    with the GPU's bf16 rounding, FP64 elsewhere, replaying the GPU run's
-    iteration counts).  Tolerance 1e-3 of the largest component, as for the
-    plain adjoint: bf16 decision flips between the two summation orders move
-    single warm-start values by ~2^-8, far below that."""
+    """A learned-mode rollout with its exact adjoint against the oracle's learned
+    step and adjoint (oracle/warmnet.hpp with the GPU's bf16 rounding, replaying
+    the GPU run's iteration counts), for a fixed network, twice:
+
+    * the oracle with the GPU's precision layout (FP32 fluid, FP64 rigid):
+      1e-4 of the largest component (the two differ in summation order and
+      a few bf16 roundings only; measured 3e-7);
+    * the FP64 oracle: 5e-3.  With the learned warm start the pressure solve
+      stops at its two-iteration floor, so the warm start's inputs (rho/rho0 - 1
+      and dt delta, small differences of FP32 sums, rounded to bf16) carry
+      FP32 noise into the trajectory: the oracle's own FP32 and FP64 runs of
+      this scene differ by 2.0e-3 of the largest component, as does the GPU."""
     from oracle import pyoracle as po
     steps = 12
     sc = scenes.submerged_body(body="ellipsoid", steps=steps)
-    trainer = _sim(sc, compute_gradient=0)
-    _capture_and_train(trainer, steps, 400, fraction=1.0)
-    P, norm = trainer.mlp_get_params()
-    trainer.close()
+    P, norm = _seeded_network(sc, steps)
     seed = np.zeros((2, 7))
     seed[1] = np.random.default_rng(8).normal(size=7)
     print(f"[learned adjoint vs oracle] steps={steps} n={sc.n_fluid}", flush=True)
@@ -196,19 +227,26 @@ def test_learned_adjoint_matches_oracle(gpu_ready):
     iters = s.adjoint_iterations()
     rig = s.rigid(1)
     s.close()
-    o = po.OracleSim(sc, "f64")
-    o.set_warm_net(P, norm, bf16=True)
-    o.set_fixed_iterations(*iters)
-    o.set_record(True)
-    o.step(steps)
-    ref = o.adjoint(seed)[1]
-    ro = o.rigid(1)
-    print("gpu   ", g, "\noracle", ref, "\niterations", iters, flush=True)
+    refs = {}
+    for variant in ("f32_rigid64", "f64"):
+        o = po.OracleSim(sc, variant)
+        o.set_warm_net(P, norm, bf16=True)
+        o.set_fixed_iterations(*iters)
+        o.set_record(True)
+        o.step(steps)
+        refs[variant] = (o.adjoint(seed)[1], o.rigid(1))
+    ref, ro = refs["f64"]
+    scale = np.abs(ref).max()
+    print("gpu        ", g, "\noracle f32 ", refs["f32_rigid64"][0], "\noracle f64 ", ref, "\niterations", iters,
+          flush=True)
+    for variant, tol in (("f32_rigid64", 1e-4), ("f64", 5e-3)):
+        print(f"{variant}: largest difference / scale {np.abs(g - refs[variant][0]).max() / scale:.2e} "
+              f"(tolerance {tol:.0e})", flush=True)
     np.testing.assert_allclose(rig["x"], ro["x"], rtol=0, atol=1e-5)  # FP32 fluid against FP64, 12 steps
     np.testing.assert_allclose(rig["q"], ro["q"], rtol=0, atol=1e-5)
-    scale = np.abs(ref).max()
     assert np.all(np.isfinite(g))
-    np.testing.assert_allclose(g, ref, rtol=0, atol=1e-3 * scale)
+    np.testing.assert_allclose(g, refs["f32_rigid64"][0], rtol=0, atol=1e-4 * scale)
+    np.testing.assert_allclose(g, ref, rtol=0, atol=5e-3 * scale)
     # the learned warm start shapes the gradient: the plain oracle's is far off
     o2 = po.OracleSim(sc, "f64")
     o2.set_fixed_iterations(*iters)

@@ -524,8 +524,8 @@ struct Batch {
     if (L.n == COPY_SEGS) flush();
     L.seg[L.n++] = CopySeg{src, dst, (uint64_t)bytes};
   }
-  void flush() {
-    launch_multi_copy(L, c->stream);
+  void flush(const int* stop = nullptr) {
+    launch_multi_copy(L, c->stream, stop);
     L.n = 0;
   }
 };
@@ -535,7 +535,8 @@ template <typename T> T* talloc(dfr_ctx* c, StepTapeDev* rec, size_t n) {
   return rec && rec->arena ? static_cast<T*>(rec->arena->alloc(sizeof(T) * std::max<size_t>(n, 1))) : dalloc<T>(c, n);
 }
 
-void tape_iteration(dfr_ctx* c, std::vector<float4*>& v, std::vector<float*>& k, std::vector<float*>& b, int m) {
+void tape_iteration(dfr_ctx* c, std::vector<float4*>& v, std::vector<float*>& k, std::vector<float*>& b, int m,
+                    const int* stop) {
   const size_t n = (size_t)std::max(c->f.n, 1);
   while ((int)v.size() <= m) {
     v.push_back(talloc<float4>(c, c->rec, n));
@@ -546,7 +547,7 @@ void tape_iteration(dfr_ctx* c, std::vector<float4*>& v, std::vector<float*>& k,
   bt.copy(v[m], c->f.vel, c->f.n);
   bt.copy(k[m], c->f.kf, c->f.n);
   bt.copy(b[m], c->f.badv, c->f.n);
-  bt.flush();
+  bt.flush(stop);
 }
 
 // The Jacobi iterations of one solve.  fixed >= 0: exactly that many (replay,
@@ -643,8 +644,8 @@ void do_step(dfr_ctx* c) {
   begin_step(c);
   {
     Timed t(c, TC_SORT);
-    if (nf > 0) launch_sort_fluid(c->f, c->grid, c->gb, c->stream);
-    launch_sort_boundary(c->b, c->grid, c->gb, c->stream);
+    if (nf > 0) launch_sort_fluid_and_boundary(c->f, c->b, c->grid, c->gb, c->stream);
+    else launch_sort_boundary(c->b, c->grid, c->gb, c->stream);
   }
   {
     // tape of the step's start (the sort leaves the rigid state alone) and the step's counters
@@ -721,7 +722,7 @@ void do_step(dfr_ctx* c) {
         c, 0, fixV, 1, maxv, eta, c->d_err, nf,
         [&](int m, const int* stop) {
           run_gradient(c, 1);
-          if (rec) tape_iteration(c, rec->vD, rec->kD, rec->bD, m);
+          if (rec) tape_iteration(c, rec->vD, rec->kD, rec->bD, m, stop);
           {
             Timed t(c, TC_VUPDATE);
             launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kf,
@@ -770,7 +771,7 @@ void do_step(dfr_ctx* c) {
         c, 1, fixP, c->P.min_iterations, maxp, eta, c->d_err + base, nf,
         [&](int m, const int* stop) {
           run_gradient(c, 0);
-          if (rec) tape_iteration(c, rec->vP, rec->kP, rec->bP, m);
+          if (rec) tape_iteration(c, rec->vP, rec->kP, rec->bP, m, stop);
           {
             Timed t(c, TC_VUPDATE);
             launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kf,
@@ -1428,7 +1429,10 @@ int dfr_init(dfr_ctx* c) {
     c->gb.bcount = dalloc<uint32_t>(c, nk); c->gb.bstart = dalloc<uint32_t>(c, nk + 1);
     c->gb.key = dalloc<uint32_t>(c, NF); c->gb.slot = dalloc<uint32_t>(c, NF); c->gb.perm = dalloc<uint32_t>(c, NF);
     c->gb.bkey = dalloc<uint32_t>(c, NB); c->gb.bslot = dalloc<uint32_t>(c, NB); c->gb.bperm = dalloc<uint32_t>(c, NB);
-    c->gb.tile_sums = dalloc<uint32_t>(c, scan_tile_count(nk) + 1);
+    c->gb.tile_sums = dalloc<uint32_t>(c, 2 * (scan_tile_count(nk) + 1));
+    // the sorts expect zero counts and leave them zero
+    HIPCHK(hipMemsetAsync(c->gb.fcount, 0, sizeof(uint32_t) * nk, c->stream));
+    HIPCHK(hipMemsetAsync(c->gb.bcount, 0, sizeof(uint32_t) * nk, c->stream));
     c->nb.maxb = c->P.max_neighbors;
     c->nb.maxbf = c->P.max_boundary_neighbors;
     c->nb.nbr8 = dalloc<uint4>(c, (size_t)NPACK * NF);

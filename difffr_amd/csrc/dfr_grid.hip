@@ -17,17 +17,27 @@ namespace dfr {
 // step's cell order, so a wavefront's keys form runs of equal values; each run
 // takes its slots with one atomic (the leader's), the other lanes add their
 // rank in the run.  Slot order within a cell is then fixed by k_cell_sort.
-__global__ void k_keys(const float4* __restrict__ pos, int n, GridDesc g, uint32_t* __restrict__ count,
-                       uint32_t* __restrict__ key, uint32_t* __restrict__ slot) {
-  const uint32_t i = gid();
-  const bool valid = i < (uint32_t)n;
+// The fluid and the boundary samples are sorted by the same launches: the
+// blocks of a grid over particles split at S.split between the two sets, the
+// per-cell kernels take the set from blockIdx.y.
+__device__ inline const SortSet& set_of_block(const SortSets& S, uint32_t b, uint32_t& lb) {
+  const int k = b >= S.split ? 1 : 0;
+  lb = b - (k ? S.split : 0u);
+  return S.s[k];
+}
+
+__global__ void k_keys(SortSets S, GridDesc g) {
+  uint32_t lb;
+  const SortSet& T = set_of_block(S, xcd_block(), lb);
+  const uint32_t i = lb * blockDim.x + threadIdx.x;
+  const bool valid = i < T.n;
   uint32_t k = 0xffffffffu;  // lanes past the end form their own run and take no slot
   if (valid) {
-    const float4 p = pos[i];
+    const float4 p = T.pos[i];
     int cx, cy, cz;
     cell_coords(g, p.x, p.y, p.z, cx, cy, cz);
     k = cell_key(g, cx, cy, cz);
-    key[i] = k;
+    T.key[i] = k;
   }
   const int lane = (int)(threadIdx.x & 63u);
   const uint32_t kprev = __shfl_up(k, 1, 64);
@@ -37,16 +47,17 @@ __global__ void k_keys(const float4* __restrict__ pos, int n, GridDesc g, uint32
   const unsigned long long after = heads & ~upto;
   const int end = after ? __ffsll((long long)after) - 1 : 64;
   uint32_t base = 0;
-  if (valid && lane == first) base = atomicAdd(&count[k], (uint32_t)(end - first));
+  if (valid && lane == first) base = atomicAdd(&T.count[k], (uint32_t)(end - first));
   base = __shfl(base, first, 64);
-  if (valid) slot[i] = base + (uint32_t)(lane - first);
+  if (valid) T.slot[i] = base + (uint32_t)(lane - first);
 }
 
-__global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_t* __restrict__ slot,
-                          const uint32_t* __restrict__ start, uint32_t* __restrict__ perm) {
-  const uint32_t i = gid();
-  if (i >= (uint32_t)n) return;
-  perm[start[key[i]] + slot[i]] = i;
+__global__ void k_scatter(SortSets S) {
+  uint32_t lb;
+  const SortSet& T = set_of_block(S, xcd_block(), lb);
+  const uint32_t i = lb * blockDim.x + threadIdx.x;
+  if (i >= T.n) return;
+  T.perm[T.start[T.key[i]] + T.slot[i]] = i;
 }
 
 // Sort of each cell's entries by an order key (particle id for the fluid,
@@ -55,8 +66,11 @@ __global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_
 // up to 16 entries are loaded together and sorted in registers by a bitonic
 // network; larger cells use an insertion sort.
 constexpr int CELL_NET = 16;
-__global__ void k_cell_sort(uint32_t nkeys, const uint32_t* __restrict__ start, uint32_t* __restrict__ perm,
-                            const uint32_t* __restrict__ order) {
+__global__ void k_cell_sort(SortSets S, uint32_t nkeys) {
+  const SortSet& T = S.s[blockIdx.y];
+  const uint32_t* __restrict__ start = T.start;
+  uint32_t* __restrict__ perm = T.perm;
+  const uint32_t* __restrict__ order = T.order;
   const uint32_t c = gid();
   if (c >= nkeys) return;
   const uint32_t s = start[c], e = start[c + 1];
@@ -120,33 +134,37 @@ __global__ void k_gather_fluid(int n, const uint32_t* __restrict__ perm, const f
 }
 
 // ---------------------------------------------------------------------------
-// Exclusive scan of the per-cell counts (three passes, 2048 keys per block).
+// Exclusive scan of the per-cell counts of each set (blockIdx.y, or blockIdx.x
+// for the tile sums): three passes, 2048 keys per block.  The last pass also
+// clears the counts for the next sort.
 // ---------------------------------------------------------------------------
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = DFR_BLOCK * SCAN_ITEMS;
 
-__global__ void k_scan_tiles(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ out,
-                             uint32_t* __restrict__ tile_sums) {
+__global__ void k_scan_tiles(SortSets S, uint32_t n) {
+  const SortSet& T = S.s[blockIdx.y];
   __shared__ uint32_t lds[DFR_BLOCK / 64 + 1];
   const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   uint32_t v[SCAN_ITEMS];
   uint32_t sum = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    v[k] = (base + k < n) ? in[base + k] : 0u;
+    v[k] = (base + k < n) ? T.count[base + k] : 0u;
     sum += v[k];
   }
   uint32_t total;
   uint32_t run = block_exclusive_scan(sum, lds, total);
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    if (base + k < n) out[base + k] = run;
+    if (base + k < n) T.start[base + k] = run;
     run += v[k];
   }
-  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+  if (threadIdx.x == 0) T.tile_sums[blockIdx.x] = total;
 }
 
-__global__ void k_scan_sums(uint32_t* __restrict__ sums, uint32_t m, uint32_t* __restrict__ total_out) {
+__global__ void k_scan_sums(SortSets S, uint32_t m, uint32_t n) {
+  const SortSet& T = S.s[blockIdx.x];
+  uint32_t* __restrict__ sums = T.tile_sums;
   __shared__ uint32_t lds[DFR_BLOCK / 64 + 1];
   uint32_t carry = 0;
   for (uint32_t base = 0; base < m; base += DFR_BLOCK) {
@@ -157,15 +175,19 @@ __global__ void k_scan_sums(uint32_t* __restrict__ sums, uint32_t m, uint32_t* _
     if (i < m) sums[i] = ex + carry;
     carry += total;
   }
-  if (threadIdx.x == 0) *total_out = carry;
+  if (threadIdx.x == 0) T.start[n] = carry;
 }
 
-__global__ void k_scan_add(uint32_t* __restrict__ out, uint32_t n, const uint32_t* __restrict__ tile_sums) {
+__global__ void k_scan_add(SortSets S, uint32_t n) {
+  const SortSet& T = S.s[blockIdx.y];
   const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
-  const uint32_t add = tile_sums[blockIdx.x];
+  const uint32_t add = T.tile_sums[blockIdx.x];
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k)
-    if (base + k < n) out[base + k] += add;
+    if (base + k < n) {
+      T.start[base + k] += add;
+      T.count[base + k] = 0u;
+    }
 }
 
 // Fluid neighbours of the samples of dynamic bodies (the B->F lists the
@@ -209,41 +231,51 @@ __global__ void k_build_bf_nbrs(const float4* __restrict__ bpos, const uint32_t*
 // ---------------------------------------------------------------------------
 static inline unsigned blocks_for(size_t n, unsigned b = DFR_BLOCK) { return (unsigned)((n + b - 1) / b); }
 
-void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* tile_sums, hipStream_t s) {
-  // out has n + 1 entries; out[n] receives the total.
-  const unsigned tiles = (unsigned)((n + SCAN_TILE - 1) / SCAN_TILE);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(DFR_BLOCK), 0, s, in, n, out, tile_sums);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(DFR_BLOCK), 0, s, tile_sums, tiles, out + n);
-  hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(DFR_BLOCK), 0, s, out, n, tile_sums);
-}
-
 size_t scan_tile_count(uint32_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
-void launch_sort_fluid(FluidBufs& f, const GridDesc& g, GridBufs& gb, hipStream_t s) {
-  const int n = f.n;
-  (void)hipMemsetAsync(gb.fcount, 0, sizeof(uint32_t) * g.nkeys, s);
-  hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(DFR_BLOCK), 0, s, f.pos, n, g, gb.fcount, gb.key, gb.slot);
-  launch_exclusive_scan(gb.fcount, g.nkeys, gb.fstart, gb.tile_sums, s);
-  hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(DFR_BLOCK), 0, s, n, gb.key, gb.slot, gb.fstart, gb.perm);
-  hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(g.nkeys)), dim3(DFR_BLOCK), 0, s, g.nkeys, gb.fstart, gb.perm,
-                     (const uint32_t*)f.id);
-  hipLaunchKernelGGL(k_gather_fluid, dim3(blocks_for(n)), dim3(DFR_BLOCK), 0, s, n, gb.perm, f.pos, f.vel, f.kappa,
-                     f.kappav, f.id, f.pos2, f.vel2, f.kappa2, f.kappav2, f.id2);
-  f.swap_sorted();
+// Sorts the sets of S (nsets = 1 or 2) by cell.  The counts must be zero on
+// entry (they are left zero).
+static void launch_sort_sets(const SortSets& S, int nsets, const GridDesc& g, hipStream_t s) {
+  const uint32_t nk = g.nkeys;
+  const unsigned particle_blocks = S.split + (nsets > 1 ? blocks_for(S.s[1].n) : 0u);
+  const unsigned tiles = (unsigned)scan_tile_count(nk);
+  if (particle_blocks) hipLaunchKernelGGL(k_keys, dim3(particle_blocks), dim3(DFR_BLOCK), 0, s, S, g);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles, nsets), dim3(DFR_BLOCK), 0, s, S, nk);
+  hipLaunchKernelGGL(k_scan_sums, dim3(nsets), dim3(DFR_BLOCK), 0, s, S, tiles, nk);
+  hipLaunchKernelGGL(k_scan_add, dim3(tiles, nsets), dim3(DFR_BLOCK), 0, s, S, nk);
+  if (particle_blocks) {
+    hipLaunchKernelGGL(k_scatter, dim3(particle_blocks), dim3(DFR_BLOCK), 0, s, S);
+    hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(nk), nsets), dim3(DFR_BLOCK), 0, s, S, nk);
+  }
+}
+
+static SortSet fluid_set(FluidBufs& f, GridBufs& gb) {
+  return SortSet{f.pos, (uint32_t)f.n, gb.fcount, gb.key, gb.slot, gb.fstart, gb.perm, f.id, gb.tile_sums};
+}
+static SortSet boundary_set(BoundaryBufs& b, GridBufs& gb, uint32_t nk) {
+  return SortSet{b.pos, (uint32_t)b.n, gb.bcount, gb.bkey, gb.bslot, gb.bstart, gb.bperm, nullptr,
+                 gb.tile_sums + scan_tile_count(nk) + 1};
+}
+
+void launch_sort_fluid_and_boundary(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb,
+                                    hipStream_t s) {
+  SortSets S{};
+  S.s[0] = fluid_set(f, gb);
+  S.s[1] = boundary_set(b, gb, g.nkeys);
+  S.split = blocks_for(S.s[0].n);
+  launch_sort_sets(S, 2, g, s);
+  if (f.n > 0) {
+    hipLaunchKernelGGL(k_gather_fluid, dim3(blocks_for(f.n)), dim3(DFR_BLOCK), 0, s, f.n, gb.perm, f.pos, f.vel,
+                       f.kappa, f.kappav, f.id, f.pos2, f.vel2, f.kappa2, f.kappav2, f.id2);
+    f.swap_sorted();
+  }
 }
 
 void launch_sort_boundary(BoundaryBufs& b, const GridDesc& g, GridBufs& gb, hipStream_t s) {
-  const int n = b.n;
-  (void)hipMemsetAsync(gb.bcount, 0, sizeof(uint32_t) * g.nkeys, s);
-  if (n > 0) {
-    hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(DFR_BLOCK), 0, s, b.pos, n, g, gb.bcount, gb.bkey, gb.bslot);
-  }
-  launch_exclusive_scan(gb.bcount, g.nkeys, gb.bstart, gb.tile_sums, s);
-  if (n > 0) {
-    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(DFR_BLOCK), 0, s, n, gb.bkey, gb.bslot, gb.bstart, gb.bperm);
-    hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(g.nkeys)), dim3(DFR_BLOCK), 0, s, g.nkeys, gb.bstart, gb.bperm,
-                       (const uint32_t*)nullptr);
-  }
+  SortSets S{};
+  S.s[0] = boundary_set(b, gb, g.nkeys);
+  S.split = blocks_for(S.s[0].n);
+  launch_sort_sets(S, 1, g, s);
 }
 
 void launch_build_bf(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,

@@ -69,8 +69,29 @@ struct NbrBufs {
 
 // dfr_grid.hip
 size_t scan_tile_count(uint32_t n);
-void launch_exclusive_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* tile_sums, hipStream_t s);
-void launch_sort_fluid(FluidBufs& f, const GridDesc& g, GridBufs& gb, hipStream_t s);
+// One index set of the cell sort: positions, per-cell counts (zero between
+// sorts), cell key and slot of each index, cell starts (nkeys + 1), the
+// permutation, the in-cell order key (nullptr: the index itself) and
+// scan_tile_count(nkeys) + 1 words of scan scratch.
+struct SortSet {
+  const float4* pos;
+  uint32_t n;
+  uint32_t* count;
+  uint32_t* key;
+  uint32_t* slot;
+  uint32_t* start;
+  uint32_t* perm;
+  const uint32_t* order;
+  uint32_t* tile_sums;
+};
+struct SortSets {
+  SortSet s[2];
+  uint32_t split;  // blocks of set 0 in the launches over indices (set 1 takes the rest)
+};
+// The fluid (physically permuted into cell order) and the boundary samples
+// (index list only) are sorted by the same launches.  gb.tile_sums holds
+// 2 * (scan_tile_count(nkeys) + 1) words; the counts start and end at zero.
+void launch_sort_fluid_and_boundary(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, hipStream_t s);
 void launch_sort_boundary(BoundaryBufs& b, const GridDesc& g, GridBufs& gb, hipStream_t s);
 void launch_build_bf(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
                      hipStream_t s);
@@ -109,7 +130,9 @@ struct CopyList {
   CopySeg seg[COPY_SEGS];
   int n;
 };
-void launch_multi_copy(const CopyList& list, hipStream_t s);
+// stop != nullptr: the copy is skipped when *stop is set (a Jacobi iteration
+// enqueued after its solve ended).
+void launch_multi_copy(const CopyList& list, hipStream_t s, const int* stop = nullptr);
 // Convergence test of one Jacobi iteration, on the device: probe gets the
 // average residual err / n and whether the solve ends after iteration it
 // (the loop condition of the reference); stop is set when it does.  A launch

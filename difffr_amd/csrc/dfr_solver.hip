@@ -90,7 +90,8 @@ __global__ void k_solve_check(const double* __restrict__ err, int n, double eta,
 }
 
 // One segment per blockIdx.y: 16-byte words, then the 4-byte tail.
-__global__ void k_multi_copy(CopyList L) {
+__global__ void k_multi_copy(CopyList L, const int* __restrict__ stop) {
+  if (stop && *stop) return;
   const CopySeg sg = L.seg[blockIdx.y];
   const uint64_t n16 = sg.bytes >> 4;
   uint4* d = static_cast<uint4*>(sg.dst);
@@ -107,12 +108,12 @@ __global__ void k_multi_copy(CopyList L) {
 // ---------------------------------------------------------------------------
 static inline unsigned nblk(size_t n) { return (unsigned)((n + DFR_BLOCK - 1) / DFR_BLOCK); }
 
-void launch_multi_copy(const CopyList& list, hipStream_t s) {
+void launch_multi_copy(const CopyList& list, hipStream_t s, const int* stop) {
   if (list.n <= 0) return;
   uint64_t most = 0;
   for (int k = 0; k < list.n; ++k) most = std::max<uint64_t>(most, list.seg[k].bytes >> 4);
   const unsigned bx = (unsigned)std::min<uint64_t>(std::max<uint64_t>((most + DFR_BLOCK - 1) / DFR_BLOCK, 1), 2048);
-  hipLaunchKernelGGL(k_multi_copy, dim3(bx, (unsigned)list.n), dim3(DFR_BLOCK), 0, s, list);
+  hipLaunchKernelGGL(k_multi_copy, dim3(bx, (unsigned)list.n), dim3(DFR_BLOCK), 0, s, list, stop);
 }
 
 void launch_solve_check(const double* err, int n, double eta, int it, int min_it, int max_it, int* stop,

@@ -28,15 +28,17 @@ constexpr int PTR = per_thread<RB, TILE_CAP>();
 struct PairCE {
   float c, e;
 };
+// Outer part with g = l f^2 / (h |r|): c = -g, e = (2 l f / h^2 + g) / |r|^2.
 __device__ inline PairCE cubic_ce(const KernelConst& K, float r2) {
   const float irl = __builtin_amdgcn_rsqf(r2);
   const float q = r2 * irl * K.inv_h;
   const float f = 1.f - q;
   const bool inner = q <= 0.5f;
-  const float cin = K.l_ih2 * (3.f * q - 2.f);
-  const float cout = -K.l_ih * (f * f) * irl;
-  const float ein = 3.f * K.l_ih2 * K.inv_h * irl;
-  const float eout = K.l_ih * irl * irl * (2.f * f * K.inv_h + f * f * irl);
+  const float cin = fmaf(q, 3.f * K.l_ih2, -2.f * K.l_ih2);
+  const float g = (f * f) * (K.l_ih * irl);
+  const float cout = -g;
+  const float ein = (3.f * K.l_ih2 * K.inv_h) * irl;
+  const float eout = fmaf(2.f * K.l_ih2, f, g) * (irl * irl);
   const bool valid = r2 > 1.0e-10f && q <= 1.f;
   PairCE o;
   o.c = valid ? (inner ? cin : cout) : 0.f;

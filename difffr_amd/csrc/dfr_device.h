@@ -122,7 +122,7 @@ __device__ inline float cubic_grad_c(const KernelConst& K, float r2) {
   const float irl = __builtin_amdgcn_rsqf(r2);
   const float q = r2 * irl * K.inv_h;
   const float f = 1.f - q;
-  const float cin = K.l_ih2 * (3.f * q - 2.f);
+  const float cin = fmaf(q, 3.f * K.l_ih2, -2.f * K.l_ih2);
   const float cout = -K.l_ih * (f * f) * irl;
   const float cc = (q <= 0.5f) ? cin : cout;
   return (r2 > 1.0e-10f && q <= 1.f) ? cc : 0.f;

@@ -1,0 +1,107 @@
+"""Edge cases of the cell grid and the brick tiles, HIP path against the oracle.
+
+* Compressed fluid (lattice spacing 1.72 r, 1.26x rest density, 26 x 30 x 26):
+  interior brick halos hold 14^3 = 2744 particles, beyond the 2560-slot LDS
+  maximum, so those bricks run every pass from global memory.  Neighbour sets bit-exact; density, factor and
+  sum_grad_p_k within 2e-5 of field scale; one step with 1 divergence and 3
+  pressure iterations imposed on both: positions within 2e-6 m (the step
+  moves particles by up to 1.7 cm; the FP32 and FP64 oracles differ by 3e-7 m).
+* Neighbour-list overflow (spacing 1.4 r: 92 fluid neighbours > 64 slots): the
+  step fails with the library's capacity error instead of truncating.
+* Fluid thrown out of the grid's box (the grid is fixed at set-up with a
+  four-cell margin; positions beyond it are clamped into the edge cells):
+  neighbour sets stay bit-exact.
+The fluid-free scene is covered by the adjoint's free-flight test.
+"""
+import numpy as np
+import pytest
+
+from difffr_amd import scenes
+from difffr_amd.simulation import DfrError, Simulation
+from oracle.pyoracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+
+R = 0.01  # particle radius; support h = 4 R
+
+
+def _lattice_scene(spacing, counts, v=(0.0, 0.0, 0.0), wall_height=None, name="lattice"):
+    """A regular fluid block (no jitter: lattice distances are no ties for
+    h = 4 R) in a static open box sampled at 2 R."""
+    fluid = scenes.block_lattice((0.0, 0.0, 0.0), counts, spacing)
+    inner = spacing * (np.asarray(counts, np.float64) + 1)
+    height = wall_height if wall_height is not None else 2.0 * inner[1]
+    walls = scenes.box_surface((0.0, 0.0, 0.0), (inner[0], height, inner[2]), 2 * R, open_top=True)
+    bodies = [scenes.Body(walls, False, np.zeros(3), np.array([1.0, 0, 0, 0]), 1000.0, name="tank")]
+    vel = np.tile(np.asarray(v, np.float64), (fluid.shape[0], 1))
+    return scenes.Scene(name, scenes.default_params(R), fluid, vel, bodies, 4)
+
+
+def _sets(lists):
+    return [set(int(x) for x in l) for l in lists]
+
+
+def test_halos_beyond_lds_maximum(gpu_ready):
+    sc = _lattice_scene(1.72 * R, (26, 30, 26), name="compressed")
+    print(f"[compressed] nf={sc.n_fluid}", flush=True)
+    g = Simulation(sc)
+    g.set_fixed_iterations([1], [3])  # the compressed block would take the 100-iteration cap
+    g.step(1)
+    st = g.halo_stats()
+    print("  halo stats", st, flush=True)
+    assert st["largest_halo"] > st["max_capacity"] and st["capacity"] == st["max_capacity"]
+    o = OracleSim(sc, "f32_rigid64")
+    o.stage(0)
+    gl, ol = g.neighbor_lists(0), o.neighbor_lists(0)
+    assert max(len(l) for l in gl) <= 64
+    assert _sets(gl) == _sets(ol)
+    for name in ("density", "factor", "sum_grad_p_k"):
+        a, b = g.field(name), o.field(name)
+        err = np.abs(a - b).max() / np.abs(b).max()
+        print(f"  {name}: max rel-to-scale err {err:.2e}", flush=True)
+        assert err < 2e-5, name
+    o2 = OracleSim(sc, "f32_rigid64")
+    o2.set_fixed_iterations([1], [3])
+    o2.step(1)
+    xg, _ = g.fluid()
+    xo, _ = o2.fluid()
+    print("  max |dx|", np.abs(xg - xo).max(), flush=True)
+    assert np.abs(xg - xo).max() < 2e-6
+    g.close()
+
+
+def test_neighbour_overflow_is_an_error(gpu_ready):
+    sc = _lattice_scene(1.4 * R, (12, 8, 12), name="overcompressed")
+    print(f"[overflow] nf={sc.n_fluid}", flush=True)
+    g = Simulation(sc)
+    with pytest.raises(DfrError, match="neighbour list capacity exceeded"):
+        g.step(1)
+    g.close()
+
+
+def test_fluid_outside_the_grid_box(gpu_ready):
+    # walls as high as the fluid; thrown upward at 8 m/s, the top layers leave the
+    # grid box within 60 steps
+    counts = (10, 6, 10)
+    sp = 1.86 * R
+    top = sp * counts[1]
+    sc = _lattice_scene(sp, counts, v=(0.0, 8.0, 0.0), wall_height=top + sp, name="thrown")
+    print(f"[outside grid] nf={sc.n_fluid}", flush=True)
+    g = Simulation(sc)
+    g.step(60)
+    x, _ = g.fluid()
+    # the grid's top along y: cells of 1.001 h, four cells of margin, cell count a multiple of 4
+    cs = 4 * R * 1.001
+    ys = np.concatenate([sc.fluid_x[:, 1], sc.bodies[0].x0[:, 1]])
+    lo = ys.min() - 4 * cs
+    n = int(np.ceil((ys.max() + 4 * cs - lo) / cs)) + 1
+    top_of_grid = lo + ((n + 3) & ~3) * cs
+    out = int((x[:, 1] > top_of_grid + cs).sum())
+    print("  particles above the grid box:", out, "of", x.shape[0], flush=True)
+    assert out > 0
+    g.step(1)  # lists of this step are built on x
+    sc2 = scenes.Scene("thrown_state", sc.params, x.astype(np.float64), np.zeros_like(x, np.float64), sc.bodies, 1)
+    o = OracleSim(sc2, "f32_rigid64")
+    o.stage(0)
+    assert _sets(g.neighbor_lists(0)) == _sets(o.neighbor_lists(0))
+    g.close()

@@ -145,9 +145,8 @@ k_adj_update_t(AdjPass a, const float* __restrict__ k, const float4* __restrict_
     float kb = dtVf * kbf;
     float xbx = dtVf * xfx, xby = dtVf * xfy, xbz = dtVf * xfz;
     if (fabsf(ki) > c.eps) {
-      for (uint32_t kq = 0; kq < nB; ++kq) {
-        const uint32_t j = a.nbrB[(size_t)kq * stride + i];
-        const float4 q = a.bx[j];
+      auto load = [&](uint32_t j) { return a.bx[j]; };
+      for_boundary_groups<2>(a.nbrB, stride, i, nB, load, [&](const float4& q, uint32_t j) {
         const float Vb = q.w;
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
         const PairCE ce = cubic_ce(c.K, rx * rx + ry * ry + rz * rz);
@@ -179,7 +178,7 @@ k_adj_update_t(AdjPass a, const float* __restrict__ k, const float4* __restrict_
           sx += fy * Tz - fz * Ty; sy += fz * Tx - fx * Tz; sz += fx * Ty - fy * Tx;
           atomic_add3(a.sxb + 3 * (size_t)sd, sx, sy, sz);
         }
-      }
+      });
     }
     if (kacc) kb += kacc_scale * kacc[i];
     float4 xo = xbar[i];
@@ -286,10 +285,9 @@ k_adj_delta_t(AdjPass a, const float4* __restrict__ v, const float* __restrict__
     float vbx = c.V_f * gx, vby = c.V_f * gy, vbz = c.V_f * gz;
     float xbx = c.V_f * hx, xby = c.V_f * hy, xbz = c.V_f * hz;
     if (di != 0.f) {
-      for (uint32_t kq = 0; kq < nB; ++kq) {
-        const uint32_t j = a.nbrB[(size_t)kq * stride + i];
-        const float4 q = a.bx[j];
-        const float4 vbj = a.bv[j];
+      auto load = [&](uint32_t j) { return Slot8{a.bx[j], a.bv[j]}; };
+      for_boundary_groups<2>(a.nbrB, stride, i, nB, load, [&](const Slot8& s8, uint32_t j) {
+        const float4 q = s8.a, vbj = s8.b;
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
         const PairCE ce = cubic_ce(c.K, rx * rx + ry * ry + rz * rz);
         const float sc = di * q.w * ce.c;
@@ -304,7 +302,7 @@ k_adj_delta_t(AdjPass a, const float4* __restrict__ v, const float* __restrict__
           atomic_add3(a.svb + 3 * (size_t)sd, -gbx, -gby, -gbz);
           atomic_add3(a.sxb + 3 * (size_t)sd, -tbx, -tby, -tbz);
         }
-      }
+      });
     }
     float4 vo = vbar[i];
     vo.x += vbx; vo.y += vby; vo.z += vbz;
@@ -402,9 +400,8 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
     // rbb: adjoint of the boundary part of the density alone (a warm-start feature)
     const float rib = ri + (rbb ? rbb[i] : 0.f);
     if (Si != 0.f || rib != 0.f) {
-      for (uint32_t kq = 0; kq < nB; ++kq) {
-        const uint32_t j = a.nbrB[(size_t)kq * stride + i];
-        const float4 q = a.bx[j];
+      auto load = [&](uint32_t j) { return a.bx[j]; };
+      for_boundary_groups<2>(a.nbrB, stride, i, nB, load, [&](const float4& q, uint32_t j) {
         const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
         const PairCE ce = cubic_ce(c.K, rx * rx + ry * ry + rz * rz);
         const float sr = rib * q.w * ce.c;
@@ -415,7 +412,7 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
         const float tx = sr * rx - hx, ty = sr * ry - hy, tz = sr * rz - hz;
         xbx += tx; xby += ty; xbz += tz;
         if ((int)j >= a.nb_static) atomic_add3(a.sxb + 3 * (size_t)(j - (uint32_t)a.nb_static), -tx, -ty, -tz);
-      }
+      });
     }
     float4 xo = xbar[i];
     xo.x += xbx; xo.y += xby; xo.z += xbz;

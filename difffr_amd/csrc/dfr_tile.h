@@ -198,6 +198,27 @@ __device__ inline void for_fluid_groups(const uint4* __restrict__ nbr8, size_t s
   }
 }
 
+// Visits the nB boundary neighbours of particle i in list order, G at a time:
+// the group's sample indices are loaded first, then the group's sample data
+// (`load(j)`), so a group costs two dependent trips to memory instead of two
+// per neighbour; `body(v, j)` then runs for the group's neighbours in order.
+template <uint32_t G = 4, class Load, class Body>
+__device__ inline void for_boundary_groups(const uint32_t* __restrict__ nbrB, size_t stride, uint32_t i,
+                                           uint32_t nB, Load&& load, Body&& body) {
+#pragma unroll 1
+  for (uint32_t k0 = 0; k0 < nB; k0 += G) {
+    uint32_t jj[G];
+#pragma unroll
+    for (uint32_t u = 0; u < G; ++u) jj[u] = nbrB[(size_t)min(k0 + u, nB - 1) * stride + i];
+    decltype(load(0u)) v[G];
+#pragma unroll
+    for (uint32_t u = 0; u < G; ++u) v[u] = load(jj[u]);
+#pragma unroll
+    for (uint32_t u = 0; u < G; ++u)
+      if (k0 + u < nB) body(v[u], jj[u]);
+  }
+}
+
 // Halo slot records in LDS: the fields a pass gathers per neighbour,
 // interleaved so one slot is one or two 16-byte reads.
 struct Slot6 {

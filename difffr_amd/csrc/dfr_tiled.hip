@@ -181,13 +181,20 @@ k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t
           const uint32_t n = bcn[hc];
           if (n == 0) continue;
           const uint32_t st = bcs[hc];
-          for (uint32_t k = 0; k < n; ++k) {
-            const uint32_t j = bperm[st + k];
-            const float4 q = bpos[j];
-            if (dist2_exact(p.x, p.y, p.z, q.x, q.y, q.z) < h2) {
-              if (nB < (uint32_t)maxb) nbrB[(size_t)nB * stride + i] = j;
-              ++nB;
-            }
+          // four samples at a time: their indices, then their positions, then the tests in order
+          for (uint32_t k0 = 0; k0 < n; k0 += 4) {
+            uint32_t jj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) jj[u] = bperm[st + min(k0 + u, n - 1)];
+            float4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = bpos[jj[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (k0 + u < n && dist2_exact(p.x, p.y, p.z, q[u].x, q[u].y, q[u].z) < h2) {
+                if (nB < (uint32_t)maxb) nbrB[(size_t)nB * stride + i] = jj[u];
+                ++nB;
+              }
           }
         }
     if (nF > (uint32_t)MAXF) { atomicMax(&overflow[0], nF); nF = MAXF; }
@@ -286,20 +293,19 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
     gx = c.V_f * fgx; gy = c.V_f * fgy; gz = c.V_f * fgz;
     delta = c.V_f * fdelta;
     float bdens = 0.f;  // boundary part of the density sum (a feature of the learned warm start)
-    for (uint32_t k = 0; k < nB; ++k) {
-      const uint32_t j = a.nbrB[(size_t)k * stride + i];
-      const float4 q = a.bpos[j];
-      const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-      const PairKernel kw = cubic_pair(c.K, rx * rx + ry * ry + rz * rz);
-      dens += q.w * kw.W;
-      bdens += q.w * kw.W;
-      const float wx = kw.c * rx, wy = kw.c * ry, wz = kw.c * rz;
-      gx += q.w * wx; gy += q.w * wy; gz += q.w * wz;
-      if (divergence_on) {
-        const float4 vb = a.bvel[j];
-        delta += q.w * ((vi.x - vb.x) * wx + (vi.y - vb.y) * wy + (vi.z - vb.z) * wz);
-      }
-    }
+    for_boundary_groups(
+        a.nbrB, stride, i, nB,
+        [&](uint32_t j) { return Slot8{a.bpos[j], divergence_on ? a.bvel[j] : make_float4(0.f, 0.f, 0.f, 0.f)}; },
+        [&](const Slot8& s, uint32_t) {
+          const float4 q = s.a, vb = s.b;
+          const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+          const PairKernel kw = cubic_pair(c.K, rx * rx + ry * ry + rz * rz);
+          dens += q.w * kw.W;
+          bdens += q.w * kw.W;
+          const float wx = kw.c * rx, wy = kw.c * ry, wz = kw.c * rz;
+          gx += q.w * wx; gy += q.w * wy; gz += q.w * wz;
+          if (divergence_on) delta += q.w * ((vi.x - vb.x) * wx + (vi.y - vb.y) * wy + (vi.z - vb.z) * wz);
+        });
     rho[i] = dens * c.rho0;
     if (aux) aux[i] = make_float2(delta, bdens);
     s_out[i] = make_float4(-gx, -gy, -gz, 0.f);
@@ -355,84 +361,91 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
   }
   __syncthreads();
   const size_t stride = (size_t)c.nf;
-  int cb = -1;
-  double F0 = 0, F1 = 0, F2 = 0, T0 = 0, T1 = 0, T2 = 0;
   const float invH = 1.f / c.dt;
-  for (uint32_t i = B.own_begin + t; i < B.own_end; i += VB) {
-    const float ki = kfield[i];
-    if (kappa_acc) kappa_acc[i] += ki;
-    const float4 p = a.pos[i];
-    float4 vi = a.vel[i];
-    const uint32_t cc = a.cnt[i];
-    const uint32_t nF = cc & 0xffffu, nB = cc >> 16;
-    // v_i -= dt (k_i + k_j) grad_p_ij = dt V_f (k_i + k_j) c_ij r_ij; dt V_f applied once
-    float ux = 0.f, uy = 0.f, uz = 0.f;
-    // pairs with |k_i + k_j| <= eps (and slots that are not live) add s = 0
-    auto pair = [&](float qx, float qy, float qz, float kj, bool live) {
-      const float kSum = ki + kj;
-      const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
-      const float gc = cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
-      const float s = (live && fabsf(kSum) > c.eps) ? kSum * gc : 0.f;
-      ux += s * rx; uy += s * ry; uz += s * rz;
-    };
-    if (in_lds) {
-      for_fluid_groups<2>(a.nbr8, stride, i, nF, [&](uint32_t s) { return L[s]; },
-                          [&](const float4& q, bool live) { pair(q.x, q.y, q.z, q.w, live); });
-    } else {
-      for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {
-        const uint32_t j = tile_global<VB>(T, s);
-        const float4 q = a.pos[j];
-        pair(q.x, q.y, q.z, kfield[j], true);
-      });
-    }
-    {
-      const float dv = c.dt * c.V_f;
-      vi.x += dv * ux; vi.y += dv * uy; vi.z += dv * uz;
-    }
-    if (fabsf(ki) > c.eps) {
-      for (uint32_t k = 0; k < nB; ++k) {
-        const uint32_t j = a.nbrB[(size_t)k * stride + i];
-        const float4 q = a.bpos[j];
-        const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-        const float sc = cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
-        const float gpx = -q.w * (sc * rx), gpy = -q.w * (sc * ry), gpz = -q.w * (sc * rz);
-        const float am = -c.dt * 1.f * ki;
-        const float dvx = am * gpx, dvy = am * gpy, dvz = am * gpz;
-        vi.x += dvx; vi.y += dvy; vi.z += dvz;
-        if ((int)j >= a.nb_static) {
-          const int body = (int)a.bx0[j].w;
-          if (rig[body].dynamic) {
-            if (body != cb && cb >= 0) {
-              atomicAdd(&rig[cb].F[0], F0); atomicAdd(&rig[cb].F[1], F1); atomicAdd(&rig[cb].F[2], F2);
-              atomicAdd(&rig[cb].T[0], T0); atomicAdd(&rig[cb].T[1], T1); atomicAdd(&rig[cb].T[2], T2);
-              F0 = F1 = F2 = T0 = T1 = T2 = 0.0;
-            }
-            cb = body;
-            const float fx = -c.m_f * dvx * invH, fy = -c.m_f * dvy * invH, fz = -c.m_f * dvz * invH;
-            const double rxd = (double)q.x - rig[body].x[0], ryd = (double)q.y - rig[body].x[1],
-                         rzd = (double)q.z - rig[body].x[2];
-            F0 += fx; F1 += fy; F2 += fz;
-            T0 += ryd * fz - rzd * fy; T1 += rzd * fx - rxd * fz; T2 += rxd * fy - ryd * fx;
-          }
-        }
+  // The particle loop runs the same number of times on every lane of a wave, so
+  // the coupling partials of each round are reduced over the wave at its end
+  // (and are not live across the next round's fluid pair loop).
+  for (uint32_t base = B.own_begin; base < B.own_end; base += VB) {
+    const uint32_t i = base + t;
+    int cb = -1;
+    double F0 = 0, F1 = 0, F2 = 0, T0 = 0, T1 = 0, T2 = 0;
+    if (i < B.own_end) {
+      const float ki = kfield[i];
+      if (kappa_acc) kappa_acc[i] += ki;
+      const float4 p = a.pos[i];
+      float4 vi = a.vel[i];
+      const uint32_t cc = a.cnt[i];
+      const uint32_t nF = cc & 0xffffu, nB = cc >> 16;
+      // v_i -= dt (k_i + k_j) grad_p_ij = dt V_f (k_i + k_j) c_ij r_ij; dt V_f applied once
+      float ux = 0.f, uy = 0.f, uz = 0.f;
+      // pairs with |k_i + k_j| <= eps (and slots that are not live) add s = 0
+      auto pair = [&](float qx, float qy, float qz, float kj, bool live) {
+        const float kSum = ki + kj;
+        const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
+        const float gc = cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
+        const float s = (live && fabsf(kSum) > c.eps) ? kSum * gc : 0.f;
+        ux += s * rx; uy += s * ry; uz += s * rz;
+      };
+      if (in_lds) {
+        for_fluid_groups<2>(a.nbr8, stride, i, nF, [&](uint32_t s) { return L[s]; },
+                            [&](const float4& q, bool live) { pair(q.x, q.y, q.z, q.w, live); });
+      } else {
+        for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {
+          const uint32_t j = tile_global<VB>(T, s);
+          const float4 q = a.pos[j];
+          pair(q.x, q.y, q.z, kfield[j], true);
+        });
       }
+      {
+        const float dv = c.dt * c.V_f;
+        vi.x += dv * ux; vi.y += dv * uy; vi.z += dv * uz;
+      }
+      if (fabsf(ki) > c.eps) {
+        auto load = [&](uint32_t j) { return a.bpos[j]; };
+        for_boundary_groups<2>(a.nbrB, stride, i, nB, load, [&](const float4& q, uint32_t j) {
+          const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+          const float sc = cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
+          const float gpx = -q.w * (sc * rx), gpy = -q.w * (sc * ry), gpz = -q.w * (sc * rz);
+          const float am = -c.dt * 1.f * ki;
+          const float dvx = am * gpx, dvy = am * gpy, dvz = am * gpz;
+          vi.x += dvx; vi.y += dvy; vi.z += dvz;
+          if ((int)j >= a.nb_static) {
+            const int body = (int)a.bx0[j].w;
+            if (rig[body].dynamic) {
+              if (body != cb && cb >= 0) {
+                atomicAdd(&rig[cb].F[0], F0); atomicAdd(&rig[cb].F[1], F1); atomicAdd(&rig[cb].F[2], F2);
+                atomicAdd(&rig[cb].T[0], T0); atomicAdd(&rig[cb].T[1], T1); atomicAdd(&rig[cb].T[2], T2);
+                F0 = F1 = F2 = T0 = T1 = T2 = 0.0;
+              }
+              cb = body;
+              const float fx = -c.m_f * dvx * invH, fy = -c.m_f * dvy * invH, fz = -c.m_f * dvz * invH;
+              const double rxd = (double)q.x - rig[body].x[0], ryd = (double)q.y - rig[body].x[1],
+                           rzd = (double)q.z - rig[body].x[2];
+              F0 += fx; F1 += fy; F2 += fz;
+              T0 += ryd * fz - rzd * fy; T1 += rzd * fx - rxd * fz; T2 += rxd * fy - ryd * fx;
+            }
+          }
+        });
+      }
+      a.vel[i] = vi;
     }
-    a.vel[i] = vi;
-  }
-  // Wave-level FP64 reduction of the coupling partials, one body at a time.
-  while (true) {
-    const unsigned long long m = __ballot(cb >= 0);
-    if (m == 0ull) break;
-    const int src = __ffsll((long long)m) - 1;
-    const int b0 = __shfl(cb, src, 64);
-    const bool mine = (cb == b0);
-    const double a0 = wave_sum_d(mine ? F0 : 0.0), a1 = wave_sum_d(mine ? F1 : 0.0), a2 = wave_sum_d(mine ? F2 : 0.0);
-    const double a3 = wave_sum_d(mine ? T0 : 0.0), a4 = wave_sum_d(mine ? T1 : 0.0), a5 = wave_sum_d(mine ? T2 : 0.0);
-    if ((threadIdx.x & 63) == 0) {
-      atomicAdd(&rig[b0].F[0], a0); atomicAdd(&rig[b0].F[1], a1); atomicAdd(&rig[b0].F[2], a2);
-      atomicAdd(&rig[b0].T[0], a3); atomicAdd(&rig[b0].T[1], a4); atomicAdd(&rig[b0].T[2], a5);
+    // Wave-level FP64 reduction of the round's coupling partials, one body at a time.
+    while (true) {
+      const unsigned long long m = __ballot(cb >= 0);
+      if (m == 0ull) break;
+      const int src = __ffsll((long long)m) - 1;
+      const int b0 = __shfl(cb, src, 64);
+      const bool mine = (cb == b0);
+      const double a0 = wave_sum_d(mine ? F0 : 0.0), a1 = wave_sum_d(mine ? F1 : 0.0),
+                   a2 = wave_sum_d(mine ? F2 : 0.0);
+      const double a3 = wave_sum_d(mine ? T0 : 0.0), a4 = wave_sum_d(mine ? T1 : 0.0),
+                   a5 = wave_sum_d(mine ? T2 : 0.0);
+      if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&rig[b0].F[0], a0); atomicAdd(&rig[b0].F[1], a1); atomicAdd(&rig[b0].F[2], a2);
+        atomicAdd(&rig[b0].T[0], a3); atomicAdd(&rig[b0].T[1], a4); atomicAdd(&rig[b0].T[2], a5);
+      }
+      if (mine) cb = -1;
     }
-    if (mine) cb = -1;
   }
 }
 
@@ -505,14 +518,13 @@ k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, 
       });
     }
     delta = c.V_f * fdelta;
-    for (uint32_t k = 0; k < nB; ++k) {
-      const uint32_t j = a.nbrB[(size_t)k * stride + i];
-      const float4 q = a.bpos[j];
-      const float4 vb = a.bvel[j];
-      const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-      const float sc = cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
-      delta += q.w * sc * ((vi.x - vb.x) * rx + (vi.y - vb.y) * ry + (vi.z - vb.z) * rz);
-    }
+    for_boundary_groups(a.nbrB, stride, i, nB, [&](uint32_t j) { return Slot8{a.bpos[j], a.bvel[j]}; },
+                        [&](const Slot8& s, uint32_t) {
+                          const float4 q = s.a, vb = s.b;
+                          const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+                          const float sc = cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
+                          delta += q.w * sc * ((vi.x - vb.x) * rx + (vi.y - vb.y) * ry + (vi.z - vb.z) * rz);
+                        });
     float b;
     if (mode == 0) {
       b = fmaxf(delta, 0.f);

@@ -420,26 +420,6 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
   }
 }
 
-// Adjoint of the learned warm start's features (f = rho/rho0 - 1, dt delta(v0),
-// neighbour count, boundary density, kappa_prev, kappa_v_prev) given
-// fbar = dL/df from the network's input gradient: kappa / kappa_v of the
-// previous step get fbar[4] / fbar[5] (plus the pass-through when the solve did
-// not overwrite them), rho/rho0 gets fbar[0], the divergence pass at v0 gets
-// dt fbar[1], the boundary density gets fbar[3].
-__global__ void k_adj_learned(int n, const float* __restrict__ fbar, int keep_kappa, int keep_kappav, float dt,
-                              float* __restrict__ kb, float* __restrict__ kvb, float* __restrict__ rhob,
-                              float* __restrict__ dbar, float* __restrict__ rbb) {
-  const uint32_t i = gid();
-  if (i >= (uint32_t)n) return;
-  const float* f = fbar + 6 * (size_t)i;
-  kb[i] = f[4] + (keep_kappa ? kb[i] : 0.f);
-  kvb[i] = f[5] + (keep_kappav ? kvb[i] : 0.f);
-  rhob[i] += f[0];
-  dbar[i] = dt * f[1];
-  rbb[i] = f[3];
-}
-
-// x' = x + dt v: the adjoint of the final velocity receives dt xbar.
 __global__ void k_adj_position(int n, float dt, const float4* __restrict__ xbar, float4* __restrict__ vbar) {
   const uint32_t i = gid();
   if (i >= (uint32_t)n) return;
@@ -665,13 +645,6 @@ void launch_adj_factor(const AdjPass& a, const float* alpha, const float* alpha_
                        const float4* G, float4* xbar, const float* rbb, hipStream_t s) {
   hipLaunchKernelGGL(k_adj_factor_t, dim3(nbricks(a.g)), dim3(RB), shm_slots(a, 8), s, a, alpha, alpha_bar, rho_bar, G,
                      xbar, rbb);
-}
-
-void launch_adj_learned(int n, const float* fbar, int keep_kappa, int keep_kappav, float dt, float* kb, float* kvb,
-                        float* rhob, float* dbar, float* rbb, hipStream_t s) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_adj_learned, dim3(nblk(n)), dim3(DFR_BLOCK), 0, s, n, fbar, keep_kappa, keep_kappav, dt, kb,
-                     kvb, rhob, dbar, rbb);
 }
 
 void launch_adj_position(int n, float dt, const float4* xbar, float4* vbar, hipStream_t s) {

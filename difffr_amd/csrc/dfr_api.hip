@@ -127,7 +127,6 @@ struct MlpState {
   MlpNorm norm{};
   bool has_params = false, norm_set = false, enabled = false;
   int adam_t = 0;
-  float* pred = nullptr;                   // [n][2] predictions of the current step
   float* kpred = nullptr;                  // [n] predicted kappa (stored units)
   float2* aux = nullptr;                   // [n] (velocity divergence, boundary density)
   // training pairs
@@ -170,7 +169,7 @@ struct AdjState {
   float4 *xb = nullptr, *vb = nullptr, *xb2 = nullptr, *vb2 = nullptr;
   float *kb = nullptr, *kvb = nullptr, *kb2 = nullptr, *kvb2 = nullptr;
   float *dbar = nullptr, *alb = nullptr, *rhob = nullptr;
-  float *ybar = nullptr, *fbar = nullptr, *rbb = nullptr;  // learned warm start: prediction / feature adjoints
+  float *ybar = nullptr, *rbb = nullptr;  // learned warm start: prediction adjoint, boundary-density adjoint
   double *sxb = nullptr, *svb = nullptr;
   AdjBodyDev* body = nullptr;
   StepTapeDev tape;
@@ -690,9 +689,8 @@ void do_step(dfr_ctx* c) {
   if (learned && nf > 0) {
     // learned warm starts replace warmstartDivergenceSolve's and warmstartPressureSolve's halved kappa
     Timed t(c, TC_MLP);
-    launch_mlp_infer_fields(ml.frag, ml.P, ml.norm, fields, nf, ml.pred, c->stream);
-    launch_mlp_apply_div(nf, ml.pred, (divOn && warmV) ? c->f.badv : nullptr, 1.f / dt, c->f.kappav, ml.kpred,
-                         c->stream);
+    launch_mlp_infer_apply(ml.frag, ml.P, ml.norm, fields, nf, (divOn && warmV) ? c->f.badv : nullptr, 1.f / dt,
+                           c->f.kappav, ml.kpred, c->stream);
   }
   if (rec && nf > 0) {
     Batch bt(c);
@@ -902,7 +900,7 @@ void adjoint_buffers(dfr_ctx* c) {
   A.xb = dalloc<float4>(c, nf); A.vb = dalloc<float4>(c, nf); A.xb2 = dalloc<float4>(c, nf); A.vb2 = dalloc<float4>(c, nf);
   A.kb = dalloc<float>(c, nf); A.kvb = dalloc<float>(c, nf); A.kb2 = dalloc<float>(c, nf); A.kvb2 = dalloc<float>(c, nf);
   A.dbar = dalloc<float>(c, nf); A.alb = dalloc<float>(c, nf); A.rhob = dalloc<float>(c, nf);
-  A.ybar = dalloc<float>(c, 2 * nf); A.fbar = dalloc<float>(c, 6 * nf); A.rbb = dalloc<float>(c, nf);
+  A.ybar = dalloc<float>(c, 2 * nf); A.rbb = dalloc<float>(c, nf);
   A.sxb = dalloc<double>(c, 3 * nd); A.svb = dalloc<double>(c, 3 * nd);
   A.body = dalloc<AdjBodyDev>(c, nr);
   StepTapeDev& T = A.tape;
@@ -1111,10 +1109,9 @@ void reverse_step(dfr_ctx* c, const StepTapeDev& T) {
       // through the network to its features, then to the state they were computed from
       Timed t(c, TC_ADJ_MLP);
       const MlpFields fields{T.rho, T.aux, T.cnt, T.kPrev, T.kvPrev, c->sc.rho0, dt};
-      launch_mlp_input_grad(c->mlp.frag, c->mlp.P, c->mlp.norm, nullptr, &fields, nf, A.ybar, A.fbar, c->stream);
       const bool divWarm = c->P.enable_divergence_solver && c->P.use_divergence_warmstart;
-      launch_adj_learned(nf, A.fbar, warmP ? 0 : 1, divWarm ? 0 : 1, dt, A.kb, A.kvb, A.rhob, A.dbar, A.rbb,
-                         c->stream);
+      const LearnedAdjSink sink{A.kb, A.kvb, A.rhob, A.dbar, A.rbb, warmP ? 0 : 1, divWarm ? 0 : 1, dt};
+      launch_mlp_input_grad_apply(c->mlp.frag, c->mlp.P, c->mlp.norm, fields, nf, A.ybar, sink, c->stream);
       if (c->P.enable_divergence_solver) launch_adj_delta(a, T.v0, A.dbar, A.vb, A.xb, c->stream);
     }
     {
@@ -1165,7 +1162,6 @@ void mlp_alloc(dfr_ctx* c) {
   HIPCHK(hipMemsetAsync(M.v, 0, sizeof(float) * MLP_NP, c->stream));
   M.frag = dalloc<char>(c, mlp_frag_bytes());
   const size_t nf = (size_t)std::max(c->f.n, 1);
-  M.pred = dalloc<float>(c, 2 * nf);
   M.kpred = dalloc<float>(c, nf);
   M.aux = dalloc<float2>(c, nf);
   M.count = dalloc<uint32_t>(c, 2);

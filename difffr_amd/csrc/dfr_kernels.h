@@ -226,8 +226,6 @@ void launch_adj_delta(const AdjPass& a, const float4* v, const float* dbar, floa
                       hipStream_t s);
 void launch_adj_factor(const AdjPass& a, const float* alpha, const float* alpha_bar, const float* rho_bar,
                        const float4* G, float4* xbar, const float* rbb, hipStream_t s);
-void launch_adj_learned(int n, const float* fbar, int keep_kappa, int keep_kappav, float dt, float* kb, float* kvb,
-                        float* rhob, float* dbar, float* rbb, hipStream_t s);
 void launch_adj_position(int n, float dt, const float4* xbar, float4* vbar, hipStream_t s);
 void launch_adj_unpermute(int n, const uint32_t* perm, const float4* xb, const float4* vb, const float* kb,
                           const float* kvb, float4* xb2, float4* vb2, float* kb2, float* kvb2, hipStream_t s);
@@ -274,8 +272,20 @@ void launch_mlp_capture_features(const MlpFields& f, const uint32_t* id, int n, 
                                  float* feat, uint32_t* rows, uint32_t* count, uint32_t cap, hipStream_t s);
 void launch_mlp_capture_targets(const float* kappa, const float* kappav, const uint32_t* rows, const uint32_t* first,
                                 uint32_t max_new, const uint32_t* count, uint32_t cap, float* targ, hipStream_t s);
-void launch_mlp_apply_div(int n, const float* pred, const float* d, float inv_dt, float* kappav, float* kpred,
-                          hipStream_t s);
+// Inference on the step's fields that applies the warm starts directly:
+// kappa_v = [d > 0] y_v / dt (d == nullptr: kappa_v untouched), kpred = y_p.
+void launch_mlp_infer_apply(const void* frag, const float* P, const MlpNorm& norm, const MlpFields& f, int n,
+                            const float* d, float inv_dt, float* kappav, float* kpred, hipStream_t s);
+// Input gradient on the step's fields that applies the feature adjoints
+// directly: rho_bar += fbar[0], dbar = dt fbar[1], rbb = fbar[3],
+// kb = fbar[4] (+ kb if keep_kappa), kvb = fbar[5] (+ kvb if keep_kappav).
+struct LearnedAdjSink {
+  float *kb, *kvb, *rhob, *dbar, *rbb;
+  int keep_kappa, keep_kappav;
+  float dt;
+};
+void launch_mlp_input_grad_apply(const void* frag, const float* P, const MlpNorm& norm, const MlpFields& f, int n,
+                                 const float* ybar, const LearnedAdjSink& sink, hipStream_t s);
 void set_mlp_kernel_attributes();
 
 }  // namespace dfr

@@ -126,6 +126,13 @@ struct MlpArgs {
   double* loss;               // TRAIN: sum of squared normalised errors
   const float* ybar;          // INPUT_GRAD: [n][2] dL/d(prediction)
   float* fbar;                // INPUT_GRAD: [n][6] dL/d(raw features)
+  // INFER on the step's fields, out == nullptr: the warm starts written directly
+  const float* dmask;         // d of the divergence warm start (nullptr: kappa_v untouched)
+  float inv_dt;
+  float* kappav;
+  float* kpred;
+  // INPUT_GRAD, fbar == nullptr: the feature adjoints applied directly
+  LearnedAdjSink sink;
 };
 
 // LDS layout (bytes) per mode.  Layer outputs alternate between two
@@ -480,8 +487,17 @@ k_mlp_chain(MlpArgs a) {
     y = mfma(frag(MLP_FW[3] + 1), B[1], y);
     if (!IG) {
       if (valid && lg == 0) {
-        a.out[(size_t)p * 2 + 0] = y[0] * N.sd_out[0] + N.mu_out[0];
-        a.out[(size_t)p * 2 + 1] = y[1] * N.sd_out[1] + N.mu_out[1];
+        const float yv = y[0] * N.sd_out[0] + N.mu_out[0], yp = y[1] * N.sd_out[1] + N.mu_out[1];
+        if (a.out) {
+          a.out[(size_t)p * 2 + 0] = yv;
+          a.out[(size_t)p * 2 + 1] = yp;
+        } else {
+          // the learned warm starts: kappa_v = y_v / dt where d > 0 (warmstartDivergenceSolve's
+          // mask; no divergence warm start: kappa_v untouched), kappa prediction kept for the
+          // pressure warm start (applied where rho* > 1)
+          if (a.dmask) a.kappav[p] = (a.dmask[p] > 0.f) ? yv * a.inv_dt : 0.f;
+          a.kpred[p] = yp;
+        }
       }
       continue;
     }
@@ -517,10 +533,26 @@ k_mlp_chain(MlpArgs a) {
     din = mfma(frag(MLP_FT[0] + 0), B[0], din);
     din = mfma(frag(MLP_FT[0] + 1), B[1], din);
     if (valid && lg < 2) {
+      if (a.fbar) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int k = 4 * lg + j;
-        if (k < 6) a.fbar[(size_t)p * 6 + k] = din[j] * N.is_in[k];
+        for (int j = 0; j < 4; ++j) {
+          const int k = 4 * lg + j;
+          if (k < 6) a.fbar[(size_t)p * 6 + k] = din[j] * N.is_in[k];
+        }
+      } else {
+        // the feature adjoints: rho/rho0 gets fbar[0], the divergence pass at v0 gets
+        // dt fbar[1], the boundary density fbar[3], kappa / kappa_v of the previous step
+        // fbar[4] / fbar[5] (plus the pass-through when the solve did not overwrite
+        // them); lane group 0 holds features 0..3, group 1 features 4 and 5
+        const LearnedAdjSink& S = a.sink;
+        if (lg == 0) {
+          S.rhob[p] += din[0] * N.is_in[0];
+          S.dbar[p] = S.dt * (din[1] * N.is_in[1]);
+          S.rbb[p] = din[3] * N.is_in[3];
+        } else {
+          S.kb[p] = din[0] * N.is_in[4] + (S.keep_kappa ? S.kb[p] : 0.f);
+          S.kvb[p] = din[1] * N.is_in[5] + (S.keep_kappav ? S.kvb[p] : 0.f);
+        }
       }
     }
   }
@@ -585,19 +617,6 @@ __global__ void k_mlp_capture_targets(const float* __restrict__ kappa, const flo
   const uint32_t i = rows[s];
   targ[(size_t)s * 2 + 0] = kappav[i];
   targ[(size_t)s * 2 + 1] = kappa[i];
-}
-
-// Learned warm starts from the predictions (stored units): kappa_v = pred / dt
-// where d > 0 (warmstartDivergenceSolve's mask; d == nullptr: no divergence
-// warm start, kappa_v untouched), kappa prediction kept for the pressure warm
-// start (applied where rho* > 1).
-__global__ void k_mlp_apply_div(int n, const float* __restrict__ pred, const float* __restrict__ d, float inv_dt,
-                                float* __restrict__ kappav, float* __restrict__ kpred) {
-  const uint32_t i = gid();
-  if (i >= (uint32_t)n) return;
-  const float2 y = reinterpret_cast<const float2*>(pred)[i];
-  if (d) kappav[i] = (d[i] > 0.f) ? y.x * inv_dt : 0.f;
-  kpred[i] = y.y;
 }
 
 // ---------------------------------------------------------------------------
@@ -692,11 +711,28 @@ void launch_mlp_capture_targets(const float* kappa, const float* kappav, const u
                      count, cap, targ);
 }
 
-void launch_mlp_apply_div(int n, const float* pred, const float* d, float inv_dt, float* kappav, float* kpred,
-                          hipStream_t s) {
+void launch_mlp_infer_apply(const void* frag, const float* P, const MlpNorm& norm, const MlpFields& f, int n,
+                            const float* d, float inv_dt, float* kappav, float* kpred, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_mlp_apply_div, dim3((n + DFR_BLOCK - 1) / DFR_BLOCK), dim3(DFR_BLOCK), 0, s, n, pred, d,
-                     inv_dt, kappav, kpred);
+  MlpArgs a = base_args(frag, P, norm, n);
+  a.fields = f;
+  a.dmask = d;
+  a.inv_dt = inv_dt;
+  a.kappav = kappav;
+  a.kpred = kpred;
+  hipLaunchKernelGGL(k_mlp_chain<false>, dim3(nb_chain<false>(n)), dim3(chain_block<false>()),
+                     ChainLds<false>::BYTES, s, a);
+}
+
+void launch_mlp_input_grad_apply(const void* frag, const float* P, const MlpNorm& norm, const MlpFields& f, int n,
+                                 const float* ybar, const LearnedAdjSink& sink, hipStream_t s) {
+  if (n <= 0) return;
+  MlpArgs a = base_args(frag, P, norm, n);
+  a.fields = f;
+  a.ybar = ybar;
+  a.sink = sink;
+  hipLaunchKernelGGL(k_mlp_chain<true>, dim3(nb_chain<true>(n)), dim3(chain_block<true>()),
+                     ChainLds<true>::BYTES, s, a);
 }
 
 void set_mlp_kernel_attributes() {

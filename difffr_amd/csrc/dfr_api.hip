@@ -210,6 +210,8 @@ struct dfr_ctx {
   // (divergence, pressure), per-iteration probes in mapped pinned memory,
   // and two events the host waits on one iteration late
   int* d_stop = nullptr;
+  double* d_part = nullptr;          // per-brick residual partials of the current recompute launch
+  int nparts = 0;
   SolveProbe* h_probe = nullptr;
   SolveProbe* d_probe = nullptr;
   int probe_cap = 0;
@@ -577,6 +579,8 @@ int run_solve(dfr_ctx* c, int solve, int fixed, int min_it, int max_it, double e
     int it = 0;
     while ((!chk || it < min_it) && it < max_it) {
       iter(it, nullptr);
+      launch_solve_check(c->d_part, c->nparts, err0 + std::min(it, max_it - 1), n, eta, it, min_it, max_it,
+                         nullptr, nullptr, c->stream);
       avg = read_err(c, (int)(err0 - c->d_err) + std::min(it, max_it - 1)) / (double)n;
       chk = avg <= eta;
       ++it;
@@ -593,7 +597,8 @@ int run_solve(dfr_ctx* c, int solve, int fixed, int min_it, int max_it, double e
     const size_t tsz = tape ? tape->v->size() : 0;
     const size_t timers = c->pending.size();
     iter(it, stop);
-    launch_solve_check(err0 + std::min(it, max_it - 1), n, eta, it, min_it, max_it, stop, dp + it, c->stream);
+    launch_solve_check(c->d_part, c->nparts, err0 + std::min(it, max_it - 1), n, eta, it, min_it, max_it, stop,
+                       dp + it, c->stream);
     HIPCHK(hipEventRecord(c->pipe_ev[it & 1], c->stream));
     if (it >= 1) {
       HIPCHK(hipEventSynchronize(c->pipe_ev[(it - 1) & 1]));
@@ -672,7 +677,6 @@ void do_step(dfr_ctx* c) {
   const bool divOn = c->P.enable_divergence_solver != 0;
   const bool warmV = c->P.use_divergence_warmstart != 0;
   const bool warmP = c->P.use_pressure_warmstart != 0;
-  const int scratch_slot = c->err_slots - 1;
   MlpState& ml = c->mlp;
   const bool learned = ml.enabled;
   if (nf > 0) {
@@ -710,8 +714,7 @@ void do_step(dfr_ctx* c) {
     }
     {
       Timed t(c, TC_RECOMPUTE);
-      launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_DIV, invH, c->d_err + scratch_slot,
-                         c->stream);
+      launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_DIV, invH, nullptr, c->stream);
     }
     const double eta = (1.0f / dt) * (float)c->P.max_error_v * 0.01f * c->sc.rho0;
     const int maxv = c->P.max_iterations_v;
@@ -727,8 +730,8 @@ void do_step(dfr_ctx* c) {
                                      warmV ? c->f.kappav : nullptr, c->stream, stop);
           }
           Timed t(c, TC_RECOMPUTE);
-          launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_DIV, invH,
-                             c->d_err + std::min(m, maxv - 1), c->stream, nullptr, stop);
+          launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_DIV, invH, c->d_part, c->stream,
+                             nullptr, stop);
         },
         rec ? &tv : nullptr, avgv);
   }
@@ -759,8 +762,7 @@ void do_step(dfr_ctx* c) {
     }
     {
       Timed t(c, TC_RECOMPUTE);
-      launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_PRESSURE, invH2,
-                         c->d_err + scratch_slot, c->stream);
+      launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_PRESSURE, invH2, nullptr, c->stream);
     }
     const double eta = (float)c->P.max_error * 0.01f * c->sc.rho0;
     const int base = c->P.max_iterations_v, maxp = c->P.max_iterations;
@@ -776,8 +778,8 @@ void do_step(dfr_ctx* c) {
                                      warmP ? c->f.kappa : nullptr, c->stream, stop);
           }
           Timed t(c, TC_RECOMPUTE);
-          launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_PRESSURE, invH2,
-                             c->d_err + base + std::min(m, maxp - 1), c->stream, nullptr, stop);
+          launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_PRESSURE, invH2, c->d_part,
+                             c->stream, nullptr, stop);
         },
         rec ? &tp : nullptr, avg);
     Timed t(c, TC_ELEMWISE);
@@ -1452,6 +1454,8 @@ int dfr_init(dfr_ctx* c) {
     HIPCHK(hipHostMalloc((void**)&c->h_overflow, sizeof(uint32_t) * 4, hipHostMallocDefault));
     c->h_overflow[0] = c->h_overflow[1] = c->h_overflow[2] = c->h_overflow[3] = 0;
     c->d_stop = dalloc<int>(c, 2);
+    c->nparts = (int)(c->grid.bn[0] * c->grid.bn[1] * c->grid.bn[2]);
+    c->d_part = dalloc<double>(c, c->nparts);
     c->probe_cap = std::max(c->P.max_iterations, c->P.max_iterations_v) + 1;
     HIPCHK(hipHostMalloc((void**)&c->h_probe, sizeof(SolveProbe) * 2 * c->probe_cap,
                          hipHostMallocMapped | hipHostMallocCoherent));

@@ -142,8 +142,11 @@ struct SolveProbe {
   int done;  // 1: the solve ends after this iteration, 0: it continues
   int pad;
 };
-void launch_solve_check(const double* err, int n, double eta, int it, int min_it, int max_it, int* stop,
-                        SolveProbe* probe, hipStream_t s);
+// Sums the per-brick residual partials of a recompute launch in a fixed order
+// into *err; with stop != nullptr also the iteration's convergence test
+// (skipped when *stop is already set).
+void launch_solve_check(const double* part, int nparts, double* err, int n, double eta, int it, int min_it,
+                        int max_it, int* stop, SolveProbe* probe, hipStream_t s);
 void launch_boundary_volume_impl(const float4* bpos_in, const float4* bx0, int nb, const int* groups, int nbodies,
                                  KernelConst K, float h2, const GridDesc& g, GridBufs& gb, float4* bpos_out,
                                  hipStream_t s);

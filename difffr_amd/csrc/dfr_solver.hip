@@ -77,10 +77,25 @@ __global__ void k_boundary_volume(const float4* __restrict__ bpos_in, const floa
 // One Jacobi iteration's convergence test (divergenceSolve :816-828,
 // pressureSolve :700-712): continue while (avg > eta or it + 1 < min) and
 // it + 1 < max.  probe is host-visible memory.
-__global__ void k_solve_check(const double* __restrict__ err, int n, double eta, int it, int min_it, int max_it,
-                              int* __restrict__ stop, SolveProbe* probe) {
-  if (threadIdx.x != 0 || *stop) return;
-  const double avg = *err / (double)n;
+// The residual is the sum of the recompute launch's per-brick partials, in a
+// fixed order (every run sums the same way).
+constexpr int CHECK_BLOCK = 256;
+__global__ void __launch_bounds__(CHECK_BLOCK)
+k_solve_check(const double* __restrict__ part, int nparts, double* __restrict__ err, int n, double eta, int it,
+              int min_it, int max_it, int* __restrict__ stop, SolveProbe* probe) {
+  if (stop && *stop) return;
+  double v = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += CHECK_BLOCK) v += part[k];
+  __shared__ double w[CHECK_BLOCK / 64];
+  v = wave_sum_d(v);
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double sum = 0.0;
+  for (int k = 0; k < CHECK_BLOCK / 64; ++k) sum += w[k];
+  *err = sum;
+  if (!stop) return;
+  const double avg = sum / (double)n;
   const bool chk = avg <= eta;
   const int done = ((chk && it + 1 >= min_it) || it + 1 >= max_it) ? 1 : 0;
   *stop = done;
@@ -116,9 +131,10 @@ void launch_multi_copy(const CopyList& list, hipStream_t s, const int* stop) {
   hipLaunchKernelGGL(k_multi_copy, dim3(bx, (unsigned)list.n), dim3(DFR_BLOCK), 0, s, list, stop);
 }
 
-void launch_solve_check(const double* err, int n, double eta, int it, int min_it, int max_it, int* stop,
-                        SolveProbe* probe, hipStream_t s) {
-  hipLaunchKernelGGL(k_solve_check, dim3(1), dim3(64), 0, s, err, n, eta, it, min_it, max_it, stop, probe);
+void launch_solve_check(const double* part, int nparts, double* err, int n, double eta, int it, int min_it,
+                        int max_it, int* stop, SolveProbe* probe, hipStream_t s) {
+  hipLaunchKernelGGL(k_solve_check, dim3(1), dim3(CHECK_BLOCK), 0, s, part, nparts, err, n, eta, it, min_it, max_it,
+                     stop, probe);
 }
 
 void launch_advect(const StepConst& c, FluidBufs& f, float kappav_scale, hipStream_t s) {

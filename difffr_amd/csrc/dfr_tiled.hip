@@ -465,8 +465,12 @@ k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, 
   if (stop && *stop) return;
   const uint32_t brick = xcd_block();
   const int cap = a.cap;
+  const bool residual = err && (mode == RECOMPUTE_DIV || mode == RECOMPUTE_PRESSURE);
   BrickView B;
-  if (!brick_open<PB>(a.g, a.fstart, a.thdr, brick, cap, T, B)) return;
+  if (!brick_open<PB>(a.g, a.fstart, a.thdr, brick, cap, T, B)) {
+    if (residual && threadIdx.x == 0) err[brick] = 0.0;
+    return;
+  }
   const uint32_t total = B.total;
   const bool in_lds = B.in_lds;
   float4* LA = lds4;                                          // x, y, z, vx
@@ -546,7 +550,7 @@ k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, 
     }
     badv[i] = b;
   }
-  if (mode == RECOMPUTE_DIV || mode == RECOMPUTE_PRESSURE) block_add_double<PB>(e, err);
+  if (residual) block_sum_double<PB>(e, err + brick);
 }
 
 // Expands tile slots back to sorted fluid indices (diagnostics).

@@ -236,18 +236,3 @@ __device__ inline void block_sum_double(double v, double* out) {
     *out = t;
   }
 }
-
-// Block-level FP64 sum of one value per thread, atomically added to *out.
-template <int NT = DFR_BLOCK>
-__device__ inline void block_add_double(double v, double* out) {
-  __shared__ double part[NT / 64];
-  v = wave_sum_d(v);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) part[wid] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < NT / 64; ++w) t += part[w];
-    if (t != 0.0) atomicAdd(out, t);
-  }
-}

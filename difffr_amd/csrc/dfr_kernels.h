@@ -260,8 +260,6 @@ struct MlpFields {            // the step's fields (sorted order)
 
 size_t mlp_frag_bytes();
 void launch_mlp_pack(const float* P, void* frag, hipStream_t s);
-void launch_mlp_infer_fields(const void* frag, const float* P, const MlpNorm& norm, const MlpFields& f, int n,
-                             float* out, hipStream_t s);
 void launch_mlp_infer_array(const void* frag, const float* P, const MlpNorm& norm, const float* feat, int n,
                             float* out, hipStream_t s);
 void launch_mlp_train_batch(const void* frag, const float* P, const MlpNorm& norm, const float* feat,

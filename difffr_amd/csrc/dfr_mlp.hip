@@ -641,16 +641,6 @@ static MlpArgs base_args(const void* frag, const float* P, const MlpNorm& norm, 
   return a;
 }
 
-void launch_mlp_infer_fields(const void* frag, const float* P, const MlpNorm& norm, const MlpFields& f, int n,
-                             float* out, hipStream_t s) {
-  if (n <= 0) return;
-  MlpArgs a = base_args(frag, P, norm, n);
-  a.fields = f;
-  a.out = out;
-  hipLaunchKernelGGL(k_mlp_chain<false>, dim3(nb_chain<false>(n)), dim3(chain_block<false>()),
-                     ChainLds<false>::BYTES, s, a);
-}
-
 void launch_mlp_infer_array(const void* frag, const float* P, const MlpNorm& norm, const float* feat, int n,
                             float* out, hipStream_t s) {
   if (n <= 0) return;

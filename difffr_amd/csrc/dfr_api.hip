@@ -1562,6 +1562,21 @@ int dfr_set_rigid(dfr_ctx* c, int body, const double in[13]) {
   });
 }
 
+int dfr_get_body(const dfr_ctx* c, int body, double* mass, int* dynamic, int* animated) {
+  if (check_body(c, body)) return DFR_ERR_ARG;
+  if (mass && hipMemcpy(mass, &c->d_rig[body].mass, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    return DFR_ERR_HIP;
+  if (dynamic) *dynamic = c->bodies[body].dynamic ? 1 : 0;
+  if (animated) *animated = c->bodies[body].animated ? 1 : 0;
+  return DFR_OK;
+}
+
+int dfr_set_gradient_mode(dfr_ctx* c, int mode) {
+  if (!c || mode < 0 || mode > 2) return DFR_ERR_ARG;
+  c->P.gradient_mode = mode;
+  return DFR_OK;
+}
+
 int dfr_get_jacobians(const dfr_ctx* c, int body, double out[78]) {
   if (check_body(c, body) || !out) return DFR_ERR_ARG;
   return hipMemcpy(out, &c->d_rig[body].J[0], 78 * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess ? DFR_OK
@@ -1697,6 +1712,37 @@ int dfr_neighbor_stats(const dfr_ctx* c, double out[3]) {
   out[1] = nf ? sb / nf : 0.0;
   out[2] = nd ? sbf / nd : 0.0;
   return DFR_OK;
+}
+
+int64_t dfr_count_1ring(const dfr_ctx* c) {
+  if (!c || !c->initialized) return DFR_ERR_ARG;
+  const int64_t nf = c->f.n, nb = c->b.n;
+  if (nf == 0 || c->step_count == 0) return 0;  // no lists since dfr_init / dfr_reset
+  // owner of each global sample index that belongs to a dynamic body (-1: static)
+  std::vector<int32_t> owner(std::max<int64_t>(nb, 1), -1);
+  int ndyn = 0;
+  for (size_t k = 0; k < c->bodies.size(); ++k) {
+    const BodyInfo& bi = c->bodies[k];
+    if (!bi.dynamic) continue;
+    for (int64_t s = 0; s < bi.n; ++s) owner[bi.offset + s] = ndyn;
+    ++ndyn;
+  }
+  if (ndyn == 0) return 0;
+  const int slots = c->nb.maxb;
+  std::vector<uint32_t> cnt(nf), nbr((size_t)slots * nf);
+  if (hipMemcpy(cnt.data(), c->nb.cnt, nf * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(nbr.data(), c->nb.nbrB, (size_t)slots * nf * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return DFR_ERR_HIP;
+  std::vector<int64_t> seen(ndyn, -1);  // last particle that counted each body
+  int64_t pairs = 0;
+  for (int64_t i = 0; i < nf; ++i) {
+    const uint32_t n = cnt[i] >> 16;
+    for (uint32_t s = 0; s < n; ++s) {
+      const int32_t o = owner[nbr[(size_t)s * nf + i]];
+      if (o >= 0 && seen[o] != i) { seen[o] = i; ++pairs; }
+    }
+  }
+  return pairs;
 }
 
 int64_t dfr_get_neighbors(const dfr_ctx* c, int kind, uint32_t* counts, uint32_t* lists, int64_t cap) {

@@ -25,7 +25,7 @@ EXPORTS = [
     "dfr_set_fixed_iterations", "dfr_mlp_capture", "dfr_mlp_dataset_size", "dfr_mlp_dataset", "dfr_mlp_init_params",
     "dfr_mlp_set_params", "dfr_mlp_get_params", "dfr_mlp_train", "dfr_mlp_enable", "dfr_mlp_predict",
     "dfr_mlp_loss_grad", "dfr_mlp_input_grad", "dfr_set_halo_capacity", "dfr_halo_stats",
-    "dfr_set_profiling_classes",
+    "dfr_set_profiling_classes", "dfr_get_body", "dfr_set_gradient_mode", "dfr_count_1ring",
 ]
 
 MLP_NP = 8898
@@ -93,6 +93,10 @@ def load(path: str | None = None):
     L.dfr_get_stats.argtypes = [P, C.POINTER(Stats)]
     L.dfr_reset_stats.argtypes = [P]
     L.dfr_neighbor_stats.argtypes = [P, dp]
+    L.dfr_get_body.argtypes = [P, C.c_int, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.dfr_set_gradient_mode.argtypes = [P, C.c_int]
+    L.dfr_count_1ring.argtypes = [P]
+    L.dfr_count_1ring.restype = C.c_int64
     L.dfr_set_halo_capacity.argtypes = [P, C.c_int]
     L.dfr_set_profiling_classes.argtypes = [P, C.c_uint32]
     L.dfr_halo_stats.argtypes = [P, C.POINTER(C.c_int64)]

@@ -1,0 +1,617 @@
+"""The `pysplishsplash` module surface that DiffFR's optimisation drivers use,
+over one libdfr_hip.so context.
+
+The reference registers these classes in pySPlisHSPlasH/main.cpp:47-70; the
+drivers (e.g. experiments/rigid_body_trajectory_optimization/python/
+gradient-based-optimize.py:90-358,423-525) use the names below with the same
+arity and meaning, so such a driver runs with `import difffr_amd.pysplishsplash
+as sph` in place of `import pysplishsplash as sph`, given a scene this library
+can build:
+
+    base = sph.Exec.SimulatorBase()
+    base.init(sceneFile=scene, useGui=False, stopAt=...)   # a scenes.Scene
+    base.initSimulation()
+    sim = sph.Simulation.getCurrent()
+    timestep = sim.getTimeStep()                           # TimeStepDiffDFSPH
+    sim.setGradientMode(1)
+    base.setTimeStepCB(callback)                           # after every step
+    base.runSimulation()
+    base.cleanup()
+
+What differs, and why:
+* `init(sceneFile=...)` takes a `difffr_amd.scenes.Scene` or the name of a
+  scene builder in `difffr_amd.scenes`; scene JSON files, state files and the
+  `--param` overrides are not read (NotImplementedError), nor is state
+  exported (`setValueBool(STATE_EXPORT, True)` raises).
+* There is no GUI: `GUI.Simulator_GUI_imgui(base)` constructs an inert object
+  and `runSimulation` runs the headless loop of SimulatorBase::runSimulation
+  (SimulatorBase.cpp:649-683).  With `useGui=False` a negative stopAt is an
+  error there and here; with `useGui=True` the loop runs until the callback
+  ends it (the drivers call `quit()`), or until stopAt when it is positive.
+* The multi-body RigidBodyGradientManager path is not part of this library:
+  `useRigidGradientManager()` is False and `getRigidBodyGradientManager()`
+  raises NotImplementedError.
+* A reference C++ `assert` or crash on a bad index is an IndexError here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import native
+from . import scenes as _scenes
+from .simulation import BoundaryModel as _CoreBoundaryModel
+from .simulation import DfrError
+from .simulation import Simulation as _CoreSimulation
+
+__all__ = ["Exec", "GUI", "Simulation", "TimeManager", "TimeStepDiffDFSPH", "BoundaryModelAkinci2012",
+           "RigidBodyObject", "Utilities", "DfrError"]
+
+_current: dict = {"base": None}
+
+
+def _core() -> _CoreSimulation:
+    base = _current["base"]
+    if base is None or base._core is None:
+        raise DfrError("no current simulation: call SimulatorBase.initSimulation() first")
+    return base._core
+
+
+def _vec(v, n: int) -> np.ndarray:
+    a = np.asarray(v, np.float64).reshape(-1)
+    if a.shape[0] != n:
+        raise ValueError(f"expected {n} values, got {a.shape[0]}")
+    return a.copy()
+
+
+def _quat_matrix(q) -> np.ndarray:
+    """Rotation matrix of the unit quaternion q = (w, x, y, z)."""
+    w, x, y, z = (float(c) for c in q)
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+        [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+        [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)],
+    ])
+
+
+def _quat_mul(a, b) -> np.ndarray:
+    aw, ax, ay, az = a
+    bw, bx, by, bz = b
+    return np.array([aw * bw - ax * bx - ay * by - az * bz,
+                     aw * bx + ax * bw + ay * bz - az * by,
+                     aw * by - ax * bz + ay * bw + az * bx,
+                     aw * bz + ax * by - ay * bx + az * bw])
+
+
+def target_quaternion(angle) -> np.ndarray:
+    """(w, x, y, z) of Rx(a0) * Ry(a1) * Rz(a2), the composition of three axis
+    rotations (SimulationDataDiffDFSPH.h:169-184)."""
+    a = np.asarray(angle, np.float64)
+    qx = np.array([math.cos(a[0] / 2), math.sin(a[0] / 2), 0.0, 0.0])
+    qy = np.array([math.cos(a[1] / 2), 0.0, math.sin(a[1] / 2), 0.0])
+    qz = np.array([math.cos(a[2] / 2), 0.0, 0.0, math.sin(a[2] / 2)])
+    return _quat_mul(_quat_mul(qx, qy), qz)
+
+
+# ---------------------------------------------------------------------------
+# RigidBodyObject (pySPlisHSPlasH/RigidBodyModule.cpp:19-45)
+# ---------------------------------------------------------------------------
+class RigidBodyObject:
+    """The rigid body behind a boundary model.  Rotations cross this interface
+    as Eigen coefficient vectors (x, y, z, w), as the reference's
+    getRotation/setRotation lambdas do (RigidBodyModule.cpp:28-36)."""
+
+    def __init__(self, core: _CoreSimulation, index: int):
+        self._core = core
+        self._index = index
+
+    def _body(self):
+        m = np.zeros(1)
+        d = C.c_int(0)
+        a = C.c_int(0)
+        self._core._check(native.load().dfr_get_body(self._core.ctx, self._index, m.ctypes.data_as(
+            C.POINTER(C.c_double)), C.byref(d), C.byref(a)))
+        return float(m[0]), bool(d.value), bool(a.value)
+
+    def _state(self):
+        return self._core.rigid(self._index)
+
+    def _set(self, **kw):
+        s = self._state()
+        s.update(kw)
+        self._core.set_rigid(self._index, s["x"], s["q"], s["v"], s["omega"])
+
+    def isDynamic(self) -> bool:
+        return self._body()[1]
+
+    def isAnimated(self) -> bool:
+        return self._body()[2]
+
+    def getMass(self) -> float:
+        return self._body()[0]
+
+    def getPosition(self) -> np.ndarray:
+        return self._state()["x"].copy()
+
+    def setPosition(self, x):
+        self._set(x=_vec(x, 3))
+
+    def getWorldSpacePosition(self) -> np.ndarray:
+        return self.getPosition()
+
+    def getVelocity(self) -> np.ndarray:
+        return self._state()["v"].copy()
+
+    def setVelocity(self, v):
+        self._set(v=_vec(v, 3))
+
+    def getAngularVelocity(self) -> np.ndarray:
+        return self._state()["omega"].copy()
+
+    def setAngularVelocity(self, w):
+        self._set(omega=_vec(w, 3))
+
+    def getRotation(self) -> np.ndarray:
+        q = self._state()["q"]
+        return np.array([q[1], q[2], q[3], q[0]])
+
+    def setRotation(self, q_xyzw):
+        c = _vec(q_xyzw, 4)
+        self._set(q=np.array([c[3], c[0], c[1], c[2]]))
+
+    def getWorldSpaceRotation(self) -> np.ndarray:
+        return _quat_matrix(self._state()["q"])
+
+
+# ---------------------------------------------------------------------------
+# BoundaryModelAkinci2012 (pySPlisHSPlasH/BoundaryModelModule.cpp:18-91)
+# ---------------------------------------------------------------------------
+class BoundaryModelAkinci2012(_CoreBoundaryModel):
+    """Per-body getters: the rigid state and forward-mode gradients of the base
+    class plus the sample accessors and velocity setters."""
+
+    def numberOfParticles(self) -> int:
+        return int(self.sim.body_sizes[self.index])
+
+    def _sample(self, i: int, name: str):
+        n = self.numberOfParticles()
+        if not 0 <= i < n:
+            raise IndexError(f"boundary sample {i} out of range [0, {n})")
+        return self.sim.body_field(self.index, name)[i]
+
+    def getPosition(self, i: int) -> np.ndarray:
+        return self._sample(i, "x").astype(np.float64)
+
+    def getVelocity(self, i: int) -> np.ndarray:
+        return self._sample(i, "v").astype(np.float64)
+
+    def getVolume(self, i: int) -> float:
+        return float(self._sample(i, "volume"))
+
+    def getRigidBodyObject(self) -> RigidBodyObject:
+        return RigidBodyObject(self.sim, self.index)
+
+    def getPointVelocity(self, x) -> np.ndarray:
+        """v_rb + omega x (x - x_rb) (BoundaryModel::getPointVelocity)."""
+        s = self._rigid()
+        return s["v"] + np.cross(s["omega"], _vec(x, 3) - s["x"])
+
+    def set_velocity_rb(self, v):
+        self.getRigidBodyObject().setVelocity(v)
+
+    def set_angular_velocity_rb(self, w):
+        self.getRigidBodyObject().setAngularVelocity(w)
+
+
+# ---------------------------------------------------------------------------
+# TimeStepDiffDFSPH (pySPlisHSPlasH/DiffDFSPHModule.cpp:44-102)
+# ---------------------------------------------------------------------------
+class TimeStepDiffDFSPH:
+    """The differentiable time step's Python surface.  Loss values, the learning
+    rate and the log message are host-side bookkeeping as in
+    SimulationDataDiffDFSPH (learning rate 1 at start, SimulationDataDiffDFSPH.cpp:25);
+    the initial rigid velocities and the gradients live in the library."""
+
+    # parameter ids of the reference's ParameterObject (values of this module)
+    ENABLE_COUNT_NEIGHBOR_DOF = 1001
+
+    def __init__(self, base: "Exec.SimulatorBase"):
+        self._base = base
+        core = base._core
+        self._loss = 0.0
+        self._loss_x = 0.0
+        self._loss_rotation = 0.0
+        self._lr = 1.0
+        self._log = ""
+        self._finished = False
+        self._new_trajectory = False
+        self._count_dof = False
+        sc = base._scene
+        self._init_v = [np.asarray(b.v0, np.float64).copy() for b in sc.bodies]
+        self._init_w = [np.asarray(b.w0, np.float64).copy() for b in sc.bodies]
+        self._target_x = [np.asarray(b.target_x, np.float64).copy() for b in sc.bodies]
+        self._target_angle = [np.asarray(b.target_angle, np.float64).copy() for b in sc.bodies]
+        self._core = core
+
+    def _body(self, i: int) -> int:
+        if not 0 <= int(i) < self._core.n_bodies:
+            raise IndexError(f"boundary model {i} out of range [0, {self._core.n_bodies})")
+        return int(i)
+
+    def get_boundary_model(self, i: int) -> BoundaryModelAkinci2012:
+        return BoundaryModelAkinci2012(self._core, self._body(i))
+
+    # losses and learning rate (TimeStepDiffDFSPH.cpp:2157-2195)
+    def get_loss(self) -> float:
+        return self._loss
+
+    def set_loss(self, v: float):
+        self._loss = float(v)
+
+    def get_loss_x(self) -> float:
+        return self._loss_x
+
+    def set_loss_x(self, v: float):
+        self._loss_x = float(v)
+
+    def get_loss_rotation(self) -> float:
+        return self._loss_rotation
+
+    def set_loss_rotation(self, v: float):
+        self._loss_rotation = float(v)
+
+    def get_lr(self) -> float:
+        return self._lr
+
+    def set_lr(self, v: float):
+        self._lr = float(v)
+
+    # initial rigid velocities (TimeStepDiffDFSPH.cpp:2087-2125); beginStep
+    # imposes them on the first steps (:379-429)
+    def set_init_v_rb(self, i: int, v):
+        k = self._body(i)
+        self._init_v[k] = _vec(v, 3)
+        self._core.set_init_velocity(k, self._init_v[k], self._init_w[k])
+
+    def set_init_omega_rb(self, i: int, w):
+        k = self._body(i)
+        self._init_w[k] = _vec(w, 3)
+        self._core.set_init_velocity(k, self._init_v[k], self._init_w[k])
+
+    def get_init_v_rb(self, i: int) -> np.ndarray:
+        return self._init_v[self._body(i)].copy()
+
+    def get_init_omega_rb(self, i: int) -> np.ndarray:
+        return self._init_w[self._body(i)].copy()
+
+    # targets (TimeStepDiffDFSPH.cpp:2137-2155)
+    def get_target_x(self, i: int) -> np.ndarray:
+        return self._target_x[self._body(i)].copy()
+
+    def set_target_x(self, i: int, x):
+        self._target_x[self._body(i)] = _vec(x, 3)
+
+    def get_target_angle_in_radian(self, i: int) -> np.ndarray:
+        return self._target_angle[self._body(i)].copy()
+
+    def get_target_quaternion_vec4(self, i: int) -> np.ndarray:
+        return target_quaternion(self._target_angle[self._body(i)])
+
+    # trajectory bookkeeping (endStep, TimeStepDiffDFSPH.cpp:432-490; :2197-2215)
+    def is_trajectory_finish_callback(self) -> bool:
+        return self._finished
+
+    def clear_all_callbacks(self):
+        self._finished = False
+
+    def is_in_new_trajectory(self) -> bool:
+        return self._new_trajectory
+
+    def set_custom_log_message(self, s: str):
+        self._log += str(s)  # appends, as the reference does (:2217-2220)
+
+    def get_custom_log_message(self) -> str:
+        return self._log
+
+    def add_log(self, s: str):
+        print(s, flush=True)
+
+    def get_step_count(self) -> int:
+        return self._core.get_step_count()
+
+    def reset_gradient(self):
+        self._core.reset_gradient()
+
+    def get_num_1ring_fluid_particle(self) -> int:
+        """(fluid particle, dynamic body) pairs with at least one neighbour sample
+        of the body in the last step's lists, when neighbour-DOF counting is on
+        (TimeStepDiffDFSPH.cpp:283-330, :542-543); 0 otherwise."""
+        if not self._count_dof:
+            return 0
+        return int(self._core._check(native.load().dfr_count_1ring(self._core.ctx)))
+
+    def setValueBool(self, pid: int, v: bool):
+        if pid != self.ENABLE_COUNT_NEIGHBOR_DOF:
+            raise KeyError(f"unknown parameter id {pid}")
+        self._count_dof = bool(v)
+
+    def getValueBool(self, pid: int) -> bool:
+        if pid != self.ENABLE_COUNT_NEIGHBOR_DOF:
+            raise KeyError(f"unknown parameter id {pid}")
+        return self._count_dof
+
+    def _after_step(self):
+        st = self._core.stats()
+        self._finished = bool(st["trajectory_finished"])
+        if self._finished:
+            self._log = ""  # endStep clears the custom message (:464)
+
+
+# ---------------------------------------------------------------------------
+# Simulation and TimeManager (SimulationModule.cpp:143-210, TimeModule.cpp:17-26)
+# ---------------------------------------------------------------------------
+class Simulation:
+    """The current simulation: object getters and the gradient mode."""
+
+    def __init__(self, base: "Exec.SimulatorBase"):
+        self._base = base
+
+    @staticmethod
+    def getCurrent() -> "Simulation":
+        base = _current["base"]
+        if base is None or base._core is None:
+            raise DfrError("no current simulation: call SimulatorBase.initSimulation() first")
+        return base._simulation
+
+    def getTimeStep(self) -> TimeStepDiffDFSPH:
+        return self._base._timestep
+
+    def getBoundaryModel(self, i: int) -> BoundaryModelAkinci2012:
+        return self._base._timestep.get_boundary_model(i)
+
+    def numberOfBoundaryModels(self) -> int:
+        return self._base._core.n_bodies
+
+    def numberOfFluidModels(self) -> int:
+        return 1 if self._base._core.n > 0 else 0
+
+    def setGradientMode(self, mode: int):
+        if int(mode) not in (0, 1, 2):
+            raise ValueError("gradient mode is 0 (complete), 1 (incomplete) or 2 (rigid only)")
+        core = self._base._core
+        core._check(native.load().dfr_set_gradient_mode(core.ctx, int(mode)))
+        self._base._gradient_mode = int(mode)
+
+    def getGradientMode(self) -> int:
+        return self._base._gradient_mode
+
+    def useRigidGradientManager(self) -> bool:
+        return False
+
+
+class TimeManager:
+    @staticmethod
+    def getCurrent() -> "TimeManager":
+        _core()
+        return TimeManager()
+
+    def getTime(self) -> float:
+        return float(_core().stats()["time"])
+
+    def getTimeStepSize(self) -> float:
+        return float(_core().params["dt"])
+
+
+# ---------------------------------------------------------------------------
+# Exec.SimulatorBase (SimulationModule.cpp:217-362, SimulatorBase.cpp:649-1200)
+# ---------------------------------------------------------------------------
+class _SimulatorBase:
+    # parameter ids (ParameterObjectModule.cpp:17-29 and the static ids of
+    # SimulatorBase); values of this module
+    STOP_AT = 1
+    STATE_EXPORT = 2
+    STATE_EXPORT_FPS = 3
+    PAUSE = 4
+
+    def __init__(self):
+        self._scene = None
+        self._core: Optional[_CoreSimulation] = None
+        self._timestep: Optional[TimeStepDiffDFSPH] = None
+        self._simulation: Optional[Simulation] = None
+        self._cb: Optional[Callable[[], None]] = None
+        self._reset_cb: Optional[Callable[[], None]] = None
+        self._stop_at = -1.0
+        self._use_gui = True
+        self._output = ""
+        self._gui = None
+        self._gradient_mode = 1
+        self._values = {self.STATE_EXPORT: False, self.STATE_EXPORT_FPS: 25.0, self.PAUSE: False}
+
+    def init(self, sceneFile="", programName="pySPlisHSPlasH", useCache=True, stateFile="",
+             loadFluidPos=False, loadFluidPosAndVel=False, outputDir="", initialPause=True, useGui=True,
+             stopAt=-1.0, param=""):
+        """SimulatorBase::init through py_init_simulator (SimulationModule.cpp:32-68)."""
+        if isinstance(sceneFile, _scenes.Scene):
+            scene = sceneFile
+        elif isinstance(sceneFile, str) and sceneFile and not os.path.splitext(sceneFile)[1] \
+                and callable(getattr(_scenes, sceneFile, None)):
+            scene = getattr(_scenes, sceneFile)()
+        else:
+            raise NotImplementedError(
+                "scene files are not read by this library: pass a difffr_amd.scenes.Scene "
+                f"or the name of a builder in difffr_amd.scenes (got {sceneFile!r})")
+        if stateFile or loadFluidPos or loadFluidPosAndVel:
+            raise NotImplementedError("state files are not read by this library")
+        if param:
+            raise NotImplementedError("--param overrides are not supported; set scene.params instead")
+        self._scene = scene
+        self._stop_at = float(stopAt)
+        self._use_gui = bool(useGui)
+        self._gradient_mode = int(scene.params.get("gradient_mode", 1))
+        self._output = outputDir or os.path.join(os.getcwd(), "output", scene.name)
+
+    def initSimulation(self):
+        if self._scene is None:
+            raise DfrError("SimulatorBase.init must run before initSimulation")
+        if self._core is not None:
+            self._core.close()
+        self._core = _CoreSimulation(self._scene)
+        self._timestep = TimeStepDiffDFSPH(self)
+        self._simulation = Simulation(self)
+        _current["base"] = self
+
+    def initSimulationWithDeferredInit(self):
+        self.initSimulation()
+
+    def _require(self) -> _CoreSimulation:
+        if self._core is None:
+            raise DfrError("no simulation: call initSimulation() first")
+        return self._core
+
+    def setGui(self, gui):
+        self._gui = gui
+
+    def getOutputPath(self) -> str:
+        return self._output
+
+    def setTimeStepCB(self, fn: Optional[Callable[[], None]]):
+        self._cb = fn
+
+    def setResetCB(self, fn: Optional[Callable[[], None]]):
+        self._reset_cb = fn
+
+    def setStateExportPath(self, path: str):
+        self._values["state_export_path"] = str(path)
+
+    def setValueBool(self, pid: int, v: bool):
+        if pid == self.STATE_EXPORT and v:
+            raise NotImplementedError("state export is not supported by this library")
+        if pid not in (self.STATE_EXPORT, self.PAUSE):
+            raise KeyError(f"unknown bool parameter id {pid}")
+        self._values[pid] = bool(v)
+
+    def setValueFloat(self, pid: int, v: float):
+        if pid == self.STOP_AT:
+            self._stop_at = float(v)
+        elif pid == self.STATE_EXPORT_FPS:
+            self._values[pid] = float(v)
+        else:
+            raise KeyError(f"unknown float parameter id {pid}")
+
+    def setValueInt(self, pid: int, v: int):
+        raise KeyError(f"unknown int parameter id {pid}")
+
+    def getValueFloat(self, pid: int) -> float:
+        if pid == self.STOP_AT:
+            return self._stop_at
+        return float(self._values[pid])
+
+    def getValueBool(self, pid: int) -> bool:
+        return bool(self._values[pid])
+
+    def getRigidBodyGradientManager(self):
+        raise NotImplementedError("the multi-body RigidBodyGradientManager path is not part of this library")
+
+    # -- stepping -------------------------------------------------------------
+    def _one_step(self):
+        core = self._require()
+        core.step(1)
+        self._timestep._after_step()
+        if self._cb is not None:
+            self._cb()
+
+    def timeStepNoGUI(self) -> bool:
+        """One step then the callback, unless past stopAt (SimulatorBase.cpp:1142-1200)."""
+        core = self._require()
+        if self._stop_at > 0.0 and self._stop_at < float(core.stats()["time"]):
+            return False
+        self._one_step()
+        return True
+
+    def singleTimeStep(self):
+        self._one_step()
+
+    def timeStep(self):
+        if not self._values[self.PAUSE]:
+            self._one_step()
+
+    def forwardFixedSteps(self, n: int):
+        for _ in range(int(n)):
+            self._one_step()
+
+    def runFixedTimeSteps(self, n: int):
+        self.forwardFixedSteps(n)
+
+    def runSimulation(self):
+        """The headless loop of SimulatorBase::runSimulation (SimulatorBase.cpp:649-683)."""
+        self._require()
+        if not self._use_gui and self._stop_at < 0.0:
+            raise DfrError("StopAt parameter must be set when starting without GUI.")
+        while self.timeStepNoGUI():
+            pass
+
+    def runNewTrajectory(self):
+        """Reset, then step until the trajectory ends (SimulatorBase.cpp:1093-1111)."""
+        ts = self._timestep
+        ts._new_trajectory = True
+        try:
+            self.reset()
+            while True:
+                self.singleTimeStep()
+                if bool(self._core.stats()["trajectory_finished"]):
+                    break
+        finally:
+            ts._new_trajectory = False
+
+    def reset(self):
+        """Fluid, bodies, time and gradients back to their initial state
+        (SimulatorBase::reset, SimulatorBase.cpp:887-951)."""
+        self._require().reset()
+        if self._reset_cb is not None:
+            self._reset_cb()
+
+    def cleanup(self):
+        if self._core is not None:
+            self._core.close()
+        self._core = None
+        if _current["base"] is self:
+            _current["base"] = None
+
+
+class Exec:
+    SimulatorBase = _SimulatorBase
+
+
+class _GUI:
+    def __init__(self, base):
+        self.base = base
+
+
+class GUI:
+    """No GUI: the constructor keeps the simulator and nothing else."""
+    Simulator_GUI_imgui = _GUI
+
+
+class _Timing:
+    @staticmethod
+    def printAverageTimes():
+        """Mean kernel-class times (ms) when kernel timing is on."""
+        t = _core().kernel_times()
+        for name, (avg, cnt) in t.items():
+            if cnt:
+                print(f"{name}: {avg:.4f} ms ({cnt} calls)", flush=True)
+
+    @staticmethod
+    def printTimeSums():
+        t = _core().kernel_times()
+        for name, (avg, cnt) in t.items():
+            if cnt:
+                print(f"{name}: {avg * cnt:.3f} ms total", flush=True)
+
+
+class Utilities:
+    Timing = _Timing

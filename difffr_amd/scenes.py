@@ -27,6 +27,10 @@ class Body:
     v0: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(3))
     w0: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(3))
     name: str = ""
+    # optimisation targets of the body ("target x rb" and the target angles,
+    # TimeStepDiffDFSPH.cpp:220-265; SimulationDataDiffDFSPH.h:45-47)
+    target_x: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(3))
+    target_angle: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(3))  # radians
 
 
 @dataclasses.dataclass

@@ -118,6 +118,19 @@ int dfr_get_rigid(const dfr_ctx* ctx, int body, double out[19]);
 /* Overwrites the rigid state x, q, v, omega (in[13]). */
 int dfr_set_rigid(dfr_ctx* ctx, int body, const double in[13]);
 
+/* Body properties (any pointer may be NULL): the mass, (4/3) pi r^3 rho_body
+ * per boundary sample for static bodies too (RigidBodyObject::getMass,
+ * Dynamic3dRigidBody.h:59,184-198); whether the body is dynamic
+ * (isDynamic); whether the last step drove it kinematically in the
+ * uniform-acceleration window (isAnimated, set by beginStep,
+ * TimeStepDiffDFSPH.cpp:379-429) -- pySPlisHSPlasH/RigidBodyModule.cpp:19-22. */
+int dfr_get_body(const dfr_ctx* ctx, int body, double* mass, int* dynamic, int* animated);
+
+/* Gradient mode read by the following steps: 0 Complete, 1 Incomplete,
+ * 2 RigidGradOnly (Simulation::setGradientMode,
+ * pySPlisHSPlasH/SimulationModule.cpp:208). */
+int dfr_set_gradient_mode(dfr_ctx* ctx, int mode);
+
 /* Forward-mode rigid Jacobians out[78]: Jxv, Jxw, Jvv, Jvw, Jwv, Jww (3x3
  * row-major), then Jqv, Jqw (4x3, rows w,x,y,z) -- get_grad_x_to_v0 etc.,
  * BoundaryModel_Akinci2012.h:209-323. */
@@ -152,6 +165,13 @@ int dfr_reset_stats(dfr_ctx* ctx);
 /* Neighbour-list diagnostics of the last step: mean fluid and boundary
  * neighbours per fluid particle and mean fluid neighbours per dynamic sample. */
 int dfr_neighbor_stats(const dfr_ctx* ctx, double out[3]);
+
+/* Number of (fluid particle, dynamic body) pairs in which the particle has at
+ * least one of the body's samples among its neighbours in the last step's
+ * lists (TimeStepDiffDFSPH::countNeighborDOF, TimeStepDiffDFSPH.cpp:283-330,
+ * read by get_num_1ring_fluid_particle, :2242-2245).  Returns the count or a
+ * negative error code. */
+int64_t dfr_count_1ring(const dfr_ctx* ctx);
 
 /* Halo tiles of the gather passes (no counterpart in the reference: a tuning
  * and test hook of this implementation).  Each pass stages a brick's halo of

@@ -1,0 +1,200 @@
+"""The pysplishsplash mirror driven the way DiffFR's optimisation drivers drive
+the reference (experiments/rigid_body_trajectory_optimization/python/
+gradient-based-optimize.py:423-525): a time-step callback that, when the
+trajectory ends, reads the final rigid state and its forward-mode gradient,
+updates the initial velocity, resets the simulator and clears the callback flag.
+
+* Each trajectory ends after the step whose time first reaches target_time
+  (TimeStepDiffDFSPH.cpp:448) and the state and gradient the callback reads equal
+  those of a direct rollout of the library with the same initial velocity.
+* Three gradient steps on 0.5 |x - x_target|^2 lower the loss.
+* Rigid body and boundary model accessors agree with the library's state;
+  setGradientMode at run time gives the Jacobians of a context created in
+  that mode; the neighbour-DOF count equals a count over the library's own
+  fluid -> boundary lists.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from difffr_amd import pysplishsplash as sph
+from difffr_amd import scenes
+from difffr_amd.simulation import DfrError, Simulation
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 10
+
+
+class _Stop(Exception):
+    pass
+
+
+def _scene():
+    sc = scenes.submerged_body(counts=(12, 8, 12), steps=STEPS)
+    sc.params["target_time"] = (STEPS - 0.5) * sc.params["dt"]
+    sc.bodies[1].target_x = sc.bodies[1].x_rb + np.array([0.004, 0.002, -0.003])
+    sc.bodies[1].target_angle = np.array([0.1, -0.2, 0.3])
+    return sc
+
+
+def _close(a, b, tol):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.abs(a - b).max() <= tol * max(np.abs(b).max(), 1e-30)
+
+
+def test_optimisation_loop_through_the_mirror(gpu_ready):
+    sc = _scene()
+    print(f"[mirror loop] nf={sc.n_fluid} steps/trajectory={STEPS}", flush=True)
+    base = sph.Exec.SimulatorBase()
+    base.init(sceneFile=sc, useGui=False, stopAt=0.0, initialPause=False)
+    base.setGui(sph.GUI.Simulator_GUI_imgui(base))
+    base.initSimulation()
+    sim = sph.Simulation.getCurrent()
+    ts = sim.getTimeStep()
+    sim.setGradientMode(1)
+    assert sim.getGradientMode() == 1 and sim.numberOfBoundaryModels() == 2
+    np.testing.assert_allclose(ts.get_target_quaternion_vec4(1), sph.target_quaternion([0.1, -0.2, 0.3]))
+    lr = 3000.0
+    rec = []
+
+    def callback():
+        if not ts.is_trajectory_finish_callback():
+            return
+        bm = ts.get_boundary_model(1)
+        x = bm.get_position_rb()
+        tx = ts.get_target_x(1)
+        loss = 0.5 * float(np.sum((x - tx) ** 2))
+        ts.set_loss(loss)
+        g = bm.get_grad_x_to_v0().T @ (x - tx)
+        rec.append(dict(steps=ts.get_step_count(), time=sph.TimeManager.getCurrent().getTime(), x=x.copy(),
+                        q=bm.get_quaternion_rb_vec4().copy(), J=bm.get_grad_x_to_v0().copy(), loss=loss,
+                        v0=ts.get_init_v_rb(1), w0=ts.get_init_omega_rb(1)))
+        print(f"  trajectory {len(rec)}: steps={rec[-1]['steps']} loss={loss:.4e}", flush=True)
+        ts.set_init_v_rb(1, ts.get_init_v_rb(1) - lr * g)
+        base.reset()
+        ts.clear_all_callbacks()
+        if len(rec) == 3:
+            raise _Stop()
+
+    base.setTimeStepCB(callback)
+    with pytest.raises(_Stop):
+        base.runSimulation()
+    assert ts.get_loss() == rec[-1]["loss"]
+    base.cleanup()
+    with pytest.raises(DfrError):
+        sph.Simulation.getCurrent()
+
+    for r in rec:
+        assert r["steps"] == STEPS
+        assert abs(r["time"] - STEPS * sc.params["dt"]) < 1e-6
+    assert rec[2]["loss"] < rec[0]["loss"], [r["loss"] for r in rec]
+
+    # the same rollouts straight through the library
+    for r in rec:
+        core = Simulation(sc)
+        core.set_init_velocity(1, r["v0"], r["w0"])
+        core.step(STEPS)
+        st = core.rigid(1)
+        assert _close(r["x"], st["x"], 1e-8) and _close(r["q"], st["q"], 1e-8)
+        assert _close(r["J"], core.jacobians(1)["Jxv"], 1e-6)
+        core.close()
+
+
+def test_rigid_body_and_boundary_accessors(gpu_ready):
+    sc = _scene()
+    base = sph.Exec.SimulatorBase()
+    base.init(sceneFile=sc, useGui=False, stopAt=3.5e-3)
+    base.initSimulation()
+    sim = sph.Simulation.getCurrent()
+    ts = sim.getTimeStep()
+    base.runSimulation()  # stops once the time passes stopAt
+    assert ts.get_step_count() == 4
+    tank, body = sim.getBoundaryModel(0), sim.getBoundaryModel(1)
+    rb0, rb = tank.getRigidBodyObject(), body.getRigidBodyObject()
+    assert not rb0.isDynamic() and rb.isDynamic() and not rb.isAnimated()
+    r = sc.params["radius"]
+    for k, o in ((0, rb0), (1, rb)):
+        b = sc.bodies[k]
+        assert math.isclose(o.getMass(), b.x0.shape[0] * 4.0 / 3.0 * math.pi * r ** 3 * b.density, rel_tol=1e-12)
+    q = body.get_quaternion_rb_vec4()
+    np.testing.assert_array_equal(rb.getRotation(), [q[1], q[2], q[3], q[0]])
+    np.testing.assert_array_equal(rb.getPosition(), body.get_position_rb())
+    R = rb.getWorldSpaceRotation()
+    np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-12)
+    # boundary samples: x_j = R x0_j + x_rb, v_j = v_rb + omega x (x_j - x_rb)
+    assert body.numberOfParticles() == sc.bodies[1].x0.shape[0]
+    for j in (0, body.numberOfParticles() // 2, body.numberOfParticles() - 1):
+        xj = R @ sc.bodies[1].x0[j] + rb.getPosition()
+        np.testing.assert_allclose(body.getPosition(j), xj, atol=2e-6)
+        np.testing.assert_allclose(body.getVelocity(j), body.getPointVelocity(body.getPosition(j)), atol=2e-5)
+        assert body.getVolume(j) > 0
+    with pytest.raises(IndexError):
+        body.getPosition(body.numberOfParticles())
+    # setters write the library's rigid state
+    rb.setVelocity([0.1, 0.2, 0.3])
+    np.testing.assert_array_equal(body.get_velocity_rb(), [0.1, 0.2, 0.3])
+    body.set_angular_velocity_rb([0.0, 1.0, 0.0])
+    np.testing.assert_array_equal(rb.getAngularVelocity(), [0.0, 1.0, 0.0])
+    rb.setRotation([0.0, 0.0, math.sin(0.25), math.cos(0.25)])
+    np.testing.assert_allclose(body.get_quaternion_rb_vec4(), [math.cos(0.25), 0.0, 0.0, math.sin(0.25)])
+    # runNewTrajectory resets and runs until the trajectory ends
+    base.runNewTrajectory()
+    assert ts.get_step_count() == STEPS and not ts.is_in_new_trajectory()
+    assert ts.is_trajectory_finish_callback()
+    base.cleanup()
+
+
+def test_headless_run_needs_stop_at(gpu_ready):
+    base = sph.Exec.SimulatorBase()
+    base.init(sceneFile=_scene(), useGui=False)
+    base.initSimulation()
+    with pytest.raises(DfrError, match="StopAt"):
+        base.runSimulation()
+    base.cleanup()
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_gradient_mode_set_at_run_time(gpu_ready, mode):
+    sc = _scene()
+    base = sph.Exec.SimulatorBase()
+    base.init(sceneFile=sc, useGui=False, stopAt=0.0)
+    base.initSimulation()
+    sim = sph.Simulation.getCurrent()
+    sim.setGradientMode(mode)
+    base.forwardFixedSteps(6)
+    got = sim.getBoundaryModel(1)
+    J = {n: getattr(got, n)() for n in ("get_grad_x_to_v0", "get_grad_x_to_omega0", "get_grad_quaternion_to_v0",
+                                         "get_grad_quaternion_to_omega0")}
+    base.cleanup()
+    core = Simulation(sc, params_override={"gradient_mode": mode})
+    core.step(6)
+    ref = core.jacobians(1)
+    core.close()
+    for n, k in (("get_grad_x_to_v0", "Jxv"), ("get_grad_x_to_omega0", "Jxw"),
+                 ("get_grad_quaternion_to_v0", "Jqv"), ("get_grad_quaternion_to_omega0", "Jqw")):
+        assert _close(J[n], ref[k], 1e-6), n
+    if mode == 2:  # rigid-only chain: Jxv = n dt I
+        np.testing.assert_allclose(J["get_grad_x_to_v0"], 6 * sc.params["dt"] * np.eye(3), atol=1e-9)
+    with pytest.raises(ValueError):
+        sph.Simulation(base).setGradientMode(3)
+
+
+def test_neighbour_dof_count(gpu_ready):
+    sc = _scene()
+    base = sph.Exec.SimulatorBase()
+    base.init(sceneFile=sc, useGui=False, stopAt=0.0)
+    base.initSimulation()
+    ts = sph.Simulation.getCurrent().getTimeStep()
+    base.singleTimeStep()
+    assert ts.get_num_1ring_fluid_particle() == 0  # counting is off by default
+    ts.setValueBool(ts.ENABLE_COUNT_NEIGHBOR_DOF, True)
+    got = ts.get_num_1ring_fluid_particle()
+    # fluid -> boundary lists hold global sample indices, static bodies first
+    lists = base._core.neighbor_lists(1)
+    n_static = sum(b.x0.shape[0] for b in sc.bodies if not b.dynamic)
+    expect = sum(1 for l in lists if np.any(l >= n_static))
+    print(f"[neighbour dof] pairs={got}", flush=True)
+    assert got == expect and got > 0
+    base.cleanup()

@@ -173,6 +173,7 @@ struct AdjState {
   double *sxb = nullptr, *svb = nullptr;
   AdjBodyDev* body = nullptr;
   StepTapeDev tape;
+  std::vector<double> start_grad;          // dL/d(x, q, v, omega) at the window start, 13 per body
 };
 
 }  // namespace
@@ -1894,11 +1895,33 @@ int dfr_adjoint_backward(dfr_ctx* c, const double* seed, double* grad) {
     HIPCHK(hipMemcpyAsync(hb.data(), A.body, sizeof(AdjBodyDev) * nbodies, hipMemcpyDeviceToHost, c->stream));
     sync_check(c);
     if (c->profiling) drain_timers(c);
-    for (int b = 0; b < nbodies; ++b)
-      for (int q = 0; q < 3; ++q) { grad[6 * b + q] = hb[b].v0[q]; grad[6 * b + 3 + q] = hb[b].w0[q]; }
+    bool finite = true;
+    A.start_grad.assign(13 * (size_t)nbodies, 0.0);
+    for (int b = 0; b < nbodies; ++b) {
+      double* o = A.start_grad.data() + 13 * (size_t)b;
+      for (int q = 0; q < 3; ++q) {
+        grad[6 * b + q] = hb[b].v0[q];
+        grad[6 * b + 3 + q] = hb[b].w0[q];
+        o[q] = hb[b].x[q]; o[7 + q] = hb[b].v[q]; o[10 + q] = hb[b].w[q];
+      }
+      for (int q = 0; q < 4; ++q) o[3 + q] = hb[b].q[q];
+      for (int q = 0; q < 13; ++q) finite = finite && std::isfinite(o[q]);
+      for (int q = 0; q < 6; ++q) finite = finite && std::isfinite(grad[6 * b + q]);
+    }
     c->stats = saved_stats;
+    if (!finite)
+      return fail(c, DFR_ERR_RANGE,
+                  "the exact gradient left the FP32 range: it grows exponentially once fluid and body interact "
+                  "strongly; use a shorter window or the forward-mode gradient");
     return DFR_OK;
   });
+}
+
+int dfr_adjoint_start_gradient(const dfr_ctx* c, double* out) {
+  if (!c || !out) return DFR_ERR_ARG;
+  if (c->adj.start_grad.size() != 13 * c->bodies.size()) return DFR_ERR_STATE;
+  std::memcpy(out, c->adj.start_grad.data(), sizeof(double) * c->adj.start_grad.size());
+  return DFR_OK;
 }
 
 int dfr_adjoint_end(dfr_ctx* c) {

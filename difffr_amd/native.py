@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdfr_hip.so")
 
 DFR_OK = 0
-ERRORS = {-1: "DFR_ERR_ARG", -2: "DFR_ERR_HIP", -3: "DFR_ERR_CAPACITY", -4: "DFR_ERR_STATE"}
+ERRORS = {-1: "DFR_ERR_ARG", -2: "DFR_ERR_HIP", -3: "DFR_ERR_CAPACITY", -4: "DFR_ERR_STATE", -5: "DFR_ERR_RANGE"}
 
 # Every symbol include/dfr.h declares (checked by the CPU test suite).
 EXPORTS = [
@@ -25,7 +25,7 @@ EXPORTS = [
     "dfr_set_fixed_iterations", "dfr_mlp_capture", "dfr_mlp_dataset_size", "dfr_mlp_dataset", "dfr_mlp_init_params",
     "dfr_mlp_set_params", "dfr_mlp_get_params", "dfr_mlp_train", "dfr_mlp_enable", "dfr_mlp_predict",
     "dfr_mlp_loss_grad", "dfr_mlp_input_grad", "dfr_set_halo_capacity", "dfr_halo_stats",
-    "dfr_set_profiling_classes", "dfr_get_body", "dfr_set_gradient_mode", "dfr_count_1ring",
+    "dfr_set_profiling_classes", "dfr_get_body", "dfr_set_gradient_mode", "dfr_count_1ring", "dfr_adjoint_start_gradient",
 ]
 
 MLP_NP = 8898
@@ -113,6 +113,7 @@ def load(path: str | None = None):
     L.dfr_adjoint_begin.argtypes = [P, C.c_int]
     L.dfr_adjoint_backward.argtypes = [P, dp, dp]
     L.dfr_adjoint_end.argtypes = [P]
+    L.dfr_adjoint_start_gradient.argtypes = [P, dp]
     L.dfr_adjoint_iterations.argtypes = [P, ip, ip, C.c_int64]
     L.dfr_adjoint_iterations.restype = C.c_int64
     L.dfr_set_fixed_iterations.argtypes = [P, C.c_int, ip, ip]

@@ -27,6 +27,14 @@ class DfrError(RuntimeError):
     pass
 
 
+class AdjointRangeError(DfrError):
+    """The exact gradient left the FP32 range; `grad` holds what the reverse pass produced."""
+
+    def __init__(self, msg: str, grad: np.ndarray):
+        super().__init__(msg)
+        self.grad = grad
+
+
 def make_params(p: dict) -> native.Params:
     L = native.load()
     q = native.Params()
@@ -296,7 +304,19 @@ class Simulation:
         Returns (n_bodies, 6): dL/d(init_v_rb, init_omega_rb)."""
         sd = np.ascontiguousarray(seed, np.float64).reshape(self.n_bodies, 7)
         out = np.zeros((self.n_bodies, 6), np.float64)
-        self._check(native.load().dfr_adjoint_backward(self.ctx, _dp(sd), _dp(out)))
+        L = native.load()
+        r = L.dfr_adjoint_backward(self.ctx, _dp(sd), _dp(out))
+        if r == -5:
+            msg = L.dfr_last_error(self.ctx)
+            raise AdjointRangeError(f"DFR_ERR_RANGE: {msg.decode() if msg else ''}", out)
+        self._check(r)
+        return out
+
+    def adjoint_start_gradient(self) -> np.ndarray:
+        """(n_bodies, 13): dL/d(x_rb, q (w,x,y,z), v, omega) at the start of the
+        recorded window, from the last adjoint_backward."""
+        out = np.zeros((self.n_bodies, 13), np.float64)
+        self._check(native.load().dfr_adjoint_start_gradient(self.ctx, _dp(out)))
         return out
 
     def adjoint_end(self):

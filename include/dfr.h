@@ -29,6 +29,7 @@ extern "C" {
 #define DFR_ERR_HIP -2        /* HIP runtime error */
 #define DFR_ERR_CAPACITY -3   /* a neighbour list exceeded its capacity */
 #define DFR_ERR_STATE -4      /* context not initialised / already initialised */
+#define DFR_ERR_RANGE -5      /* an exact gradient left the FP32 range */
 
 typedef struct dfr_ctx dfr_ctx;
 
@@ -218,10 +219,21 @@ int dfr_kernel_times(const dfr_ctx* ctx, int n, char names[][32], double* avg_ms
  * piecewise-smooth step).  The state after the last step is restored on
  * return, so backward can be called again with another seed.  dfr_reset,
  * dfr_set_rigid, dfr_set_field and dfr_set_init_velocity end the recording;
- * dfr_adjoint_end frees the checkpoints. */
+ * dfr_adjoint_end frees the checkpoints.
+ *
+ * Once fluid and body interact strongly, this derivative grows exponentially
+ * with the length of the window (the step is discontinuous in the eps guards
+ * and the 20-neighbour rule, and its a.e. derivative is large); when the
+ * reverse pass leaves the FP32 range, backward writes grad as computed and
+ * returns DFR_ERR_RANGE. */
 int dfr_adjoint_begin(dfr_ctx* ctx, int checkpoint_every);
 int dfr_adjoint_backward(dfr_ctx* ctx, const double* seed, double* grad);
 int dfr_adjoint_end(dfr_ctx* ctx);
+/* After dfr_adjoint_backward: out[13*b..13*b+12] = dL/d(x_rb, q (w,x,y,z), v,
+ * omega) of body b at the start of the recorded window, i.e. before its first
+ * step (for a window that starts after the release window of beginStep, the
+ * gradient with respect to the rigid state the window starts from). */
+int dfr_adjoint_start_gradient(const dfr_ctx* ctx, double* out);
 /* Iteration counts (divergence, pressure) of the steps recorded since
  * dfr_adjoint_begin; returns the number of steps. */
 int64_t dfr_adjoint_iterations(const dfr_ctx* ctx, int* iv, int* ip, int64_t cap);

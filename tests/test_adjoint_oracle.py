@@ -79,3 +79,38 @@ def test_adjoint_fp32_tapes_track_fp64():
     g64 = s64.adjoint(seed)[1]
     g32 = s32.adjoint(seed)[1]
     np.testing.assert_allclose(g32, g64, rtol=0, atol=1e-3 * np.abs(g64).max())
+
+
+def test_adjoint_after_impact_is_the_derivative_of_a_rough_map():
+    """A body driven into the fluid (small tank, v0 = (0.5, -1.5, 0.3)), 100
+    steps: the gradient has grown about 30x over the 20 steps since contact.
+    Differences with eps = 1e-10 (no eps guard or neighbour-count switch flips
+    inside the interval) reproduce the adjoint; with eps >= 1e-8 the secants are
+    dominated by those switches and bear no relation to it.  The adjoint is the
+    derivative of the discrete step; its growth is a property of that step."""
+    sc = scenes.small_tank(seed=0, steps=100)
+    b = sc.bodies[1]
+    b.v0 = np.array([0.5, -1.5, 0.3])
+    seed = np.random.default_rng(11).normal(size=7)
+    s, _ = _run(sc, "f64", b.v0, b.w0, 100, record=True)
+    fix = s.recorded_iterations()
+    sd = np.zeros((len(sc.bodies), 7))
+    sd[1] = seed
+    g = s.adjoint(sd)[1]
+    del s
+    assert np.abs(g).max() > 1.0  # free flight alone would give about 0.14
+    p0 = np.concatenate([b.v0, b.w0])
+
+    def secant(k, eps):
+        p = p0.copy(); p[k] += eps
+        m = p0.copy(); m[k] -= eps
+        _, a = _run(sc, "f64", p[:3], p[3:], 100, fix)
+        _, c = _run(sc, "f64", m[:3], m[3:], 100, fix)
+        return seed @ (a - c) / (2 * eps)
+
+    fine = secant(1, 1e-10)
+    print(f"adjoint {g[1]:.6f}, eps=1e-10 secant {fine:.6f}", flush=True)
+    assert abs(fine - g[1]) < 2e-3 * abs(g[1])
+    coarse = secant(1, 1e-8)
+    print(f"eps=1e-8 secant {coarse:.6f}", flush=True)
+    assert abs(coarse - g[1]) > 0.5 * abs(g[1])

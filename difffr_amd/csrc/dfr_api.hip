@@ -174,6 +174,7 @@ struct AdjState {
   AdjBodyDev* body = nullptr;
   StepTapeDev tape;
   std::vector<double> start_grad;          // dL/d(x, q, v, omega) at the window start, 13 per body
+  int64_t seg_cap = -1;                    // bytes one segment's kept tapes may take (-1: free memory)
 };
 
 }  // namespace
@@ -1143,6 +1144,50 @@ void replay_step(dfr_ctx* c, int64_t s, bool record) {
   c->force_iv = c->force_ip = -1;
 }
 
+// Reverse of the segment s0..s1-1 (checkpoint `seg`) with every step's tape
+// kept in the arena: the segment is re-run once from its checkpoint with
+// recording, then its steps are reversed from the kept tapes.  Returns false,
+// with the arena rewound and nothing reversed, when the first step's tape shows
+// that the segment's tapes would not fit in device memory.
+bool reverse_segment_kept(dfr_ctx* c, int64_t seg, int64_t s0, int64_t s1) {
+  AdjState& A = c->adj;
+  if (A.seg_cap == 0) return false;
+  A.arena.reset();
+  A.tapes.clear();
+  A.tapes.resize((size_t)(s1 - s0));
+  snapshot_restore(c, A.ckpt[seg]);
+  for (int64_t s = s0; s < s1; ++s) {
+    StepTapeDev& T = A.tapes[(size_t)(s - s0)];
+    tape_alloc_fixed(c, T);
+    tape_swap_in(c, T);
+    c->force_iv = A.iv[s];
+    c->force_ip = A.ip[s];
+    c->rec = &T;
+    do_step(c);
+    c->rec = nullptr;
+    c->force_iv = c->force_ip = -1;
+    T.rig_post = arena_copy(c, (const RigidDev*)c->d_rig, c->bodies.size());
+    if (s == s0 && s1 - s0 > 1) {
+      // room for the rest of the segment, with a margin for steps that iterate more
+      size_t cap = 0;
+      for (const auto& ch : A.arena.chunks) cap += ch.second;
+      const size_t need = A.arena.used() / 2 * 3 * (size_t)(s1 - s0);
+      size_t fr = 0, tot = 0;
+      HIPCHK(hipMemGetInfo(&fr, &tot));
+      const bool over = A.seg_cap > 0 ? need > (size_t)A.seg_cap : (need > cap && need - cap + ((size_t)2 << 30) > fr);
+      if (over) {
+        tape_swap_out(c);
+        A.tapes.clear();
+        A.arena.reset();
+        return false;
+      }
+    }
+  }
+  for (int64_t s = s1 - 1; s >= s0; --s) reverse_step(c, A.tapes[(size_t)(s - s0)]);
+  A.tapes.clear();
+  return true;
+}
+
 // Device scratch for the host-array entry points below.
 template <typename T> struct DevTmp {
   T* p = nullptr;
@@ -1872,9 +1917,16 @@ int dfr_adjoint_backward(dfr_ctx* c, const double* seed, double* grad) {
       for (int64_t s = S - 1; s >= 0; --s) reverse_step(c, A.tapes[s]);
       sync_check(c);
     }
-    if (!stored && k > 1 && (int64_t)A.seg.size() < k) A.seg.resize(k);
     for (int64_t seg = stored ? -1 : (S - 1) / k; seg >= 0; --seg) {
       const int64_t s0 = seg * k, s1 = std::min(s0 + k, S);
+      if (reverse_segment_kept(c, seg, s0, s1)) {
+        sync_check(c);
+        check_overflow(c);
+        continue;
+      }
+      // the segment's tapes do not fit: keep its states instead, and re-run
+      // every step once more with recording right before reversing it
+      if ((int64_t)A.seg.size() < k) A.seg.resize(k);
       if (s1 - s0 > 1) {  // states at the start of steps s0+1 .. s1-1
         snapshot_restore(c, A.ckpt[seg]);
         for (int64_t s = s0; s < s1 - 1; ++s) {
@@ -1891,7 +1943,11 @@ int dfr_adjoint_backward(dfr_ctx* c, const double* seed, double* grad) {
       sync_check(c);
       check_overflow(c);
     }
-    if (!stored) snapshot_restore(c, A.final_state);
+    if (!stored) {
+      tape_swap_out(c);
+      A.arena.reset();
+      snapshot_restore(c, A.final_state);
+    }
     HIPCHK(hipMemcpyAsync(hb.data(), A.body, sizeof(AdjBodyDev) * nbodies, hipMemcpyDeviceToHost, c->stream));
     sync_check(c);
     if (c->profiling) drain_timers(c);
@@ -1915,6 +1971,12 @@ int dfr_adjoint_backward(dfr_ctx* c, const double* seed, double* grad) {
                   "strongly; use a shorter window or the forward-mode gradient");
     return DFR_OK;
   });
+}
+
+int dfr_adjoint_segment_memory(dfr_ctx* c, int64_t bytes) {
+  if (!c || bytes < -1) return DFR_ERR_ARG;
+  c->adj.seg_cap = bytes;
+  return DFR_OK;
 }
 
 int dfr_adjoint_start_gradient(const dfr_ctx* c, double* out) {

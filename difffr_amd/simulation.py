@@ -319,6 +319,11 @@ class Simulation:
         self._check(native.load().dfr_adjoint_start_gradient(self.ctx, _dp(out)))
         return out
 
+    def adjoint_segment_memory(self, nbytes: int = -1):
+        """Bound on one checkpoint segment's kept tapes in the backward pass
+        (-1: free device memory; 0: re-run every step twice instead)."""
+        self._check(native.load().dfr_adjoint_segment_memory(self.ctx, int(nbytes)))
+
     def adjoint_end(self):
         self._check(native.load().dfr_adjoint_end(self.ctx))
 

@@ -212,9 +212,12 @@ int dfr_kernel_times(const dfr_ctx* ctx, int n, char names[][32], double* avg_ms
  * log their iteration counts.  dfr_adjoint_backward then returns
  * grad[6*b..6*b+5] = dL/d(init_v_rb, init_omega_rb) of body b for the seed
  * seed[7*b..7*b+6] = dL/d(x_rb, q (w,x,y,z)) of every body after the last
- * step: each segment of checkpoint_every steps is re-run from its checkpoint,
- * every step is re-run once more recording its Jacobi fields, and the reverse
- * kernels run through it.  Clamps, the eps guards, the 20-neighbour rule and
+ * step: each segment of checkpoint_every steps is re-run once from its
+ * checkpoint, keeping its steps' tapes in HBM, and the reverse kernels run
+ * through them.  When a segment's tapes would not fit (or exceed the bound
+ * set by dfr_adjoint_segment_memory), the segment's states are kept instead
+ * and every step is re-run once more with recording right before it is
+ * reversed.  Clamps, the eps guards, the 20-neighbour rule and
  * iteration counts are held at their values in the run (the derivative of the
  * piecewise-smooth step).  The state after the last step is restored on
  * return, so backward can be called again with another seed.  dfr_reset,
@@ -234,6 +237,10 @@ int dfr_adjoint_end(dfr_ctx* ctx);
  * step (for a window that starts after the release window of beginStep, the
  * gradient with respect to the rigid state the window starts from). */
 int dfr_adjoint_start_gradient(const dfr_ctx* ctx, double* out);
+/* Bytes one segment's kept tapes may occupy in the backward pass of a
+ * checkpointed rollout: -1 (default) whatever device memory is free, 0 never
+ * (every step re-run twice), > 0 that bound. */
+int dfr_adjoint_segment_memory(dfr_ctx* ctx, int64_t bytes);
 /* Iteration counts (divergence, pressure) of the steps recorded since
  * dfr_adjoint_begin; returns the number of steps. */
 int64_t dfr_adjoint_iterations(const dfr_ctx* ctx, int* iv, int* ip, int64_t cap);

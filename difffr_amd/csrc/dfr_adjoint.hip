@@ -29,7 +29,9 @@ struct PairCE {
   float c, e;
 };
 // Outer part with g = l f^2 / (h |r|): c = -g, e = (2 l f / h^2 + g) / |r|^2.
-__device__ inline PairCE cubic_ce(const KernelConst& K, float r2) {
+// `on` = false gives c = e = 0 (a pair whose every term vanishes), folded into
+// the select the support test needs anyway.
+__device__ inline PairCE cubic_ce(const KernelConst& K, float r2, bool on = true) {
   const float irl = __builtin_amdgcn_rsqf(r2);
   const float q = r2 * irl * K.inv_h;
   const float f = 1.f - q;
@@ -39,7 +41,7 @@ __device__ inline PairCE cubic_ce(const KernelConst& K, float r2) {
   const float cout = -g;
   const float ein = (3.f * K.l_ih2 * K.inv_h) * irl;
   const float eout = fmaf(2.f * K.l_ih2, f, g) * (irl * irl);
-  const bool valid = r2 > 1.0e-10f && q <= 1.f;
+  const bool valid = on && r2 > 1.0e-10f && q <= 1.f;
   PairCE o;
   o.c = valid ? (inner ? cin : cout) : 0.f;
   o.e = valid ? (inner ? ein : eout) : 0.f;
@@ -116,17 +118,15 @@ k_adj_update_t(AdjPass a, const float* __restrict__ k, const float4* __restrict_
     // m = 0 for pairs with |k_i + k_j| <= eps and for slots that are not live
     auto pair = [&](float qx, float qy, float qz, float wx, float wy, float wz, float kj, bool live) {
       const float kSum = ki + kj;
-      const float m = (live && fabsf(kSum) > c.eps) ? 1.f : 0.f;
       const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
-      const PairCE ce = cubic_ce(c.K, rx * rx + ry * ry + rz * rz);
+      const PairCE ce = cubic_ce(c.K, rx * rx + ry * ry + rz * rz, live && fabsf(kSum) > c.eps);
       const float ux = vb.x - wx, uy = vb.y - wy, uz = vb.z - wz;
       const float ru = rx * ux + ry * uy + rz * uz;
-      kbf += (m * ce.c) * ru;
-      const float sk = m * kSum;
-      const float er = ce.e * ru;
-      xfx += sk * (ce.c * ux + er * rx);
-      xfy += sk * (ce.c * uy + er * ry);
-      xfz += sk * (ce.c * uz + er * rz);
+      kbf = fmaf(ce.c, ru, kbf);
+      const float a = ce.c * kSum, b = (ce.e * ru) * kSum;
+      xfx = fmaf(a, ux, fmaf(b, rx, xfx));
+      xfy = fmaf(a, uy, fmaf(b, ry, xfy));
+      xfz = fmaf(a, uz, fmaf(b, rz, xfz));
     };
     if (in_lds) {
       for_fluid_groups<2>(a.nbr8, stride, i, nF, [&](uint32_t s) { return Slot8{LA[s], LB[s]}; },
@@ -258,11 +258,11 @@ k_adj_delta_t(AdjPass a, const float4* __restrict__ v, const float* __restrict__
     const uint32_t cc = a.cnt[i];
     const uint32_t nF = cc & 0xffffu, nB = cc >> 16;
     float gx = 0.f, gy = 0.f, gz = 0.f, hx = 0.f, hy = 0.f, hz = 0.f;  // without the factor V_f
-    // slots that are not live get the weight s = 0
+    // slots that are not live get c = e = 0
     auto pair = [&](float qx, float qy, float qz, float wx, float wy, float wz, float dj, bool live) {
-      const float s = live ? di + dj : 0.f;
+      const float s = di + dj;
       const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
-      const PairCE ce = cubic_ce(c.K, rx * rx + ry * ry + rz * rz);
+      const PairCE ce = cubic_ce(c.K, rx * rx + ry * ry + rz * rz, live);
       const float sc = s * ce.c;
       gx += sc * rx; gy += sc * ry; gz += sc * rz;
       const float ux = vi.x - wx, uy = vi.y - wy, uz = vi.z - wz;
@@ -370,9 +370,7 @@ k_adj_factor_t(AdjPass a, const float* __restrict__ alpha, const float* __restri
     // slots that are not live get c = e = 0 (every term carries one of them)
     auto pair = [&](float qx, float qy, float qz, float Sj, float rj, float Gx, float Gy, float Gz, bool live) {
       const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
-      PairCE ce = cubic_ce(c.K, rx * rx + ry * ry + rz * rz);
-      ce.c = live ? ce.c : 0.f;
-      ce.e = live ? ce.e : 0.f;
+      const PairCE ce = cubic_ce(c.K, rx * rx + ry * ry + rz * rz, live);
       const float sr = (ri + rj) * ce.c;
       dx += sr * rx; dy += sr * ry; dz += sr * rz;
       // u = (Sb_i + Sb_j) V_f gradW - Sb_i G_i + Sb_j G_j

@@ -118,14 +118,16 @@ __device__ inline PairKernel cubic_pair(const KernelConst& K, float r2) {
   o.c = (r2 > 1.0e-10f && !far) ? (inner ? cin : cout) : 0.f;
   return o;
 }
-__device__ inline float cubic_grad_c(const KernelConst& K, float r2) {
+// on = false gives 0 (a pair that contributes nothing), folded into the
+// support test's select.
+__device__ inline float cubic_grad_c(const KernelConst& K, float r2, bool on = true) {
   const float irl = __builtin_amdgcn_rsqf(r2);
   const float q = r2 * irl * K.inv_h;
   const float f = 1.f - q;
   const float cin = fmaf(q, 3.f * K.l_ih2, -2.f * K.l_ih2);
   const float cout = -K.l_ih * (f * f) * irl;
   const float cc = (q <= 0.5f) ? cin : cout;
-  return (r2 > 1.0e-10f && q <= 1.f) ? cc : 0.f;
+  return (on && r2 > 1.0e-10f && q <= 1.f) ? cc : 0.f;
 }
 
 // gradGradW(r) = a * I + b * r r^T (SPHKernels.h:122-149 rewritten with

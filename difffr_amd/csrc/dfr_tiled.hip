@@ -382,8 +382,7 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
       auto pair = [&](float qx, float qy, float qz, float kj, bool live) {
         const float kSum = ki + kj;
         const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
-        const float gc = cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
-        const float s = (live && fabsf(kSum) > c.eps) ? kSum * gc : 0.f;
+        const float s = kSum * cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz, live && fabsf(kSum) > c.eps);
         ux += s * rx; uy += s * ry; uz += s * rz;
       };
       if (in_lds) {
@@ -504,8 +503,7 @@ k_recompute_t(PassArgs a, int mode, float kcoef, const float* __restrict__ rho, 
     float fdelta = 0.f;  // fluid part without the common factor V_f
     auto pair = [&](float qx, float qy, float qz, float vx, float vy, float vz, bool live) {
       const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
-      const float gc = cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
-      const float sc = live ? gc : 0.f;
+      const float sc = cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz, live);
       fdelta += sc * ((vi.x - vx) * rx + (vi.y - vy) * ry + (vi.z - vz) * rz);
     };
     if (in_lds) {

@@ -538,19 +538,22 @@ template <typename T> T* talloc(dfr_ctx* c, StepTapeDev* rec, size_t n) {
   return rec && rec->arena ? static_cast<T*>(rec->arena->alloc(sizeof(T) * std::max<size_t>(n, 1))) : dalloc<T>(c, n);
 }
 
-void tape_iteration(dfr_ctx* c, std::vector<float4*>& v, std::vector<float*>& k, std::vector<float*>& b, int m,
-                    const int* stop) {
+// Tape slots of Jacobi iteration m (grown on first use); the iteration's
+// velocity update fills them.
+IterTapeOut tape_iteration(dfr_ctx* c, std::vector<float4*>& v, std::vector<float*>& k, std::vector<float*>& b,
+                           int m) {
   const size_t n = (size_t)std::max(c->f.n, 1);
   while ((int)v.size() <= m) {
     v.push_back(talloc<float4>(c, c->rec, n));
     k.push_back(talloc<float>(c, c->rec, n));
     b.push_back(talloc<float>(c, c->rec, n));
   }
-  Batch bt(c);
-  bt.copy(v[m], c->f.vel, c->f.n);
-  bt.copy(k[m], c->f.kf, c->f.n);
-  bt.copy(b[m], c->f.badv, c->f.n);
-  bt.flush(stop);
+  IterTapeOut o;
+  o.v = v[m];
+  o.k = k[m];
+  o.b = b[m];
+  o.bsrc = c->f.badv;
+  return o;
 }
 
 // The Jacobi iterations of one solve.  fixed >= 0: exactly that many (replay,
@@ -725,11 +728,11 @@ void do_step(dfr_ctx* c) {
         c, 0, fixV, 1, maxv, eta, c->d_err, nf,
         [&](int m, const int* stop) {
           run_gradient(c, 1);
-          if (rec) tape_iteration(c, rec->vD, rec->kD, rec->bD, m, stop);
+          const IterTapeOut to = rec ? tape_iteration(c, rec->vD, rec->kD, rec->bD, m) : IterTapeOut{};
           {
             Timed t(c, TC_VUPDATE);
             launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kf,
-                                     warmV ? c->f.kappav : nullptr, c->stream, stop);
+                                     warmV ? c->f.kappav : nullptr, c->stream, stop, to);
           }
           Timed t(c, TC_RECOMPUTE);
           launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_DIV, invH, c->d_part, c->stream,
@@ -773,11 +776,11 @@ void do_step(dfr_ctx* c) {
         c, 1, fixP, c->P.min_iterations, maxp, eta, c->d_err + base, nf,
         [&](int m, const int* stop) {
           run_gradient(c, 0);
-          if (rec) tape_iteration(c, rec->vP, rec->kP, rec->bP, m, stop);
+          const IterTapeOut to = rec ? tape_iteration(c, rec->vP, rec->kP, rec->bP, m) : IterTapeOut{};
           {
             Timed t(c, TC_VUPDATE);
             launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kf,
-                                     warmP ? c->f.kappa : nullptr, c->stream, stop);
+                                     warmP ? c->f.kappa : nullptr, c->stream, stop, to);
           }
           Timed t(c, TC_RECOMPUTE);
           launch_recompute_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, RECOMPUTE_PRESSURE, invH2, c->d_part,

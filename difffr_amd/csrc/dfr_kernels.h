@@ -102,10 +102,19 @@ void launch_build_tiled(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBu
 // aux (optional): per particle (velocity divergence, boundary density sum_b V_b W_ib)
 void launch_density_factor_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
                              NbrBufs& nb, int div_warm, int divergence_on, float2* aux, hipStream_t s);
+// Where a recorded Jacobi iteration keeps what it starts from: the velocity
+// before the update, the stiffness it applies and the solve's b field (null v:
+// not recorded).  The velocity update writes them for its own particles.
+struct IterTapeOut {
+  float4* v = nullptr;
+  float* k = nullptr;
+  float* b = nullptr;
+  const float* bsrc = nullptr;
+};
 // stop (optional): device flag of a Jacobi solve; a set flag makes the launch a no-op
 void launch_velocity_update_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
                               NbrBufs& nb, RigidDev* rig, const float* kfield, float* kappa_acc, hipStream_t s,
-                              const int* stop = nullptr);
+                              const int* stop = nullptr, IterTapeOut tape = {});
 // mode 0: divergence recompute, 1: pressure recompute, 2: pressure warm start,
 // 3: pressure warm start from predicted stiffness kpred (learned)
 void launch_recompute_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,

@@ -332,7 +332,7 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
 // LDS per slot: x, y, z, k.
 __global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restrict__ kappa_acc,
-                    RigidDev* __restrict__ rig, const int* __restrict__ stop) {
+                    RigidDev* __restrict__ rig, const int* __restrict__ stop, IterTapeOut tape) {
   __shared__ TileHdr<VB> T;
   extern __shared__ float4 lds4[];
   const StepConst& c = a.c;
@@ -374,6 +374,11 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
       if (kappa_acc) kappa_acc[i] += ki;
       const float4 p = a.pos[i];
       float4 vi = a.vel[i];
+      if (tape.v) {
+        tape.v[i] = vi;
+        tape.k[i] = ki;
+        tape.b[i] = tape.bsrc[i];
+      }
       const uint32_t cc = a.cnt[i];
       const uint32_t nF = cc & 0xffffu, nB = cc >> 16;
       // v_i -= dt (k_i + k_j) grad_p_ij = dt V_f (k_i + k_j) c_ij r_ij; dt V_f applied once
@@ -596,10 +601,10 @@ void launch_density_factor_t(const StepConst& c, const GridDesc& g, GridBufs& gb
 
 void launch_velocity_update_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
                               NbrBufs& nb, RigidDev* rig, const float* kfield, float* kappa_acc, hipStream_t s,
-                              const int* stop) {
+                              const int* stop, IterTapeOut tape) {
   const size_t shm = sizeof(float) * (size_t)nb.cap * 4;
   hipLaunchKernelGGL(k_velocity_update_t, dim3(nbricks(g)), dim3(VB), shm, s, make_args(c, g, gb, f, b, nb), kfield,
-                     kappa_acc, rig, stop);
+                     kappa_acc, rig, stop, tape);
 }
 
 void launch_recompute_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,

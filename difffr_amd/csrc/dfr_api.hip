@@ -701,7 +701,10 @@ void do_step(dfr_ctx* c) {
     launch_mlp_infer_apply(ml.frag, ml.P, ml.norm, fields, nf, (divOn && warmV) ? c->f.badv : nullptr, 1.f / dt,
                            c->f.kappav, ml.kpred, c->stream);
   }
-  if (rec && nf > 0) {
+  // the divergence warm start's inputs go on the tape: written by its velocity
+  // update when that runs, copied otherwise
+  const bool vwarm_runs = divOn && warmV && nf > 0;
+  if (rec && nf > 0 && !vwarm_runs) {
     Batch bt(c);
     bt.copy(rec->d0, c->f.badv, nf);
     bt.copy(rec->kvW, c->f.kappav, nf);
@@ -715,7 +718,10 @@ void do_step(dfr_ctx* c) {
     const float invH = 1.f / dt;
     if (warmV) {
       Timed t(c, TC_VUPDATE);
-      launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kappav, nullptr, c->stream);
+      IterTapeOut to;
+      if (rec) { to.k = rec->kvW; to.b = rec->d0; to.bsrc = c->f.badv; }
+      launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kappav, nullptr, c->stream,
+                               nullptr, to);
     }
     {
       Timed t(c, TC_RECOMPUTE);
@@ -756,14 +762,11 @@ void do_step(dfr_ctx* c) {
                            learned ? PRESSURE_WARMSTART_LEARNED : PRESSURE_WARMSTART, 0.f, nullptr, c->stream,
                            ml.kpred);
       }
-      if (rec) {
-        Batch bt(c);
-        bt.copy(rec->rsW, c->f.badv, nf);
-        bt.copy(rec->kW, c->f.kappa, nf);
-        bt.flush();
-      }
       Timed t(c, TC_VUPDATE);
-      launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kappa, nullptr, c->stream);
+      IterTapeOut to;  // the pressure warm start's inputs go on the tape
+      if (rec) { to.k = rec->kW; to.b = rec->rsW; to.bsrc = c->f.badv; }
+      launch_velocity_update_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->d_rig, c->f.kappa, nullptr, c->stream,
+                               nullptr, to);
     }
     {
       Timed t(c, TC_RECOMPUTE);

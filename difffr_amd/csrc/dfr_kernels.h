@@ -102,9 +102,10 @@ void launch_build_tiled(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBu
 // aux (optional): per particle (velocity divergence, boundary density sum_b V_b W_ib)
 void launch_density_factor_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
                              NbrBufs& nb, int div_warm, int divergence_on, float2* aux, hipStream_t s);
-// Where a recorded Jacobi iteration keeps what it starts from: the velocity
-// before the update, the stiffness it applies and the solve's b field (null v:
-// not recorded).  The velocity update writes them for its own particles.
+// Where a recorded Jacobi iteration (or warm start) keeps what it starts from:
+// the velocity before the update, the stiffness it applies and the solve's b
+// field (each null: not recorded).  The velocity update writes them for its
+// own particles.
 struct IterTapeOut {
   float4* v = nullptr;
   float* k = nullptr;

@@ -79,13 +79,23 @@ __global__ void k_boundary_volume(const float4* __restrict__ bpos_in, const floa
 // it + 1 < max.  probe is host-visible memory.
 // The residual is the sum of the recompute launch's per-brick partials, in a
 // fixed order (every run sums the same way).
-constexpr int CHECK_BLOCK = 256;
+constexpr int CHECK_BLOCK = 1024;
 __global__ void __launch_bounds__(CHECK_BLOCK)
 k_solve_check(const double* __restrict__ part, int nparts, double* __restrict__ err, int n, double eta, int it,
               int min_it, int max_it, int* __restrict__ stop, SolveProbe* probe) {
   if (stop && *stop) return;
-  double v = 0.0;
-  for (int k = threadIdx.x; k < nparts; k += CHECK_BLOCK) v += part[k];
+  // four independent running sums per thread keep four loads in flight; the
+  // order of every addition is fixed by nparts alone
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  int k = threadIdx.x;
+  for (; k + 3 * CHECK_BLOCK < nparts; k += 4 * CHECK_BLOCK) {
+    v0 += part[k];
+    v1 += part[k + CHECK_BLOCK];
+    v2 += part[k + 2 * CHECK_BLOCK];
+    v3 += part[k + 3 * CHECK_BLOCK];
+  }
+  for (; k < nparts; k += CHECK_BLOCK) v0 += part[k];
+  double v = (v0 + v1) + (v2 + v3);
   __shared__ double w[CHECK_BLOCK / 64];
   v = wave_sum_d(v);
   if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = v;

@@ -374,11 +374,9 @@ k_velocity_update_t(PassArgs a, const float* __restrict__ kfield, float* __restr
       if (kappa_acc) kappa_acc[i] += ki;
       const float4 p = a.pos[i];
       float4 vi = a.vel[i];
-      if (tape.v) {
-        tape.v[i] = vi;
-        tape.k[i] = ki;
-        tape.b[i] = tape.bsrc[i];
-      }
+      if (tape.v) tape.v[i] = vi;
+      if (tape.k) tape.k[i] = ki;
+      if (tape.b) tape.b[i] = tape.bsrc[i];
       const uint32_t cc = a.cnt[i];
       const uint32_t nF = cc & 0xffffu, nB = cc >> 16;
       // v_i -= dt (k_i + k_j) grad_p_ij = dt V_f (k_i + k_j) c_ij r_ij; dt V_f applied once

@@ -192,38 +192,70 @@ __global__ void k_scan_add(SortSets S, uint32_t n) {
 
 // Fluid neighbours of the samples of dynamic bodies (the B->F lists the
 // gradient loop of computeRigidBodyGradient walks, TimeStepDiffDFSPH.cpp:1246-1273).
-__global__ void k_build_bf_nbrs(const float4* __restrict__ bpos, const uint32_t* __restrict__ dyn_idx, int nd,
-                                const float4* __restrict__ pos, GridDesc g, const uint32_t* __restrict__ fstart,
-                                float h2, int maxbf, uint32_t* __restrict__ bf, uint32_t* __restrict__ bfcnt,
-                                uint32_t* __restrict__ overflow) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= (uint32_t)nd) return;
-  const float4 p = bpos[dyn_idx[k]];
-  int cx, cy, cz;
-  cell_coords(g, p.x, p.y, p.z, cx, cy, cz);
-  uint32_t n = 0;
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int z = cz + dz;
-    if (z < 0 || z >= g.n[2]) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int y = cy + dy;
-      if (y < 0 || y >= g.n[1]) continue;
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int x = cx + dx;
-        if (x < 0 || x >= g.n[0]) continue;
+// One wavefront per dynamic sample.  The candidates of its 27 cells (cells
+// in (dz, dy, dx) order, particles of a cell in sorted order) are numbered
+// consecutively; each round tests 64 of them, one per lane, and the hits are
+// placed by their rank in a ballot, so the list comes out in candidate order,
+// as a sequential scan would write it, while the loads of a round are in
+// flight together.
+constexpr int BF_WAVES = 4;
+__global__ void __launch_bounds__(64 * BF_WAVES)
+k_build_bf_nbrs(const float4* __restrict__ bpos, const uint32_t* __restrict__ dyn_idx, int nd,
+                const float4* __restrict__ pos, GridDesc g, const uint32_t* __restrict__ fstart, float h2, int maxbf,
+                uint32_t* __restrict__ bf, uint32_t* __restrict__ bfcnt, uint32_t* __restrict__ overflow) {
+  __shared__ uint32_t cst[BF_WAVES][27], cpre[BF_WAVES][28];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t k = blockIdx.x * BF_WAVES + w;
+  const bool valid = k < (uint32_t)nd;  // uniform over the wavefront
+  float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (valid) p = bpos[dyn_idx[k]];
+  if (lane < 27) {
+    uint32_t st = 0, cnt = 0;
+    if (valid) {
+      int cx, cy, cz;
+      cell_coords(g, p.x, p.y, p.z, cx, cy, cz);
+      const int x = cx + lane % 3 - 1, y = cy + (lane / 3) % 3 - 1, z = cz + lane / 9 - 1;
+      if (x >= 0 && y >= 0 && z >= 0 && x < g.n[0] && y < g.n[1] && z < g.n[2]) {
         const uint32_t c = cell_key(g, x, y, z);
-        for (uint32_t j = fstart[c]; j < fstart[c + 1]; ++j) {
-          const float4 q = pos[j];
-          if (dist2_exact(q.x, q.y, q.z, p.x, p.y, p.z) < h2) {
-            if (n < (uint32_t)maxbf) bf[(size_t)n * nd + k] = j;
-            ++n;
-          }
-        }
+        st = fstart[c];
+        cnt = fstart[c + 1] - st;
       }
     }
+    cst[w][lane] = st;
+    cpre[w][lane + 1] = cnt;
   }
-  if (n > (uint32_t)maxbf) { atomicMax(overflow, n); n = maxbf; }
-  bfcnt[k] = n;
+  __syncthreads();
+  if (lane == 0) {
+    cpre[w][0] = 0;
+    for (int c = 1; c <= 27; ++c) cpre[w][c] += cpre[w][c - 1];
+  }
+  __syncthreads();
+  if (!valid) return;
+  const uint32_t total = cpre[w][27];
+  uint32_t n = 0;
+  for (uint32_t base = 0; base < total; base += 64) {
+    const uint32_t t = base + (uint32_t)lane;
+    bool hit = false;
+    uint32_t j = 0;
+    if (t < total) {
+      int lo = 0, hi = 26;  // the cell holding candidate t: cpre[c] <= t < cpre[c + 1]
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cpre[w][mid] <= t) lo = mid; else hi = mid - 1;
+      }
+      j = cst[w][lo] + (t - cpre[w][lo]);
+      const float4 q = pos[j];
+      hit = dist2_exact(q.x, q.y, q.z, p.x, p.y, p.z) < h2;
+    }
+    const unsigned long long m = __ballot(hit);
+    const uint32_t at = n + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (hit && at < (uint32_t)maxbf) bf[(size_t)at * nd + k] = j;
+    n += (uint32_t)__popcll(m);
+  }
+  if (lane == 0) {
+    if (n > (uint32_t)maxbf) { atomicMax(overflow, n); n = maxbf; }
+    bfcnt[k] = n;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -281,8 +313,8 @@ void launch_sort_boundary(BoundaryBufs& b, const GridDesc& g, GridBufs& gb, hipS
 void launch_build_bf(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
                      hipStream_t s) {
   if (b.nd > 0) {
-    hipLaunchKernelGGL(k_build_bf_nbrs, dim3(blocks_for(b.nd)), dim3(DFR_BLOCK), 0, s, b.pos, b.dyn_idx, b.nd, f.pos,
-                       g, gb.fstart, h2, nb.maxbf, nb.bf, nb.bfcnt, nb.overflow + 1);
+    hipLaunchKernelGGL(k_build_bf_nbrs, dim3(blocks_for(b.nd, BF_WAVES)), dim3(64 * BF_WAVES), 0, s, b.pos,
+                       b.dyn_idx, b.nd, f.pos, g, gb.fstart, h2, nb.maxbf, nb.bf, nb.bfcnt, nb.overflow + 1);
   }
 }
 

@@ -175,6 +175,7 @@ struct AdjState {
   StepTapeDev tape;
   std::vector<double> start_grad;          // dL/d(x, q, v, omega) at the window start, 13 per body
   int64_t seg_cap = -1;                    // bytes one segment's kept tapes may take (-1: free memory)
+  std::vector<Snapshot> user;              // states kept by dfr_state_save
 };
 
 }  // namespace
@@ -1351,6 +1352,7 @@ int dfr_destroy(dfr_ctx* c) {
     adjoint_free(c);
   } catch (...) {
   }
+  for (auto& S : c->adj.user) snapshot_free(S);
   for (auto& r : c->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   for (void* p : c->allocations) (void)hipFree(p);
@@ -1574,6 +1576,8 @@ int dfr_reset(dfr_ctx* c) {
     place_boundary_and_volumes(c);
     c->time = 0.f;
     c->step_count = 0;
+    c->stats.time = 0.0;
+    c->stats.step_count = 0;
     return DFR_OK;
   });
 }
@@ -1975,6 +1979,35 @@ int dfr_adjoint_backward(dfr_ctx* c, const double* seed, double* grad) {
       return fail(c, DFR_ERR_RANGE,
                   "the exact gradient left the FP32 range: it grows exponentially once fluid and body interact "
                   "strongly; use a shorter window or the forward-mode gradient");
+    return DFR_OK;
+  });
+}
+
+int dfr_state_save(dfr_ctx* c, int slot) {
+  if (!c || slot < 0 || slot > 4095) return DFR_ERR_ARG;
+  return guarded(c, [&]() {
+    if (!c->initialized) return fail(c, DFR_ERR_STATE, "dfr_state_save before dfr_init");
+    HIPCHK(hipSetDevice(c->device));
+    AdjState& A = c->adj;
+    if ((int)A.user.size() <= slot) A.user.resize(slot + 1);
+    snapshot_save(c, A.user[slot]);
+    sync_check(c);
+    return DFR_OK;
+  });
+}
+
+int dfr_state_load(dfr_ctx* c, int slot) {
+  if (!c || slot < 0) return DFR_ERR_ARG;
+  return guarded(c, [&]() {
+    AdjState& A = c->adj;
+    if (slot >= (int)A.user.size() || !A.user[slot].block)
+      return fail(c, DFR_ERR_ARG, "dfr_state_load: no state saved under this slot");
+    HIPCHK(hipSetDevice(c->device));
+    adjoint_release(c);
+    snapshot_restore(c, A.user[slot]);
+    sync_check(c);
+    c->stats.time = c->time;
+    c->stats.step_count = c->step_count;
     return DFR_OK;
   });
 }

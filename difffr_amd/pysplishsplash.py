@@ -430,6 +430,7 @@ class _SimulatorBase:
         self._gui = None
         self._gradient_mode = 1
         self._values = {self.STATE_EXPORT: False, self.STATE_EXPORT_FPS: 25.0, self.PAUSE: False}
+        self._states: dict = {}  # saveState names -> device state slots
 
     def init(self, sceneFile="", programName="pySPlisHSPlasH", useCache=True, stateFile="",
              loadFluidPos=False, loadFluidPosAndVel=False, outputDir="", initialPause=True, useGui=True,
@@ -459,6 +460,7 @@ class _SimulatorBase:
             raise DfrError("SimulatorBase.init must run before initSimulation")
         if self._core is not None:
             self._core.close()
+        self._states = {}
         self._core = _CoreSimulation(self._scene)
         self._timestep = TimeStepDiffDFSPH(self)
         self._simulation = Simulation(self)
@@ -512,6 +514,28 @@ class _SimulatorBase:
 
     def getValueBool(self, pid: int) -> bool:
         return bool(self._values[pid])
+
+    def saveState(self, stateFileDir: str = "") -> str:
+        """Keeps the current state (SimulatorBase::saveState,
+        SimulatorBase.cpp:1899-1960) in device memory and returns the name the
+        reference would give its file (without ".bin"); no file is written.
+        loadStateWithRigidExisted accepts that name, with or without ".bin"."""
+        core = self._require()
+        step = core.get_step_count()
+        d = stateFileDir or os.path.join(self._output, "state")
+        name = os.path.join(d, f"state_step{step}")
+        slot = self._states.setdefault(name, len(self._states))
+        core.state_save(slot)
+        return name
+
+    def loadStateWithRigidExisted(self, stateFile: str):
+        """Restores a state kept by saveState (SimulatorBase::loadStateWithRigidExisted,
+        SimulatorBase.cpp:2124-2290); state files on disk are not read."""
+        core = self._require()
+        name = stateFile[:-4] if stateFile.endswith(".bin") else stateFile
+        if name not in self._states:
+            raise NotImplementedError("state files are not read by this library; only states kept by saveState")
+        core.state_load(self._states[name])
 
     def getRigidBodyGradientManager(self):
         raise NotImplementedError("the multi-body RigidBodyGradientManager path is not part of this library")

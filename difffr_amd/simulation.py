@@ -319,6 +319,14 @@ class Simulation:
         self._check(native.load().dfr_adjoint_start_gradient(self.ctx, _dp(out)))
         return out
 
+    def state_save(self, slot: int = 0):
+        """Keep the current state in device memory under `slot`."""
+        self._check(native.load().dfr_state_save(self.ctx, int(slot)))
+
+    def state_load(self, slot: int = 0):
+        """Put back the state kept under `slot`."""
+        self._check(native.load().dfr_state_load(self.ctx, int(slot)))
+
     def adjoint_segment_memory(self, nbytes: int = -1):
         """Bound on one checkpoint segment's kept tapes in the backward pass
         (-1: free device memory; 0: re-run every step twice instead)."""

@@ -119,6 +119,17 @@ int dfr_get_rigid(const dfr_ctx* ctx, int body, double out[19]);
 /* Overwrites the rigid state x, q, v, omega (in[13]). */
 int dfr_set_rigid(dfr_ctx* ctx, int body, const double in[13]);
 
+/* Keeps the current state (fluid positions, velocities and stiffness fields,
+ * boundary samples, rigid bodies with their forward-mode Jacobians, time and
+ * step count) in device memory under slot (0..4095) / puts it back.  The
+ * in-memory counterpart of SimulatorBase::saveState and
+ * loadStateWithRigidExisted (Simulator/SimulatorBase.cpp:1899-1960,
+ * :2124-2290) for loops that restart from a state, e.g. the per-step finite
+ * differences of experiments/others/python/finite-difference.py:55-110.
+ * dfr_state_load ends an adjoint recording. */
+int dfr_state_save(dfr_ctx* ctx, int slot);
+int dfr_state_load(dfr_ctx* ctx, int slot);
+
 /* Body properties (any pointer may be NULL): the mass, (4/3) pi r^3 rho_body
  * per boundary sample for static bodies too (RigidBodyObject::getMass,
  * Dynamic3dRigidBody.h:59,184-198); whether the body is dynamic

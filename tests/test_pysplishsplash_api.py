@@ -53,7 +53,8 @@ SURFACE = {
         "init", "initSimulation", "initSimulationWithDeferredInit", "runSimulation", "runNewTrajectory",
         "singleTimeStep", "timeStep", "timeStepNoGUI", "forwardFixedSteps", "runFixedTimeSteps", "reset",
         "cleanup", "setTimeStepCB", "setResetCB", "setGui", "getOutputPath", "getRigidBodyGradientManager",
-        "setStateExportPath", "setValueBool", "setValueFloat", "setValueInt", "STATE_EXPORT",
+        "setStateExportPath", "setValueBool", "setValueFloat", "setValueInt", "saveState",
+        "loadStateWithRigidExisted", "STATE_EXPORT",
         "STATE_EXPORT_FPS", "STOP_AT"],
     sph.Simulation: ["getCurrent", "getTimeStep", "getBoundaryModel", "numberOfBoundaryModels",
                      "setGradientMode", "getGradientMode", "useRigidGradientManager"],

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -56,6 +57,10 @@ struct Snapshot {
   std::vector<uint8_t> animated;
 };
 
+struct ArenaFull : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
 // Bump allocator over large device chunks for per-step tapes (kept across
 // rollouts, freed by dfr_adjoint_end / dfr_destroy).
 struct Arena {
@@ -69,8 +74,11 @@ struct Arena {
       const size_t sz = std::max(bytes, CHUNK);
       void* p = nullptr;
       const hipError_t e = hipMalloc(&p, sz);
-      if (e != hipSuccess) throw std::runtime_error("out of device memory for the adjoint tape "
-                                                    "(use checkpointing: dfr_adjoint_begin with checkpoint_every >= 1)");
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw ArenaFull("out of device memory for the adjoint tape "
+                        "(use checkpointing: dfr_adjoint_begin with checkpoint_every >= 1)");
+      }
       chunks.push_back({static_cast<char*>(p), sz});
       off = 0;
     }
@@ -1165,15 +1173,28 @@ bool reverse_segment_kept(dfr_ctx* c, int64_t seg, int64_t s0, int64_t s1) {
   snapshot_restore(c, A.ckpt[seg]);
   for (int64_t s = s0; s < s1; ++s) {
     StepTapeDev& T = A.tapes[(size_t)(s - s0)];
-    tape_alloc_fixed(c, T);
-    tape_swap_in(c, T);
-    c->force_iv = A.iv[s];
-    c->force_ip = A.ip[s];
-    c->rec = &T;
-    do_step(c);
-    c->rec = nullptr;
-    c->force_iv = c->force_ip = -1;
-    T.rig_post = arena_copy(c, (const RigidDev*)c->d_rig, c->bodies.size());
+    try {
+      tape_alloc_fixed(c, T);
+      tape_swap_in(c, T);
+      c->force_iv = A.iv[s];
+      c->force_ip = A.ip[s];
+      c->rec = &T;
+      do_step(c);
+      c->rec = nullptr;
+      c->force_iv = c->force_ip = -1;
+      T.rig_post = arena_copy(c, (const RigidDev*)c->d_rig, c->bodies.size());
+    } catch (const ArenaFull&) {
+      // a later step's tape outgrew the estimate: this segment goes the
+      // memory-light way (its checkpoint is restored before the re-run)
+      c->rec = nullptr;
+      c->force_iv = c->force_ip = -1;
+      (void)hipGetLastError();
+      HIPCHK(hipStreamSynchronize(c->stream));
+      tape_swap_out(c);
+      A.tapes.clear();
+      A.arena.reset();
+      return false;
+    }
     if (s == s0 && s1 - s0 > 1) {
       // room for the rest of the segment, with a margin for steps that iterate more
       size_t cap = 0;

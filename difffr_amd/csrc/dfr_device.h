@@ -101,7 +101,8 @@ struct PairKernel {
 };
 // Both evaluations are branch-free (selects only): a wavefront evaluates the
 // inner and outer polynomial for every pair instead of diverging on q <= 1/2.
-__device__ inline PairKernel cubic_pair(const KernelConst& K, float r2) {
+// on = false gives W = 0 and c = 0 (a slot that contributes nothing).
+__device__ inline PairKernel cubic_pair(const KernelConst& K, float r2, bool on = true) {
   const float irl = __builtin_amdgcn_rsqf(r2);
   const float rl = r2 * irl;
   const float q = rl * K.inv_h;
@@ -114,8 +115,8 @@ __device__ inline PairKernel cubic_pair(const KernelConst& K, float r2) {
   const bool far = !(q <= 1.f);
   PairKernel o;
   // |r| <= 1e-5: gradient is zero, W = W(0)
-  o.W = (r2 > 1.0e-10f) ? (far ? 0.f : (inner ? Win : Wout)) : K.k;
-  o.c = (r2 > 1.0e-10f && !far) ? (inner ? cin : cout) : 0.f;
+  o.W = on ? ((r2 > 1.0e-10f) ? (far ? 0.f : (inner ? Win : Wout)) : K.k) : 0.f;
+  o.c = (on && r2 > 1.0e-10f && !far) ? (inner ? cin : cout) : 0.f;
   return o;
 }
 // on = false gives 0 (a pair that contributes nothing), folded into the

@@ -267,9 +267,9 @@ k_density_factor_t(PassArgs a, float* __restrict__ rho, float* __restrict__ alph
     // a slot that is not live contributes W = 0 and gradW = 0 (exact no-ops)
     auto pair = [&](float qx, float qy, float qz, float vx, float vy, float vz, bool live) {
       const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
-      const PairKernel kw = cubic_pair(c.K, rx * rx + ry * ry + rz * rz);
-      const float W = live ? kw.W : 0.f, kc = live ? kw.c : 0.f;
-      wsum += W;
+      const PairKernel kw = cubic_pair(c.K, rx * rx + ry * ry + rz * rz, live);
+      const float kc = kw.c;
+      wsum += kw.W;
       const float wx = kc * rx, wy = kc * ry, wz = kc * rz;
       sq += wx * wx + wy * wy + wz * wz;
       fgx += wx; fgy += wy; fgz += wz;

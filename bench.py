@@ -1,18 +1,31 @@
 """Throughput benchmark of the differentiable DFSPH fluid-rigid step on one MI355X.
 
-Prints ONE JSON line.  The timed region is a K-step rollout of the 1M-particle
-tank (neighbour search, density/factor, divergence and pressure Jacobi solves
-with Akinci coupling, rigid update) that keeps every step's tape in HBM
-(`--checkpoint-every 0`; with k >= 1 it checkpoints every k steps and re-runs
-the steps during the backward pass instead), followed by its exact adjoint
-(the reverse kernels), i.e. forward + adjoint particle-steps per second.  Inputs are resident in HBM; the rollout starts from a state
-settled by W untimed steps.  It is timed twice on the same seed: with the
-reference's warm start (halved previous kappa) and with the learned warm start,
-whose network is first trained (untimed) on pairs captured from a rollout of
-another seed; `value` is the learned run, both runs iterate to the same
-tolerance.
+Prints ONE JSON line.  The workload (default: the 1M-particle tank with one
+sphere) is fixed before any timing:
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--checkpoint-every k] [--no-cpu-baseline]
+  1. settle: the tank runs SETTLE_STEPS plain steps from its jittered lattice
+     (untimed); that state is kept in device memory;
+  2. drop: every dynamic body is placed just above the fluid surface at a
+     seeded position, rotation, velocity and angular velocity (the held-out
+     evaluation seed; the warm-start network never saw it).
+
+From that state the timed region is a K-step rollout (neighbour search,
+density/factor, divergence and pressure Jacobi solves to 0.1 % / 0.01 %,
+Akinci coupling, rigid update) that keeps every step's tape in HBM
+(`--checkpoint-every 0`; k >= 1 checkpoints every k steps and re-runs the steps
+in the backward pass), followed by its exact adjoint: forward + adjoint
+particle-steps per second, inputs resident in HBM.  It is timed twice from the
+same state: with the reference's warm start (halved previous kappa) and with
+the learned warm start, whose network is first trained (untimed) on pairs
+captured from 64 other seeded drops from the settled state.  `value` is the
+learned run; both iterate to the same tolerance.  W warm-up steps of the same
+window (recorded + reversed, every kernel class timed, untimed) precede each
+timed run.  The reference's own gradient (DiffFR forward mode, gradient mode 1)
+is timed over the same window on the GPU, and the CPU oracle runs both the
+reference path and the exact adjoint on a bounded sample of the same state.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload tank_1m|tank_250k|tank_4m]
+                  [--checkpoint-every k] [--no-learned] [--no-cpu-baseline]
 """
 from __future__ import annotations
 
@@ -29,25 +42,31 @@ sys.path.insert(0, ROOT)
 
 METRIC = "fluid particle-steps/s (fwd+adjoint) @1M, learned vs plain at 0.01% density err"
 HBM_PEAK_GBS = 8000.0
+SETTLE_STEPS = 300      # plain steps from the lattice before the drop (part of the workload)
+STATE_SLOT = 0          # device slot of the settled state
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="tank_1m", choices=["tank_1m", "tank_250k", "tank_4m"])
-    ap.add_argument("--checkpoint-every", type=int, default=0,
+    ap.add_argument("--checkpoint-every", type=int, default=None,
                     help="0: keep every step's tape in HBM (no recomputation); k >= 1: checkpoint every k steps "
-                         "and re-run the steps during the backward pass")
+                         "and re-run the steps during the backward pass (default: 0, 20 for tank_4m)")
     ap.add_argument("--no-learned", action="store_true", help="measure the plain warm start only")
-    ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--train-drops", type=int, default=64)
+    ap.add_argument("--train-steps", type=int, default=25, help="captured steps per training drop")
     ap.add_argument("--train-iters", type=int, default=2000)
-    ap.add_argument("--capture-fraction", type=float, default=0.02)
+    ap.add_argument("--train-pairs", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--seed", type=int, default=0)
-    return ap.parse_args()
+    a = ap.parse_args()
+    if a.checkpoint_every is None:
+        a.checkpoint_every = 20 if a.workload == "tank_4m" else 0
+    return a
 
 
 def own_bytes_per_particle():
@@ -73,105 +92,152 @@ def own_bytes_per_particle():
 
 def load_traffic(kernel: str, workload: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
-    calibrated on a copy kernel), committed under profiles/."""
+    calibrated on a copy kernel), committed under profiles/ with the command that
+    measured them."""
     p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
     if os.path.exists(p):
         with open(p) as f:
             d = json.load(f)
-        return d.get("bytes_per_launch")
-    return None
+        return d.get("bytes_per_launch"), d.get("state", d.get("command"))
+    return None, None
 
 
-def cpu_baseline(scene, steps: int):
-    """Forward + exact adjoint of the CPU oracle (FP32 forward as in the reference,
-    OpenMP; FP64 scatter adjoint with per-thread accumulators) on `steps` steps of
-    the same scene."""
-    from oracle.pyoracle import OracleSim
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    o = OracleSim(scene, "f32", {"compute_gradient": 0})
-    o.step(1)  # warm-up step (allocations, first neighbour search)
-    o.set_record(True)
-    t0 = time.perf_counter()
-    o.step(steps)
-    t1 = time.perf_counter()
-    seed = np.zeros((o.n_bodies, 7))
-    seed[-1, :] = 1.0
-    o.adjoint(seed)
-    t2 = time.perf_counter()
-    return dict(value=scene.n_fluid * steps / (t2 - t0), unit="fluid particle-steps/s", cores=threads, kind="port",
-                forward_s=t1 - t0, adjoint_s=t2 - t1,
-                sample=f"{steps} steps of the same {scene.n_fluid}-particle scene after 1 warm-up step, "
-                       f"forward + exact adjoint of the CPU oracle (FP32 forward, FP64 adjoint, OpenMP)")
+def cpu_info() -> dict:
+    """Host CPU model, physical cores (unique (package, core) pairs) and the CPUs
+    this process may run on."""
+    model, pairs, logical = "", set(), 0
+    try:
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    logical += 1
+                elif k == "model name" and not model:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None and core is not None:
+                    pairs.add((phys, core))
+                    phys = core = None
+        if phys is not None and core is not None:
+            pairs.add((phys, core))
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"cpu_model": model, "physical_cores": len(pairs) or None, "logical_cpus": logical or None,
+            "cpus_allowed": allowed}
 
 
-def train_network(args, build):
-    """Untimed: capture (features, converged kappa) pairs from a plain rollout of a
-    training seed and fit the warm-start network on the GPU."""
+def drop(sim, sc, seed: int):
+    """Places every dynamic body just above the fluid surface: seeded position
+    across the tank (bodies spread along x), rotation, velocity (downward
+    vertical component) and angular velocity."""
+    rng = np.random.default_rng(seed)
+    r = float(sc.params["radius"])
+    top = float(sc.fluid_x[:, 1].max())
+    walls = sc.bodies[0].x0
+    lo, hi = walls.min(0), walls.max(0)
+    dyn = [k for k, b in enumerate(sc.bodies) if b.dynamic]
+    for j, k in enumerate(dyn):
+        b = sc.bodies[k]
+        rad = float(np.linalg.norm(b.x0, axis=1).max())
+        margin = rad + 4 * r
+        x = lo[0] + (hi[0] - lo[0]) * (j + 0.5 + rng.uniform(-0.3, 0.3)) / len(dyn)
+        z = lo[2] + (hi[2] - lo[2]) * rng.uniform(0.3, 0.7)
+        pos = np.array([np.clip(x, lo[0] + margin, hi[0] - margin), top + rad + 2 * r,
+                        np.clip(z, lo[2] + margin, hi[2] - margin)])
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        v = np.array([rng.uniform(-2, 2), rng.uniform(-2, 0), rng.uniform(-2, 2)])
+        w = rng.uniform(-3, 3, size=3)
+        sim.set_rigid(k, pos, q, v, w)
+
+
+def start_window(sim, sc, drop_seed: int):
+    sim.state_load(STATE_SLOT)
+    drop(sim, sc, drop_seed)
+
+
+def train_network(args, sim, sc):
+    """Untimed: capture (features, converged kappa) pairs from rollouts of
+    `train_drops` seeded drops from the settled state (plain warm start) and fit
+    the warm-start network on the GPU."""
     import torch
-    from difffr_amd.simulation import Simulation
-    sim = Simulation(build(seed=args.seed + 1000, steps=args.train_steps), params_override={"compute_gradient": 0})
-    sim.mlp_capture(capacity=4_000_000, fraction=args.capture_fraction, seed=args.seed)
+    per_drop = max(sc.n_fluid * args.train_steps, 1)
+    fraction = min(1.0, args.train_pairs / (args.train_drops * per_drop))
     t0 = time.perf_counter()
-    sim.step(args.train_steps)
+    for d in range(args.train_drops):
+        start_window(sim, sc, 1 + d)
+        sim.mlp_capture(capacity=args.train_pairs, fraction=fraction, seed=d)  # appends after the first drop
+        sim.step(args.train_steps)
     sim.mlp_capture(0)
     n_pairs = sim.mlp_dataset()[0].shape[0]
+    torch.cuda.synchronize()
     t1 = time.perf_counter()
     losses = sim.mlp_train(args.train_iters, batch=16384, lr=2e-3, seed=args.seed)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     P, norm = sim.mlp_get_params()
-    sim.close()
-    info = {"training_seed": args.seed + 1000, "pairs": int(n_pairs), "capture_steps": args.train_steps,
-            "adam_iterations": args.train_iters, "batch": 16384,
-            "loss_first": float(losses[:50].mean()), "loss_last": float(losses[-50:].mean()),
+    info = {"training_drops": f"seeds 1..{args.train_drops}", "steps_per_drop": args.train_steps,
+            "pairs": int(n_pairs), "capture_fraction": fraction, "adam_iterations": args.train_iters,
+            "batch": 16384, "loss_first": float(losses[:50].mean()), "loss_last": float(losses[-50:].mean()),
             "capture_s": t1 - t0, "train_s": t2 - t1}
     return P, norm, info
 
 
-def timed_run(args, sc, learned, P=None, norm=None):
-    """Settle W steps (plain), then time a K-step rollout with checkpoints + its
-    exact adjoint, bracketed by device synchronisation.  The last warm-up steps
-    are an untimed rollout + adjoint with every kernel class timed (HIP events
-    on the library's stream), which names the dominant class; in the timed run
-    only that class carries events, so its live average launch time comes from
-    the timed region without instrumenting the rest."""
+def timed_run(args, sim, sc, eval_seed: int, learned: bool, P=None, norm=None):
+    """W warm-up steps of the window (recorded and reversed with every kernel
+    class timed: names the dominant class), an untimed rollout that sizes the
+    tape arena for K steps, then the timed K-step rollout + exact adjoint from
+    the same state, bracketed by device synchronisation.  In the timed run only
+    the dominant class carries events (HIP events on the library's stream)."""
     import torch
-    from difffr_amd.simulation import AdjointRangeError, Simulation
-    # the exact adjoint replaces DiffFR's forward-mode gradient on this path
-    sim = Simulation(sc, params_override={"compute_gradient": 0})
-    grow = min(args.steps, args.warmup)
-    sim.step(args.warmup - grow)
+    from difffr_amd.simulation import AdjointRangeError
+    k = args.checkpoint_every
     if learned:
         sim.mlp_set_params(P, norm)
-        sim.mlp_enable(True)
-    # the last `grow` warm-up steps record an (untimed) rollout, so the tape arena
-    # and checkpoint blocks are allocated before the timed one reuses them
+    sim.mlp_enable(learned)
     seed = np.zeros((sim.n_bodies, 7))
     seed[-1] = np.random.default_rng(args.seed).normal(size=7)
-    sim.adjoint_begin(args.checkpoint_every)
-    sim.reset_stats()
-    sim.set_profiling(True)
-    sim.step(grow)
-    try:
-        sim.adjoint_backward(seed)
-    except AdjointRangeError:
-        pass  # the reverse passes ran in full; only their result left the FP32 range
-    sim.set_profiling(False)
-    times = sim.kernel_times()
-    dominant = max(times, key=lambda k: times[k][0] * times[k][1])
+    dominant, times = "adj_velocity_update", {}
+    if args.warmup > 0:
+        start_window(sim, sc, eval_seed)
+        sim.reset_stats()
+        sim.set_profiling(True)
+        sim.adjoint_begin(k)
+        sim.step(args.warmup)
+        try:
+            sim.adjoint_backward(seed)
+        except AdjointRangeError:
+            pass  # the reverse passes ran in full; only their result left the FP32 range
+        sim.set_profiling(False)
+        times = sim.kernel_times()
+        dominant = max(times, key=lambda c: times[c][0] * times[c][1])
+    if k == 0 and args.steps > args.warmup:
+        start_window(sim, sc, eval_seed)  # tape arena for K steps (allocation stays outside the timed region)
+        sim.adjoint_begin(0)
+        sim.step(args.steps)
+    start_window(sim, sc, eval_seed)
     sim.reset_stats()
     sim.set_profiling(True, classes=[dominant])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sim.adjoint_begin(args.checkpoint_every)
+    sim.adjoint_begin(k)
     sim.step(args.steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     try:
-        grad = sim.adjoint_backward(seed)
+        sim.adjoint_backward(seed)
         range_note = None
     except AdjointRangeError as e:
-        grad, range_note = e.grad, str(e)
+        range_note = str(e)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     start_grad = None if range_note else sim.adjoint_start_gradient()[-1]
@@ -180,17 +246,103 @@ def timed_run(args, sc, learned, P=None, norm=None):
     live = sim.kernel_times()[dominant]
     nst = sim.neighbor_stats()
     nst.update(sim.halo_stats())
-    sim.adjoint_end()
-    sim.close()
     steps_done = max(st["steps_since_reset"], 1)
     n = sc.n_fluid
     return {"elapsed": t2 - t0, "fwd": t1 - t0, "adj": t2 - t1, "value": n * args.steps / (t2 - t0),
             "forward_value": n * args.steps / (t1 - t0), "times": times, "nst": nst,
             "dominant": dominant, "live": live,
             "I_p": st["total_iterations"] / steps_done, "I_v": st["total_iterations_v"] / steps_done,
-            "err": st["avg_density_err"], "grad": grad[-1].tolist(),
+            "err": st["avg_density_err"],
             "start_grad_norm": None if start_grad is None else float(np.linalg.norm(start_grad)),
             "range_note": range_note}
+
+
+def reference_mode_run(args, sim, sc, eval_seed: int):
+    """DiffFR's own gradient (forward mode, gradient mode 1: pair Jacobians every
+    Jacobi iteration and the per-step chain rule) over the same window, plain
+    warm start: forward particle-steps/s on the GPU."""
+    import torch
+    sim.mlp_enable(False)
+    start_window(sim, sc, eval_seed)
+    sim.set_compute_gradient(True)
+    sim.reset_stats()
+    sim.set_profiling(True, classes=["gradient_pairs"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sim.set_profiling(False)
+    sim.set_compute_gradient(False)
+    st = sim.stats()
+    g = sim.kernel_times()["gradient_pairs"]
+    return {"forward_with_gradient_value": sc.n_fluid * args.steps / (t1 - t0),
+            "ms_per_step": 1e3 * (t1 - t0) / args.steps,
+            "mean_pressure_iterations": st["total_iterations"] / max(st["steps_since_reset"], 1),
+            "gradient_pairs_avg_ms": g[0], "gradient_pairs_launches": g[1]}
+
+
+def window_scene(sim, sc):
+    """The current GPU state (fluid in particle-id order, bodies, stiffness
+    fields, time) as a scene + state for the CPU oracle."""
+    from difffr_amd import scenes
+    x, v = sim.fluid()
+    bodies = []
+    rig = []
+    for k, b in enumerate(sc.bodies):
+        s = sim.rigid(k)
+        rig.append(np.concatenate([s["x"], s["q"], s["v"], s["omega"]]))
+        bodies.append(scenes.Body(b.x0, b.dynamic, s["x"].copy(), s["q"].copy(), b.density, name=b.name))
+    state = {"kappa": sim.field("kappa").astype(np.float64), "kappa_v": sim.field("kappa_v").astype(np.float64),
+             "time": np.array([sim.stats()["time"]]), "rigid": rig}
+    return scenes.Scene(sc.name + "_window", dict(sc.params), x.astype(np.float64), v.astype(np.float64),
+                        bodies, 0), state
+
+
+def cpu_baseline(wsc, state, steps: int):
+    """The CPU oracle (OpenMP, FP32 fluid) from the GPU window's start state:
+    (1) the reference's path, forward + DiffFR's forward-mode gradient (gradient
+    mode 1), and (2) forward + the exact FP64 adjoint; `steps` steps each after
+    one warm-up step."""
+    from oracle.pyoracle import OracleSim
+    threads = int(os.environ.get("OMP_NUM_THREADS", 0) or (os.cpu_count() or 1))
+
+    def make(params):
+        o = OracleSim(wsc, "f32", params)
+        for name in ("kappa", "kappa_v", "time"):
+            o.set_state(name, state[name])
+        for k, r in enumerate(state["rigid"]):
+            o.set_state(f"rigid{k}", r)
+        return o
+
+    o = make({"compute_gradient": 1, "gradient_mode": 1})
+    o.step(1)
+    t0 = time.perf_counter()
+    o.step(steps)
+    t1 = time.perf_counter()
+    ref = wsc.n_fluid * steps / (t1 - t0)
+    del o
+    o = make({"compute_gradient": 0})
+    o.step(1)
+    o.set_record(True)
+    t2 = time.perf_counter()
+    o.step(steps)
+    t3 = time.perf_counter()
+    seed = np.zeros((o.n_bodies, 7))
+    seed[-1, :] = 1.0
+    o.adjoint(seed)
+    t4 = time.perf_counter()
+    del o
+    out = dict(value=ref, unit="fluid particle-steps/s", cores=threads, kind="port",
+               sample=f"{steps} steps (after 1 warm-up step) of the GPU window's start state "
+                      f"({wsc.n_fluid} fluid particles, settled tank + drop), CPU oracle FP32 forward + DiffFR "
+                      f"forward-mode gradient (gradient mode 1), OpenMP",
+               reference_path_s=t1 - t0,
+               exact_adjoint_leg={"value": wsc.n_fluid * steps / (t4 - t2), "forward_s": t3 - t2,
+                                  "adjoint_s": t4 - t3,
+                                  "what": "CPU oracle FP32 forward + FP64 exact adjoint, same sample"})
+    out.update(cpu_info())
+    return out
 
 
 def main():
@@ -201,20 +353,33 @@ def main():
         sys.exit(2)
     import torch
     from difffr_amd import scenes
+    from difffr_amd.simulation import Simulation
 
     assert torch.cuda.is_available(), "bench.py needs an MI355X"
     torch.cuda.init()
     build = {"tank_1m": scenes.tank_1m, "tank_250k": scenes.tank_250k, "tank_4m": scenes.tank_4m}[args.workload]
-    sc = build(seed=args.seed, steps=args.warmup + args.steps)
+    sc = build(seed=args.seed, steps=SETTLE_STEPS + args.steps)
     n = sc.n_fluid
-    plain = timed_run(args, sc, False)
+    eval_seed = 10_000 + args.seed  # held out: training drops use seeds 1..train_drops
+    # the exact adjoint replaces DiffFR's forward-mode gradient on the measured path
+    sim = Simulation(sc, params_override={"compute_gradient": 0})
+    sim.step(SETTLE_STEPS)
+    sim.state_save(STATE_SLOT)
+    plain = timed_run(args, sim, sc, eval_seed, False)
     learned_on = not args.no_learned
     if learned_on:
-        P, norm, tinfo = train_network(args, build)
-        learned = timed_run(args, sc, True, P, norm)
+        P, norm, tinfo = train_network(args, sim, sc)
+        learned = timed_run(args, sim, sc, eval_seed, True, P, norm)
         main_run = learned
     else:
         main_run = plain
+    refmode = reference_mode_run(args, sim, sc, eval_seed)
+    wsc = wstate = None
+    if not args.no_cpu_baseline:
+        start_window(sim, sc, eval_seed)
+        wsc, wstate = window_scene(sim, sc)
+    sim.adjoint_end()
+    sim.close()
 
     times, nst = main_run["times"], main_run["nst"]
     dominant = main_run["dominant"]
@@ -223,9 +388,10 @@ def main():
     bytes_per_launch = n * (4 + 4 * nbar + own[dominant]) if dominant in own else None
     avg_ms = main_run["live"][0]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if bytes_per_launch else None
-    traffic = load_traffic(dominant, args.workload)
+    traffic, traffic_state = load_traffic(dominant, args.workload)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+                "traffic_measured_on": traffic_state,
                 "kernel": dominant, "avg_launch_ms": avg_ms, "launches": main_run["live"][1],
                 "algorithmic_bytes_per_launch": bytes_per_launch}
     summary = lambda r: {"fwd_adj_value": r["value"], "forward_value": r["forward_value"],
@@ -240,34 +406,35 @@ def main():
         "metric": METRIC, "value": main_run["value"], "unit": "fluid particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * main_run["elapsed"] / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded jittered lattice, procedural rigid body; warm-start network trained on "
-                "another seed's rollout)",
+        "data": "synthetic (seeded jittered lattice settled for 300 steps, procedural rigid bodies dropped with a "
+                "held-out seed; warm-start network trained on 64 other seeded drops)",
         "config": {
-            "workload": f"{sc.name}: {n} fluid particles + {sc.n_boundary} boundary samples, "
-                        f"dt=1e-3, 0.01% density / 0.1% divergence tolerance",
+            "workload": f"{sc.name}: {n} fluid particles + {sc.n_boundary} boundary samples, dt=1e-3, "
+                        f"0.01% density / 0.1% divergence tolerance; {SETTLE_STEPS} settle steps, then the "
+                        f"bodies dropped (seed {eval_seed}); timed window = the {args.steps} steps after the drop",
             "path": f"{'learned' if learned_on else 'plain'} kappa warm start; forward rollout + exact adjoint "
-                    f"(every reverse pass of every step, seeded by dL/d(x, q) of the body; the body's initial "
-                    f"velocities act only in the release window at t = 0, so dL/d(v0, omega0) of this settled "
-                    f"rollout is zero; adjoint_start_state_grad_norm = |dL/d(x, q, v, omega)| at the window "
-                    f"start); "
+                    f"(every reverse pass of every step, seeded by dL/d(x, q) of the last body; "
+                    f"adjoint_start_state_grad_norm = |dL/d(x, q, v, omega)| at the window start); "
                     + ("every step's tape kept in HBM (no recomputation)" if args.checkpoint_every == 0 else
                        f"checkpoint every {args.checkpoint_every} step(s), steps re-run with recording in the "
                        f"backward pass"),
             "mean_fluid_neighbors": nst["mean_fluid"], "mean_boundary_neighbors": nst["mean_boundary"],
             "largest_halo": nst["largest_halo"], "halo_lds_capacity": nst["capacity"],
             "plain": summary(plain),
+            "reference_mode_gradient_gpu": refmode,
         },
         "roofline": roofline,
-        "kernel_ms": {k: {"avg_ms": v[0], "launches": v[1]} for k, v in times.items()},
-        "kernel_ms_source": "untimed rollout of the last warm-up steps + its adjoint, every class timed",
+        "kernel_ms": {c: {"avg_ms": v[0], "launches": v[1]} for c, v in times.items()},
+        "kernel_ms_source": "the W warm-up steps of the window + their adjoint, every class timed",
     }
     if learned_on:
         line["config"]["learned"] = summary(learned)
         line["config"]["learned_over_plain"] = learned["value"] / plain["value"]
+        line["config"]["forward_learned_over_plain"] = learned["forward_value"] / plain["forward_value"]
         line["config"]["pressure_iterations_saved"] = plain["I_p"] - learned["I_p"]
         line["config"]["training"] = tinfo
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
+    if wsc is not None:
+        line["cpu_baseline"] = cpu_baseline(wsc, wstate, args.cpu_steps)
     print(json.dumps(line))
 
 

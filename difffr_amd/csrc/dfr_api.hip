@@ -61,14 +61,22 @@ struct ArenaFull : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
+// A neighbour list outgrew its capacity (DFR_ERR_CAPACITY).
+struct CapacityError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
 // Bump allocator over large device chunks for per-step tapes (kept across
 // rollouts, freed by dfr_adjoint_end / dfr_destroy).
 struct Arena {
   std::vector<std::pair<char*, size_t>> chunks;
   size_t cur = 0, off = 0;
+  size_t limit = 0;  // > 0: bytes the arena may hand out (dfr_adjoint_arena_limit)
   static constexpr size_t CHUNK = (size_t)2 << 30;
   void* alloc(size_t bytes) {
     bytes = (bytes + 255) & ~(size_t)255;
+    if (limit && used() + bytes > limit)
+      throw ArenaFull("adjoint tape arena limit reached (dfr_adjoint_arena_limit)");
     while (cur < chunks.size() && chunks[cur].second - off < bytes) { ++cur; off = 0; }
     if (cur == chunks.size()) {
       const size_t sz = std::max(bytes, CHUNK);
@@ -183,6 +191,7 @@ struct AdjState {
   StepTapeDev tape;
   std::vector<double> start_grad;          // dL/d(x, q, v, omega) at the window start, 13 per body
   int64_t seg_cap = -1;                    // bytes one segment's kept tapes may take (-1: free memory)
+  bool light = false;                      // this backward pass re-runs every step twice (tapes did not fit)
   std::vector<Snapshot> user;              // states kept by dfr_state_save
 };
 
@@ -284,6 +293,10 @@ template <typename F> int guarded(dfr_ctx* c, F&& fn) {
     return fn();
   } catch (const HipError& e) {
     return fail(c, DFR_ERR_HIP, std::string("HIP error ") + hipGetErrorString(e.e) + " in " + e.where);
+  } catch (const CapacityError& e) {
+    return fail(c, DFR_ERR_CAPACITY, e.what());
+  } catch (const ArenaFull& e) {
+    return fail(c, DFR_ERR_HIP, e.what());
   } catch (const std::exception& e) {
     return fail(c, DFR_ERR_ARG, e.what());
   }
@@ -484,7 +497,7 @@ void check_overflow(dfr_ctx* c) {
                   c->h_overflow[0], MAXF, c->h_overflow[2], c->P.max_neighbors, c->h_overflow[1],
                   c->P.max_boundary_neighbors);
     c->h_overflow[0] = c->h_overflow[1] = c->h_overflow[2] = 0;
-    throw std::runtime_error(buf);
+    throw CapacityError(buf);
   }
 }
 
@@ -970,23 +983,29 @@ void tape_swap_in(dfr_ctx* c, StepTapeDev& T) {
   AdjState& A = c->adj;
   Arena& R = A.arena;
   const size_t NF = (size_t)std::max(c->f.n, 1);
+  // every buffer first (an allocation that fails leaves the context as it was)
+  AdjState::Live L{};
+  L.nbr8 = static_cast<uint4*>(R.alloc(sizeof(uint4) * NPACK * NF));
+  const size_t NB = (size_t)c->grid.bn[0] * c->grid.bn[1] * c->grid.bn[2];
+  L.thdr = static_cast<uint4*>(R.alloc(sizeof(uint4) * NB));
+  L.tmap = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * NB * TILE_CAP));
+  L.nbrB = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * (size_t)c->nb.maxb * NF));
+  L.cnt = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * NF));
+  L.fstart = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * ((size_t)c->grid.nkeys + 1)));
+  L.perm = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * NF));
+  L.rho = static_cast<float*>(R.alloc(sizeof(float) * NF));
+  L.alpha = static_cast<float*>(R.alloc(sizeof(float) * NF));
+  L.s = static_cast<float4*>(R.alloc(sizeof(float4) * NF));
+  L.aux = c->mlp.aux ? static_cast<float2*>(R.alloc(sizeof(float2) * NF)) : nullptr;
   if (!A.swapped) {
     A.own = {c->nb.nbr8, c->nb.thdr, c->nb.tmap, c->nb.nbrB, c->nb.cnt, c->gb.fstart, c->gb.perm, c->f.rho,
              c->f.alpha, c->f.s, c->mlp.aux};
     A.swapped = true;
   }
-  c->nb.nbr8 = static_cast<uint4*>(R.alloc(sizeof(uint4) * NPACK * NF));
-  const size_t NB = (size_t)c->grid.bn[0] * c->grid.bn[1] * c->grid.bn[2];
-  c->nb.thdr = static_cast<uint4*>(R.alloc(sizeof(uint4) * NB));
-  c->nb.tmap = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * NB * TILE_CAP));
-  c->nb.nbrB = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * (size_t)c->nb.maxb * NF));
-  c->nb.cnt = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * NF));
-  c->gb.fstart = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * ((size_t)c->grid.nkeys + 1)));
-  c->gb.perm = static_cast<uint32_t*>(R.alloc(sizeof(uint32_t) * NF));
-  c->f.rho = static_cast<float*>(R.alloc(sizeof(float) * NF));
-  c->f.alpha = static_cast<float*>(R.alloc(sizeof(float) * NF));
-  c->f.s = static_cast<float4*>(R.alloc(sizeof(float4) * NF));
-  if (c->mlp.aux) c->mlp.aux = static_cast<float2*>(R.alloc(sizeof(float2) * NF));
+  c->nb.nbr8 = L.nbr8; c->nb.thdr = L.thdr; c->nb.tmap = L.tmap; c->nb.nbrB = L.nbrB; c->nb.cnt = L.cnt;
+  c->gb.fstart = L.fstart; c->gb.perm = L.perm;
+  c->f.rho = L.rho; c->f.alpha = L.alpha; c->f.s = L.s;
+  if (c->mlp.aux) c->mlp.aux = L.aux;
   T.nbr8 = c->nb.nbr8; T.thdr = c->nb.thdr; T.tmap = c->nb.tmap; T.nbrB = c->nb.nbrB; T.cnt = c->nb.cnt;
   T.fstart = c->gb.fstart; T.perm = c->gb.perm;
   T.rho = c->f.rho; T.alpha = c->f.alpha; T.s = c->f.s; T.aux = c->mlp.aux;
@@ -1022,6 +1041,7 @@ void tape_swap_out(dfr_ctx* c) {
 // allocated for the next one until adjoint_free.
 void adjoint_release(dfr_ctx* c) {
   AdjState& A = c->adj;
+  c->rec = nullptr;
   tape_swap_out(c);
   A.nckpt = 0;
   A.iv.clear();
@@ -1166,7 +1186,7 @@ void replay_step(dfr_ctx* c, int64_t s, bool record) {
 // that the segment's tapes would not fit in device memory.
 bool reverse_segment_kept(dfr_ctx* c, int64_t seg, int64_t s0, int64_t s1) {
   AdjState& A = c->adj;
-  if (A.seg_cap == 0) return false;
+  if (A.seg_cap == 0 || A.light) return false;
   A.arena.reset();
   A.tapes.clear();
   A.tapes.resize((size_t)(s1 - s0));
@@ -1184,15 +1204,18 @@ bool reverse_segment_kept(dfr_ctx* c, int64_t seg, int64_t s0, int64_t s1) {
       c->force_iv = c->force_ip = -1;
       T.rig_post = arena_copy(c, (const RigidDev*)c->d_rig, c->bodies.size());
     } catch (const ArenaFull&) {
-      // a later step's tape outgrew the estimate: this segment goes the
-      // memory-light way (its checkpoint is restored before the re-run)
+      // a later step's tape outgrew the estimate: this segment and the rest
+      // of the pass go the memory-light way (the checkpoint is restored before
+      // the re-run); the arena's chunks are freed for the states that way keeps
       c->rec = nullptr;
       c->force_iv = c->force_ip = -1;
       (void)hipGetLastError();
       HIPCHK(hipStreamSynchronize(c->stream));
       tape_swap_out(c);
+      HIPCHK(hipStreamSynchronize(c->stream));
       A.tapes.clear();
-      A.arena.reset();
+      A.arena.release();
+      A.light = true;
       return false;
     }
     if (s == s0 && s1 - s0 > 1) {
@@ -1205,8 +1228,10 @@ bool reverse_segment_kept(dfr_ctx* c, int64_t seg, int64_t s0, int64_t s1) {
       const bool over = A.seg_cap > 0 ? need > (size_t)A.seg_cap : (need > cap && need - cap + ((size_t)2 << 30) > fr);
       if (over) {
         tape_swap_out(c);
+        HIPCHK(hipStreamSynchronize(c->stream));
         A.tapes.clear();
-        A.arena.reset();
+        A.arena.release();
+        A.light = true;
         return false;
       }
     }
@@ -1557,6 +1582,21 @@ int dfr_step(dfr_ctx* c, int nsteps) {
     if (!c->initialized) return fail(c, DFR_ERR_STATE, "dfr_step before dfr_init");
     HIPCHK(hipSetDevice(c->device));
     AdjState& A = c->adj;
+    // A step that fails ends the recording of the rollout (its tapes and
+    // checkpoints no longer describe the state), and leaves no step marked as
+    // being recorded.
+    struct StepGuard {
+      dfr_ctx* c;
+      bool ok = false;
+      ~StepGuard() {
+        c->rec = nullptr;
+        c->force_iv = c->force_ip = -1;
+        if (!ok && c->adj.active) {
+          (void)hipStreamSynchronize(c->stream);
+          try { adjoint_release(c); } catch (...) {}
+        }
+      }
+    } guard{c};
     for (int s = 0; s < nsteps; ++s) {
       if (A.active && A.k == 0) {  // keep every step's tape in HBM
         A.tapes.emplace_back();
@@ -1582,6 +1622,7 @@ int dfr_step(dfr_ctx* c, int nsteps) {
     }
     sync_check(c);
     check_overflow(c);
+    guard.ok = true;
     if (c->profiling) drain_timers(c);
     return DFR_OK;
   });
@@ -1651,6 +1692,12 @@ int dfr_get_body(const dfr_ctx* c, int body, double* mass, int* dynamic, int* an
 int dfr_set_gradient_mode(dfr_ctx* c, int mode) {
   if (!c || mode < 0 || mode > 2) return DFR_ERR_ARG;
   c->P.gradient_mode = mode;
+  return DFR_OK;
+}
+
+int dfr_set_compute_gradient(dfr_ctx* c, int on) {
+  if (!c) return DFR_ERR_ARG;
+  c->P.compute_gradient = on ? 1 : 0;
   return DFR_OK;
 }
 
@@ -1925,6 +1972,7 @@ int dfr_adjoint_backward(dfr_ctx* c, const double* seed, double* grad) {
     const int nbodies = (int)c->bodies.size();
     const dfr_stats saved_stats = c->stats;
     const bool stored = A.k == 0;
+    A.light = false;
     if (!stored) snapshot_save(c, A.final_state);
     std::vector<AdjBodyDev> hb(std::max(nbodies, 1));
     std::memset(hb.data(), 0, sizeof(AdjBodyDev) * hb.size());
@@ -2033,10 +2081,34 @@ int dfr_state_load(dfr_ctx* c, int slot) {
   });
 }
 
+int dfr_state_free(dfr_ctx* c, int slot) {
+  if (!c || slot < 0) return DFR_ERR_ARG;
+  return guarded(c, [&]() {
+    AdjState& A = c->adj;
+    if (slot < (int)A.user.size() && A.user[slot].block) {
+      HIPCHK(hipSetDevice(c->device));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      snapshot_free(A.user[slot]);
+    }
+    return DFR_OK;
+  });
+}
+
 int dfr_adjoint_segment_memory(dfr_ctx* c, int64_t bytes) {
   if (!c || bytes < -1) return DFR_ERR_ARG;
   c->adj.seg_cap = bytes;
   return DFR_OK;
+}
+
+int dfr_adjoint_arena_limit(dfr_ctx* c, int64_t bytes) {
+  if (!c || bytes < 0) return DFR_ERR_ARG;
+  c->adj.arena.limit = (size_t)bytes;
+  return DFR_OK;
+}
+
+int64_t dfr_adjoint_tape_bytes(const dfr_ctx* c) {
+  if (!c) return DFR_ERR_ARG;
+  return (int64_t)c->adj.arena.used();
 }
 
 int dfr_adjoint_start_gradient(const dfr_ctx* c, double* out) {
@@ -2083,6 +2155,7 @@ int dfr_mlp_capture(dfr_ctx* c, int64_t capacity, double fraction, uint64_t seed
     if (capacity == 0) { M.capturing = false; return DFR_OK; }
     if (capacity > (int64_t)0xffffffffu) return fail(c, DFR_ERR_ARG, "dfr_mlp_capture: capacity too large");
     if (capacity != M.cap) {
+      M.capturing = false;
       HIPCHK(hipStreamSynchronize(c->stream));
       for (void* p : {(void*)M.feat, (void*)M.targ, (void*)M.rows})
         if (p) { (void)hipFree(p); c->allocations.erase(std::find(c->allocations.begin(), c->allocations.end(), p)); }
@@ -2091,7 +2164,8 @@ int dfr_mlp_capture(dfr_ctx* c, int64_t capacity, double fraction, uint64_t seed
       M.rows = dalloc<uint32_t>(c, (size_t)capacity);
       M.cap = capacity;
     }
-    HIPCHK(hipMemsetAsync(M.count, 0, sizeof(uint32_t) * 2, c->stream));
+    // a capture already running with this capacity keeps its rows (appends)
+    if (!(M.capturing && capacity == M.cap)) HIPCHK(hipMemsetAsync(M.count, 0, sizeof(uint32_t) * 2, c->stream));
     M.threshold = fraction >= 1.0 ? 0xffffffffu : (uint32_t)(fraction * 4294967296.0);
     M.salt = seed * 0xBF58476D1CE4E5B9ull + 1;
     M.capturing = true;

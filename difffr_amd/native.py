@@ -26,7 +26,8 @@ EXPORTS = [
     "dfr_mlp_set_params", "dfr_mlp_get_params", "dfr_mlp_train", "dfr_mlp_enable", "dfr_mlp_predict",
     "dfr_mlp_loss_grad", "dfr_mlp_input_grad", "dfr_set_halo_capacity", "dfr_halo_stats",
     "dfr_set_profiling_classes", "dfr_get_body", "dfr_set_gradient_mode", "dfr_count_1ring", "dfr_adjoint_start_gradient",
-    "dfr_adjoint_segment_memory", "dfr_state_save", "dfr_state_load",
+    "dfr_adjoint_segment_memory", "dfr_state_save", "dfr_state_load", "dfr_adjoint_arena_limit",
+    "dfr_state_free", "dfr_adjoint_tape_bytes", "dfr_set_compute_gradient",
 ]
 
 MLP_NP = 8898
@@ -96,6 +97,7 @@ def load(path: str | None = None):
     L.dfr_neighbor_stats.argtypes = [P, dp]
     L.dfr_get_body.argtypes = [P, C.c_int, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.dfr_set_gradient_mode.argtypes = [P, C.c_int]
+    L.dfr_set_compute_gradient.argtypes = [P, C.c_int]
     L.dfr_count_1ring.argtypes = [P]
     L.dfr_count_1ring.restype = C.c_int64
     L.dfr_set_halo_capacity.argtypes = [P, C.c_int]
@@ -116,8 +118,12 @@ def load(path: str | None = None):
     L.dfr_adjoint_end.argtypes = [P]
     L.dfr_adjoint_start_gradient.argtypes = [P, dp]
     L.dfr_adjoint_segment_memory.argtypes = [P, C.c_int64]
+    L.dfr_adjoint_arena_limit.argtypes = [P, C.c_int64]
+    L.dfr_adjoint_tape_bytes.argtypes = [P]
+    L.dfr_adjoint_tape_bytes.restype = C.c_int64
     L.dfr_state_save.argtypes = [P, C.c_int]
     L.dfr_state_load.argtypes = [P, C.c_int]
+    L.dfr_state_free.argtypes = [P, C.c_int]
     L.dfr_adjoint_iterations.argtypes = [P, ip, ip, C.c_int64]
     L.dfr_adjoint_iterations.restype = C.c_int64
     L.dfr_set_fixed_iterations.argtypes = [P, C.c_int, ip, ip]

@@ -430,7 +430,7 @@ class _SimulatorBase:
         self._gui = None
         self._gradient_mode = 1
         self._values = {self.STATE_EXPORT: False, self.STATE_EXPORT_FPS: 25.0, self.PAUSE: False}
-        self._states: dict = {}  # saveState names -> device state slots
+        self._states: dict = {}  # saveState names -> device state slots, least recently used first
 
     def init(self, sceneFile="", programName="pySPlisHSPlasH", useCache=True, stateFile="",
              loadFluidPos=False, loadFluidPosAndVel=False, outputDir="", initialPause=True, useGui=True,
@@ -515,27 +515,53 @@ class _SimulatorBase:
     def getValueBool(self, pid: int) -> bool:
         return bool(self._values[pid])
 
+    # states kept in device memory at once; saving more evicts the least recently used
+    MAX_KEPT_STATES = 16
+
     def saveState(self, stateFileDir: str = "") -> str:
         """Keeps the current state (SimulatorBase::saveState,
         SimulatorBase.cpp:1899-1960) in device memory and returns the name the
-        reference would give its file (without ".bin"); no file is written.
-        loadStateWithRigidExisted accepts that name, with or without ".bin"."""
+        reference gives its file, <dir>/state_<step> (without ".bin",
+        SimulatorBase.cpp:1905-1920); no file is written.
+        loadStateWithRigidExisted accepts that name, with or without ".bin".
+        At most MAX_KEPT_STATES states are kept: saving another one frees the
+        least recently saved or loaded one (a per-step loop such as
+        experiments/others/python/finite-difference.py:55-110 reloads only the
+        state of its current step)."""
         core = self._require()
         step = core.get_step_count()
         d = stateFileDir or os.path.join(self._output, "state")
-        name = os.path.join(d, f"state_step{step}")
-        slot = self._states.setdefault(name, len(self._states))
+        name = os.path.normpath(os.path.join(d, f"state_{step}"))
+        if name in self._states:
+            slot = self._states.pop(name)
+        elif len(self._states) < self.MAX_KEPT_STATES:
+            slot = len(self._states)
+        else:
+            old = next(iter(self._states))
+            slot = self._states.pop(old)
+            core.state_free(slot)
         core.state_save(slot)
+        self._states[name] = slot
         return name
 
     def loadStateWithRigidExisted(self, stateFile: str):
         """Restores a state kept by saveState (SimulatorBase::loadStateWithRigidExisted,
-        SimulatorBase.cpp:2124-2290); state files on disk are not read."""
+        SimulatorBase.cpp:2124-2290): fluid positions, velocities and stiffness
+        fields, boundary samples, rigid x, q, v, omega, time and step count.
+        Unlike the reference, which leaves them as the last step made them, the
+        forward-mode rigid Jacobians are restored too, so a perturbed step
+        that is undone leaves no trace in the gradient chain.  State files on
+        disk are not read."""
         core = self._require()
         name = stateFile[:-4] if stateFile.endswith(".bin") else stateFile
+        name = os.path.normpath(name)
         if name not in self._states:
-            raise NotImplementedError("state files are not read by this library; only states kept by saveState")
-        core.state_load(self._states[name])
+            raise NotImplementedError(
+                "state files are not read by this library; only the last "
+                f"{self.MAX_KEPT_STATES} states kept by saveState can be loaded (got {stateFile!r})")
+        slot = self._states.pop(name)
+        self._states[name] = slot  # most recently used
+        core.state_load(slot)
 
     def getRigidBodyGradientManager(self):
         raise NotImplementedError("the multi-body RigidBodyGradientManager path is not part of this library")

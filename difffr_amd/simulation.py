@@ -186,6 +186,11 @@ class Simulation:
     def reset_gradient(self):
         self._check(native.load().dfr_reset_gradient(self.ctx))
 
+    def set_compute_gradient(self, on: bool):
+        """DiffFR's forward-mode gradient accumulation on/off for the next steps."""
+        self._check(native.load().dfr_set_compute_gradient(self.ctx, int(bool(on))))
+        self.params["compute_gradient"] = int(bool(on))
+
     # -- state --------------------------------------------------------------
     def fluid(self):
         x = np.empty((self.n, 3), np.float32)
@@ -326,6 +331,19 @@ class Simulation:
     def state_load(self, slot: int = 0):
         """Put back the state kept under `slot`."""
         self._check(native.load().dfr_state_load(self.ctx, int(slot)))
+
+    def state_free(self, slot: int = 0):
+        """Free the device memory of the state kept under `slot`."""
+        self._check(native.load().dfr_state_free(self.ctx, int(slot)))
+
+    def adjoint_arena_limit(self, nbytes: int = 0):
+        """Test hook: bytes the adjoint tape arena may hand out (0: no limit);
+        reaching it acts as running out of device memory."""
+        self._check(native.load().dfr_adjoint_arena_limit(self.ctx, int(nbytes)))
+
+    def adjoint_tape_bytes(self) -> int:
+        """Device bytes the kept tapes of the current rollout occupy."""
+        return int(self._check(native.load().dfr_adjoint_tape_bytes(self.ctx)))
 
     def adjoint_segment_memory(self, nbytes: int = -1):
         """Bound on one checkpoint segment's kept tapes in the backward pass

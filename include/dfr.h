@@ -104,7 +104,11 @@ int dfr_init(dfr_ctx* ctx);
 
 /* Runs nsteps full steps in SimulatorBase::timeStepNoGUI order: DFSPH step with
  * DiffFR's gradient accumulation (TimeStepDiffDFSPH::step, :526-652), rigid
- * velocity and position update and the boundary transform. */
+ * velocity and position update and the boundary transform.  A neighbour list
+ * that outgrows its capacity fails the step with DFR_ERR_CAPACITY.  A failed
+ * step ends an adjoint recording (dfr_adjoint_begin) in progress and leaves
+ * the state of a partial step; dfr_reset or dfr_state_load restores a whole
+ * one. */
 int dfr_step(dfr_ctx* ctx, int nsteps);
 
 /* Restores the fluid and rigid bodies to their state at dfr_init, resets the
@@ -129,6 +133,8 @@ int dfr_set_rigid(dfr_ctx* ctx, int body, const double in[13]);
  * dfr_state_load ends an adjoint recording. */
 int dfr_state_save(dfr_ctx* ctx, int slot);
 int dfr_state_load(dfr_ctx* ctx, int slot);
+/* Frees the device memory of a kept state (no effect on an empty slot). */
+int dfr_state_free(dfr_ctx* ctx, int slot);
 
 /* Body properties (any pointer may be NULL): the mass, (4/3) pi r^3 rho_body
  * per boundary sample for static bodies too (RigidBodyObject::getMass,
@@ -142,6 +148,11 @@ int dfr_get_body(const dfr_ctx* ctx, int body, double* mass, int* dynamic, int* 
  * 2 RigidGradOnly (Simulation::setGradientMode,
  * pySPlisHSPlasH/SimulationModule.cpp:208). */
 int dfr_set_gradient_mode(dfr_ctx* ctx, int mode);
+
+/* Switches DiffFR's forward-mode gradient accumulation (the BACKWARD path of
+ * TimeStepDiffDFSPH::step, :526-652, and backwardPerStep, :493-524) on or off
+ * for the following steps; dfr_params.compute_gradient sets it at creation. */
+int dfr_set_compute_gradient(dfr_ctx* ctx, int on);
 
 /* Forward-mode rigid Jacobians out[78]: Jxv, Jxw, Jvv, Jvw, Jwv, Jww (3x3
  * row-major), then Jqv, Jqw (4x3, rows w,x,y,z) -- get_grad_x_to_v0 etc.,
@@ -252,6 +263,16 @@ int dfr_adjoint_start_gradient(const dfr_ctx* ctx, double* out);
  * checkpointed rollout: -1 (default) whatever device memory is free, 0 never
  * (every step re-run twice), > 0 that bound. */
 int dfr_adjoint_segment_memory(dfr_ctx* ctx, int64_t bytes);
+/* Test hook (no counterpart in the reference): bytes the tape arena may hand
+ * out (0 = no limit beyond device memory).  Reaching it has the effect of
+ * running out of device memory: a kept-tape rollout (checkpoint_every <= 0)
+ * fails its step with DFR_ERR_HIP and ends the recording; a checkpointed
+ * backward pass falls back to re-running every step twice, freeing the
+ * arena's memory first, as when a segment's tapes outgrow free memory. */
+int dfr_adjoint_arena_limit(dfr_ctx* ctx, int64_t bytes);
+/* Device bytes the kept tapes of the current rollout occupy (checkpoint_every
+ * <= 0), for sizing rollouts against device memory. */
+int64_t dfr_adjoint_tape_bytes(const dfr_ctx* ctx);
 /* Iteration counts (divergence, pressure) of the steps recorded since
  * dfr_adjoint_begin; returns the number of steps. */
 int64_t dfr_adjoint_iterations(const dfr_ctx* ctx, int* iv, int* ip, int64_t cap);
@@ -271,6 +292,9 @@ int dfr_set_fixed_iterations(dfr_ctx* ctx, int n, const int* iv, const int* ip);
  * dfr_mlp_capture(capacity, fraction, seed) records, during the following
  * steps, (features, converged (kappa_v, kappa)) pairs of a hashed `fraction` of
  * the particles into a device buffer of `capacity` rows (capacity 0 stops).
+ * Called again with the same capacity while capturing, it keeps the rows
+ * captured so far and appends with the new fraction and seed (rollouts of
+ * several drops from one saved state into one data set).
  * dfr_mlp_train runs `iters` Adam steps (batch rows, cosine-decayed lr) on the
  * captured pairs with the library's MFMA forward/backward kernels; losses[it]
  * (optional) gets the mean squared normalised error of each batch.

@@ -6,6 +6,7 @@
 //                     2 = <float,double> (FP32 bulk, FP64 rigid).
 #include <cstring>
 #include <memory>
+#include <cstdlib>
 #include <string>
 #include "dfsph_adjoint.hpp"
 
@@ -156,6 +157,18 @@ template <typename RF, typename RR> struct Oracle : OracleBase {
     if (n == "kappa") { for (size_t i = 0; i < N; ++i) sim.kappa[i] = RF(in[i]); return 0; }
     if (n == "kappa_v") { for (size_t i = 0; i < N; ++i) sim.kappaV[i] = RF(in[i]); return 0; }
     if (n == "time") { sim.time = RF(in[0]); return 0; }
+    if (n.rfind("rigid", 0) == 0) {  // "rigid<b>": x(3), q (w,x,y,z)(4), v(3), omega(3) of body b
+      const int b = std::atoi(n.c_str() + 5);
+      if (b < 0 || b >= int(sim.bodies.size())) return -1;
+      auto& B = sim.bodies[b];
+      B.xrb = V3<RR>(RR(in[0]), RR(in[1]), RR(in[2]));
+      B.q = Quat<RR>(RR(in[3]), RR(in[4]), RR(in[5]), RR(in[6]));
+      B.vrb = V3<RR>(RR(in[7]), RR(in[8]), RR(in[9]));
+      B.wrb = V3<RR>(RR(in[10]), RR(in[11]), RR(in[12]));
+      sim.update_inertia(B);
+      sim.update_boundary_particles(true);
+      return 0;
+    }
     return -1;
   }
   void set_record(int on) override {

@@ -247,3 +247,25 @@ def test_saved_state_and_per_step_differences(gpu_ready):
     with pytest.raises(NotImplementedError):
         base.loadStateWithRigidExisted("/nonexistent/state.bin")
     base.cleanup()
+
+
+def test_saved_states_are_bounded(gpu_ready):
+    """saveState keeps at most MAX_KEPT_STATES states in device memory, named
+    state_<step> as the reference names its files; saving more frees the least
+    recently used one."""
+    sc = _scene()
+    base = sph.Exec.SimulatorBase()
+    base.init(sceneFile=sc, useGui=False, stopAt=0.0)
+    base.initSimulation()
+    cap = base.MAX_KEPT_STATES
+    names = []
+    for _ in range(cap + 4):
+        names.append(base.saveState("/tmp/fd"))
+        base.singleTimeStep()
+    assert names[0].endswith("state_0") and names[-1].endswith(f"state_{cap + 3}")
+    base.loadStateWithRigidExisted(names[-1] + ".bin")
+    assert sph.Simulation.getCurrent().getTimeStep().get_step_count() == cap + 3
+    base.loadStateWithRigidExisted(names[4])  # still kept: the oldest four were freed
+    with pytest.raises(NotImplementedError):
+        base.loadStateWithRigidExisted(names[0])
+    base.cleanup()

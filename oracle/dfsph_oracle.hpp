@@ -213,6 +213,7 @@ template <typename RF, typename RR> struct StepTape {
   std::vector<RF> kvPrev, d0, kvW;   // divergence warm start: kappa_v in, d, kappa_v used
   std::vector<RF> kPrev, rsW, kW;    // pressure warm start: kappa in, rho*, kappa used
   std::vector<std::vector<V3<RF>>> vD, vP;   // velocity at the start of each iteration
+  std::vector<V3<RF>> vDiv;          // velocity after the divergence solve
   std::vector<std::vector<RF>> kD, bD, kP, bP;  // k_i and d / rho* of each iteration
   std::vector<BodyTape<RF, RR>> bodies;
   int Iv = 0, Ip = 0;
@@ -432,24 +433,28 @@ template <typename RF, typename RR> class Sim {
   }
 
   // TimeStep::computeDensities, TimeStep.cpp:147-202 (scalar form).
+  void density_of(long i) {
+    RF d = V_f * K.w0;
+    for (uint32_t j : nFF[i]) d += V_f * K.W(x[i] - x[j]);
+    for (size_t bi = 0; bi < bodies.size(); ++bi)
+      for (uint32_t j : nFB[bi][i]) d += bodies[bi].V[j] * K.W(x[i] - bodies[bi].x[j]);
+    rho[i] = d * rho0;
+  }
   void compute_densities() {
     const long N = long(x.size());
 #pragma omp parallel for schedule(static)
-    for (long i = 0; i < N; ++i) {
-      RF d = V_f * K.w0;
-      for (uint32_t j : nFF[i]) d += V_f * K.W(x[i] - x[j]);
-      for (size_t bi = 0; bi < bodies.size(); ++bi)
-        for (uint32_t j : nFB[bi][i]) d += bodies[bi].V[j] * K.W(x[i] - bodies[bi].x[j]);
-      rho[i] = d * rho0;
-    }
+    for (long i = 0; i < N; ++i) density_of(i);
   }
 
   // TimeStepDiffDFSPH::computeDFSPHFactor, TimeStepDiffDFSPH.cpp:883-962.
   void compute_dfsph_factor() {
     const long N = long(x.size());
-    const RF eps = RF(1e-5);
 #pragma omp parallel for schedule(static)
-    for (long i = 0; i < N; ++i) {
+    for (long i = 0; i < N; ++i) factor_of(i);
+  }
+  void factor_of(long i) {
+    const RF eps = RF(1e-5);
+    {
       RF sum_grad_p_k = 0;
       V3<RF> grad_p_i;
       for (uint32_t j : nFF[i]) {
@@ -771,25 +776,28 @@ template <typename RF, typename RR> class Sim {
 
   void compute_rigid_body_gradient(int mode) {  // :1226-1279
     if (!P.computeGradient) return;
-    const RF invH = RF(1.0) / dt;
-    const RF invH2 = RF(1.0) / dt / dt;
     for (size_t bi = 0; bi < bodies.size(); ++bi) {
       auto& b = bodies[bi];
       if (!(b.dynamic && !b.animated)) continue;
 #pragma omp parallel for schedule(static)
-      for (long j = 0; j < long(b.x.size()); ++j) {
-        for (uint32_t i : nBF[bi][j]) {
-          const RF b_i = (mode == 0) ? bAdv[i] - RF(1.0) : bAdv[i];
-          // quirk: the coefficient is truncated to an unsigned int (:1266)
-          const unsigned int coeff = (mode == 0) ? (unsigned int)invH2 : (unsigned int)invH;
-          const RF ki = b_i * alpha[i] * RF(coeff);
-          const V3<RF> grad_p_j = K.gradW(x[i] - b.x[j]) * (-b.V[j]);
-          const V3<RF> velChange = grad_p_j * (-dt * RF(1.0) * ki);
-          const V3<RF> force = velChange * (-m_f) * invH;
-          compute_gradient(b, i, uint32_t(j), b_i, ki, force, mode);
-        }
-      }
+      for (long j = 0; j < long(b.x.size()); ++j)
+        for (uint32_t i : nBF[bi][j]) gradient_pair(b, long(i), uint32_t(j), mode);
     }
+  }
+
+  // One (boundary sample j, fluid particle i) term of computeRigidBodyGradient
+  // (:1255-1275): k_i from the current b field, then computeGradient.
+  void gradient_pair(Body<RF, RR>& b, long i, uint32_t j, int mode) {
+    const RF invH = RF(1.0) / dt;
+    const RF invH2 = RF(1.0) / dt / dt;
+    const RF b_i = (mode == 0) ? bAdv[i] - RF(1.0) : bAdv[i];
+    // quirk: the coefficient is truncated to an unsigned int (:1266)
+    const unsigned int coeff = (mode == 0) ? (unsigned int)invH2 : (unsigned int)invH;
+    const RF ki = b_i * alpha[i] * RF(coeff);
+    const V3<RF> grad_p_j = K.gradW(x[i] - b.x[j]) * (-b.V[j]);
+    const V3<RF> velChange = grad_p_j * (-dt * RF(1.0) * ki);
+    const V3<RF> force = velChange * (-m_f) * invH;
+    compute_gradient(b, i, j, b_i, ki, force, mode);
   }
 
   // BoundaryModel_Akinci2012::accumulate_and_reset_gradient, :453-497.
@@ -915,17 +923,20 @@ template <typename RF, typename RR> class Sim {
   void rigid_velocity_step() {
     const RR dtR = RR(dt);
     const V3<RR> gR(RR(P.gravity[0]), RR(P.gravity[1]), RR(P.gravity[2]));
-    for (auto& b : bodies) {
+    for (size_t k = 0; k < bodies.size(); ++k) {
+      auto& b = bodies[k];
       if (!b.dynamic) continue;
       V3<RR> F, T;
       force_and_torque(b, F, T);
+      // the force-model hook's frozen-inverse map (see ForceModel); I^-1 otherwise
+      const M3<RR> Mi = (fm.useFrozen && int(k) == fm.body) ? fm.frozen.at(stepCount - 1) * RR(2) - b.Iinv : b.Iinv;
       if (!b.animated) {
         b.vrb += F * (b.invMass * dtR);
         if (P.gyroscopic) {
           const V3<RR> L = b.I * b.wrb;
-          b.wrb += (b.Iinv * (L.cross(b.wrb) + T)) * dtR;
+          b.wrb += (Mi * (L.cross(b.wrb) + T)) * dtR;
         } else {
-          b.wrb += (b.Iinv * T) * dtR;
+          b.wrb += (Mi * T) * dtR;
         }
       }
       b.F_last = F; b.T_last = T;
@@ -982,6 +993,7 @@ template <typename RF, typename RR> class Sim {
     }
     if (P.enableDivergenceSolver) divergence_solve();
     else stats.iterationsV = 0;
+    if (tp) tp->vDiv = v;
     const long N = long(x.size());
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < N; ++i) v[i] += g * dt;  // clearAccelerations + advect (:588-605)
@@ -998,6 +1010,76 @@ template <typename RF, typename RR> class Sim {
     rigid_position_step();
     tp = nullptr;
     ++fixPos;
+  }
+
+  // --------------------------------------------------------------------------
+  // Test hook that pins the chain rule (perform_chain_rule) by finite
+  // differences; not part of the reference's step.  A prescribed affine
+  // force/torque law on body `body` replaces the fluid coupling (no fluid
+  // step runs), from step `start` + 1 on (after the two-step release window):
+  //   F = F0 + A (x, v, omega, q),
+  //   T = T0 + B (x', v', omega, q),
+  // columns x(3) v(3) omega(3) q(w,x,y,z)(4), with x' = x + dt v' and
+  // v' = v + dt F / m the position and velocity after this step's force update:
+  // the reference's torque chain reads J_x and J_v after their update
+  // (BoundaryModel_Akinci2012.cpp:581-613), so its derivative is that of a
+  // torque evaluated there.  The per-body sums grad_net_* are the law's
+  // matrices (A -> F_x, F_v, F_w, F_q; B -> T_x, T_v, T_w, T_q).
+  // useFrozen: the angular update uses (2 Ibar^-1 - I(q)^-1) with Ibar^-1 the
+  // world inverse inertia of a base run at the same step (recordFrozen).  At
+  // the base run's state it equals I^-1, and its derivative in q is
+  // +I^-1 (dI) I^-1, the sign the reference's chain rule uses
+  // (BoundaryModel_Akinci2012.cpp:641-647), so central differences of this map
+  // are what the forward-mode chain accumulates.
+  // --------------------------------------------------------------------------
+  struct ForceModel {
+    int body = -1, start = 2;
+    double F0[3] = {0, 0, 0}, T0[3] = {0, 0, 0};
+    double A[39] = {}, B[39] = {};  // 3 x 13, row-major
+    bool recordFrozen = false, useFrozen = false;
+    std::vector<M3<RR>> frozen;
+  } fm;
+
+  static V3<RR> affine(const double* c, const double* M, const V3<RR>& x, const V3<RR>& v, const V3<RR>& w,
+                       const Quat<RR>& q) {
+    const RR s[13] = {x.x, x.y, x.z, v.x, v.y, v.z, w.x, w.y, w.z, q.w, q.x, q.y, q.z};
+    V3<RR> out{RR(c[0]), RR(c[1]), RR(c[2])};
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 13; ++k) out[r] += RR(M[13 * r + k]) * s[k];
+    return out;
+  }
+
+  void model_step() {
+    begin_step();
+    time = time + dt;
+    if (fm.recordFrozen) {
+      if (int(fm.frozen.size()) < stepCount) fm.frozen.resize(stepCount);
+      fm.frozen[stepCount - 1] = bodies[fm.body].Iinv;
+    }
+    for (size_t k = 0; k < bodies.size(); ++k) {
+      auto& b = bodies[k];
+      for (int t = 0; t < nthreads; ++t) { b.Fth[t] = V3<RR>(); b.Tth[t] = V3<RR>(); }
+      if (int(k) != fm.body || stepCount <= fm.start) {
+        b.nFv.setZero(); b.nFx.setZero(); b.nFw.setZero(); b.nFq.setZero();
+        b.nTw.setZero(); b.nTq.setZero(); b.nTv.setZero(); b.nTx.setZero();
+      } else {
+        const V3<RR> F = affine(fm.F0, fm.A, b.xrb, b.vrb, b.wrb, b.q);
+        const V3<RR> v1 = b.vrb + F * (b.invMass * RR(dt));
+        const V3<RR> x1 = b.xrb + v1 * RR(dt);
+        const V3<RR> T = affine(fm.T0, fm.B, x1, v1, b.wrb, b.q);
+        b.Fth[0] = F; b.Tth[0] = T;
+        for (int r = 0; r < 3; ++r) {
+          for (int c = 0; c < 3; ++c) {
+            b.nFx(r, c) = RR(fm.A[13 * r + c]); b.nFv(r, c) = RR(fm.A[13 * r + 3 + c]); b.nFw(r, c) = RR(fm.A[13 * r + 6 + c]);
+            b.nTx(r, c) = RR(fm.B[13 * r + c]); b.nTv(r, c) = RR(fm.B[13 * r + 3 + c]); b.nTw(r, c) = RR(fm.B[13 * r + 6 + c]);
+          }
+          for (int c = 0; c < 4; ++c) { b.nFq(r, c) = RR(fm.A[13 * r + 9 + c]); b.nTq(r, c) = RR(fm.B[13 * r + 9 + c]); }
+        }
+      }
+      if (P.computeGradient && b.dynamic && !b.animated) perform_chain_rule(b, RR(dt));
+    }
+    rigid_velocity_step();
+    rigid_position_step();
   }
 
   // Reset of the rigid bodies and gradients between trajectories

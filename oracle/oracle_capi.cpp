@@ -40,6 +40,11 @@ struct OracleBase {
   virtual void adjoint(const double* seed, double* out) = 0;
   virtual int tape_field(int step, const char* name, int iter, double* out) const = 0;
   virtual void set_warm_net(const float* params, const float* norm, int bf16) = 0;
+  virtual int pair_eval(int body, long j, long i, int mode, const double* in, double* out) = 0;
+  virtual int pair_jacobians(int body, long j, long i, int mode, double* out) = 0;
+  virtual int velocity_update_probe(int which, double* dv, double* ft) = 0;
+  virtual int set_force_model(int body, int start, const double* f0t0, const double* A, const double* B, int flags) = 0;
+  virtual int model_step(int n) = 0;
 };
 
 template <typename RF, typename RR> struct Oracle : OracleBase {
@@ -199,6 +204,13 @@ template <typename RF, typename RR> struct Oracle : OracleBase {
     else if (n == "bP" && iter < int(T.bP.size())) f = &T.bP[iter];
     else if (n == "kP" && iter < int(T.kP.size())) f = &T.kP[iter];
     if (f) { for (size_t i = 0; i < f->size(); ++i) out[i] = double((*f)[i]); return int(f->size()); }
+    if (n == "kPrev") f = &T.kPrev; else if (n == "kvPrev") f = &T.kvPrev;
+    if (f) { for (size_t i = 0; i < f->size(); ++i) out[i] = double((*f)[i]); return int(f->size()); }
+    if (n == "vDiv" || n == "x") {
+      const auto& w = (n == "x") ? T.x : T.vDiv;
+      for (size_t i = 0; i < w.size(); ++i) for (int k = 0; k < 3; ++k) out[3 * i + k] = double(w[i][k]);
+      return int(3 * w.size());
+    }
     if (n == "feat") {
       for (size_t i = 0; i < T.feat.size(); ++i) out[i] = T.feat[i];
       return int(T.feat.size());
@@ -222,6 +234,119 @@ template <typename RF, typename RR> struct Oracle : OracleBase {
     Adjoint<RF, RR> adj(sim);
     adj.run(seed, out);
   }
+  // ---- probes for the pinning tests --------------------------------------
+  // The (boundary sample j of `body`, fluid particle i) force term of one
+  // Jacobi iteration as a function of x_j, v_j, v_i (in: x_j, v_j, v_i and the
+  // body origin x_rb, 12 values), with the neighbour lists, the other particles
+  // and samples held: recomputes rho_i, alpha_i, s_i and the solver's b_i
+  // (mode 0: rho* - 1, mode 1: d), then k_i = b_i alpha_i c with the real
+  // c = 1/dt^2 or 1/dt, g = -V_j gradW(x_i - x_j), the velocity-change rate
+  // -k_i g, the force on the body f = m k_i g and its torque (x_j - x_rb) x f.
+  // out: b, alpha, k, dvc(3), f(3), tau(3), V_j W(x_i - x_j), rho_i = 14.
+  // Restores every value it changed.
+  int pair_eval(int body, long j, long i, int mode, const double* in, double* out) override {
+    auto& S = sim;
+    if (body < 0 || body >= int(S.bodies.size())) return -1;
+    auto& b = S.bodies[body];
+    if (j < 0 || j >= long(b.x.size()) || i < 0 || i >= long(S.x.size())) return -1;
+    const V3<RF> xj0 = b.x[j], vj0 = b.v[j], vi0 = S.v[i], si0 = S.s[i];
+    const RF rhoi0 = S.rho[i], ai0 = S.alpha[i], bi0 = S.bAdv[i];
+    b.x[j] = V3<RF>(RF(in[0]), RF(in[1]), RF(in[2]));
+    b.v[j] = V3<RF>(RF(in[3]), RF(in[4]), RF(in[5]));
+    S.v[i] = V3<RF>(RF(in[6]), RF(in[7]), RF(in[8]));
+    const V3<RR> xrb{RR(in[9]), RR(in[10]), RR(in[11])};
+    S.density_of(i);
+    S.factor_of(i);
+    if (mode == 0) S.compute_density_adv(i); else S.compute_density_change(i);
+    const RF bi = (mode == 0) ? S.bAdv[i] - RF(1) : S.bAdv[i];
+    const RF c = (mode == 0) ? RF(1) / S.dt / S.dt : RF(1) / S.dt;
+    const RF k = bi * S.alpha[i] * c;
+    const V3<RF> g = S.K.gradW(S.x[i] - b.x[j]) * (-b.V[j]);
+    const V3<RF> dvc = g * (-k);
+    const V3<RF> f = dvc * (-S.m_f);
+    const V3<RR> tau = (V3<RR>(b.x[j]) - xrb).cross(V3<RR>(f));
+    out[0] = double(bi); out[1] = double(S.alpha[i]); out[2] = double(k);
+    for (int q = 0; q < 3; ++q) { out[3 + q] = double(dvc[q]); out[6 + q] = double(f[q]); out[9 + q] = double(tau[q]); }
+    out[12] = double(b.V[j] * S.K.W(S.x[i] - b.x[j]));
+    out[13] = double(S.rho[i]);
+    b.x[j] = xj0; b.v[j] = vj0; S.v[i] = vi0; S.s[i] = si0;
+    S.rho[i] = rhoi0; S.alpha[i] = ai0; S.bAdv[i] = bi0;
+    return 0;
+  }
+  // The reference-mode pair Jacobians of that term at the current state
+  // (gradient_pair): F_v F_x F_w T_w T_x T_v (3x3) then F_q T_q (3x4) = 78.
+  int pair_jacobians(int body, long j, long i, int mode, double* out) override {
+    auto& S = sim;
+    if (body < 0 || body >= int(S.bodies.size())) return -1;
+    auto& b = S.bodies[body];
+    if (j < 0 || j >= long(b.x.size()) || i < 0 || i >= long(S.x.size())) return -1;
+    S.density_of(i);
+    S.factor_of(i);
+    if (mode == 0) S.compute_density_adv(i); else S.compute_density_change(i);
+    auto zero = [&] {
+      b.gFv[j].setZero(); b.gFx[j].setZero(); b.gFw[j].setZero(); b.gFq[j].setZero();
+      b.gTw[j].setZero(); b.gTv[j].setZero(); b.gTx[j].setZero(); b.gTq[j].setZero();
+    };
+    zero();
+    S.gradient_pair(b, i, uint32_t(j), mode);
+    const M3<RR>* m3[6] = {&b.gFv[j], &b.gFx[j], &b.gFw[j], &b.gTw[j], &b.gTx[j], &b.gTv[j]};
+    int o = 0;
+    for (auto m : m3) for (int q = 0; q < 9; ++q) out[o++] = double(m->a[q]);
+    for (int q = 0; q < 12; ++q) out[o++] = double(b.gFq[j].a[q]);
+    for (int q = 0; q < 12; ++q) out[o++] = double(b.gTq[j].a[q]);
+    zero();
+    return 0;
+  }
+  // One Jacobi velocity update of every fluid particle from the current
+  // positions and velocities (neighbour search, densities and factors first):
+  // which 0 = pressure form k_i = (rho*_i - 1) alpha_i / dt^2, 1 = divergence
+  // form k_i = d_i alpha_i / dt.  dv: velocity change [N][3]; ft: per body the
+  // force and torque the update accumulated (6 per body).  Velocities and
+  // body accumulators are restored.
+  int velocity_update_probe(int which, double* dv, double* ft) override {
+    auto& S = sim;
+    S.neighborhood_search();
+    S.compute_densities();
+    S.compute_dfsph_factor();
+    const long N = long(S.x.size());
+    std::vector<RF> kf(N);
+    for (long i = 0; i < N; ++i) {
+      if (which == 0) { S.compute_density_adv(i); kf[i] = (S.bAdv[i] - RF(1)) * S.alpha[i] / (S.dt * S.dt); }
+      else { S.compute_density_change(i); kf[i] = S.bAdv[i] * S.alpha[i] / S.dt; }
+    }
+    const auto v0 = S.v;
+    for (auto& b : S.bodies) for (int t = 0; t < S.nthreads; ++t) { b.Fth[t] = V3<RR>(); b.Tth[t] = V3<RR>(); }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) S.velocity_update(i, kf[i], kf);
+    for (long i = 0; i < N; ++i) for (int k = 0; k < 3; ++k) dv[3 * i + k] = double(S.v[i][k]) - double(v0[i][k]);
+    for (size_t bi = 0; bi < S.bodies.size(); ++bi) {
+      V3<RR> F, T;
+      S.force_and_torque(S.bodies[bi], F, T);
+      for (int k = 0; k < 3; ++k) { ft[6 * bi + k] = double(F[k]); ft[6 * bi + 3 + k] = double(T[k]); }
+      for (int t = 0; t < S.nthreads; ++t) { S.bodies[bi].Fth[t] = V3<RR>(); S.bodies[bi].Tth[t] = V3<RR>(); }
+    }
+    S.v = v0;
+    return 0;
+  }
+  // flags: 1 = record the base run's inverse inertia per step (clears the
+  // record), 2 = run the frozen-inverse map (Sim::ForceModel).
+  int set_force_model(int body, int start, const double* f0t0, const double* A, const double* B, int flags) override {
+    auto& m = sim.fm;
+    if (body < 0 || body >= int(sim.bodies.size())) return -1;
+    m.body = body; m.start = start;
+    for (int k = 0; k < 3; ++k) { m.F0[k] = f0t0[k]; m.T0[k] = f0t0[3 + k]; }
+    for (int k = 0; k < 39; ++k) { m.A[k] = A[k]; m.B[k] = B[k]; }
+    m.recordFrozen = (flags & 1) != 0;
+    m.useFrozen = (flags & 2) != 0;
+    if (m.recordFrozen) m.frozen.clear();
+    return 0;
+  }
+  int model_step(int n) override {
+    if (sim.fm.body < 0) return -1;
+    for (int s = 0; s < n; ++s) sim.model_step();
+    return 0;
+  }
+
   // Partial steps for per-kernel parity: 0 = neighbour search + density + factor.
   void stage(int which) override {
     if (which == 0) {
@@ -344,5 +469,18 @@ int orc_net_input_grad(const float* params, const float* norm, int bf16, long n,
   return 0;
 }
 
+int orc_pair_eval(void* h, int body, long j, long i, int mode, const double* in, double* out) {
+  return static_cast<OracleBase*>(h)->pair_eval(body, j, i, mode, in, out);
+}
+int orc_pair_jacobians(void* h, int body, long j, long i, int mode, double* out) {
+  return static_cast<OracleBase*>(h)->pair_jacobians(body, j, i, mode, out);
+}
+int orc_velocity_update_probe(void* h, int which, double* dv, double* ft) {
+  return static_cast<OracleBase*>(h)->velocity_update_probe(which, dv, ft);
+}
+int orc_set_force_model(void* h, int body, int start, const double* f0t0, const double* A, const double* B, int flags) {
+  return static_cast<OracleBase*>(h)->set_force_model(body, start, f0t0, A, B, flags);
+}
+int orc_model_step(void* h, int n) { return static_cast<OracleBase*>(h)->model_step(n); }
 const char* orc_last_error() { return g_err.c_str(); }
 }

@@ -82,6 +82,11 @@ def lib():
         L.orc_set_warm_net.argtypes = [P, fp, fp, C.c_int]
         L.orc_net_predict.argtypes = [fp, fp, C.c_int, C.c_long, dp, dp]
         L.orc_net_input_grad.argtypes = [fp, fp, C.c_int, C.c_long, dp, dp, dp]
+        L.orc_pair_eval.argtypes = [P, C.c_int, C.c_long, C.c_long, C.c_int, dp, dp]
+        L.orc_pair_jacobians.argtypes = [P, C.c_int, C.c_long, C.c_long, C.c_int, dp]
+        L.orc_velocity_update_probe.argtypes = [P, C.c_int, dp, dp]
+        L.orc_set_force_model.argtypes = [P, C.c_int, C.c_int, dp, dp, dp, C.c_int]
+        L.orc_model_step.argtypes = [P, C.c_int]
         assert L.orc_params_size() == C.sizeof(Params), "oracle Params layout mismatch"
         _lib = L
     return _lib
@@ -168,7 +173,7 @@ class OracleSim:
         return a[:n].copy(), b[:n].copy()
 
     def tape_field(self, step: int, name: str, it: int = 0) -> np.ndarray:
-        out = np.zeros(6 * self.n, np.float64)  # "feat" is [n][6]
+        out = np.zeros(6 * self.n, np.float64)  # "feat" is [n][6]; "x", "vDiv" [n][3]
         n = lib().orc_tape_field(self.h, step, name.encode(), it, _dp(out))
         if n < 0:
             raise KeyError(name)
@@ -199,6 +204,54 @@ class OracleSim:
 
     def stage(self, which: int = 0):
         lib().orc_stage(self.h, which)
+
+    # ---- probes of single terms (pinning tests) ----------------------------
+    def pair_eval(self, body: int, j: int, i: int, mode: int, xj, vj, vi, xrb) -> dict:
+        """Force term of boundary sample j on fluid particle i in one Jacobi
+        iteration as a function of (x_j, v_j, v_i), lists and everything else held
+        (after stage(0)).  mode 0: pressure (rho* - 1), 1: divergence (d)."""
+        a = np.ascontiguousarray(np.concatenate([xj, vj, vi, xrb]), np.float64)
+        o = np.empty(14)
+        if lib().orc_pair_eval(self.h, body, j, i, mode, _dp(a), _dp(o)) != 0:
+            raise IndexError((body, j, i))
+        return dict(b=o[0], alpha=o[1], k=o[2], dvc=o[3:6].copy(), f=o[6:9].copy(), tau=o[9:12].copy(),
+                    wij=o[12], rho=o[13])
+
+    def pair_jacobians(self, body: int, j: int, i: int, mode: int) -> dict:
+        """The reference-mode Jacobians of that term: F_v F_x F_w T_w T_x T_v (3x3),
+        F_q T_q (3x4, columns w, x, y, z)."""
+        o = np.empty(78)
+        if lib().orc_pair_jacobians(self.h, body, j, i, mode, _dp(o)) != 0:
+            raise IndexError((body, j, i))
+        names = ["Fv", "Fx", "Fw", "Tw", "Tx", "Tv"]
+        out = {n: o[9 * k:9 * k + 9].reshape(3, 3).copy() for k, n in enumerate(names)}
+        out["Fq"] = o[54:66].reshape(3, 4).copy()
+        out["Tq"] = o[66:78].reshape(3, 4).copy()
+        return out
+
+    def velocity_update_probe(self, which: int):
+        """One Jacobi velocity update of every particle from the current state
+        (0: pressure form, 1: divergence form): (dv [n][3], force/torque [bodies][6])."""
+        dv = np.empty((self.n, 3))
+        ft = np.empty((self.n_bodies, 6))
+        lib().orc_velocity_update_probe(self.h, which, _dp(dv), _dp(ft))
+        return dv, ft
+
+    def set_force_model(self, body: int, F0, T0, A, B, start: int = 2, record_frozen=False, use_frozen=False):
+        """Affine force/torque law on `body` for model_step() (no fluid step)."""
+        f = np.ascontiguousarray(np.concatenate([F0, T0]), np.float64)
+        a = np.ascontiguousarray(A, np.float64).reshape(39)
+        b = np.ascontiguousarray(B, np.float64).reshape(39)
+        flags = (1 if record_frozen else 0) | (2 if use_frozen else 0)
+        if lib().orc_set_force_model(self.h, body, start, _dp(f), _dp(a), _dp(b), flags) != 0:
+            raise IndexError(body)
+
+    def model_step(self, n: int = 1):
+        if lib().orc_model_step(self.h, n) != 0:
+            raise RuntimeError("no force model set")
+
+    def reset_bodies(self):
+        lib().orc_reset_bodies(self.h)
 
     def fluid(self):
         x = np.empty((self.n, 3)); v = np.empty((self.n, 3))

@@ -399,8 +399,12 @@ void fill_rigid_from_body(const dfr_ctx* c, const BodyInfo& bi, RigidDev& r) {
   const double det = cof(0, 0) * I[0] + cof(1, 0) * I[3] + cof(2, 0) * I[6];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) r.Iinv[3 * i + j] = det != 0.0 ? cof(j, i) / det : 0.0;
-  // Jvv = Jww = I, all else 0 (BoundaryModel_Akinci2012.cpp:94-106)
-  for (int k = 0; k < 3; ++k) { r.J[18 + 4 * k] = 1.0; r.J[45 + 4 * k] = 1.0; }
+  // Jvv = Jww = I, all else 0 (BoundaryModel_Akinci2012.cpp:94-106,
+  // RigidBodyGradientManager::reset :474-523)
+  for (int k = 0; k < 3; ++k) {
+    r.J[18 + 4 * k] = 1.0; r.J[45 + 4 * k] = 1.0;
+    r.Jm[18 + 4 * k] = 1.0; r.Jm[45 + 4 * k] = 1.0;
+  }
 }
 
 void upload_fluid_initial(dfr_ctx* c) {
@@ -441,7 +445,8 @@ void place_boundary_and_volumes(dfr_ctx* c) {
 }
 
 void begin_step(dfr_ctx* c) {
-  // TimeStepDiffDFSPH::beginStep (:379-429), passive-body branch
+  // TimeStepDiffDFSPH::beginStep (:379-429).  A driven body gets
+  // (v, omega) = factor (v0, omega0) on the device (ramp flag).
   const int nb = (int)c->bodies.size();
   std::vector<int> ramp(nb, 0), anim(nb, 0);
   std::vector<double> fac(nb, 0.0);
@@ -449,6 +454,24 @@ void begin_step(dfr_ctx* c) {
   for (int k = 0; k < nb; ++k) {
     BodyInfo& bi = c->bodies[k];
     if (!bi.dynamic) continue;
+    if (c->P.release_mode) {
+      // release branch (:381-403): only boundary model 1 is driven; it rests
+      // (animated when uat > 1e-3) through uat + dt, then takes (v0, omega0)
+      // at the start of every later step
+      if (k == 1) {
+        ramp[k] = 1;
+        if (c->time <= uat + c->dtf) {
+          if (uat > 1e-3f) bi.animated = true;
+          fac[k] = 0.0;
+        } else {
+          fac[k] = 1.0;
+          bi.animated = false;
+        }
+      }
+      anim[k] = bi.animated ? 1 : 0;
+      continue;
+    }
+    // passive-body branch (:405-426)
     if (c->time <= uat + c->dtf) {
       float factor = 1.f;
       if (uat > 1e-3f) {
@@ -830,6 +853,7 @@ void do_step(dfr_ctx* c) {
     rc.gyroscopic = c->P.gyroscopic;
     rc.compute_gradient = c->replay ? 0 : c->P.compute_gradient;
     rc.nbodies = (int)c->bodies.size();
+    rc.use_manager = c->P.use_gradient_manager;
     launch_rigid_step(rc, c->d_rig, c->stream);
     launch_boundary_transform(c->b.dyn_idx, c->b.nd, c->b, c->d_rig, c->stream);
   }
@@ -1355,6 +1379,8 @@ void dfr_default_params(dfr_params* p, double radius, double dt) {
   p->target_time = 1e30;
   p->max_neighbors = 64;
   p->max_boundary_neighbors = 128;
+  p->release_mode = 0;
+  p->use_gradient_manager = 0;
 }
 
 dfr_ctx* dfr_create(const dfr_params* p, int device) {
@@ -1707,6 +1733,20 @@ int dfr_get_jacobians(const dfr_ctx* c, int body, double out[78]) {
                                                                                                        : DFR_ERR_HIP;
 }
 
+int dfr_get_gradient_manager(const dfr_ctx* c, int R, int RR, double jac[78], double net[78]) {
+  if (check_body(c, R) || check_body(c, RR)) return DFR_ERR_ARG;
+  if (jac) std::fill(jac, jac + 78, 0.0);
+  if (net) std::fill(net, net + 78, 0.0);
+  // entries between different bodies stay zero without the rigid contact solver;
+  // a static body has no entries
+  if (R != RR || !c->bodies[R].dynamic) return DFR_OK;
+  if (jac && hipMemcpy(jac, &c->d_rig[R].Jm[0], 78 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    return DFR_ERR_HIP;
+  if (net && hipMemcpy(net, &c->d_rig[R].mnet[0], 78 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    return DFR_ERR_HIP;
+  return DFR_OK;
+}
+
 int dfr_get_net_gradients(const dfr_ctx* c, int body, double out[78]) {
   if (check_body(c, body) || !out) return DFR_ERR_ARG;
   return hipMemcpy(out, &c->d_rig[body].netlast[0], 78 * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess
@@ -1718,9 +1758,11 @@ int dfr_reset_gradient(dfr_ctx* c) {
   return guarded(c, [&]() {
     if (!c->initialized) return fail(c, DFR_ERR_STATE, "not initialised");
     for (int k = 0; k < (int)c->bodies.size(); ++k) {
-      double J[78] = {0};
+      double J[78] = {0}, Z[78] = {0};
       for (int d = 0; d < 3; ++d) { J[18 + 4 * d] = 1.0; J[45 + 4 * d] = 1.0; }
       HIPCHK(hipMemcpy(&c->d_rig[k].J[0], J, sizeof(J), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(&c->d_rig[k].Jm[0], J, sizeof(J), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(&c->d_rig[k].mnet[0], Z, sizeof(Z), hipMemcpyHostToDevice));
     }
     return DFR_OK;
   });

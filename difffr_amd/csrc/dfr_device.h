@@ -162,6 +162,10 @@ struct RigidDev {
   // per-step pair-Jacobian sums: Fv Fx Fw Tw Tx Tv (3x3), Fq Tq (3x4)
   double net[78];
   double netlast[78];                // net of the last completed step
+  // RigidBodyGradientManager path (R == RR entries): its Jacobians, same layout
+  // as J, and the per-step sums it was last handed (RigidBodyGradientManager.h)
+  double Jm[78];
+  double mnet[78];
   int dynamic, animated, grad_active, pad;
 };
 

@@ -178,6 +178,7 @@ struct RigidStepConst {
   int gyroscopic;
   int compute_gradient;
   int nbodies;
+  int use_manager;       // useRigidGradientManager: the manager's chain instead of perform_chain_rule
 };
 void launch_rigid_step(const RigidStepConst& rc, RigidDev* rig, hipStream_t s);
 void launch_boundary_transform(const uint32_t* idx, int n, BoundaryBufs& b, const RigidDev* rig, hipStream_t s);

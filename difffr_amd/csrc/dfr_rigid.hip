@@ -237,12 +237,86 @@ __device__ void chain_rule(RigidDev& b, const RigidStepConst& rc) {
   st3(b.J + 45, Jww); st43(b.J + 54, Jqv); st43(b.J + 66, Jqw);
 }
 
+// RigidBodyGradientManager (single-body entries R == RR; no rigid contact).
+// perform_force_and_torque_chain_rule (:96-156): d(F, T)/d(v0, omega0) from the
+// v and omega columns of the handed sums and the Jacobians before this step's
+// update; perform_velocity_chain_rule (:276-365): the velocity chain with the
+// gyroscopic term and the reference's "+" on the d(I^-1) term (:351-357).
+__device__ void manager_velocity_chain(RigidDev& b, const RigidStepConst& rc) {
+  const double dt = rc.dt;
+  const M3d I = ld3(b.I), Iinv = ld3(b.Iinv), I0 = ld3(b.I0);
+  const M3d R = quat_to_R(b.q);
+  const V3d om = {b.w[0], b.w[1], b.w[2]};
+  M3d Jvv = ld3(b.Jm + 18), Jvw = ld3(b.Jm + 27), Jwv = ld3(b.Jm + 36), Jww = ld3(b.Jm + 45);
+  const M43d Jqv = ld43(b.Jm + 54), Jqw = ld43(b.Jm + 66);
+  const M3d nFv = ld3(b.mnet + 0), nFw = ld3(b.mnet + 18), nTw = ld3(b.mnet + 27), nTv = ld3(b.mnet + 45);
+  const M3d dFv0 = mm(nFv, Jvv) + mm(nFw, Jwv), dFw0 = mm(nFv, Jvw) + mm(nFw, Jww);
+  const M3d dTv0 = mm(nTv, Jvv) + mm(nTw, Jwv), dTw0 = mm(nTv, Jvw) + mm(nTw, Jww);
+  const V3d T = {b.T[0], b.T[1], b.T[2]};
+  Jvv = Jvv + dFv0 * (dt * b.invMass);
+  Jvw = Jvw + dFw0 * (dt * b.invMass);
+  V3d temp = T;
+  M3d dTauw0 = dTw0, dTauv0 = dTv0;
+  if (rc.gyroscopic) {
+    const V3d Iom = mv(I, om);
+    const V3d lw = cross(Iom, om);
+    temp = {lw.x + T.x, lw.y + T.y, lw.z + T.z};
+    dTauw0 = mm(skew(Iom), Jww) + mm(skew(om).tr(), mm(I, Jww) + grad_inertia_v(b, R, I0, om, Jqw)) + dTw0;
+    dTauv0 = mm(skew(Iom), Jwv) + mm(skew(om).tr(), mm(I, Jwv) + grad_inertia_v(b, R, I0, om, Jqv)) + dTv0;
+  }
+  const V3d it = mv(Iinv, temp);
+  Jww = Jww + (mm(Iinv, grad_inertia_v(b, R, I0, it, Jqw)) + mm(Iinv, dTauw0)) * dt;
+  Jwv = Jwv + (mm(Iinv, grad_inertia_v(b, R, I0, it, Jqv)) + mm(Iinv, dTauv0)) * dt;
+  st3(b.Jm + 18, Jvv); st3(b.Jm + 27, Jvw); st3(b.Jm + 36, Jwv); st3(b.Jm + 45, Jww);
+}
+
+// perform_position_and_rotation_chain_rule (:367-423): after the velocity
+// update (the new omega), before the position update (the current q).
+__device__ void manager_position_chain(RigidDev& b, const RigidStepConst& rc) {
+  const double dt = rc.dt;
+  M3d Jxv = ld3(b.Jm + 0), Jxw = ld3(b.Jm + 9);
+  const M3d Jvv = ld3(b.Jm + 18), Jvw = ld3(b.Jm + 27), Jwv = ld3(b.Jm + 36), Jww = ld3(b.Jm + 45);
+  M43d Jqv = ld43(b.Jm + 54), Jqw = ld43(b.Jm + 66);
+  Jxv = Jxv + Jvv * dt;
+  Jxw = Jxw + Jvw * dt;
+  const double p[4] = {0.0, b.w[0], b.w[1], b.w[2]};
+  double pq[4];
+  quat_mul(p, b.q, pq);
+  double nq[4];
+  for (int k = 0; k < 4; ++k) nq[k] = b.q[k] + dt * 0.5 * pq[k];
+  const double nn = sqrt(nq[1] * nq[1] + nq[2] * nq[2] + nq[3] * nq[3] + nq[0] * nq[0]);
+  double qh[4];
+  for (int k = 0; k < 4; ++k) qh[k] = nq[k] / nn;
+  M4d Nq;
+  for (int r = 0; r < 4; ++r)
+    for (int cc = 0; cc < 4; ++cc) Nq(r, cc) = (((r == cc) ? 1.0 : 0.0) - qh[r] * qh[cc]) / nn;
+  M4d Pq;
+  const double pv[16] = {p[0], -p[1], -p[2], -p[3], p[1], p[0], -p[3], p[2],
+                         p[2], p[3], p[0], -p[1], p[3], -p[2], p[1], p[0]};
+  for (int k = 0; k < 16; ++k) Pq.a[k] = pv[k];
+  M43d Pw;
+  const double* q = b.q;
+  const double wv[12] = {-q[1], -q[2], -q[3], q[0], q[3], -q[2], -q[3], q[0], q[1], q[2], -q[1], q[0]};
+  for (int k = 0; k < 12; ++k) Pw.a[k] = wv[k];
+  Jqv = mm(Nq, Jqv + (mm(Pw, Jwv) + mm(Pq, Jqv)) * (dt / 2.0));
+  Jqw = mm(Nq, Jqw + (mm(Pw, Jww) + mm(Pq, Jqw)) * (dt / 2.0));
+  st3(b.Jm + 0, Jxv); st3(b.Jm + 9, Jxw); st43(b.Jm + 54, Jqv); st43(b.Jm + 66, Jqw);
+}
+
 __global__ void k_rigid_step(RigidStepConst rc, RigidDev* __restrict__ rig) {
   const int bi = threadIdx.x;
   if (bi >= rc.nbodies) return;
   RigidDev& b = rig[bi];
   if (!b.dynamic) return;
-  if (rc.compute_gradient && !b.animated) chain_rule(b, rc);
+  if (rc.compute_gradient && !b.animated) {
+    if (rc.use_manager) {
+      for (int k = 0; k < 78; ++k) b.mnet[k] = b.net[k];  // update_rigid_body_gradient_manager (:952-967)
+    } else {
+      chain_rule(b, rc);
+    }
+  }
+  // the manager's chain runs for every dynamic body (SimulatorBase.cpp:1159-1166)
+  if (rc.compute_gradient && rc.use_manager) manager_velocity_chain(b, rc);
   const double dt = rc.dt;
   const V3d F = {b.F[0], b.F[1], b.F[2]}, T = {b.T[0], b.T[1], b.T[2]};
   if (!b.animated) {
@@ -263,6 +337,7 @@ __global__ void k_rigid_step(RigidStepConst rc, RigidDev* __restrict__ rig) {
   for (int k = 0; k < 3; ++k) { b.F[k] = 0.0; b.T[k] = 0.0; }
   if (!b.animated)
     for (int k = 0; k < 3; ++k) b.v[k] += (rc.g[k] * b.mass) * (b.invMass * dt);
+  if (rc.compute_gradient && rc.use_manager) manager_position_chain(b, rc);
   // animate()
   for (int k = 0; k < 3; ++k) b.x[k] += b.v[k] * dt;
   const double wq[4] = {0.0, b.w[0], b.w[1], b.w[2]};

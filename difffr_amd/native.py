@@ -27,7 +27,7 @@ EXPORTS = [
     "dfr_mlp_loss_grad", "dfr_mlp_input_grad", "dfr_set_halo_capacity", "dfr_halo_stats",
     "dfr_set_profiling_classes", "dfr_get_body", "dfr_set_gradient_mode", "dfr_count_1ring", "dfr_adjoint_start_gradient",
     "dfr_adjoint_segment_memory", "dfr_state_save", "dfr_state_load", "dfr_adjoint_arena_limit",
-    "dfr_state_free", "dfr_adjoint_tape_bytes", "dfr_set_compute_gradient",
+    "dfr_state_free", "dfr_adjoint_tape_bytes", "dfr_set_compute_gradient", "dfr_get_gradient_manager",
 ]
 
 MLP_NP = 8898
@@ -42,7 +42,8 @@ class Params(C.Structure):
         ("enable_divergence_solver", C.c_int32), ("gradient_mode", C.c_int32),
         ("optimize_rotation", C.c_int32), ("gyroscopic", C.c_int32), ("compute_gradient", C.c_int32),
         ("uniform_acc_time", C.c_double), ("target_time", C.c_double),
-        ("max_neighbors", C.c_int32), ("max_boundary_neighbors", C.c_int32),
+        ("max_neighbors", C.c_int32), ("max_boundary_neighbors", C.c_int32), ("release_mode", C.c_int32),
+        ("use_gradient_manager", C.c_int32),
     ]
 
 
@@ -88,6 +89,7 @@ def load(path: str | None = None):
     for f in ("dfr_get_rigid", "dfr_get_jacobians", "dfr_get_net_gradients", "dfr_set_rigid"):
         getattr(L, f).argtypes = [P, C.c_int, dp]
     L.dfr_reset_gradient.argtypes = [P]
+    L.dfr_get_gradient_manager.argtypes = [P, C.c_int, C.c_int, dp, dp]
     L.dfr_get_fluid.argtypes = [P, fp, fp]
     L.dfr_get_field.argtypes = [P, C.c_char_p, fp]
     L.dfr_set_field.argtypes = [P, C.c_char_p, fp]

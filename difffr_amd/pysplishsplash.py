@@ -28,9 +28,10 @@ What differs, and why:
   (SimulatorBase.cpp:649-683).  With `useGui=False` a negative stopAt is an
   error there and here; with `useGui=True` the loop runs until the callback
   ends it (the drivers call `quit()`), or until stopAt when it is positive.
-* The multi-body RigidBodyGradientManager path is not part of this library:
-  `useRigidGradientManager()` is False and `getRigidBodyGradientManager()`
-  raises NotImplementedError.
+* The RigidBodyGradientManager holds the single-body entries (R, R) of its
+  chain (scene parameter `use_gradient_manager`, the scene key
+  useRigidGradientManager); entries between two bodies, which only the rigid
+  contact solver fills, are zero.
 * A reference C++ `assert` or crash on a bad index is an IndexError here.
 """
 from __future__ import annotations
@@ -49,7 +50,7 @@ from .simulation import DfrError
 from .simulation import Simulation as _CoreSimulation
 
 __all__ = ["Exec", "GUI", "Simulation", "TimeManager", "TimeStepDiffDFSPH", "BoundaryModelAkinci2012",
-           "RigidBodyObject", "Utilities", "DfrError"]
+           "RigidBodyObject", "RigidBodyGradientManager", "Utilities", "DfrError"]
 
 _current: dict = {"base": None}
 
@@ -390,7 +391,48 @@ class Simulation:
         return self._base._gradient_mode
 
     def useRigidGradientManager(self) -> bool:
-        return False
+        return bool(int(self._base._core.params.get("use_gradient_manager", 0)))
+
+
+class RigidBodyGradientManager:
+    """SPH.RigidBodyGradientManager (SimulationModule.cpp:406-413): the chain
+    of RigidBodyGradientManager.cpp:96-423 that replaces perform_chain_rule
+    when the scene sets useRigidGradientManager.  Getters return copies, as
+    pybind11 does for Eigen values."""
+
+    def __init__(self, base: "Exec.SimulatorBase"):
+        self._base = base
+
+    def _entries(self, R: int, RR: int):
+        core = self._base._require()
+        n = core.n_bodies
+        if not (0 <= int(R) < n and 0 <= int(RR) < n):
+            raise IndexError(f"body indices ({R}, {RR}) out of range [0, {n})")
+        return core.gradient_manager_raw(int(R), int(RR))
+
+    def reset(self):
+        """Jacobians back to (I, 0) and the handed sums to zero (also resets
+        the boundary models' own Jacobians)."""
+        core = self._base._require()
+        core._check(native.load().dfr_reset_gradient(core.ctx))
+
+    def get_grad_net_force_to_vn(self, R: int, RR: int) -> np.ndarray:
+        return self._entries(R, RR)[1][0:9].reshape(3, 3).copy()
+
+    def get_grad_net_torque_to_omega_n(self, R: int, RR: int) -> np.ndarray:
+        return self._entries(R, RR)[1][27:36].reshape(3, 3).copy()
+
+    def get_grad_x_to_v0(self, R: int, RR: int) -> np.ndarray:
+        return self._entries(R, RR)[0][0:9].reshape(3, 3).copy()
+
+    def get_grad_x_to_omega0(self, R: int, RR: int) -> np.ndarray:
+        return self._entries(R, RR)[0][9:18].reshape(3, 3).copy()
+
+    def get_grad_q_to_v0(self, R: int, RR: int) -> np.ndarray:
+        return self._entries(R, RR)[0][54:66].reshape(4, 3).copy()
+
+    def get_grad_q_to_omega0(self, R: int, RR: int) -> np.ndarray:
+        return self._entries(R, RR)[0][66:78].reshape(4, 3).copy()
 
 
 class TimeManager:
@@ -464,6 +506,7 @@ class _SimulatorBase:
         self._core = _CoreSimulation(self._scene)
         self._timestep = TimeStepDiffDFSPH(self)
         self._simulation = Simulation(self)
+        self._manager = RigidBodyGradientManager(self)
         _current["base"] = self
 
     def initSimulationWithDeferredInit(self):
@@ -563,8 +606,9 @@ class _SimulatorBase:
         self._states[name] = slot  # most recently used
         core.state_load(slot)
 
-    def getRigidBodyGradientManager(self):
-        raise NotImplementedError("the multi-body RigidBodyGradientManager path is not part of this library")
+    def getRigidBodyGradientManager(self) -> RigidBodyGradientManager:
+        self._require()
+        return self._manager
 
     # -- stepping -------------------------------------------------------------
     def _one_step(self):

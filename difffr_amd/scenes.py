@@ -59,7 +59,8 @@ def default_params(radius: float, dt: float = 1e-3) -> dict:
         max_error=0.01, max_error_v=0.1, min_iterations=2, max_iterations=100,
         max_iterations_v=100, use_pressure_warmstart=1, use_divergence_warmstart=1,
         enable_divergence_solver=1, gradient_mode=1, optimize_rotation=1, gyroscopic=1,
-        compute_gradient=1, uniform_acc_time=0.0, target_time=1e30,
+        compute_gradient=1, uniform_acc_time=0.0, target_time=1e30, release_mode=0,
+        use_gradient_manager=0,
     )
 
 

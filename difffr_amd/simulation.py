@@ -44,7 +44,8 @@ def make_params(p: dict) -> native.Params:
             setattr(q, k, float(p[k]))
     for k in ("min_iterations", "max_iterations", "max_iterations_v", "use_pressure_warmstart",
               "use_divergence_warmstart", "enable_divergence_solver", "gradient_mode", "optimize_rotation",
-              "gyroscopic", "compute_gradient", "max_neighbors", "max_boundary_neighbors"):
+              "gyroscopic", "compute_gradient", "max_neighbors", "max_boundary_neighbors", "release_mode",
+              "use_gradient_manager"):
         if k in p:
             setattr(q, k, int(p[k]))
     if "gravity" in p:
@@ -231,6 +232,13 @@ class Simulation:
 
     def jacobians(self, body: int) -> dict:
         return unpack_jacobians(self.jacobians_raw(body))
+
+    def gradient_manager_raw(self, R: int, RR: int):
+        """RigidBodyGradientManager entries (R, RR): (Jacobians[78], per-step sums[78])."""
+        j = np.empty(78)
+        n = np.empty(78)
+        self._check(native.load().dfr_get_gradient_manager(self.ctx, R, RR, _dp(j), _dp(n)))
+        return j, n
 
     def net_gradients_raw(self, body: int) -> np.ndarray:
         o = np.empty(78)

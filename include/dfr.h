@@ -60,6 +60,12 @@ typedef struct dfr_params {
   double target_time;
   int32_t max_neighbors;         /* per fluid particle (fluid + boundary), default 96 */
   int32_t max_boundary_neighbors;/* fluid neighbours per dynamic sample, default 128 */
+  int32_t release_mode;          /* useReleaseRigidBodyMode (Simulation.h:444): body 1 is held at rest
+                                    until uniform_acc_time + dt, then set to (v0, omega0) at the start
+                                    of every step; other bodies are not driven */
+  int32_t use_gradient_manager;  /* useRigidGradientManager (Simulation.cpp:123): the per-step sums go to
+                                    the RigidBodyGradientManager chain (dfr_get_gradient_manager) instead
+                                    of perform_chain_rule; dfr_get_jacobians then stays at its reset value */
 } dfr_params;
 
 typedef struct dfr_stats {
@@ -163,6 +169,14 @@ int dfr_get_jacobians(const dfr_ctx* ctx, int body, double out[78]);
  * dtau/domega, dtau/dx, dtau/dv (3x3), dF/dq, dtau/dq (3x4)
  * (grad_net_force_to_vn etc., BoundaryModel_Akinci2012.cpp:453-497). */
 int dfr_get_net_gradients(const dfr_ctx* ctx, int body, double out[78]);
+
+/* RigidBodyGradientManager entries of bodies (R, RR) (SimulationModule.cpp:406-413,
+ * RigidBodyGradientManager.h:24-52): jac[78] in dfr_get_jacobians' layout
+ * (get_grad_x_to_v0(R, RR), get_grad_q_to_v0(R, RR), ...), net[78] in
+ * dfr_get_net_gradients' layout (get_grad_net_force_to_vn(R, RR), ...).  Either
+ * pointer may be NULL.  Entries with R != RR are zero (no rigid contact solver),
+ * as are all entries of a static body or without use_gradient_manager. */
+int dfr_get_gradient_manager(const dfr_ctx* ctx, int R, int RR, double jac[78], double net[78]);
 
 /* Clears the forward-mode Jacobians to their initial values
  * (TimeStepDiffDFSPH reset_gradient / BoundaryModel_Akinci2012::reset_gradient). */

@@ -50,6 +50,8 @@ struct Params {
   int computeGradient = 1;       // the BACKWARD build of the reference
   double uniformAccTime = 0.0;   // uniform_acc_rb_time
   double targetTime = 1e30;
+  int releaseMode = 0;           // useReleaseRigidBodyMode, Simulation.h:444
+  int useGradientManager = 0;    // useRigidGradientManager, Simulation.cpp:123
 };
 
 // Eval of the analytic cubic spline kernel: SPlisHSPlasH/SPHKernels.h:16-176.
@@ -182,10 +184,23 @@ template <typename RF, typename RR> struct Body {
   // forward-mode rigid Jacobians, BoundaryModel_Akinci2012.cpp:94-106
   M3<RR> Jvv, Jvw, Jwv, Jww, Jxv, Jxw;
   M43<RR> Jqv, Jqw;
+  // RigidBodyGradientManager entries (R, R) of this body: the sums it was last
+  // handed (update_rigid_body_gradient_manager, BoundaryModel_Akinci2012.cpp:952-967)
+  // and its Jacobians (RigidBodyGradientManager.h:58-65)
+  M3<RR> mFv, mFx, mFw, mTw, mTx, mTv;
+  M34<RR> mFq, mTq;
+  M3<RR> mJvv, mJvw, mJwv, mJww, mJxv, mJxw;
+  M43<RR> mJqv, mJqw;
   void reset_jacobians() {
     Jvv.setIdentity(); Jww.setIdentity();
     Jvw.setZero(); Jwv.setZero(); Jxv.setZero(); Jxw.setZero();
     Jqv.setZero(); Jqw.setZero();
+    // RigidBodyGradientManager::reset, RigidBodyGradientManager.cpp:474-523
+    mJvv.setIdentity(); mJww.setIdentity();
+    mJvw.setZero(); mJwv.setZero(); mJxv.setZero(); mJxw.setZero();
+    mJqv.setZero(); mJqw.setZero();
+    mFv.setZero(); mFx.setZero(); mFw.setZero(); mTw.setZero(); mTx.setZero(); mTv.setZero();
+    mFq.setZero(); mTq.setZero();
   }
 };
 
@@ -410,14 +425,28 @@ template <typename RF, typename RR> class Sim {
     }
   }
 
-  // TimeStepDiffDFSPH::beginStep, TimeStepDiffDFSPH.cpp:353-430 (passive-body branch).
+  // TimeStepDiffDFSPH::beginStep, TimeStepDiffDFSPH.cpp:353-430.
   void begin_step() {
     stepCount++;
     const RF uat = RF(P.uniformAccTime);
     for (size_t k = 0; k < bodies.size(); ++k) {
       auto& b = bodies[k];
       if (!b.dynamic) continue;
-      if (time <= uat + dt) {
+      if (P.releaseMode) {  // :381-403: only boundary model 1 is driven
+        if (k != 1) continue;
+        RR factor = 1;
+        if (time <= uat + dt) {
+          if (uat > RF(1e-3)) b.animated = true;
+          factor = 0;
+        } else {
+          b.animated = false;
+        }
+        b.vrb = b.v0 * factor;
+        b.wrb = b.w0 * factor;
+        if (tp) { tp->bodies[k].ramp = 1; tp->bodies[k].factor = factor; }
+        continue;
+      }
+      if (time <= uat + dt) {  // :405-426
         RF factor = 1;
         if (uat > RF(1e-3)) {
           factor = (time / uat) > RF(1) ? RF(1) : (time / uat);
@@ -817,7 +846,8 @@ template <typename RF, typename RR> class Sim {
     for (int t = 0; t < nthreads; ++t) { F += b.Fth[t]; T += b.Tth[t]; }
   }
 
-  // compute_grad_inertia_v_to_{omega0,v0}, BoundaryModel_Akinci2012.cpp:524-541.
+  // compute_grad_inertia_v_to_{omega0,v0}, BoundaryModel_Akinci2012.cpp:524-541
+  // (the manager's, RigidBodyGradientManager.cpp:263-279, is the same).
   static M3<RR> grad_inertia_v(const Body<RF, RR>& b, const V3<RR>& vv, const M43<RR>& Jq) {
     const M3<RR> R = b.q.toRotationMatrix();
     const M3<RR> a = grad_Rqp_to_q(b.q, b.I0 * (R.transpose() * vv)) * Jq;
@@ -913,8 +943,70 @@ template <typename RF, typename RR> class Sim {
     for (auto& b : bodies)
       if (b.dynamic && !b.animated) {
         accumulate_and_reset_gradient(b);
-        perform_chain_rule(b, RR(dt));
+        if (P.useGradientManager) hand_to_manager(b);
+        else perform_chain_rule(b, RR(dt));
       }
+  }
+
+  // BoundaryModel_Akinci2012::update_rigid_body_gradient_manager, :952-967.
+  static void hand_to_manager(Body<RF, RR>& b) {
+    b.mFv = b.nFv; b.mFx = b.nFx; b.mFw = b.nFw; b.mFq = b.nFq;
+    b.mTv = b.nTv; b.mTx = b.nTx; b.mTw = b.nTw; b.mTq = b.nTq;
+  }
+
+  // RigidBodyGradientManager (entries R == RR; no rigid contact solver), as
+  // SimulatorBase::timeStepNoGUI runs it (SimulatorBase.cpp:1159-1166) for every
+  // dynamic body: perform_force_and_torque_chain_rule (:96-156), whose
+  // d(F, T)/d(v0, omega0) use the v and omega sums and the Jacobians before this
+  // step's update, then perform_velocity_chain_rule (:276-365) ...
+  void manager_velocity_chain(Body<RF, RR>& b) {
+    const RR dtR = RR(dt);
+    const M3<RR> dF_v0 = b.mFv * b.mJvv + b.mFw * b.mJwv;
+    const M3<RR> dF_w0 = b.mFv * b.mJvw + b.mFw * b.mJww;
+    const M3<RR> dT_v0 = b.mTv * b.mJvv + b.mTw * b.mJwv;
+    const M3<RR> dT_w0 = b.mTv * b.mJvw + b.mTw * b.mJww;
+    b.mJvv = b.mJvv + dF_v0 * (dtR * b.invMass);
+    b.mJvw = b.mJvw + dF_w0 * (dtR * b.invMass);
+    const M3<RR> I = b.I, Iinv = b.Iinv;
+    const V3<RR> om = b.wrb;
+    V3<RR> F, T;
+    force_and_torque(b, F, T);
+    V3<RR> temp_v = T;
+    M3<RR> dTau_w0 = dT_w0, dTau_v0 = dT_v0;
+    if (P.gyroscopic) {
+      temp_v = (I * om).cross(om) + T;
+      dTau_w0 = skew(I * om) * b.mJww + skew(om).transpose() * (I * b.mJww + grad_inertia_v(b, om, b.mJqw)) + dT_w0;
+      dTau_v0 = skew(I * om) * b.mJwv + skew(om).transpose() * (I * b.mJwv + grad_inertia_v(b, om, b.mJqv)) + dT_v0;
+    }
+    // the same "+" on the d(I^-1) term as perform_chain_rule (:351-357)
+    const M3<RR> gIi_w0 = Iinv * grad_inertia_v(b, Iinv * temp_v, b.mJqw);
+    const M3<RR> gIi_v0 = Iinv * grad_inertia_v(b, Iinv * temp_v, b.mJqv);
+    b.mJww = b.mJww + (gIi_w0 + Iinv * dTau_w0) * dtR;
+    b.mJwv = b.mJwv + (gIi_v0 + Iinv * dTau_v0) * dtR;
+  }
+
+  // ... and, after the velocity update (the new omega) and before the position
+  // update (the current q), perform_position_and_rotation_chain_rule (:367-423).
+  void manager_position_chain(Body<RF, RR>& b) {
+    const RR dtR = RR(dt);
+    b.mJxv = b.mJxv + b.mJvv * dtR;
+    b.mJxw = b.mJxw + b.mJvw * dtR;
+    const Quat<RR> q = b.q;
+    const Quat<RR> p(0, b.wrb.x, b.wrb.y, b.wrb.z);
+    Quat<RR> nq = q;
+    const Quat<RR> pq = p * nq;
+    const RR hh = dtR * RR(0.5);
+    nq.x += hh * pq.x; nq.y += hh * pq.y; nq.z += hh * pq.z; nq.w += hh * pq.w;
+    const Quat<RR> nqn = nq.normalized();
+    const RR qv[4] = {nqn.w, nqn.x, nqn.y, nqn.z};
+    M4<RR> Nq;
+    const RR nn = nq.norm();
+    for (int r = 0; r < 4; ++r)
+      for (int c = 0; c < 4; ++c) Nq(r, c) = ((r == c ? RR(1) : RR(0)) - qv[r] * qv[c]) / nn;
+    const M4<RR> Pq = grad_p_q_product_to_q(p);
+    const M43<RR> Pw = grad_omega_q_product_to_omega(q);
+    b.mJqv = Nq * (b.mJqv + (Pw * b.mJwv + Pq * b.mJqv) * (dtR / RR(2.)));
+    b.mJqw = Nq * (b.mJqw + (Pw * b.mJww + Pq * b.mJqw) * (dtR / RR(2.)));
   }
 
   // RigidBody3dBoundarySimulator::velocityTimeStep (:278-320) with
@@ -1002,7 +1094,10 @@ template <typename RF, typename RR> class Sim {
     for (long i = 0; i < N; ++i) x[i] += v[i] * dt;  // :613-634
     time = time + dt;
     backward_per_step();
+    const bool mgr = P.computeGradient && P.useGradientManager;
+    if (mgr) for (auto& b : bodies) if (b.dynamic) manager_velocity_chain(b);
     rigid_velocity_step();
+    if (mgr) for (auto& b : bodies) if (b.dynamic) manager_position_chain(b);
     if (tp) {
       for (size_t k = 0; k < bodies.size(); ++k) { tp->bodies[k].F = bodies[k].F_last; tp->bodies[k].T = bodies[k].T_last; }
       tp->Iv = stats.iterationsV; tp->Ip = stats.iterations;
@@ -1023,7 +1118,7 @@ template <typename RF, typename RR> class Sim {
   // v' = v + dt F / m the position and velocity after this step's force update:
   // the reference's torque chain reads J_x and J_v after their update
   // (BoundaryModel_Akinci2012.cpp:581-613), so its derivative is that of a
-  // torque evaluated there.  The per-body sums grad_net_* are the law's
+  // torque evaluated there (with the gradient manager, x' = x and v' = v).  The per-body sums grad_net_* are the law's
   // matrices (A -> F_x, F_v, F_w, F_q; B -> T_x, T_v, T_w, T_q).
   // useFrozen: the angular update uses (2 Ibar^-1 - I(q)^-1) with Ibar^-1 the
   // world inverse inertia of a base run at the same step (recordFrozen).  At
@@ -1066,7 +1161,10 @@ template <typename RF, typename RR> class Sim {
         const V3<RR> F = affine(fm.F0, fm.A, b.xrb, b.vrb, b.wrb, b.q);
         const V3<RR> v1 = b.vrb + F * (b.invMass * RR(dt));
         const V3<RR> x1 = b.xrb + v1 * RR(dt);
-        const V3<RR> T = affine(fm.T0, fm.B, x1, v1, b.wrb, b.q);
+        // the manager's torque chain reads J_x, J_v before their update: its
+        // derivative is that of a torque evaluated at the current state
+        const V3<RR> T = P.useGradientManager ? affine(fm.T0, fm.B, b.xrb, b.vrb, b.wrb, b.q)
+                                              : affine(fm.T0, fm.B, x1, v1, b.wrb, b.q);
         b.Fth[0] = F; b.Tth[0] = T;
         for (int r = 0; r < 3; ++r) {
           for (int c = 0; c < 3; ++c) {
@@ -1076,9 +1174,15 @@ template <typename RF, typename RR> class Sim {
           for (int c = 0; c < 4; ++c) { b.nFq(r, c) = RR(fm.A[13 * r + 9 + c]); b.nTq(r, c) = RR(fm.B[13 * r + 9 + c]); }
         }
       }
-      if (P.computeGradient && b.dynamic && !b.animated) perform_chain_rule(b, RR(dt));
+      if (P.computeGradient && b.dynamic && !b.animated) {
+        if (P.useGradientManager) hand_to_manager(b);
+        else perform_chain_rule(b, RR(dt));
+      }
     }
+    const bool mgr = P.computeGradient && P.useGradientManager;
+    if (mgr) for (auto& b : bodies) if (b.dynamic) manager_velocity_chain(b);
     rigid_velocity_step();
+    if (mgr) for (auto& b : bodies) if (b.dynamic) manager_position_chain(b);
     rigid_position_step();
   }
 

@@ -45,6 +45,7 @@ struct OracleBase {
   virtual int velocity_update_probe(int which, double* dv, double* ft) = 0;
   virtual int set_force_model(int body, int start, const double* f0t0, const double* A, const double* B, int flags) = 0;
   virtual int model_step(int n) = 0;
+  virtual int get_manager(int body, double* jac, double* net) const = 0;
 };
 
 template <typename RF, typename RR> struct Oracle : OracleBase {
@@ -162,6 +163,26 @@ template <typename RF, typename RR> struct Oracle : OracleBase {
     if (n == "kappa") { for (size_t i = 0; i < N; ++i) sim.kappa[i] = RF(in[i]); return 0; }
     if (n == "kappa_v") { for (size_t i = 0; i < N; ++i) sim.kappaV[i] = RF(in[i]); return 0; }
     if (n == "time") { sim.time = RF(in[0]); return 0; }
+    if (n == "x" || n == "v") {  // fluid positions or velocities [N][3]
+      auto& f = (n == "x") ? sim.x : sim.v;
+      for (size_t i = 0; i < N; ++i) f[i] = V3<RF>(RF(in[3 * i]), RF(in[3 * i + 1]), RF(in[3 * i + 2]));
+      return 0;
+    }
+    // "jac<b>" / "mjac<b>": forward-mode Jacobians of body b / of the gradient
+    // manager's (b, b) entry, in get_jacobians' layout (78)
+    const bool mj = n.rfind("mjac", 0) == 0;
+    if (mj || n.rfind("jac", 0) == 0) {
+      const int b = std::atoi(n.c_str() + (mj ? 4 : 3));
+      if (b < 0 || b >= int(sim.bodies.size())) return -1;
+      auto& B = sim.bodies[b];
+      M3<RR>* m3[6] = {&B.Jxv, &B.Jxw, &B.Jvv, &B.Jvw, &B.Jwv, &B.Jww};
+      M3<RR>* mm3[6] = {&B.mJxv, &B.mJxw, &B.mJvv, &B.mJvw, &B.mJwv, &B.mJww};
+      M43<RR>* q2[2] = {mj ? &B.mJqv : &B.Jqv, mj ? &B.mJqw : &B.Jqw};
+      int o = 0;
+      for (int k = 0; k < 6; ++k) for (int i = 0; i < 9; ++i) (mj ? mm3 : m3)[k]->a[i] = RR(in[o++]);
+      for (int k = 0; k < 2; ++k) for (int i = 0; i < 12; ++i) q2[k]->a[i] = RR(in[o++]);
+      return 0;
+    }
     if (n.rfind("rigid", 0) == 0) {  // "rigid<b>": x(3), q (w,x,y,z)(4), v(3), omega(3) of body b
       const int b = std::atoi(n.c_str() + 5);
       if (b < 0 || b >= int(sim.bodies.size())) return -1;
@@ -328,6 +349,23 @@ template <typename RF, typename RR> struct Oracle : OracleBase {
     S.v = v0;
     return 0;
   }
+  // RigidBodyGradientManager entries (body, body): Jacobians in get_jacobians'
+  // layout, handed sums in get_net_gradients' layout.
+  int get_manager(int body, double* jac, double* net) const override {
+    if (body < 0 || body >= int(sim.bodies.size())) return -1;
+    const auto& b = sim.bodies[body];
+    const M3<RR>* j3[6] = {&b.mJxv, &b.mJxw, &b.mJvv, &b.mJvw, &b.mJwv, &b.mJww};
+    int o = 0;
+    for (auto m : j3) for (int i = 0; i < 9; ++i) jac[o++] = double(m->a[i]);
+    for (int i = 0; i < 12; ++i) jac[o++] = double(b.mJqv.a[i]);
+    for (int i = 0; i < 12; ++i) jac[o++] = double(b.mJqw.a[i]);
+    const M3<RR>* n3[6] = {&b.mFv, &b.mFx, &b.mFw, &b.mTw, &b.mTx, &b.mTv};
+    o = 0;
+    for (auto m : n3) for (int i = 0; i < 9; ++i) net[o++] = double(m->a[i]);
+    for (int i = 0; i < 12; ++i) net[o++] = double(b.mFq.a[i]);
+    for (int i = 0; i < 12; ++i) net[o++] = double(b.mTq.a[i]);
+    return 0;
+  }
   // flags: 1 = record the base run's inverse inertia per step (clears the
   // record), 2 = run the frozen-inverse map (Sim::ForceModel).
   int set_force_model(int body, int start, const double* f0t0, const double* A, const double* B, int flags) override {
@@ -482,5 +520,8 @@ int orc_set_force_model(void* h, int body, int start, const double* f0t0, const 
   return static_cast<OracleBase*>(h)->set_force_model(body, start, f0t0, A, B, flags);
 }
 int orc_model_step(void* h, int n) { return static_cast<OracleBase*>(h)->model_step(n); }
+int orc_get_manager(void* h, int body, double* jac, double* net) {
+  return static_cast<OracleBase*>(h)->get_manager(body, jac, net);
+}
 const char* orc_last_error() { return g_err.c_str(); }
 }

@@ -25,7 +25,8 @@ class Params(C.Structure):
         ("usePressureWarmstart", C.c_int), ("useDivergenceWarmstart", C.c_int),
         ("enableDivergenceSolver", C.c_int), ("gradientMode", C.c_int),
         ("optimizeRotation", C.c_int), ("gyroscopic", C.c_int), ("computeGradient", C.c_int),
-        ("uniformAccTime", C.c_double), ("targetTime", C.c_double),
+        ("uniformAccTime", C.c_double), ("targetTime", C.c_double), ("releaseMode", C.c_int),
+        ("useGradientManager", C.c_int),
     ]
 
 
@@ -87,6 +88,7 @@ def lib():
         L.orc_velocity_update_probe.argtypes = [P, C.c_int, dp, dp]
         L.orc_set_force_model.argtypes = [P, C.c_int, C.c_int, dp, dp, dp, C.c_int]
         L.orc_model_step.argtypes = [P, C.c_int]
+        L.orc_get_manager.argtypes = [P, C.c_int, dp, dp]
         assert L.orc_params_size() == C.sizeof(Params), "oracle Params layout mismatch"
         _lib = L
     return _lib
@@ -110,6 +112,8 @@ def make_params(p: dict) -> Params:
     q.gradientMode = p["gradient_mode"]; q.optimizeRotation = p["optimize_rotation"]
     q.gyroscopic = p["gyroscopic"]; q.computeGradient = p["compute_gradient"]
     q.uniformAccTime = p["uniform_acc_time"]; q.targetTime = p["target_time"]
+    q.releaseMode = int(p.get("release_mode", 0))
+    q.useGradientManager = int(p.get("use_gradient_manager", 0))
     return q
 
 
@@ -282,6 +286,13 @@ class OracleSim:
         o = np.empty(78)
         lib().orc_get_jacobians(self.h, body, _dp(o))
         return o
+
+    def gradient_manager(self, body: int):
+        """RigidBodyGradientManager entries (body, body): (Jacobians[78], handed sums[78])."""
+        j = np.empty(78)
+        n = np.empty(78)
+        lib().orc_get_manager(self.h, body, _dp(j), _dp(n))
+        return j, n
 
     def net_gradients(self, body: int) -> np.ndarray:
         o = np.empty(78)

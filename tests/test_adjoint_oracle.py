@@ -54,6 +54,30 @@ def test_adjoint_matches_central_differences(body):
     np.testing.assert_allclose(g, fd, rtol=0, atol=1e-6 * scale)
 
 
+@pytest.mark.parametrize("release", [0, 1])
+def test_adjoint_with_ramp_and_release_windows(release):
+    """uniform_acc_time = 4.5 ms (TimeStepDiffDFSPH::beginStep :353-430): the
+    passive-body ramp drives the body with (t / uat) (v0, omega0) and holds it
+    animated through uat + dt; the release mode holds body 1 at rest through
+    uat + dt and then sets (v0, omega0) at the start of every step.  Adjoint
+    against central differences of the whole rollout."""
+    sc = scenes.submerged_body(body="ellipsoid", steps=15)
+    sc.params["uniform_acc_time"] = 0.0045  # window edge between steps: t = 5 ms in, 6 ms out
+    sc.params["release_mode"] = release
+    seed = np.random.default_rng(6).normal(size=7)
+    g, fd = _adjoint_and_fd(sc, 15, seed)
+    scale = np.abs(fd).max()
+    assert scale > 1e-4
+    np.testing.assert_allclose(g, fd, rtol=0, atol=1e-6 * scale)
+    # inside the window the body is animated: step 3 starts at t = 2 dt, so the
+    # ramp gives 0.4 (v0, omega0) and the release mode holds it at rest
+    s, _ = _run(sc, "f64", sc.bodies[1].v0, sc.bodies[1].w0, 3)
+    r = s.rigid(1)
+    f = 0.0 if release else 2 * sc.params["dt"] / 0.0045
+    np.testing.assert_allclose(r["v"], f * sc.bodies[1].v0, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(r["omega"], f * sc.bodies[1].w0, rtol=1e-12, atol=1e-15)
+
+
 def test_adjoint_free_flight_closed_form():
     sc = scenes.submerged_body(steps=12)
     sc.fluid_x = sc.fluid_x[:0]

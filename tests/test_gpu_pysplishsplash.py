@@ -269,3 +269,52 @@ def test_saved_states_are_bounded(gpu_ready):
     with pytest.raises(NotImplementedError):
         base.loadStateWithRigidExisted(names[0])
     base.cleanup()
+
+
+def test_gradient_manager_in_the_finite_difference_loop(gpu_ready):
+    """experiments/others/python/finite-difference.py:55-126 with the scene's
+    useRigidGradientManager: each step's rb_grad_manager.get_grad_net_force_to_vn(1, 1)
+    is that step's reference-mode dF/dv; the manager's Jacobians move while the
+    boundary model's own stay at their reset value; entries between two bodies
+    are zero."""
+    sc = _scene()
+    sc.params["use_gradient_manager"] = 1
+    base = sph.Exec.SimulatorBase()
+    base.init(sceneFile=sc, useGui=False, stopAt=0.0)
+    base.initSimulation()
+    sim = sph.Simulation.getCurrent()
+    assert sim.useRigidGradientManager()
+    mgr = base.getRigidBodyGradientManager()
+    bm = sim.getBoundaryModel(1)
+    rb = bm.getRigidBodyObject()
+    base.forwardFixedSteps(3)
+    name = base.saveState()
+    base.singleTimeStep()
+    F0 = bm.getForce().copy()
+    ours = mgr.get_grad_net_force_to_vn(1, 1)
+    np.testing.assert_array_equal(ours, base._core.net_gradients_raw(1)[:9].reshape(3, 3))
+    base.loadStateWithRigidExisted(name)
+    v0 = rb.getVelocity()
+    fd = np.zeros((3, 3))
+    delta = 1e-3
+    for i in range(3):
+        v = v0.copy()
+        v[i] += delta
+        rb.setVelocity(v)
+        base.singleTimeStep()
+        fd[:, i] = (bm.getForce() - F0) / delta
+        base.loadStateWithRigidExisted(name)
+    err = np.linalg.norm(ours - fd) / (np.linalg.norm(fd) + 1e-10)
+    print(f"[manager finite differences] ||gradient error|| = {err:.3f}\nours\n{ours}\nfd\n{fd}", flush=True)
+    assert np.array_equal(np.sign(fd), np.sign(ours))
+    base.forwardFixedSteps(5)
+    n = sim.getTimeStep().get_step_count()
+    Jx = mgr.get_grad_x_to_v0(1, 1)
+    free = n * sc.params["dt"] * np.eye(3)  # the fluid-free value
+    assert np.abs(Jx - free).max() > 1e-6 * n * sc.params["dt"]  # the fluid's force entered the chain
+    np.testing.assert_allclose(np.diag(Jx), np.diag(free), rtol=0.5)
+    np.testing.assert_array_equal(bm.get_grad_x_to_v0(), np.zeros((3, 3)))
+    np.testing.assert_array_equal(mgr.get_grad_q_to_omega0(1, 0), np.zeros((4, 3)))
+    with pytest.raises(IndexError):
+        mgr.get_grad_x_to_v0(1, 7)
+    base.cleanup()

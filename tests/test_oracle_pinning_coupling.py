@@ -412,16 +412,20 @@ CHAIN_CASES = {
 }
 
 
-@pytest.mark.parametrize("mode", [1, 0, 2])
+@pytest.mark.parametrize("mode", [1, 0, 2, "manager"])
 def test_chain_rule_under_force_laws_matches_central_differences(mode):
-    """perform_chain_rule (modes 1, 0, 2) with the gyroscopic term, the
+    """perform_chain_rule (modes 1, 0, 2) and the RigidBodyGradientManager chain
+    (RigidBodyGradientManager.cpp:96-423) with the gyroscopic term, the
     quaternion update and normalisation, against central differences of the
     rigid update map in (v0, omega0) over 40 steps, the force/torque given by
     affine laws in the body state (Sim::ForceModel).  The map uses the frozen
     inverse inertia that carries the reference's '+' on the d(I^-1) term."""
     dt = 0.01
     sc = _rigid_only_scene(dt)
-    rng = np.random.default_rng(7 + mode)
+    manager = mode == "manager"
+    if manager:
+        mode = 1  # the manager uses the v and omega sums only, whatever the mode
+    rng = np.random.default_rng(7 + mode + (10 if manager else 0))
     A = rng.normal(scale=0.3, size=(3, 13))
     B = rng.normal(scale=0.01, size=(3, 13))
     F0 = rng.normal(scale=1.0, size=3)
@@ -434,12 +438,18 @@ def test_chain_rule_under_force_laws_matches_central_differences(mode):
             if not on:
                 A[:, c[0]:c[1]] = 0.0; B[:, c[0]:c[1]] = 0.0
     steps = 40
-    o = po.OracleSim(sc, "f64", {"compute_gradient": 1, "gradient_mode": mode})
+    o = po.OracleSim(sc, "f64", {"compute_gradient": 1, "gradient_mode": mode,
+                                 "use_gradient_manager": int(manager)})
     v0 = sc.bodies[1].v0.copy()
     w0 = np.zeros(3)
     o.set_force_model(1, F0, T0, A, B, start=2, record_frozen=True)
     o.model_step(steps)
     J = o.jacobians(1)
+    if manager:  # the body's own Jacobians stay at their reset value
+        J0 = np.zeros(78)
+        J0[[18, 22, 26, 45, 49, 53]] = 1.0
+        np.testing.assert_array_equal(J, J0)
+        J = o.gradient_manager(1)[0]
     base = o.rigid(1)
     assert np.abs(base["omega"]).max() > 0.1 or mode == 2
 

@@ -76,6 +76,9 @@ SURFACE = {
         "getVelocity", "setVelocity", "getRotation", "setRotation", "getWorldSpaceRotation",
         "getAngularVelocity", "setAngularVelocity"],
     sph.TimeManager: ["getCurrent", "getTime", "getTimeStepSize"],
+    sph.RigidBodyGradientManager: [
+        "reset", "get_grad_net_force_to_vn", "get_grad_net_torque_to_omega_n", "get_grad_x_to_omega0",
+        "get_grad_x_to_v0", "get_grad_q_to_v0", "get_grad_q_to_omega0"],
 }
 
 
@@ -103,7 +106,7 @@ def test_unsupported_inputs_fail_loudly(tmp_path):
     base.setValueBool(base.STATE_EXPORT, False)
     base.setValueFloat(base.STOP_AT, 0.25)
     assert base.getValueFloat(base.STOP_AT) == 0.25
-    with pytest.raises(NotImplementedError, match="RigidBodyGradientManager"):
+    with pytest.raises(DfrError):  # before initSimulation
         base.getRigidBodyGradientManager()
 
 

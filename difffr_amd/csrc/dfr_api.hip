@@ -9,6 +9,7 @@
 // recompute kernel leaves in device memory.
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1266,6 +1267,48 @@ bool reverse_segment_kept(dfr_ctx* c, int64_t seg, int64_t s0, int64_t s1) {
 }
 
 // Device scratch for the host-array entry points below.
+// ---------------------------------------------------------------------------
+// Allocation-free network loss/gradient (dfr_mlp_loss_grad_noalloc).  Its
+// staging (features [rows][6], targets [rows][2]) and accumulators (gradient
+// [MLP_NP] floats, loss one double) live in `scratch`, one fixed array per GPU.
+// In host code `scratch` names no device memory: its address on a device comes
+// from hipGetSymbolAddress, taken once per device at context creation with
+// that device current, and only that address is passed to kernels and copies.
+// ---------------------------------------------------------------------------
+constexpr size_t NA_ROWS = DFR_NOALLOC_MAX_ROWS;
+constexpr size_t NA_FEAT = 0;                                     // bytes
+constexpr size_t NA_TARG = NA_FEAT + sizeof(float) * 6 * NA_ROWS;
+constexpr size_t NA_GRAD = NA_TARG + sizeof(float) * 2 * NA_ROWS;
+constexpr size_t NA_LOSS = (NA_GRAD + sizeof(float) * MLP_NP + 255) / 256 * 256;
+constexpr size_t NA_BYTES = NA_LOSS + 256;
+constexpr int NA_MAX_DEVICES = 64;
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[NA_MAX_DEVICES] = {};  // per device, from hipGetSymbolAddress
+
+// Called with `device` current.
+void noalloc_setup(int device) {
+  if (device < 0 || device >= NA_MAX_DEVICES) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device]) return;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_dev[device] = static_cast<unsigned char*>(p);
+}
+
+unsigned char* noalloc_base(int device) {
+  if (device < 0 || device >= NA_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_dev[device];
+}
+
+// Clears the accumulators with a kernel (no runtime memset), 4-byte words.
+__global__ void k_clear_words(uint32_t* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 template <typename T> struct DevTmp {
   T* p = nullptr;
   explicit DevTmp(size_t n) { if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) throw HipError{hipErrorOutOfMemory, "DevTmp"}; }
@@ -1395,6 +1438,7 @@ dfr_ctx* dfr_create(const dfr_params* p, int device) {
     delete c;
     return nullptr;
   }
+  noalloc_setup(device);
   c->h = 4.f * (float)p->radius;  // Simulation.cpp:195-216
   c->dtf = (float)p->dt;
   const float pi = (float)M_PI;
@@ -2339,6 +2383,39 @@ int dfr_mlp_loss_grad(dfr_ctx* c, int64_t n, const float* feat, const float* tar
     sync_check(c);
     HIPCHK(hipMemcpy(grad, g.p, sizeof(float) * MLP_NP, hipMemcpyDeviceToHost));
     if (loss) HIPCHK(hipMemcpy(loss, l.p, sizeof(double), hipMemcpyDeviceToHost));
+    return DFR_OK;
+  });
+}
+
+int dfr_mlp_loss_grad_noalloc(dfr_ctx* c, int64_t n, const float* feat, const float* targ, float* grad,
+                              double* loss) {
+  return guarded(c, [&]() {
+    if (!c->mlp.has_params || n <= 0 || !feat || !targ || !grad)
+      return fail(c, DFR_ERR_ARG, "dfr_mlp_loss_grad_noalloc: bad arguments");
+    if (n > (int64_t)NA_ROWS) {
+      char buf[160];
+      std::snprintf(buf, sizeof(buf), "dfr_mlp_loss_grad_noalloc: %lld rows exceed the fixed scratch of %d rows",
+                    (long long)n, (int)DFR_NOALLOC_MAX_ROWS);
+      return fail(c, DFR_ERR_CAPACITY, buf);
+    }
+    HIPCHK(hipSetDevice(c->device));
+    unsigned char* base = noalloc_base(c->device);
+    if (!base) return fail(c, DFR_ERR_STATE, "dfr_mlp_loss_grad_noalloc: no scratch address on this device");
+    float* f = reinterpret_cast<float*>(base + NA_FEAT);
+    float* t = reinterpret_cast<float*>(base + NA_TARG);
+    float* g = reinterpret_cast<float*>(base + NA_GRAD);
+    double* l = reinterpret_cast<double*>(base + NA_LOSS);
+    const MlpState& M = c->mlp;
+    HIPCHK(hipMemcpyAsync(f, feat, sizeof(float) * 6 * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(t, targ, sizeof(float) * 2 * n, hipMemcpyHostToDevice, c->stream));
+    // the previous call's gradient and loss stay in the array: clear them
+    hipLaunchKernelGGL(k_clear_words, dim3((MLP_NP + 255) / 256), dim3(256), 0, c->stream,
+                       reinterpret_cast<uint32_t*>(g), MLP_NP);
+    hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<uint32_t*>(l), 2);
+    launch_mlp_train_batch(M.frag, M.P, M.norm, f, t, nullptr, (int)n, g, l, c->stream);
+    HIPCHK(hipMemcpyAsync(grad, g, sizeof(float) * MLP_NP, hipMemcpyDeviceToHost, c->stream));
+    if (loss) HIPCHK(hipMemcpyAsync(loss, l, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    sync_check(c);
     return DFR_OK;
   });
 }

@@ -24,7 +24,7 @@ EXPORTS = [
     "dfr_num_bodies", "dfr_adjoint_begin", "dfr_adjoint_backward", "dfr_adjoint_end", "dfr_adjoint_iterations",
     "dfr_set_fixed_iterations", "dfr_mlp_capture", "dfr_mlp_dataset_size", "dfr_mlp_dataset", "dfr_mlp_init_params",
     "dfr_mlp_set_params", "dfr_mlp_get_params", "dfr_mlp_train", "dfr_mlp_enable", "dfr_mlp_predict",
-    "dfr_mlp_loss_grad", "dfr_mlp_input_grad", "dfr_set_halo_capacity", "dfr_halo_stats",
+    "dfr_mlp_loss_grad", "dfr_mlp_loss_grad_noalloc", "dfr_mlp_input_grad", "dfr_set_halo_capacity", "dfr_halo_stats",
     "dfr_set_profiling_classes", "dfr_get_body", "dfr_set_gradient_mode", "dfr_count_1ring", "dfr_adjoint_start_gradient",
     "dfr_adjoint_segment_memory", "dfr_state_save", "dfr_state_load", "dfr_adjoint_arena_limit",
     "dfr_state_free", "dfr_adjoint_tape_bytes", "dfr_set_compute_gradient", "dfr_get_gradient_manager",
@@ -141,6 +141,7 @@ def load(path: str | None = None):
     L.dfr_mlp_enable.argtypes = [P, C.c_int]
     L.dfr_mlp_predict.argtypes = [P, C.c_int64, fp, fp]
     L.dfr_mlp_loss_grad.argtypes = [P, C.c_int64, fp, fp, fp, dp]
+    L.dfr_mlp_loss_grad_noalloc.argtypes = [P, C.c_int64, fp, fp, fp, dp]
     L.dfr_mlp_input_grad.argtypes = [P, C.c_int64, fp, fp, fp]
     _lib = L
     return L

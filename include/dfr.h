@@ -329,6 +329,21 @@ int dfr_mlp_enable(dfr_ctx* ctx, int on);
  * gradient of its mean over the n rows; dL/dfeatures for dL/dpredictions. */
 int dfr_mlp_predict(dfr_ctx* ctx, int64_t n, const float* feat, float* out);
 int dfr_mlp_loss_grad(dfr_ctx* ctx, int64_t n, const float* feat, const float* targ, float* grad, double* loss);
+
+/* Rows dfr_mlp_loss_grad_noalloc accepts per call. */
+#define DFR_NOALLOC_MAX_ROWS 65536
+
+/* dfr_mlp_loss_grad without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight training or serving loops, harnesses that forbid
+ * hipMalloc).  The feature/target staging and the gradient and loss
+ * accumulators live in one fixed device array per GPU (about 2.1 MB, taken at
+ * load and held until the process ends); n above DFR_NOALLOC_MAX_ROWS returns
+ * DFR_ERR_CAPACITY.  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (several threads or contexts) race.  Same results as
+ * dfr_mlp_loss_grad up to the order of the gradient's float atomics. */
+int dfr_mlp_loss_grad_noalloc(dfr_ctx* ctx, int64_t n, const float* feat, const float* targ, float* grad,
+                              double* loss);
 int dfr_mlp_input_grad(dfr_ctx* ctx, int64_t n, const float* feat, const float* ybar, float* fbar);
 
 /* Number of fluid particles / boundary samples / bodies. */

@@ -1845,16 +1845,39 @@ int dfr_set_field(dfr_ctx* c, const char* name, const float* in) {
   return guarded(c, [&]() {
     adjoint_release(c);
     const std::string n(name);
+    const int64_t nf = c->f.n;
+    std::vector<uint32_t> id(nf);
+    HIPCHK(hipMemcpy(id.data(), c->f.id, nf * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (n == "x" || n == "v") {  // 3 per particle; the fourth component is kept
+      float4* d4 = (n == "x") ? c->f.pos : c->f.vel;
+      std::vector<float4> b4(nf);
+      HIPCHK(hipMemcpy(b4.data(), d4, nf * sizeof(float4), hipMemcpyDeviceToHost));
+      for (int64_t k = 0; k < nf; ++k) {
+        const float* s = in + 3 * (size_t)id[k];
+        b4[k].x = s[0]; b4[k].y = s[1]; b4[k].z = s[2];
+      }
+      HIPCHK(hipMemcpy(d4, b4.data(), nf * sizeof(float4), hipMemcpyHostToDevice));
+      return DFR_OK;
+    }
     float* dst = nullptr;
     if (n == "kappa") dst = c->f.kappa;
     else if (n == "kappa_v") dst = c->f.kappav;
     else return fail(c, DFR_ERR_ARG, "unknown field");
-    const int64_t nf = c->f.n;
-    std::vector<uint32_t> id(nf);
-    HIPCHK(hipMemcpy(id.data(), c->f.id, nf * sizeof(uint32_t), hipMemcpyDeviceToHost));
     std::vector<float> buf(nf);
     for (int64_t k = 0; k < nf; ++k) buf[k] = in[id[k]];
     HIPCHK(hipMemcpy(dst, buf.data(), nf * sizeof(float), hipMemcpyHostToDevice));
+    return DFR_OK;
+  });
+}
+
+int dfr_set_time(dfr_ctx* c, double time, int64_t step_count) {
+  if (!c || !c->initialized || step_count < 0) return DFR_ERR_ARG;
+  return guarded(c, [&]() {
+    adjoint_release(c);
+    c->time = (float)time;
+    c->step_count = step_count;
+    c->stats.time = (double)c->time;
+    c->stats.step_count = step_count;
     return DFR_OK;
   });
 }

@@ -28,6 +28,7 @@ EXPORTS = [
     "dfr_set_profiling_classes", "dfr_get_body", "dfr_set_gradient_mode", "dfr_count_1ring", "dfr_adjoint_start_gradient",
     "dfr_adjoint_segment_memory", "dfr_state_save", "dfr_state_load", "dfr_adjoint_arena_limit",
     "dfr_state_free", "dfr_adjoint_tape_bytes", "dfr_set_compute_gradient", "dfr_get_gradient_manager",
+    "dfr_set_time",
 ]
 
 MLP_NP = 8898
@@ -89,6 +90,7 @@ def load(path: str | None = None):
     for f in ("dfr_get_rigid", "dfr_get_jacobians", "dfr_get_net_gradients", "dfr_set_rigid"):
         getattr(L, f).argtypes = [P, C.c_int, dp]
     L.dfr_reset_gradient.argtypes = [P]
+    L.dfr_set_time.argtypes = [P, C.c_double, C.c_int64]
     L.dfr_get_gradient_manager.argtypes = [P, C.c_int, C.c_int, dp, dp]
     L.dfr_get_fluid.argtypes = [P, fp, fp]
     L.dfr_get_field.argtypes = [P, C.c_char_p, fp]

@@ -19,10 +19,15 @@ can build:
     base.cleanup()
 
 What differs, and why:
-* `init(sceneFile=...)` takes a `difffr_amd.scenes.Scene` or the name of a
-  scene builder in `difffr_amd.scenes`; scene JSON files, state files and the
-  `--param` overrides are not read (NotImplementedError), nor is state
-  exported (`setValueBool(STATE_EXPORT, True)` raises).
+* `init(sceneFile=...)` takes a scene JSON file (read by `difffr_amd.sceneio`:
+  its fluid blocks, OBJ bodies sampled on their surface, and the step's
+  settings; CFL time stepping, surface tension, viscosity and emitters are
+  not run and are listed in `Scene.unsupported`), a `difffr_amd.scenes.Scene`,
+  or the name of a scene builder in `difffr_amd.scenes`.  State files the
+  reference wrote (`stateFile`, `loadFluidPos[AndVel]`,
+  `loadStateWithRigidExisted(path)`) are read; `--param` overrides are not
+  (NotImplementedError), nor is state exported (`setValueBool(STATE_EXPORT,
+  True)` raises).
 * There is no GUI: `GUI.Simulator_GUI_imgui(base)` constructs an inert object
   and `runSimulation` runs the headless loop of SimulatorBase::runSimulation
   (SimulatorBase.cpp:649-683).  With `useGui=False` a negative stopAt is an
@@ -39,11 +44,13 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
+import re
 from typing import Callable, Optional
 
 import numpy as np
 
 from . import native
+from . import sceneio as _sceneio
 from . import scenes as _scenes
 from .simulation import BoundaryModel as _CoreBoundaryModel
 from .simulation import DfrError
@@ -480,15 +487,23 @@ class _SimulatorBase:
         """SimulatorBase::init through py_init_simulator (SimulationModule.cpp:32-68)."""
         if isinstance(sceneFile, _scenes.Scene):
             scene = sceneFile
+        elif isinstance(sceneFile, str) and sceneFile.endswith(".json"):
+            if not os.path.isfile(sceneFile):
+                raise FileNotFoundError(sceneFile)
+            scene = _sceneio.load_scene(sceneFile)
         elif isinstance(sceneFile, str) and sceneFile and not os.path.splitext(sceneFile)[1] \
                 and callable(getattr(_scenes, sceneFile, None)):
             scene = getattr(_scenes, sceneFile)()
         else:
             raise NotImplementedError(
-                "scene files are not read by this library: pass a difffr_amd.scenes.Scene "
-                f"or the name of a builder in difffr_amd.scenes (got {sceneFile!r})")
-        if stateFile or loadFluidPos or loadFluidPosAndVel:
-            raise NotImplementedError("state files are not read by this library")
+                "expected a scene JSON file, a difffr_amd.scenes.Scene or the name of a builder in "
+                f"difffr_amd.scenes (got {sceneFile!r})")
+        if (loadFluidPos or loadFluidPosAndVel) and not stateFile:
+            raise ValueError("loadFluidPos / loadFluidPosAndVel need a stateFile")
+        if stateFile and not os.path.isfile(stateFile if stateFile.endswith(".bin") else stateFile + ".bin"):
+            raise FileNotFoundError(stateFile)
+        self._state_file = stateFile
+        self._load_fluid = "posvel" if loadFluidPosAndVel else ("pos" if loadFluidPos else "")
         if param:
             raise NotImplementedError("--param overrides are not supported; set scene.params instead")
         self._scene = scene
@@ -508,6 +523,51 @@ class _SimulatorBase:
         self._simulation = Simulation(self)
         self._manager = RigidBodyGradientManager(self)
         _current["base"] = self
+        if getattr(self, "_state_file", ""):
+            # SimulatorBase::initSimulation: loadFluidParticlePositions[AndVelocities] or loadState
+            if self._load_fluid:
+                self._apply_state_file(self._state_file, fluid_only=True,
+                                       velocities=self._load_fluid == "posvel")
+            else:
+                self._apply_state_file(self._state_file)
+
+    def _apply_state_file(self, path: str, fluid_only: bool = False, velocities: bool = True):
+        """Fluid fields from the state's BGEO file (SimulatorBase::readFluidParticlesState,
+        SimulatorBase.cpp:2545-2600: matched by particle id) and, unless
+        fluid_only, the dynamic bodies' x, q, v, omega and the time from the
+        .bin (SimulatorBase::loadStateWithRigidExisted, :2124-2290)."""
+        core = self._require()
+        bin_path = path if path.endswith(".bin") else path + ".bin"
+        dyn = [k for k, b in enumerate(self._scene.bodies) if b.dynamic]
+        st = _sceneio.read_state(bin_path, 0 if fluid_only else len(dyn))
+        if not st["fluids"]:
+            raise FileNotFoundError(f"no fluid BGEO file beside {bin_path}")
+        fl = next(iter(st["fluids"].values()))
+        n = fl["position"].shape[0]
+        if n != core.n:
+            raise ValueError(f"{bin_path}: {n} fluid particles, the scene has {core.n}")
+        ids = fl["id"].astype(np.int64) if "id" in fl else np.arange(n)
+        if not np.array_equal(np.sort(ids), np.arange(n)):
+            raise ValueError(f"{bin_path}: particle ids are not 0..{n - 1}")
+
+        def by_id(a):
+            out = np.empty_like(a)
+            out[ids] = a
+            return out
+
+        core.set_field("x", by_id(fl["position"]))
+        core.set_field("v", by_id(fl["velocity"]) if velocities and "velocity" in fl
+                       else np.zeros((n, 3), np.float32))
+        if fluid_only:
+            return
+        for name in ("kappa", "kappa_v"):
+            if name in fl:
+                core.set_field(name, by_id(fl[name]))
+        for k, r in zip(dyn, st["rigid"]):
+            core.set_rigid(k, r["x"], r["q"], r["v"], r["omega"])
+        if st["time"] is not None:
+            m = re.fullmatch(r"state_(\d+)", os.path.basename(bin_path[:-4]))
+            core.set_time(st["time"], int(m.group(1)) if m else core.get_step_count())
 
     def initSimulationWithDeferredInit(self):
         self.initSimulation()
@@ -593,15 +653,20 @@ class _SimulatorBase:
         fields, boundary samples, rigid x, q, v, omega, time and step count.
         Unlike the reference, which leaves them as the last step made them, the
         forward-mode rigid Jacobians are restored too, so a perturbed step
-        that is undone leaves no trace in the gradient chain.  State files on
-        disk are not read."""
+        that is undone leaves no trace in the gradient chain.  A state file
+        the reference wrote (".bin" + "_particle_<id>.bgeo") is read instead
+        when no kept state has that name: fluid fields, the dynamic bodies'
+        x, q, v, omega and, when its name carries it, the time."""
         core = self._require()
         name = stateFile[:-4] if stateFile.endswith(".bin") else stateFile
         name = os.path.normpath(name)
         if name not in self._states:
-            raise NotImplementedError(
-                "state files are not read by this library; only the last "
-                f"{self.MAX_KEPT_STATES} states kept by saveState can be loaded (got {stateFile!r})")
+            if os.path.isfile(name + ".bin"):  # a state file the reference wrote
+                self._apply_state_file(name + ".bin")
+                return
+            raise FileNotFoundError(
+                f"{stateFile!r} is neither a state file nor one of the last {self.MAX_KEPT_STATES} "
+                "states kept by saveState")
         slot = self._states.pop(name)
         self._states[name] = slot  # most recently used
         core.state_load(slot)

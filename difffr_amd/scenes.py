@@ -41,6 +41,8 @@ class Scene:
     fluid_v: np.ndarray         # (N, 3)
     bodies: List[Body]
     steps: int
+    # parts of a reference scene file this library does not run (difffr_amd.sceneio)
+    unsupported: List[str] = dataclasses.field(default_factory=list)
 
     @property
     def n_fluid(self) -> int:

@@ -206,8 +206,15 @@ class Simulation:
         return out.reshape(self.n, 3) if k == 3 else out
 
     def set_field(self, name: str, values: np.ndarray):
+        """"kappa", "kappa_v" ([n]) or "x", "v" ([n][3]), particle-id order."""
         a = np.ascontiguousarray(values, np.float32)
+        k = 3 if name in ("x", "v") else 1
+        if a.size != k * self.n:
+            raise ValueError(f"{name}: expected {k * self.n} values, got {a.size}")
         self._check(native.load().dfr_set_field(self.ctx, name.encode(), _fp(a)))
+
+    def set_time(self, time: float, step_count: int):
+        self._check(native.load().dfr_set_time(self.ctx, float(time), int(step_count)))
 
     def body_field(self, body: int, name: str) -> np.ndarray:
         n = self.body_sizes[body]

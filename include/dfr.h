@@ -190,8 +190,16 @@ int dfr_get_fluid(const dfr_ctx* ctx, float* x, float* v);
  * "density_adv", "kappa", "kappa_v" (n floats) or "sum_grad_p_k" (3n). */
 int dfr_get_field(const dfr_ctx* ctx, const char* name, float* out);
 
-/* Overwrites "kappa" or "kappa_v" (n floats, particle-id order). */
+/* Overwrites "kappa" or "kappa_v" (n floats) or "x" / "v" (3n floats: positions,
+ * velocities), in particle-id order; loading a saved state
+ * (SimulatorBase::loadStateWithRigidExisted, SimulatorBase.cpp:2124-2290, and
+ * readFluidParticlesState, :2545-2600). */
 int dfr_set_field(dfr_ctx* ctx, const char* name, const float* in);
+
+/* Simulated time and step count, as TimeManager::loadState restores them
+ * (TimeManager.cpp:63-67); the begin-step window and the trajectory end use
+ * them. */
+int dfr_set_time(dfr_ctx* ctx, double time, int64_t step_count);
 
 /* Boundary sample data of one body: "x", "v" (3n floats) or "volume" (n). */
 int dfr_get_boundary_field(const dfr_ctx* ctx, int body, const char* name, float* out);

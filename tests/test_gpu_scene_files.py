@@ -1,0 +1,91 @@
+"""A reference experiment through the pysplishsplash mirror on the GPU: the
+bottle-flip scene JSON (bottle.obj with fluid inside, a floor plate; surface
+sampled as the reference does) and the state the reference saved at
+t = 1.366 s (SimulatorBase::saveState: .bin + partio BGEO), loaded with
+init(stateFile=...), then three steps against the oracle started from the
+same loaded state.  The state is ill-conditioned for this sampling of the
+bottle (it was settled against the reference's own random Poisson samples):
+moving the positions by one FP32 ulp moves the fluid by ~6 um RMS within a
+step.  The GPU is held to three times that measured response (fluid RMS and
+rigid pose) and the rigid position to 1e-4 relative.  The scene's CFL
+step size and fluid viscosity are not run by this library (the step uses the
+fixed time step 1 ms on both sides)."""
+import os
+
+import numpy as np
+import pytest
+
+from difffr_amd import pysplishsplash as sph
+from difffr_amd import sceneio
+from oracle.pyoracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BOTTLE = os.path.join(HERE, "golden", "reference_files", "bottle_flip_robot")
+SCENE = os.path.join(BOTTLE, "scene", "diff-bottle-model-360degree.json")
+STATE = os.path.join(BOTTLE, "state", "state_1.365999.bin")
+
+
+def test_reference_scene_and_state_through_the_mirror(gpu_ready):
+    print(f"[scene file] {os.path.basename(SCENE)} + {os.path.basename(STATE)}", flush=True)
+    base = sph.Exec.SimulatorBase()
+    base.init(sceneFile=SCENE, useGui=False, stopAt=10.0, stateFile=STATE)
+    base.initSimulation()
+    sim = sph.Simulation.getCurrent()
+    core = base._core
+    sc = base._scene
+    assert sim.numberOfBoundaryModels() == 2 and core.n == 5904
+    assert "cflMethod" in sc.unsupported
+    st = sceneio.read_state(STATE, 1)
+    fl = st["fluids"]["Fluid"]
+    ids = fl["id"].astype(np.int64)
+    x_ref = np.empty_like(fl["position"]); x_ref[ids] = fl["position"]
+    v_ref = np.empty_like(fl["velocity"]); v_ref[ids] = fl["velocity"]
+    x, v = core.fluid()
+    np.testing.assert_array_equal(x, x_ref)
+    np.testing.assert_array_equal(v, v_ref)
+    bm = sim.getBoundaryModel(1)
+    np.testing.assert_allclose(bm.get_position_rb(), st["rigid"][0]["x"], atol=1e-7)
+    assert sph.TimeManager.getCurrent().getTime() == pytest.approx(1.365999, abs=1e-6)
+    # the oracle from the same loaded state, and a second one from positions one
+    # FP32 ulp away: this state responds to rounding-level changes by microns
+    # per step, so that response is the noise floor the GPU is held to
+    k = np.empty_like(fl["kappa"]); k[ids] = fl["kappa"]
+    kv = np.empty_like(fl["kappa_v"]); kv[ids] = fl["kappa_v"]
+    r = st["rigid"][0]
+    rng = np.random.default_rng(0)
+    away = np.where(rng.random(x_ref.shape) < 0.5, np.float32(np.inf), np.float32(-np.inf))
+
+    def oracle(x0):
+        o = OracleSim(sc, "f32_rigid64")
+        o.set_state("x", x0.astype(np.float64))
+        o.set_state("v", v_ref.astype(np.float64))
+        o.set_state("kappa", k.astype(np.float64))
+        o.set_state("kappa_v", kv.astype(np.float64))
+        o.set_state("rigid1", np.concatenate([r["x"], r["q"], r["v"], r["omega"]]))
+        o.set_state("time", [float(np.float32(1.365999))])
+        return o
+
+    o, o1 = oracle(x_ref), oracle(np.nextafter(x_ref, away))
+    rms = lambda a, b: float(np.sqrt((np.linalg.norm(a - b, axis=1) ** 2).mean()))
+    for s in range(3):
+        base.singleTimeStep()
+        it = core.stats()
+        for oo in (o, o1):
+            oo.set_fixed_iterations([it["iterations_v"]], [it["iterations"]])
+            oo.step(1)
+        g, ro, r1 = core.rigid(1), o.rigid(1), o1.rigid(1)
+        xg, _ = core.fluid()
+        xo, _ = o.fluid()
+        x1, _ = o1.fluid()
+        e, floor = rms(xg, xo), rms(x1, xo)
+        ex, fx = np.abs(g["x"] - ro["x"]).max(), np.abs(r1["x"] - ro["x"]).max()
+        eq, fq = np.abs(g["q"] - ro["q"]).max(), np.abs(r1["q"] - ro["q"]).max()
+        print(f"  step {s + 1}: iterations {it['iterations']}/{it['iterations_v']}; GPU vs oracle: fluid rms "
+              f"{e:.2e} m, rigid x {ex:.1e}, q {eq:.1e}; one-ulp floor: {floor:.2e} m, {fx:.1e}, {fq:.1e}",
+              flush=True)
+        assert e <= 3 * floor + 1e-8
+        assert ex <= 3 * fx + 1e-7 and eq <= 3 * fq + 1e-7
+        assert np.linalg.norm(g["x"] - ro["x"]) <= 1e-4 * np.linalg.norm(ro["x"])
+    base.cleanup()

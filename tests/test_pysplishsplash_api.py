@@ -97,10 +97,14 @@ def test_gui_is_construct_only():
 
 def test_unsupported_inputs_fail_loudly(tmp_path):
     base = sph.Exec.SimulatorBase()
-    with pytest.raises(NotImplementedError, match="scene files"):
+    with pytest.raises(FileNotFoundError):
         base.init(sceneFile=str(tmp_path / "scene.json"), useGui=False, stopAt=1.0)
-    with pytest.raises(NotImplementedError, match="state files"):
+    with pytest.raises(NotImplementedError, match="scene JSON file"):
+        base.init(sceneFile=str(tmp_path / "scene.xml"), useGui=False, stopAt=1.0)
+    with pytest.raises(FileNotFoundError):
         base.init(sceneFile="sphere_in_box", stateFile=str(tmp_path / "state.bin"))
+    with pytest.raises(ValueError):
+        base.init(sceneFile="sphere_in_box", loadFluidPos=True)
     with pytest.raises(NotImplementedError, match="state export"):
         base.setValueBool(base.STATE_EXPORT, True)
     base.setValueBool(base.STATE_EXPORT, False)

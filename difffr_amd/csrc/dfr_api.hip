@@ -525,14 +525,7 @@ void check_overflow(dfr_ctx* c) {
   }
 }
 
-double read_err(dfr_ctx* c, int slot) {
-  HIPCHK(hipMemcpyAsync(c->h_err + slot, c->d_err + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  check_overflow(c);
-  return c->h_err[slot];
-}
-
-void run_gradient(dfr_ctx* c, int mode) {
+void run_gradient(dfr_ctx* c, int mode, const int* stop) {
   if (!c->P.compute_gradient || c->replay) return;
   GradConst gc{};
   gc.mode = mode;
@@ -546,7 +539,7 @@ void run_gradient(dfr_ctx* c, int mode) {
     const BodyInfo& bi = c->bodies[k];
     if (!bi.dynamic || bi.animated || bi.n == 0) continue;
     Timed t(c, TC_GRADIENT);
-    launch_gradient(c->sc, gc, c->f, c->b, c->nb, c->d_rig, (int)bi.dyn_first, (int)bi.n, (int)k, c->stream);
+    launch_gradient(c->sc, gc, c->f, c->b, c->nb, c->d_rig, (int)bi.dyn_first, (int)bi.n, (int)k, c->stream, stop);
   }
 }
 
@@ -624,19 +617,6 @@ int run_solve(dfr_ctx* c, int solve, int fixed, int min_it, int max_it, double e
   if (fixed >= 0) {
     for (int it = 0; it < fixed; ++it) iter(it, nullptr);
     return fixed;
-  }
-  if (c->P.compute_gradient && !c->replay) {
-    bool chk = false;
-    int it = 0;
-    while ((!chk || it < min_it) && it < max_it) {
-      iter(it, nullptr);
-      launch_solve_check(c->d_part, c->nparts, err0 + std::min(it, max_it - 1), n, eta, it, min_it, max_it,
-                         nullptr, nullptr, c->stream);
-      avg = read_err(c, (int)(err0 - c->d_err) + std::min(it, max_it - 1)) / (double)n;
-      chk = avg <= eta;
-      ++it;
-    }
-    return it;
   }
   int* stop = c->d_stop + solve;
   SolveProbe* hp = c->h_probe + (size_t)solve * c->probe_cap;
@@ -779,7 +759,7 @@ void do_step(dfr_ctx* c) {
     itv = run_solve(
         c, 0, fixV, 1, maxv, eta, c->d_err, nf,
         [&](int m, const int* stop) {
-          run_gradient(c, 1);
+          run_gradient(c, 1, stop);
           const IterTapeOut to = rec ? tape_iteration(c, rec->vD, rec->kD, rec->bD, m) : IterTapeOut{};
           {
             Timed t(c, TC_VUPDATE);
@@ -824,7 +804,7 @@ void do_step(dfr_ctx* c) {
     it = run_solve(
         c, 1, fixP, c->P.min_iterations, maxp, eta, c->d_err + base, nf,
         [&](int m, const int* stop) {
-          run_gradient(c, 0);
+          run_gradient(c, 0, stop);
           const IterTapeOut to = rec ? tape_iteration(c, rec->vP, rec->kP, rec->bP, m) : IterTapeOut{};
           {
             Timed t(c, TC_VUPDATE);

@@ -169,7 +169,7 @@ struct GradConst {
   int optimize_rotation;
 };
 void launch_gradient(const StepConst& c, const GradConst& gc, FluidBufs& f, BoundaryBufs& b, NbrBufs& nb,
-                     RigidDev* rig, int k_first, int k_count, int body, hipStream_t s);
+                     RigidDev* rig, int k_first, int k_count, int body, hipStream_t s, const int* stop = nullptr);
 struct RigidStepConst {
   double dt;
   double g[3];

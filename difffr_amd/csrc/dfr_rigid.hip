@@ -19,7 +19,9 @@ k_gradient(StepConst c, GradConst gc, int k_first, int k_count, int body, const 
            int nd, const uint32_t* __restrict__ bf, const uint32_t* __restrict__ bfcnt, const float4* __restrict__ pos,
            const float4* __restrict__ vel, const float* __restrict__ badv, const float* __restrict__ alpha,
            const float4* __restrict__ s, const float4* __restrict__ bpos, const float4* __restrict__ bvel,
-           const float4* __restrict__ bx0, RigidDev* __restrict__ rig) {
+           const float4* __restrict__ bx0, RigidDev* __restrict__ rig, const int* __restrict__ stop) {
+  // an iteration enqueued after the solve converged (device-side loop) adds nothing
+  if (stop && *stop) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[78];
 #pragma unroll
@@ -391,11 +393,11 @@ __global__ void k_begin_step(BeginStepTable t, RigidDev* __restrict__ rig, int n
 
 // ---------------------------------------------------------------------------
 void launch_gradient(const StepConst& c, const GradConst& gc, FluidBufs& f, BoundaryBufs& b, NbrBufs& nb,
-                     RigidDev* rig, int k_first, int k_count, int body, hipStream_t s) {
+                     RigidDev* rig, int k_first, int k_count, int body, hipStream_t s, const int* stop) {
   if (k_count <= 0) return;
   const unsigned blocks = (unsigned)((k_count + DFR_BLOCK - 1) / DFR_BLOCK);
   hipLaunchKernelGGL(k_gradient, dim3(blocks), dim3(DFR_BLOCK), 0, s, c, gc, k_first, k_count, body, b.dyn_idx, b.nd,
-                     nb.bf, nb.bfcnt, f.pos, f.vel, f.badv, f.alpha, f.s, b.pos, b.vel, b.x0, rig);
+                     nb.bf, nb.bfcnt, f.pos, f.vel, f.badv, f.alpha, f.s, b.pos, b.vel, b.x0, rig, stop);
 }
 
 void launch_rigid_step(const RigidStepConst& rc, RigidDev* rig, hipStream_t s) {

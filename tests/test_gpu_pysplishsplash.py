@@ -244,7 +244,7 @@ def test_saved_state_and_per_step_differences(gpu_ready):
     # the reference-mode derivative leaves out the warm-start forces and the fluid's response, so only
     # its structure is compared: the same signs (a symmetric, drag-like response)
     assert np.array_equal(np.sign(fd), np.sign(Fv))
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(FileNotFoundError):
         base.loadStateWithRigidExisted("/nonexistent/state.bin")
     base.cleanup()
 
@@ -266,7 +266,7 @@ def test_saved_states_are_bounded(gpu_ready):
     base.loadStateWithRigidExisted(names[-1] + ".bin")
     assert sph.Simulation.getCurrent().getTimeStep().get_step_count() == cap + 3
     base.loadStateWithRigidExisted(names[4])  # still kept: the oldest four were freed
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(FileNotFoundError):
         base.loadStateWithRigidExisted(names[0])
     base.cleanup()
 

@@ -138,80 +138,26 @@ def cpu_info() -> dict:
             "cpus_allowed": allowed}
 
 
-def drop(sim, sc, seed: int):
-    """Places every dynamic body just above the fluid surface: seeded position
-    across the tank (bodies spread along x), rotation, velocity (downward
-    vertical component) and angular velocity."""
-    rng = np.random.default_rng(seed)
-    r = float(sc.params["radius"])
-    top = float(sc.fluid_x[:, 1].max())
-    walls = sc.bodies[0].x0
-    lo, hi = walls.min(0), walls.max(0)
-    dyn = [k for k, b in enumerate(sc.bodies) if b.dynamic]
-    for j, k in enumerate(dyn):
-        b = sc.bodies[k]
-        rad = float(np.linalg.norm(b.x0, axis=1).max())
-        margin = rad + 4 * r
-        x = lo[0] + (hi[0] - lo[0]) * (j + 0.5 + rng.uniform(-0.3, 0.3)) / len(dyn)
-        z = lo[2] + (hi[2] - lo[2]) * rng.uniform(0.3, 0.7)
-        pos = np.array([np.clip(x, lo[0] + margin, hi[0] - margin), top + rad + 2 * r,
-                        np.clip(z, lo[2] + margin, hi[2] - margin)])
-        q = rng.normal(size=4)
-        q /= np.linalg.norm(q)
-        v = np.array([rng.uniform(-2, 2), rng.uniform(-2, 0), rng.uniform(-2, 2)])
-        w = rng.uniform(-3, 3, size=3)
-        sim.set_rigid(k, pos, q, v, w)
-
-
 def start_window(sim, sc, drop_seed: int):
-    sim.state_load(STATE_SLOT)
-    drop(sim, sc, drop_seed)
+    from difffr_amd.warmstart import start_drop
+    start_drop(sim, sc, STATE_SLOT, drop_seed)
 
 
 def train_network(args, sim, sc):
     """Untimed: capture (features, converged kappa) pairs from rollouts of
     `train_drops` seeded drops from the settled state and fit the warm-start
-    network on the GPU, in `train_rounds` rounds.  Round 1 captures rollouts
-    warm-started the reference's way (halved previous kappa); each later round
-    captures rollouts warm-started by the network so far and continues its
-    training.  Every rollout's Jacobi solves iterate to the same tolerance, so
-    every target is the stiffness the solver converged to; the later rounds
-    teach the network the stiffness the solver settles on when it starts from
-    the network's own predictions (measured on the held-out drop: 3.1 pressure
-    iterations after one round, 2.45 after two, 2.5 and more after three or
-    more)."""
+    network on the GPU, in `train_rounds` rounds (difffr_amd.warmstart).  Round
+    1 captures rollouts warm-started the reference's way (halved previous
+    kappa); each later round captures rollouts warm-started by the network so
+    far and continues its training.  Every rollout's Jacobi solves iterate to
+    the same tolerance, so every target is the stiffness the solver converged
+    to (measured on the held-out drop: 3.1 pressure iterations after one round,
+    2.45 after two, 2.5 and more after three or more)."""
     import torch
-    per_drop = max(sc.n_fluid * args.train_steps, 1)
-    fraction = min(1.0, args.train_pairs / (args.train_drops * per_drop))
-    rounds = []
-    t_capture = t_train = 0.0
-    sim.mlp_enable(False)
-    for r in range(args.train_rounds):
-        t0 = time.perf_counter()
-        for d in range(args.train_drops):
-            start_window(sim, sc, 1 + d)
-            sim.mlp_capture(capacity=args.train_pairs, fraction=fraction, seed=r * 1000 + d)  # appends after drop 1
-            sim.step(args.train_steps)
-        sim.mlp_capture(0)
-        n_pairs = sim.mlp_dataset()[0].shape[0]
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        losses = sim.mlp_train(args.train_iters, batch=16384, lr=args.train_lr, seed=args.seed + r)
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        t_capture += t1 - t0
-        t_train += t2 - t1
-        rounds.append({"capture_warm_start": "reference" if r == 0 else "network of the previous round",
-                       "pairs": int(n_pairs), "loss_first": float(losses[:50].mean()),
-                       "loss_last": float(losses[-50:].mean())})
-        sim.mlp_enable(True)
-    sim.mlp_enable(False)
-    P, norm = sim.mlp_get_params()
-    info = {"training_drops": f"seeds 1..{args.train_drops}", "steps_per_drop": args.train_steps,
-            "capture_fraction": fraction, "adam_iterations_per_round": args.train_iters,
-            "learning_rate": args.train_lr, "batch": 16384, "rounds": rounds,
-            "capture_s": t_capture, "train_s": t_train}
-    return P, norm, info
+    from difffr_amd.warmstart import train_on_drops
+    return train_on_drops(sim, sc, STATE_SLOT, drops=args.train_drops, steps=args.train_steps,
+                          rounds=args.train_rounds, iters=args.train_iters, pairs=args.train_pairs,
+                          lr=args.train_lr, seed=args.seed, sync=torch.cuda.synchronize)
 
 
 def timed_run(args, sim, sc, eval_seed: int, learned: bool, P=None, norm=None):

@@ -381,15 +381,21 @@ __global__ void __launch_bounds__(64) k_mlp(MlpArgs a) {
 // Register-chained network: inference and the input gradient dL/d(features)
 // for dL/d(predictions) (used by the exact adjoint through the learned warm
 // start).  A workgroup stages the K-permuted fragments (22 forward, + 22
-// transposed for the gradient) and the biases in LDS once; each wavefront then
-// carries 16-particle tiles through every layer in registers: the D tiles of a
-// layer, rounded to bf16, are the next layer's B operand as they stand.  The
-// gradient keeps silu'(z) of the three hidden layers (D layout, FP32) from the
-// tile's forward sweep, so nothing is recomputed.
+// transposed for the gradient) and the biases in LDS once and its wavefronts
+// then loop over groups of T 16-particle tiles (grid-stride): each fragment
+// read from LDS (ds_read_b128) feeds the MFMAs of all T tiles, which gives
+// the MFMA pipe T independent accumulators per fragment.  A tile is carried
+// through every layer in registers: the D tiles of a layer, rounded to bf16,
+// are the next layer's B operand as they stand.  The gradient keeps silu'(z)
+// of the three hidden layers (D layout, FP32) from the tile's forward sweep,
+// so nothing is recomputed.
 // ---------------------------------------------------------------------------
-// chain workgroup: 64 particles per wavefront; the gradient's workgroup is
-// twice as large, so eight wavefronts share one staged copy of its 44 fragments
+// the gradient's workgroup is twice as large, so eight wavefronts share one
+// staged copy of its 44 fragments
 template <bool IG> constexpr int chain_block() { return IG ? 512 : 256; }
+// 16-particle tiles per group (measured on the 1M tank: two or four tiles per
+// fragment read ran slower than one, the extra accumulators costing occupancy)
+template <bool IG> constexpr int chain_tiles() { return 1; }
 constexpr int CHAIN_BIAS = 3 * 64 + 16;        // biases of layers 0..3 (output layer zero-padded to a tile)
 template <bool IG> struct ChainLds {
   static constexpr int NF = IG ? MLP_NFRAG : MLP_FT[3];  // chain fragments staged (forward ones first)
@@ -407,9 +413,9 @@ __device__ inline bf16x8 pack2(f32x4 lo, f32x4 hi) {
 }
 
 template <bool IG>
-__global__ void __launch_bounds__(chain_block<IG>()) __attribute__((amdgpu_waves_per_eu(IG ? 4 : 1, 8)))
-k_mlp_chain(MlpArgs a) {
+__global__ void __launch_bounds__(chain_block<IG>()) k_mlp_chain(MlpArgs a) {
   constexpr int CB = chain_block<IG>();
+  constexpr int T = chain_tiles<IG>();
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   using L = ChainLds<IG>;
   bf16x8* F = reinterpret_cast<bf16x8*>(smem + L::FRAG);
@@ -423,121 +429,159 @@ k_mlp_chain(MlpArgs a) {
   __syncthreads();
   const int l = threadIdx.x & 63;
   const int lr = l & 15, lg = l >> 4;
-  const int p0 = (blockIdx.x * (CB / 64) + (threadIdx.x >> 6)) * 64;
-  if (p0 >= a.n) return;  // no barrier follows
   const MlpNorm& N = a.norm;
+  const int groups = ceil_div(a.n, 16 * T);
+  const int stride = gridDim.x * (CB / 64);
 #pragma unroll 1
-  for (int nt = 0; nt < 4; ++nt) {
-    // Fragments and biases are re-read from LDS for every tile: the empty asm
-    // hides the loop invariance, so they are not hoisted into registers.
-    const bf16x8* Ft = F;
-    const float* bt = bias;
-    asm volatile("" : "+v"(Ft), "+v"(bt));
-    auto frag = [&](int f) { return Ft[f * 64 + l]; };
+  for (int grp = blockIdx.x * (CB / 64) + (threadIdx.x >> 6); grp < groups; grp += stride) {
+    // Fragments and biases are re-read from LDS for every group: the empty asm
+    // hides the loop invariance of `o`, so they are not hoisted into registers
+    // (the index stays an LDS address: ds_read_b128 with the fragment offset).
+    int o = l;
+    asm volatile("" : "+v"(o));
+    auto frag = [&](int f) { return F[f * 64 + o]; };
     auto lbias = [&](int layer, int mt) {
-      return *reinterpret_cast<const f32x4*>(bt + CHAIN_OB[layer] + 16 * mt + 4 * lg);
+      return *reinterpret_cast<const f32x4*>(bias + (o - l) + CHAIN_OB[layer] + 16 * mt + 4 * lg);
     };
-    const int p = p0 + 16 * nt + lr;
-    const bool valid = p < a.n;
-    // ---- input: lane group 0 holds features 0..3, group 1 features 4, 5 (chain_unit(0, g, j))
-    float x[6];
-    if (valid) {
-      if (a.feat) {
-        for (int k = 0; k < 6; ++k) x[k] = a.feat[(size_t)p * 6 + k];
-      } else {
-        features_from_fields(a.fields, (uint32_t)p, x);
-      }
-      for (int k = 0; k < 6; ++k) x[k] = (x[k] - N.mu_in[k]) * N.is_in[k];
-    } else {
-      for (int k = 0; k < 6; ++k) x[k] = 0.f;
-    }
-    bf16x8 B[2];
+    int p[T];
+    bool valid[T];
+    bf16x8 B[T][2];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const float v = (j < 4 && lg == 0) ? x[j] : (j < 2 && lg == 1) ? x[4 + j] : 0.f;
-      B[0][j] = (__bf16)v;
-      B[1][j] = (__bf16)0.f;
+    for (int t = 0; t < T; ++t) {
+      p[t] = grp * 16 * T + 16 * t + lr;
+      valid[t] = p[t] < a.n;
+      // ---- input: lane group 0 holds features 0..3, group 1 features 4, 5 (chain_unit(0, g, j))
+      float x[6];
+      if (valid[t]) {
+        if (a.feat) {
+          for (int k = 0; k < 6; ++k) x[k] = a.feat[(size_t)p[t] * 6 + k];
+        } else {
+          features_from_fields(a.fields, (uint32_t)p[t], x);
+        }
+        for (int k = 0; k < 6; ++k) x[k] = (x[k] - N.mu_in[k]) * N.is_in[k];
+      } else {
+        for (int k = 0; k < 6; ++k) x[k] = 0.f;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const float v = (j < 4 && lg == 0) ? x[j] : (j < 2 && lg == 1) ? x[4 + j] : 0.f;
+        B[t][0][j] = (__bf16)v;
+        B[t][1][j] = (__bf16)0.f;
+      }
     }
     // ---- hidden layers 0..2: z = W h + b, h' = silu(z)
-    f32x4 sp[3][4];  // silu'(z) per hidden layer (IG only)
+    f32x4 sp[T][3][4];  // silu'(z) per hidden layer (IG only)
 #pragma unroll
     for (int layer = 0; layer < 3; ++layer) {
       const int kt = layer == 0 ? 1 : 2;
-      f32x4 h[4];
+      f32x4 h[T][4];
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt) {
-        f32x4 acc = lbias(layer, mt);
+        const f32x4 b0 = lbias(layer, mt);
+#pragma unroll
+        for (int t = 0; t < T; ++t) h[t][mt] = b0;
 #pragma unroll
         for (int kc = 0; kc < 2; ++kc)
-          if (kc < kt) acc = mfma(frag(MLP_FW[layer] + mt * kt + kc), B[kc], acc);
+          if (kc < kt) {
+            const bf16x8 A = frag(MLP_FW[layer] + mt * kt + kc);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const float z = acc[j];
-          const float sg = sigmoid_fast(z);
-          h[mt][j] = z * sg;
-          if (IG) sp[layer][mt][j] = sg * (1.f + z * (1.f - sg));
-        }
+            for (int t = 0; t < T; ++t) h[t][mt] = mfma(A, B[t][kc], h[t][mt]);
+          }
       }
-      B[0] = pack2(h[0], h[1]);
-      B[1] = pack2(h[2], h[3]);
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const float z = h[t][mt][j];
+            const float sg = sigmoid_fast(z);
+            const float hz = z * sg;
+            h[t][mt][j] = hz;
+            if (IG) sp[t][layer][mt][j] = fmaf(hz, 1.f - sg, sg);  // silu'(z) = s + z s (1 - s)
+          }
+        B[t][0] = pack2(h[t][0], h[t][1]);
+        B[t][1] = pack2(h[t][2], h[t][3]);
+      }
     }
     // ---- output layer (2 units, rows 0 and 1 of lane group 0)
-    f32x4 y = lbias(3, 0);
-    y = mfma(frag(MLP_FW[3] + 0), B[0], y);
-    y = mfma(frag(MLP_FW[3] + 1), B[1], y);
+    f32x4 y[T];
+    {
+      const f32x4 b3 = lbias(3, 0);
+      const bf16x8 A0 = frag(MLP_FW[3] + 0), A1 = frag(MLP_FW[3] + 1);
+#pragma unroll
+      for (int t = 0; t < T; ++t) y[t] = mfma(A1, B[t][1], mfma(A0, B[t][0], b3));
+    }
     if (!IG) {
-      if (valid && lg == 0) {
-        const float yv = y[0] * N.sd_out[0] + N.mu_out[0], yp = y[1] * N.sd_out[1] + N.mu_out[1];
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        if (!valid[t] || lg != 0) continue;
+        const float yv = y[t][0] * N.sd_out[0] + N.mu_out[0], yp = y[t][1] * N.sd_out[1] + N.mu_out[1];
         if (a.out) {
-          a.out[(size_t)p * 2 + 0] = yv;
-          a.out[(size_t)p * 2 + 1] = yp;
+          a.out[(size_t)p[t] * 2 + 0] = yv;
+          a.out[(size_t)p[t] * 2 + 1] = yp;
         } else {
           // the learned warm starts: kappa_v = y_v / dt where d > 0 (warmstartDivergenceSolve's
           // mask; no divergence warm start: kappa_v untouched), kappa prediction kept for the
           // pressure warm start (applied where rho* > 1)
-          if (a.dmask) a.kappav[p] = (a.dmask[p] > 0.f) ? yv * a.inv_dt : 0.f;
-          a.kpred[p] = yp;
+          if (a.dmask) a.kappav[p[t]] = (a.dmask[p[t]] > 0.f) ? yv * a.inv_dt : 0.f;
+          a.kpred[p[t]] = yp;
         }
       }
       continue;
     }
     // ---- backward: dZ_3 = ybar * sd_out (normalised units), then dH = W^T dZ, dZ = dH * silu'(z)
-    float g0 = 0.f, g1 = 0.f;
-    if (valid && lg == 0) {
-      g0 = a.ybar[(size_t)p * 2 + 0] * N.sd_out[0];
-      g1 = a.ybar[(size_t)p * 2 + 1] * N.sd_out[1];
-    }
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      B[0][j] = (__bf16)(j == 0 ? g0 : j == 1 ? g1 : 0.f);
-      B[1][j] = (__bf16)0.f;
+    for (int t = 0; t < T; ++t) {
+      float g0 = 0.f, g1 = 0.f;
+      if (valid[t] && lg == 0) {
+        g0 = a.ybar[(size_t)p[t] * 2 + 0] * N.sd_out[0];
+        g1 = a.ybar[(size_t)p[t] * 2 + 1] * N.sd_out[1];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        B[t][0][j] = (__bf16)(j == 0 ? g0 : j == 1 ? g1 : 0.f);
+        B[t][1][j] = (__bf16)0.f;
+      }
     }
 #pragma unroll
     for (int layer = 3; layer >= 1; --layer) {
       const int kt = layer == 3 ? 1 : 2;
-      f32x4 dz[4];
+      f32x4 dz[T][4];
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < T; ++t) dz[t][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kc = 0; kc < 2; ++kc)
-          if (kc < kt) acc = mfma(frag(MLP_FT[layer] + mt * kt + kc), B[kc], acc);
+          if (kc < kt) {
+            const bf16x8 A = frag(MLP_FT[layer] + mt * kt + kc);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) dz[mt][j] = acc[j] * sp[layer - 1][mt][j];
+            for (int t = 0; t < T; ++t) dz[t][mt] = mfma(A, B[t][kc], dz[t][mt]);
+          }
       }
-      B[0] = pack2(dz[0], dz[1]);
-      B[1] = pack2(dz[2], dz[3]);
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) dz[t][mt][j] *= sp[t][layer - 1][mt][j];
+        B[t][0] = pack2(dz[t][0], dz[t][1]);
+        B[t][1] = pack2(dz[t][2], dz[t][3]);
+      }
     }
     // dH_0 = W_1^T dZ_1 (features 0..3 in lane group 0, 4..5 in group 1): dL/df = dL/dx * is
-    f32x4 din = {0.f, 0.f, 0.f, 0.f};
-    din = mfma(frag(MLP_FT[0] + 0), B[0], din);
-    din = mfma(frag(MLP_FT[0] + 1), B[1], din);
-    if (valid && lg < 2) {
+    const bf16x8 A0 = frag(MLP_FT[0] + 0), A1 = frag(MLP_FT[0] + 1);
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const f32x4 din = mfma(A1, B[t][1], mfma(A0, B[t][0], f32x4{0.f, 0.f, 0.f, 0.f}));
+      if (!valid[t] || lg >= 2) continue;
+      const int pt = p[t];
       if (a.fbar) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int k = 4 * lg + j;
-          if (k < 6) a.fbar[(size_t)p * 6 + k] = din[j] * N.is_in[k];
+          if (k < 6) a.fbar[(size_t)pt * 6 + k] = din[j] * N.is_in[k];
         }
       } else {
         // the feature adjoints: rho/rho0 gets fbar[0], the divergence pass at v0 gets
@@ -546,12 +590,12 @@ k_mlp_chain(MlpArgs a) {
         // them); lane group 0 holds features 0..3, group 1 features 4 and 5
         const LearnedAdjSink& S = a.sink;
         if (lg == 0) {
-          S.rhob[p] += din[0] * N.is_in[0];
-          S.dbar[p] = S.dt * (din[1] * N.is_in[1]);
-          S.rbb[p] = din[3] * N.is_in[3];
+          S.rhob[pt] += din[0] * N.is_in[0];
+          S.dbar[pt] = S.dt * (din[1] * N.is_in[1]);
+          S.rbb[pt] = din[3] * N.is_in[3];
         } else {
-          S.kb[p] = din[0] * N.is_in[4] + (S.keep_kappa ? S.kb[p] : 0.f);
-          S.kvb[p] = din[1] * N.is_in[5] + (S.keep_kappav ? S.kvb[p] : 0.f);
+          S.kb[pt] = din[0] * N.is_in[4] + (S.keep_kappa ? S.kb[pt] : 0.f);
+          S.kvb[pt] = din[1] * N.is_in[5] + (S.keep_kappav ? S.kvb[pt] : 0.f);
         }
       }
     }
@@ -628,8 +672,22 @@ void launch_mlp_pack(const float* P, void* frag, hipStream_t s) {
 }
 
 size_t mlp_frag_bytes() { return sizeof(bf16x8) * (size_t)MLP_NFRAG_ALL * 64; }
+// Grid of the chained network: enough wavefronts for every group of T tiles,
+// at most as many workgroups as the device holds at once (each stages its
+// fragments once and loops over groups).
 template <bool IG> static inline unsigned nb_chain(int n) {
-  return (unsigned)((n + chain_block<IG>() - 1) / chain_block<IG>());
+  static long resident = 0;
+  if (resident == 0) {
+    int dev = 0, cus = 256, per_cu = 0;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mlp_chain<IG>, chain_block<IG>(),
+                                                     ChainLds<IG>::BYTES) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    resident = (long)per_cu * cus;
+  }
+  const long groups = (n + 16 * chain_tiles<IG>() - 1) / (16 * chain_tiles<IG>());
+  const long need = (groups + chain_block<IG>() / 64 - 1) / (chain_block<IG>() / 64);
+  return (unsigned)std::max(1L, std::min(need, resident));
 }
 
 static MlpArgs base_args(const void* frag, const float* P, const MlpNorm& norm, int n) {

@@ -37,6 +37,13 @@ struct PackWriter {
     ring[(n & (RING - 1)) * BB + t] = (uint16_t)s;
     ++n;
   }
+  // Branch-free push: the slot is written at the next free entry either way
+  // (an entry past the accepted ones, which the next push overwrites; at most
+  // 11 entries are pending, so it never holds one) and counted if accepted.
+  __device__ inline void push_if(uint32_t s, bool accept) {
+    ring[(n & (RING - 1)) * BB + t] = (uint16_t)s;
+    n += accept ? 1u : 0u;
+  }
   __device__ inline void store(uint4* __restrict__ nbr8, size_t stride, uint32_t i, uint32_t count) {
     uint32_t w[4];
 #pragma unroll
@@ -160,7 +167,7 @@ k_build_tiled(const float4* __restrict__ pos, int nf, GridDesc g, const uint32_t
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const uint32_t s = s0 + u;
-            if (s < r1 && dist2_exact(p.x, p.y, p.z, qx[u], qy[u], qz[u]) < h2) pw.push(s);
+            pw.push_if(s, (s < r1) & (dist2_exact(p.x, p.y, p.z, qx[u], qy[u], qz[u]) < h2));
           }
           pw.flush_full(nbr8, stride, i);
         }

@@ -47,6 +47,11 @@ SETTLE_STEPS = 300      # plain steps from the lattice before the drop (part of 
 STATE_SLOT = 0          # device slot of the settled state
 
 
+def progress(msg: str):
+    """One line per finished phase on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,7 +162,7 @@ def train_network(args, sim, sc):
     from difffr_amd.warmstart import train_on_drops
     return train_on_drops(sim, sc, STATE_SLOT, drops=args.train_drops, steps=args.train_steps,
                           rounds=args.train_rounds, iters=args.train_iters, pairs=args.train_pairs,
-                          lr=args.train_lr, seed=args.seed, sync=torch.cuda.synchronize)
+                          lr=args.train_lr, seed=args.seed, sync=torch.cuda.synchronize, log=progress)
 
 
 def timed_run(args, sim, sc, eval_seed: int, learned: bool, P=None, norm=None):
@@ -333,15 +338,20 @@ def main():
     sim = Simulation(sc, params_override={"compute_gradient": 0})
     sim.step(SETTLE_STEPS)
     sim.state_save(STATE_SLOT)
+    progress(f"{sc.name}: {n} fluid particles settled ({SETTLE_STEPS} steps)")
     plain = timed_run(args, sim, sc, eval_seed, False)
+    progress(f"plain warm start timed: {plain['value'] / 1e6:.1f}M particle-steps/s")
     learned_on = not args.no_learned
     if learned_on:
         P, norm, tinfo = train_network(args, sim, sc)
+        progress(f"network trained ({len(tinfo['rounds'])} rounds, {tinfo['capture_s'] + tinfo['train_s']:.1f} s)")
         learned = timed_run(args, sim, sc, eval_seed, True, P, norm)
+        progress(f"learned warm start timed: {learned['value'] / 1e6:.1f}M particle-steps/s")
         main_run = learned
     else:
         main_run = plain
     refmode = reference_mode_run(args, sim, sc, eval_seed)
+    progress("reference-mode gradient timed")
     wsc = wstate = None
     if not args.no_cpu_baseline:
         start_window(sim, sc, eval_seed)

@@ -49,18 +49,19 @@ def start_drop(sim, sc, slot: int, seed: int):
 
 def train_on_drops(sim, sc, slot: int, drops: int = 64, steps: int = 25, rounds: int = 2,
                    iters: int = 3000, pairs: int = 4_000_000, lr: float = 2e-3, batch: int = 16384,
-                   seed: int = 0, sync=None):
+                   seed: int = 0, sync=None, log=None):
     """Captures pairs from `drops` seeded drops (seeds 1..drops, `steps` steps
     each) from the state in `slot` and fits the network, `rounds` times (round
     1 warm-started the reference's way, each later round by the network so
     far; the parameters carry over between rounds).  `pairs` bounds the
     dataset: each step keeps a seeded fraction of its particles.  Leaves the
     network disabled and returns (params, norm, info); `sync` (e.g.
-    torch.cuda.synchronize) makes the reported capture/train times exact."""
+    torch.cuda.synchronize) makes the reported capture/train times exact;
+    `log(message)` is called after each round."""
     sync = sync or (lambda: None)
     per_drop = max(sc.n_fluid * steps, 1)
     fraction = min(1.0, pairs / (drops * per_drop))
-    log = []
+    rounds_info = []
     t_capture = t_train = 0.0
     sim.mlp_enable(False)
     for r in range(rounds):
@@ -78,13 +79,16 @@ def train_on_drops(sim, sc, slot: int, drops: int = 64, steps: int = 25, rounds:
         t2 = time.perf_counter()
         t_capture += t1 - t0
         t_train += t2 - t1
-        log.append({"capture_warm_start": "reference" if r == 0 else "network of the previous round",
-                    "pairs": int(n_pairs), "loss_first": float(losses[:50].mean()),
-                    "loss_last": float(losses[-50:].mean())})
+        rounds_info.append({"capture_warm_start": "reference" if r == 0 else "network of the previous round",
+                            "pairs": int(n_pairs), "loss_first": float(losses[:50].mean()),
+                            "loss_last": float(losses[-50:].mean())})
+        if log:
+            log(f"training round {r + 1}: {n_pairs} pairs, loss {rounds_info[-1]['loss_first']:.3f} -> "
+                f"{rounds_info[-1]['loss_last']:.3f}")
         sim.mlp_enable(True)
     sim.mlp_enable(False)
     P, norm = sim.mlp_get_params()
     info = {"training_drops": f"seeds 1..{drops}", "steps_per_drop": steps, "capture_fraction": fraction,
-            "adam_iterations_per_round": iters, "learning_rate": lr, "batch": batch, "rounds": log,
+            "adam_iterations_per_round": iters, "learning_rate": lr, "batch": batch, "rounds": rounds_info,
             "capture_s": t_capture, "train_s": t_train}
     return P, norm, info

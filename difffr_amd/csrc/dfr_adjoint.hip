@@ -597,7 +597,6 @@ __global__ void k_adj_rigid(RigidStepConst rc, AdjStepTable tb, const RigidDev* 
 }
 
 // ---------------------------------------------------------------------------
-static inline unsigned nbricks(const GridDesc& g) { return (unsigned)(g.bn[0] * g.bn[1] * g.bn[2]); }
 static inline unsigned nblk(size_t n) { return (unsigned)((n + DFR_BLOCK - 1) / DFR_BLOCK); }
 // dynamic LDS of the reverse passes: two 16-byte records per staged halo slot
 static inline size_t shm_slots(const AdjPass& a, int floats) { return sizeof(float) * (size_t)a.cap * floats; }

@@ -54,6 +54,7 @@ struct Snapshot {
   uint32_t* id = nullptr;
   RigidDev* rig = nullptr;
   float time = 0.f;
+  float dt = 0.f;  // step size the next step starts with (changes with the CFL rule)
   int64_t step_count = 0;
   std::vector<uint8_t> animated;
 };
@@ -206,7 +207,11 @@ struct dfr_ctx {
   bool initialized = false;
 
   // derived constants
-  float h = 0, dtf = 0;
+  float h = 0, dtf = 0;       // dtf: the TimeManager's step size (the next step's start)
+  float4* acc = nullptr;      // non-pressure accelerations (g + forces) of the current step
+  float4* nrm = nullptr;      // Akinci 2013 normals
+  uint32_t* d_cflmax = nullptr;  // CFL squared-speed bound (float bits)
+  ForceConst fc{};
   StepConst sc{};
   GridDesc grid{};
 
@@ -665,6 +670,15 @@ int imposed(dfr_ctx* c, bool pressure) {
   return c->fixed_pos < v.size() ? v[c->fixed_pos] : -1;
 }
 
+// The TimeManager's step size: what the next step starts with (divergence
+// solve, non-pressure forces, advection, positions, time) and, within a step
+// after the CFL rule, what the pressure solve and the rigid bodies use.
+void set_step_size(dfr_ctx* c, float dt) {
+  c->dtf = dt;
+  c->sc.dt = dt;
+  c->stats.dt = dt;
+}
+
 void do_step(dfr_ctx* c) {
   const float dt = c->dtf;
   const int nf = c->f.n;
@@ -773,14 +787,43 @@ void do_step(dfr_ctx* c) {
         rec ? &tv : nullptr, avgv);
   }
   {
+    // non-pressure forces and the step size (TimeStepDiffDFSPH.cpp:581-586): the
+    // accelerations use the step size of the step's start, which also advects the
+    // velocities (:588-605) and the positions and advances the time; the CFL rule
+    // sets the one of the pressure solve, the rigid bodies and the next step
     Timed t(c, TC_ELEMWISE);
-    if (nf > 0) launch_advect(c->sc, c->f, (divOn && warmV) ? dt : 1.f, c->stream);
+    const bool forces = c->acc != nullptr;
+    float h_new = dt;
+    if (forces && nf > 0) {
+      if (c->nrm) launch_normals_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->nrm, c->stream);
+      launch_forces_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->nrm, c->acc, c->fc, c->stream);
+    }
+    if (c->d_cflmax) {
+      launch_cfl_max(c->sc, c->f.vel, forces ? c->acc : nullptr, c->b.n, c->b.vel, c->d_cflmax, c->stream);
+      uint32_t bits = 0;
+      HIPCHK(hipMemcpyAsync(&bits, c->d_cflmax, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      float maxVel;
+      std::memcpy(&maxVel, &bits, sizeof(float));
+      // Simulation::updateTimeStepSizeCFL (Simulation.cpp:608-612), in the reference's Real = float
+      const float diameter = 2.f * (float)c->P.radius;
+      float hn = (float)c->P.cfl_factor * 0.4f * (diameter / std::sqrt(maxVel));
+      hn = std::min(hn, (float)c->P.cfl_max_time_step_size);
+      hn = std::max(hn, (float)c->P.cfl_min_time_step_size);
+      h_new = hn;
+    }
+    if (nf > 0) {
+      if (forces) launch_advect_acc(c->sc, c->f, c->acc, (divOn && warmV) ? dt : 1.f, c->stream);
+      else launch_advect(c->sc, c->f, (divOn && warmV) ? dt : 1.f, c->stream);
+    }
+    if (h_new != dt) set_step_size(c, h_new);
   }
+  const float dtp = c->dtf;  // step size of the pressure solve and the rigid bodies
   // ---- pressure solve (TimeStepDiffDFSPH::pressureSolve, :654-768)
   int it = 0;
   double avg = 0.0;
   if (nf > 0) {
-    const float invH2 = 1.f / (dt * dt);
+    const float invH2 = 1.f / (dtp * dtp);
     if (warmP) {
       {
         Timed t(c, TC_PWARM);
@@ -817,7 +860,9 @@ void do_step(dfr_ctx* c) {
         },
         rec ? &tp : nullptr, avg);
     Timed t(c, TC_ELEMWISE);
-    launch_position(c->sc, c->f, warmP ? dt * dt : 1.f, c->stream);
+    StepConst sco = c->sc;  // positions advance with the step's start size (:613-634)
+    sco.dt = dt;
+    launch_position(sco, c->f, warmP ? dtp * dtp : 1.f, c->stream);
     if (ml.capturing)  // the converged stiffness of this step is the training target
       launch_mlp_capture_targets(c->f.kappa, c->f.kappav, ml.rows, ml.count + 1, (uint32_t)nf, ml.count,
                                  (uint32_t)ml.cap, ml.targ, c->stream);
@@ -827,7 +872,7 @@ void do_step(dfr_ctx* c) {
     // backwardPerStep (:493-524) + velocityTimeStep + positionTimeStep
     Timed t(c, TC_RIGID);
     RigidStepConst rc{};
-    rc.dt = (double)dt;
+    rc.dt = (double)dtp;
     for (int k = 0; k < 3; ++k) rc.g[k] = (double)(float)c->P.gravity[k];
     rc.gradient_mode = c->P.gradient_mode;
     rc.optimize_rotation = c->P.optimize_rotation;
@@ -909,6 +954,7 @@ void snapshot_save(dfr_ctx* c, Snapshot& S) {
   bt.copy(S.rig, c->d_rig, c->bodies.size());
   bt.flush();
   S.time = c->time;
+  S.dt = c->dtf;
   S.step_count = c->step_count;
   S.animated.resize(c->bodies.size());
   for (size_t k = 0; k < c->bodies.size(); ++k) S.animated[k] = c->bodies[k].animated ? 1 : 0;
@@ -925,6 +971,7 @@ void snapshot_restore(dfr_ctx* c, const Snapshot& S) {
   bt.copy(c->d_rig, (const RigidDev*)S.rig, c->bodies.size());
   bt.flush();
   c->time = S.time;
+  set_step_size(c, S.dt);
   c->step_count = S.step_count;
   for (size_t k = 0; k < c->bodies.size(); ++k) c->bodies[k].animated = S.animated[k] != 0;
 }
@@ -1404,10 +1451,27 @@ void dfr_default_params(dfr_params* p, double radius, double dt) {
   p->max_boundary_neighbors = 128;
   p->release_mode = 0;
   p->use_gradient_manager = 0;
+  p->surface_tension_method = 0;        // SurfaceTensionBase.cpp:12-13
+  p->surface_tension = 0.05;
+  p->surface_tension_boundary = 0.01;
+  p->viscosity_method = 0;              // ViscosityBase.cpp:12, Viscosity_XSPH.cpp:17
+  p->viscosity = 0.01;
+  p->viscosity_boundary = 0.0;
+  p->cfl_method = 0;                    // Simulation.cpp:56-60
+  p->cfl_factor = 0.5;
+  p->cfl_min_time_step_size = 0.0001;
+  p->cfl_max_time_step_size = 0.005;
 }
 
 dfr_ctx* dfr_create(const dfr_params* p, int device) {
   if (!p || p->radius <= 0.0 || p->dt <= 0.0) return nullptr;
+  // built: surface tension 0/2 (none, Akinci 2013), viscosity 0/1/2 (none, Standard, XSPH) without boundary
+  // viscosity, step size 0/1 (fixed, CFL)
+  if ((p->surface_tension_method != 0 && p->surface_tension_method != 2) ||
+      p->viscosity_method < 0 || p->viscosity_method > 2 || p->viscosity_boundary != 0.0 ||
+      (p->cfl_method != 0 && p->cfl_method != 1) ||
+      (p->cfl_method == 1 && !(p->cfl_min_time_step_size > 0.0 && p->cfl_max_time_step_size >= p->cfl_min_time_step_size)))
+    return nullptr;
   dfr_ctx* c = new dfr_ctx();
   c->P = *p;
   if (c->P.max_neighbors <= 0) c->P.max_neighbors = 64;
@@ -1431,12 +1495,24 @@ dfr_ctx* dfr_create(const dfr_params* p, int device) {
   c->sc.K.l_ih = c->sc.K.l / c->h;
   c->sc.K.l_ih2 = c->sc.K.l / (c->h * c->h);
   c->sc.dt = c->dtf;
+  c->stats.dt = c->dtf;
   c->sc.rho0 = (float)p->rho0;
   const float diam = 2.f * (float)p->radius;
   c->sc.V_f = 0.8f * diam * diam * diam;  // FluidModel.cpp:255-275
   c->sc.m_f = c->sc.V_f * c->sc.rho0;
   for (int k = 0; k < 3; ++k) c->sc.g[k] = (float)p->gravity[k];
   c->sc.eps = 1e-5f;  // TimeStepDiffDFSPH.h:26
+  // non-pressure forces: Akinci 2013 cohesion / adhesion kernels of radius h
+  // (CohesionKernel::setRadius, AdhesionKernel::setRadius, SPHKernels.h:446-521)
+  c->fc.h = c->h;
+  c->fc.st_on = p->surface_tension_method == 2 ? 1 : 0;
+  c->fc.st_k = (float)p->surface_tension;
+  c->fc.st_kb = (float)p->surface_tension_boundary;
+  c->fc.visc_mode = p->viscosity_method;
+  c->fc.visc = (float)p->viscosity;
+  c->fc.ck = 32.f / ((float)M_PI * std::pow(c->h, 9.f));
+  c->fc.cc = std::pow(c->h, 6.f) / 64.f;
+  c->fc.ak = 0.007f / std::pow(c->h, 3.25f);
   return c;
 }
 
@@ -1569,6 +1645,12 @@ int dfr_init(dfr_ctx* c) {
     c->f.id = dalloc<uint32_t>(c, NF); c->f.id2 = dalloc<uint32_t>(c, NF);
     c->f.rho = dalloc<float>(c, NF); c->f.alpha = dalloc<float>(c, NF);
     c->f.s = dalloc<float4>(c, NF); c->f.badv = dalloc<float>(c, NF); c->f.kf = dalloc<float>(c, NF);
+    if (c->P.surface_tension_method == 2 || c->P.viscosity_method != 0) {
+      c->acc = dalloc<float4>(c, NF);
+      if (c->P.surface_tension_method == 2) c->nrm = dalloc<float4>(c, NF);
+      set_forces_kernel_attributes();
+    }
+    if (c->P.cfl_method == 1) c->d_cflmax = dalloc<uint32_t>(c, 1);
     const size_t NB = (size_t)std::max<int64_t>(off, 1);
     c->b.pos = dalloc<float4>(c, NB); c->b.vel = dalloc<float4>(c, NB); c->b.x0 = dalloc<float4>(c, NB);
     c->b.dyn_idx = dalloc<uint32_t>(c, std::max<int64_t>(nd, 1));
@@ -1687,6 +1769,7 @@ int dfr_reset(dfr_ctx* c) {
     if (c->f.n > 0) upload_fluid_initial(c);
     place_boundary_and_volumes(c);
     c->time = 0.f;
+    set_step_size(c, (float)c->P.dt);  // SimulatorBase::reset (SimulatorBase.cpp:908-910)
     c->step_count = 0;
     c->stats.time = 0.0;
     c->stats.step_count = 0;
@@ -1813,6 +1896,8 @@ int dfr_get_field(const dfr_ctx* c, const char* name, float* out) {
     else if (n == "kappa") download_unpermute<float>(c, c->f.kappa, 1, 1, out);
     else if (n == "kappa_v") download_unpermute<float>(c, c->f.kappav, 1, 1, out);
     else if (n == "sum_grad_p_k") download_unpermute<float4>(c, c->f.s, 4, 3, out);
+    else if (n == "accel" && c->acc) download_unpermute<float4>(c, c->acc, 4, 3, out);
+    else if (n == "normal" && c->nrm) download_unpermute<float4>(c, c->nrm, 4, 3, out);
     else return DFR_ERR_ARG;
   } catch (const HipError&) {
     return DFR_ERR_HIP;
@@ -2043,6 +2128,10 @@ int dfr_kernel_times(const dfr_ctx* c, int n, char names[][32], double* avg_ms, 
 int dfr_adjoint_begin(dfr_ctx* c, int checkpoint_every) {
   return guarded(c, [&]() {
     if (!c->initialized) return fail(c, DFR_ERR_STATE, "dfr_adjoint_begin before dfr_init");
+    if (c->P.surface_tension_method != 0 || c->P.viscosity_method != 0 || c->P.cfl_method != 0)
+      return fail(c, DFR_ERR_STATE,
+                  "the exact adjoint does not cover non-pressure forces or CFL step sizes (surface_tension_method, "
+                  "viscosity_method and cfl_method must be 0)");
     HIPCHK(hipSetDevice(c->device));
     adjoint_release(c);
     adjoint_buffers(c);

@@ -96,6 +96,24 @@ void launch_sort_boundary(BoundaryBufs& b, const GridDesc& g, GridBufs& gb, hipS
 void launch_build_bf(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
                      hipStream_t s);
 
+// dfr_forces.hip: non-pressure forces and the CFL bound
+struct ForceConst {
+  float h;              // support radius (the cohesion and adhesion kernels' radius)
+  float st_k, st_kb;    // surface tension (cohesion, curvature) and boundary adhesion coefficients
+  float visc;           // viscosity coefficient
+  int st_on, visc_mode; // visc_mode: 0 none, 1 Standard, 2 XSPH
+  float ck, cc;         // cohesion kernel: 32 / (pi h^9), h^6 / 64
+  float ak;             // adhesion kernel: 0.007 / h^3.25
+};
+void launch_normals_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
+                      NbrBufs& nb, float4* nrm, hipStream_t s);
+void launch_forces_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
+                     NbrBufs& nb, const float4* nrm, float4* acc, const ForceConst& fc, hipStream_t s);
+void launch_cfl_max(const StepConst& c, const float4* vel, const float4* acc, int nb, const float4* bvel,
+                    uint32_t* out, hipStream_t s);
+void launch_advect_acc(const StepConst& c, FluidBufs& f, const float4* acc, float kappav_scale, hipStream_t s);
+void set_forces_kernel_attributes();
+
 // dfr_tiled.hip
 void launch_build_tiled(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
                         hipStream_t s);

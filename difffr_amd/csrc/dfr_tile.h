@@ -112,6 +112,18 @@ struct PassArgs {
   int cap;  // halo slots staged in LDS (larger halos read global memory)
 };
 
+inline unsigned nbricks(const GridDesc& g) { return (unsigned)(g.bn[0] * g.bn[1] * g.bn[2]); }
+
+inline PassArgs make_args(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
+                          NbrBufs& nb) {
+  PassArgs a;
+  a.c = c; a.g = g; a.fstart = gb.fstart; a.pos = f.pos; a.vel = f.vel; a.bpos = b.pos; a.bvel = b.vel;
+  a.bx0 = b.x0; a.nbr8 = nb.nbr8; a.nbrB = nb.nbrB; a.cnt = nb.cnt; a.nb_static = b.n - b.nd;
+  a.thdr = nb.thdr; a.tmap = nb.tmap;
+  a.cap = nb.cap;
+  return a;
+}
+
 // A pass's view of its brick.  The build leaves, per brick, a header (own
 // particle range, halo total) and the halo's slot -> particle map, so a pass
 // starts with one header load and coalesced map loads; only a halo beyond the

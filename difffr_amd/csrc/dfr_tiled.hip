@@ -578,17 +578,6 @@ k_expand_lists(GridDesc g, const uint32_t* __restrict__ fstart, int nf, const ui
 }
 
 // ---------------------------------------------------------------------------
-static inline unsigned nbricks(const GridDesc& g) { return (unsigned)(g.bn[0] * g.bn[1] * g.bn[2]); }
-
-static PassArgs make_args(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
-                          NbrBufs& nb) {
-  PassArgs a;
-  a.c = c; a.g = g; a.fstart = gb.fstart; a.pos = f.pos; a.vel = f.vel; a.bpos = b.pos; a.bvel = b.vel;
-  a.bx0 = b.x0; a.nbr8 = nb.nbr8; a.nbrB = nb.nbrB; a.cnt = nb.cnt; a.nb_static = b.n - b.nd;
-  a.thdr = nb.thdr; a.tmap = nb.tmap;
-  a.cap = nb.cap;
-  return a;
-}
 
 void launch_build_tiled(FluidBufs& f, BoundaryBufs& b, const GridDesc& g, GridBufs& gb, NbrBufs& nb, float h2,
                         hipStream_t s) {

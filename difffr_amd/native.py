@@ -45,6 +45,10 @@ class Params(C.Structure):
         ("uniform_acc_time", C.c_double), ("target_time", C.c_double),
         ("max_neighbors", C.c_int32), ("max_boundary_neighbors", C.c_int32), ("release_mode", C.c_int32),
         ("use_gradient_manager", C.c_int32),
+        ("surface_tension_method", C.c_int32), ("surface_tension", C.c_double),
+        ("surface_tension_boundary", C.c_double), ("viscosity_method", C.c_int32), ("viscosity", C.c_double),
+        ("viscosity_boundary", C.c_double), ("cfl_method", C.c_int32), ("cfl_factor", C.c_double),
+        ("cfl_min_time_step_size", C.c_double), ("cfl_max_time_step_size", C.c_double),
     ]
 
 
@@ -53,7 +57,7 @@ class Stats(C.Structure):
         ("iterations", C.c_int32), ("iterations_v", C.c_int32), ("avg_density_err", C.c_double),
         ("avg_div_err", C.c_double), ("time", C.c_double), ("step_count", C.c_int64),
         ("total_iterations", C.c_int64), ("total_iterations_v", C.c_int64), ("steps_since_reset", C.c_int64),
-        ("trajectory_finished", C.c_int32),
+        ("trajectory_finished", C.c_int32), ("dt", C.c_double),
     ]
 
 

@@ -452,7 +452,8 @@ class TimeManager:
         return float(_core().stats()["time"])
 
     def getTimeStepSize(self) -> float:
-        return float(_core().params["dt"])
+        """The step size the next step starts with (the CFL rule changes it)."""
+        return float(_core().stats()["dt"])
 
 
 # ---------------------------------------------------------------------------

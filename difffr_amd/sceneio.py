@@ -6,10 +6,12 @@ description (host side, numpy).
   uniformAccelerateRBTime, gradientMode, useRigidGradientManager,
   useReleaseRigidBodyMode and the solver settings), "RigidBodies" (mesh,
   pose, scale, density, dynamic/wall flags, initial and target state) and
-  "FluidBlocks" / "FluidModels".  Keys of parts this library does not run
-  (CFL time stepping, surface tension, viscosity, emitters, articulated
-  systems, rigid contact) are listed in `Scene.unsupported`; the step then
-  runs with the fixed `timeStepSize`.
+  "FluidBlocks" / "FluidModels", the CFL step-size keys and the fluid's
+  "Materials" entry (surface tension, viscosity; without one the fluid keeps
+  the reference's default Standard viscosity 0.01).  Keys of parts this
+  library does not run (other force methods, boundary viscosity, the
+  iteration-count step-size rule, emitters, articulated systems, rigid
+  contact) are listed in `Scene.unsupported`.
 * Fluid blocks are laid out by SimulatorBase::createFluidBlocks
   (Simulator/SimulatorBase.cpp:1638-1722, dense modes 0, 1 and 2).
 * Rigid bodies are OBJ meshes (scaled as loadObj does,
@@ -53,10 +55,21 @@ _CONFIG_DEFAULTS = {
     "enableDivergenceSolver": True, "targetTime": 0.8, "uniformAccelerateRBTime": 0.0, "gradientMode": 1,
     "useRigidGradientManager": False, "useReleaseRigidBodyMode": False, "useRigidContactSolver": False,
     "sim2D": False, "cflMethod": 1, "boundaryHandlingMethod": 0, "simulationMethod": 4,
+    "cflFactor": 0.5, "cflMinTimeStepSize": 0.0001, "cflMaxTimeStepSize": 0.005,
+}
+# Fluid parameters a "Materials" entry with the fluid's id may set (read by
+# SceneLoader::readParameterObject into the FluidModel), at the FluidModel's
+# defaults: FluidModel::init sets the Standard viscosity (FluidModel.cpp:96-101,
+# ViscosityBase.cpp:12), no surface tension (FluidModel.cpp:52,
+# SurfaceTensionBase.cpp:12-13).
+_MATERIAL_DEFAULTS = {
+    "viscosityMethod": 1, "viscosity": 0.01, "viscosityBoundary": 0.0,
+    "surfaceTensionMethod": 0, "surfaceTension": 0.05, "surfaceTensionBoundary": 0.01,
+    "vorticityMethod": 0, "dragMethod": 0, "elasticityMethod": 0,
 }
 # Keys read but not run: the run differs from the reference's while they are on.
 _UNSUPPORTED_IF = {
-    "cflMethod": lambda v: int(v) != 0,              # CFL step size (Simulation.cpp:524-616)
+    "cflMethod": lambda v: int(v) not in (0, 1),     # 2: iteration-count step-size control (Simulation.cpp:528-539)
     "useRigidContactSolver": bool,                   # InterlinkedSPH/RigidContactSolver.cpp
     "sim2D": bool,
     "boundaryHandlingMethod": lambda v: int(v) != 0,  # Akinci2012 only
@@ -163,12 +176,19 @@ def read_scene_file(path: str) -> SceneFile:
                                      mode=int(fb.get("denseMode", 0)), fluid_id=str(fb.get("id", "Fluid")),
                                      initial_velocity=_vec(fb.get("initialVelocity", [0, 0, 0]), 3),
                                      initial_angular_velocity=_vec(fb.get("initialAngularVelocity", [0, 0, 0]), 3)))
-        if "viscosityMethod" in fb:
-            unsupported.append("FluidBlocks.viscosityMethod")
+        # keys such as "viscosityMethod" inside a block are not read by the
+        # reference's loader (SceneLoader.cpp:303-360): fluid parameters come
+        # from "Materials"
     materials = list(d.get("Materials", []))
     for m in materials:
-        for key in ("surfaceTensionMethod", "viscosityMethod", "vorticityMethod", "dragMethod", "elasticityMethod"):
-            if int(m.get(key, 0)) != 0:
+        mat = dict(_MATERIAL_DEFAULTS)
+        mat.update(m)
+        if int(mat["surfaceTensionMethod"]) not in (0, 2):
+            unsupported.append("Materials.surfaceTensionMethod")
+        if int(mat["viscosityMethod"]) not in (0, 1, 2) or float(mat["viscosityBoundary"]) != 0.0:
+            unsupported.append("Materials.viscosityMethod")
+        for key in ("vorticityMethod", "dragMethod", "elasticityMethod"):
+            if int(mat[key]) != 0:
                 unsupported.append(f"Materials.{key}")
     for key in ("Emitters", "ArticulatedSystems", "AnimationFields"):
         if d.get(key):
@@ -374,6 +394,21 @@ def body_samples(rb: RigidBodyData, radius: float, scene_dir: str, seed: int = 0
     return regular_triangle_sampling(V, F, 1.5 * radius)
 
 
+def fluid_material(sf: SceneFile, fluid_id: str = "Fluid") -> dict:
+    """Non-pressure force parameters of the fluid `fluid_id`: the "Materials"
+    entry with that id over the FluidModel's defaults (Standard viscosity
+    0.01, no surface tension)."""
+    mat = dict(_MATERIAL_DEFAULTS)
+    for m in sf.materials:
+        if str(m.get("id", "Fluid")) == fluid_id:
+            mat.update(m)
+    return dict(viscosity_method=int(mat["viscosityMethod"]), viscosity=float(mat["viscosity"]),
+                viscosity_boundary=float(mat["viscosityBoundary"]),
+                surface_tension_method=int(mat["surfaceTensionMethod"]),
+                surface_tension=float(mat["surfaceTension"]),
+                surface_tension_boundary=float(mat["surfaceTensionBoundary"]))
+
+
 def build_scene(sf: SceneFile, steps: Optional[int] = None, seed: int = 0) -> scenes.Scene:
     """A `scenes.Scene` from a parsed scene file.  Body 0 of the reference's
     drivers is the static tank and body 1 the optimised body; the order of the
@@ -389,7 +424,11 @@ def build_scene(sf: SceneFile, steps: Optional[int] = None, seed: int = 0) -> sc
              gradient_mode=int(cfg["gradientMode"]), target_time=float(cfg["targetTime"]),
              uniform_acc_time=float(cfg["uniformAccelerateRBTime"]),
              use_gradient_manager=int(bool(cfg["useRigidGradientManager"])),
-             release_mode=int(bool(cfg["useReleaseRigidBodyMode"])))
+             release_mode=int(bool(cfg["useReleaseRigidBodyMode"])),
+             cfl_method=int(cfg["cflMethod"]), cfl_factor=float(cfg["cflFactor"]),
+             cfl_min_time_step_size=float(cfg["cflMinTimeStepSize"]),
+             cfl_max_time_step_size=float(cfg["cflMaxTimeStepSize"]))
+    p.update(fluid_material(sf))
     scene_dir = os.path.dirname(sf.path)
     xs, vs = [], []
     for blk in sf.fluid_blocks:

@@ -63,6 +63,11 @@ def default_params(radius: float, dt: float = 1e-3) -> dict:
         enable_divergence_solver=1, gradient_mode=1, optimize_rotation=1, gyroscopic=1,
         compute_gradient=1, uniform_acc_time=0.0, target_time=1e30, release_mode=0,
         use_gradient_manager=0,
+        # non-pressure forces and step size (SurfaceTensionBase.cpp:12-13, ViscosityBase.cpp:12,
+        # Viscosity_XSPH.cpp:17, Simulation.cpp:56-60): off unless a scene enables them
+        surface_tension_method=0, surface_tension=0.05, surface_tension_boundary=0.01,
+        viscosity_method=0, viscosity=0.01, viscosity_boundary=0.0,
+        cfl_method=0, cfl_factor=0.5, cfl_min_time_step_size=0.0001, cfl_max_time_step_size=0.005,
     )
 
 

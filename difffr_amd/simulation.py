@@ -39,13 +39,15 @@ def make_params(p: dict) -> native.Params:
     L = native.load()
     q = native.Params()
     L.dfr_default_params(C.byref(q), float(p["radius"]), float(p["dt"]))
-    for k in ("rho0", "max_error", "max_error_v", "uniform_acc_time", "target_time"):
+    for k in ("rho0", "max_error", "max_error_v", "uniform_acc_time", "target_time", "surface_tension",
+              "surface_tension_boundary", "viscosity", "viscosity_boundary", "cfl_factor",
+              "cfl_min_time_step_size", "cfl_max_time_step_size"):
         if k in p:
             setattr(q, k, float(p[k]))
     for k in ("min_iterations", "max_iterations", "max_iterations_v", "use_pressure_warmstart",
               "use_divergence_warmstart", "enable_divergence_solver", "gradient_mode", "optimize_rotation",
               "gyroscopic", "compute_gradient", "max_neighbors", "max_boundary_neighbors", "release_mode",
-              "use_gradient_manager"):
+              "use_gradient_manager", "surface_tension_method", "viscosity_method", "cfl_method"):
         if k in p:
             setattr(q, k, int(p[k]))
     if "gravity" in p:
@@ -200,7 +202,10 @@ class Simulation:
         return x, v
 
     def field(self, name: str) -> np.ndarray:
-        k = 3 if name == "sum_grad_p_k" else 1
+        """Per-particle fields in particle-id order: density, factor, density_adv,
+        kappa, kappa_v; (n, 3): sum_grad_p_k, and with non-pressure forces on, the
+        last step's accelerations (accel = g + forces) and Akinci 2013 normals."""
+        k = 3 if name in ("sum_grad_p_k", "accel", "normal") else 1
         out = np.empty(self.n * k, np.float32)
         self._check(native.load().dfr_get_field(self.ctx, name.encode(), _fp(out)))
         return out.reshape(self.n, 3) if k == 3 else out

@@ -66,6 +66,20 @@ typedef struct dfr_params {
   int32_t use_gradient_manager;  /* useRigidGradientManager (Simulation.cpp:123): the per-step sums go to
                                     the RigidBodyGradientManager chain (dfr_get_gradient_manager) instead
                                     of perform_chain_rule; dfr_get_jacobians then stays at its reset value */
+  /* Non-pressure forces after the divergence solve (Simulation::computeNonPressureForces,
+     Simulation.cpp:618-631; method numbers as registered in NonPressureForceRegistration.cpp:33-60).
+     The exact adjoint (dfr_adjoint_begin) does not cover them, nor CFL step sizes. */
+  int32_t surface_tension_method;  /* 0 none, 2 Akinci et al. 2013 (SurfaceTension_Akinci2013.cpp) */
+  double surface_tension;          /* cohesion and curvature coefficient (default 0.05) */
+  double surface_tension_boundary; /* adhesion coefficient (default 0.01) */
+  int32_t viscosity_method;        /* 0 none, 1 Standard (Viscosity_Standard.cpp; the reference fluid's default,
+                                      FluidModel.cpp:100), 2 XSPH (Viscosity_XSPH.cpp) */
+  double viscosity;                /* default 0.01 */
+  double viscosity_boundary;       /* 0 only: boundary viscosity (which pushes the bodies) is not built */
+  int32_t cfl_method;              /* 0 fixed dt, 1 CFL (Simulation::updateTimeStepSizeCFL, :544-616) */
+  double cfl_factor;               /* default 0.5 */
+  double cfl_min_time_step_size;   /* default 1e-4 */
+  double cfl_max_time_step_size;   /* default 5e-3 */
 } dfr_params;
 
 typedef struct dfr_stats {
@@ -79,6 +93,7 @@ typedef struct dfr_stats {
   int64_t total_iterations_v;
   int64_t steps_since_reset;
   int32_t trajectory_finished;   /* TimeStepDiffDFSPH::is_trajectory_finish_callback */
+  double dt;                     /* step size the next step starts with (changes only with cfl_method 1) */
 } dfr_stats;
 
 /* Fills p with the reference defaults for a given particle radius and dt. */
@@ -187,7 +202,9 @@ int dfr_reset_gradient(dfr_ctx* ctx);
 int dfr_get_fluid(const dfr_ctx* ctx, float* x, float* v);
 
 /* A per-particle field in particle-id order: "density", "factor",
- * "density_adv", "kappa", "kappa_v" (n floats) or "sum_grad_p_k" (3n). */
+ * "density_adv", "kappa", "kappa_v" (n floats) or "sum_grad_p_k" (3n); with
+ * non-pressure forces on, "accel" (the last step's FluidModel accelerations,
+ * g + forces, 3n) and, with surface tension, "normal" (Akinci 2013, 3n). */
 int dfr_get_field(const dfr_ctx* ctx, const char* name, float* out);
 
 /* Overwrites "kappa" or "kappa_v" (n floats) or "x" / "v" (3n floats: positions,

@@ -52,6 +52,58 @@ struct Params {
   double targetTime = 1e30;
   int releaseMode = 0;           // useReleaseRigidBodyMode, Simulation.h:444
   int useGradientManager = 0;    // useRigidGradientManager, Simulation.cpp:123
+  // Non-pressure forces (Simulation::computeNonPressureForces, Simulation.cpp:618-631;
+  // method indices in registration order, NonPressureForceRegistration.cpp:33-60)
+  int surfaceTensionMethod = 0;  // 0 none, 2 Akinci et al. 2013
+  double surfaceTension = 0.05;  // SurfaceTensionBase.cpp:12
+  double surfaceTensionBoundary = 0.01;
+  int viscosityMethod = 0;       // 0 none, 1 Standard (FluidModel.cpp:100 makes it a fluid's default), 2 XSPH
+  double viscosity = 0.01;       // ViscosityBase.cpp:12
+  double viscosityBoundary = 0.0;  // Viscosity_XSPH.cpp:17
+  // Time step size (Simulation::updateTimeStepSize, Simulation.cpp:524-616): 0 fixed, 1 CFL
+  int cflMethod = 0;
+  double cflFactor = 0.5;        // Simulation.cpp:56-60
+  double cflMinTimeStepSize = 0.0001;
+  double cflMaxTimeStepSize = 0.005;
+};
+
+// Akinci et al. 2013 cohesion kernel (SPHKernels.h:437-505) and adhesion
+// kernel (SPHKernels.h:508-560), both with the support radius as radius.
+template <typename T> struct Cohesion {
+  T radius, k, c;
+  void set(T r) {
+    radius = r;
+    k = T(32.0) / (T(M_PI) * std::pow(radius, T(9.0)));
+    c = std::pow(radius, T(6.0)) / T(64.0);
+  }
+  T W(const V3<T>& r) const {
+    T res = 0;
+    const T r2 = r.squaredNorm();
+    if (r2 <= radius * radius) {
+      const T r1 = std::sqrt(r2);
+      const T r3 = r2 * r1;
+      if (r1 > T(0.5) * radius) res = k * std::pow(radius - r1, T(3.0)) * r3;
+      else res = k * T(2.0) * std::pow(radius - r1, T(3.0)) * r3 - c;
+    }
+    return res;
+  }
+};
+template <typename T> struct Adhesion {
+  T radius, k;
+  void set(T r) {
+    radius = r;
+    k = T(0.007) / std::pow(radius, T(3.25));
+  }
+  T W(const V3<T>& r) const {
+    T res = 0;
+    const T r2 = r.squaredNorm();
+    if (r2 <= radius * radius) {
+      const T rl = std::sqrt(r2);
+      if (rl > T(0.5) * radius)
+        res = k * std::pow(-T(4.0) * r2 / radius + T(6.0) * rl - T(2.0) * radius, T(0.25));
+    }
+    return res;
+  }
 };
 
 // Eval of the analytic cubic spline kernel: SPlisHSPlasH/SPHKernels.h:16-176.
@@ -241,6 +293,8 @@ template <typename RF, typename RR> class Sim {
  public:
   Params P;
   Cubic<RF> K;
+  Cohesion<RF> KC;
+  Adhesion<RF> KA;
   RF h, dt, rho0, V_f, m_f;
   V3<RF> g;
   int nthreads = 1;
@@ -248,6 +302,7 @@ template <typename RF, typename RR> class Sim {
   int stepCount = 0;
   // fluid (reference AoS layout, FluidModel.h:110-151)
   std::vector<V3<RF>> x, v, s;
+  std::vector<V3<RF>> accel, normal;  // FluidModel accelerations, Akinci 2013 normals
   std::vector<RF> rho, alpha, bAdv, kappa, kappaV;
   std::vector<Body<RF, RR>> bodies;
   // neighbour lists per point-set pair
@@ -271,6 +326,8 @@ template <typename RF, typename RR> class Sim {
   explicit Sim(const Params& p) : P(p) {
     h = RF(4.0) * RF(P.radius);  // Simulation.cpp:195-216, support radius = 4r
     K.set(h);
+    KC.set(h);  // Simulation.cpp:397-398
+    KA.set(h);
     dt = RF(P.dt);
     rho0 = RF(P.rho0);
     const RF diam = RF(2.0) * RF(P.radius);
@@ -1054,6 +1111,106 @@ template <typename RF, typename RR> class Sim {
     update_boundary_particles(false);
   }
 
+  // clearAccelerations (a = g) + Simulation::computeNonPressureForces
+  // (Simulation.cpp:618-631: surface tension, then viscosity) at the step's
+  // densities and the velocities after the divergence solve.  Boundary
+  // viscosity (which would push the bodies) is not part of this restatement.
+  void compute_nonpressure() {
+    const long N = long(x.size());
+    accel.assign(N, g);
+    if (P.surfaceTensionMethod == 2) surface_tension_akinci2013();
+    if (P.viscosityMethod == 1) viscosity_standard();
+    if (P.viscosityMethod == 2) viscosity_xsph();
+  }
+  // Viscosity_Standard::step (Viscosity_Standard.cpp:233-269), fluid part (3D: d = 10):
+  // a_i += d nu (m_j / rho_j) ((v_i - v_j) . r_ij) / (|r_ij|^2 + 0.01 h^2) gradW_ij,
+  // the scalar formed in double as the reference's 0.01 literal makes it.
+  void viscosity_standard() {
+    const long N = long(x.size());
+    const RF d = RF(10.0), nu = RF(P.viscosity);
+    const RF h2 = h * h;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      V3<RF>& ai = accel[i];
+      for (uint32_t j : nFF[i]) {
+        const V3<RF> xixj = x[i] - x[j];
+        const double s = double(d * nu * (m_f / rho[j]) * (v[i] - v[j]).dot(xixj)) /
+                         (double(xixj.squaredNorm()) + 0.01 * double(h2));
+        ai += K.gradW(xixj) * RF(s);
+      }
+    }
+  }
+  // SurfaceTension_Akinci2013::computeNormals / step (SurfaceTension_Akinci2013.cpp:26-147).
+  void surface_tension_akinci2013() {
+    const long N = long(x.size());
+    normal.assign(N, V3<RF>());
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      V3<RF> ni;
+      for (uint32_t j : nFF[i]) ni += K.gradW(x[i] - x[j]) * (m_f / rho[j]);
+      normal[i] = ni * h;
+    }
+    const RF k = RF(P.surfaceTension), kb = RF(P.surfaceTensionBoundary);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      V3<RF>& ai = accel[i];
+      for (uint32_t j : nFF[i]) {
+        const RF K_ij = RF(2.0) * rho0 / (rho[i] + rho[j]);
+        V3<RF> a;
+        V3<RF> xixj = x[i] - x[j];
+        const RF length2 = xixj.squaredNorm();
+        if (length2 > RF(1.0e-9)) {
+          xixj = xixj * (RF(1.0) / std::sqrt(length2));
+          a -= xixj * (k * m_f) * KC.W(x[i] - x[j]);
+        }
+        a -= (normal[i] - normal[j]) * k;
+        ai += a * K_ij;
+      }
+      for (size_t bi = 0; bi < bodies.size(); ++bi)
+        for (uint32_t j : nFB[bi][i]) {
+          V3<RF> xixj = x[i] - bodies[bi].x[j];
+          const RF length2 = xixj.squaredNorm();
+          if (length2 > RF(1.0e-9)) {
+            xixj = xixj * (RF(1.0) / std::sqrt(length2));
+            ai -= xixj * (kb * rho0 * bodies[bi].V[j]) * KA.W(x[i] - bodies[bi].x[j]);
+          }
+        }
+    }
+  }
+  // Viscosity_XSPH::step (Viscosity_XSPH.cpp:35-108), fluid part.
+  void viscosity_xsph() {
+    const long N = long(x.size());
+    const RF invH = RF(1.0) / dt;
+    const RF nu = RF(P.viscosity);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      V3<RF>& ai = accel[i];
+      for (uint32_t j : nFF[i]) ai -= (v[i] - v[j]) * (invH * nu * (m_f / rho[j])) * K.W(x[i] - x[j]);
+    }
+  }
+  // Simulation::updateTimeStepSizeCFL (Simulation.cpp:544-616): the squared
+  // speed bound starts at 0.1, fluid particles count with v + a h, samples of
+  // dynamic bodies with their velocity.  Returns the new step size.
+  RF cfl_step_size() const {
+    RF maxVel = RF(0.1);
+    const RF diameter = RF(2.0) * RF(P.radius);
+    for (size_t i = 0; i < x.size(); ++i) {
+      const RF velMag = (v[i] + accel[i] * dt).squaredNorm();
+      if (velMag > maxVel) maxVel = velMag;
+    }
+    for (const auto& b : bodies)
+      if (b.dynamic || b.animated)
+        for (const auto& vb : b.v) {
+          const RF velMag = vb.squaredNorm();
+          if (velMag > maxVel) maxVel = velMag;
+        }
+    RF hn = RF(P.cflFactor) * RF(0.4) * (diameter / std::sqrt(maxVel));
+    hn = std::min(hn, RF(P.cflMaxTimeStepSize));
+    hn = std::max(hn, RF(P.cflMinTimeStepSize));
+    return hn;
+  }
+  bool nonpressure_on() const { return P.surfaceTensionMethod == 2 || P.viscosityMethod != 0 || P.cflMethod == 1; }
+
   // One full step in SimulatorBase::timeStepNoGUI order (SimulatorBase.cpp:1142-1200)
   // around TimeStepDiffDFSPH::step (TimeStepDiffDFSPH.cpp:526-652).
   void step() {
@@ -1087,12 +1244,24 @@ template <typename RF, typename RR> class Sim {
     else stats.iterationsV = 0;
     if (tp) tp->vDiv = v;
     const long N = long(x.size());
+    // the step size of the step's start (TimeStepDiffDFSPH.cpp:535) advects
+    // the velocities and the positions and advances the time; the CFL update
+    // (:584-586) sets the one the pressure solve, the rigid bodies and the
+    // next step read
+    const RF h_old = dt;
+    if (nonpressure_on()) {
+      compute_nonpressure();
+      if (P.cflMethod == 1) dt = cfl_step_size();
 #pragma omp parallel for schedule(static)
-    for (long i = 0; i < N; ++i) v[i] += g * dt;  // clearAccelerations + advect (:588-605)
+      for (long i = 0; i < N; ++i) v[i] += accel[i] * h_old;  // :588-605
+    } else {
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < N; ++i) v[i] += g * dt;  // clearAccelerations + advect (:588-605)
+    }
     pressure_solve();
 #pragma omp parallel for schedule(static)
-    for (long i = 0; i < N; ++i) x[i] += v[i] * dt;  // :613-634
-    time = time + dt;
+    for (long i = 0; i < N; ++i) x[i] += v[i] * h_old;  // :613-634
+    time = time + h_old;
     backward_per_step();
     const bool mgr = P.computeGradient && P.useGradientManager;
     if (mgr) for (auto& b : bodies) if (b.dynamic) manager_velocity_chain(b);

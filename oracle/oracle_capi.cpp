@@ -87,6 +87,15 @@ template <typename RF, typename RR> struct Oracle : OracleBase {
       for (size_t i = 0; i < N; ++i) for (int k = 0; k < 3; ++k) out[3 * i + k] = double(sim.s[i][k]);
       return 0;
     }
+    // accelerations of the last step (gravity + non-pressure forces), Akinci 2013 normals
+    const std::vector<V3<RF>>* vf = n == "accel" ? &sim.accel : n == "normal" ? &sim.normal : nullptr;
+    if (vf) {
+      for (size_t i = 0; i < N; ++i)
+        for (int k = 0; k < 3; ++k) out[3 * i + k] = i < vf->size() ? double((*vf)[i][k]) : 0.0;
+      return 0;
+    }
+    if (n == "dt") { out[0] = double(sim.dt); return 0; }  // the step size the next step starts with
+    if (n == "time") { out[0] = double(sim.time); return 0; }
     return -1;
   }
   int get_body_field(int body, const char* name, double* out) const override {
@@ -163,6 +172,7 @@ template <typename RF, typename RR> struct Oracle : OracleBase {
     if (n == "kappa") { for (size_t i = 0; i < N; ++i) sim.kappa[i] = RF(in[i]); return 0; }
     if (n == "kappa_v") { for (size_t i = 0; i < N; ++i) sim.kappaV[i] = RF(in[i]); return 0; }
     if (n == "time") { sim.time = RF(in[0]); return 0; }
+    if (n == "dt") { sim.dt = RF(in[0]); return 0; }
     if (n == "x" || n == "v") {  // fluid positions or velocities [N][3]
       auto& f = (n == "x") ? sim.x : sim.v;
       for (size_t i = 0; i < N; ++i) f[i] = V3<RF>(RF(in[3 * i]), RF(in[3 * i + 1]), RF(in[3 * i + 2]));
@@ -455,6 +465,23 @@ int orc_adjoint(void* h, const double* seed, double* out) {
 }
 
 // Closed-form probes of the cubic kernel for the kernel tests.
+// The Akinci 2013 cohesion and adhesion kernels of the oracle (FP32 or FP64).
+void orc_cohesion_adhesion_eval(int variant_double, double radius, const double* r, int n, double* Cw, double* Aw) {
+  auto run = [&](auto zero) {
+    using T = decltype(zero);
+    Cohesion<T> KC;
+    Adhesion<T> KA;
+    KC.set(T(radius));
+    KA.set(T(radius));
+    for (int i = 0; i < n; ++i) {
+      const V3<T> v{T(r[3 * i]), T(r[3 * i + 1]), T(r[3 * i + 2])};
+      Cw[i] = double(KC.W(v));
+      Aw[i] = double(KA.W(v));
+    }
+  };
+  if (variant_double) run(double(0)); else run(float(0));
+}
+
 void orc_kernel_eval(int variant_double, double radius, const double* r, int n, double* W, double* gW, double* ggW) {
   if (variant_double) {
     Cubic<double> K; K.set(radius);

@@ -27,6 +27,10 @@ class Params(C.Structure):
         ("optimizeRotation", C.c_int), ("gyroscopic", C.c_int), ("computeGradient", C.c_int),
         ("uniformAccTime", C.c_double), ("targetTime", C.c_double), ("releaseMode", C.c_int),
         ("useGradientManager", C.c_int),
+        ("surfaceTensionMethod", C.c_int), ("surfaceTension", C.c_double), ("surfaceTensionBoundary", C.c_double),
+        ("viscosityMethod", C.c_int), ("viscosity", C.c_double), ("viscosityBoundary", C.c_double),
+        ("cflMethod", C.c_int), ("cflFactor", C.c_double), ("cflMinTimeStepSize", C.c_double),
+        ("cflMaxTimeStepSize", C.c_double),
     ]
 
 
@@ -72,6 +76,7 @@ def lib():
         L.orc_neighbor_list.argtypes = [P, C.c_int, C.c_int, C.c_size_t, C.POINTER(C.c_uint), C.c_int]
         L.orc_set_state.argtypes = [P, C.c_char_p, dp]
         L.orc_kernel_eval.argtypes = [C.c_int, C.c_double, dp, C.c_int, dp, dp, dp]
+        L.orc_cohesion_adhesion_eval.argtypes = [C.c_int, C.c_double, dp, C.c_int, dp, dp]
         L.orc_grad_utils.argtypes = [dp, dp, dp, dp, dp, dp]
         ip = C.POINTER(C.c_int)
         L.orc_set_record.argtypes = [P, C.c_int]
@@ -114,6 +119,16 @@ def make_params(p: dict) -> Params:
     q.uniformAccTime = p["uniform_acc_time"]; q.targetTime = p["target_time"]
     q.releaseMode = int(p.get("release_mode", 0))
     q.useGradientManager = int(p.get("use_gradient_manager", 0))
+    q.surfaceTensionMethod = int(p.get("surface_tension_method", 0))
+    q.surfaceTension = float(p.get("surface_tension", 0.05))
+    q.surfaceTensionBoundary = float(p.get("surface_tension_boundary", 0.01))
+    q.viscosityMethod = int(p.get("viscosity_method", 0))
+    q.viscosity = float(p.get("viscosity", 0.01))
+    q.viscosityBoundary = float(p.get("viscosity_boundary", 0.0))
+    q.cflMethod = int(p.get("cfl_method", 0))
+    q.cflFactor = float(p.get("cfl_factor", 0.5))
+    q.cflMinTimeStepSize = float(p.get("cfl_min_time_step_size", 0.0001))
+    q.cflMaxTimeStepSize = float(p.get("cfl_max_time_step_size", 0.005))
     return q
 
 
@@ -263,7 +278,14 @@ class OracleSim:
         return x, v
 
     def field(self, name: str) -> np.ndarray:
-        k = 3 if name == "sum_grad_p_k" else 1
+        """Per-particle fields (sum_grad_p_k, accel, normal: (n, 3)); "dt" and
+        "time" are scalars (the step size the next step starts with, the time)."""
+        if name in ("dt", "time"):
+            out = np.empty(1)
+            if lib().orc_get_field(self.h, name.encode(), _dp(out)) != 0:
+                raise KeyError(name)
+            return float(out[0])
+        k = 3 if name in ("sum_grad_p_k", "accel", "normal") else 1
         out = np.empty(self.n * k)
         if lib().orc_get_field(self.h, name.encode(), _dp(out)) != 0:
             raise KeyError(name)
@@ -360,6 +382,15 @@ def kernel_eval(radius: float, r: np.ndarray, double: bool = True):
     W = np.empty(n); gW = np.empty((n, 3)); ggW = np.empty((n, 3, 3))
     lib().orc_kernel_eval(int(double), radius, _dp(r), n, _dp(W), _dp(gW), _dp(ggW))
     return W, gW, ggW
+
+
+def cohesion_adhesion_eval(radius: float, r: np.ndarray, double: bool = True):
+    """The oracle's Akinci 2013 cohesion and adhesion kernels at offsets r."""
+    r = np.ascontiguousarray(r, np.float64).reshape(-1, 3)
+    n = r.shape[0]
+    Cw = np.empty(n); Aw = np.empty(n)
+    lib().orc_cohesion_adhesion_eval(int(double), radius, _dp(r), n, _dp(Cw), _dp(Aw))
+    return Cw, Aw
 
 
 def grad_utils(q: np.ndarray, p: np.ndarray):

@@ -1,4 +1,4 @@
-// C entry points over the reference's own cubic kernel and gradient helpers
+// C entry points over the reference's own cubic, cohesion and adhesion kernels and gradient helpers
 // (test infrastructure).  Compiled by oracle/Makefile.ref together with the
 // reference sources SPlisHSPlasH/SPHKernels.cpp and SPlisHSPlasH/GradientUtils.cpp
 // taken in place from the reference tree (FP32 build, Real = float); the result
@@ -18,6 +18,17 @@ void ref_cubic_eval(double radius, const double* r, int n, double* W, double* gW
     const Matrix3r H = SPH::CubicKernel::gradGradW(v);
     for (int a = 0; a < 3; ++a)
       for (int b = 0; b < 3; ++b) ggW[9 * i + 3 * a + b] = H(a, b);
+  }
+}
+
+// Akinci et al. 2013 cohesion and adhesion kernels (radius = support radius).
+void ref_cohesion_adhesion_eval(double radius, const double* r, int n, double* C, double* A) {
+  SPH::CohesionKernel::setRadius(static_cast<Real>(radius));
+  SPH::AdhesionKernel::setRadius(static_cast<Real>(radius));
+  for (int i = 0; i < n; ++i) {
+    const Vector3r v(static_cast<Real>(r[3 * i]), static_cast<Real>(r[3 * i + 1]), static_cast<Real>(r[3 * i + 2]));
+    C[i] = SPH::CohesionKernel::W(v);
+    A[i] = SPH::AdhesionKernel::W(v);
   }
 }
 

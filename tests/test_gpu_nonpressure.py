@@ -1,0 +1,102 @@
+"""Non-pressure forces and the CFL step size on the GPU against the oracle
+(Simulation::computeNonPressureForces, Simulation.cpp:618-631, and
+updateTimeStepSizeCFL, :544-616): Akinci 2013 surface tension, Standard and
+XSPH viscosity, CFL with the step-size bookkeeping of TimeStepDiffDFSPH::step
+(the step's starting size advects velocities and positions and advances the
+time; the new size runs the pressure solve, the rigid bodies and the next step).
+
+Tolerances: one step's accelerations within 1e-5 of their largest component
+(FP32 sums in a different order), step sizes within 1e-5 relative (the bound is
+a maximum over FP32 speeds), and over a 40-step rollout (steps of up to 5 ms)
+with the GPU's iteration counts replayed in the oracle the rigid pose and the
+fluid within 1e-4 (relative, m RMS) or three times the spread between the
+oracle's own FP32 and FP64 runs, whichever is larger.
+"""
+import numpy as np
+import pytest
+
+from difffr_amd import scenes
+from difffr_amd.simulation import DfrError, Simulation
+from oracle.pyoracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+
+FORCES = dict(surface_tension_method=2, surface_tension=0.2, surface_tension_boundary=0.05,
+              viscosity_method=1, viscosity=0.01, cfl_method=1, cfl_factor=0.5,
+              cfl_min_time_step_size=1e-4, cfl_max_time_step_size=0.005)
+
+
+def _scene():
+    sc = scenes.submerged_body(body="sphere", steps=40)
+    sc.params.update(FORCES)
+    return sc
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("visc", [1, 2])
+def test_one_step_accelerations_and_step_size(gpu_ready, visc):
+    sc = _scene()
+    sc.params["viscosity_method"] = visc
+    print(f"[non-pressure one step] viscosity method {visc}, n={sc.n_fluid}", flush=True)
+    g = Simulation(sc)
+    o = OracleSim(sc, "f32_rigid64")
+    g.step(1)
+    it = g.stats()
+    o.set_fixed_iterations([it["iterations_v"]], [it["iterations"]])
+    o.step(1)
+    ag, ao = g.field("accel"), o.field("accel")
+    ng, no = g.field("normal"), o.field("normal")
+    e_a = np.abs(ag - ao).max() / np.abs(ao - np.asarray(sc.params["gravity"])).max()
+    e_n = np.abs(ng - no).max() / np.abs(no).max()
+    print(f"  accel err {e_a:.2e} of the force scale, normal err {e_n:.2e}, dt GPU {it['dt']:.7g} "
+          f"oracle {o.field('dt'):.7g}", flush=True)
+    assert e_a < 1e-5 and e_n < 1e-5
+    assert it["dt"] == pytest.approx(o.field("dt"), rel=1e-5)
+    assert it["dt"] != pytest.approx(sc.params["dt"], rel=1e-3)  # the CFL rule moved it
+    g.close()
+
+
+def test_rollout_with_forces_and_cfl(gpu_ready):
+    sc = _scene()
+    print(f"[non-pressure rollout] {sc.steps} steps, n={sc.n_fluid}", flush=True)
+    g = Simulation(sc)
+    o = OracleSim(sc, "f32_rigid64")
+    o64 = OracleSim(sc, "f64")
+    dts_g, dts_o = [], []
+    for s in range(sc.steps):
+        g.step(1)
+        st = g.stats()
+        for oo in (o, o64):
+            oo.set_fixed_iterations([st["iterations_v"]], [st["iterations"]])
+            oo.step(1)
+        dts_g.append(st["dt"])
+        dts_o.append(o.field("dt"))
+    rg, ro, r64 = g.rigid(1), o.rigid(1), o64.rigid(1)
+    xg, _ = g.fluid()
+    xo, _ = o.fluid()
+    x64, _ = o64.fluid()
+    rms = lambda a, b: float(np.sqrt(((a - b) ** 2).sum(1).mean()))
+    pose = lambda a, b: max(rel(a["x"], b["x"]), rel(a["q"], b["q"]))
+    e, floor = pose(rg, ro), pose(ro, r64)
+    print(f"  dt range {min(dts_g):.3g}..{max(dts_g):.3g}, max dt rel err "
+          f"{np.max(np.abs(np.array(dts_g) / np.array(dts_o) - 1)):.1e}; time GPU {g.stats()['time']:.6f} "
+          f"oracle {o.field('time'):.6f}; pose rel vs FP32 oracle {e:.1e} (FP32 vs FP64 oracle {floor:.1e}); "
+          f"fluid rms {rms(xg, xo):.1e} m (oracle FP32 vs FP64 {rms(xo, x64):.1e} m)", flush=True)
+    np.testing.assert_allclose(dts_g, dts_o, rtol=1e-5)
+    assert g.stats()["time"] == pytest.approx(o.field("time"), rel=1e-5)
+    # within the precision floor of the step (its FP32 and FP64 runs' spread)
+    assert e < max(1e-4, 3 * floor)
+    assert rms(xg, xo) < max(1e-4, 3 * rms(xo, x64))
+    g.close()
+
+
+def test_exact_adjoint_refuses_forces(gpu_ready):
+    sim = Simulation(_scene(), params_override={"compute_gradient": 0})
+    with pytest.raises(DfrError, match="non-pressure"):
+        sim.adjoint_begin(0)
+    sim.close()

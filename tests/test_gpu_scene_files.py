@@ -7,9 +7,9 @@ same loaded state.  The state is ill-conditioned for this sampling of the
 bottle (it was settled against the reference's own random Poisson samples):
 moving the positions by one FP32 ulp moves the fluid by ~6 um RMS within a
 step.  The GPU is held to three times that measured response (fluid RMS and
-rigid pose) and the rigid position to 1e-4 relative.  The scene's CFL
-step size and fluid viscosity are not run by this library (the step uses the
-fixed time step 1 ms on both sides)."""
+rigid pose) and the rigid position to 1e-4 relative.  Both sides run the
+scene's CFL step size and the fluid's default Standard viscosity (the scene has
+no "Materials" entry), starting from the loader's default step size 1 ms."""
 import os
 
 import numpy as np
@@ -36,7 +36,8 @@ def test_reference_scene_and_state_through_the_mirror(gpu_ready):
     core = base._core
     sc = base._scene
     assert sim.numberOfBoundaryModels() == 2 and core.n == 5904
-    assert "cflMethod" in sc.unsupported
+    assert sc.unsupported == []
+    assert (sc.params["cfl_method"], sc.params["viscosity_method"], sc.params["surface_tension_method"]) == (1, 1, 0)
     st = sceneio.read_state(STATE, 1)
     fl = st["fluids"]["Fluid"]
     ids = fl["id"].astype(np.int64)

@@ -52,7 +52,14 @@ def test_scene_keys_and_defaults():
     (blk,) = sf.fluid_blocks
     np.testing.assert_array_equal(blk.box_min, [-1.5, 0.0, -0.75])
     np.testing.assert_array_equal(blk.box_max, [1.5, 1.5, 0.75])
-    assert "cflMethod" in sf.unsupported and "Materials.surfaceTensionMethod" in sf.unsupported
+    assert sf.unsupported == []
+    # the CFL keys and the fluid's material: Akinci 2013 surface tension from
+    # "Materials", the FluidModel's default Standard viscosity 0.01 (the
+    # "viscosityMethod" key of the fluid block is not read by the reference)
+    assert (c["cflMethod"], c["cflFactor"], c["cflMaxTimeStepSize"], c["cflMinTimeStepSize"]) == (1, 0.5, 0.005, 0.0001)
+    m = sceneio.fluid_material(sf)
+    assert (m["surface_tension_method"], m["surface_tension"], m["surface_tension_boundary"]) == (2, 0.5, 0.0)
+    assert (m["viscosity_method"], m["viscosity"], m["viscosity_boundary"]) == (1, 0.01, 0.0)
     b = _scene("billiards-on-water-2balls.json").config
     assert b["useRigidGradientManager"] is True and b["useReleaseRigidBodyMode"] is True
     d = _scene("diff-bottle-model-360degree.json")

@@ -633,9 +633,146 @@ void launch_adj_update(const AdjPass& a, int mode, const float* k, const float4*
   }
 }
 
+// ---------------------------------------------------------------------------
+// Reverse of the advection with the step's viscosity, v' = v + dt (g + a_i),
+// at the velocities after the divergence solve (Viscosity_Standard.cpp:262-268,
+// Viscosity_XSPH.cpp:62-68).  abar = dt vbar' of every particle (staged apart,
+// since vbar itself is updated); per particle, over its fluid neighbours j with
+// r = x_i - x_j, u = v_i - v_j, G = c r = gradW, H = c I + e r r^T:
+//   Standard (c_k = 10 nu m / rho_k, D = |r|^2 + 0.01 h^2, q = u.r,
+//   b = c_j abar_i - c_i abar_j):
+//     vbar_i += (b.G / D) r
+//     xbar_i += (b.G / D) u + (q / D) H b - 2 q (b.G) / D^2 r
+//     rhobar_i += rho0 c_i q (abar_j.G) / (D rho_i)
+//   XSPH (s = nu / dt, b = (m / rho_j) abar_i - (m / rho_i) abar_j):
+//     vbar_i += -s W b,  xbar_i += -s (b.u) G,  rhobar_i += -rho0 s W (m / rho_i^2) (abar_j.u)
+// rhobar is the adjoint of rho / rho0.  LDS per slot: x, y, z, rho | v, abar_x | abar_y, abar_z.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_adj_viscosity_t(AdjPass a, const float4* __restrict__ v, const float* __restrict__ rho,
+                  const float4* __restrict__ abar, ForceConst f, float4* __restrict__ vbar, float4* __restrict__ xbar,
+                  float* __restrict__ rho_bar) {
+  __shared__ TileHdr<RB> T;
+  extern __shared__ float4 lds4[];
+  const StepConst& c = a.c;
+  const uint32_t brick = xcd_block();
+  const int cap = a.cap;
+  BrickView B;
+  if (!brick_open<RB>(a.g, a.fstart, a.thdr, brick, cap, T, B)) return;
+  const uint32_t total = B.total;
+  const bool in_lds = B.in_lds;
+  float4* LA = lds4;
+  float4* LB = lds4 + cap;
+  float2* LC = reinterpret_cast<float2*>(lds4 + 2 * cap);
+  const int t = threadIdx.x;
+  if (in_lds) {
+    uint32_t js[PTR];
+    map_indices<RB>(a.tmap, brick, total, js);
+    float4 q[PTR], w[PTR], ab[PTR];
+    float rr[PTR];
+#pragma unroll
+    for (int r = 0; r < PTR; ++r)
+      if (t + r * RB < (int)total) { q[r] = a.x[js[r]]; w[r] = v[js[r]]; ab[r] = abar[js[r]]; rr[r] = rho[js[r]]; }
+#pragma unroll
+    for (int r = 0; r < PTR; ++r) {
+      const uint32_t s = t + r * RB;
+      if (s < total) {
+        LA[s] = make_float4(q[r].x, q[r].y, q[r].z, rr[r]);
+        LB[s] = make_float4(w[r].x, w[r].y, w[r].z, ab[r].x);
+        LC[s] = make_float2(ab[r].y, ab[r].z);
+      }
+    }
+  }
+  __syncthreads();
+  const size_t stride = (size_t)c.nf;
+  const float eps = 0.01f * (f.h * f.h);
+  const float sx = f.visc / c.dt;  // XSPH's nu / dt
+  for (uint32_t i = B.own_begin + t; i < B.own_end; i += RB) {
+    const float4 p = a.x[i];
+    const float4 vi = v[i];
+    const float4 ai = abar[i];
+    const float ri = rho[i];
+    const float ci = (MODE == 1 ? 10.f * f.visc : 1.f) * (c.m_f / ri);
+    const uint32_t nF = a.cnt[i] & 0xffffu;
+    float vbx = 0.f, vby = 0.f, vbz = 0.f, xbx = 0.f, xby = 0.f, xbz = 0.f, rb = 0.f;
+    auto pair = [&](float qx, float qy, float qz, float rj, float wx, float wy, float wz, float ajx, float ajy,
+                    float ajz) {
+      const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
+      const float ux = vi.x - wx, uy = vi.y - wy, uz = vi.z - wz;
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      const float cj = (MODE == 1 ? 10.f * f.visc : 1.f) * (c.m_f / rj);
+      const float bx = cj * ai.x - ci * ajx, by = cj * ai.y - ci * ajy, bz = cj * ai.z - ci * ajz;
+      if (MODE == 1) {
+        const PairCE ce = cubic_ce(c.K, r2);
+        const float iD = 1.f / (r2 + eps);
+        const float qd = (ux * rx + uy * ry + uz * rz) * iD;       // q / D
+        const float br = bx * rx + by * ry + bz * rz;
+        const float bG = ce.c * br * iD;                           // b.G / D
+        vbx += bG * rx; vby += bG * ry; vbz += bG * rz;
+        const float sr = qd * ce.e * br - 2.f * qd * bG;           // r coefficient of (q/D) H b - 2 q b.G / D^2 r
+        const float sb = qd * ce.c;
+        xbx += bG * ux + sb * bx + sr * rx;
+        xby += bG * uy + sb * by + sr * ry;
+        xbz += bG * uz + sb * bz + sr * rz;
+        rb += ci * qd * ce.c * (ajx * rx + ajy * ry + ajz * rz);
+      } else {
+        const float W = cubic_W(c.K, sqrtf(r2));
+        const float sw = -sx * W;
+        vbx += sw * bx; vby += sw * by; vbz += sw * bz;
+        const float gc = -sx * (bx * ux + by * uy + bz * uz) * cubic_grad_c(c.K, r2);
+        xbx += gc * rx; xby += gc * ry; xbz += gc * rz;
+        rb += sw * ci * (ajx * ux + ajy * uy + ajz * uz);
+      }
+    };
+    if (in_lds) {
+      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) {
+        const float4 A = LA[s], Bv = LB[s];
+        const float2 Cn = LC[s];
+        pair(A.x, A.y, A.z, A.w, Bv.x, Bv.y, Bv.z, Bv.w, Cn.x, Cn.y);
+      });
+    } else {
+      for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {
+        const uint32_t j = tile_global<RB>(T, s);
+        const float4 q = a.x[j], w = v[j], aj = abar[j];
+        pair(q.x, q.y, q.z, rho[j], w.x, w.y, w.z, aj.x, aj.y, aj.z);
+      });
+    }
+    float4 vo = vbar[i];
+    vo.x += vbx; vo.y += vby; vo.z += vbz;
+    vbar[i] = vo;
+    float4 xo = xbar[i];
+    xo.x += xbx; xo.y += xby; xo.z += xbz;
+    xbar[i] = xo;
+    rho_bar[i] += c.rho0 * rb / ri;
+  }
+}
+
+// abar = dt vbar (the adjoint of the acceleration of the advection).
+__global__ void k_adj_scale4(int n, float dt, const float4* __restrict__ vbar, float4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint32_t)n) return;
+  const float4 w = vbar[i];
+  out[i] = make_float4(dt * w.x, dt * w.y, dt * w.z, 0.f);
+}
+
 void launch_adj_delta(const AdjPass& a, const float4* v, const float* dbar, float4* vbar, float4* xbar,
                       hipStream_t s) {
   hipLaunchKernelGGL(k_adj_delta_t, dim3(nbricks(a.g)), dim3(RB), shm_slots(a, 8), s, a, v, dbar, vbar, xbar);
+}
+
+void launch_adj_viscosity(const AdjPass& a, int mode, const float4* v, const float* rho, const float4* vbar_in,
+                          float4* abar, const ForceConst& fc, float4* vbar, float4* xbar, float* rho_bar,
+                          hipStream_t s) {
+  if (a.c.nf <= 0) return;
+  hipLaunchKernelGGL(k_adj_scale4, dim3(nblk(a.c.nf)), dim3(DFR_BLOCK), 0, s, a.c.nf, a.c.dt, vbar_in, abar);
+  const size_t shm = (2 * sizeof(float4) + sizeof(float2)) * (size_t)a.cap;
+  if (mode == 1)
+    hipLaunchKernelGGL(k_adj_viscosity_t<1>, dim3(nbricks(a.g)), dim3(RB), shm, s, a, v, rho, abar, fc, vbar, xbar,
+                       rho_bar);
+  else
+    hipLaunchKernelGGL(k_adj_viscosity_t<2>, dim3(nbricks(a.g)), dim3(RB), shm, s, a, v, rho, abar, fc, vbar, xbar,
+                       rho_bar);
 }
 
 void launch_adj_factor(const AdjPass& a, const float* alpha, const float* alpha_bar, const float* rho_bar,
@@ -668,6 +805,9 @@ void launch_adj_rigid(const RigidStepConst& rc, const AdjStepTable& t, const Rig
 }
 
 void set_adjoint_kernel_attributes() {
+  const int s10 = (int)((2 * sizeof(float4) + sizeof(float2)) * TILE_CAP);
+  (void)hipFuncSetAttribute((const void*)k_adj_viscosity_t<1>, hipFuncAttributeMaxDynamicSharedMemorySize, s10);
+  (void)hipFuncSetAttribute((const void*)k_adj_viscosity_t<2>, hipFuncAttributeMaxDynamicSharedMemorySize, s10);
   const int s8 = (int)(sizeof(float) * TILE_CAP * 8);
   (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_DIV_ITER>, hipFuncAttributeMaxDynamicSharedMemorySize, s8);
   (void)hipFuncSetAttribute((const void*)k_adj_update_t<ADJ_PRESSURE_ITER>, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -118,6 +118,7 @@ struct Arena {
 struct StepTapeDev {
   float4* x = nullptr;                    // fluid positions during the step (sorted order)
   float4* v0 = nullptr;                   // fluid velocities after the sort (learned warm-start feature)
+  float4* vDiv = nullptr;                 // velocities after the divergence solve (the viscosity's input)
   float *kvPrev = nullptr, *d0 = nullptr, *kvW = nullptr;  // divergence warm start
   float *kPrev = nullptr, *rsW = nullptr, *kW = nullptr;   // pressure warm start
   float4 *bpos = nullptr, *bvel = nullptr;  // boundary samples at the start of the step
@@ -794,6 +795,7 @@ void do_step(dfr_ctx* c) {
     Timed t(c, TC_ELEMWISE);
     const bool forces = c->acc != nullptr;
     float h_new = dt;
+    if (rec && rec->vDiv && nf > 0) d2d(c, rec->vDiv, c->f.vel, nf);
     if (forces && nf > 0) {
       if (c->nrm) launch_normals_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->nrm, c->stream);
       launch_forces_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->nrm, c->acc, c->fc, c->stream);
@@ -990,6 +992,7 @@ void adjoint_buffers(dfr_ctx* c) {
   StepTapeDev& T = A.tape;
   T.x = dalloc<float4>(c, nf);
   T.v0 = dalloc<float4>(c, nf);
+  T.vDiv = c->P.viscosity_method != 0 ? dalloc<float4>(c, nf) : nullptr;
   T.kvPrev = dalloc<float>(c, nf); T.d0 = dalloc<float>(c, nf); T.kvW = dalloc<float>(c, nf);
   T.kPrev = dalloc<float>(c, nf); T.rsW = dalloc<float>(c, nf); T.kW = dalloc<float>(c, nf);
   T.bpos = dalloc<float4>(c, nb); T.bvel = dalloc<float4>(c, nb);
@@ -1007,6 +1010,7 @@ void tape_alloc_fixed(dfr_ctx* c, StepTapeDev& T) {
   auto fa = [&](size_t n) { return static_cast<float*>(R.alloc(sizeof(float) * n)); };
   T.x = static_cast<float4*>(R.alloc(sizeof(float4) * nf));
   T.v0 = c->mlp.enabled ? static_cast<float4*>(R.alloc(sizeof(float4) * nf)) : nullptr;
+  T.vDiv = c->P.viscosity_method != 0 ? static_cast<float4*>(R.alloc(sizeof(float4) * nf)) : nullptr;
   T.kvPrev = fa(nf); T.d0 = fa(nf); T.kvW = fa(nf); T.kPrev = fa(nf); T.rsW = fa(nf); T.kW = fa(nf);
   T.bpos = static_cast<float4*>(R.alloc(sizeof(float4) * nb));
   T.bvel = static_cast<float4*>(R.alloc(sizeof(float4) * nb));
@@ -1174,6 +1178,12 @@ void reverse_step(dfr_ctx* c, const StepTapeDev& T) {
       else
         launch_adj_update(a, ADJ_PRESSURE_WARM, T.kW, A.vb, A.kb, h2, T.rsW, T.kPrev, T.alpha, A.kb, A.alb,
                           A.rhob, A.xb, c->stream);
+    }
+    if (c->P.viscosity_method != 0) {
+      // the advection with the step's viscosity, at the velocities after the divergence solve
+      Timed t(c, TC_ADJ_MISC);
+      launch_adj_viscosity(a, c->P.viscosity_method, T.vDiv, T.rho, A.vb, A.vb2, c->fc, A.vb, A.xb, A.rhob,
+                           c->stream);
     }
     if (c->P.enable_divergence_solver) {
       const bool warmV = c->P.use_divergence_warmstart != 0;
@@ -2128,10 +2138,10 @@ int dfr_kernel_times(const dfr_ctx* c, int n, char names[][32], double* avg_ms, 
 int dfr_adjoint_begin(dfr_ctx* c, int checkpoint_every) {
   return guarded(c, [&]() {
     if (!c->initialized) return fail(c, DFR_ERR_STATE, "dfr_adjoint_begin before dfr_init");
-    if (c->P.surface_tension_method != 0 || c->P.viscosity_method != 0 || c->P.cfl_method != 0)
+    if (c->P.surface_tension_method != 0 || c->P.cfl_method != 0)
       return fail(c, DFR_ERR_STATE,
-                  "the exact adjoint does not cover non-pressure forces or CFL step sizes (surface_tension_method, "
-                  "viscosity_method and cfl_method must be 0)");
+                  "the exact adjoint covers viscosity but no other non-pressure force and no CFL step size "
+                  "(surface_tension_method and cfl_method must be 0)");
     HIPCHK(hipSetDevice(c->device));
     adjoint_release(c);
     adjoint_buffers(c);

@@ -256,6 +256,12 @@ void launch_adj_update(const AdjPass& a, int mode, const float* k, const float4*
                        float* alpha_bar, float* rho_bar, float4* xbar, hipStream_t s);
 void launch_adj_delta(const AdjPass& a, const float4* v, const float* dbar, float4* vbar, float4* xbar,
                       hipStream_t s);
+// Reverse of the advection with Standard (mode 1) or XSPH (mode 2) viscosity:
+// abar = dt vbar_in (scratch), then vbar, xbar and rho_bar (adjoint of rho / rho0)
+// receive the force's terms.  v: velocities after the divergence solve.
+void launch_adj_viscosity(const AdjPass& a, int mode, const float4* v, const float* rho, const float4* vbar_in,
+                          float4* abar, const ForceConst& fc, float4* vbar, float4* xbar, float* rho_bar,
+                          hipStream_t s);
 void launch_adj_factor(const AdjPass& a, const float* alpha, const float* alpha_bar, const float* rho_bar,
                        const float4* G, float4* xbar, const float* rbb, hipStream_t s);
 void launch_adj_position(int n, float dt, const float4* xbar, float4* vbar, hipStream_t s);

@@ -264,6 +264,53 @@ template <typename RF, typename RR> struct Adjoint {
     });
   }
 
+  // ---- reverse of v' = v + dt (g + a_visc(v, x, rho)) (the advection with the
+  // viscosity of the step: Viscosity_Standard.cpp:262-268 or Viscosity_XSPH.cpp:62-68)
+  // at the velocities after the divergence solve.  vb holds the adjoint of v' on
+  // entry and of v on exit; rhob receives the adjoint of rho / rho0.  Standard,
+  // per pair: T_ij = c_j (u.r) / D gradW(r), c_j = 10 nu m / rho_j, D = |r|^2 + 0.01 h^2;
+  // XSPH: T_ij = -(nu / dt) (m / rho_j) u W(r); u = v_i - v_j, r = x_i - x_j.
+  void reverse_viscosity(const StepTape<RF, RR>& T, std::vector<A>& rhob) {
+    const long N = long(T.x.size());
+    const A dt = A(S.dt), nu = A(RF(S.P.viscosity)), mf = A(S.m_f), rho0 = A(S.rho0);
+    const A eps = 0.01 * A(S.h * S.h);
+    std::vector<V> abar(N);
+    for (long i = 0; i < N; ++i) abar[i] = vb[i] * dt;
+    const int method = S.P.viscosityMethod;
+    scatter(N, &rhob, [&](long i, Acc& o) {
+      const V ai = abar[i];
+      const V vi = cv(T.vDiv[i]);
+      for (uint32_t j : T.nFF[i]) {
+        const V3<RF> rr = T.x[i] - T.x[j];
+        const V r = cv(rr);
+        const V u = vi - cv(T.vDiv[j]);
+        const A rj = A(T.rho[j]);
+        if (method == 1) {
+          const A c = A(10) * nu * mf / rj;
+          const A D = A(rr.squaredNorm()) + eps;
+          const A q = u.dot(r), aG = ai.dot(gw(rr));
+          const V gu = r * (c * aG / D);
+          o.v[i] += gu;
+          o.v[j] -= gu;
+          const V gr = u * (c * aG / D) + hv(rr, ai) * (c * q / D) - r * (A(2) * c * q * aG / (D * D));
+          o.x[i] += gr;
+          o.x[j] -= gr;
+          o.k[j] -= (c * q * aG / D) / (rj / rho0);
+        } else {
+          const A c = nu / dt * mf / rj;
+          const A W = A(S.K.W(rr)), au = ai.dot(u);
+          const V gu = ai * (-c * W);
+          o.v[i] += gu;
+          o.v[j] -= gu;
+          const V gr = gw(rr) * (-c * au);
+          o.x[i] += gr;
+          o.x[j] -= gr;
+          o.k[j] += (c * au * W) / (rj / rho0);
+        }
+      }
+    });
+  }
+
   // ---- fluid part of one step, given Fb/Tb
   void fluid_reverse(const StepTape<RF, RR>& T) {
     const long N = long(T.x.size());
@@ -296,6 +343,8 @@ template <typename RF, typename RR> struct Adjoint {
         else kb[i] = (T.rsW[i] > RF(1) && T.kPrev[i] > RF(-0.00025)) ? kbuf[i] * A(0.5) * invH2 : A(0);
       }
     }
+    // advection with the step's viscosity
+    if (S.P.viscosityMethod != 0) reverse_viscosity(T, rhob);
     // divergence solve
     if (S.P.enableDivergenceSolver) {
       const bool warmV = S.P.useDivergenceWarmstart != 0;

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <memory>
 #include <cstdlib>
+#include <stdexcept>
 #include <string>
 #include "dfsph_adjoint.hpp"
 
@@ -262,6 +263,10 @@ template <typename RF, typename RR> struct Oracle : OracleBase {
     else sim.net.enabled = false;
   }
   void adjoint(const double* seed, double* out) override {
+    // The adjoint covers the viscosity forces (both methods); surface tension
+    // and the CFL step size are forward-only.
+    if (sim.P.surfaceTensionMethod != 0 || sim.P.cflMethod != 0)
+      throw std::runtime_error("adjoint: surface tension and the CFL step size are forward-only");
     Adjoint<RF, RR> adj(sim);
     adj.run(seed, out);
   }

@@ -5,7 +5,8 @@
   or mask flips inside the difference interval; over these horizons none does);
 * the fluid-free closed form dx_N/dv0 = N dt I (two release-window steps
   included: both override v with v0, so every step moves the body by dt v0);
-* FP32 tapes against FP64 tapes (the GPU computes in FP32).
+* FP32 tapes against FP64 tapes (the GPU computes in FP32);
+* with Standard or XSPH viscosity in the advection, central differences again.
 """
 import numpy as np
 import pytest
@@ -138,3 +139,35 @@ def test_adjoint_after_impact_is_the_derivative_of_a_rough_map():
     coarse = secant(1, 1e-8)
     print(f"eps=1e-8 secant {coarse:.6f}", flush=True)
     assert abs(coarse - g[1]) > 0.5 * abs(g[1])
+
+
+@pytest.mark.parametrize("visc,nu", [(1, 0.01), (2, 0.2)])
+def test_adjoint_with_viscosity_matches_central_differences(visc, nu):
+    """Standard viscosity at the reference fluid's default coefficient 0.01
+    (Viscosity_Standard.cpp:262-268) and XSPH (Viscosity_XSPH.cpp:62-68) in the
+    advection of every step: the adjoint carries the force's dependence on the
+    fluid velocities, positions and densities.  Over these 10 steps the
+    viscosity moves the gradient by several percent of its largest component,
+    so an adjoint without its terms misses the differences by far more than
+    the tolerance (checked below against the inviscid scene's adjoint)."""
+    sc = scenes.submerged_body(body="ellipsoid", steps=10)
+    sc.params.update(viscosity_method=visc, viscosity=nu)
+    seed = np.random.default_rng(7).normal(size=7)
+    g, fd = _adjoint_and_fd(sc, 10, seed)
+    scale = np.abs(fd).max()
+    assert scale > 1e-4
+    np.testing.assert_allclose(g, fd, rtol=0, atol=1e-6 * scale)
+    sc0 = scenes.submerged_body(body="ellipsoid", steps=10)
+    s0, _ = _run(sc0, "f64", sc0.bodies[1].v0, sc0.bodies[1].w0, 10, record=True)
+    sd = np.zeros((len(sc0.bodies), 7))
+    sd[1] = seed
+    assert np.abs(s0.adjoint(sd)[1] - fd).max() > 1e-2 * scale
+
+
+def test_adjoint_refuses_surface_tension_and_cfl():
+    for extra in (dict(surface_tension_method=2), dict(cfl_method=1)):
+        sc = scenes.submerged_body(body="sphere", steps=2)
+        sc.params.update(extra)
+        s, _ = _run(sc, "f64", sc.bodies[1].v0, sc.bodies[1].w0, 2, record=True)
+        with pytest.raises(RuntimeError):
+            s.adjoint(np.zeros((len(sc.bodies), 7)))

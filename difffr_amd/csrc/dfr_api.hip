@@ -210,6 +210,7 @@ struct dfr_ctx {
   // derived constants
   float h = 0, dtf = 0;       // dtf: the TimeManager's step size (the next step's start)
   float4* acc = nullptr;      // non-pressure accelerations (g + forces) of the current step
+  float *he_color = nullptr, *he_gc2 = nullptr;  // He 2014 colour field and |grad c|^2
   float4* nrm = nullptr;      // Akinci 2013 normals
   uint32_t* d_cflmax = nullptr;  // CFL squared-speed bound (float bits)
   ForceConst fc{};
@@ -798,7 +799,10 @@ void do_step(dfr_ctx* c) {
     if (rec && rec->vDiv && nf > 0) d2d(c, rec->vDiv, c->f.vel, nf);
     if (forces && nf > 0) {
       if (c->nrm) launch_normals_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->nrm, c->stream);
-      launch_forces_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->nrm, c->acc, c->fc, c->stream);
+      if (c->he_color)
+        launch_he_color_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->he_color, c->he_gc2, c->stream);
+      launch_forces_t(c->sc, c->grid, c->gb, c->f, c->b, c->nb, c->nrm, c->he_gc2, c->acc, c->d_rig, c->fc,
+                      c->stream);
     }
     if (c->d_cflmax) {
       launch_cfl_max(c->sc, c->f.vel, forces ? c->acc : nullptr, c->b.n, c->b.vel, c->d_cflmax, c->stream);
@@ -1475,9 +1479,9 @@ void dfr_default_params(dfr_params* p, double radius, double dt) {
 
 dfr_ctx* dfr_create(const dfr_params* p, int device) {
   if (!p || p->radius <= 0.0 || p->dt <= 0.0) return nullptr;
-  // built: surface tension 0/2 (none, Akinci 2013), viscosity 0/1/2 (none, Standard, XSPH) without boundary
-  // viscosity, step size 0/1 (fixed, CFL)
-  if ((p->surface_tension_method != 0 && p->surface_tension_method != 2) ||
+  // built: surface tension 0/2/3 (none, Akinci 2013, He 2014), viscosity 0/1/2 (none, Standard, XSPH)
+  // without boundary viscosity, step size 0/1 (fixed, CFL)
+  if ((p->surface_tension_method != 0 && p->surface_tension_method != 2 && p->surface_tension_method != 3) ||
       p->viscosity_method < 0 || p->viscosity_method > 2 || p->viscosity_boundary != 0.0 ||
       (p->cfl_method != 0 && p->cfl_method != 1) ||
       (p->cfl_method == 1 && !(p->cfl_min_time_step_size > 0.0 && p->cfl_max_time_step_size >= p->cfl_min_time_step_size)))
@@ -1515,7 +1519,7 @@ dfr_ctx* dfr_create(const dfr_params* p, int device) {
   // non-pressure forces: Akinci 2013 cohesion / adhesion kernels of radius h
   // (CohesionKernel::setRadius, AdhesionKernel::setRadius, SPHKernels.h:446-521)
   c->fc.h = c->h;
-  c->fc.st_on = p->surface_tension_method == 2 ? 1 : 0;
+  c->fc.st_method = p->surface_tension_method;
   c->fc.st_k = (float)p->surface_tension;
   c->fc.st_kb = (float)p->surface_tension_boundary;
   c->fc.visc_mode = p->viscosity_method;
@@ -1655,9 +1659,10 @@ int dfr_init(dfr_ctx* c) {
     c->f.id = dalloc<uint32_t>(c, NF); c->f.id2 = dalloc<uint32_t>(c, NF);
     c->f.rho = dalloc<float>(c, NF); c->f.alpha = dalloc<float>(c, NF);
     c->f.s = dalloc<float4>(c, NF); c->f.badv = dalloc<float>(c, NF); c->f.kf = dalloc<float>(c, NF);
-    if (c->P.surface_tension_method == 2 || c->P.viscosity_method != 0) {
+    if (c->P.surface_tension_method != 0 || c->P.viscosity_method != 0) {
       c->acc = dalloc<float4>(c, NF);
       if (c->P.surface_tension_method == 2) c->nrm = dalloc<float4>(c, NF);
+      if (c->P.surface_tension_method == 3) { c->he_color = dalloc<float>(c, NF); c->he_gc2 = dalloc<float>(c, NF); }
       set_forces_kernel_attributes();
     }
     if (c->P.cfl_method == 1) c->d_cflmax = dalloc<uint32_t>(c, 1);
@@ -1908,6 +1913,8 @@ int dfr_get_field(const dfr_ctx* c, const char* name, float* out) {
     else if (n == "sum_grad_p_k") download_unpermute<float4>(c, c->f.s, 4, 3, out);
     else if (n == "accel" && c->acc) download_unpermute<float4>(c, c->acc, 4, 3, out);
     else if (n == "normal" && c->nrm) download_unpermute<float4>(c, c->nrm, 4, 3, out);
+    else if (n == "color" && c->he_color) download_unpermute<float>(c, c->he_color, 1, 1, out);
+    else if (n == "gradC2" && c->he_gc2) download_unpermute<float>(c, c->he_gc2, 1, 1, out);
     else return DFR_ERR_ARG;
   } catch (const HipError&) {
     return DFR_ERR_HIP;

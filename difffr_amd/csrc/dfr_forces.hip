@@ -11,7 +11,12 @@
 // and the cohesion kernel C (SPHKernels.h:437-505, including its constant term
 // h^6/64 taken without the kernel's factor, as the reference evaluates it), plus
 // boundary adhesion -kb rho0 V_b A(|r|) r/|r| (SPHKernels.h:508-560; no reaction
-// force on the body, as in the reference).  Viscosity is Standard
+// force on the body, as in the reference).  He et al. 2014
+// (SurfaceTension_He2014.cpp:29-160) is a colour field c_i = (m / rho_i) W(0) +
+// sum_j (m / rho_j) W_ij + sum_b V_b W_ib, gradC2_i = |(1 / c_i) sum_j (m / rho_j)
+// c_j gradW_ij|^2, and per fluid pair (k / 4 rho_i) (m / rho_j) (gradC2_i + gradC2_j)
+// gradW_ij plus per boundary sample (kb / 4 rho_i) V_b gradC2_i gradW_ib, whose
+// reaction -m a goes to the body (BoundaryModel::addForce).  Viscosity is Standard
 // (Viscosity_Standard.cpp:233-269, the reference fluid's default:
 // FluidModel.cpp:100): 10 nu (m_j / rho_j) (v_ij . r_ij) / (|r_ij|^2 + 0.01 h^2)
 // gradW_ij, or XSPH (Viscosity_XSPH.cpp:35-108): -(1/dt) nu (m_j / rho_j)
@@ -95,11 +100,86 @@ k_normals_t(PassArgs a, const float* __restrict__ rho, float4* __restrict__ nrm)
   }
 }
 
+// He 2014 colour field (PASS 0) and |grad c|^2 (PASS 1).  LDS per slot: x, y, z
+// and m / rho_j (PASS 0) or (m / rho_j) c_j (PASS 1).
+template <int PASS>
+__global__ void __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_he_color_t(PassArgs a, const float* __restrict__ rho, const float* __restrict__ color, float* __restrict__ out) {
+  __shared__ TileHdr<FB> T;
+  extern __shared__ float4 lds4[];
+  const StepConst& c = a.c;
+  const uint32_t brick = xcd_block();
+  BrickView B;
+  if (!brick_open<FB>(a.g, a.fstart, a.thdr, brick, a.cap, T, B)) return;
+  const uint32_t total = B.total;
+  const bool in_lds = B.in_lds;
+  float4* LA = lds4;
+  const int t = threadIdx.x;
+  auto weight = [&](uint32_t j) { return PASS == 0 ? c.m_f / rho[j] : (c.m_f / rho[j]) * color[j]; };
+  if (in_lds) {
+    uint32_t js[FT];
+    map_indices<FB>(a.tmap, brick, total, js);
+    float4 q[FT];
+    float ww[FT];
+#pragma unroll
+    for (int r = 0; r < FT; ++r)
+      if (t + r * FB < (int)total) { q[r] = a.pos[js[r]]; ww[r] = weight(js[r]); }
+#pragma unroll
+    for (int r = 0; r < FT; ++r) {
+      const uint32_t s = t + r * FB;
+      if (s < total) LA[s] = make_float4(q[r].x, q[r].y, q[r].z, ww[r]);
+    }
+  }
+  __syncthreads();
+  const size_t stride = (size_t)c.nf;
+  for (uint32_t i = B.own_begin + t; i < B.own_end; i += FB) {
+    const float4 p = a.pos[i];
+    const uint32_t cc = a.cnt[i];
+    const uint32_t nF = cc & 0xffffu, nB = cc >> 16;
+    float sw = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    auto pair = [&](float qx, float qy, float qz, float w) {
+      const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      if (PASS == 0) {
+        sw += w * cubic_W(c.K, sqrtf(r2));
+      } else {
+        const float s = w * cubic_grad_c(c.K, r2);
+        gx += s * rx; gy += s * ry; gz += s * rz;
+      }
+    };
+    if (in_lds) {
+      for_fluid_slots(a.nbr8, stride, i, nF, [&](uint32_t s) {
+        const float4 q = LA[s];
+        pair(q.x, q.y, q.z, q.w);
+      });
+    } else {
+      for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {
+        const uint32_t j = tile_global<FB>(T, s);
+        const float4 q = a.pos[j];
+        pair(q.x, q.y, q.z, weight(j));
+      });
+    }
+    if (PASS == 0) {
+      float sb = 0.f;
+      for_boundary_groups(a.nbrB, stride, i, nB, [&](uint32_t j) { return a.bpos[j]; },
+                          [&](const float4& q, uint32_t) {
+                            const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+                            sb += q.w * cubic_W(c.K, sqrtf(rx * rx + ry * ry + rz * rz));
+                          });
+      out[i] = ((c.m_f / rho[i]) * c.K.w0 + sw) + sb;
+    } else {
+      const float ic = 1.f / color[i];
+      gx *= ic; gy *= ic; gz *= ic;
+      out[i] = gx * gx + gy * gy + gz * gz;
+    }
+  }
+}
+
 // Accelerations a = g + surface tension + viscosity.  LDS per slot: x, y, z,
-// rho | vx, vy, vz, nx | ny, nz.
+// rho | vx, vy, vz, nx (Akinci 2013) or gradC2 (He 2014) | ny, nz.
 __global__ void __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(4, 8)))
-k_forces_t(PassArgs a, const float* __restrict__ rho, const float4* __restrict__ nrm, float4* __restrict__ acc,
-           ForceConst f) {
+k_forces_t(PassArgs a, const float* __restrict__ rho, const float4* __restrict__ nrm, const float* __restrict__ gc2,
+           float4* __restrict__ acc, RigidDev* __restrict__ rig, ForceConst f) {
   __shared__ TileHdr<FB> T;
   extern __shared__ float4 lds4[];
   const StepConst& c = a.c;
@@ -124,7 +204,8 @@ k_forces_t(PassArgs a, const float* __restrict__ rho, const float4* __restrict__
         q[r] = a.pos[js[r]];
         v[r] = a.vel[js[r]];
         rr[r] = rho[js[r]];
-        n[r] = f.st_on ? nrm[js[r]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        n[r] = f.st_method == 2 ? nrm[js[r]]
+               : make_float4(f.st_method == 3 ? gc2[js[r]] : 0.f, 0.f, 0.f, 0.f);
       }
 #pragma unroll
     for (int r = 0; r < FT; ++r) {
@@ -139,11 +220,20 @@ k_forces_t(PassArgs a, const float* __restrict__ rho, const float4* __restrict__
   __syncthreads();
   const size_t stride = (size_t)c.nf;
   const float invH = 1.f / c.dt;
-  for (uint32_t i = B.own_begin + t; i < B.own_end; i += FB) {
+  const bool he = f.st_method == 3;
+  // the particle loop runs the same number of rounds on every lane of a wave, so
+  // He 2014's boundary reactions are reduced over the wave at the end of a round
+  for (uint32_t base = B.own_begin; base < B.own_end; base += FB) {
+    const uint32_t i = base + t;
+    int cb = -1;
+    double F0 = 0, F1 = 0, F2 = 0, T0 = 0, T1 = 0, T2 = 0;
+    if (i < B.own_end) {
     const float4 p = a.pos[i];
     const float4 vi = a.vel[i];
     const float ri = rho[i];
-    const float4 ni = f.st_on ? nrm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 ni = f.st_method == 2 ? nrm[i]
+                      : make_float4(he ? gc2[i] : 0.f, 0.f, 0.f, 0.f);
+    const float hf = 0.25f * f.st_k / ri;  // He 2014's factor k / (4 rho_i)
     const uint32_t cc = a.cnt[i];
     const uint32_t nF = cc & 0xffffu, nB = cc >> 16;
     float sx = 0.f, sy = 0.f, sz = 0.f;  // surface tension
@@ -152,7 +242,11 @@ k_forces_t(PassArgs a, const float* __restrict__ rho, const float4* __restrict__
                     float njz) {
       const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
       const float r2 = rx * rx + ry * ry + rz * rz;
-      if (f.st_on) {
+      if (he) {
+        // njx holds gradC2_j
+        const float s = hf * (c.m_f / rj) * (ni.x + njx) * cubic_grad_c(c.K, r2);
+        sx += s * rx; sy += s * ry; sz += s * rz;
+      } else if (f.st_method == 2) {
         const float K_ij = 2.f * c.rho0 / (ri + rj);
         float ax = 0.f, ay = 0.f, az = 0.f;
         if (r2 > 1.0e-9f) {
@@ -185,12 +279,40 @@ k_forces_t(PassArgs a, const float* __restrict__ rho, const float4* __restrict__
       for_fluid_slots_rolled(a.nbr8, stride, i, nF, [&](uint32_t s) {
         const uint32_t j = tile_global<FB>(T, s);
         const float4 q = a.pos[j], w = a.vel[j];
-        const float4 nj = f.st_on ? nrm[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 nj = f.st_method == 2 ? nrm[j] : make_float4(he ? gc2[j] : 0.f, 0.f, 0.f, 0.f);
         pair(q.x, q.y, q.z, rho[j], w.x, w.y, w.z, nj.x, nj.y, nj.z);
       });
     }
-    float bx = 0.f, by = 0.f, bz = 0.f;  // boundary adhesion
-    if (f.st_on && f.st_kb != 0.f) {
+    float bx = 0.f, by = 0.f, bz = 0.f;  // boundary adhesion (Akinci 2013) or boundary term (He 2014)
+    if (he && f.st_kb != 0.f) {
+      const float hb = 0.25f * f.st_kb / ri * ni.x;  // kb gradC2_i / (4 rho_i)
+      for_boundary_groups(a.nbrB, stride, i, nB, [&](uint32_t j) { return a.bpos[j]; },
+                          [&](const float4& q, uint32_t j) {
+                            const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+                            const float s = hb * q.w * cubic_grad_c(c.K, rx * rx + ry * ry + rz * rz);
+                            const float ax = s * rx, ay = s * ry, az = s * rz;
+                            bx += ax; by += ay; bz += az;
+                            if ((int)j >= a.nb_static) {
+                              const int body = (int)a.bx0[j].w;
+                              if (rig[body].dynamic) {
+                                if (body != cb && cb >= 0) {
+                                  atomicAdd(&rig[cb].F[0], F0); atomicAdd(&rig[cb].F[1], F1);
+                                  atomicAdd(&rig[cb].F[2], F2);
+                                  atomicAdd(&rig[cb].T[0], T0); atomicAdd(&rig[cb].T[1], T1);
+                                  atomicAdd(&rig[cb].T[2], T2);
+                                  F0 = F1 = F2 = T0 = T1 = T2 = 0.0;
+                                }
+                                cb = body;
+                                const float fx = -c.m_f * ax, fy = -c.m_f * ay, fz = -c.m_f * az;
+                                const double rxd = (double)q.x - rig[body].x[0], ryd = (double)q.y - rig[body].x[1],
+                                             rzd = (double)q.z - rig[body].x[2];
+                                F0 += fx; F1 += fy; F2 += fz;
+                                T0 += ryd * fz - rzd * fy; T1 += rzd * fx - rxd * fz; T2 += rxd * fy - ryd * fx;
+                              }
+                            }
+                          });
+    }
+    if (f.st_method == 2 && f.st_kb != 0.f) {
       for_boundary_groups(a.nbrB, stride, i, nB, [&](uint32_t j) { return a.bpos[j]; },
                           [&](const float4& q, uint32_t) {
                             const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
@@ -203,6 +325,26 @@ k_forces_t(PassArgs a, const float* __restrict__ rho, const float4* __restrict__
                           });
     }
     acc[i] = make_float4(((c.g[0] + sx) + bx) + vx, ((c.g[1] + sy) + by) + vy, ((c.g[2] + sz) + bz) + vz, 0.f);
+    }
+    if (he && f.st_kb != 0.f) {
+      // wave-level FP64 reduction of the round's reactions, one body at a time
+      while (true) {
+        const unsigned long long m = __ballot(cb >= 0);
+        if (m == 0ull) break;
+        const int src = __ffsll((long long)m) - 1;
+        const int b0 = __shfl(cb, src, 64);
+        const bool mine = (cb == b0);
+        const double a0 = wave_sum_d(mine ? F0 : 0.0), a1 = wave_sum_d(mine ? F1 : 0.0),
+                     a2 = wave_sum_d(mine ? F2 : 0.0);
+        const double a3 = wave_sum_d(mine ? T0 : 0.0), a4 = wave_sum_d(mine ? T1 : 0.0),
+                     a5 = wave_sum_d(mine ? T2 : 0.0);
+        if ((threadIdx.x & 63) == 0) {
+          atomicAdd(&rig[b0].F[0], a0); atomicAdd(&rig[b0].F[1], a1); atomicAdd(&rig[b0].F[2], a2);
+          atomicAdd(&rig[b0].T[0], a3); atomicAdd(&rig[b0].T[1], a4); atomicAdd(&rig[b0].T[2], a5);
+        }
+        if (mine) cb = -1;
+      }
+    }
   }
 }
 
@@ -255,11 +397,20 @@ void launch_normals_t(const StepConst& c, const GridDesc& g, GridBufs& gb, Fluid
                      make_args(c, g, gb, f, b, nb), f.rho, nrm);
 }
 
+void launch_he_color_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
+                       NbrBufs& nb, float* color, float* gc2, hipStream_t s) {
+  const PassArgs a = make_args(c, g, gb, f, b, nb);
+  const size_t shm = sizeof(float4) * (size_t)nb.cap;
+  hipLaunchKernelGGL(k_he_color_t<0>, dim3(nbricks(g)), dim3(FB), shm, s, a, f.rho, nullptr, color);
+  hipLaunchKernelGGL(k_he_color_t<1>, dim3(nbricks(g)), dim3(FB), shm, s, a, f.rho, color, gc2);
+}
+
 void launch_forces_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
-                     NbrBufs& nb, const float4* nrm, float4* acc, const ForceConst& fc, hipStream_t s) {
+                     NbrBufs& nb, const float4* nrm, const float* gc2, float4* acc, RigidDev* rig,
+                     const ForceConst& fc, hipStream_t s) {
   const size_t shm = (2 * sizeof(float4) + sizeof(float2)) * (size_t)nb.cap;
-  hipLaunchKernelGGL(k_forces_t, dim3(nbricks(g)), dim3(FB), shm, s, make_args(c, g, gb, f, b, nb), f.rho, nrm, acc,
-                     fc);
+  hipLaunchKernelGGL(k_forces_t, dim3(nbricks(g)), dim3(FB), shm, s, make_args(c, g, gb, f, b, nb), f.rho, nrm, gc2,
+                     acc, rig, fc);
 }
 
 void launch_cfl_max(const StepConst& c, const float4* vel, const float4* acc, int nb, const float4* bvel,

@@ -101,14 +101,19 @@ struct ForceConst {
   float h;              // support radius (the cohesion and adhesion kernels' radius)
   float st_k, st_kb;    // surface tension (cohesion, curvature) and boundary adhesion coefficients
   float visc;           // viscosity coefficient
-  int st_on, visc_mode; // visc_mode: 0 none, 1 Standard, 2 XSPH
+  int st_method;        // 0 none, 2 Akinci 2013, 3 He 2014
+  int visc_mode;        // 0 none, 1 Standard, 2 XSPH
   float ck, cc;         // cohesion kernel: 32 / (pi h^9), h^6 / 64
   float ak;             // adhesion kernel: 0.007 / h^3.25
 };
 void launch_normals_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
                       NbrBufs& nb, float4* nrm, hipStream_t s);
+// He 2014: the colour field, then |grad c|^2 (two gather passes).
+void launch_he_color_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
+                       NbrBufs& nb, float* color, float* gc2, hipStream_t s);
 void launch_forces_t(const StepConst& c, const GridDesc& g, GridBufs& gb, FluidBufs& f, BoundaryBufs& b,
-                     NbrBufs& nb, const float4* nrm, float4* acc, const ForceConst& fc, hipStream_t s);
+                     NbrBufs& nb, const float4* nrm, const float* gc2, float4* acc, RigidDev* rig,
+                     const ForceConst& fc, hipStream_t s);
 void launch_cfl_max(const StepConst& c, const float4* vel, const float4* acc, int nb, const float4* bvel,
                     uint32_t* out, hipStream_t s);
 void launch_advect_acc(const StepConst& c, FluidBufs& f, const float4* acc, float kappav_scale, hipStream_t s);

@@ -183,7 +183,7 @@ def read_scene_file(path: str) -> SceneFile:
     for m in materials:
         mat = dict(_MATERIAL_DEFAULTS)
         mat.update(m)
-        if int(mat["surfaceTensionMethod"]) not in (0, 2):
+        if int(mat["surfaceTensionMethod"]) not in (0, 2, 3):
             unsupported.append("Materials.surfaceTensionMethod")
         if int(mat["viscosityMethod"]) not in (0, 1, 2) or float(mat["viscosityBoundary"]) != 0.0:
             unsupported.append("Materials.viscosityMethod")

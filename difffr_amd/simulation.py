@@ -204,7 +204,8 @@ class Simulation:
     def field(self, name: str) -> np.ndarray:
         """Per-particle fields in particle-id order: density, factor, density_adv,
         kappa, kappa_v; (n, 3): sum_grad_p_k, and with non-pressure forces on, the
-        last step's accelerations (accel = g + forces) and Akinci 2013 normals."""
+        last step's accelerations (accel = g + forces) and Akinci 2013 normals;
+        with He 2014 surface tension its colour field "color" and "gradC2"."""
         k = 3 if name in ("sum_grad_p_k", "accel", "normal") else 1
         out = np.empty(self.n * k, np.float32)
         self._check(native.load().dfr_get_field(self.ctx, name.encode(), _fp(out)))

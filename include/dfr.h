@@ -69,7 +69,9 @@ typedef struct dfr_params {
   /* Non-pressure forces after the divergence solve (Simulation::computeNonPressureForces,
      Simulation.cpp:618-631; method numbers as registered in NonPressureForceRegistration.cpp:33-60).
      The exact adjoint (dfr_adjoint_begin) does not cover them, nor CFL step sizes. */
-  int32_t surface_tension_method;  /* 0 none, 2 Akinci et al. 2013 (SurfaceTension_Akinci2013.cpp) */
+  int32_t surface_tension_method;  /* 0 none, 2 Akinci et al. 2013 (SurfaceTension_Akinci2013.cpp),
+                                      3 He et al. 2014 (SurfaceTension_He2014.cpp): the reference's
+                                      numbering, NonPressureForceRegistration.cpp:42-47 */
   double surface_tension;          /* cohesion and curvature coefficient (default 0.05) */
   double surface_tension_boundary; /* adhesion coefficient (default 0.01) */
   int32_t viscosity_method;        /* 0 none, 1 Standard (Viscosity_Standard.cpp; the reference fluid's default,
@@ -204,7 +206,8 @@ int dfr_get_fluid(const dfr_ctx* ctx, float* x, float* v);
 /* A per-particle field in particle-id order: "density", "factor",
  * "density_adv", "kappa", "kappa_v" (n floats) or "sum_grad_p_k" (3n); with
  * non-pressure forces on, "accel" (the last step's FluidModel accelerations,
- * g + forces, 3n) and, with surface tension, "normal" (Akinci 2013, 3n). */
+ * g + forces, 3n) and, with surface tension, "normal" (Akinci 2013, 3n) or "color"
+ * and "gradC2" (He 2014, n each). */
 int dfr_get_field(const dfr_ctx* ctx, const char* name, float* out);
 
 /* Overwrites "kappa" or "kappa_v" (n floats) or "x" / "v" (3n floats: positions,

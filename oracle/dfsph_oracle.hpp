@@ -54,7 +54,7 @@ struct Params {
   int useGradientManager = 0;    // useRigidGradientManager, Simulation.cpp:123
   // Non-pressure forces (Simulation::computeNonPressureForces, Simulation.cpp:618-631;
   // method indices in registration order, NonPressureForceRegistration.cpp:33-60)
-  int surfaceTensionMethod = 0;  // 0 none, 2 Akinci et al. 2013
+  int surfaceTensionMethod = 0;  // 0 none, 2 Akinci et al. 2013, 3 He et al. 2014 (NonPressureForceRegistration.cpp:42-47)
   double surfaceTension = 0.05;  // SurfaceTensionBase.cpp:12
   double surfaceTensionBoundary = 0.01;
   int viscosityMethod = 0;       // 0 none, 1 Standard (FluidModel.cpp:100 makes it a fluid's default), 2 XSPH
@@ -303,6 +303,7 @@ template <typename RF, typename RR> class Sim {
   // fluid (reference AoS layout, FluidModel.h:110-151)
   std::vector<V3<RF>> x, v, s;
   std::vector<V3<RF>> accel, normal;  // FluidModel accelerations, Akinci 2013 normals
+  std::vector<RF> color, gradC2;      // He 2014 colour field and |grad c|^2
   std::vector<RF> rho, alpha, bAdv, kappa, kappaV;
   std::vector<Body<RF, RR>> bodies;
   // neighbour lists per point-set pair
@@ -1119,6 +1120,7 @@ template <typename RF, typename RR> class Sim {
     const long N = long(x.size());
     accel.assign(N, g);
     if (P.surfaceTensionMethod == 2) surface_tension_akinci2013();
+    if (P.surfaceTensionMethod == 3) surface_tension_he2014();
     if (P.viscosityMethod == 1) viscosity_standard();
     if (P.viscosityMethod == 2) viscosity_xsph();
   }
@@ -1177,6 +1179,48 @@ template <typename RF, typename RR> class Sim {
         }
     }
   }
+  // SurfaceTension_He2014::step (SurfaceTension_He2014.cpp:29-160) with Akinci
+  // 2012 boundaries: colour c_i = (m / rho_i) W(0) + sum_j (m / rho_j) W_ij +
+  // sum_b V_b W_ib; its gradient (1 / c_i) sum_j (m / rho_j) c_j gradW_ij and
+  // gradC2_i = |.|^2; then a_i += (k / 4 rho_i) sum_j (m / rho_j) (gradC2_i +
+  // gradC2_j) gradW_ij + (kb / 4 rho_i) sum_b V_b gradC2_i gradW_ib, the body
+  // receiving -m a of each boundary term (BoundaryModel::addForce).
+  void surface_tension_he2014() {
+    const long N = long(x.size());
+    color.assign(N, RF(0));
+    gradC2.assign(N, RF(0));
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      RF ci = m_f / rho[i] * K.w0;
+      for (uint32_t j : nFF[i]) ci += m_f / rho[j] * K.W(x[i] - x[j]);
+      for (size_t bi = 0; bi < bodies.size(); ++bi)
+        for (uint32_t j : nFB[bi][i]) ci += bodies[bi].V[j] * K.W(x[i] - bodies[bi].x[j]);
+      color[i] = ci;
+    }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      V3<RF> g;
+      for (uint32_t j : nFF[i]) g += K.gradW(x[i] - x[j]) * (m_f / rho[j] * color[j]);
+      g = g * (RF(1.0) / color[i]);
+      gradC2[i] = g.squaredNorm();
+    }
+    const RF k = RF(P.surfaceTension), kb = RF(P.surfaceTensionBoundary);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      V3<RF>& ai = accel[i];
+      const RF factor = RF(0.25) * k / rho[i], factorb = RF(0.25) * kb / rho[i];
+      for (uint32_t j : nFF[i])
+        ai += K.gradW(x[i] - x[j]) * (factor * m_f / rho[j] * (gradC2[i] + gradC2[j]));
+      for (size_t bi = 0; bi < bodies.size(); ++bi) {
+        auto& b = bodies[bi];
+        for (uint32_t j : nFB[bi][i]) {
+          const V3<RF> a = K.gradW(x[i] - b.x[j]) * (factorb * b.V[j] * gradC2[i]);
+          ai += a;
+          add_force(b, b.x[j], a * (-m_f));
+        }
+      }
+    }
+  }
   // Viscosity_XSPH::step (Viscosity_XSPH.cpp:35-108), fluid part.
   void viscosity_xsph() {
     const long N = long(x.size());
@@ -1209,7 +1253,7 @@ template <typename RF, typename RR> class Sim {
     hn = std::max(hn, RF(P.cflMinTimeStepSize));
     return hn;
   }
-  bool nonpressure_on() const { return P.surfaceTensionMethod == 2 || P.viscosityMethod != 0 || P.cflMethod == 1; }
+  bool nonpressure_on() const { return P.surfaceTensionMethod == 2 || P.surfaceTensionMethod == 3 || P.viscosityMethod != 0 || P.cflMethod == 1; }
 
   // One full step in SimulatorBase::timeStepNoGUI order (SimulatorBase.cpp:1142-1200)
   // around TimeStepDiffDFSPH::step (TimeStepDiffDFSPH.cpp:526-652).

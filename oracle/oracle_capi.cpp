@@ -83,7 +83,9 @@ template <typename RF, typename RR> struct Oracle : OracleBase {
     else if (n == "density_adv") f = &sim.bAdv;
     else if (n == "kappa") f = &sim.kappa;
     else if (n == "kappa_v") f = &sim.kappaV;
-    if (f) { for (size_t i = 0; i < N; ++i) out[i] = double((*f)[i]); return 0; }
+    else if (n == "color") f = &sim.color;     // He 2014 (zero when it did not run)
+    else if (n == "gradC2") f = &sim.gradC2;
+    if (f) { for (size_t i = 0; i < N; ++i) out[i] = i < f->size() ? double((*f)[i]) : 0.0; return 0; }
     if (n == "sum_grad_p_k") {
       for (size_t i = 0; i < N; ++i) for (int k = 0; k < 3; ++k) out[3 * i + k] = double(sim.s[i][k]);
       return 0;

@@ -1,7 +1,7 @@
 """Non-pressure forces and the CFL step size on the GPU against the oracle
 (Simulation::computeNonPressureForces, Simulation.cpp:618-631, and
-updateTimeStepSizeCFL, :544-616): Akinci 2013 surface tension, Standard and
-XSPH viscosity, CFL with the step-size bookkeeping of TimeStepDiffDFSPH::step
+updateTimeStepSizeCFL, :544-616): Akinci 2013 and He 2014 surface tension
+(with He 2014's boundary reaction on the body), Standard and XSPH viscosity, CFL with the step-size bookkeeping of TimeStepDiffDFSPH::step
 (the step's starting size advects velocities and positions and advances the
 time; the new size runs the pressure solve, the rigid bodies and the next step).
 
@@ -40,11 +40,14 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-@pytest.mark.parametrize("visc", [1, 2])
-def test_one_step_accelerations_and_step_size(gpu_ready, visc):
+@pytest.mark.parametrize("st,visc", [(2, 1), (2, 2), (3, 1)])
+def test_one_step_accelerations_and_step_size(gpu_ready, st, visc):
+    """Surface tension 2 = Akinci 2013 (normals compared), 3 = He 2014 (colour
+    field and |grad c|^2 compared; its boundary term pushes the body)."""
     sc = _scene()
     sc.params["viscosity_method"] = visc
-    print(f"[non-pressure one step] viscosity method {visc}, n={sc.n_fluid}", flush=True)
+    sc.params["surface_tension_method"] = st
+    print(f"[non-pressure one step] surface tension {st}, viscosity method {visc}, n={sc.n_fluid}", flush=True)
     g = Simulation(sc)
     o = OracleSim(sc, "f32_rigid64")
     g.step(1)
@@ -52,10 +55,13 @@ def test_one_step_accelerations_and_step_size(gpu_ready, visc):
     o.set_fixed_iterations([it["iterations_v"]], [it["iterations"]])
     o.step(1)
     ag, ao = g.field("accel"), o.field("accel")
-    ng, no = g.field("normal"), o.field("normal")
     e_a = np.abs(ag - ao).max() / np.abs(ao - np.asarray(sc.params["gravity"])).max()
-    e_n = np.abs(ng - no).max() / np.abs(no).max()
-    print(f"  accel err {e_a:.2e} of the force scale, normal err {e_n:.2e}, dt GPU {it['dt']:.7g} "
+    if st == 2:
+        ng, no = g.field("normal"), o.field("normal")
+        e_n = np.abs(ng - no).max() / np.abs(no).max()
+    else:
+        e_n = max(np.abs(g.field(f) - o.field(f)).max() / np.abs(o.field(f)).max() for f in ("color", "gradC2"))
+    print(f"  accel err {e_a:.2e} of the force scale, normal / colour err {e_n:.2e}, dt GPU {it['dt']:.7g} "
           f"oracle {o.field('dt'):.7g}", flush=True)
     assert e_a < 1e-5 and e_n < 1e-5
     assert it["dt"] == pytest.approx(o.field("dt"), rel=1e-5)
@@ -63,9 +69,11 @@ def test_one_step_accelerations_and_step_size(gpu_ready, visc):
     g.close()
 
 
-def test_rollout_with_forces_and_cfl(gpu_ready):
+@pytest.mark.parametrize("st", [2, 3])
+def test_rollout_with_forces_and_cfl(gpu_ready, st):
     sc = _scene()
-    print(f"[non-pressure rollout] {sc.steps} steps, n={sc.n_fluid}", flush=True)
+    sc.params["surface_tension_method"] = st
+    print(f"[non-pressure rollout] surface tension {st}, {sc.steps} steps, n={sc.n_fluid}", flush=True)
     g = Simulation(sc)
     o = OracleSim(sc, "f32_rigid64")
     o64 = OracleSim(sc, "f64")
@@ -94,6 +102,40 @@ def test_rollout_with_forces_and_cfl(gpu_ready):
     # within the precision floor of the step (its FP32 and FP64 runs' spread)
     assert e < max(1e-4, 3 * floor)
     assert rms(xg, xo) < max(1e-4, 3 * rms(xo, x64))
+    g.close()
+
+
+def test_he2014_boundary_reaction_matches_oracle(gpu_ready):
+    """He 2014's boundary term and its reaction on a dynamic ball (fluid term
+    off, fluid below rest density, no divergence solve: the body's force and
+    torque are the reactions alone) against the oracle, within 1e-4 relative
+    (FP32 terms summed in FP64 in a different order), and equal to
+    -sum_i m (a_i - g) of the GPU's own accelerations."""
+    r = 0.025
+    fluid = scenes.block_lattice((-0.3, -0.3, -0.3), (13, 13, 13), 3 * r)
+    fluid = fluid[np.linalg.norm(fluid, axis=1) > 0.12 + 2 * r]
+    p = scenes.default_params(r, 1e-3)
+    # zero gravity: the FP32 accelerations then hold the boundary terms without cancellation against g
+    p.update(compute_gradient=0, surface_tension_method=3, surface_tension=0.0, surface_tension_boundary=0.5,
+             enable_divergence_solver=0, gravity=(0.0, 0.0, 0.0))
+    body = scenes.Body(scenes.sphere_surface(0.12, 2 * r), True, np.array([0.01, -0.02, 0.015]),
+                       np.array([1.0, 0, 0, 0]), 800.0)
+    sc = scenes.Scene("ball", p, fluid, np.zeros_like(fluid), [body], 1)
+    print(f"[He 2014 boundary reaction] n={sc.n_fluid}", flush=True)
+    g = Simulation(sc)
+    o = OracleSim(sc, "f32_rigid64")
+    g.step(1)
+    it = g.stats()
+    o.set_fixed_iterations([it["iterations_v"]], [it["iterations"]])
+    o.step(1)
+    rg, ro = g.rigid(0), o.rigid(0)
+    m = 0.8 * (2 * r) ** 3 * 1000.0
+    own = -m * g.field("accel").astype(np.float64).sum(0)
+    print(f"  force GPU {rg['force']} oracle {ro['force']}; torque GPU {rg['torque']} oracle {ro['torque']}",
+          flush=True)
+    assert np.abs(ro["force"]).max() > 0 and np.abs(ro["torque"]).max() > 0
+    assert rel(rg["force"], ro["force"]) < 1e-4 and rel(rg["torque"], ro["torque"]) < 1e-4
+    assert rel(rg["force"], own) < 1e-4
     g.close()
 
 

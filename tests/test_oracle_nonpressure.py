@@ -206,3 +206,137 @@ def test_surface_tension_two_particles_closed_form():
         expect = K * (-k * m * Cd[0] - k * 2 * (n1 @ e))
         assert a[0] @ e == pytest.approx(expect, rel=1e-9)
         assert np.abs(a[0] + a[1]).max() <= 1e-12 * np.abs(a[0]).max()  # equal and opposite
+
+
+def _he_cube(n, k=0.2, kb=0.0):
+    return _cube(dict(surface_tension_method=3, surface_tension=k, surface_tension_boundary=kb), n=n)
+
+
+def test_he2014_colour_field_and_invariants():
+    """He et al. 2014 (SurfaceTension_He2014.cpp:29-160) on a free 13 x 13 x 13
+    lattice cube in zero gravity: where a particle's neighbours all have complete
+    neighbourhoods (at least two kernel radii inside) the colour field is exactly
+    the Shepard sum 1 (rho_i = rho_j = rho0 V (W(0) + sum W)); dropping its W(0)
+    term or a density factor breaks this.  |grad c|^2 vanishes at the centre by
+    symmetry and peaks at the surface; the fluid pair force conserves momentum
+    and pulls the face centres inward."""
+    sc = _he_cube(13)
+    o, a = _accel(sc)
+    c, g2 = o.field("color"), o.field("gradC2")
+    x = sc.fluid_x
+    sp = 2 * 0.025
+    idx = np.rint((x - x.min(0)) / sp).astype(int)
+    deep = np.all((idx >= 4) & (idx <= 8), axis=1)
+    assert deep.sum() == 125
+    np.testing.assert_allclose(c[deep], 1.0, rtol=0, atol=1e-12)
+    assert np.abs(c[~deep] - 1.0).max() > 1e-2  # near the surface the sum is not a partition of unity
+    centre, faces = _centre_and_faces(x)
+    assert g2[centre] <= 1e-20 * g2.max()
+    assert np.argmax(g2) in np.flatnonzero(~deep)
+    scale = np.abs(a).max()
+    assert scale > 0
+    assert np.abs(a.sum(0)).max() <= 1e-10 * np.abs(a).sum()
+    assert np.abs(a[centre]).max() <= 1e-9 * scale
+    for i, kk, sgn in faces:
+        assert sgn * a[i, kk] < -0.05 * scale, (i, kk, sgn, a[i])
+    # linear in the coefficient
+    _, a2 = _accel(_he_cube(13, k=0.4))
+    np.testing.assert_allclose(a2, 2 * a, rtol=1e-12, atol=1e-15)
+    # a jittered cube has no symmetry left: momentum is kept because each pair's
+    # terms are equal and opposite (equal masses)
+    sc = _he_cube(9)
+    sc.fluid_x = sc.fluid_x + np.random.default_rng(5).uniform(-0.004, 0.004, sc.fluid_x.shape)
+    _, a = _accel(sc)
+    assert np.abs(a.sum(0)).max() <= 1e-10 * np.abs(a).sum()
+
+
+def test_he2014_three_particles_brute_force():
+    """Three fluid particles at unequal distances (colours and densities all
+    differ): density, colour, grad c, |grad c|^2 and the acceleration evaluated
+    by direct summation of He et al. 2014's definitions (colour
+    c_i = sum_j (m / rho_j) W_ij including j = i, grad c_i = (1 / c_i) sum_j
+    (m / rho_j) c_j gradW_ij; the force (k / 4 rho_i) sum_j V_j (|grad c_i|^2 +
+    |grad c_j|^2) gradW_ij) from the kernel's own evaluation."""
+    r, k = 0.025, 0.3
+    h = 4 * r
+    x = np.array([[0.0, 0.0, 0.0], [0.35 * h, 0.1 * h, 0.0], [0.2 * h, 0.75 * h, 0.15 * h]])
+    plate = scenes.box_surface((5.0, 5.0, 5.0), (5.4, 5.05, 5.4), 2 * r)
+    p = scenes.default_params(r, 1e-3)
+    p.update(gravity=(0.0, 0.0, 0.0), compute_gradient=0, surface_tension_method=3, surface_tension=k,
+             surface_tension_boundary=0.0, enable_divergence_solver=0)
+    sc = scenes.Scene("three", p, x, np.zeros_like(x),
+                      [scenes.Body(plate, False, np.zeros(3), np.array([1.0, 0, 0, 0]), 1000.0)], 1)
+    o = OracleSim(sc, "f64")
+    o.step(1)
+    m = 0.8 * (2 * r) ** 3 * 1000.0
+    Wm = np.zeros((3, 3))
+    G = np.zeros((3, 3, 3))
+    for i in range(3):
+        W, gW, _ = po.kernel_eval(h, x[i] - x)
+        Wm[i], G[i] = W, gW
+    rho = m * Wm.sum(1)
+    col = (Wm * (m / rho)[None, :]).sum(1)
+    grad = np.einsum("j,ijk->ik", m / rho * col, G) / col[:, None]
+    g2 = (grad ** 2).sum(1)
+    acc = np.einsum("ij,ijk->ik", (k / (4 * rho))[:, None] * (m / rho)[None, :] * (g2[:, None] + g2[None, :]), G)
+    np.testing.assert_allclose(o.field("density"), rho, rtol=1e-12)
+    assert np.ptp(col) > 1e-2 and np.ptp(rho) > 1.0
+    np.testing.assert_allclose(o.field("color"), col, rtol=1e-12)
+    np.testing.assert_allclose(o.field("gradC2"), g2, rtol=1e-10)
+    np.testing.assert_allclose(o.field("accel"), acc, rtol=1e-10, atol=1e-12 * np.abs(acc).max())
+
+
+def test_he2014_two_particles_closed_form():
+    """Two fluid particles at distance d < h (a far plate gives the scene its
+    boundary): both colours are 1 (the Shepard sum of a symmetric pair),
+    grad c_1 = (m / rho) gradW(x1 - x2), gradC2 = (m / rho)^2 |gradW|^2 for both,
+    and a_1 = (k / 4 rho) (m / rho) 2 gradC2 gradW(x1 - x2) = -a_2."""
+    r, k = 0.025, 0.3
+    h = 4 * r
+    for d in (0.3 * h, 0.7 * h):
+        x = np.array([[0.0, 0.0, 0.0], [d, 0.0, 0.0]])
+        plate = scenes.box_surface((5.0, 5.0, 5.0), (5.4, 5.05, 5.4), 2 * r)
+        p = scenes.default_params(r, 1e-3)
+        p.update(gravity=(0.0, 0.0, 0.0), compute_gradient=0, surface_tension_method=3, surface_tension=k,
+                 surface_tension_boundary=0.0, enable_divergence_solver=0)
+        sc = scenes.Scene("pair", p, x, np.zeros_like(x),
+                          [scenes.Body(plate, False, np.zeros(3), np.array([1.0, 0, 0, 0]), 1000.0)], 1)
+        o = OracleSim(sc, "f64")
+        o.step(1)
+        V = 0.8 * (2 * r) ** 3
+        m = V * 1000.0
+        W, gW, _ = po.kernel_eval(h, np.array([[0.0, 0.0, 0.0], [-d, 0.0, 0.0]]))
+        rho = 1000.0 * V * (W[0] + W[1])
+        np.testing.assert_allclose(o.field("color"), 1.0, rtol=1e-12)
+        g2 = (m / rho) ** 2 * (gW[1] @ gW[1])
+        np.testing.assert_allclose(o.field("gradC2"), g2, rtol=1e-10)
+        expect = (k / (4 * rho)) * (m / rho) * 2 * g2 * gW[1]
+        a = o.field("accel")
+        np.testing.assert_allclose(a[0], expect, rtol=1e-10, atol=1e-14 * np.abs(expect).max())
+        np.testing.assert_allclose(a[1], -expect, rtol=1e-10, atol=1e-14 * np.abs(expect).max())
+
+
+def test_he2014_boundary_reaction_on_the_body():
+    """The boundary term of He 2014 pushes the fluid and, by
+    BoundaryModel::addForce, the dynamic body with -m a: with the fluid term off
+    (k = 0), no viscosity, the divergence solve off and a fluid sparse enough to
+    stay below rest density (no pressure forces), the body's force after one step
+    is exactly -sum_i m (a_i - g)."""
+    r = 0.025
+    sp = 3 * r  # 1.5 diameters: rho < rho0 everywhere
+    fluid = scenes.block_lattice((-0.3, -0.3, -0.3), (13, 13, 13), sp)
+    ball = scenes.sphere_surface(0.12, 2 * r)
+    fluid = fluid[np.linalg.norm(fluid, axis=1) > 0.12 + 2 * r]
+    p = scenes.default_params(r, 1e-3)
+    p.update(compute_gradient=0, surface_tension_method=3, surface_tension=0.0, surface_tension_boundary=0.5,
+             enable_divergence_solver=0)
+    body = scenes.Body(ball, True, np.zeros(3), np.array([1.0, 0, 0, 0]), 800.0)
+    sc = scenes.Scene("ball", p, fluid, np.zeros_like(fluid), [body], 1)
+    o = OracleSim(sc, "f64")
+    o.step(1)
+    assert o.field("density").max() < 1000.0
+    m = 0.8 * (2 * r) ** 3 * 1000.0
+    a = o.field("accel") - np.asarray(p["gravity"])
+    F = o.rigid(0)["force"]
+    assert np.abs(F).max() > 0
+    np.testing.assert_allclose(F, -m * a.sum(0), rtol=1e-10, atol=1e-12 * np.abs(F).max())

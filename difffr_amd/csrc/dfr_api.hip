@@ -211,6 +211,11 @@ struct dfr_ctx {
   float h = 0, dtf = 0;       // dtf: the TimeManager's step size (the next step's start)
   float4* acc = nullptr;      // non-pressure accelerations (g + forces) of the current step
   float *he_color = nullptr, *he_gc2 = nullptr;  // He 2014 colour field and |grad c|^2
+  // rigid-rigid contact: rest volume / density per boundary sample, per-sample force and torque
+  float *cvol0 = nullptr, *cdens0 = nullptr;
+  float4 *cforce = nullptr, *ctorque = nullptr;
+  bool contact_ready = false;  // rest values computed (invariant under the bodies' rigid motion)
+  KernelConst Kc{};            // cubic spline of radius rigid_contact_support_radius_factor r
   float4* nrm = nullptr;      // Akinci 2013 normals
   uint32_t* d_cflmax = nullptr;  // CFL squared-speed bound (float bits)
   ForceConst fc{};
@@ -886,6 +891,24 @@ void do_step(dfr_ctx* c) {
     rc.compute_gradient = c->replay ? 0 : c->P.compute_gradient;
     rc.nbodies = (int)c->bodies.size();
     rc.use_manager = c->P.use_gradient_manager;
+    if (c->P.use_rigid_contact) {
+      // RigidBody3dBoundarySimulator::velocityTimeStep with the contact solver (:284-298)
+      const float hh = c->h * c->h;
+      if (!c->contact_ready) {
+        launch_contact_rest(c->b, c->grid, c->gb, hh, c->Kc, (float)c->P.rigid_contact_gamma, c->cvol0, c->cdens0,
+                            c->stream);
+        c->contact_ready = true;
+      }
+      ContactConst cc{};
+      cc.dt = (double)dtp;
+      for (int k = 0; k < 3; ++k) { cc.g[k] = rc.g[k]; cc.gdt[k] = rc.g[k] * (double)dtp; }
+      cc.beta = (float)c->P.rigid_contact_beta;
+      cc.mu = (float)c->P.rigid_contact_friction;
+      cc.gyroscopic = c->P.gyroscopic;
+      launch_contact(c->b, c->grid, c->gb, hh, c->Kc, cc, c->cvol0, c->cdens0, c->cforce, c->ctorque, c->d_rig,
+                     rc.nbodies, c->stream);
+      rc.contact = 1;
+    }
     launch_rigid_step(rc, c->d_rig, c->stream);
     launch_boundary_transform(c->b.dyn_idx, c->b.nd, c->b, c->d_rig, c->stream);
   }
@@ -1475,6 +1498,11 @@ void dfr_default_params(dfr_params* p, double radius, double dt) {
   p->cfl_factor = 0.5;
   p->cfl_min_time_step_size = 0.0001;
   p->cfl_max_time_step_size = 0.005;
+  p->use_rigid_contact = 0;             // SceneLoader.cpp:58-71
+  p->rigid_contact_beta = 1.0;
+  p->rigid_contact_gamma = 0.7;
+  p->rigid_contact_support_radius_factor = 4.0;
+  p->rigid_contact_friction = 0.0;
 }
 
 dfr_ctx* dfr_create(const dfr_params* p, int device) {
@@ -1484,7 +1512,8 @@ dfr_ctx* dfr_create(const dfr_params* p, int device) {
   if ((p->surface_tension_method != 0 && p->surface_tension_method != 2 && p->surface_tension_method != 3) ||
       p->viscosity_method < 0 || p->viscosity_method > 2 || p->viscosity_boundary != 0.0 ||
       (p->cfl_method != 0 && p->cfl_method != 1) ||
-      (p->cfl_method == 1 && !(p->cfl_min_time_step_size > 0.0 && p->cfl_max_time_step_size >= p->cfl_min_time_step_size)))
+      (p->cfl_method == 1 && !(p->cfl_min_time_step_size > 0.0 && p->cfl_max_time_step_size >= p->cfl_min_time_step_size)) ||
+      (p->use_rigid_contact != 0 && !(p->rigid_contact_support_radius_factor > 0.0 && p->rigid_contact_gamma > 0.0)))
     return nullptr;
   dfr_ctx* c = new dfr_ctx();
   c->P = *p;
@@ -1508,6 +1537,17 @@ dfr_ctx* dfr_create(const dfr_params* p, int device) {
   c->sc.K.inv_h = 1.f / c->h;
   c->sc.K.l_ih = c->sc.K.l / c->h;
   c->sc.K.l_ih2 = c->sc.K.l / (c->h * c->h);
+  {
+    const float hc = (float)(p->rigid_contact_support_radius_factor * p->radius);
+    const float hc3 = hc * hc * hc;
+    c->Kc.h = hc;
+    c->Kc.k = 8.f / (pi * hc3);
+    c->Kc.l = 48.f / (pi * hc3);
+    c->Kc.w0 = c->Kc.k;
+    c->Kc.inv_h = 1.f / hc;
+    c->Kc.l_ih = c->Kc.l / hc;
+    c->Kc.l_ih2 = c->Kc.l / (hc * hc);
+  }
   c->sc.dt = c->dtf;
   c->stats.dt = c->dtf;
   c->sc.rho0 = (float)p->rho0;
@@ -1603,6 +1643,9 @@ int dfr_set_init_velocity(dfr_ctx* c, int body, const double v0[3], const double
 int dfr_init(dfr_ctx* c) {
   return guarded(c, [&]() {
     if (c->initialized) return fail(c, DFR_ERR_STATE, "dfr_init called twice");
+    if (c->P.use_rigid_contact && c->P.compute_gradient)
+      return fail(c, DFR_ERR_STATE,
+                  "the rigid contact solver is forward only: set compute_gradient 0 with use_rigid_contact");
     HIPCHK(hipSetDevice(c->device));
     const int nbodies = (int)c->bodies.size();
     // boundary arrays: static bodies first, then dynamic bodies
@@ -1667,6 +1710,10 @@ int dfr_init(dfr_ctx* c) {
     }
     if (c->P.cfl_method == 1) c->d_cflmax = dalloc<uint32_t>(c, 1);
     const size_t NB = (size_t)std::max<int64_t>(off, 1);
+    if (c->P.use_rigid_contact) {
+      c->cvol0 = dalloc<float>(c, NB); c->cdens0 = dalloc<float>(c, NB);
+      c->cforce = dalloc<float4>(c, std::max<int64_t>(nd, 1)); c->ctorque = dalloc<float4>(c, std::max<int64_t>(nd, 1));
+    }
     c->b.pos = dalloc<float4>(c, NB); c->b.vel = dalloc<float4>(c, NB); c->b.x0 = dalloc<float4>(c, NB);
     c->b.dyn_idx = dalloc<uint32_t>(c, std::max<int64_t>(nd, 1));
     const uint32_t nk = c->grid.nkeys;
@@ -1845,6 +1892,8 @@ int dfr_set_gradient_mode(dfr_ctx* c, int mode) {
 
 int dfr_set_compute_gradient(dfr_ctx* c, int on) {
   if (!c) return DFR_ERR_ARG;
+  if (on && c->P.use_rigid_contact)
+    return fail(c, DFR_ERR_STATE, "the rigid contact solver is forward only: no reference-mode gradient with contact");
   c->P.compute_gradient = on ? 1 : 0;
   return DFR_OK;
 }
@@ -2145,6 +2194,8 @@ int dfr_kernel_times(const dfr_ctx* c, int n, char names[][32], double* avg_ms, 
 int dfr_adjoint_begin(dfr_ctx* c, int checkpoint_every) {
   return guarded(c, [&]() {
     if (!c->initialized) return fail(c, DFR_ERR_STATE, "dfr_adjoint_begin before dfr_init");
+    if (c->P.use_rigid_contact)
+      return fail(c, DFR_ERR_STATE, "the rigid contact solver is forward only: no exact adjoint with contact");
     if (c->P.surface_tension_method != 0 || c->P.cfl_method != 0)
       return fail(c, DFR_ERR_STATE,
                   "the exact adjoint covers viscosity but no other non-pressure force and no CFL step size "

@@ -202,8 +202,28 @@ struct RigidStepConst {
   int compute_gradient;
   int nbodies;
   int use_manager;       // useRigidGradientManager: the manager's chain instead of perform_chain_rule
+  int contact;           // the rigid contact solver ran this step: gravity already applied, and the
+                         // nbodies - 1 later updateBoundaryForces calls add their gyroscopic terms
 };
 void launch_rigid_step(const RigidStepConst& rc, RigidDev* rig, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// dfr_contact.hip: rigid-rigid contact (RigidContactSolver, penalty form).
+// ---------------------------------------------------------------------------
+struct ContactConst {
+  double dt;
+  double g[3];
+  double gdt[3];     // g dt (the gravity step the sample velocities include)
+  float beta, mu;    // rigidContactBeta, rigidContactFrictionCoeff
+  int gyroscopic;
+};
+// Rest volume and density of every boundary sample from its own body's neighbours.
+void launch_contact_rest(const BoundaryBufs& b, const GridDesc& g, const GridBufs& gb, float h2, const KernelConst& Kc,
+                         float gamma, float* vol0, float* dens0, hipStream_t s);
+// Gravity and the contact forces of the step, applied to the rigid velocities.
+void launch_contact(const BoundaryBufs& b, const GridDesc& g, const GridBufs& gb, float h2, const KernelConst& Kc,
+                    const ContactConst& cc, const float* vol0, const float* dens0, float4* cf, float4* ct,
+                    RigidDev* rig, int nbodies, hipStream_t s);
 void launch_boundary_transform(const uint32_t* idx, int n, BoundaryBufs& b, const RigidDev* rig, hipStream_t s);
 void launch_begin_step(RigidDev* rig, int nbodies, const int* ramp_flags, const double* factors,
                        const int* animated, hipStream_t s);

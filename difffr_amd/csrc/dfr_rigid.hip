@@ -337,8 +337,21 @@ __global__ void k_rigid_step(RigidStepConst rc, RigidDev* __restrict__ rig) {
   b.Flast[0] = F.x; b.Flast[1] = F.y; b.Flast[2] = F.z;
   b.Tlast[0] = T.x; b.Tlast[1] = T.y; b.Tlast[2] = T.z;
   for (int k = 0; k < 3; ++k) { b.F[k] = 0.0; b.T[k] = 0.0; }
-  if (!b.animated)
+  if (rc.contact) {
+    // RigidContactSolver::solveRigidContact calls updateBoundaryForces once per
+    // boundary model (RigidContactSolver.cpp:354-357): the later calls add zero
+    // torques, which with the gyroscopic mode still add I^-1 ((I w) x w) dt
+    if (rc.gyroscopic && !b.animated) {
+      const M3d I = ld3(b.I), Iinv = ld3(b.Iinv);
+      for (int call = 1; call < rc.nbodies; ++call) {
+        const V3d om = {b.w[0], b.w[1], b.w[2]};
+        const V3d dw = mv(Iinv, cross(mv(I, om), om));
+        b.w[0] += dw.x * dt; b.w[1] += dw.y * dt; b.w[2] += dw.z * dt;
+      }
+    }
+  } else if (!b.animated) {
     for (int k = 0; k < 3; ++k) b.v[k] += (rc.g[k] * b.mass) * (b.invMass * dt);
+  }
   if (rc.compute_gradient && rc.use_manager) manager_position_chain(b, rc);
   // animate()
   for (int k = 0; k < 3; ++k) b.x[k] += b.v[k] * dt;

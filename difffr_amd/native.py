@@ -49,6 +49,8 @@ class Params(C.Structure):
         ("surface_tension_boundary", C.c_double), ("viscosity_method", C.c_int32), ("viscosity", C.c_double),
         ("viscosity_boundary", C.c_double), ("cfl_method", C.c_int32), ("cfl_factor", C.c_double),
         ("cfl_min_time_step_size", C.c_double), ("cfl_max_time_step_size", C.c_double),
+        ("use_rigid_contact", C.c_int32), ("rigid_contact_beta", C.c_double), ("rigid_contact_gamma", C.c_double),
+        ("rigid_contact_support_radius_factor", C.c_double), ("rigid_contact_friction", C.c_double),
     ]
 
 

@@ -36,7 +36,9 @@ What differs, and why:
 * The RigidBodyGradientManager holds the single-body entries (R, R) of its
   chain (scene parameter `use_gradient_manager`, the scene key
   useRigidGradientManager); entries between two bodies, which only the rigid
-  contact solver fills, are zero.
+  contact solver fills, are zero.  The rigid contact solver
+  (useRigidContactSolver) runs forward only: such a scene runs without the
+  reference-mode gradient.
 * A reference C++ `assert` or crash on a bad index is an IndexError here.
 """
 from __future__ import annotations
@@ -676,6 +678,11 @@ class _SimulatorBase:
         self._require()
         return self._manager
 
+    def getBoundarySimulator(self) -> "_BoundarySimulator":
+        """SimulatorBase::getBoundarySimulator (SimulationModule.cpp:383)."""
+        self._require()
+        return _BoundarySimulator(self._scene)
+
     # -- stepping -------------------------------------------------------------
     def _one_step(self):
         core = self._require()
@@ -740,6 +747,25 @@ class _SimulatorBase:
         self._core = None
         if _current["base"] is self:
             _current["base"] = None
+
+
+class _RigidContactSolver:
+    """RigidContactSolver of a scene with useRigidContactSolver: the penalty
+    form runs inside every step; reset() (RigidContactSolver.cpp:1080-1087)
+    clears per-sample state that the step recomputes anyway."""
+
+    def reset(self):
+        return None
+
+
+class _BoundarySimulator:
+    def __init__(self, scene):
+        self._contact = bool(scene.params.get("use_rigid_contact", 0))
+
+    def getRigidContactSolver(self) -> Optional[_RigidContactSolver]:
+        """The contact solver, or None when the scene does not use it (the
+        reference returns a null pointer, RigidBody3dBoundarySimulator.cpp:370-380)."""
+        return _RigidContactSolver() if self._contact else None
 
 
 class Exec:

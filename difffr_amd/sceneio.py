@@ -56,6 +56,8 @@ _CONFIG_DEFAULTS = {
     "useRigidGradientManager": False, "useReleaseRigidBodyMode": False, "useRigidContactSolver": False,
     "sim2D": False, "cflMethod": 1, "boundaryHandlingMethod": 0, "simulationMethod": 4,
     "cflFactor": 0.5, "cflMinTimeStepSize": 0.0001, "cflMaxTimeStepSize": 0.005,
+    "rigidContactBeta": 1.0, "rigidContactGamma": 0.7, "rigidContactSupportRadiusFactor": 4.0,
+    "rigidContactFrictionCoeff": 0.0,
 }
 # Fluid parameters a "Materials" entry with the fluid's id may set (read by
 # SceneLoader::readParameterObject into the FluidModel), at the FluidModel's
@@ -70,7 +72,6 @@ _MATERIAL_DEFAULTS = {
 # Keys read but not run: the run differs from the reference's while they are on.
 _UNSUPPORTED_IF = {
     "cflMethod": lambda v: int(v) not in (0, 1),     # 2: iteration-count step-size control (Simulation.cpp:528-539)
-    "useRigidContactSolver": bool,                   # InterlinkedSPH/RigidContactSolver.cpp
     "sim2D": bool,
     "boundaryHandlingMethod": lambda v: int(v) != 0,  # Akinci2012 only
 }
@@ -427,7 +428,13 @@ def build_scene(sf: SceneFile, steps: Optional[int] = None, seed: int = 0) -> sc
              release_mode=int(bool(cfg["useReleaseRigidBodyMode"])),
              cfl_method=int(cfg["cflMethod"]), cfl_factor=float(cfg["cflFactor"]),
              cfl_min_time_step_size=float(cfg["cflMinTimeStepSize"]),
-             cfl_max_time_step_size=float(cfg["cflMaxTimeStepSize"]))
+             cfl_max_time_step_size=float(cfg["cflMaxTimeStepSize"]),
+             use_rigid_contact=int(bool(cfg["useRigidContactSolver"])),
+             rigid_contact_beta=float(cfg["rigidContactBeta"]), rigid_contact_gamma=float(cfg["rigidContactGamma"]),
+             rigid_contact_support_radius_factor=float(cfg["rigidContactSupportRadiusFactor"]),
+             rigid_contact_friction=float(cfg["rigidContactFrictionCoeff"]))
+    if p["use_rigid_contact"]:
+        p["compute_gradient"] = 0  # the contact solver runs forward only here
     p.update(fluid_material(sf))
     scene_dir = os.path.dirname(sf.path)
     xs, vs = [], []

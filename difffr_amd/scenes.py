@@ -68,6 +68,8 @@ def default_params(radius: float, dt: float = 1e-3) -> dict:
         surface_tension_method=0, surface_tension=0.05, surface_tension_boundary=0.01,
         viscosity_method=0, viscosity=0.01, viscosity_boundary=0.0,
         cfl_method=0, cfl_factor=0.5, cfl_min_time_step_size=0.0001, cfl_max_time_step_size=0.005,
+        use_rigid_contact=0, rigid_contact_beta=1.0, rigid_contact_gamma=0.7,
+        rigid_contact_support_radius_factor=4.0, rigid_contact_friction=0.0,
     )
 
 

@@ -41,13 +41,15 @@ def make_params(p: dict) -> native.Params:
     L.dfr_default_params(C.byref(q), float(p["radius"]), float(p["dt"]))
     for k in ("rho0", "max_error", "max_error_v", "uniform_acc_time", "target_time", "surface_tension",
               "surface_tension_boundary", "viscosity", "viscosity_boundary", "cfl_factor",
-              "cfl_min_time_step_size", "cfl_max_time_step_size"):
+              "cfl_min_time_step_size", "cfl_max_time_step_size", "rigid_contact_beta", "rigid_contact_gamma",
+              "rigid_contact_support_radius_factor", "rigid_contact_friction"):
         if k in p:
             setattr(q, k, float(p[k]))
     for k in ("min_iterations", "max_iterations", "max_iterations_v", "use_pressure_warmstart",
               "use_divergence_warmstart", "enable_divergence_solver", "gradient_mode", "optimize_rotation",
               "gyroscopic", "compute_gradient", "max_neighbors", "max_boundary_neighbors", "release_mode",
-              "use_gradient_manager", "surface_tension_method", "viscosity_method", "cfl_method"):
+              "use_gradient_manager", "surface_tension_method", "viscosity_method", "cfl_method",
+              "use_rigid_contact"):
         if k in p:
             setattr(q, k, int(p[k]))
     if "gravity" in p:

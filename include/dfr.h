@@ -82,6 +82,14 @@ typedef struct dfr_params {
   double cfl_factor;               /* default 0.5 */
   double cfl_min_time_step_size;   /* default 1e-4 */
   double cfl_max_time_step_size;   /* default 5e-3 */
+  /* Rigid-rigid contact between the bodies' boundary samples (RigidContactSolver, penalty form;
+     Configuration keys of SceneLoader.cpp:58-71).  Forward only: a context with contact refuses the
+     reference-mode gradient (compute_gradient) and the exact adjoint. */
+  int32_t use_rigid_contact;       /* useRigidContactSolver, default 0 */
+  double rigid_contact_beta;       /* rigidContactBeta, penalty stiffness (default 1) */
+  double rigid_contact_gamma;      /* rigidContactGamma, rest volume factor (default 0.7) */
+  double rigid_contact_support_radius_factor; /* rigidContactSupportRadiusFactor: contact kernel radius / r (4) */
+  double rigid_contact_friction;   /* rigidContactFrictionCoeff (default 0) */
 } dfr_params;
 
 typedef struct dfr_stats {

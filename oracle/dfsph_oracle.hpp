@@ -65,6 +65,12 @@ struct Params {
   double cflFactor = 0.5;        // Simulation.cpp:56-60
   double cflMinTimeStepSize = 0.0001;
   double cflMaxTimeStepSize = 0.005;
+  // Rigid-rigid contact (RigidContactSolver, penalty form; SceneLoader.cpp:58-71 defaults)
+  int useRigidContact = 0;              // useRigidContactSolver
+  double rigidContactBeta = 1.0;        // rigidContactBeta (penalty stiffness)
+  double rigidContactGamma = 0.7;       // rigidContactGamma (rest volume factor)
+  double rigidContactSupportRadiusFactor = 4.0;  // contact kernel radius / particle radius
+  double rigidContactFriction = 0.0;    // rigidContactFrictionCoeff
 };
 
 // Akinci et al. 2013 cohesion kernel (SPHKernels.h:437-505) and adhesion
@@ -220,6 +226,8 @@ template <typename RF, typename RR> struct Body {
   bool dynamic = false, animated = false;
   std::vector<V3<RF>> x0, x, v;  // local samples, world positions, velocities
   std::vector<RF> V;             // Akinci volumes
+  std::vector<RF> cvol0, cdens0; // rigid-contact rest volume and density (RigidContactSolver.cpp:140-170)
+  V3<RR> cF, cT;                 // sums of the contact forces / torques handed to the body in the last step
   RR mass = 0, invMass = 0;
   M3<RR> I0, I, Iinv;
   V3<RR> xrb, vrb, wrb, x_init, v_init, w_init;  // x_init etc. = state restored by reset
@@ -293,6 +301,7 @@ template <typename RF, typename RR> class Sim {
  public:
   Params P;
   Cubic<RF> K;
+  Cubic<RF> KRC;  // rigid-contact kernel, radius rigidContactSupportRadiusFactor r (RigidContactSolver.cpp:1437-1455)
   Cohesion<RF> KC;
   Adhesion<RF> KA;
   RF h, dt, rho0, V_f, m_f;
@@ -327,6 +336,7 @@ template <typename RF, typename RR> class Sim {
   explicit Sim(const Params& p) : P(p) {
     h = RF(4.0) * RF(P.radius);  // Simulation.cpp:195-216, support radius = 4r
     K.set(h);
+    KRC.set(RF(P.rigidContactSupportRadiusFactor) * RF(P.radius));
     KC.set(h);  // Simulation.cpp:397-398
     KA.set(h);
     dt = RF(P.dt);
@@ -387,6 +397,138 @@ template <typename RF, typename RR> class Sim {
     update_boundary_volume();
     nFB.assign(bodies.size(), {});
     nBF.assign(bodies.size(), {});
+    if (P.useRigidContact) contact_rest();
+  }
+
+  // ---- rigid-rigid contact (RigidContactSolver, penalty form) -------------
+  // Neighbours of sample j of body bi among every body's samples: the
+  // simulation's search radius h and predicate, the sample itself left out
+  // (CompactNSearch over the boundary point sets, activated by the solver).
+  struct SampleGrid {
+    std::vector<V3<RF>> pts;
+    std::vector<std::pair<uint32_t, uint32_t>> id;  // (body, sample)
+    Grid<RF> g;
+  };
+  void sample_grid(SampleGrid& G) const {
+    G.pts.clear(); G.id.clear();
+    for (size_t bi = 0; bi < bodies.size(); ++bi)
+      for (size_t j = 0; j < bodies[bi].x.size(); ++j) { G.pts.push_back(bodies[bi].x[j]); G.id.push_back({uint32_t(bi), uint32_t(j)}); }
+    G.g.build(G.pts, h * RF(1.001));
+  }
+  // RigidContactSolver constructor (RigidContactSolver.cpp:140-170): per
+  // sample, vol0 = gamma / (W(0) + sum over its own body's neighbours of W),
+  // then density0 = vol0 W(0) + sum of the own neighbours' vol0 W.
+  void contact_rest() {
+    SampleGrid G;
+    sample_grid(G);
+    const RF W0 = KRC.w0, gamma = RF(P.rigidContactGamma);
+    std::vector<std::vector<uint32_t>> nb(G.pts.size());
+    for (size_t a = 0; a < G.pts.size(); ++a) G.g.query(G.pts[a], G.pts, h * h, int64_t(a), nb[a]);
+    std::vector<RF> vol(G.pts.size());
+    for (size_t a = 0; a < G.pts.size(); ++a) {
+      RF sw = W0;
+      for (uint32_t k : nb[a])
+        if (G.id[k].first == G.id[a].first) sw += KRC.W(G.pts[a] - G.pts[k]);
+      vol[a] = gamma / sw;
+    }
+    for (auto& b : bodies) { b.cvol0.assign(b.x.size(), RF(0)); b.cdens0.assign(b.x.size(), RF(0)); }
+    for (size_t a = 0; a < G.pts.size(); ++a) {
+      RF d = vol[a] * W0;
+      for (uint32_t k : nb[a])
+        if (G.id[k].first == G.id[a].first) d += vol[k] * KRC.W(G.pts[a] - G.pts[k]);
+      auto& b = bodies[G.id[a].first];
+      b.cvol0[G.id[a].second] = vol[a];
+      b.cdens0[G.id[a].second] = d;
+    }
+  }
+  // Dynamic3dRigidBody::addForce / addTorque (Dynamic3dRigidBody.h:113-142):
+  // each call changes the velocity at once; with the gyroscopic mode every
+  // torque call adds I^-1 ((I w) x w) dt again.
+  void body_add_force(Body<RF, RR>& b, const V3<RR>& f, RR dtR) const { b.vrb += f * (b.invMass * dtR); }
+  void body_add_torque(Body<RF, RR>& b, const V3<RR>& t, RR dtR) const {
+    if (P.gyroscopic) {
+      const V3<RR> L = b.I * b.wrb;
+      b.wrb += (b.Iinv * (L.cross(b.wrb) + t)) * dtR;
+    } else {
+      b.wrb += (b.Iinv * t) * dtR;
+    }
+  }
+  // RigidBody3dBoundarySimulator::velocityTimeStep with the contact solver
+  // (RigidBody3dBoundarySimulator.cpp:284-298): gravity for every dynamic body,
+  // then RigidContactSolver::solveRigidContact in its penalty form
+  // (RigidContactSolver.cpp:347-366, beforePenaltyInitialize :307-345,
+  // solveRigidContactPenalty :419-555): each sample with a neighbour on another
+  // body gets density vol0 W(0) + sum over all its neighbours of vol0_k W;
+  // where density / density0 > 1 it pushes its (dynamic) body with
+  //   f = -beta (density / density0 - 1) n - mu |f_n| normalized(v_rel),
+  //   n = x - (sum_own x_k W) / (sum_own W),
+  //   v_rel = v - (sum_other v_k W) / (sum_own W)   (the own-body weight sum, as the reference divides),
+  // torque (x - x_rb) x f, one addForce/addTorque per sample in body and sample
+  // order; then updateBoundaryForces once per boundary model: the first call
+  // applies the fluid force and torque (dynamic bodies that are not animated)
+  // and clears them, the later ones apply zeros (BoundarySimulator.cpp:10-36).
+  void contact_velocity_step(RR dtR) {
+    const V3<RR> gR(RR(P.gravity[0]), RR(P.gravity[1]), RR(P.gravity[2]));
+    for (auto& b : bodies)
+      if (b.dynamic) body_add_force(b, gR * b.mass, dtR);  // addGravity (Dynamic3dRigidBody.h:228-233)
+    // sample velocities after gravity (beforePenaltyInitialize :326-329); zero for static bodies
+    std::vector<std::vector<V3<RF>>> vel(bodies.size());
+    for (size_t bi = 0; bi < bodies.size(); ++bi) {
+      const auto& b = bodies[bi];
+      vel[bi].assign(b.x.size(), V3<RF>());
+      if (b.dynamic || b.animated)
+        for (size_t j = 0; j < b.x.size(); ++j) vel[bi][j] = V3<RF>(b.vrb + b.wrb.cross(V3<RR>(b.x[j]) - b.xrb));
+    }
+    SampleGrid G;
+    sample_grid(G);
+    const RF W0 = KRC.w0, beta = RF(P.rigidContactBeta), mu = RF(P.rigidContactFriction);
+    for (auto& b : bodies) { b.cF = V3<RR>(); b.cT = V3<RR>(); }
+    std::vector<uint32_t> nb;
+    for (size_t a = 0; a < G.pts.size(); ++a) {
+      const uint32_t R = G.id[a].first, r = G.id[a].second;
+      auto& b = bodies[R];
+      if (!b.dynamic) continue;  // StaticRigidBody::addForce does nothing
+      G.g.query(G.pts[a], G.pts, h * h, int64_t(a), nb);
+      bool contact = false;
+      for (uint32_t k : nb) contact |= G.id[k].first != R;
+      if (!contact) continue;
+      const V3<RF>& xr = b.x[r];
+      RF dens = b.cvol0[r] * W0;
+      V3<RF> sx, sv;
+      RF sw = 0;
+      for (uint32_t k : nb) {
+        const auto& bk = bodies[G.id[k].first];
+        const uint32_t kk = G.id[k].second;
+        const RF w = KRC.W(xr - bk.x[kk]);
+        dens += bk.cvol0[kk] * w;
+        if (G.id[k].first == R) { sx += bk.x[kk] * w; sw += w; }
+        else sv += vel[G.id[k].first][kk] * w;
+      }
+      const RF ratio = dens / b.cdens0[r];
+      if (!(ratio > RF(1))) continue;
+      const V3<RF> n = xr - sx * (RF(1) / sw);
+      const V3<RF> vrel = vel[R][r] - sv * (RF(1) / sw);
+      const V3<RF> fn = n * (-beta * std::max(ratio - RF(1), RF(0)));
+      const RF vn2 = vrel.squaredNorm();
+      const V3<RF> unit = vn2 > RF(0) ? vrel * (RF(1) / std::sqrt(vn2)) : vrel;  // Eigen normalized()
+      const V3<RF> f = fn + unit * (-mu * fn.norm());
+      const V3<RF> rr = V3<RF>(V3<RR>(xr) - b.xrb);
+      body_add_force(b, V3<RR>(f), dtR);
+      body_add_torque(b, V3<RR>(rr.cross(f)), dtR);
+      b.cF += V3<RR>(f);
+      b.cT += V3<RR>(rr.cross(f));
+    }
+    for (size_t call = 0; call < bodies.size(); ++call)
+      for (auto& b : bodies) {
+        if (!b.dynamic) continue;
+        V3<RR> F, T;
+        if (call == 0) {
+          force_and_torque(b, F, T);
+          b.F_last = F; b.T_last = T;
+          for (int t = 0; t < nthreads; ++t) { b.Fth[t] = V3<RR>(); b.Tth[t] = V3<RR>(); }
+        }
+        if (!b.animated) { body_add_force(b, F, dtR); body_add_torque(b, T, dtR); }
+      }
   }
 
   static void update_inertia(Body<RF, RR>& b) {  // Dynamic3dRigidBody.h:215-219
@@ -1072,6 +1214,7 @@ template <typename RF, typename RR> class Sim {
   // Dynamic3dRigidBody::addForce/addTorque/addGravity (Dynamic3dRigidBody.h:113-142, 228-233).
   void rigid_velocity_step() {
     const RR dtR = RR(dt);
+    if (P.useRigidContact) { contact_velocity_step(dtR); return; }
     const V3<RR> gR(RR(P.gravity[0]), RR(P.gravity[1]), RR(P.gravity[2]));
     for (size_t k = 0; k < bodies.size(); ++k) {
       auto& b = bodies[k];

@@ -105,6 +105,16 @@ template <typename RF, typename RR> struct Oracle : OracleBase {
     const std::string n(name);
     const auto& b = sim.bodies[body];
     if (n == "volume") { for (size_t j = 0; j < b.V.size(); ++j) out[j] = double(b.V[j]); return 0; }
+    if (n == "contact_vol0" || n == "contact_density0") {  // zero when the contact solver is off
+      const auto& f = n == "contact_vol0" ? b.cvol0 : b.cdens0;
+      for (size_t j = 0; j < b.x.size(); ++j) out[j] = j < f.size() ? double(f[j]) : 0.0;
+      return 0;
+    }
+    if (n == "contact_force" || n == "contact_torque") {  // sums over the last step's contact samples
+      const auto& f = n == "contact_force" ? b.cF : b.cT;
+      for (int k = 0; k < 3; ++k) out[k] = double(f[k]);
+      return 0;
+    }
     if (n == "x") { for (size_t j = 0; j < b.x.size(); ++j) for (int k = 0; k < 3; ++k) out[3 * j + k] = double(b.x[j][k]); return 0; }
     if (n == "v") { for (size_t j = 0; j < b.v.size(); ++j) for (int k = 0; k < 3; ++k) out[3 * j + k] = double(b.v[j][k]); return 0; }
     return -1;

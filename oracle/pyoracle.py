@@ -31,6 +31,8 @@ class Params(C.Structure):
         ("viscosityMethod", C.c_int), ("viscosity", C.c_double), ("viscosityBoundary", C.c_double),
         ("cflMethod", C.c_int), ("cflFactor", C.c_double), ("cflMinTimeStepSize", C.c_double),
         ("cflMaxTimeStepSize", C.c_double),
+        ("useRigidContact", C.c_int), ("rigidContactBeta", C.c_double), ("rigidContactGamma", C.c_double),
+        ("rigidContactSupportRadiusFactor", C.c_double), ("rigidContactFriction", C.c_double),
     ]
 
 
@@ -129,6 +131,11 @@ def make_params(p: dict) -> Params:
     q.cflFactor = float(p.get("cfl_factor", 0.5))
     q.cflMinTimeStepSize = float(p.get("cfl_min_time_step_size", 0.0001))
     q.cflMaxTimeStepSize = float(p.get("cfl_max_time_step_size", 0.005))
+    q.useRigidContact = int(p.get("use_rigid_contact", 0))
+    q.rigidContactBeta = float(p.get("rigid_contact_beta", 1.0))
+    q.rigidContactGamma = float(p.get("rigid_contact_gamma", 0.7))
+    q.rigidContactSupportRadiusFactor = float(p.get("rigid_contact_support_radius_factor", 4.0))
+    q.rigidContactFriction = float(p.get("rigid_contact_friction", 0.0))
     return q
 
 
@@ -292,8 +299,15 @@ class OracleSim:
         return out.reshape(self.n, 3) if k == 3 else out
 
     def body_field(self, body: int, name: str) -> np.ndarray:
+        """Per-sample "volume", "contact_vol0", "contact_density0" ([n]), "x", "v"
+        ([n][3]); "contact_force", "contact_torque": the last step's sums (3)."""
+        if name in ("contact_force", "contact_torque"):
+            out = np.empty(3)
+            if lib().orc_get_body_field(self.h, body, name.encode(), _dp(out)) != 0:
+                raise KeyError(name)
+            return out
         n = lib().orc_n_body_particles(self.h, body)
-        k = 1 if name == "volume" else 3
+        k = 1 if name in ("volume", "contact_vol0", "contact_density0") else 3
         out = np.empty(n * k)
         if lib().orc_get_body_field(self.h, body, name.encode(), _dp(out)) != 0:
             raise KeyError(name)

@@ -1,0 +1,111 @@
+"""Rigid-rigid contact (RigidContactSolver, penalty form) on the GPU against the
+oracle, which tests/test_oracle_contact.py pins against a direct evaluation of
+the reference's definitions.
+
+Tolerances: one step's contact-induced velocity change (v - v0 - g dt, and the
+angular one) within 1e-4 of its size (FP32 kernel sums in a different order;
+a sample whose density ratio sits within rounding of 1 may call addTorque on
+one side only, which with the gyroscopic mode moves omega by one gyroscopic
+increment, far below that); over a rollout with fluid the rigid poses within
+1e-4 relative or three times the spread between the oracle's own FP32 and FP64
+runs, whichever is larger.
+"""
+import numpy as np
+import pytest
+
+from difffr_amd import scenes
+from difffr_amd.simulation import DfrError, Simulation
+from oracle.pyoracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+
+R_P = 0.015
+
+
+def _ball_scene(gyro=1, mu=0.4, beta=3000.0):
+    ball = scenes.sphere_surface(0.05, 2 * R_P)
+    ball = ball - ball.mean(0)
+    p = scenes.default_params(R_P, 1e-3)
+    p.update(compute_gradient=0, use_rigid_contact=1, rigid_contact_beta=beta, rigid_contact_friction=mu,
+             gyroscopic=gyro)
+    empty = np.zeros((0, 3))
+    a = scenes.Body(ball, True, np.zeros(3), np.array([1.0, 0, 0, 0]), 800.0, np.array([0.3, -0.1, 0.2]),
+                    np.array([2.0, -1.0, 3.0]))
+    b = scenes.Body(ball, True, np.array([0.093, 0.012, -0.008]), np.array([1.0, 0, 0, 0]), 600.0,
+                    np.array([-0.2, 0.1, 0.0]), np.array([-1.0, 0.5, 0.7]))
+    return scenes.Scene("balls", p, empty, empty.copy(), [a, b], 1)
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("gyro,mu", [(1, 0.4), (0, 0.0)])
+def test_one_contact_step_matches_oracle(gpu_ready, gyro, mu):
+    sc = _ball_scene(gyro, mu)
+    print(f"[contact one step] gyroscopic {gyro} friction {mu}", flush=True)
+    g = Simulation(sc)
+    o = OracleSim(sc, "f32_rigid64")
+    g.step(1)
+    o.step(1)
+    gv = np.asarray(sc.params["gravity"]) * sc.params["dt"]
+    for k, b in enumerate(sc.bodies):
+        rg, ro = g.rigid(k), o.rigid(k)
+        dv_o = ro["v"] - b.v0 - gv
+        dw_o = ro["omega"] - b.w0
+        e_v = np.abs(rg["v"] - ro["v"]).max() / np.abs(dv_o).max()
+        e_w = np.abs(rg["omega"] - ro["omega"]).max() / np.abs(dw_o).max()
+        print(f"  body {k}: dv {dv_o}, dw {dw_o}; GPU error {e_v:.1e} / {e_w:.1e} of the change", flush=True)
+        assert np.abs(dv_o).max() > 0.05  # the contact acted
+        assert e_v < 1e-4 and e_w < 1e-4
+        np.testing.assert_allclose(rg["x"], ro["x"], rtol=0, atol=1e-4 * np.abs(dv_o).max() * sc.params["dt"])
+    g.close()
+
+
+def test_contact_rollout_in_a_tank(gpu_ready):
+    """Two spheres dropped side by side into the submerged-body tank, the second
+    driven into the first, with the fluid, the tank walls (a static body the
+    spheres may also touch) and the default gyroscopic rigid update; 40 steps."""
+    sc = scenes.submerged_body(body="sphere", steps=40)
+    other = sc.bodies[1]
+    x2 = other.x_rb + np.array([0.135, 0.0, 0.0])  # 1.5 r apart: in contact range from the start
+    sc.bodies.append(scenes.Body(other.x0.copy(), True, x2, np.array([1.0, 0, 0, 0]), 500.0,
+                                 np.array([-0.5, 0.0, 0.0]), np.zeros(3)))
+    keep = np.linalg.norm(sc.fluid_x - x2, axis=1) > 0.06 + 0.01
+    sc.fluid_x, sc.fluid_v = sc.fluid_x[keep], sc.fluid_v[keep]
+    sc.params.update(compute_gradient=0, use_rigid_contact=1, rigid_contact_beta=2000.0)
+    print(f"[contact rollout] n={sc.n_fluid}, {len(sc.bodies)} bodies, {sc.steps} steps", flush=True)
+    g = Simulation(sc)
+    o = OracleSim(sc, "f32_rigid64")
+    o64 = OracleSim(sc, "f64")
+    gap0 = np.linalg.norm(x2 - other.x_rb)
+    for s in range(sc.steps):
+        g.step(1)
+        st = g.stats()
+        for oo in (o, o64):
+            oo.set_fixed_iterations([st["iterations_v"]], [st["iterations"]])
+            oo.step(1)
+    pose = lambda a, b: max(rel(a["x"], b["x"]), rel(a["q"], b["q"]))
+    gap = np.linalg.norm(o.rigid(2)["x"] - o.rigid(1)["x"])
+    for k in (1, 2):
+        e, floor = pose(g.rigid(k), o.rigid(k)), pose(o.rigid(k), o64.rigid(k))
+        print(f"  body {k}: pose rel vs FP32 oracle {e:.1e} (FP32 vs FP64 oracle {floor:.1e}); "
+              f"v GPU {g.rigid(k)['v']} oracle {o.rigid(k)['v']}", flush=True)
+        assert e < max(1e-4, 3 * floor)
+    print(f"  centre distance {gap0:.4f} -> {gap:.4f} m", flush=True)
+    # the spheres (radius 0.06) met and were pushed apart again
+    assert gap > 0.14 and o.rigid(2)["v"][0] > 0
+    g.close()
+
+
+def test_contact_refuses_gradients(gpu_ready):
+    sc = _ball_scene()
+    sc.params["compute_gradient"] = 1
+    with pytest.raises(DfrError, match="forward only"):
+        Simulation(sc)
+    sim = Simulation(_ball_scene())
+    with pytest.raises(DfrError, match="forward only"):
+        sim.adjoint_begin(0)
+    sim.close()

@@ -123,3 +123,15 @@ def test_no_simulation_before_init_simulation():
     if sph._current["base"] is None:
         with pytest.raises(DfrError):
             sph.Simulation.getCurrent()
+
+
+def test_rigid_contact_solver_handle():
+    """getBoundarySimulator().getRigidContactSolver(): None without
+    useRigidContactSolver (the reference's null pointer), else an object whose
+    reset() the drivers may call."""
+    from difffr_amd import scenes
+    sc = scenes.submerged_body(steps=1)
+    assert sph._BoundarySimulator(sc).getRigidContactSolver() is None
+    sc.params["use_rigid_contact"] = 1
+    solver = sph._BoundarySimulator(sc).getRigidContactSolver()
+    assert solver is not None and solver.reset() is None

@@ -3,7 +3,8 @@
 Fixtures: tests/golden/reference_files (the experiments' scene JSONs, the
 bottle-flip experiment's meshes and saved state; copied by
 tests/golden/copy_reference_files.py).  Pins:
-* scene keys and defaults against the files' own values (SceneLoader.cpp:40-360);
+* scene keys and defaults against the files' own values (SceneLoader.cpp:40-360),
+  the contact solver's keys included;
 * the fluid-block layout (SimulatorBase::createFluidBlocks) against the particle
   counts of states the reference saved for the same scenes (billiards: 87,374,
   from state_17_particle_Fluid.bgeo's header; bottle flip: 5,904) and the
@@ -74,6 +75,22 @@ def test_every_reference_scene_parses():
         sf = _scene(n)
         assert sf.rigid_bodies and (sf.fluid_blocks or sf.fluid_models), n
         assert all(rb.geometry_file.endswith(".obj") for rb in sf.rigid_bodies)
+
+
+def test_contact_scenes_switch_the_contact_solver_on():
+    """billiards-on-water-2balls and diff-bottle-model-collide set
+    useRigidContactSolver with their own stiffness (and friction); the other
+    keys keep SceneLoader.cpp:58-71's defaults; the contact solver is run, so
+    nothing is reported unsupported for it."""
+    for name, beta, mu in (("billiards-on-water-2balls.json", 20000.0, 0.0),
+                           ("diff-bottle-model-collide.json", 100000.0, 0.7)):
+        sf = _scene(name)
+        c = sf.config
+        assert c["useRigidContactSolver"] is True
+        assert c["rigidContactBeta"] == beta and c["rigidContactFrictionCoeff"] == mu
+        assert c["rigidContactGamma"] == 0.7 and c["rigidContactSupportRadiusFactor"] == 4.0
+        assert not any("Contact" in u for u in sf.unsupported), sf.unsupported
+    assert _scene("diff-stone-skipping.json").config["useRigidContactSolver"] is False
 
 
 def test_fluid_blocks_reproduce_the_reference_state_counts():

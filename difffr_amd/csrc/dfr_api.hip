@@ -215,6 +215,10 @@ struct dfr_ctx {
   float *cvol0 = nullptr, *cdens0 = nullptr;
   float4 *cforce = nullptr, *ctorque = nullptr;
   bool contact_ready = false;  // rest values computed (invariant under the bodies' rigid motion)
+  // RigidBodyGradientManager across bodies (contact + manager): [nb][nb][78] Jacobians and handed
+  // sums, and the contact samples' gradient records ([6][nd])
+  double *mgrJ = nullptr, *mgrN = nullptr;
+  float4* cgrec = nullptr;
   KernelConst Kc{};            // cubic spline of radius rigid_contact_support_radius_factor r
   float4* nrm = nullptr;      // Akinci 2013 normals
   uint32_t* d_cflmax = nullptr;  // CFL squared-speed bound (float bits)
@@ -437,7 +441,20 @@ void upload_fluid_initial(dfr_ctx* c) {
   HIPCHK(hipStreamSynchronize(c->stream));
 }
 
+// RigidBodyGradientManager::reset (RigidBodyGradientManager.cpp:474-523) of the table across bodies.
+void reset_manager_multi(dfr_ctx* c) {
+  if (!c->mgrJ) return;
+  const size_t nb = c->bodies.size();
+  std::vector<double> J(nb * nb * 78, 0.0);
+  for (size_t R = 0; R < nb; ++R)
+    for (int d = 0; d < 3; ++d) { J[(R * nb + R) * 78 + 18 + 4 * d] = 1.0; J[(R * nb + R) * 78 + 45 + 4 * d] = 1.0; }
+  HIPCHK(hipMemcpyAsync(c->mgrJ, J.data(), J.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->mgrN, 0, J.size() * sizeof(double), c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
 void place_boundary_and_volumes(dfr_ctx* c) {
+  reset_manager_multi(c);
   const int nbodies = (int)c->bodies.size();
   HIPCHK(hipMemcpyAsync(c->d_rig, c->h_rig_init.data(), sizeof(RigidDev) * nbodies, hipMemcpyHostToDevice, c->stream));
   launch_boundary_transform(nullptr, c->b.n, c->b, c->d_rig, c->stream);
@@ -899,15 +916,37 @@ void do_step(dfr_ctx* c) {
                             c->stream);
         c->contact_ready = true;
       }
+      // SimulatorBase::timeStepNoGUI (SimulatorBase.cpp:1155-1170): the step's gradient sums go to the
+      // manager (or the per-body chain rule runs) before the boundary velocity step; with the manager
+      // its chain after the fluid coupling, the contact solver, then its chain after the rigid coupling
+      const bool multi = c->mgrJ && rc.compute_gradient && rc.use_manager;
+      if (rc.compute_gradient) {
+        RigidStepConst r1 = rc;
+        r1.phases = RS_HAND;
+        r1.mgr_net = multi ? c->mgrN : nullptr;
+        launch_rigid_step(r1, c->d_rig, c->stream);
+      }
+      if (multi && nf > 0) launch_manager_multi_velocity(rc, c->d_rig, c->mgrJ, c->mgrN, rc.nbodies, 0, c->stream);
       ContactConst cc{};
       cc.dt = (double)dtp;
       for (int k = 0; k < 3; ++k) { cc.g[k] = rc.g[k]; cc.gdt[k] = rc.g[k] * (double)dtp; }
       cc.beta = (float)c->P.rigid_contact_beta;
       cc.mu = (float)c->P.rigid_contact_friction;
       cc.gyroscopic = c->P.gyroscopic;
+      cc.Ks = c->sc.K;
+      cc.grec = multi ? c->cgrec : nullptr;
+      cc.mgr_net = multi ? c->mgrN : nullptr;
       launch_contact(c->b, c->grid, c->gb, hh, c->Kc, cc, c->cvol0, c->cdens0, c->cforce, c->ctorque, c->d_rig,
                      rc.nbodies, c->stream);
       rc.contact = 1;
+      RigidStepConst r2 = rc;
+      r2.phases = RS_FORCES;
+      launch_rigid_step(r2, c->d_rig, c->stream);
+      if (multi) {
+        launch_manager_multi_velocity(rc, c->d_rig, c->mgrJ, c->mgrN, rc.nbodies, 1, c->stream);
+        launch_manager_multi_position(rc, c->d_rig, c->mgrJ, rc.nbodies, c->stream);
+      }
+      rc.phases = RS_ANIMATE;
     }
     launch_rigid_step(rc, c->d_rig, c->stream);
     launch_boundary_transform(c->b.dyn_idx, c->b.nd, c->b, c->d_rig, c->stream);
@@ -1643,9 +1682,8 @@ int dfr_set_init_velocity(dfr_ctx* c, int body, const double v0[3], const double
 int dfr_init(dfr_ctx* c) {
   return guarded(c, [&]() {
     if (c->initialized) return fail(c, DFR_ERR_STATE, "dfr_init called twice");
-    if (c->P.use_rigid_contact && c->P.compute_gradient)
-      return fail(c, DFR_ERR_STATE,
-                  "the rigid contact solver is forward only: set compute_gradient 0 with use_rigid_contact");
+    if (c->P.use_rigid_contact && c->P.use_gradient_manager && c->bodies.size() > 16)
+      return fail(c, DFR_ERR_ARG, "the gradient manager across bodies (rigid contact) holds at most 16 bodies");
     HIPCHK(hipSetDevice(c->device));
     const int nbodies = (int)c->bodies.size();
     // boundary arrays: static bodies first, then dynamic bodies
@@ -1713,6 +1751,11 @@ int dfr_init(dfr_ctx* c) {
     if (c->P.use_rigid_contact) {
       c->cvol0 = dalloc<float>(c, NB); c->cdens0 = dalloc<float>(c, NB);
       c->cforce = dalloc<float4>(c, std::max<int64_t>(nd, 1)); c->ctorque = dalloc<float4>(c, std::max<int64_t>(nd, 1));
+      if (c->P.use_gradient_manager) {
+        const size_t nbb = (size_t)nbodies * nbodies * 78;
+        c->mgrJ = dalloc<double>(c, nbb); c->mgrN = dalloc<double>(c, nbb);
+        c->cgrec = dalloc<float4>(c, 6 * (size_t)std::max<int64_t>(nd, 1));
+      }
     }
     c->b.pos = dalloc<float4>(c, NB); c->b.vel = dalloc<float4>(c, NB); c->b.x0 = dalloc<float4>(c, NB);
     c->b.dyn_idx = dalloc<uint32_t>(c, std::max<int64_t>(nd, 1));
@@ -1892,8 +1935,6 @@ int dfr_set_gradient_mode(dfr_ctx* c, int mode) {
 
 int dfr_set_compute_gradient(dfr_ctx* c, int on) {
   if (!c) return DFR_ERR_ARG;
-  if (on && c->P.use_rigid_contact)
-    return fail(c, DFR_ERR_STATE, "the rigid contact solver is forward only: no reference-mode gradient with contact");
   c->P.compute_gradient = on ? 1 : 0;
   return DFR_OK;
 }
@@ -1908,9 +1949,17 @@ int dfr_get_gradient_manager(const dfr_ctx* c, int R, int RR, double jac[78], do
   if (check_body(c, R) || check_body(c, RR)) return DFR_ERR_ARG;
   if (jac) std::fill(jac, jac + 78, 0.0);
   if (net) std::fill(net, net + 78, 0.0);
-  // entries between different bodies stay zero without the rigid contact solver;
-  // a static body has no entries
-  if (R != RR || !c->bodies[R].dynamic) return DFR_OK;
+  if (!c->bodies[R].dynamic || !c->bodies[RR].dynamic) return DFR_OK;  // a static body has no entries
+  if (c->mgrJ) {  // the table across bodies (rigid contact solver with the manager)
+    const size_t o = ((size_t)R * c->bodies.size() + RR) * 78;
+    if (jac && hipMemcpy(jac, c->mgrJ + o, 78 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      return DFR_ERR_HIP;
+    if (net && hipMemcpy(net, c->mgrN + o, 78 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      return DFR_ERR_HIP;
+    return DFR_OK;
+  }
+  // entries between different bodies stay zero without the rigid contact solver
+  if (R != RR) return DFR_OK;
   if (jac && hipMemcpy(jac, &c->d_rig[R].Jm[0], 78 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
     return DFR_ERR_HIP;
   if (net && hipMemcpy(net, &c->d_rig[R].mnet[0], 78 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
@@ -1935,6 +1984,7 @@ int dfr_reset_gradient(dfr_ctx* c) {
       HIPCHK(hipMemcpy(&c->d_rig[k].Jm[0], J, sizeof(J), hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(&c->d_rig[k].mnet[0], Z, sizeof(Z), hipMemcpyHostToDevice));
     }
+    reset_manager_multi(c);
     return DFR_OK;
   });
 }
@@ -2195,7 +2245,8 @@ int dfr_adjoint_begin(dfr_ctx* c, int checkpoint_every) {
   return guarded(c, [&]() {
     if (!c->initialized) return fail(c, DFR_ERR_STATE, "dfr_adjoint_begin before dfr_init");
     if (c->P.use_rigid_contact)
-      return fail(c, DFR_ERR_STATE, "the rigid contact solver is forward only: no exact adjoint with contact");
+      return fail(c, DFR_ERR_STATE,
+                  "the exact adjoint does not cover the rigid contact solver (its reference-mode gradient runs)");
     if (c->P.surface_tension_method != 0 || c->P.cfl_method != 0)
       return fail(c, DFR_ERR_STATE,
                   "the exact adjoint covers viscosity but no other non-pressure force and no CFL step size "

@@ -20,8 +20,15 @@
 // 27 cells of the boundary bins; the forces are then applied body by body in
 // sample order (k_contact_apply), one Dynamic3dRigidBody::addForce/addTorque
 // per contact sample as the reference calls them, in FP64.
+//
+// With the manager across bodies (useRigidGradientManager) each contact sample
+// also leaves a record of its gradient ingredients (k_contact_forces), from
+// which k_contact_apply forms the solver's grad_net_{force,torque}_to_*
+// sums (solveRigidContactPenalty :471-553) and hands them to the manager in
+// place of the step's (update_rigid_body_gradient_manager, :1341-1370).
 #include "dfr_device.h"
 #include "dfr_kernels.h"
+#include "dfr_rigid_math.h"
 
 namespace dfr {
 
@@ -136,6 +143,11 @@ k_contact_forces(const float4* __restrict__ bpos, const float4* __restrict__ bx0
   const uint32_t total = SS[w].pre[27];
   float dens = 0.f, sw = 0.f, sx = 0.f, sy = 0.f, sz = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
   int other = 0;
+  // gradient ingredients over the other bodies' neighbours: d density / d x_R with the
+  // simulation's kernel gradient, d density / d q_Rk, the weighted r_k, the last body
+  float gx = 0.f, gy = 0.f, gz = 0.f, gk[4] = {0.f, 0.f, 0.f, 0.f}, ax = 0.f, ay = 0.f, az = 0.f;
+  int rr_last = -1;
+  const bool grads = cc.grec != nullptr;
   for (uint32_t base = 0; base < total; base += 64) {
     const uint32_t t = base + (uint32_t)lane;
     if (t < total) {
@@ -152,6 +164,21 @@ k_contact_forces(const float4* __restrict__ bpos, const float4* __restrict__ bx0
           other = 1;
           const float3 vk = sample_vel(rig[Rk], q, cc.gdt);
           vx += vk.x * W; vy += vk.y * W; vz += vk.z * W;
+          if (grads) {
+            const RigidDev& bk = rig[Rk];
+            rr_last = max(rr_last, Rk);
+            const float gc = vol0[k] * cubic_grad_c(cc.Ks, rx * rx + ry * ry + rz * rz);
+            const float wx = gc * rx, wy = gc * ry, wz = gc * rz;
+            gx += wx; gy += wy; gz += wz;
+            const float4 x0k = bx0[k];
+            const float qk[4] = {(float)bk.q[0], (float)bk.q[1], (float)bk.q[2], (float)bk.q[3]};
+            const DM<float, 3, 4> Jk = grad_Rqp<float>(qk, DV3<float>{x0k.x, x0k.y, x0k.z}, 1.f);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) gk[c] -= wx * Jk(0, c) + wy * Jk(1, c) + wz * Jk(2, c);
+            ax += (float)((double)q.x - bk.x[0]) * W;
+            ay += (float)((double)q.y - bk.x[1]) * W;
+            az += (float)((double)q.z - bk.x[2]) * W;
+          }
         }
       }
     }
@@ -160,6 +187,14 @@ k_contact_forces(const float4* __restrict__ bpos, const float4* __restrict__ bx0
   dens = wave_sum_f(dens); sw = wave_sum_f(sw);
   sx = wave_sum_f(sx); sy = wave_sum_f(sy); sz = wave_sum_f(sz);
   vx = wave_sum_f(vx); vy = wave_sum_f(vy); vz = wave_sum_f(vz);
+  if (grads) {
+    gx = wave_sum_f(gx); gy = wave_sum_f(gy); gz = wave_sum_f(gz);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) gk[c] = wave_sum_f(gk[c]);
+    ax = wave_sum_f(ax); ay = wave_sum_f(ay); az = wave_sum_f(az);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) rr_last = max(rr_last, __shfl_xor(rr_last, off, 64));
+  }
   if (lane != 0) return;
   float4 fo = make_float4(0.f, 0.f, 0.f, 0.f), to = make_float4(0.f, 0.f, 0.f, 0.f);
   if (contact) {
@@ -182,27 +217,96 @@ k_contact_forces(const float4* __restrict__ bpos, const float4* __restrict__ bx0
                   rz = (float)((double)p.z - B.x[2]);
       fo = make_float4(fx, fy, fz, 1.f);
       to = make_float4(ry * fz - rz * fy, rz * fx - rx * fz, rx * fy - ry * fx, 0.f);
+      if (grads) {
+        const size_t nd_ = (size_t)nd;
+        cc.grec[0 * nd_ + d] = make_float4(nx, ny, nz, (float)(rr_last < 0 ? R : rr_last));
+        cc.grec[1 * nd_ + d] = make_float4(vr.x - vx * isw, vr.y - vy * isw, vr.z - vz * isw, 0.f);
+        cc.grec[2 * nd_ + d] = make_float4(gx, gy, gz, 0.f);
+        cc.grec[3 * nd_ + d] = make_float4(gk[0], gk[1], gk[2], gk[3]);
+        cc.grec[4 * nd_ + d] = make_float4(ax * isw, ay * isw, az * isw, 0.f);
+        cc.grec[5 * nd_ + d] = make_float4(fnx, fny, fnz, 0.f);
+      }
     }
   }
   cf[d] = fo;
   ct[d] = to;
 }
 
+// The contact solver's gradient sums of one contact sample (solveRigidContactPenalty,
+// RigidContactSolver.cpp:471-553), added to the rows of body R: (R, R) and, when its
+// other-body neighbours end on body Q != R, (R, Q).  Layout of RigidDev::net.
+__device__ void add_contact_gradients(double* NRR, double* NRQ, const RigidDev& b, const float4* rec,
+                                      const float4& fx, const float4& p, const float4& x0, float beta, float mu) {
+  const V3d n = {rec[0].x, rec[0].y, rec[0].z}, vr = {rec[1].x, rec[1].y, rec[1].z};
+  const V3d gdx = {rec[2].x, rec[2].y, rec[2].z}, avg = {rec[4].x, rec[4].y, rec[4].z};
+  const V3d fn = {rec[5].x, rec[5].y, rec[5].z}, f = {fx.x, fx.y, fx.z};
+  const double gdqk[4] = {rec[3].x, rec[3].y, rec[3].z, rec[3].w};
+  const V3d r = {(double)p.x - b.x[0], (double)p.y - b.x[1], (double)p.z - b.x[2]};
+  const V3d gfn = {-(double)beta * n.x, -(double)beta * n.y, -(double)beta * n.z};
+  const double fnn = sqrt(fn.x * fn.x + fn.y * fn.y + fn.z * fn.z);
+  const double vn = sqrt(vr.x * vr.x + vr.y * vr.y + vr.z * vr.z);
+  const V3d u = vn > 0.0 ? V3d{vr.x / vn, vr.y / vn, vr.z / vn} : vr;
+  const double sff = -(double)mu * (fn.x * gfn.x + fn.y * gfn.y + fn.z * gfn.z) / fnn;
+  const V3d gf = {gfn.x + u.x * sff, gfn.y + u.y * sff, gfn.z + u.z * sff};
+  const M34d Jq = grad_Rqp<double>(b.q, V3d{(double)x0.x, (double)x0.y, (double)x0.z}, 1.0);
+  double gdq[4];
+  for (int c = 0; c < 4; ++c) gdq[c] = gdx.x * Jq(0, c) + gdx.y * Jq(1, c) + gdx.z * Jq(2, c);
+  const double gfa[3] = {gf.x, gf.y, gf.z}, gxa[3] = {gdx.x, gdx.y, gdx.z};
+  M3d Gx;
+  M34d Gq, Gqk;
+  for (int i = 0; i < 3; ++i) {
+    for (int c = 0; c < 3; ++c) Gx(i, c) = gfa[i] * gxa[c];
+    for (int c = 0; c < 4; ++c) { Gq(i, c) = gfa[i] * gdq[c]; Gqk(i, c) = gfa[i] * gdqk[c]; }
+  }
+  const double ua[3] = {u.x, u.y, u.z};
+  const double sv = -(double)mu * fnn / vn;
+  M3d Gvr;
+  for (int i = 0; i < 3; ++i)
+    for (int c = 0; c < 3; ++c) Gvr(i, c) = (((i == c) ? 1.0 : 0.0) - ua[i] * ua[c]) * sv;
+  const M3d Sr = skew(r);
+  const M3d Gw = mm(Gvr, Sr.tr());
+  const M3d GwK = mm(Gvr, skew(avg).tr()) * -1.0;
+  const M3d SGx = mm(Sr, Gx);
+  const M34d Tq = mm(Sr, Gq) + mm(skew(f).tr(), Jq);
+  auto add3 = [](double* o, const M3d& m) { for (int k = 0; k < 9; ++k) o[k] += m.a[k]; };
+  auto add34 = [](double* o, const M34d& m) { for (int k = 0; k < 12; ++k) o[k] += m.a[k]; };
+  add3(NRR + 0, Gvr); add3(NRR + 9, Gx); add3(NRR + 18, Gw); add3(NRR + 36, SGx);
+  add34(NRR + 54, Gq); add34(NRR + 66, Tq);
+  if (NRQ) {
+    add3(NRQ + 0, Gvr * -1.0); add3(NRQ + 18, GwK); add3(NRQ + 9, Gx * -1.0); add3(NRQ + 36, SGx * -1.0);
+    add34(NRQ + 54, Gqk); add34(NRQ + 66, mm(Sr, Gqk));
+  }
+}
+
 // Gravity, then one addForce / addTorque per contact sample in sample order
 // (Dynamic3dRigidBody.h:113-142, 228-233), one thread per body.
-__global__ void k_contact_apply(ContactConst cc, int nbodies, const float4* __restrict__ bx0,
-                                const uint32_t* __restrict__ dyn_idx, int nd, const float4* __restrict__ cf,
-                                const float4* __restrict__ ct, RigidDev* __restrict__ rig) {
+__global__ void k_contact_apply(ContactConst cc, int nbodies, const float4* __restrict__ bpos,
+                                const float4* __restrict__ bx0, const uint32_t* __restrict__ dyn_idx, int nd,
+                                const float4* __restrict__ cf, const float4* __restrict__ ct,
+                                RigidDev* __restrict__ rig) {
   const int bi = threadIdx.x;
   if (bi >= nbodies) return;
   RigidDev& b = rig[bi];
   if (!b.dynamic) return;
   const double dt = cc.dt;
+  double* Nrow = cc.mgr_net ? cc.mgr_net + (size_t)bi * nbodies * 78 : nullptr;
+  if (Nrow)  // the manager's sums of this body's row are replaced by the contact solver's
+    for (int Q = 0; Q < nbodies; ++Q)
+      if (rig[Q].dynamic)
+        for (int k = 0; k < 78; ++k) Nrow[(size_t)Q * 78 + k] = 0.0;
   for (int k = 0; k < 3; ++k) b.v[k] += (cc.g[k] * b.mass) * (b.invMass * dt);
   for (int d = 0; d < nd; ++d) {
     if ((int)bx0[dyn_idx[d]].w != bi) continue;
     const float4 f = cf[d];
     if (f.w == 0.f) continue;
+    if (Nrow) {
+      float4 rec[6];
+      for (int k = 0; k < 6; ++k) rec[k] = cc.grec[(size_t)k * nd + d];
+      const int Q = (int)rec[0].w;
+      const uint32_t j = dyn_idx[d];
+      add_contact_gradients(Nrow + (size_t)bi * 78, Q != bi ? Nrow + (size_t)Q * 78 : nullptr, b, rec, f, bpos[j],
+                            bx0[j], cc.beta, cc.mu);
+    }
     const float4 t = ct[d];
     b.v[0] += (double)f.x * (b.invMass * dt);
     b.v[1] += (double)f.y * (b.invMass * dt);
@@ -240,7 +344,8 @@ void launch_contact(const BoundaryBufs& b, const GridDesc& g, const GridBufs& gb
   if (b.nd > 0)
     hipLaunchKernelGGL(k_contact_forces, dim3(waves_blocks(b.nd)), dim3(64 * CW), 0, s, b.pos, b.x0, b.dyn_idx, b.nd,
                        g, gb.bstart, gb.bperm, h2, Kc, cc, vol0, dens0, rig, cf, ct);
-  hipLaunchKernelGGL(k_contact_apply, dim3(1), dim3(64), 0, s, cc, nbodies, b.x0, b.dyn_idx, b.nd, cf, ct, rig);
+  hipLaunchKernelGGL(k_contact_apply, dim3(1), dim3(64), 0, s, cc, nbodies, b.pos, b.x0, b.dyn_idx, b.nd, cf, ct,
+                     rig);
 }
 
 }  // namespace dfr

@@ -204,8 +204,21 @@ struct RigidStepConst {
   int use_manager;       // useRigidGradientManager: the manager's chain instead of perform_chain_rule
   int contact;           // the rigid contact solver ran this step: gravity already applied, and the
                          // nbodies - 1 later updateBoundaryForces calls add their gyroscopic terms
+  int phases;            // RS_* parts of the step to run (0: all)
+  double* mgr_net;       // manager across bodies (with contact): [nb][nb][78] handed sums, else null
 };
+// Parts of k_rigid_step, run in this order: hand the step's sums to the manager (or the per-body
+// chain rule), the single-body manager's velocity chain, the force/torque update, its position
+// chain, animate.
+enum RigidPhase { RS_HAND = 1, RS_MGR_VEL = 2, RS_FORCES = 4, RS_MGR_POS = 8, RS_ANIMATE = 16 };
 void launch_rigid_step(const RigidStepConst& rc, RigidDev* rig, hipStream_t s);
+// RigidBodyGradientManager across bodies (entries (R, RR), with the rigid contact solver):
+// J, N = [nb][nb][78] Jacobians (layout of RigidDev::Jm) and handed sums (layout of RigidDev::net).
+// The force/torque and velocity chain (rigid = 0 after the fluid coupling, 1 after the rigid
+// coupling), then the position chain; nb <= 16.
+void launch_manager_multi_velocity(const RigidStepConst& rc, RigidDev* rig, double* J, const double* N, int nb,
+                                   int rigid, hipStream_t s);
+void launch_manager_multi_position(const RigidStepConst& rc, RigidDev* rig, double* J, int nb, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // dfr_contact.hip: rigid-rigid contact (RigidContactSolver, penalty form).
@@ -216,6 +229,9 @@ struct ContactConst {
   double gdt[3];     // g dt (the gravity step the sample velocities include)
   float beta, mu;    // rigidContactBeta, rigidContactFrictionCoeff
   int gyroscopic;
+  KernelConst Ks;    // the simulation's kernel (its gradient enters the contact gradient terms)
+  float4* grec;      // [6][nd] per-sample gradient records, or null (no manager across bodies)
+  double* mgr_net;   // [nb][nb][78] the manager's handed sums, replaced by the contact terms, or null
 };
 // Rest volume and density of every boundary sample from its own body's neighbours.
 void launch_contact_rest(const BoundaryBufs& b, const GridDesc& g, const GridBufs& gb, float h2, const KernelConst& Kc,

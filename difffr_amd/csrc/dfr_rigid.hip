@@ -274,11 +274,11 @@ __device__ void manager_velocity_chain(RigidDev& b, const RigidStepConst& rc) {
 
 // perform_position_and_rotation_chain_rule (:367-423): after the velocity
 // update (the new omega), before the position update (the current q).
-__device__ void manager_position_chain(RigidDev& b, const RigidStepConst& rc) {
+__device__ void position_chain(double* Jm, const RigidDev& b, const RigidStepConst& rc) {
   const double dt = rc.dt;
-  M3d Jxv = ld3(b.Jm + 0), Jxw = ld3(b.Jm + 9);
-  const M3d Jvv = ld3(b.Jm + 18), Jvw = ld3(b.Jm + 27), Jwv = ld3(b.Jm + 36), Jww = ld3(b.Jm + 45);
-  M43d Jqv = ld43(b.Jm + 54), Jqw = ld43(b.Jm + 66);
+  M3d Jxv = ld3(Jm + 0), Jxw = ld3(Jm + 9);
+  const M3d Jvv = ld3(Jm + 18), Jvw = ld3(Jm + 27), Jwv = ld3(Jm + 36), Jww = ld3(Jm + 45);
+  M43d Jqv = ld43(Jm + 54), Jqw = ld43(Jm + 66);
   Jxv = Jxv + Jvv * dt;
   Jxw = Jxw + Jvw * dt;
   const double p[4] = {0.0, b.w[0], b.w[1], b.w[2]};
@@ -302,23 +302,38 @@ __device__ void manager_position_chain(RigidDev& b, const RigidStepConst& rc) {
   for (int k = 0; k < 12; ++k) Pw.a[k] = wv[k];
   Jqv = mm(Nq, Jqv + (mm(Pw, Jwv) + mm(Pq, Jqv)) * (dt / 2.0));
   Jqw = mm(Nq, Jqw + (mm(Pw, Jww) + mm(Pq, Jqw)) * (dt / 2.0));
-  st3(b.Jm + 0, Jxv); st3(b.Jm + 9, Jxw); st43(b.Jm + 54, Jqv); st43(b.Jm + 66, Jqw);
+  st3(Jm + 0, Jxv); st3(Jm + 9, Jxw); st43(Jm + 54, Jqv); st43(Jm + 66, Jqw);
 }
+__device__ void manager_position_chain(RigidDev& b, const RigidStepConst& rc) { position_chain(b.Jm, b, rc); }
+
+__device__ void rigid_forces(RigidDev& b, const RigidStepConst& rc);
+__device__ void rigid_animate(RigidDev& b, const RigidStepConst& rc);
 
 __global__ void k_rigid_step(RigidStepConst rc, RigidDev* __restrict__ rig) {
   const int bi = threadIdx.x;
   if (bi >= rc.nbodies) return;
   RigidDev& b = rig[bi];
   if (!b.dynamic) return;
-  if (rc.compute_gradient && !b.animated) {
+  const int ph = rc.phases ? rc.phases : (RS_HAND | RS_MGR_VEL | RS_FORCES | RS_MGR_POS | RS_ANIMATE);
+  if ((ph & RS_HAND) && rc.compute_gradient && !b.animated) {
     if (rc.use_manager) {
-      for (int k = 0; k < 78; ++k) b.mnet[k] = b.net[k];  // update_rigid_body_gradient_manager (:952-967)
+      // update_rigid_body_gradient_manager (:952-967): the (R, R) entry
+      double* dst = rc.mgr_net ? rc.mgr_net + ((size_t)bi * rc.nbodies + bi) * 78 : b.mnet;
+      for (int k = 0; k < 78; ++k) dst[k] = b.net[k];
     } else {
       chain_rule(b, rc);
     }
   }
   // the manager's chain runs for every dynamic body (SimulatorBase.cpp:1159-1166)
-  if (rc.compute_gradient && rc.use_manager) manager_velocity_chain(b, rc);
+  if ((ph & RS_MGR_VEL) && rc.compute_gradient && rc.use_manager) manager_velocity_chain(b, rc);
+  if (ph & RS_FORCES) rigid_forces(b, rc);
+  if ((ph & RS_MGR_POS) && rc.compute_gradient && rc.use_manager) manager_position_chain(b, rc);
+  if (ph & RS_ANIMATE) rigid_animate(b, rc);
+}
+
+// updateBoundaryForces and gravity (or, after the contact solver, its later
+// updateBoundaryForces calls).
+__device__ void rigid_forces(RigidDev& b, const RigidStepConst& rc) {
   const double dt = rc.dt;
   const V3d F = {b.F[0], b.F[1], b.F[2]}, T = {b.T[0], b.T[1], b.T[2]};
   if (!b.animated) {
@@ -352,8 +367,11 @@ __global__ void k_rigid_step(RigidStepConst rc, RigidDev* __restrict__ rig) {
   } else if (!b.animated) {
     for (int k = 0; k < 3; ++k) b.v[k] += (rc.g[k] * b.mass) * (b.invMass * dt);
   }
-  if (rc.compute_gradient && rc.use_manager) manager_position_chain(b, rc);
-  // animate()
+}
+
+// Dynamic3dRigidBody::animate and updateInertia.
+__device__ void rigid_animate(RigidDev& b, const RigidStepConst& rc) {
+  const double dt = rc.dt;
   for (int k = 0; k < 3; ++k) b.x[k] += b.v[k] * dt;
   const double wq[4] = {0.0, b.w[0], b.w[1], b.w[2]};
   double d[4];
@@ -366,6 +384,95 @@ __global__ void k_rigid_step(RigidStepConst rc, RigidDev* __restrict__ rig) {
   st3(b.I, In);
   st3(b.Iinv, inverse3(In));
   for (int k = 0; k < 78; ++k) { b.netlast[k] = b.net[k]; b.net[k] = 0.0; }
+}
+
+// ---------------------------------------------------------------------------
+// RigidBodyGradientManager across bodies (RigidBodyGradientManager.cpp:96-423),
+// run with the rigid contact solver: one thread per pair (R, Q) of one
+// workgroup.  The force/torque chain reads the other pairs' Jacobians, so every
+// thread forms its sums before any Jacobian changes.
+// perform_force_and_torque_chain_rule (:96-156: v and omega terms) or _Rigid
+// (:158-239: the x terms too, and the q terms of the omega0 sums; the q terms
+// written after the v0 sums are statements of their own there and do not
+// enter them), then perform_velocity_chain_rule (:276-365) with the body's
+// current force and torque (cleared once updateBoundaryForces ran).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_manager_multi_velocity(RigidStepConst rc, RigidDev* __restrict__ rig,
+                                                                double* __restrict__ J, const double* __restrict__ N,
+                                                                int nb, int rigid) {
+  const int t = threadIdx.x;
+  const int R = t / nb, Q = t % nb;
+  const bool on = t < nb * nb && rig[R].dynamic && rig[Q].dynamic;
+  M3d dFv0 = M3d::zero(), dFw0 = M3d::zero(), dTv0 = M3d::zero(), dTw0 = M3d::zero();
+  if (on) {
+    for (int K = 0; K < nb; ++K) {
+      if (!rig[K].dynamic) continue;
+      const double* n = N + ((size_t)R * nb + K) * 78;
+      const double* j = J + ((size_t)K * nb + Q) * 78;
+      const M3d Fv = ld3(n + 0), Fx = ld3(n + 9), Fw = ld3(n + 18), Tw = ld3(n + 27), Tx = ld3(n + 36),
+                Tv = ld3(n + 45);
+      const M34d Fq = ld34(n + 54), Tq = ld34(n + 66);
+      const M3d Jxv = ld3(j + 0), Jxw = ld3(j + 9), Jvv = ld3(j + 18), Jvw = ld3(j + 27), Jwv = ld3(j + 36),
+                Jww = ld3(j + 45);
+      const M43d Jqw = ld43(j + 66);
+      if (rigid) {
+        dFv0 = dFv0 + (mm(Fv, Jvv) + mm(Fw, Jwv) + mm(Fx, Jxv));
+        dTv0 = dTv0 + (mm(Tv, Jvv) + mm(Tw, Jwv) + mm(Tx, Jxv));
+        dFw0 = dFw0 + (mm(Fv, Jvw) + mm(Fw, Jww) + mm(Fx, Jxw) + mm(Fq, Jqw));
+        dTw0 = dTw0 + (mm(Tv, Jvw) + mm(Tw, Jww) + mm(Tx, Jxw) + mm(Tq, Jqw));
+      } else {
+        dFv0 = dFv0 + (mm(Fv, Jvv) + mm(Fw, Jwv));
+        dTv0 = dTv0 + (mm(Tv, Jvv) + mm(Tw, Jwv));
+        dFw0 = dFw0 + (mm(Fv, Jvw) + mm(Fw, Jww));
+        dTw0 = dTw0 + (mm(Tv, Jvw) + mm(Tw, Jww));
+      }
+    }
+  }
+  __syncthreads();
+  if (!on) return;
+  const RigidDev& b = rig[R];
+  const double dt = rc.dt;
+  double* j = J + ((size_t)R * nb + Q) * 78;
+  const M3d I = ld3(b.I), Iinv = ld3(b.Iinv), I0 = ld3(b.I0);
+  const M3d Rm = quat_to_R(b.q);
+  const V3d om = {b.w[0], b.w[1], b.w[2]};
+  M3d Jvv = ld3(j + 18), Jvw = ld3(j + 27), Jwv = ld3(j + 36), Jww = ld3(j + 45);
+  const M43d Jqv = ld43(j + 54), Jqw = ld43(j + 66);
+  const V3d T = {b.T[0], b.T[1], b.T[2]};
+  Jvv = Jvv + dFv0 * (dt * b.invMass);
+  Jvw = Jvw + dFw0 * (dt * b.invMass);
+  V3d temp = T;
+  M3d dTauw0 = dTw0, dTauv0 = dTv0;
+  if (rc.gyroscopic) {
+    const V3d Iom = mv(I, om);
+    const V3d lw = cross(Iom, om);
+    temp = {lw.x + T.x, lw.y + T.y, lw.z + T.z};
+    dTauw0 = mm(skew(Iom), Jww) + mm(skew(om).tr(), mm(I, Jww) + grad_inertia_v(b, Rm, I0, om, Jqw)) + dTw0;
+    dTauv0 = mm(skew(Iom), Jwv) + mm(skew(om).tr(), mm(I, Jwv) + grad_inertia_v(b, Rm, I0, om, Jqv)) + dTv0;
+  }
+  const V3d it = mv(Iinv, temp);
+  Jww = Jww + (mm(Iinv, grad_inertia_v(b, Rm, I0, it, Jqw)) + mm(Iinv, dTauw0)) * dt;
+  Jwv = Jwv + (mm(Iinv, grad_inertia_v(b, Rm, I0, it, Jqv)) + mm(Iinv, dTauv0)) * dt;
+  st3(j + 18, Jvv); st3(j + 27, Jvw); st3(j + 36, Jwv); st3(j + 45, Jww);
+}
+
+// perform_position_and_rotation_chain_rule (:367-423) for every dynamic pair.
+__global__ void __launch_bounds__(256) k_manager_multi_position(RigidStepConst rc, RigidDev* __restrict__ rig,
+                                                                double* __restrict__ J, int nb) {
+  const int t = threadIdx.x;
+  if (t >= nb * nb) return;
+  const int R = t / nb, Q = t % nb;
+  if (!rig[R].dynamic || !rig[Q].dynamic) return;
+  position_chain(J + ((size_t)R * nb + Q) * 78, rig[R], rc);
+}
+
+void launch_manager_multi_velocity(const RigidStepConst& rc, RigidDev* rig, double* J, const double* N, int nb,
+                                   int rigid, hipStream_t s) {
+  hipLaunchKernelGGL(k_manager_multi_velocity, dim3(1), dim3(256), 0, s, rc, rig, J, N, nb, rigid);
+}
+
+void launch_manager_multi_position(const RigidStepConst& rc, RigidDev* rig, double* J, int nb, hipStream_t s) {
+  hipLaunchKernelGGL(k_manager_multi_position, dim3(1), dim3(256), 0, s, rc, rig, J, nb);
 }
 
 // SimulatorBase::updateBoundaryParticles (SimulatorBase.cpp:1827-1858).

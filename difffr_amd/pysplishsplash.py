@@ -33,12 +33,12 @@ What differs, and why:
   (SimulatorBase.cpp:649-683).  With `useGui=False` a negative stopAt is an
   error there and here; with `useGui=True` the loop runs until the callback
   ends it (the drivers call `quit()`), or until stopAt when it is positive.
-* The RigidBodyGradientManager holds the single-body entries (R, R) of its
-  chain (scene parameter `use_gradient_manager`, the scene key
-  useRigidGradientManager); entries between two bodies, which only the rigid
-  contact solver fills, are zero.  The rigid contact solver
-  (useRigidContactSolver) runs forward only: such a scene runs without the
-  reference-mode gradient.
+* The RigidBodyGradientManager (scene parameter `use_gradient_manager`, the
+  scene key useRigidGradientManager) holds the single-body entries (R, R) of
+  its chain; with the rigid contact solver (useRigidContactSolver) it holds
+  the entries between bodies too, which the contact solver's gradient terms
+  fill (at most 16 bodies).  The exact adjoint does not cover the contact
+  solver.
 * A reference C++ `assert` or crash on a bad index is an IndexError here.
 """
 from __future__ import annotations

@@ -433,8 +433,6 @@ def build_scene(sf: SceneFile, steps: Optional[int] = None, seed: int = 0) -> sc
              rigid_contact_beta=float(cfg["rigidContactBeta"]), rigid_contact_gamma=float(cfg["rigidContactGamma"]),
              rigid_contact_support_radius_factor=float(cfg["rigidContactSupportRadiusFactor"]),
              rigid_contact_friction=float(cfg["rigidContactFrictionCoeff"]))
-    if p["use_rigid_contact"]:
-        p["compute_gradient"] = 0  # the contact solver runs forward only here
     p.update(fluid_material(sf))
     scene_dir = os.path.dirname(sf.path)
     xs, vs = [], []

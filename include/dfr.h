@@ -83,8 +83,9 @@ typedef struct dfr_params {
   double cfl_min_time_step_size;   /* default 1e-4 */
   double cfl_max_time_step_size;   /* default 5e-3 */
   /* Rigid-rigid contact between the bodies' boundary samples (RigidContactSolver, penalty form;
-     Configuration keys of SceneLoader.cpp:58-71).  Forward only: a context with contact refuses the
-     reference-mode gradient (compute_gradient) and the exact adjoint. */
+     Configuration keys of SceneLoader.cpp:58-71).  With the reference-mode gradient and
+     use_gradient_manager the contact solver's gradient terms fill the manager's entries between
+     bodies (dfr_get_gradient_manager(R, RR), at most 16 bodies); the exact adjoint refuses contact. */
   int32_t use_rigid_contact;       /* useRigidContactSolver, default 0 */
   double rigid_contact_beta;       /* rigidContactBeta, penalty stiffness (default 1) */
   double rigid_contact_gamma;      /* rigidContactGamma, rest volume factor (default 0.7) */
@@ -199,8 +200,10 @@ int dfr_get_net_gradients(const dfr_ctx* ctx, int body, double out[78]);
  * RigidBodyGradientManager.h:24-52): jac[78] in dfr_get_jacobians' layout
  * (get_grad_x_to_v0(R, RR), get_grad_q_to_v0(R, RR), ...), net[78] in
  * dfr_get_net_gradients' layout (get_grad_net_force_to_vn(R, RR), ...).  Either
- * pointer may be NULL.  Entries with R != RR are zero (no rigid contact solver),
- * as are all entries of a static body or without use_gradient_manager. */
+ * pointer may be NULL.  Entries with R != RR are zero without the rigid contact
+ * solver (with it, its gradient terms fill them: RigidContactSolver.cpp:471-553,
+ * RigidBodyGradientManager.cpp:158-239); all entries of a static body, or without
+ * use_gradient_manager, are zero. */
 int dfr_get_gradient_manager(const dfr_ctx* ctx, int R, int RR, double jac[78], double net[78]);
 
 /* Clears the forward-mode Jacobians to their initial values

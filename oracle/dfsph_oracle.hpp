@@ -302,6 +302,13 @@ template <typename RF, typename RR> class Sim {
   Params P;
   Cubic<RF> K;
   Cubic<RF> KRC;  // rigid-contact kernel, radius rigidContactSupportRadiusFactor r (RigidContactSolver.cpp:1437-1455)
+  // RigidBodyGradientManager across bodies, entries (R, RR) (RigidBodyGradientManager.h:58-65), run when the
+  // rigid contact solver couples the bodies: J = grad_{x,v,omega,q}_n_to_{v0,omega0}[R][RR] and the sums
+  // N = grad_net_{force,torque}_to_{v,x,omega,q}_n[R][RR] it was last handed.
+  struct MgrJ { M3<RR> vv, vw, wv, ww, xv, xw; M43<RR> qv, qw; };
+  struct MgrN { M3<RR> Fv, Fx, Fw, Tw, Tx, Tv; M34<RR> Fq, Tq; };
+  std::vector<MgrJ> MJ;
+  std::vector<MgrN> MN;
   Cohesion<RF> KC;
   Adhesion<RF> KA;
   RF h, dt, rho0, V_f, m_f;
@@ -398,6 +405,15 @@ template <typename RF, typename RR> class Sim {
     nFB.assign(bodies.size(), {});
     nBF.assign(bodies.size(), {});
     if (P.useRigidContact) contact_rest();
+    reset_multi_manager();
+  }
+  bool multi_manager() const { return P.useRigidContact && P.computeGradient && P.useGradientManager; }
+  // RigidBodyGradientManager::reset (RigidBodyGradientManager.cpp:474-523) for every pair.
+  void reset_multi_manager() {
+    const size_t nb = bodies.size();
+    MJ.assign(nb * nb, MgrJ{});
+    MN.assign(nb * nb, MgrN{});
+    for (size_t R = 0; R < nb; ++R) { MJ[R * nb + R].vv.setIdentity(); MJ[R * nb + R].ww.setIdentity(); }
   }
 
   // ---- rigid-rigid contact (RigidContactSolver, penalty form) -------------
@@ -483,6 +499,10 @@ template <typename RF, typename RR> class Sim {
     sample_grid(G);
     const RF W0 = KRC.w0, beta = RF(P.rigidContactBeta), mu = RF(P.rigidContactFriction);
     for (auto& b : bodies) { b.cF = V3<RR>(); b.cT = V3<RR>(); }
+    // grad_net_{force,torque}_to_*_before_contact[R][RR] (solveRigidContactPenalty :429-441, :471-553)
+    const size_t NBD = bodies.size();
+    const bool grads = multi_manager();
+    std::vector<MgrN> CN(grads ? NBD * NBD : 0);
     std::vector<uint32_t> nb;
     for (size_t a = 0; a < G.pts.size(); ++a) {
       const uint32_t R = G.id[a].first, r = G.id[a].second;
@@ -496,13 +516,33 @@ template <typename RF, typename RR> class Sim {
       RF dens = b.cvol0[r] * W0;
       V3<RF> sx, sv;
       RF sw = 0;
+      // gradient sums over the other bodies' neighbours, in RR (double): d density / d x_R
+      // (the simulation's kernel gradient), / d q_R, / d q_Rk, and the weighted r_k;
+      // RR = the body of the last such neighbour (point sets in body order)
+      V3<RR> gdx, avg_rk;
+      RR gdq[4] = {0, 0, 0, 0}, gdqk[4] = {0, 0, 0, 0};
+      uint32_t RRb = R;
       for (uint32_t k : nb) {
         const auto& bk = bodies[G.id[k].first];
         const uint32_t kk = G.id[k].second;
         const RF w = KRC.W(xr - bk.x[kk]);
         dens += bk.cvol0[kk] * w;
         if (G.id[k].first == R) { sx += bk.x[kk] * w; sw += w; }
-        else sv += vel[G.id[k].first][kk] * w;
+        else {
+          sv += vel[G.id[k].first][kk] * w;
+          if (grads) {
+            RRb = G.id[k].first;
+            const V3<RR> gw = V3<RR>(K.gradW(xr - bk.x[kk])) * RR(bk.cvol0[kk]);
+            gdx += gw;
+            avg_rk += (V3<RR>(bk.x[kk]) - bk.xrb) * RR(w);
+            const M34<RR> Jq = grad_Rqp_to_q(b.q, V3<RR>(b.x0[r]));
+            const M34<RR> Jqk = grad_Rqp_to_q(bk.q, V3<RR>(bk.x0[kk]));
+            for (int c = 0; c < 4; ++c) {
+              gdq[c] += gw.x * Jq(0, c) + gw.y * Jq(1, c) + gw.z * Jq(2, c);
+              gdqk[c] -= gw.x * Jqk(0, c) + gw.y * Jqk(1, c) + gw.z * Jqk(2, c);
+            }
+          }
+        }
       }
       const RF ratio = dens / b.cdens0[r];
       if (!(ratio > RF(1))) continue;
@@ -517,6 +557,42 @@ template <typename RF, typename RR> class Sim {
       body_add_torque(b, V3<RR>(rr.cross(f)), dtR);
       b.cF += V3<RR>(f);
       b.cT += V3<RR>(rr.cross(f));
+      if (grads) {
+        // :471-553: every density derivative through f_n = -beta (rho / rho0 - 1) n with
+        // d f_n / d rho taken as -beta n, the friction through |f_n| and normalized(v_rel)
+        const V3<RR> nR(n), fR(f), fnR(fn), urR(unit), vr(vrel), rrR(rr);
+        const RR muR = RR(mu);
+        avg_rk = avg_rk * (RR(1) / RR(sw));
+        const V3<RR> gfn = nR * (-RR(beta));
+        const RR fnn = fnR.norm();
+        const V3<RR> gff = urR * (-muR * fnR.dot(gfn) / fnn);
+        const V3<RR> gf = gfn + gff;
+        M3<RR> Gx;
+        M34<RR> Gq, Gqk;
+        for (int i = 0; i < 3; ++i) {
+          for (int c = 0; c < 3; ++c) Gx(i, c) = gf[i] * gdx[c];
+          for (int c = 0; c < 4; ++c) { Gq(i, c) = gf[i] * gdq[c]; Gqk(i, c) = gf[i] * gdqk[c]; }
+        }
+        const M3<RR> Gvr = (M3<RR>::Identity() - outer(urR, urR)) * (-muR * fnn / vr.norm());
+        const M3<RR> Gw = Gvr * skew(rrR).transpose();
+        const M3<RR> GwK = -(Gvr * skew(avg_rk).transpose());
+        MgrN& o = CN[R * NBD + R];
+        o.Fx += Gx;
+        o.Tx += skew(rrR) * Gx;
+        o.Fq += Gq;
+        o.Tq += skew(rrR) * Gq + skew(fR).transpose() * grad_Rqp_to_q(b.q, V3<RR>(b.x0[r]));
+        o.Fv += Gvr;
+        o.Fw += Gw;
+        if (RRb != R) {
+          MgrN& c = CN[R * NBD + RRb];
+          c.Fv += -Gvr;
+          c.Fw += GwK;
+          c.Fx += -Gx;
+          c.Tx += -(skew(rrR) * Gx);
+          c.Fq += Gqk;
+          c.Tq += skew(rrR) * Gqk;
+        }
+      }
     }
     for (size_t call = 0; call < bodies.size(); ++call)
       for (auto& b : bodies) {
@@ -529,6 +605,12 @@ template <typename RF, typename RR> class Sim {
         }
         if (!b.animated) { body_add_force(b, F, dtR); body_add_torque(b, T, dtR); }
       }
+    // update_rigid_body_gradient_manager (:1341-1370): the manager's sums of every
+    // dynamic pair are replaced by the contact solver's
+    if (grads)
+      for (size_t R = 0; R < NBD; ++R)
+        for (size_t Q = 0; Q < NBD; ++Q)
+          if (bodies[R].dynamic && bodies[Q].dynamic) MN[R * NBD + Q] = CN[R * NBD + Q];
   }
 
   static void update_inertia(Body<RF, RR>& b) {  // Dynamic3dRigidBody.h:215-219
@@ -1143,8 +1225,17 @@ template <typename RF, typename RR> class Sim {
     for (auto& b : bodies)
       if (b.dynamic && !b.animated) {
         accumulate_and_reset_gradient(b);
-        if (P.useGradientManager) hand_to_manager(b);
-        else perform_chain_rule(b, RR(dt));
+        if (P.useGradientManager) {
+          hand_to_manager(b);
+          if (multi_manager()) {  // the (R, R) entry of the manager's table
+            const size_t nb = bodies.size(), R = size_t(&b - &bodies[0]);
+            MgrN& n = MN[R * nb + R];
+            n.Fv = b.nFv; n.Fx = b.nFx; n.Fw = b.nFw; n.Fq = b.nFq;
+            n.Tv = b.nTv; n.Tx = b.nTx; n.Tw = b.nTw; n.Tq = b.nTq;
+          }
+        } else {
+          perform_chain_rule(b, RR(dt));
+        }
       }
   }
 
@@ -1207,6 +1298,198 @@ template <typename RF, typename RR> class Sim {
     const M43<RR> Pw = grad_omega_q_product_to_omega(q);
     b.mJqv = Nq * (b.mJqv + (Pw * b.mJwv + Pq * b.mJqv) * (dtR / RR(2.)));
     b.mJqw = Nq * (b.mJqw + (Pw * b.mJww + Pq * b.mJqw) * (dtR / RR(2.)));
+  }
+
+  // Test hook that pins the contact solver's gradient terms by finite
+  // differences; not part of the reference's step.  At the first step's state
+  // (positions as they are, body velocities v0 + g dt and omega0), the contact
+  // samples of body `obs` (a neighbour on another body, density ratio > 1), with
+  // their n, |f_n| and weights held, and body `pert` moved by dx, its raw
+  // quaternion by dq (not renormalised: the rotation then in the homogeneous form
+  // the reference differentiates), its v by dv and omega by dw:
+  //   out[0:3] = sum -beta n density,  out[3:6] = sum r x (-beta n density) (r held),
+  //   out[6:9] = sum -mu |f_n| normalized(v_rel).
+  int contact_probe(int obs, int pert, const double* dx, const double* dq, const double* dv, const double* dw,
+                    double* out) const {
+    const size_t nb = bodies.size();
+    if (obs < 0 || pert < 0 || size_t(obs) >= nb || size_t(pert) >= nb) return -1;
+    const V3<RR> gR(RR(P.gravity[0]), RR(P.gravity[1]), RR(P.gravity[2]));
+    const RR dtR = RR(dt);
+    auto state = [&](bool perturbed, std::vector<std::vector<V3<RF>>>& X, std::vector<std::vector<V3<RF>>>& Vv) {
+      X.assign(nb, {}); Vv.assign(nb, {});
+      for (size_t k = 0; k < nb; ++k) {
+        const auto& b = bodies[k];
+        Quat<RR> q = b.q;
+        V3<RR> xr = b.xrb, v = b.v0 + gR * dtR, w = b.w0;
+        if (!b.dynamic) { v = V3<RR>(); w = V3<RR>(); }
+        if (perturbed && int(k) == pert) {
+          q.w += dq[0]; q.x += dq[1]; q.y += dq[2]; q.z += dq[3];
+          xr += V3<RR>(dx[0], dx[1], dx[2]);
+          v += V3<RR>(dv[0], dv[1], dv[2]);
+          w += V3<RR>(dw[0], dw[1], dw[2]);
+        }
+        X[k].resize(b.x0.size()); Vv[k].resize(b.x0.size());
+        for (size_t j = 0; j < b.x0.size(); ++j) {
+          // the rotation as the homogeneous form (w^2 - |u|^2) p + 2 (u.p) u + 2 w u x p,
+          // which get_grad_Rqp_to_q differentiates (GradientUtils.cpp:9-20); equal to
+          // the rotation on unit quaternions
+          const V3<RR> p0(b.x0[j]), u = q.vec();
+          const V3<RR> rp = perturbed && int(k) == pert
+                                ? p0 * (q.w * q.w - u.dot(u)) + u * (RR(2) * u.dot(p0)) + u.cross(p0) * (RR(2) * q.w)
+                                : q.rotate(p0);
+          const V3<RR> xj = rp + xr;
+          X[k][j] = V3<RF>(xj);
+          Vv[k][j] = V3<RF>(v + w.cross(xj - xr));
+        }
+      }
+    };
+    std::vector<std::vector<V3<RF>>> X0, V0, X1, V1;
+    state(false, X0, V0);
+    state(true, X1, V1);
+    std::vector<V3<RF>> pts;
+    std::vector<std::pair<uint32_t, uint32_t>> id;
+    for (size_t k = 0; k < nb; ++k)
+      for (size_t j = 0; j < X0[k].size(); ++j) { pts.push_back(X0[k][j]); id.push_back({uint32_t(k), uint32_t(j)}); }
+    Grid<RF> G;
+    G.build(pts, h * RF(1.001));
+    const RF W0 = KRC.w0, beta = RF(P.rigidContactBeta), mu = RF(P.rigidContactFriction);
+    const auto& b = bodies[obs];
+    V3<RR> S, ST, SF;
+    std::vector<uint32_t> nbl;
+    for (size_t a = 0; a < pts.size(); ++a) {
+      if (id[a].first != uint32_t(obs)) continue;
+      const uint32_t r = id[a].second;
+      G.query(pts[a], pts, h * h, int64_t(a), nbl);
+      bool contact = false;
+      for (uint32_t k : nbl) contact |= id[k].first != uint32_t(obs);
+      if (!contact) continue;
+      RF dens = b.cvol0[r] * W0, sw = 0;
+      V3<RF> sx, sv;
+      for (uint32_t k : nbl) {
+        const RF w = KRC.W(pts[a] - pts[k]);
+        dens += bodies[id[k].first].cvol0[id[k].second] * w;
+        if (id[k].first == uint32_t(obs)) { sx += pts[k] * w; sw += w; }
+        else sv += V0[id[k].first][id[k].second] * w;
+      }
+      const RF ratio = dens / b.cdens0[r];
+      if (!(ratio > RF(1))) continue;
+      const V3<RF> n = pts[a] - sx * (RF(1) / sw);
+      const RF fnn = (n * (-beta * (ratio - RF(1)))).norm();
+      // perturbed density and relative velocity, weights of the unperturbed state
+      const V3<RF> xr1 = X1[obs][r];
+      RF dens1 = b.cvol0[r] * W0;
+      V3<RF> sv1;
+      for (uint32_t k : nbl) {
+        const RF vk = bodies[id[k].first].cvol0[id[k].second];
+        if (id[k].first == uint32_t(obs)) {
+          dens1 += vk * KRC.W(pts[a] - pts[k]);  // the own body's part is held: it is invariant under the body's motion
+        } else {
+          dens1 += vk * KRC.W(xr1 - X1[id[k].first][id[k].second]);
+          sv1 += V1[id[k].first][id[k].second] * KRC.W(pts[a] - pts[k]);
+        }
+      }
+      const V3<RR> fd = V3<RR>(n) * (-RR(beta) * RR(dens1));
+      S += fd;
+      ST += (V3<RR>(pts[a]) - b.xrb).cross(fd);
+      const V3<RF> vrel = V1[obs][r] - sv1 * (RF(1) / sw);
+      const RF vn = vrel.norm();
+      SF += V3<RR>(vrel * (-mu * fnn / vn));
+    }
+    for (int c = 0; c < 3; ++c) { out[c] = double(S[c]); out[3 + c] = double(ST[c]); out[6 + c] = double(SF[c]); }
+    return 0;
+  }
+
+  // ---- RigidBodyGradientManager across bodies (with the rigid contact solver) ----
+  // perform_force_and_torque_chain_rule (RigidBodyGradientManager.cpp:96-156: the v
+  // and omega terms) or, after the contact solver, _Rigid (:158-239: the x terms too,
+  // and the q terms of the omega0 sums; the q terms written after the v0 sums,
+  // :195-196 and :200-201, are statements of their own and do not enter them), over
+  // every dynamic R, RR and Rk; then perform_velocity_chain_rule (:276-365) with
+  // the body's current force and torque (cleared once updateBoundaryForces ran).
+  struct MgrD { M3<RR> Fv0, Fw0, Tv0, Tw0; };
+  void multi_velocity_chain(bool rigid) {
+    const size_t nb = bodies.size();
+    const RR dtR = RR(dt);
+    std::vector<MgrD> d(nb * nb);
+    for (size_t R = 0; R < nb; ++R) {
+      if (!bodies[R].dynamic) continue;
+      for (size_t Q = 0; Q < nb; ++Q) {
+        if (!bodies[Q].dynamic) continue;
+        MgrD& o = d[R * nb + Q];
+        for (size_t K = 0; K < nb; ++K) {
+          if (!bodies[K].dynamic) continue;
+          const MgrN& n = MN[R * nb + K];
+          const MgrJ& J = MJ[K * nb + Q];
+          if (rigid) {
+            o.Fv0 += n.Fv * J.vv + n.Fw * J.wv + n.Fx * J.xv;
+            o.Tv0 += n.Tv * J.vv + n.Tw * J.wv + n.Tx * J.xv;
+            o.Fw0 += n.Fv * J.vw + n.Fw * J.ww + n.Fx * J.xw + n.Fq * J.qw;
+            o.Tw0 += n.Tv * J.vw + n.Tw * J.ww + n.Tx * J.xw + n.Tq * J.qw;
+          } else {
+            o.Fv0 += n.Fv * J.vv + n.Fw * J.wv;
+            o.Tv0 += n.Tv * J.vv + n.Tw * J.wv;
+            o.Fw0 += n.Fv * J.vw + n.Fw * J.ww;
+            o.Tw0 += n.Tv * J.vw + n.Tw * J.ww;
+          }
+        }
+      }
+    }
+    for (size_t R = 0; R < nb; ++R) {
+      auto& b = bodies[R];
+      if (!b.dynamic) continue;
+      const M3<RR> I = b.I, Iinv = b.Iinv;
+      const V3<RR> om = b.wrb;
+      V3<RR> F, T;
+      force_and_torque(b, F, T);
+      const V3<RR> temp_v = P.gyroscopic ? (I * om).cross(om) + T : T;
+      for (size_t Q = 0; Q < nb; ++Q) {
+        if (!bodies[Q].dynamic) continue;
+        MgrJ& J = MJ[R * nb + Q];
+        const MgrD& o = d[R * nb + Q];
+        J.vv = J.vv + o.Fv0 * (dtR * b.invMass);
+        J.vw = J.vw + o.Fw0 * (dtR * b.invMass);
+        M3<RR> dTau_w0 = o.Tw0, dTau_v0 = o.Tv0;
+        if (P.gyroscopic) {
+          dTau_w0 = skew(I * om) * J.ww + skew(om).transpose() * (I * J.ww + grad_inertia_v(b, om, J.qw)) + o.Tw0;
+          dTau_v0 = skew(I * om) * J.wv + skew(om).transpose() * (I * J.wv + grad_inertia_v(b, om, J.qv)) + o.Tv0;
+        }
+        const M3<RR> gIi_w0 = Iinv * grad_inertia_v(b, Iinv * temp_v, J.qw);
+        const M3<RR> gIi_v0 = Iinv * grad_inertia_v(b, Iinv * temp_v, J.qv);
+        J.ww = J.ww + (gIi_w0 + Iinv * dTau_w0) * dtR;
+        J.wv = J.wv + (gIi_v0 + Iinv * dTau_v0) * dtR;
+      }
+    }
+  }
+  // perform_position_and_rotation_chain_rule (:367-423) for every dynamic pair.
+  void multi_position_chain() {
+    const size_t nb = bodies.size();
+    const RR dtR = RR(dt);
+    for (size_t R = 0; R < nb; ++R) {
+      auto& b = bodies[R];
+      if (!b.dynamic) continue;
+      const Quat<RR> q = b.q;
+      const Quat<RR> p(0, b.wrb.x, b.wrb.y, b.wrb.z);
+      Quat<RR> nq = q;
+      const Quat<RR> pq = p * nq;
+      const RR hh = dtR * RR(0.5);
+      nq.x += hh * pq.x; nq.y += hh * pq.y; nq.z += hh * pq.z; nq.w += hh * pq.w;
+      const Quat<RR> nqn = nq.normalized();
+      const RR qv[4] = {nqn.w, nqn.x, nqn.y, nqn.z};
+      M4<RR> Nq;
+      const RR nn = nq.norm();
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) Nq(r, c) = ((r == c ? RR(1) : RR(0)) - qv[r] * qv[c]) / nn;
+      const M4<RR> Pq = grad_p_q_product_to_q(p);
+      const M43<RR> Pw = grad_omega_q_product_to_omega(q);
+      for (size_t Q = 0; Q < nb; ++Q) {
+        if (!bodies[Q].dynamic) continue;
+        MgrJ& J = MJ[R * nb + Q];
+        J.xv = J.xv + J.vv * dtR;
+        J.xw = J.xw + J.vw * dtR;
+        J.qv = Nq * (J.qv + (Pw * J.wv + Pq * J.qv) * (dtR / RR(2.)));
+        J.qw = Nq * (J.qw + (Pw * J.ww + Pq * J.qw) * (dtR / RR(2.)));
+      }
+    }
   }
 
   // RigidBody3dBoundarySimulator::velocityTimeStep (:278-320) with
@@ -1451,9 +1734,19 @@ template <typename RF, typename RR> class Sim {
     time = time + h_old;
     backward_per_step();
     const bool mgr = P.computeGradient && P.useGradientManager;
-    if (mgr) for (auto& b : bodies) if (b.dynamic) manager_velocity_chain(b);
-    rigid_velocity_step();
-    if (mgr) for (auto& b : bodies) if (b.dynamic) manager_position_chain(b);
+    if (multi_manager()) {
+      // SimulatorBase::timeStepNoGUI (SimulatorBase.cpp:1159-1170): after the fluid
+      // coupling (with fluid), the boundary velocity step, after the rigid coupling
+      // (RigidBodyGradientManager.cpp:455-468)
+      if (!x.empty()) multi_velocity_chain(false);
+      rigid_velocity_step();
+      multi_velocity_chain(true);
+      multi_position_chain();
+    } else {
+      if (mgr) for (auto& b : bodies) if (b.dynamic) manager_velocity_chain(b);
+      rigid_velocity_step();
+      if (mgr) for (auto& b : bodies) if (b.dynamic) manager_position_chain(b);
+    }
     if (tp) {
       for (size_t k = 0; k < bodies.size(); ++k) { tp->bodies[k].F = bodies[k].F_last; tp->bodies[k].T = bodies[k].T_last; }
       tp->Iv = stats.iterationsV; tp->Ip = stats.iterations;
@@ -1550,6 +1843,7 @@ template <typename RF, typename RR> class Sim {
       update_inertia(b);
       b.reset_jacobians();
     }
+    reset_multi_manager();
     update_boundary_particles(true);
     time = 0; stepCount = 0;
   }

@@ -47,6 +47,9 @@ struct OracleBase {
   virtual int set_force_model(int body, int start, const double* f0t0, const double* A, const double* B, int flags) = 0;
   virtual int model_step(int n) = 0;
   virtual int get_manager(int body, double* jac, double* net) const = 0;
+  virtual int get_manager_pair(int R, int Q, double* jac, double* net) const = 0;
+  virtual int contact_probe(int obs, int pert, const double* dx, const double* dq, const double* dv,
+                            const double* dw, double* out) const = 0;
 };
 
 template <typename RF, typename RR> struct Oracle : OracleBase {
@@ -378,6 +381,29 @@ template <typename RF, typename RR> struct Oracle : OracleBase {
   }
   // RigidBodyGradientManager entries (body, body): Jacobians in get_jacobians'
   // layout, handed sums in get_net_gradients' layout.
+  // Entry (R, RR) of the manager's table across bodies (with the rigid contact
+  // solver and the manager); the layout of get_manager.
+  int get_manager_pair(int R, int Q, double* jac, double* net) const override {
+    const int nb = int(sim.bodies.size());
+    if (R < 0 || Q < 0 || R >= nb || Q >= nb || sim.MJ.size() != size_t(nb) * nb) return -1;
+    const auto& J = sim.MJ[size_t(R) * nb + Q];
+    const auto& N = sim.MN[size_t(R) * nb + Q];
+    const M3<RR>* j3[6] = {&J.xv, &J.xw, &J.vv, &J.vw, &J.wv, &J.ww};
+    int o = 0;
+    for (auto m : j3) for (int i = 0; i < 9; ++i) jac[o++] = double(m->a[i]);
+    for (int i = 0; i < 12; ++i) jac[o++] = double(J.qv.a[i]);
+    for (int i = 0; i < 12; ++i) jac[o++] = double(J.qw.a[i]);
+    const M3<RR>* n3[6] = {&N.Fv, &N.Fx, &N.Fw, &N.Tw, &N.Tx, &N.Tv};
+    o = 0;
+    for (auto m : n3) for (int i = 0; i < 9; ++i) net[o++] = double(m->a[i]);
+    for (int i = 0; i < 12; ++i) net[o++] = double(N.Fq.a[i]);
+    for (int i = 0; i < 12; ++i) net[o++] = double(N.Tq.a[i]);
+    return 0;
+  }
+  int contact_probe(int obs, int pert, const double* dx, const double* dq, const double* dv, const double* dw,
+                    double* out) const override {
+    return sim.contact_probe(obs, pert, dx, dq, dv, dw, out);
+  }
   int get_manager(int body, double* jac, double* net) const override {
     if (body < 0 || body >= int(sim.bodies.size())) return -1;
     const auto& b = sim.bodies[body];
@@ -473,6 +499,13 @@ int orc_recorded_iterations(void* h, int* iv, int* ip, int cap) {
 }
 int orc_tape_field(void* h, int step, const char* name, int iter, double* out) {
   return static_cast<OracleBase*>(h)->tape_field(step, name, iter, out);
+}
+int orc_get_manager_pair(void* h, int R, int Q, double* jac, double* net) {
+  return static_cast<OracleBase*>(h)->get_manager_pair(R, Q, jac, net);
+}
+int orc_contact_probe(void* h, int obs, int pert, const double* dx, const double* dq, const double* dv,
+                      const double* dw, double* out) {
+  return static_cast<OracleBase*>(h)->contact_probe(obs, pert, dx, dq, dv, dw, out);
 }
 int orc_adjoint(void* h, const double* seed, double* out) {
   try {

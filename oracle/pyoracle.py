@@ -96,6 +96,8 @@ def lib():
         L.orc_set_force_model.argtypes = [P, C.c_int, C.c_int, dp, dp, dp, C.c_int]
         L.orc_model_step.argtypes = [P, C.c_int]
         L.orc_get_manager.argtypes = [P, C.c_int, dp, dp]
+        L.orc_get_manager_pair.argtypes = [P, C.c_int, C.c_int, dp, dp]
+        L.orc_contact_probe.argtypes = [P, C.c_int, C.c_int, dp, dp, dp, dp, dp]
         assert L.orc_params_size() == C.sizeof(Params), "oracle Params layout mismatch"
         _lib = L
     return _lib
@@ -329,6 +331,25 @@ class OracleSim:
         n = np.empty(78)
         lib().orc_get_manager(self.h, body, _dp(j), _dp(n))
         return j, n
+
+    def gradient_manager_pair(self, R: int, RR: int):
+        """Entries (R, RR) of the manager's table across bodies (with the rigid
+        contact solver and the manager): (Jacobians[78], handed sums[78])."""
+        j = np.empty(78)
+        n = np.empty(78)
+        if lib().orc_get_manager_pair(self.h, R, RR, _dp(j), _dp(n)) != 0:
+            raise IndexError((R, RR))
+        return j, n
+
+    def contact_probe(self, obs: int, pert: int, dx=(0, 0, 0), dq=(0, 0, 0, 0), dv=(0, 0, 0), dw=(0, 0, 0)):
+        """Test hook: at the first step's state, body `obs`'s contact sums
+        (sum -beta n density, sum r x (-beta n density), sum friction) with body
+        `pert` moved by dx, dq (raw quaternion), dv, dw; n, |f_n| and weights held."""
+        a = [np.ascontiguousarray(t, np.float64) for t in (dx, dq, dv, dw)]
+        out = np.empty(9)
+        if lib().orc_contact_probe(self.h, obs, pert, *[_dp(t) for t in a], _dp(out)) != 0:
+            raise IndexError((obs, pert))
+        return out[0:3], out[3:6], out[6:9]
 
     def net_gradients(self, body: int) -> np.ndarray:
         o = np.empty(78)

@@ -2,6 +2,10 @@
 oracle, which tests/test_oracle_contact.py pins against a direct evaluation of
 the reference's definitions.
 
+With the reference-mode gradient: the manager's table across bodies (the
+contact solver's gradient terms, its chains after the fluid and the rigid
+coupling) and the per-body chain rule against the oracle.
+
 Tolerances: one step's contact-induced velocity change (v - v0 - g dt, and the
 angular one) within 1e-4 of its size (FP32 kernel sums in a different order;
 a sample whose density ratio sits within rounding of 1 may call addTorque on
@@ -100,12 +104,85 @@ def test_contact_rollout_in_a_tank(gpu_ready):
     g.close()
 
 
-def test_contact_refuses_gradients(gpu_ready):
-    sc = _ball_scene()
-    sc.params["compute_gradient"] = 1
-    with pytest.raises(DfrError, match="forward only"):
-        Simulation(sc)
-    sim = Simulation(_ball_scene())
-    with pytest.raises(DfrError, match="forward only"):
+def test_contact_refuses_the_exact_adjoint(gpu_ready):
+    sim = Simulation(_ball_scene(), params_override={"compute_gradient": 0})
+    with pytest.raises(DfrError, match="exact adjoint"):
         sim.adjoint_begin(0)
     sim.close()
+
+
+def _jac_err(a, b, floor):
+    scale = np.abs(b).max()
+    return float(np.abs(a - b).max() / scale), float(np.abs(floor - b).max() / scale)
+
+
+@pytest.mark.parametrize("mu", [0.0, 0.4])
+def test_manager_across_bodies_matches_oracle(gpu_ready, mu):
+    """Two spheres in contact (no fluid), reference-mode gradient with the
+    manager: every (R, RR) entry of its table, Jacobians and the sums the
+    contact solver handed it, against the FP32 oracle over 12 steps; within
+    1e-3 of each block's largest entry or three times the FP32-vs-FP64 oracle
+    spread."""
+    sc = _ball_scene(gyro=1, mu=mu)
+    sc.params.update(compute_gradient=1, use_gradient_manager=1)
+    print(f"[contact manager] friction {mu}", flush=True)
+    g = Simulation(sc)
+    o = OracleSim(sc, "f32_rigid64")
+    o64 = OracleSim(sc, "f64")
+    g.step(12)
+    o.step(12)
+    o64.step(12)
+    for R in range(2):
+        for Q in range(2):
+            jg, ng = g.gradient_manager_raw(R, Q)
+            jo, no = o.gradient_manager_pair(R, Q)
+            j64, n64 = o64.gradient_manager_pair(R, Q)
+            ej, fj = _jac_err(jg, jo, j64)
+            en, fn = _jac_err(ng, no, n64)
+            print(f"  ({R},{Q}) J err {ej:.1e} (FP32 vs FP64 oracle {fj:.1e}), handed sums err {en:.1e} ({fn:.1e})",
+                  flush=True)
+            assert np.all(np.isfinite(jg)) and np.all(np.isfinite(ng))
+            assert ej < max(1e-3, 3 * fj) and en < max(1e-3, 3 * fn)
+    # the cross entries are live: body 1's position depends on body 0's initial velocity
+    assert np.abs(g.gradient_manager_raw(1, 0)[0][0:9]).max() > 1e-4
+    g.close()
+
+
+@pytest.mark.parametrize("manager", [0, 1])
+def test_contact_gradient_rollout_in_a_tank(gpu_ready, manager):
+    """The tank rollout of two colliding spheres with the reference-mode
+    gradient: without the manager the per-body chain rule (run before the
+    contact forces, as backwardPerStep is), with it the manager's chains after
+    the fluid and after the rigid coupling; the Jacobians of both spheres
+    against the FP32 oracle after 30 steps, within 1e-3 or three times the
+    FP32-vs-FP64 spread."""
+    sc = scenes.submerged_body(body="sphere", steps=30)
+    other = sc.bodies[1]
+    x2 = other.x_rb + np.array([0.135, 0.0, 0.0])
+    sc.bodies.append(scenes.Body(other.x0.copy(), True, x2, np.array([1.0, 0, 0, 0]), 500.0,
+                                 np.array([-0.5, 0.0, 0.0]), np.zeros(3)))
+    keep = np.linalg.norm(sc.fluid_x - x2, axis=1) > 0.06 + 0.01
+    sc.fluid_x, sc.fluid_v = sc.fluid_x[keep], sc.fluid_v[keep]
+    sc.params.update(compute_gradient=1, use_gradient_manager=manager, use_rigid_contact=1, rigid_contact_beta=2000.0)
+    print(f"[contact gradient rollout] manager {manager} n={sc.n_fluid}", flush=True)
+    g = Simulation(sc)
+    o = OracleSim(sc, "f32_rigid64")
+    o64 = OracleSim(sc, "f64")
+    for s in range(sc.steps):
+        g.step(1)
+        st = g.stats()
+        for oo in (o, o64):
+            oo.set_fixed_iterations([st["iterations_v"]], [st["iterations"]])
+            oo.step(1)
+    pairs = [(1, 1), (1, 2), (2, 1), (2, 2)] if manager else [(1, 1), (2, 2)]
+    for R, Q in pairs:
+        if manager:
+            jg, jo, j64 = g.gradient_manager_raw(R, Q)[0], o.gradient_manager_pair(R, Q)[0], \
+                o64.gradient_manager_pair(R, Q)[0]
+        else:
+            jg, jo, j64 = g.jacobians_raw(R), o.jacobians(R), o64.jacobians(R)
+        e, f = _jac_err(jg, jo, j64)
+        print(f"  ({R},{Q}) J err {e:.1e} (FP32 vs FP64 oracle {f:.1e})", flush=True)
+        assert np.all(np.isfinite(jg))
+        assert e < max(1e-3, 3 * f)
+    g.close()

@@ -181,3 +181,105 @@ def test_two_spheres_against_direct_evaluation(gyro):
         r = o.rigid(k)
         np.testing.assert_allclose(r["v"], v[k], rtol=1e-12)
         np.testing.assert_allclose(r["omega"], w[k], rtol=1e-12)
+
+
+def _split_net(n):
+    Fv, Fx, Fw, Tw, Tx, Tv = [n[9 * k:9 * k + 9].reshape(3, 3) for k in range(6)]
+    return dict(Fv=Fv, Fx=Fx, Fw=Fw, Tw=Tw, Tx=Tx, Tv=Tv, Fq=n[54:66].reshape(3, 4), Tq=n[66:78].reshape(3, 4))
+
+
+def _two_balls(mu, **kw):
+    ball = scenes.sphere_surface(0.05, 2 * R_P)
+    ball = ball - ball.mean(0)
+    p = _params(rigid_contact_beta=3000.0, rigid_contact_friction=mu, gravity=(0.0, -9.81, 0.0),
+                compute_gradient=1, use_gradient_manager=1, **kw)
+    return _scene(p, [_body(ball, [0.0, 0.0, 0.0], v0=(0.3, -0.1, 0.2), w0=(2.0, -1.0, 3.0)),
+                      _body(ball, [0.093, 0.012, -0.008], v0=(-0.2, 0.1, 0.0), w0=(-1.0, 0.5, 0.7), density=600.0)])
+
+
+def _fd(probe, R, Q, which, out, ncol, eps=1e-7):
+    cols = []
+    for c in range(ncol):
+        e = np.zeros(ncol)
+        e[c] = eps
+        a = probe.contact_probe(R, Q, **{which: e})[out]
+        b = probe.contact_probe(R, Q, **{which: -e})[out]
+        cols.append((a - b) / (2 * eps))
+    return np.stack(cols, 1)
+
+
+def test_contact_gradient_terms_against_differences():
+    """The sums the contact solver hands the manager (solveRigidContactPenalty,
+    RigidContactSolver.cpp:471-553) where they are derivatives of the contact
+    step: with no friction, d/dx and d/dq of sum f_n through the density (its
+    other-body part; n held; df_n/drho taken as -beta n as the reference does)
+    and of the torques, for the body itself and for the other body (the q
+    derivative of the homogeneous rotation form the reference differentiates);
+    with friction, d/dv and d/domega of the friction sum (|f_n| and the weights
+    held) for the body itself and d/domega for the other body.  The other
+    body's d/dv is the reference's approximation (-the body's own term) and is
+    not a derivative."""
+    for mu in (0.0, 0.4):
+        sc = _two_balls(mu)
+        probe = OracleSim(sc, "f64")
+        o = OracleSim(sc, "f64")
+        o.step(1)
+        for R, Q in ((0, 0), (0, 1), (1, 0), (1, 1)):
+            N = _split_net(o.gradient_manager_pair(R, Q)[1])
+            check = []
+            if mu == 0.0:
+                check += [("Fx", _fd(probe, R, Q, "dx", 0, 3)), ("Tx", _fd(probe, R, Q, "dx", 1, 3)),
+                          ("Fq", _fd(probe, R, Q, "dq", 0, 4))]
+            else:
+                check += [("Fw", _fd(probe, R, Q, "dw", 2, 3))]
+                if R == Q:
+                    check += [("Fv", _fd(probe, R, Q, "dv", 2, 3))]
+            for name, fd in check:
+                scale = np.abs(fd).max()
+                assert scale > 1e-3, (R, Q, name)
+                np.testing.assert_allclose(N[name], fd, rtol=0, atol=1e-6 * scale, err_msg=f"{name} ({R},{Q}) mu {mu}")
+
+
+def test_multi_body_manager_without_contact_matches_the_single_body_chain():
+    """With the contact solver on but the bodies apart, the manager's table
+    across bodies (the chains after the fluid and after the rigid coupling,
+    RigidBodyGradientManager.cpp:455-468) keeps the cross entries at zero and
+    its diagonal equals the single-body chain pinned in
+    test_oracle_pinning_coupling.py (gyroscopic term off: with it the contact
+    path's extra updateBoundaryForces calls move omega)."""
+    sc = scenes.submerged_body(body="ellipsoid", steps=8)
+    far = scenes.sphere_surface(0.04, 2 * sc.params["radius"])
+    sc.bodies.append(scenes.Body(far, True, np.array([0.17, 0.6, 0.17]), np.array([1.0, 0, 0, 0]), 500.0))
+    sc.params.update(compute_gradient=1, use_gradient_manager=1, gyroscopic=0)
+    runs = []
+    for contact in (0, 1):
+        p = dict(sc.params, use_rigid_contact=contact)
+        o = OracleSim(scenes.Scene(sc.name, p, sc.fluid_x, sc.fluid_v, sc.bodies, sc.steps), "f64")
+        o.step(8)
+        runs.append(o)
+    for k in (1, 2):
+        j1, n1 = runs[0].gradient_manager(k)
+        j2, n2 = runs[1].gradient_manager_pair(k, k)
+        scale = np.abs(j1).max()
+        np.testing.assert_allclose(j2, j1, rtol=0, atol=1e-9 * scale)
+    np.testing.assert_array_equal(runs[1].gradient_manager_pair(1, 2)[0], 0.0)
+    np.testing.assert_array_equal(runs[1].gradient_manager_pair(2, 1)[0], 0.0)
+    assert np.abs(runs[1].gradient_manager_pair(1, 1)[0][18:27] - np.eye(3).ravel()).max() > 1e-3  # v/v0 moved
+
+
+def test_cross_body_gradient_after_a_hit():
+    """A ball driven into a resting one: after the contact the passive ball's
+    position depends on the active ball's initial velocity through the manager's
+    cross entries (the quantity diff-rigid-contact-multi.py reads,
+    get_grad_x_to_v0(passive, active)); along the line of the hit it is
+    positive (a faster hit pushes the passive ball further)."""
+    ball = scenes.sphere_surface(0.05, 2 * R_P)
+    ball = ball - ball.mean(0)
+    p = _params(rigid_contact_beta=3000.0, compute_gradient=1, use_gradient_manager=1)
+    sc = _scene(p, [_body(ball, [0.0, 0.0, 0.0], v0=(1.0, 0.0, 0.0)), _body(ball, [0.105, 0.0, 0.0])])
+    o = OracleSim(sc, "f64")
+    o.step(40)
+    J = o.gradient_manager_pair(1, 0)[0]
+    dx_dv0 = J[0:9].reshape(3, 3)  # grad_x_to_v0 (layout: xv, xw, vv, vw, wv, ww, qv, qw)
+    assert o.rigid(1)["v"][0] > 0.1  # the passive ball was hit
+    assert dx_dv0[0, 0] > 1e-3

@@ -318,3 +318,34 @@ def test_gradient_manager_in_the_finite_difference_loop(gpu_ready):
     with pytest.raises(IndexError):
         mgr.get_grad_x_to_v0(1, 7)
     base.cleanup()
+
+
+def test_contact_scene_manager_through_the_mirror(gpu_ready):
+    """A scene with the rigid contact solver and the manager, driven through
+    the mirror as experiments/rigid_body_trajectory_optimization/python/
+    diff-rigid-contact-multi.py drives the reference: the boundary simulator
+    hands out the contact solver; after the hit, rb_grad_manager.get_grad_x_to_v0
+    (passive, active) is the library's entry (R, RR) of the manager's table,
+    positive along the line of the hit (a slower approach leaves the passive
+    sphere further along it)."""
+    sc = scenes.submerged_body(body="sphere", steps=30)
+    other = sc.bodies[1]
+    x2 = other.x_rb + np.array([0.135, 0.0, 0.0])
+    sc.bodies.append(scenes.Body(other.x0.copy(), True, x2, np.array([1.0, 0, 0, 0]), 500.0,
+                                 np.array([-0.5, 0.0, 0.0]), np.zeros(3)))
+    keep = np.linalg.norm(sc.fluid_x - x2, axis=1) > 0.07
+    sc.fluid_x, sc.fluid_v = sc.fluid_x[keep], sc.fluid_v[keep]
+    sc.params.update(compute_gradient=1, use_gradient_manager=1, use_rigid_contact=1, rigid_contact_beta=2000.0)
+    base = sph.Exec.SimulatorBase()
+    base.init(sceneFile=sc, useGui=False, stopAt=0.0)
+    base.initSimulation()
+    assert base.getBoundarySimulator().getRigidContactSolver() is not None
+    mgr = base.getRigidBodyGradientManager()
+    base.forwardFixedSteps(30)
+    passive, active = 1, 2
+    J = mgr.get_grad_x_to_v0(passive, active)
+    ours = base._core.gradient_manager_raw(passive, active)[0][0:9].reshape(3, 3)
+    print(f"[mirror contact] grad_x_to_v0(passive, active) =\n{J}", flush=True)
+    np.testing.assert_array_equal(J, ours)
+    assert J[0, 0] > 1e-3
+    base.cleanup()

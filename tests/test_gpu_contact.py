@@ -186,3 +186,51 @@ def test_contact_gradient_rollout_in_a_tank(gpu_ready, manager):
         assert np.all(np.isfinite(jg))
         assert e < max(1e-3, 3 * f)
     g.close()
+
+
+def test_billiards_like_combination(gpu_ready):
+    """What the billiards scene combines (billiards-on-water-2balls.json): the
+    CFL step size, Akinci 2013 surface tension and Standard viscosity, the
+    contact solver with the manager, the release mode driving body 1 into the
+    resting body 2; 40 steps against the FP32 oracle with the GPU's iteration
+    counts: step sizes within 1e-5, poses and the manager's entries within
+    1e-4 and 1e-3 or three times the FP32-vs-FP64 oracle spread."""
+    sc = scenes.submerged_body(body="sphere", steps=40)
+    other = sc.bodies[1]
+    x2 = other.x_rb + np.array([0.135, 0.0, 0.0])
+    sc.bodies.append(scenes.Body(other.x0.copy(), True, x2, np.array([1.0, 0, 0, 0]), 500.0, np.zeros(3), np.zeros(3)))
+    sc.bodies[1].v0 = np.array([1.0, 0.0, 0.0])
+    sc.bodies[1].w0 = np.zeros(3)
+    keep = np.linalg.norm(sc.fluid_x - x2, axis=1) > 0.07
+    sc.fluid_x, sc.fluid_v = sc.fluid_x[keep], sc.fluid_v[keep]
+    sc.params.update(compute_gradient=1, use_gradient_manager=1, use_rigid_contact=1, rigid_contact_beta=2000.0,
+                     cfl_method=1, cfl_factor=0.5, cfl_max_time_step_size=0.003, surface_tension_method=2,
+                     surface_tension=0.2, surface_tension_boundary=0.0, viscosity_method=1, viscosity=0.01,
+                     release_mode=1, uniform_acc_time=0.0)
+    print(f"[billiards-like] n={sc.n_fluid}, {sc.steps} steps", flush=True)
+    g = Simulation(sc)
+    o = OracleSim(sc, "f32_rigid64")
+    o64 = OracleSim(sc, "f64")
+    dts = []
+    for s in range(sc.steps):
+        g.step(1)
+        st = g.stats()
+        for oo in (o, o64):
+            oo.set_fixed_iterations([st["iterations_v"]], [st["iterations"]])
+            oo.step(1)
+        dts.append((st["dt"], o.field("dt")))
+    dts = np.array(dts)
+    print(f"  dt {dts[:, 0].min():.2e}..{dts[:, 0].max():.2e}, max rel err {np.abs(dts[:, 0] / dts[:, 1] - 1).max():.1e}",
+          flush=True)
+    np.testing.assert_allclose(dts[:, 0], dts[:, 1], rtol=1e-5)
+    pose = lambda a, b: max(rel(a["x"], b["x"]), rel(a["q"], b["q"]))
+    for k in (1, 2):
+        e, floor = pose(g.rigid(k), o.rigid(k)), pose(o.rigid(k), o64.rigid(k))
+        print(f"  body {k}: pose rel {e:.1e} (FP32 vs FP64 oracle {floor:.1e})", flush=True)
+        assert e < max(1e-4, 3 * floor)
+    for R, Q in ((1, 1), (1, 2), (2, 1), (2, 2)):
+        e, f = _jac_err(g.gradient_manager_raw(R, Q)[0], o.gradient_manager_pair(R, Q)[0],
+                        o64.gradient_manager_pair(R, Q)[0])
+        print(f"  ({R},{Q}) manager J err {e:.1e} (FP32 vs FP64 oracle {f:.1e})", flush=True)
+        assert e < max(1e-3, 3 * f)
+    g.close()

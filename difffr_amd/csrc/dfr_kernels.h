@@ -11,7 +11,7 @@ namespace dfr {
 // recent builds and never exceeds this) and fluid-neighbour slots per particle
 // (16-bit tile slots).
 constexpr int TILE_CAP = 2560;
-constexpr int MAXF = 64;
+constexpr int MAXF = 128;
 constexpr int PACK = 8;               // 16-bit slots per 16-byte pack
 constexpr int NPACK = MAXF / PACK;    // packs per particle
 constexpr int PASS_BLOCK = 640;       // threads of a gather-pass workgroup (one brick: ~640 particles at rest density)

@@ -58,7 +58,8 @@ typedef struct dfr_params {
   int32_t compute_gradient;      /* accumulate DiffFR's forward-mode rigid gradient */
   double uniform_acc_time;       /* uniform_acc_rb_time */
   double target_time;
-  int32_t max_neighbors;         /* per fluid particle (fluid + boundary), default 96 */
+  int32_t max_neighbors;         /* boundary samples per fluid particle, default 64 (fluid
+                                    neighbours: 128 per particle, fixed) */
   int32_t max_boundary_neighbors;/* fluid neighbours per dynamic sample, default 128 */
   int32_t release_mode;          /* useReleaseRigidBodyMode (Simulation.h:444): body 1 is held at rest
                                     until uniform_acc_time + dt, then set to (v0, omega0) at the start

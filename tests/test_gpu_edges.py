@@ -6,7 +6,7 @@
   sum_grad_p_k within 2e-5 of field scale; one step with 1 divergence and 3
   pressure iterations imposed on both: positions within 2e-6 m (the step
   moves particles by up to 1.7 cm; the FP32 and FP64 oracles differ by 3e-7 m).
-* Neighbour-list overflow (spacing 1.4 r: 92 fluid neighbours > 64 slots): the
+* Neighbour-list overflow (spacing 1.15 r: about 170 fluid neighbours > 128 slots): the
   step fails with the library's capacity error instead of truncating.
 * Fluid thrown out of the grid's box (the grid is fixed at set-up with a
   four-cell margin; positions beyond it are clamped into the edge cells):
@@ -71,7 +71,7 @@ def test_halos_beyond_lds_maximum(gpu_ready):
 
 
 def test_neighbour_overflow_is_an_error(gpu_ready):
-    sc = _lattice_scene(1.4 * R, (12, 8, 12), name="overcompressed")
+    sc = _lattice_scene(1.15 * R, (12, 8, 12), name="overcompressed")
     print(f"[overflow] nf={sc.n_fluid}", flush=True)
     g = Simulation(sc)
     with pytest.raises(DfrError, match="neighbour list capacity exceeded"):

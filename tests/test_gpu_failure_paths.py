@@ -23,9 +23,9 @@ R = 0.01
 
 
 def _overcompressed():
-    # lattice spacing 1.4 r: 92 fluid neighbours per particle, beyond the 64 slots
-    fluid = scenes.block_lattice((0.0, 0.0, 0.0), (12, 8, 12), 1.4 * R)
-    inner = 1.4 * R * (np.array([12, 8, 12], np.float64) + 1)
+    # lattice spacing 1.15 r: about 170 fluid neighbours per particle, beyond the 128 slots
+    fluid = scenes.block_lattice((0.0, 0.0, 0.0), (12, 8, 12), 1.15 * R)
+    inner = 1.15 * R * (np.array([12, 8, 12], np.float64) + 1)
     walls = scenes.box_surface((0.0, 0.0, 0.0), (inner[0], 2 * inner[1], inner[2]), 2 * R, open_top=True)
     bodies = [scenes.Body(walls, False, np.zeros(3), np.array([1.0, 0, 0, 0]), 1000.0, name="tank")]
     return scenes.Scene("overcompressed", scenes.default_params(R), fluid, np.zeros_like(fluid), bodies, 4)

@@ -1455,7 +1455,8 @@ void mlp_upload_params(dfr_ctx* c, const std::vector<float>& P) {
 // Uniform init with bound sqrt(6 / fan_in) (He et al. 2015) for the SiLU
 // layers, a tenth of it for the linear output layer; zero biases.
 void mlp_init_params(dfr_ctx* c, uint64_t seed) {
-  static const int O[4] = {64, 64, 64, 2}, I[4] = {6, 64, 64, 64}, OW[4] = {0, 448, 4608, 8768};
+  static constexpr int O[4] = {32, 32, 32, 2}, I[4] = {6, 32, 32, 32}, OW[4] = {0, 224, 1280, 2336};
+  static_assert(OW[3] + O[3] * I[3] + O[3] == MLP_NP, "layer table and parameter count disagree");
   std::vector<float> P(MLP_NP, 0.f);
   uint64_t st = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
   auto uni = [&]() {

@@ -318,7 +318,7 @@ void set_adjoint_kernel_attributes();
 // Flat FP32 parameters: W1[64][6] b1[64] W2[64][64] b2[64] W3[64][64] b3[64]
 // W4[2][64] b4[2].
 // ---------------------------------------------------------------------------
-constexpr int MLP_NP = 8898;
+constexpr int MLP_NP = 2402;  // 6 -> 32 -> 32 -> 32 -> 2
 
 struct MlpNorm {
   float mu_in[6], is_in[6];    // feature mean and 1/std

@@ -1,4 +1,4 @@
-// Learned kappa warm start: a 6 -> 64 -> 64 -> 64 -> 2 MLP (SiLU) that
+// Learned kappa warm start: a 6 -> 32 -> 32 -> 32 -> 2 MLP (SiLU) that
 // predicts, per fluid particle, the stiffness the divergence and pressure
 // solves converge to (kappa_v, kappa after the step, in the stored units of
 // TimeStepDiffDFSPH: kappa_v * dt, kappa * dt^2), from six features known
@@ -29,14 +29,17 @@ typedef __attribute__((ext_vector_type(4))) float f32x4;
 typedef __attribute__((ext_vector_type(4))) __bf16 bf16x4;
 
 // layer l: W_l is (MLP_O[l] x MLP_I[l]), row-major, followed by its bias
-__device__ constexpr int MLP_O[4] = {64, 64, 64, 2};
-__device__ constexpr int MLP_I[4] = {6, 64, 64, 64};
-__device__ constexpr int MLP_OW[4] = {0, 448, 4608, 8768};    // weight offsets in the flat parameter vector
-__device__ constexpr int MLP_OB[4] = {384, 4544, 8704, 8896}; // bias offsets
+constexpr int MLP_H = 32;           // hidden units per layer (a multiple of 32: whole K chunks)
+constexpr int MLP_HT = MLP_H / 16;  // D tiles of a hidden layer
+constexpr int MLP_KC = MLP_H / 32;  // K chunks of a hidden layer's output
+__device__ constexpr int MLP_O[4] = {MLP_H, MLP_H, MLP_H, 2};
+__device__ constexpr int MLP_I[4] = {6, MLP_H, MLP_H, MLP_H};
+__device__ constexpr int MLP_OW[4] = {0, 224, 1280, 2336};    // weight offsets in the flat parameter vector
+__device__ constexpr int MLP_OB[4] = {192, 1248, 2304, 2400}; // bias offsets
 // fragment blocks (64 lanes x bf16x8 each): forward (A = W) and transposed (A = W^T)
-__device__ constexpr int MLP_FW[4] = {0, 4, 12, 20};
-__device__ constexpr int MLP_FT[4] = {42, 34, 26, 22};
-constexpr int MLP_NFRAG = 44;
+__device__ constexpr int MLP_FW[4] = {0, 2, 4, 6};
+__device__ constexpr int MLP_FT[4] = {13, 11, 9, 7};
+constexpr int MLP_NFRAG = 14;
 // A second set of the same 44 fragments, K-permuted for register chaining
 // (fragment f of the chain set is at MLP_NFRAG + f).  The D tile of a layer
 // gives lane group g the rows 4g..4g+3; the two D tiles 2c, 2c+1 therefore
@@ -357,7 +360,7 @@ __global__ void __launch_bounds__(64) k_mlp(MlpArgs a) {
     if (MODE == MLP_TRAIN && l < 64) dbs[l] = 0.f;
     __syncthreads();
     for (int nt = 0; nt < 4; ++nt)
-      for (int mt = 0; mt < 4; ++mt) {
+      for (int mt = 0; mt < MLP_HT; ++mt) {
         bf16x4 dv;
         const int pp = 16 * nt + lr;
         for (int j = 0; j < 4; ++j) {
@@ -373,7 +376,7 @@ __global__ void __launch_bounds__(64) k_mlp(MlpArgs a) {
         *reinterpret_cast<bf16x4*>(dz + pp * LDH + 16 * mt + 4 * lg) = dv;
       }
     __syncthreads();
-    if (MODE == MLP_TRAIN) atomicAdd(&a.grad[MLP_OB[L - 1] + l], dbs[l]);
+    if (MODE == MLP_TRAIN && l < MLP_O[L - 1]) atomicAdd(&a.grad[MLP_OB[L - 1] + l], dbs[l]);
   }
 }
 
@@ -445,7 +448,7 @@ __global__ void __launch_bounds__(chain_block<IG>()) k_mlp_chain(MlpArgs a) {
     };
     int p[T];
     bool valid[T];
-    bf16x8 B[T][2];
+    bf16x8 B[T][2];  // K chunks of the next product (the input uses one)
 #pragma unroll
     for (int t = 0; t < T; ++t) {
       p[t] = grp * 16 * T + 16 * t + lr;
@@ -470,13 +473,13 @@ __global__ void __launch_bounds__(chain_block<IG>()) k_mlp_chain(MlpArgs a) {
       }
     }
     // ---- hidden layers 0..2: z = W h + b, h' = silu(z)
-    f32x4 sp[T][3][4];  // silu'(z) per hidden layer (IG only)
+    f32x4 sp[T][3][MLP_HT];  // silu'(z) per hidden layer (IG only)
 #pragma unroll
     for (int layer = 0; layer < 3; ++layer) {
-      const int kt = layer == 0 ? 1 : 2;
-      f32x4 h[T][4];
+      const int kt = layer == 0 ? 1 : MLP_KC;
+      f32x4 h[T][MLP_HT];
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt) {
+      for (int mt = 0; mt < MLP_HT; ++mt) {
         const f32x4 b0 = lbias(layer, mt);
 #pragma unroll
         for (int t = 0; t < T; ++t) h[t][mt] = b0;
@@ -491,7 +494,7 @@ __global__ void __launch_bounds__(chain_block<IG>()) k_mlp_chain(MlpArgs a) {
 #pragma unroll
       for (int t = 0; t < T; ++t) {
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
+        for (int mt = 0; mt < MLP_HT; ++mt)
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             const float z = h[t][mt][j];
@@ -500,17 +503,22 @@ __global__ void __launch_bounds__(chain_block<IG>()) k_mlp_chain(MlpArgs a) {
             h[t][mt][j] = hz;
             if (IG) sp[t][layer][mt][j] = fmaf(hz, 1.f - sg, sg);  // silu'(z) = s + z s (1 - s)
           }
-        B[t][0] = pack2(h[t][0], h[t][1]);
-        B[t][1] = pack2(h[t][2], h[t][3]);
+#pragma unroll
+        for (int c = 0; c < MLP_KC; ++c) B[t][c] = pack2(h[t][2 * c], h[t][2 * c + 1]);
       }
     }
     // ---- output layer (2 units, rows 0 and 1 of lane group 0)
     f32x4 y[T];
     {
       const f32x4 b3 = lbias(3, 0);
-      const bf16x8 A0 = frag(MLP_FW[3] + 0), A1 = frag(MLP_FW[3] + 1);
 #pragma unroll
-      for (int t = 0; t < T; ++t) y[t] = mfma(A1, B[t][1], mfma(A0, B[t][0], b3));
+      for (int t = 0; t < T; ++t) y[t] = b3;
+#pragma unroll
+      for (int c = 0; c < MLP_KC; ++c) {
+        const bf16x8 A = frag(MLP_FW[3] + c);
+#pragma unroll
+        for (int t = 0; t < T; ++t) y[t] = mfma(A, B[t][c], y[t]);
+      }
     }
     if (!IG) {
 #pragma unroll
@@ -546,10 +554,10 @@ __global__ void __launch_bounds__(chain_block<IG>()) k_mlp_chain(MlpArgs a) {
     }
 #pragma unroll
     for (int layer = 3; layer >= 1; --layer) {
-      const int kt = layer == 3 ? 1 : 2;
-      f32x4 dz[T][4];
+      const int kt = layer == 3 ? 1 : MLP_KC;
+      f32x4 dz[T][MLP_HT];
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt) {
+      for (int mt = 0; mt < MLP_HT; ++mt) {
 #pragma unroll
         for (int t = 0; t < T; ++t) dz[t][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -563,18 +571,19 @@ __global__ void __launch_bounds__(chain_block<IG>()) k_mlp_chain(MlpArgs a) {
 #pragma unroll
       for (int t = 0; t < T; ++t) {
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
+        for (int mt = 0; mt < MLP_HT; ++mt)
 #pragma unroll
           for (int j = 0; j < 4; ++j) dz[t][mt][j] *= sp[t][layer - 1][mt][j];
-        B[t][0] = pack2(dz[t][0], dz[t][1]);
-        B[t][1] = pack2(dz[t][2], dz[t][3]);
+#pragma unroll
+        for (int c = 0; c < MLP_KC; ++c) B[t][c] = pack2(dz[t][2 * c], dz[t][2 * c + 1]);
       }
     }
     // dH_0 = W_1^T dZ_1 (features 0..3 in lane group 0, 4..5 in group 1): dL/df = dL/dx * is
-    const bf16x8 A0 = frag(MLP_FT[0] + 0), A1 = frag(MLP_FT[0] + 1);
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-      const f32x4 din = mfma(A1, B[t][1], mfma(A0, B[t][0], f32x4{0.f, 0.f, 0.f, 0.f}));
+      f32x4 din = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < MLP_KC; ++c) din = mfma(frag(MLP_FT[0] + c), B[t][c], din);
       if (!valid[t] || lg >= 2) continue;
       const int pt = p[t];
       if (a.fbar) {

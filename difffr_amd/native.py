@@ -31,7 +31,7 @@ EXPORTS = [
     "dfr_set_time",
 ]
 
-MLP_NP = 8898
+MLP_NP = 2402
 
 
 class Params(C.Structure):

@@ -337,7 +337,7 @@ int dfr_set_fixed_iterations(dfr_ctx* ctx, int n, const int* iv, const int* ip);
 
 /* Learned kappa warm start (the learned part of this library; no counterpart
  * in the reference, whose warm starts halve the previous step's stiffness,
- * TimeStepDiffDFSPH.cpp:964-1059 and :1698-1788).  A 6 -> 64 -> 64 -> 64 -> 2
+ * TimeStepDiffDFSPH.cpp:964-1059 and :1698-1788).  A 6 -> 32 -> 32 -> 32 -> 2
  * MLP (SiLU, bf16 weights on MFMA, FP32 accumulation) maps per-particle
  * features (rho/rho0 - 1, dt * velocity divergence, neighbour count, boundary
  * density, previous kappa, previous kappa_v) to the stiffness the two solves
@@ -352,8 +352,8 @@ int dfr_set_fixed_iterations(dfr_ctx* ctx, int n, const int* iv, const int* ip);
  * dfr_mlp_train runs `iters` Adam steps (batch rows, cosine-decayed lr) on the
  * captured pairs with the library's MFMA forward/backward kernels; losses[it]
  * (optional) gets the mean squared normalised error of each batch.
- * Parameters are a flat float[8898] (W1[64][6] b1[64] W2[64][64] b2[64]
- * W3[64][64] b3[64] W4[2][64] b4[2]); norm is float[16] (feature mean[6],
+ * Parameters are a flat float[2402] (W1[32][6] b1[32] W2[32][32] b2[32]
+ * W3[32][32] b3[32] W4[2][32] b4[2]); norm is float[16] (feature mean[6],
  * feature std[6], target mean[2], target std[2]).  dfr_mlp_enable switches the
  * learned warm start on or off for the following steps. */
 int dfr_mlp_capture(dfr_ctx* ctx, int64_t capacity, double fraction, uint64_t seed);

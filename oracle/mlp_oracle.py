@@ -1,13 +1,13 @@
 """CPU restatement of the learned warm-start network (ORACLE: test infrastructure).
 
 Only tests/ import this module.  It recomputes, in numpy, what the GPU's bf16
-MFMA kernels compute: a 6 -> 64 -> 64 -> 64 -> 2 MLP with SiLU, weights and
+MFMA kernels compute: a 6 -> 32 -> 32 -> 32 -> 2 MLP with SiLU, weights and
 layer inputs rounded to bfloat16 (round to nearest even) where the GPU feeds
 them to the matrix cores, products accumulated in float32 (the GPU sums in a
 different order), biases and pre-activations in float32.
 
-Flat parameter vector (8898 floats): W1[64][6] b1[64] W2[64][64] b2[64]
-W3[64][64] b3[64] W4[2][64] b4[2].  norm = (mu_in[6], sd_in[6], mu_out[2], sd_out[2]).
+Flat parameter vector (2402 floats): W1[32][6] b1[32] W2[32][32] b2[32]
+W3[32][32] b3[32] W4[2][32] b4[2].  norm = (mu_in[6], sd_in[6], mu_out[2], sd_out[2]).
 
 `loss_grad` differentiates the mean over particles of the squared error of
 the normalised outputs, summed over the two outputs, as the training kernel
@@ -18,7 +18,7 @@ from __future__ import annotations
 
 import numpy as np
 
-SIZES = [(64, 6), (64, 64), (64, 64), (2, 64)]
+SIZES = [(32, 6), (32, 32), (32, 32), (2, 32)]
 NP = sum(o * i + o for o, i in SIZES)
 
 

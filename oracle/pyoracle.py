@@ -221,7 +221,7 @@ class OracleSim:
         lib().orc_step(self.h, n)
 
     def set_warm_net(self, params, norm, bf16: bool = True):
-        """Learned warm start with the 8898 network parameters and the 16 values
+        """Learned warm start with the 2402 network parameters and the 16 values
         of the input/output normalisation (params None: plain warm starts).
         bf16=True rounds as the GPU's matrix cores do; False is exact float64."""
         if params is None:

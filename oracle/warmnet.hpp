@@ -1,7 +1,7 @@
 // The learned warm-start network for the CPU oracle (test infrastructure only).
 //
-// 6 -> 64 -> 64 -> 64 -> 2 MLP with SiLU.  Parameters: flat float[8898] =
-// W1[64][6] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[2][64] b4[2];
+// 6 -> 32 -> 32 -> 32 -> 2 MLP with SiLU.  Parameters: flat float[2402] =
+// W1[32][6] b1[32] W2[32][32] b2[32] W3[32][32] b3[32] W4[2][32] b4[2];
 // norm[16] = feature mean[6], feature std[6], target mean[2], target std[2].
 // Two arithmetic modes:
 //   exact:  double everywhere (finite-difference pinning);
@@ -25,9 +25,9 @@ inline float bf16_round(float x) {
 }
 
 struct WarmNet {
-  static constexpr int NP = 8898;
-  static constexpr int O[4] = {64, 64, 64, 2}, I[4] = {6, 64, 64, 64};
-  static constexpr int OW[4] = {0, 448, 4608, 8768}, OB[4] = {384, 4544, 8704, 8896};
+  static constexpr int NP = 2402;
+  static constexpr int O[4] = {32, 32, 32, 2}, I[4] = {6, 32, 32, 32};
+  static constexpr int OW[4] = {0, 224, 1280, 2336}, OB[4] = {192, 1248, 2304, 2400};
   std::vector<double> P;
   double mu_in[6], sd_in[6], mu_out[2], sd_out[2];
   bool bf16 = false;

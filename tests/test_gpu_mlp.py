@@ -242,17 +242,22 @@ def test_learned_adjoint_matches_oracle(gpu_ready):
     for variant, tol in (("f32_rigid64", 1e-4), ("f64", 5e-3)):
         print(f"{variant}: largest difference / scale {np.abs(g - refs[variant][0]).max() / scale:.2e} "
               f"(tolerance {tol:.0e})", flush=True)
-    # pose: against the oracle with the GPU's precision layout (parity) and against
-    # FP64 (FP32 fluid, 12 steps)
+    # pose: against the oracle with the GPU's precision layout (parity), and against
+    # FP64 within twice the FP32 oracle's own distance to it (FP32 fluid, 12 steps)
     r32 = refs["f32_rigid64"][1]
     for k in ("x", "q"):
+        floor = np.abs(r32[k] - ro[k]).max()
         print(f"pose {k}: |gpu - oracle f32| {np.abs(rig[k] - r32[k]).max():.2e}, "
-              f"|gpu - oracle f64| {np.abs(rig[k] - ro[k]).max():.2e}", flush=True)
+              f"|gpu - oracle f64| {np.abs(rig[k] - ro[k]).max():.2e}, |oracle f32 - oracle f64| {floor:.2e}",
+              flush=True)
         np.testing.assert_allclose(rig[k], r32[k], rtol=0, atol=1e-6)
-        np.testing.assert_allclose(rig[k], ro[k], rtol=0, atol=1e-5)
+        np.testing.assert_allclose(rig[k], ro[k], rtol=0, atol=max(1e-5, 2.0 * floor))
     assert np.all(np.isfinite(g))
     np.testing.assert_allclose(g, refs["f32_rigid64"][0], rtol=0, atol=1e-4 * scale)
-    np.testing.assert_allclose(g, ref, rtol=0, atol=5e-3 * scale)
+    # FP64: 5e-3 of scale, or twice the FP32 oracle's own distance to FP64 where the
+    # network carries more FP32 noise into the trajectory
+    floor = np.abs(refs["f32_rigid64"][0] - ref).max()
+    np.testing.assert_allclose(g, ref, rtol=0, atol=max(5e-3 * scale, 2.0 * floor))
     # the learned warm start shapes the gradient: the plain oracle's is far off
     o2 = po.OracleSim(sc, "f64")
     o2.set_fixed_iterations(*iters)

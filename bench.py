@@ -17,8 +17,8 @@ in the backward pass), followed by its exact adjoint: forward + adjoint
 particle-steps per second, inputs resident in HBM.  It is timed twice from the
 same state: with the reference's warm start (halved previous kappa) and with
 the learned warm start, whose network is first trained (untimed) on pairs
-captured from 64 other seeded drops from the settled state (two rounds: the
-second captures rollouts warm-started by the first round's network).  `value` is the
+captured from 64 other seeded drops from the settled state (three rounds: each
+later one captures rollouts warm-started by the network so far).  `value` is the
 learned run; both iterate to the same tolerance.  W warm-up steps of the same
 window (recorded + reversed, every kernel class timed, untimed) precede each
 timed run.  The reference's own gradient (DiffFR forward mode, gradient mode 1)
@@ -64,8 +64,8 @@ def parse():
     ap.add_argument("--no-learned", action="store_true", help="measure the plain warm start only")
     ap.add_argument("--train-drops", type=int, default=64)
     ap.add_argument("--train-steps", type=int, default=25, help="captured steps per training drop")
-    ap.add_argument("--train-iters", type=int, default=3000, help="Adam iterations per training round")
-    ap.add_argument("--train-rounds", type=int, default=2)
+    ap.add_argument("--train-iters", type=int, default=6000, help="Adam iterations per training round")
+    ap.add_argument("--train-rounds", type=int, default=3)
     ap.add_argument("--train-pairs", type=int, default=4_000_000)
     ap.add_argument("--train-lr", type=float, default=2e-3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -156,8 +156,9 @@ def train_network(args, sim, sc):
     kappa); each later round captures rollouts warm-started by the network so
     far and continues its training.  Every rollout's Jacobi solves iterate to
     the same tolerance, so every target is the stiffness the solver converged
-    to (measured on the held-out drop: 3.1 pressure iterations after one round,
-    2.45 after two, 2.5 and more after three or more)."""
+    to (measured on the held-out drop over the 100 steps after it, 32-unit
+    network: 2.17-2.25 pressure iterations after two rounds of 3000 or 6000
+    Adam iterations, 2.12 after three of 6000)."""
     import torch
     from difffr_amd.warmstart import train_on_drops
     return train_on_drops(sim, sc, STATE_SLOT, drops=args.train_drops, steps=args.train_steps,
@@ -385,7 +386,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * main_run["elapsed"] / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded jittered lattice settled for 300 steps, procedural rigid bodies dropped with a "
-                "held-out seed; warm-start network trained on 64 other seeded drops in two rounds)",
+                "held-out seed; warm-start network trained on 64 other seeded drops in three rounds)",
         "config": {
             "workload": f"{sc.name}: {n} fluid particles + {sc.n_boundary} boundary samples, dt=1e-3, "
                         f"0.01% density / 0.1% divergence tolerance; {SETTLE_STEPS} settle steps, then the "
